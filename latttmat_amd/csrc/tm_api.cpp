@@ -1,0 +1,228 @@
+// tm_api.cpp -- the C ABI (include/latttmat.h).  Every entry point maps
+// C++ exceptions to a tm_status and a thread-local message.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "tm_internal.hpp"
+
+namespace ltm {
+tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out);
+}
+
+namespace {
+thread_local std::string g_err;
+
+tm_status fail(tm_status s, const char *msg)
+{
+  g_err = msg;
+  return s;
+}
+
+template <typename F>
+tm_status guarded(F &&f)
+{
+  try {
+    return f();
+  } catch (const ltm::Error &e) {
+    return fail(e.status, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(TM_ENOMEM, "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(TM_EINVAL, e.what());
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char *tm_last_error(void) { return g_err.c_str(); }
+const char *tm_version(void) { return "latttmat-mi355x 0.1 (gfx950)"; }
+
+tm_status tm_op_create(const tm_params *p, tm_op **out)
+{
+  if (!p || !out) return fail(TM_EINVAL, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    *out = ltm::make_op(*p);
+    return TM_OK;
+  });
+}
+
+void tm_op_destroy(tm_op *op) { delete op; }
+
+int64_t tm_op_rows(const tm_op *op) { return op ? op->ncol : 0; }
+int64_t tm_op_cols(const tm_op *op) { return op ? op->ncol : 0; }
+double tm_op_normfactor(const tm_op *op) { return op ? op->nf : std::nan(""); }
+int tm_op_is_symmetric(const tm_op *op) { return op && op->symmetric ? 1 : 0; }
+tm_kind tm_op_kind(const tm_op *op) { return op ? op->kind : (tm_kind)0; }
+int tm_op_suggested_ncv(const tm_op *, int neigs) { return neigs > 4 ? 2 * neigs + 1 : 2 * neigs + 3; }
+int tm_op_layer_size(const tm_op *op) { return op ? op->L : 0; }
+int64_t tm_op_matvec_count(const tm_op *op) { return op ? op->matvecs : 0; }
+
+tm_status tm_op_apply(const tm_op *cop, const double *x, double *y, int transpose)
+{
+  if (!cop || !x || !y) return fail(TM_EINVAL, "null argument");
+  tm_op *op = const_cast<tm_op *>(cop);
+  return guarded([&] {
+    const size_t bytes = sizeof(double) * op->ncol;
+    op->hx.ensure(bytes);
+    op->hy.ensure(bytes);
+    TM_HIP(hipMemcpyAsync(op->hx.get(), x, bytes, hipMemcpyHostToDevice, op->stream));
+    op->apply_dev(op->hx.get(), op->hy.get(), op->ncol, 1, TM_F64, TM_F64, transpose, nullptr,
+                  op->stream);
+    TM_HIP(hipMemcpyAsync(y, op->hy.get(), bytes, hipMemcpyDeviceToHost, op->stream));
+    TM_HIP(hipStreamSynchronize(op->stream));
+    return TM_OK;
+  });
+}
+
+tm_status tm_op_apply_dev(const tm_op *cop, const void *x, void *y, int64_t ld, int nvec,
+                          tm_dtype dtype, tm_dtype compute, int transpose, void *stream)
+{
+  if (!cop || !x || !y || nvec < 1 || ld < cop->ncol) return fail(TM_EINVAL, "bad argument");
+  if ((dtype != TM_F64 && dtype != TM_F32) || (compute != TM_F64 && compute != TM_F32))
+    return fail(TM_EINVAL, "bad dtype");
+  tm_op *op = const_cast<tm_op *>(cop);
+  return guarded([&] {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (compute == TM_F32) {
+      // per-vector power-of-two scale so that max|x| is in [1/2, 1)
+      ltm::DevBuf partial(sizeof(double) * ltm::kry::NBLK);
+      ltm::DevBuf scale(sizeof(double) * nvec);
+      const size_t es = dtype == TM_F64 ? 8 : 4;
+      if (dtype == TM_F64) {
+        for (int v = 0; v < nvec; ++v) {
+          ltm::kry::pow2_scale(static_cast<const double *>(x) + (size_t)v * ld, op->ncol,
+                              partial.as<double>(), scale.as<double>() + v, s);
+          op->apply_dev(static_cast<const char *>(x) + (size_t)v * ld * es,
+                        static_cast<char *>(y) + (size_t)v * ld * es, ld, 1, dtype, compute,
+                        transpose, scale.as<double>() + v, s);
+        }
+      } else {
+        op->apply_dev(x, y, ld, nvec, dtype, compute, transpose, nullptr, s);
+      }
+      TM_HIP(hipStreamSynchronize(s));  // scratch buffers are released on return
+    } else {
+      op->apply_dev(x, y, ld, nvec, dtype, compute, transpose, nullptr, s);
+    }
+    return TM_OK;
+  });
+}
+
+tm_status tm_op_apply_prec(const tm_op *cop, const double *x, double *y, int transpose,
+                           tm_dtype compute)
+{
+  if (!cop || !x || !y) return fail(TM_EINVAL, "null argument");
+  if (compute != TM_F64 && compute != TM_F32) return fail(TM_EINVAL, "bad compute dtype");
+  tm_op *op = const_cast<tm_op *>(cop);
+  return guarded([&] {
+    const size_t bytes = sizeof(double) * op->ncol;
+    op->hx.ensure(bytes);
+    op->hy.ensure(bytes);
+    TM_HIP(hipMemcpyAsync(op->hx.get(), x, bytes, hipMemcpyHostToDevice, op->stream));
+    ltm::DevBuf partial(sizeof(double) * ltm::kry::NBLK), scale(sizeof(double));
+    const double *sc = nullptr;
+    if (compute == TM_F32) {
+      ltm::kry::pow2_scale(op->hx.as<double>(), op->ncol, partial.as<double>(),
+                           scale.as<double>(), op->stream);
+      sc = scale.as<double>();
+    }
+    op->apply_dev(op->hx.get(), op->hy.get(), op->ncol, 1, TM_F64, compute, transpose, sc,
+                  op->stream);
+    TM_HIP(hipMemcpyAsync(y, op->hy.get(), bytes, hipMemcpyDeviceToHost, op->stream));
+    TM_HIP(hipStreamSynchronize(op->stream));
+    return TM_OK;
+  });
+}
+
+tm_status tm_op_time_apply(const tm_op *cop, tm_dtype io, tm_dtype compute, int transpose,
+                           int warmup, int reps, double *ms_per_apply)
+{
+  if (!cop || !ms_per_apply || reps < 1 || warmup < 0) return fail(TM_EINVAL, "bad argument");
+  tm_op *op = const_cast<tm_op *>(cop);
+  return guarded([&] {
+    const size_t es = io == TM_F64 ? 8 : 4;
+    const int64_t n = op->ncol;
+    ltm::DevBuf x(es * n), y(es * n), tmp(sizeof(double) * n);
+    ltm::kry::random_uniform(tmp.as<double>(), n, 12345, op->stream);
+    if (io == TM_F64) ltm::kry::copy(tmp.as<double>(), x.as<double>(), n, op->stream);
+    else ltm::kry::to_f32(tmp.as<double>(), x.as<float>(), n, op->stream);
+    for (int i = 0; i < warmup; ++i)
+      op->apply_dev(x.get(), y.get(), n, 1, io, compute, transpose, nullptr, op->stream);
+    hipEvent_t e0, e1;
+    TM_HIP(hipEventCreate(&e0));
+    TM_HIP(hipEventCreate(&e1));
+    TM_HIP(hipEventRecord(e0, op->stream));
+    for (int i = 0; i < reps; ++i)
+      op->apply_dev(x.get(), y.get(), n, 1, io, compute, transpose, nullptr, op->stream);
+    TM_HIP(hipEventRecord(e1, op->stream));
+    TM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    TM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_apply = ms / reps;
+    return TM_OK;
+  });
+}
+
+tm_status tm_op_orbits(const tm_op *op, uint64_t *reps, uint16_t *dups)
+{
+  if (!op || !reps || !dups) return fail(TM_EINVAL, "null argument");
+  if (op->kind != TM_KIND_XAM2D && op->kind != TM_KIND_XAM3D)
+    return fail(TM_EINVAL, "operator has no orbit tables (not a reduced operator)");
+  return guarded([&] {
+    TM_HIP(hipMemcpy(reps, op->reps.get(), sizeof(uint64_t) * op->ncol, hipMemcpyDeviceToHost));
+    TM_HIP(hipMemcpy(dups, op->dups.get(), sizeof(uint16_t) * op->ncol, hipMemcpyDeviceToHost));
+    return TM_OK;
+  });
+}
+
+void tm_solve_opts_default(tm_solve_opts *o)
+{
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->nev = 1;
+  o->ncv = 0;
+  o->tol = 1e-10;
+  o->maxit = 1000;
+  o->maxtrial = 3;
+  o->precision = TM_PREC_F64;
+  o->seed = 1;
+  o->want_left = 0;
+  o->want_vectors = 0;
+  o->mixed_tol = 1e-6;
+  o->verbose = 0;
+}
+
+tm_status tm_solve(tm_op *op, const tm_solve_opts *opts, tm_solve_out *out)
+{
+  if (!op || !opts || !out) return fail(TM_EINVAL, "null argument");
+  return guarded([&] {
+    tm_status st = ltm::solve_impl(op, *opts, out);
+    if (st != TM_OK) g_err = "eigensolver did not converge all requested pairs";
+    return st;
+  });
+}
+
+tm_status tm_grad_hf(const tm_op *cop, const double *l, const double *r, double dlam[4],
+                     double dlnnf[4])
+{
+  if (!cop || !l || !r || !dlam) return fail(TM_EINVAL, "null argument");
+  tm_op *op = const_cast<tm_op *>(cop);
+  return guarded([&] {
+    const size_t bytes = sizeof(double) * op->ncol;
+    ltm::DevBuf dl(bytes), dr(bytes);
+    TM_HIP(hipMemcpy(dl.get(), l, bytes, hipMemcpyHostToDevice));
+    TM_HIP(hipMemcpy(dr.get(), r, bytes, hipMemcpyHostToDevice));
+    double num[5];
+    ltm::grad_numerators(op, dl.as<double>(), dr.as<double>(), num);
+    // num[4] = <l, r>
+    for (int i = 0; i < 4; ++i) dlam[i] = num[i] / num[4];
+    if (dlnnf) ltm::dlnnf(op, dlnnf);
+    return TM_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+// tm_device.hpp -- device-side helpers: spin-state bit operations and the
+// popcount-indexed Boltzmann diagonal.  Bit i of a state is site i; a set
+// bit is a down spin (the reference convention, readme.md:100).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tm_internal.hpp"
+
+namespace ltm {
+namespace dev {
+
+__device__ __forceinline__ int pop(uint64_t x) { return __popcll(x); }
+
+// rotate right by d within n bits (TMat/utility.hpp:91-94)
+__device__ __forceinline__ uint64_t rotr(uint64_t s, int n, int d)
+{
+  return (s >> d) | ((s & ((1ull << d) - 1)) << (n - d));
+}
+// rotate left by d within n bits (TMat/utility.hpp:99-102)
+__device__ __forceinline__ uint64_t rotl(uint64_t s, int n, int d)
+{
+  return ((s << d) & ((1ull << n) - 1)) | (s >> (n - d));
+}
+// reverse the low n bits (same result as TMat/utility.cpp:17-31)
+__device__ __forceinline__ uint64_t brev(uint64_t s, int n)
+{
+  return __brevll(s) >> (64 - n);
+}
+__device__ __forceinline__ uint32_t brev32(uint32_t s, int n)
+{
+  return __brev(s) >> (32 - n);
+}
+
+// 3D row rotation right by 1 / 2 inside every row of s2 bits
+// (TMat/utility.cpp:52-63); whole-layer row shift is rotr(s, L, s2)
+// (TMat/utility.cpp:68-73).
+__device__ __forceinline__ uint64_t rff1(uint64_t s, const DiagSpec &g)
+{
+  return ((s >> 1) & g.cmr1) | ((s & g.cm1) << (g.s2 - 1));
+}
+__device__ __forceinline__ uint64_t rff2(uint64_t s, const DiagSpec &g)
+{
+  return ((s >> 2) & g.cmr2) | ((s & g.cm2) << (g.s2 - 2));
+}
+
+// Popcount indices (i1, i2, i0) of a state for the diagonal tables.  Each
+// branch reproduces the corresponding genTvec of the reference; the table
+// entries carry the coupling constants and constant offsets.
+struct Idx3 { int i1, i2, i0; };
+
+__device__ __forceinline__ Idx3 diag_index(const DiagSpec &g, uint64_t s)
+{
+  const int L = g.L;
+  const uint64_t mask = (L >= 64) ? ~0ull : ((1ull << L) - 1);
+  Idx3 r;
+  r.i0 = pop(s);
+  switch (g.kind) {
+    case DIAG_X2D_PBC:
+    case DIAG_ZBT:
+      r.i1 = pop(s ^ rotr(s, L, 1));
+      r.i2 = pop(s ^ rotr(s, L, 2));
+      break;
+    case DIAG_X2D_OBC:
+      r.i1 = pop((s & (mask >> 1)) ^ (s >> 1));
+      r.i2 = pop((s & (mask >> 2)) ^ (s >> 2));
+      break;
+    case DIAG_X2D_C5:
+    case DIAG_X2D_C9: {
+      uint64_t a = (s << 1) | (g.kind == DIAG_X2D_C9 ? 1ull : 0ull);
+      r.i1 = pop(s ^ a);
+      r.i2 = pop((a & mask) ^ (s >> 1));
+      break;
+    }
+    case DIAG_X2D_C13:
+    case DIAG_X2D_C17: {
+      bool one = g.kind == DIAG_X2D_C17;
+      uint64_t a = (s << 1) | (one ? 1ull : 0ull);
+      uint64_t b = (s << 2) | (one ? 3ull : 0ull);
+      r.i1 = pop(s ^ a);
+      r.i2 = pop(s ^ b);
+      break;
+    }
+    case DIAG_X3D:
+      r.i1 = pop(s ^ rff1(s, g)) + pop(s ^ rotr(s, L, g.s2));
+      r.i2 = pop(s ^ rff2(s, g));
+      break;
+    case DIAG_XDT_PBC:
+      r.i1 = pop(s ^ rotr(s, L, 1));
+      r.i2 = 0;
+      break;
+    case DIAG_XDT_C5:
+      r.i1 = pop((s << 1) ^ s);
+      r.i2 = 0;
+      break;
+    case DIAG_XDT_C9:
+      r.i1 = pop(((s << 1) | 1ull) ^ s);
+      r.i2 = 0;
+      break;
+    default:
+      r.i1 = r.i2 = 0;
+  }
+  return r;
+}
+
+template <typename C>
+__device__ __forceinline__ C diag_value(const DiagSpec &g, const C *tab, uint64_t s)
+{
+  Idx3 i = diag_index(g, s);
+  return tab[i.i1] * tab[64 + i.i2] * tab[128 + i.i0];
+}
+
+// One radix-2^R Kronecker stage group on R register values: v[i] indexed by
+// R bits, K = [[a, b], [b, a]] applied on each of the R bits.
+template <int R, typename C>
+__device__ __forceinline__ void kron_regs(C *v, C a, C b)
+{
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int m = 1 << j;
+#pragma unroll
+    for (int i = 0; i < (1 << R); ++i) {
+      if (!(i & m)) {
+        C x0 = v[i], x1 = v[i | m];
+        v[i] = a * x0 + b * x1;
+        v[i | m] = b * x0 + a * x1;
+      }
+    }
+  }
+}
+
+// Full Kronecker transform of an LDS array of 2^nbits elements (bits
+// [bit0, nbits) still to do), using every thread of the block.  Element
+// index p maps to lds[p].  Ends with a block barrier.
+template <int R, typename C>
+__device__ __forceinline__ void lds_kron_group(C *lds, int nbits, int bg, C a, C b)
+{
+  const int nitems = 1 << (nbits - R);
+  for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
+    const int lo = q & ((1 << bg) - 1);
+    const int hi = q >> bg;
+    const int base = (hi << (bg + R)) | lo;
+    C v[1 << R];
+#pragma unroll
+    for (int i = 0; i < (1 << R); ++i) v[i] = lds[base | (i << bg)];
+    kron_regs<R>(v, a, b);
+#pragma unroll
+    for (int i = 0; i < (1 << R); ++i) lds[base | (i << bg)] = v[i];
+  }
+}
+
+template <typename C>
+__device__ void lds_kron(C *lds, int nbits, int bit0, C a, C b)
+{
+  for (int bg = bit0; bg < nbits; bg += 4) {
+    const int r = nbits - bg < 4 ? nbits - bg : 4;
+    switch (r) {
+      case 1: lds_kron_group<1>(lds, nbits, bg, a, b); break;
+      case 2: lds_kron_group<2>(lds, nbits, bg, a, b); break;
+      case 3: lds_kron_group<3>(lds, nbits, bg, a, b); break;
+      default: lds_kron_group<4>(lds, nbits, bg, a, b); break;
+    }
+    __syncthreads();
+  }
+}
+
+// XCD-aware block remap: hardware dispatches workgroup i to XCD i % 8; give
+// every XCD a contiguous range of logical block ids so neighbouring tiles
+// share that XCD's L2.  Returns a logical id in [0, nlogical) or -1.
+__device__ __forceinline__ int64_t xcd_remap(int64_t i, int64_t nlogical)
+{
+  const int64_t per = (nlogical + 7) / 8;
+  const int64_t q = (i % 8) * per + i / 8;
+  return q < nlogical ? q : -1;
+}
+
+}  // namespace dev
+}  // namespace ltm

@@ -1,0 +1,269 @@
+// tm_internal.hpp -- shared declarations of the MI355X transfer-matrix library.
+//
+// Layering:
+//   tm_api.cpp      C ABI (include/latttmat.h), pfunc-style dispatch
+//   tm_ops.cpp      operator classes: tables on device, apply() launch plans
+//   tm_kron.hip     full-space Kronecker (butterfly) passes: XAT, 3D full
+//   tm_reduced.hip  orbit enumeration + reduced operators (XAM 2D/3D)
+//   tm_sweep.hip    aux-bit sweeps: XDT (DNNI), ZBT/ZBT^T (BNNNI/3NN)
+//   tm_krylov.hip   Krylov vector kernels (multi-dot, CGS2 update, restart)
+//   tm_solver.cpp   thick-restart Krylov-Schur driver, mixed precision
+//   tm_dense.cpp    small dense eigensolvers for the Ritz problem (host)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/latttmat.h"
+
+namespace ltm {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  tm_status status;
+  Error(tm_status s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char *expr, const char *file, int line);
+
+#define TM_HIP(expr)                                                   \
+  do {                                                                 \
+    hipError_t _e = (expr);                                            \
+    if (_e != hipSuccess) ::ltm::throw_hip(_e, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+#define TM_CHECK(cond, status, msg)                     \
+  do {                                                  \
+    if (!(cond)) throw ::ltm::Error((status), (msg));    \
+  } while (0)
+
+// Launch-error check after a kernel launch.
+#define TM_LAUNCH_CHECK() TM_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// device buffer (RAII)
+// ---------------------------------------------------------------------------
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { alloc(bytes); }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    return *this;
+  }
+  void alloc(size_t bytes);
+  void ensure(size_t bytes) { if (bytes > n_) alloc(bytes); }
+  void release();
+  template <typename T> T *as() const { return static_cast<T *>(p_); }
+  void *get() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  void *p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// diagonal (Tv^1/2) descriptor shared by the full-space kernels
+// ---------------------------------------------------------------------------
+// D(s) = tab[i1(s)] * tab[64 + i2(s)] * tab[128 + i0(s)], with the integer
+// indices computed from popcounts of the state (no per-state table in HBM).
+enum DiagKind : int {
+  DIAG_X2D_PBC = 0,   // XAT periodic      (tmat.hpp:218-231)
+  DIAG_X2D_OBC = 1,   // XAT open          (tmat.hpp:233-247)
+  DIAG_X2D_C5 = 5,    // XAT confined      (tmat.hpp:249-262)
+  DIAG_X2D_C9 = 9,    //                   (tmat.hpp:264-277)
+  DIAG_X2D_C13 = 13,  //                   (tmat.hpp:279-292)
+  DIAG_X2D_C17 = 17,  //                   (tmat.hpp:294-307)
+  DIAG_X3D = 30,      // 3D layer energies (tmat3d.cpp:188-199)
+  DIAG_XDT_PBC = 40,  // DNNI periodic     (tmat.hpp:445-458)
+  DIAG_XDT_C5 = 45,   //                   (tmat.hpp:475-487)
+  DIAG_XDT_C9 = 49,   //                   (tmat.hpp:489-501)
+  DIAG_ZBT = 60       // BNNNI/3NN         (tmat.hpp:537-556)
+};
+
+struct DiagSpec {
+  int kind;
+  int L;       // bits per state
+  int s1, s2;  // 3D cross-section
+  // 3D column masks (bits of columns [0,c) in every row) for row rotations
+  uint64_t cm1, cmr1, cm2, cmr2;  // cols [0,1), [0,s2-1), [0,2), [0,s2-2)
+};
+
+// Largest dynamic LDS allocation per workgroup on the current device (bytes);
+// set when the first operator is created.  Tile sizes are chosen from it.
+extern size_t g_lds_per_block;
+
+// ---------------------------------------------------------------------------
+// launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+// Kronecker operator y = Dout o (x)_{i<L} K o Din o x, K = [[ka, kb],[kb, ka]].
+// in/out are FP64 or FP32 (io), compute precision comp.  scale_in points to a
+// device double (input multiplied by it, output divided by it) or is null.
+struct KronPlan {
+  int L;
+  bool prologue, epilogue;  // apply D on the first / last pass
+  DiagSpec diag;
+  const double *tab64;      // device, 192 doubles
+  const float *tab32;       // device, 192 floats
+  double ka, kb;
+};
+void launch_kron(const KronPlan &plan, const void *in, void *out, tm_dtype io_in,
+                 tm_dtype io_out, tm_dtype comp, void *work, const double *scale_in,
+                 hipStream_t s);
+size_t kron_work_bytes(int L, tm_dtype comp);
+
+// Sweeps (XDT, ZBT); transpose only differs for ZBT.
+struct SweepPlan {
+  int kind;           // TM_KIND_XDT or TM_KIND_ZBT
+  int L;
+  int bc;             // XDT boundary (0, 5, 9)
+  DiagSpec diag;
+  const double *tab64;
+  const float *tab32;
+  double b1[2], b2[2], b3[2];
+  double nf;          // XDT output factor (ZBT folds it into the diagonal)
+};
+void launch_sweep(const SweepPlan &plan, const void *in, void *out, tm_dtype io,
+                  tm_dtype comp, int transpose, void *work, const double *scale_in,
+                  hipStream_t s);
+size_t sweep_work_bytes(int kind, int L, tm_dtype comp);
+
+// Grad kernels (dual numbers through the passes); see tm_grad in tm_ops.cpp.
+
+// ---- orbit enumeration (tm_reduced.hip) ----
+struct GroupSpec {
+  int dim;          // 2 or 3
+  int L;            // total bits
+  int s1, s2;       // 3D rows, cols
+  int bc;           // 2D: 0 periodic, 1 open
+  int censym;       // include global spin flip
+  int rotate;       // 3D: include 90-degree rotation
+};
+// Enumerate orbits on the device.  Fills reps (ascending) and dups (orbit
+// sizes), returns M.  If map != nullptr it receives map[s] = orbit index of s.
+int64_t enumerate_orbits(const GroupSpec &g, DevBuf &reps, DevBuf &dups, DevBuf *map,
+                         hipStream_t s);
+// Fused-path tables (2D periodic, L even).
+struct FusedTables {
+  int n;                 // L/2
+  int64_t Q;             // stored columns (a <= rev a)
+  int nS;                // stored rows (B_R u rev B_R)
+  int nB;                // |B_R|
+  DevBuf cols;           // uint32[Q]   column value a of stored column q
+  DevBuf cmap;           // uint32[Q << n] orbit index of state (cols[q] << n | b')
+  DevBuf srows;          // uint16[nS]  row values b (ascending)
+  DevBuf rowidx;         // int32[2^n]  index of b in srows or -1
+  DevBuf brows;          // uint16[nB]  B_R values (ascending)
+  DevBuf rstart;         // int64[nB+1] rep range of each B_R row
+  DevBuf replo;          // uint16[M]   low n bits of each rep
+};
+void build_fused_tables(const GroupSpec &g, const DevBuf &reps, int64_t M, FusedTables &ft,
+                        hipStream_t s);
+
+// reduced-operator application
+struct ReducedPlan {
+  int64_t M;
+  int L;
+  const double *tvec;     // device [M] Tvec (sqrt(nf) exp(e/2) / sqrt(dup))
+  const double *dt64;     // device [M] dup*Tvec
+  const float *tvec32, *dt32;
+  const uint64_t *reps;   // device [M]
+  const uint32_t *map;    // device [2^L] (generic path) or null
+  const FusedTables *ft;  // fused path tables or null
+  double ka, kb;
+};
+void launch_reduced(const ReducedPlan &plan, const void *in, void *out, tm_dtype io,
+                    tm_dtype comp, void *work, const double *scale_in, hipStream_t s);
+size_t reduced_work_bytes(const ReducedPlan &plan, tm_dtype comp);
+
+// ---- Krylov vector kernels (tm_krylov.hip) ----
+namespace kry {
+constexpr int NBLK = 256;   // partial-sum blocks of the reductions
+// out[k] = <V_k, w> for k < j, out[j] = <w,w>; partial buffer of NBLK*(j+1)
+void multidot(const double *V, int64_t ldv, int j, const double *w, int64_t n,
+              double *partial, double *out, hipStream_t s);
+// w -= sum_k h[k] V_k (k<j); then out[k] = <V_k, w>, out[j] = <w,w>
+void update_dots(const double *V, int64_t ldv, int j, double *w, int64_t n,
+                 const double *h, double *partial, double *out, hipStream_t s);
+// w -= sum_k h[k] V_k; out[0] = <w,w>
+void update_norm(const double *V, int64_t ldv, int j, double *w, int64_t n,
+                 const double *h, double *partial, double *out, hipStream_t s);
+// v = w * (1/sqrt(nrm2[0]))
+void normalize(double *w, int64_t n, const double *nrm2, hipStream_t s);
+// Vout[:, c] = sum_r Vin[:, r] * Y[r + c*ldy], r < m, c < k  (Vout may alias Vin)
+void gemm_vy(const double *Vin, int64_t ldv, int m, const double *Y, int ldy, int k,
+             double *Vout, int64_t ldo, int64_t n, hipStream_t s);
+void random_uniform(double *v, int64_t n, uint64_t seed, hipStream_t s);
+void copy(const double *a, double *b, int64_t n, hipStream_t s);
+void to_f32(const double *a, float *b, int64_t n, hipStream_t s);
+void fill(double *a, double val, int64_t n, hipStream_t s);
+// max|x| -> 2^-ceil(log2 max) into scale (device double)
+void pow2_scale(const double *x, int64_t n, double *partial, double *scale, hipStream_t s);
+// G[i*k + j] = <A_i, B_j>  (A, B are k column vectors)
+void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double *partial,
+          double *G, hipStream_t s);
+}  // namespace kry
+
+// ---------------------------------------------------------------------------
+// small dense eigensolvers (tm_dense.cpp)
+// ---------------------------------------------------------------------------
+// Symmetric: A (n x n, column-major) -> eigenvalues w, eigenvectors Z (columns).
+void sym_eig(int n, const double *A, double *w, double *Z);
+// General real: eigenvalues (wr, wi) and eigenvectors as complex columns
+// Zr/Zi (column j belongs to eigenvalue j; unit 2-norm).
+void gen_eig(int n, const double *A, double *wr, double *wi, double *Zr, double *Zi);
+
+}  // namespace ltm
+
+// ---------------------------------------------------------------------------
+// the operator object behind tm_op*
+// ---------------------------------------------------------------------------
+struct tm_op {
+  tm_params p{};
+  tm_kind kind{};
+  int L = 0, s1 = 0, s2 = 0;
+  int bc = 0;           // boundary code (XAT/XDT) or extend flag (XAM)
+  int censym = 0, rotate = 0, nleadbit = 0;
+  int64_t nstate = 0, ncol = 0;
+  double nf = 1.0;
+  bool symmetric = true;
+  int64_t matvecs = 0;
+  hipStream_t stream = nullptr;
+  double bzwj1[2]{}, bzwj2[2]{}, bzwj3[2]{};
+
+  // diagonal tables (full-space operators)
+  ltm::DiagSpec diag{};
+  ltm::DevBuf tab64, tab32;
+  // reduced operators
+  ltm::DevBuf reps, dups, map, tvec, dt64, tvec32, dt32;
+  std::unique_ptr<ltm::FusedTables> ft;
+  // workspaces
+  ltm::DevBuf work, one, hx, hy;
+
+  virtual ~tm_op();
+  void apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
+                 int transpose, const double *scale_in, hipStream_t s);
+};
+
+namespace ltm {
+// Construct an operator from parameters (pfunc dispatch); throws Error.
+tm_op *make_op(const tm_params &p);
+// Hellmann-Feynman numerators: num[i] = <l, dT/dtheta_i r> for theta =
+// (betaJ, kappa, kappa2, h), num[4] = <l, r>.  l, r are device vectors.
+void grad_numerators(tm_op *op, const double *l_dev, const double *r_dev, double num[5]);
+// d ln(normfactor)/d theta for the 4 params.
+void dlnnf(const tm_op *op, double out[4]);
+}  // namespace ltm

@@ -1,0 +1,176 @@
+// tm_kron.hip -- full-space Kronecker transfer-matrix operator on gfx950.
+//
+//   y = D o (x)_{i<L} K o D o x,   K = [[e^bJ, e^-bJ], [e^-bJ, e^bJ]]
+//
+// is XATMatVecProd2D::perform_op (TMat/tmat.cpp:71-102) with the normfactor
+// split as sqrt(nf) into both diagonals, and the new full-space 3D operator.
+// The diagonal D(s) is rebuilt from three popcounts of the state and a
+// 192-entry table staged in LDS, so the only HBM traffic is the vector.
+//
+// The L butterfly stages are grouped into passes.  A pass owns bits
+// [b0, b0+nb) of the index: a workgroup loads a tile of 2^nb elements along
+// those bits times W = 2^logW contiguous elements (coalescing), applies the
+// nb stages in LDS with radix-16 register groups, and stores the tile back.
+// The first pass applies D on load, the last applies D on store.
+#include "tm_device.hpp"
+
+namespace ltm {
+namespace {
+
+constexpr int KRON_THREADS = 512;
+enum { F_PRO = 1, F_EPI = 2, F_SIN = 4, F_SOUT = 8 };
+
+template <typename C, typename TI, typename TO>
+__global__ __launch_bounds__(KRON_THREADS) void k_kron_pass(
+    const TI *__restrict__ in, TO *__restrict__ out, int b0, int nb, int logW, int flags,
+    DiagSpec dg, const C *__restrict__ tab, C ka, C kb, const double *__restrict__ scale)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tileLog = nb + logW;
+  const int T = 1 << tileLog;
+  const int W = 1 << logW;
+  C *lds = reinterpret_cast<C *>(smem);
+  C *stab = lds + T;
+  if (flags & (F_PRO | F_EPI))
+    for (int i = threadIdx.x; i < 192; i += blockDim.x) stab[i] = tab[i];
+
+  const uint64_t tile = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  const uint64_t innerBlocks = (1ull << b0) >> logW;
+  const uint64_t o = tile / innerBlocks, ib = tile % innerBlocks;
+  const uint64_t base = (o << (b0 + nb)) | (ib << logW);
+  C sin = 1, sout = 1;
+  if (scale && (flags & (F_SIN | F_SOUT))) {
+    const double sc = *scale;
+    if (flags & F_SIN) sin = (C)sc;
+    if (flags & F_SOUT) sout = (C)(1.0 / sc);
+  }
+  __syncthreads();
+
+  for (int e = threadIdx.x; e < T; e += blockDim.x) {
+    const uint64_t g = base | ((uint64_t)(e >> logW) << b0) | (uint64_t)(e & (W - 1));
+    C v = (C)in[g] * sin;
+    if (flags & F_PRO) v *= dev::diag_value<C>(dg, stab, g);
+    lds[e] = v;
+  }
+  __syncthreads();
+
+  // nb stages: LDS index of element (p, w) is p * W + w, so bit j of p is
+  // bit (j + logW) of the LDS index; the w bits ride along as lo bits.
+  for (int bg = 0; bg < nb; bg += 4) {
+    const int r = nb - bg < 4 ? nb - bg : 4;
+    switch (r) {
+      case 1: dev::lds_kron_group<1>(lds, tileLog, bg + logW, ka, kb); break;
+      case 2: dev::lds_kron_group<2>(lds, tileLog, bg + logW, ka, kb); break;
+      case 3: dev::lds_kron_group<3>(lds, tileLog, bg + logW, ka, kb); break;
+      default: dev::lds_kron_group<4>(lds, tileLog, bg + logW, ka, kb); break;
+    }
+    __syncthreads();
+  }
+
+  for (int e = threadIdx.x; e < T; e += blockDim.x) {
+    const uint64_t g = base | ((uint64_t)(e >> logW) << b0) | (uint64_t)(e & (W - 1));
+    C v = lds[e] * sout;
+    if (flags & F_EPI) v *= dev::diag_value<C>(dg, stab, g);
+    out[g] = (TO)v;
+  }
+}
+
+struct Pass { int b0, nb, logW; };
+
+// 2^13 FP64 / 2^14 FP32 tiles (64 KB + table) need the gfx950 LDS size;
+// fall back to half-size tiles on a smaller LDS.
+int tile_log_for(tm_dtype comp)
+{
+  const bool big = g_lds_per_block >= (size_t)(96 * 1024);
+  return comp == TM_F64 ? (big ? 13 : 12) : (big ? 14 : 13);
+}
+
+std::vector<Pass> plan_passes(int L, tm_dtype comp)
+{
+  std::vector<Pass> ps;
+  const int tl = tile_log_for(comp);
+  const int wl = comp == TM_F64 ? 3 : 4;  // 64-byte contiguous runs
+  int nb0 = L < tl ? L : tl;
+  ps.push_back({0, nb0, 0});
+  int b = nb0;
+  while (b < L) {
+    int nb = L - b < tl - wl ? L - b : tl - wl;
+    int logW = tl - nb;
+    if (logW > b) logW = b;
+    ps.push_back({b, nb, logW});
+    b += nb;
+  }
+  return ps;
+}
+
+template <typename C, typename TI, typename TO>
+void run_pass(const Pass &p, int L, const void *in, void *out, int flags, const KronPlan &plan,
+              const C *tab, const double *scale, hipStream_t s)
+{
+  const uint64_t ntile = 1ull << (L - p.nb - p.logW);
+  dim3 grid;
+  if (ntile <= 65535) grid = dim3((unsigned)ntile, 1, 1);
+  else grid = dim3(65536, (unsigned)(ntile / 65536), 1);
+  const size_t shm = ((size_t(1) << (p.nb + p.logW)) + 192) * sizeof(C);
+  hipLaunchKernelGGL((k_kron_pass<C, TI, TO>), grid, dim3(KRON_THREADS), shm, s,
+                     static_cast<const TI *>(in), static_cast<TO *>(out), p.b0, p.nb, p.logW,
+                     flags, plan.diag, tab, (C)plan.ka, (C)plan.kb, scale);
+  TM_LAUNCH_CHECK();
+}
+
+template <typename C>
+void kron_typed(const KronPlan &plan, const void *in, void *out, tm_dtype io_in, tm_dtype io_out,
+                void *work, const double *scale, hipStream_t s, const C *tab)
+{
+  const int L = plan.L;
+  auto ps = plan_passes(L, sizeof(C) == 8 ? TM_F64 : TM_F32);
+  const int np = (int)ps.size();
+  // Intermediate buffer: the output itself when it has the compute type,
+  // else the workspace.
+  const bool out_is_c = (sizeof(C) == 8) == (io_out == TM_F64);
+  void *mid = out_is_c ? out : work;
+  for (int i = 0; i < np; ++i) {
+    int flags = 0;
+    if (i == 0 && plan.prologue) flags |= F_PRO;
+    if (i == np - 1 && plan.epilogue) flags |= F_EPI;
+    if (i == 0 && scale) flags |= F_SIN;
+    if (i == np - 1 && scale) flags |= F_SOUT;
+    const void *src = (i == 0) ? in : mid;
+    void *dst = (i == np - 1) ? out : mid;
+    const bool src_io = (i == 0);
+    const bool dst_io = (i == np - 1);
+    const double *sc = (src_io || dst_io) ? scale : nullptr;
+    if (src_io && dst_io) {
+      if (io_in == TM_F64 && io_out == TM_F64) run_pass<C, double, double>(ps[i], L, src, dst, flags, plan, tab, sc, s);
+      else if (io_in == TM_F64) run_pass<C, double, float>(ps[i], L, src, dst, flags, plan, tab, sc, s);
+      else if (io_out == TM_F64) run_pass<C, float, double>(ps[i], L, src, dst, flags, plan, tab, sc, s);
+      else run_pass<C, float, float>(ps[i], L, src, dst, flags, plan, tab, sc, s);
+    } else if (src_io) {
+      if (io_in == TM_F64) run_pass<C, double, C>(ps[i], L, src, dst, flags, plan, tab, sc, s);
+      else run_pass<C, float, C>(ps[i], L, src, dst, flags, plan, tab, sc, s);
+    } else if (dst_io) {
+      if (io_out == TM_F64) run_pass<C, C, double>(ps[i], L, src, dst, flags, plan, tab, sc, s);
+      else run_pass<C, C, float>(ps[i], L, src, dst, flags, plan, tab, sc, s);
+    } else {
+      run_pass<C, C, C>(ps[i], L, src, dst, flags, plan, tab, nullptr, s);
+    }
+  }
+}
+
+}  // namespace
+
+size_t kron_work_bytes(int L, tm_dtype comp)
+{
+  return comp == TM_F32 ? (sizeof(float) << L) : 0;
+}
+
+void launch_kron(const KronPlan &plan, const void *in, void *out, tm_dtype io_in, tm_dtype io_out,
+                 tm_dtype comp, void *work, const double *scale, hipStream_t s)
+{
+  if (comp == TM_F64)
+    kron_typed<double>(plan, in, out, io_in, io_out, work, scale, s, plan.tab64);
+  else
+    kron_typed<float>(plan, in, out, io_in, io_out, work, scale, s, plan.tab32);
+}
+
+}  // namespace ltm

@@ -1,0 +1,369 @@
+// tm_krylov.hip -- FP64 Krylov-basis kernels for the device eigensolver.
+//
+// The basis V is column-major (vector k at V + k*ldv).  Every reduction is
+// two-level and deterministic: a fixed grid of NBLK workgroups writes
+// partial sums, a single workgroup adds them in a fixed order.  Dot products
+// use wavefront (64-lane) shuffles, then LDS across the 4 wavefronts.
+// Orthogonalisation is classical Gram-Schmidt applied twice (CGS2): the
+// update kernel subtracts V h and, in the same sweep over the data, already
+// accumulates the dot products needed by the next correction.
+#include "tm_device.hpp"
+
+namespace ltm {
+namespace kry {
+namespace {
+
+constexpr int THREADS = 256;
+constexpr int MAXJ = 64;  // largest basis handled by the fused kernels
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Block-reduce `cnt` values acc[0..cnt) and write to out[0..cnt).
+__device__ __forceinline__ void block_reduce_write(double *acc, int cnt, double *out)
+{
+  __shared__ double red[THREADS / 64][MAXJ + 1];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int k = 0; k < cnt; ++k) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) red[wid][k] = v;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    double s = 0;
+    for (int w = 0; w < THREADS / 64; ++w) s += red[w][k];
+    out[k] = s;
+  }
+}
+
+// acc registers: one per dot product, limited to MAXJ+1
+template <int J>
+__global__ __launch_bounds__(THREADS) void k_multidot(const double *__restrict__ V, int64_t ldv,
+                                                      int j, const double *__restrict__ w,
+                                                      int64_t n, double *__restrict__ partial)
+{
+  double acc[J + 1];
+#pragma unroll
+  for (int k = 0; k <= J; ++k) acc[k] = 0;
+  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * THREADS) {
+    const double wi = w[i];
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k < j) acc[k] += V[k * ldv + i] * wi;
+    acc[J] += wi * wi;
+  }
+  // move <w,w> to slot j
+  double tmp[J + 1];
+#pragma unroll
+  for (int k = 0; k <= J; ++k) tmp[k] = (k < j) ? acc[k] : 0.0;
+  tmp[j < J ? j : J] = acc[J];
+  block_reduce_write(tmp, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
+}
+
+template <int J>
+__global__ __launch_bounds__(THREADS) void k_update_dots(const double *__restrict__ V,
+                                                         int64_t ldv, int j,
+                                                         double *__restrict__ w, int64_t n,
+                                                         const double *__restrict__ h,
+                                                         double *__restrict__ partial, int dots)
+{
+  double hk[J];
+#pragma unroll
+  for (int k = 0; k < J; ++k) hk[k] = k < j ? h[k] : 0.0;
+  double acc[J + 1];
+#pragma unroll
+  for (int k = 0; k <= J; ++k) acc[k] = 0;
+  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * THREADS) {
+    double vk[J];
+    double wi = w[i];
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k < j) {
+        vk[k] = V[k * ldv + i];
+        wi -= hk[k] * vk[k];
+      }
+    w[i] = wi;
+    if (dots) {
+#pragma unroll
+      for (int k = 0; k < J; ++k)
+        if (k < j) acc[k] += vk[k] * wi;
+    }
+    acc[J] += wi * wi;
+  }
+  double tmp[J + 1];
+  int cnt;
+  if (dots) {
+#pragma unroll
+    for (int k = 0; k <= J; ++k) tmp[k] = (k < j) ? acc[k] : 0.0;
+    tmp[j < J ? j : J] = acc[J];
+    cnt = j + 1;
+  } else {
+    tmp[0] = acc[J];
+#pragma unroll
+    for (int k = 1; k <= J; ++k) tmp[k] = 0;
+    cnt = 1;
+  }
+  block_reduce_write(tmp, cnt, partial + (int64_t)blockIdx.x * cnt);
+}
+
+__global__ void k_reduce_partials(const double *__restrict__ partial, int nblk, int cnt,
+                                  double *__restrict__ out)
+{
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    double s = 0;
+    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * cnt + k];
+    out[k] = s;
+  }
+}
+
+__global__ void k_normalize(double *__restrict__ w, int64_t n, const double *__restrict__ nrm2)
+{
+  const double inv = 1.0 / sqrt(nrm2[0]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    w[i] *= inv;
+}
+
+template <int M>
+__global__ __launch_bounds__(THREADS) void k_gemm_vy(const double *__restrict__ Vin, int64_t ldv,
+                                                     int m, const double *__restrict__ Y,
+                                                     int ldy, int k, double *Vout, int64_t ldo,
+                                                     int64_t n)
+{
+  __shared__ double sY[M * M];
+  for (int t = threadIdx.x; t < m * k; t += blockDim.x) {
+    const int r = t % m, c = t / m;
+    sY[r + c * M] = Y[r + (int64_t)c * ldy];
+  }
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * THREADS) {
+    double v[M];
+#pragma unroll
+    for (int r = 0; r < M; ++r) v[r] = r < m ? Vin[r * ldv + i] : 0.0;
+    // all reads of row i happen before any write of row i (Vout may alias)
+    for (int c = 0; c < k; ++c) {
+      double s = 0;
+#pragma unroll
+      for (int r = 0; r < M; ++r) s += v[r] * sY[r + c * M];
+      Vout[c * ldo + i] = s;
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z)
+{
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// uniform(0,1) from a counter-based hash of (seed, index)
+__global__ void k_random(double *v, int64_t n, uint64_t seed)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = splitmix64(splitmix64(seed) ^ (uint64_t)i);
+    v[i] = ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  }
+}
+
+__global__ void k_copy(const double *a, double *b, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+__global__ void k_to_f32(const double *a, float *b, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+__global__ void k_fill(double *a, double v, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+__global__ __launch_bounds__(THREADS) void k_maxabs(const double *x, int64_t n, double *partial)
+{
+  double m = 0;
+  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * THREADS)
+    m = fmax(m, fabs(x[i]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  __shared__ double red[THREADS / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0;
+    for (int w = 0; w < THREADS / 64; ++w) r = fmax(r, red[w]);
+    partial[blockIdx.x] = r;
+  }
+}
+
+__global__ void k_pow2_scale(const double *partial, int nblk, double *scale)
+{
+  if (threadIdx.x == 0) {
+    double m = 0;
+    for (int b = 0; b < nblk; ++b) m = fmax(m, partial[b]);
+    int e = 0;
+    if (m > 0) frexp(m, &e);
+    scale[0] = m > 0 ? ldexp(1.0, -e) : 1.0;
+  }
+}
+
+template <int J>
+__global__ __launch_bounds__(THREADS) void k_gram_row(const double *__restrict__ A,
+                                                      const double *__restrict__ B, int64_t ld,
+                                                      int k, int64_t n, int row,
+                                                      double *__restrict__ partial)
+{
+  double acc[J];
+#pragma unroll
+  for (int c = 0; c < J; ++c) acc[c] = 0;
+  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * THREADS) {
+    const double a = A[row * ld + i];
+#pragma unroll
+    for (int c = 0; c < J; ++c)
+      if (c < k) acc[c] += a * B[c * ld + i];
+  }
+  block_reduce_write(acc, k, partial + (int64_t)blockIdx.x * k);
+}
+
+unsigned nblocks(int64_t n)
+{
+  int64_t b = (n + THREADS - 1) / THREADS;
+  if (b > NBLK) b = NBLK;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace
+
+#define TM_JDISPATCH(KERNEL, jj, grid, ...)                                              \
+  do {                                                                                   \
+    if ((jj) <= 8) hipLaunchKernelGGL(KERNEL<8>, grid, dim3(THREADS), 0, s, __VA_ARGS__);  \
+    else if ((jj) <= 16) hipLaunchKernelGGL(KERNEL<16>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
+    else if ((jj) <= 32) hipLaunchKernelGGL(KERNEL<32>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
+    else hipLaunchKernelGGL(KERNEL<64>, grid, dim3(THREADS), 0, s, __VA_ARGS__);          \
+    TM_LAUNCH_CHECK();                                                                   \
+  } while (0)
+
+void multidot(const double *V, int64_t ldv, int j, const double *w, int64_t n, double *partial,
+              double *out, hipStream_t s)
+{
+  TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
+  const unsigned nb = nblocks(n);
+  TM_JDISPATCH(k_multidot, j, dim3(nb), V, ldv, j, w, n, partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
+  TM_LAUNCH_CHECK();
+}
+
+void update_dots(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h,
+                 double *partial, double *out, hipStream_t s)
+{
+  TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
+  const unsigned nb = nblocks(n);
+  TM_JDISPATCH(k_update_dots, j, dim3(nb), V, ldv, j, w, n, h, partial, 1);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
+  TM_LAUNCH_CHECK();
+}
+
+void update_norm(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h,
+                 double *partial, double *out, hipStream_t s)
+{
+  TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
+  const unsigned nb = nblocks(n);
+  TM_JDISPATCH(k_update_dots, j, dim3(nb), V, ldv, j, w, n, h, partial, 0);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, (int)nb, 1, out);
+  TM_LAUNCH_CHECK();
+}
+
+void normalize(double *w, int64_t n, const double *nrm2, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_normalize, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, w, n, nrm2);
+  TM_LAUNCH_CHECK();
+}
+
+void gemm_vy(const double *Vin, int64_t ldv, int m, const double *Y, int ldy, int k,
+             double *Vout, int64_t ldo, int64_t n, hipStream_t s)
+{
+  TM_CHECK(m <= MAXJ && k <= m, TM_EINVAL, "restart block too large");
+  const unsigned nb = nblocks(n) * 4;
+  if (m <= 8)
+    hipLaunchKernelGGL(k_gemm_vy<8>, dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y, ldy, k, Vout,
+                       ldo, n);
+  else if (m <= 16)
+    hipLaunchKernelGGL(k_gemm_vy<16>, dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y, ldy, k,
+                       Vout, ldo, n);
+  else if (m <= 32)
+    hipLaunchKernelGGL(k_gemm_vy<32>, dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y, ldy, k,
+                       Vout, ldo, n);
+  else
+    hipLaunchKernelGGL(k_gemm_vy<64>, dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y, ldy, k,
+                       Vout, ldo, n);
+  TM_LAUNCH_CHECK();
+}
+
+void random_uniform(double *v, int64_t n, uint64_t seed, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_random, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, v, n, seed);
+  TM_LAUNCH_CHECK();
+}
+
+void copy(const double *a, double *b, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_copy, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, a, b, n);
+  TM_LAUNCH_CHECK();
+}
+
+void to_f32(const double *a, float *b, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, a, b, n);
+  TM_LAUNCH_CHECK();
+}
+
+void fill(double *a, double val, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, a, val, n);
+  TM_LAUNCH_CHECK();
+}
+
+void pow2_scale(const double *x, int64_t n, double *partial, double *scale, hipStream_t s)
+{
+  const unsigned nb = nblocks(n);
+  hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(THREADS), 0, s, x, n, partial);
+  hipLaunchKernelGGL(k_pow2_scale, dim3(1), dim3(64), 0, s, partial, (int)nb, scale);
+  TM_LAUNCH_CHECK();
+}
+
+void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double *partial,
+          double *G, hipStream_t s)
+{
+  TM_CHECK(k <= MAXJ, TM_EINVAL, "gram block too large");
+  const unsigned nb = nblocks(n);
+  for (int row = 0; row < k; ++row) {
+    TM_JDISPATCH(k_gram_row, k, dim3(nb), A, B, ld, k, n, row, partial);
+    // G[row + c*k] = <A_row, B_c>
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, (int)nb, k,
+                       G + row * k);
+    TM_LAUNCH_CHECK();
+  }
+}
+
+}  // namespace kry
+}  // namespace ltm

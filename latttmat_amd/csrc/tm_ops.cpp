@@ -1,0 +1,467 @@
+// tm_ops.cpp -- operator construction (the pfunc dispatch of the reference,
+// TMat/pfunc.cpp:16-106) and device table set-up.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "tm_internal.hpp"
+
+namespace ltm {
+
+size_t g_lds_per_block = 64 * 1024;
+
+[[noreturn]] void throw_hip(hipError_t e, const char *expr, const char *file, int line)
+{
+  char buf[512];
+  std::snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e,
+                hipGetErrorString(e), expr, file, line);
+  throw Error(e == hipErrorOutOfMemory ? TM_ENOMEM : TM_EHIP, buf);
+}
+
+void DevBuf::alloc(size_t bytes)
+{
+  release();
+  if (bytes == 0) return;
+  hipError_t e = hipMalloc(&p_, bytes);
+  if (e != hipSuccess) {
+    p_ = nullptr;
+    char buf[128];
+    std::snprintf(buf, sizeof buf, "device allocation of %zu bytes failed", bytes);
+    (void)hipGetLastError();
+    throw Error(TM_ENOMEM, buf);
+  }
+  n_ = bytes;
+}
+
+void DevBuf::release()
+{
+  if (p_) (void)hipFree(p_);
+  p_ = nullptr;
+  n_ = 0;
+}
+
+namespace {
+
+int pop64(uint64_t x) { return __builtin_popcountll(x); }
+uint64_t rotr_h(uint64_t s, int n, int d) { return (s >> d) | ((s & ((1ull << d) - 1)) << (n - d)); }
+
+// 3D column masks for DiagSpec (bits of columns [0,c) of every row)
+uint64_t colmask(int s1, int s2, int c)
+{
+  uint64_t m = 0;
+  for (int r = 0; r < s1; ++r) m |= ((1ull << c) - 1) << (r * s2);
+  return m;
+}
+
+void upload_tables(tm_op *op, const std::vector<double> &t)
+{
+  std::vector<float> f(t.begin(), t.end());
+  op->tab64.alloc(192 * sizeof(double));
+  op->tab32.alloc(192 * sizeof(float));
+  TM_HIP(hipMemcpy(op->tab64.get(), t.data(), 192 * sizeof(double), hipMemcpyHostToDevice));
+  TM_HIP(hipMemcpy(op->tab32.get(), f.data(), 192 * sizeof(float), hipMemcpyHostToDevice));
+}
+
+// normfactor of the X-type 2D operators (TMat/tmat.cpp:48-55,
+// TMat/tmatreduce.cpp:349-359); exponent returned in *e (units of betaJ)
+double nf_x2d(int L, double bJ, double k, double *e)
+{
+  *e = (k < 0.5) ? L * (k - 2.0) : -L * (k + 1.0);
+  return std::exp(bJ * *e);
+}
+// 3D (TMat/tmat3d.cpp:117-127)
+double nf_x3d(int size, double bJ, double k, double *e)
+{
+  *e = (k < 0.5) ? size * (k - 3.0) : -size * (k + 2.0);
+  return std::exp(bJ * *e);
+}
+
+// Tv^1/2 tables of XAT (genTvec with betaJ/2, TMat/tmat.hpp:208-310),
+// sqrt(nf) folded into the first table.
+std::vector<double> tables_xat(int L, int bc, double bJ, double k, double h, double sqnf)
+{
+  const double J1 = -0.5 * bJ, J2 = 0.5 * bJ * k, hJ = -0.5 * bJ * h;
+  int c1 = L, c2 = L;
+  switch (bc) {
+    case 1: c1 = L - 1; c2 = L - 2; break;
+    case 5: case 9: c1 = L + 1; c2 = L; break;
+    case 13: case 17: c1 = L + 1; c2 = L + 2; break;
+    default: break;
+  }
+  std::vector<double> t(192, 0.0);
+  for (int n = 0; n < 64; ++n) {
+    t[n] = sqnf * std::exp(J1 * (2.0 * n - c1));
+    t[64 + n] = std::exp(J2 * (2.0 * n - c2));
+    t[128 + n] = std::exp(hJ * (2.0 * n - L));
+  }
+  return t;
+}
+
+double energy_half_x2d(uint64_t s, int L, int obc, double bJ, double k, double h)
+{
+  const uint64_t mask = (1ull << L) - 1;
+  int n1, n2;
+  if (!obc) {
+    n1 = pop64(s ^ rotr_h(s, L, 1));
+    n2 = pop64(s ^ rotr_h(s, L, 2));
+  } else {
+    n1 = pop64((s & (mask >> 1)) ^ (s >> 1));
+    n2 = pop64((s & (mask >> 2)) ^ (s >> 2));
+  }
+  const int n0 = pop64(s);
+  const double hb = 0.5 * bJ;
+  return -hb * (2 * n1 - L) + hb * k * (2 * n2 - L) - hb * h * (2 * n0 - L);
+}
+
+uint64_t rff_h(uint64_t s, int s1, int s2, int d)
+{
+  const uint64_t m = (1ull << s2) - 1;
+  uint64_t r = 0;
+  for (int i = 0; i < s1; ++i) {
+    const uint64_t row = (s >> (i * s2)) & m;
+    r |= rotr_h(row, s2, d) << (i * s2);
+  }
+  return r;
+}
+
+// 3D layer exponent, TMat/tmat3d.cpp:188-199 (without the normfactor term)
+double energy_half_x3d(uint64_t s, int s1, int s2, double bJ, double k, double h)
+{
+  const int size = s1 * s2;
+  const double hb = 0.5 * bJ;
+  const int n1a = pop64(s ^ rff_h(s, s1, s2, 1));
+  const int n1b = pop64(s ^ rotr_h(s, size, s2));
+  const int n2 = pop64(s ^ rff_h(s, s1, s2, 2));
+  const int n0 = pop64(s);
+  return 2 * (-hb) * (n1a + n1b - size) + hb * k * (2 * n2 - size) - hb * h * (2 * n0 - size);
+}
+
+void set_bz(tm_op *op)
+{
+  const double bJ = op->p.betaJ;
+  op->bzwj1[0] = std::exp(bJ);
+  op->bzwj1[1] = 1.0 / op->bzwj1[0];
+  op->bzwj2[0] = std::exp(-bJ * op->p.kappa);
+  op->bzwj2[1] = 1.0 / op->bzwj2[0];
+  op->bzwj3[0] = std::exp(-bJ * op->p.kappa2);
+  op->bzwj3[1] = 1.0 / op->bzwj3[0];
+}
+
+void init_device()
+{
+  static bool done = false;
+  if (done) return;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    throw Error(TM_EHIP, "no HIP device available: the MI355X kernels cannot run");
+  }
+  int dev = 0;
+  TM_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  TM_HIP(hipGetDeviceProperties(&prop, dev));
+  g_lds_per_block = prop.sharedMemPerBlock;
+  done = true;
+}
+
+// Reduced 2D/3D operators: orbit tables + Tvec on the device.
+void build_reduced(tm_op *op, const GroupSpec &g, double nfexp, bool want_fused)
+{
+  const bool fused = want_fused && g.dim == 2 && g.bc == 0 && (g.L % 2) == 0 && g.L >= 8 &&
+                     ((size_t(1) << (g.L / 2)) * sizeof(double) <= g_lds_per_block);
+  const int64_t M = enumerate_orbits(g, op->reps, op->dups, fused ? nullptr : &op->map, op->stream);
+  op->ncol = M;
+  std::vector<uint64_t> reps(M);
+  std::vector<uint16_t> dups(M);
+  TM_HIP(hipMemcpy(reps.data(), op->reps.get(), M * 8, hipMemcpyDeviceToHost));
+  TM_HIP(hipMemcpy(dups.data(), op->dups.get(), M * 2, hipMemcpyDeviceToHost));
+  std::vector<double> tv(M), dt(M);
+  const double Jn = 0.5 * op->p.betaJ * nfexp;
+  for (int64_t o = 0; o < M; ++o) {
+    double e;
+    if (g.dim == 2) e = energy_half_x2d(reps[o], g.L, g.bc, op->p.betaJ, op->p.kappa, op->p.h);
+    else e = energy_half_x3d(reps[o], g.s1, g.s2, op->p.betaJ, op->p.kappa, op->p.h);
+    tv[o] = std::exp(e + Jn) / std::sqrt((double)dups[o]);
+    dt[o] = dups[o] * tv[o];
+  }
+  std::vector<float> tv32(tv.begin(), tv.end()), dt32(dt.begin(), dt.end());
+  op->tvec.alloc(M * 8);
+  op->dt64.alloc(M * 8);
+  op->tvec32.alloc(M * 4);
+  op->dt32.alloc(M * 4);
+  TM_HIP(hipMemcpy(op->tvec.get(), tv.data(), M * 8, hipMemcpyHostToDevice));
+  TM_HIP(hipMemcpy(op->dt64.get(), dt.data(), M * 8, hipMemcpyHostToDevice));
+  TM_HIP(hipMemcpy(op->tvec32.get(), tv32.data(), M * 4, hipMemcpyHostToDevice));
+  TM_HIP(hipMemcpy(op->dt32.get(), dt32.data(), M * 4, hipMemcpyHostToDevice));
+  if (fused) {
+    op->ft.reset(new FusedTables());
+    build_fused_tables(g, op->reps, M, *op->ft, op->stream);
+  }
+}
+
+}  // namespace
+
+tm_op *make_op(const tm_params &p)
+{
+  init_device();
+  TM_CHECK(p.model == 'A' || p.model == 'B' || p.model == 'D', TM_EINVAL,
+           "invalid model (expected A, B or D)");
+  TM_CHECK(p.dim == 2 || p.dim == 3, TM_EINVAL, "invalid dim (expected 2 or 3)");
+  std::unique_ptr<tm_op> op(new tm_op());
+  op->p = p;
+  set_bz(op.get());
+  const int multieigs = p.neigs > 1 ? 2 : 0;  // TMat/pfunc.cpp:22
+  const double bJ = p.betaJ, k = p.kappa, h = p.h;
+  auto make_xat = [&](int L, int bc) {
+    TM_CHECK(L >= 4 && L <= 34, TM_EINVAL, "XAT width must be in [4, 34]");
+    op->kind = TM_KIND_XAT;
+    op->L = L;
+    op->bc = bc;
+    op->nstate = op->ncol = int64_t(1) << L;
+    double e;
+    op->nf = nf_x2d(L, bJ, k, &e);  // TMat/tmat.cpp:48-55
+    op->diag = DiagSpec{bc, L, 0, 0, 0, 0, 0, 0};
+    upload_tables(op.get(), tables_xat(L, bc, bJ, k, h, std::sqrt(op->nf)));
+  };
+  auto make_xam = [&](int L, int ext) {
+    TM_CHECK(L >= 4 && L <= 32, TM_EINVAL, "XAM width must be in [4, 32]");
+    op->kind = TM_KIND_XAM2D;
+    op->L = L;
+    op->bc = ext & 1;
+    op->censym = (h == 0 && (ext & 2) == 0);  // TMat/tmatreduce.cpp:341
+    op->nleadbit = (L < 10 || op->bc) ? 1 : 2 + (int)std::log2((double)L);
+    op->nstate = int64_t(1) << L;
+    double e;
+    op->nf = nf_x2d(L, bJ, k, &e);
+    GroupSpec g{2, L, 0, 0, op->bc, op->censym, 0};
+    build_reduced(op.get(), g, e, !(p.flags & TM_FLAG_GENERIC_REDUCED));
+  };
+  auto make_xdt = [&](int L, int bc) {
+    TM_CHECK(L >= 4 && L <= 32, TM_EINVAL, "XDT width must be in [4, 32]");
+    op->kind = TM_KIND_XDT;
+    op->L = L;
+    op->bc = bc;
+    op->nstate = op->ncol = int64_t(1) << L;
+    // TMat/tmat.cpp:333-340
+    op->nf = (k < 0.5) ? std::exp(L * bJ * 2.0 * (k - 1.0)) : std::exp(-L * bJ * 2.0 * k);
+    const int dk = bc == 0 ? DIAG_XDT_PBC : (bc == 5 ? DIAG_XDT_C5 : DIAG_XDT_C9);
+    op->diag = DiagSpec{dk, L, 0, 0, 0, 0, 0, 0};
+    // genTvec with betaJ/2 (TMat/tmat.hpp:437-504)
+    const double J1 = -0.5 * bJ, hJ = -0.5 * bJ * h;
+    const int c1 = bc == 0 ? L : L + 1;
+    std::vector<double> t(192, 0.0);
+    for (int n = 0; n < 64; ++n) {
+      t[n] = std::exp(J1 * (2.0 * n - c1));
+      t[64 + n] = 1.0;
+      t[128 + n] = std::exp(hJ * (2.0 * n - L));
+    }
+    upload_tables(op.get(), t);
+  };
+  auto make_zbt = [&](int L) {
+    TM_CHECK(L >= 4 && L <= 30 && (L % 2) == 0, TM_EINVAL,
+             "ZBT needs an even width in [4, 30] (TMat/tmat.cpp:620)");
+    op->kind = TM_KIND_ZBT;
+    op->symmetric = false;
+    op->L = L;
+    op->nstate = op->ncol = int64_t(1) << L;
+    const double k2 = p.kappa2;
+    double nfe;  // TMat/tmat.cpp:491-511
+    if (k + k2 < 0.5) nfe = L * 2.0 * (k + k2 - 1.0);
+    else if (k < 0.5 && k2 - k < 0.5) nfe = -L;
+    else if (k2 < 2 * k) nfe = -L * 2.0 * k;
+    else nfe = L * 2.0 * (k - k2);
+    const double half = std::exp(0.5 * bJ * nfe);
+    op->nf = half * half;
+    op->diag = DiagSpec{DIAG_ZBT, L, 0, 0, 0, 0, 0, 0};
+    // genTvec with betaJ/2 (TMat/tmat.hpp:537-556), halfnormfactor folded
+    const double J1 = -0.5 * bJ, J3 = 0.5 * k2 * bJ, hJ = -0.5 * bJ * h;
+    std::vector<double> t(192, 0.0);
+    for (int n = 0; n < 64; ++n) {
+      t[n] = half * std::exp(J1 * (2.0 * n - L));
+      t[64 + n] = std::exp(J3 * (2.0 * n - L));
+      t[128 + n] = std::exp(hJ * (2.0 * n - L));
+    }
+    upload_tables(op.get(), t);
+  };
+
+  if (p.model == 'A' && p.dim == 2) {
+    const int L = p.size1;
+    TM_CHECK(L >= 4, TM_EINVAL, "Invalid size (<4)");  // TMat/read_input.cpp:86-90
+    switch (p.axialdir) {
+      case 1: make_xat(L, 0); break;
+      case 9: make_xat(L, 1); break;
+      case 17: make_xat(L, 5); break;
+      case 18: make_xat(L, 9); break;
+      case 19: make_xat(L, 13); break;
+      case 20: make_xat(L, 17); break;
+      case 5: make_xam(L, multieigs); break;
+      case 13: make_xam(L, multieigs | 1); break;
+      case 0: case 2: case 6:
+        throw Error(TM_EUNSUPPORTED, "ANNNI parallel-TM paths (axialdir 0, 2, 6) are not built");
+      default:
+        throw Error(TM_EINVAL, "Function yet implemented (unknown axialdir)");
+    }
+  } else if (p.model == 'A' && p.dim == 3) {
+    TM_CHECK(p.axialdir == 1, TM_EINVAL, "3D supports axialdir 1 only (TMat/pfunc.cpp:63-75)");
+    const int s1 = p.size1, s2 = p.size2, size = s1 * s2;
+    TM_CHECK(s1 >= 2 && s2 >= 3 && size <= 32, TM_EINVAL, "3D layer must be s1>=2, s2>=3, s1*s2<=32");
+    op->s1 = s1;
+    op->s2 = s2;
+    op->L = size;
+    op->nstate = int64_t(1) << size;
+    double e;
+    op->nf = nf_x3d(size, bJ, k, &e);
+    if (p.flags & TM_FLAG_FULLSPACE) {
+      op->kind = TM_KIND_XAT3D;
+      op->ncol = op->nstate;
+      op->diag = DiagSpec{DIAG_X3D, size, s1, s2, colmask(s1, s2, 1), colmask(s1, s2, s2 - 1),
+                          colmask(s1, s2, 2), colmask(s1, s2, s2 - 2)};
+      const double hb = 0.5 * bJ, Jn = hb * e;
+      std::vector<double> t(192, 0.0);
+      for (int n = 0; n < 64; ++n) {
+        t[n] = std::exp(2 * (-hb) * (n - size) + Jn);
+        t[64 + n] = std::exp(hb * k * (2.0 * n - size));
+        t[128 + n] = std::exp(-hb * h * (2.0 * n - size));
+      }
+      upload_tables(op.get(), t);
+    } else {
+      op->kind = TM_KIND_XAM3D;
+      op->censym = (h == 0);                    // TMat/tmat3d.cpp:105
+      op->rotate = (k == 0) && (s1 == s2);
+      op->nleadbit = size < 10 ? 1 : 2 + (int)std::log2((double)size);
+      GroupSpec g{3, size, s1, s2, 0, op->censym, op->rotate};
+      build_reduced(op.get(), g, e, false);
+    }
+  } else if (p.model == 'B') {
+    TM_CHECK(p.dim == 2, TM_EINVAL, "BNNNI is 2D only");
+    switch (p.axialdir) {
+      case 3: make_zbt(p.size1); break;
+      case 6: case 7:
+        throw Error(TM_EUNSUPPORTED, "BNNNI reduced paths (axialdir 6, 7) are not built");
+      default:
+        throw Error(TM_EINVAL, "Function yet implemented (unknown axialdir)");
+    }
+  } else {  // 'D'
+    TM_CHECK(p.dim == 2, TM_EINVAL, "DNNI is 2D only");
+    switch (p.axialdir) {
+      case 1: case 2: make_xdt(p.size1, 0); break;
+      case 17: make_xdt(p.size1, 5); break;
+      case 18: make_xdt(p.size1, 9); break;
+      case 5: case 6:
+        throw Error(TM_EUNSUPPORTED, "DNNI reduced path (axialdir 5, 6) is not built");
+      default:
+        throw Error(TM_EINVAL, "Function yet implemented (unknown axialdir)");
+    }
+  }
+  op->one.alloc(sizeof(double));
+  const double one = 1.0;
+  TM_HIP(hipMemcpy(op->one.get(), &one, sizeof one, hipMemcpyHostToDevice));
+  return op.release();
+}
+
+void dlnnf(const tm_op *op, double out[4])
+{
+  // derivative of ln(normfactor) w.r.t. (betaJ, kappa, kappa2, h)
+  const double bJ = op->p.betaJ, k = op->p.kappa, k2 = op->p.kappa2;
+  const int L = op->L;
+  for (int i = 0; i < 4; ++i) out[i] = 0;
+  switch (op->kind) {
+    case TM_KIND_XAT:
+    case TM_KIND_XAM2D:
+      if (k < 0.5) { out[0] = L * (k - 2.0); out[1] = L * bJ; }
+      else { out[0] = -L * (k + 1.0); out[1] = -L * bJ; }
+      break;
+    case TM_KIND_XAM3D:
+    case TM_KIND_XAT3D:
+      if (k < 0.5) { out[0] = L * (k - 3.0); out[1] = L * bJ; }
+      else { out[0] = -L * (k + 2.0); out[1] = -L * bJ; }
+      break;
+    case TM_KIND_XDT:
+      if (k < 0.5) { out[0] = L * 2.0 * (k - 1.0); out[1] = L * bJ * 2.0; }
+      else { out[0] = -L * 2.0 * k; out[1] = -L * bJ * 2.0; }
+      break;
+    case TM_KIND_ZBT: {
+      if (k + k2 < 0.5) { out[0] = L * 2.0 * (k + k2 - 1.0); out[1] = out[2] = 2.0 * L * bJ; }
+      else if (k < 0.5 && k2 - k < 0.5) { out[0] = -L; }
+      else if (k2 < 2 * k) { out[0] = -L * 2.0 * k; out[1] = -2.0 * L * bJ; }
+      else { out[0] = L * 2.0 * (k - k2); out[1] = 2.0 * L * bJ; out[2] = -2.0 * L * bJ; }
+      break;
+    }
+    default: break;
+  }
+}
+
+}  // namespace ltm
+
+tm_op::~tm_op() = default;
+
+void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
+                      int transpose, const double *scale_in, hipStream_t s)
+{
+  using namespace ltm;
+  const size_t es = io == TM_F64 ? 8 : 4;
+  for (int v = 0; v < nvec; ++v) {
+    const void *xv = static_cast<const char *>(x) + (size_t)v * ld * es;
+    void *yv = static_cast<char *>(y) + (size_t)v * ld * es;
+    switch (kind) {
+      case TM_KIND_XAT:
+      case TM_KIND_XAT3D: {
+        KronPlan kp{};
+        kp.L = L;
+        kp.prologue = kp.epilogue = true;
+        kp.diag = diag;
+        kp.tab64 = tab64.as<double>();
+        kp.tab32 = tab32.as<float>();
+        kp.ka = bzwj1[0];
+        kp.kb = bzwj1[1];
+        work.ensure(kron_work_bytes(L, comp));
+        launch_kron(kp, xv, yv, io, io, comp, work.get(), scale_in, s);
+        break;
+      }
+      case TM_KIND_XDT:
+      case TM_KIND_ZBT: {
+        SweepPlan sp{};
+        sp.kind = kind;
+        sp.L = L;
+        sp.bc = bc;
+        sp.diag = diag;
+        sp.tab64 = tab64.as<double>();
+        sp.tab32 = tab32.as<float>();
+        for (int i = 0; i < 2; ++i) {
+          sp.b1[i] = bzwj1[i];
+          sp.b2[i] = bzwj2[i];
+          sp.b3[i] = bzwj3[i];
+        }
+        sp.nf = nf;
+        work.ensure(sweep_work_bytes(kind, L, comp));
+        launch_sweep(sp, xv, yv, io, comp, kind == TM_KIND_ZBT ? transpose : 0, work.get(),
+                     scale_in, s);
+        break;
+      }
+      case TM_KIND_XAM2D:
+      case TM_KIND_XAM3D: {
+        ReducedPlan rp{};
+        rp.M = ncol;
+        rp.L = L;
+        rp.tvec = tvec.as<double>();
+        rp.dt64 = dt64.as<double>();
+        rp.tvec32 = tvec32.as<float>();
+        rp.dt32 = dt32.as<float>();
+        rp.reps = reps.as<uint64_t>();
+        rp.map = map.as<uint32_t>();
+        rp.ft = ft.get();
+        rp.ka = bzwj1[0];
+        rp.kb = bzwj1[1];
+        if (!rp.ft) TM_CHECK(rp.map != nullptr, TM_EINVAL, "reduced operator has no state map");
+        work.ensure(reduced_work_bytes(rp, comp));
+        launch_reduced(rp, xv, yv, io, comp, work.get(), scale_in, s);
+        break;
+      }
+      default:
+        throw Error(TM_EINVAL, "operator kind not applicable");
+    }
+    ++matvecs;
+  }
+}

@@ -1,0 +1,630 @@
+// tm_reduced.hip -- symmetry-reduced transfer matrices on gfx950.
+//
+// Orbit tables.  The reference enumerates orbits by a sequential scan with a
+// visited bitset (TMat/tmatreduce.cpp:19-111, TMat/tmat3d.cpp:17-98): the
+// representative of an orbit is its smallest state, representatives come out
+// ascending and the duplicate count is the orbit size.  Here every state is
+// canonicalised in parallel (minimum over the group images), the "is its own
+// minimum" flags are packed into a bitmap, and a prefix sum over the bitmap
+// words gives each representative its index -- the same ascending order.
+// The orbit size is |G| / |Stab(s)|, counted over the same group images.
+//
+// Operator.  T_red(o,o') = (d_o d_o')^-1/2 sum_{s in o, s' in o'} T(s,s')
+// (the algebra behind XAMMatVecProd2D::perform_op, TMat/tmatreduce.cpp:406-529):
+//   y_o = d_o Tvec_o [ (x)K  u ](rep_o),   u(s) = Tvec_{o(s)} x_{o(s)}.
+// Generic path: expand u over the full space with a state->orbit map, run
+// the Kronecker passes, gather at the representatives.
+// Fused path (2D periodic, L = 2n even): uses two symmetries of u,
+//   rotation by n:      u(b, a) = u(a, b)           (state = b<<n | a)
+//   reversal + rot n:   u(b, a) = u(rev b, rev a)
+// so that the transform over the high half of every state is done on
+// contiguous rows of the map, only rows b that occur as the high half of a
+// representative (or its reversal) are kept, and only columns a <= rev(a)
+// are computed.  Pass 2 transforms the kept rows over the low half and
+// writes y at the representatives.  The expanded 2^L vector never exists.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <numeric>
+
+#include "tm_device.hpp"
+
+namespace ltm {
+namespace {
+
+using dev::pop;
+
+// ---------------------------------------------------------------------------
+// group actions
+// ---------------------------------------------------------------------------
+struct GroupDev {
+  int dim, L, s1, s2, bc, censym, rotate;
+  uint64_t mask;
+  uint64_t cm1, cmr1;  // 3D column masks for row rotation by 1
+  uint64_t rowmask;    // low s2 bits
+  int order;           // |G|
+};
+
+__device__ __forceinline__ uint64_t row_rot1(uint64_t s, const GroupDev &g)
+{
+  return ((s >> 1) & g.cmr1) | ((s & g.cm1) << (g.s2 - 1));
+}
+
+// reverse the order of the s1 rows (reflectState_3D(.,1), utility.cpp:109-117)
+__device__ __forceinline__ uint64_t rows_reverse(uint64_t s, const GroupDev &g)
+{
+  uint64_t r = 0;
+  for (int i = 0; i < g.s1; ++i) {
+    r = (r << g.s2) | (s & g.rowmask);
+    s >>= g.s2;
+  }
+  return r;
+}
+// reverse the bits inside every row (reflectState_3D(.,0), utility.cpp:100-108)
+__device__ __forceinline__ uint64_t rows_mirror(uint64_t s, const GroupDev &g)
+{
+  return rows_reverse(dev::brev(s, g.L), g);
+}
+// rotateState_3D (utility.cpp:78-91) for a square layer
+__device__ __forceinline__ uint64_t layer_rot90(uint64_t s, const GroupDev &g)
+{
+  const int n = g.s1;
+  uint64_t r = 0;
+  for (int rp = 0; rp < n; ++rp)
+    for (int rq = 0; rq < n; ++rq)
+      if (s & (1ull << (rp * n + rq))) r |= 1ull << ((rq + 1) * n - (rp + 1));
+  return r;
+}
+
+// Visit every group image of s (with multiplicity |G| in total) and return
+// the minimum; *stab receives the number of images equal to s.
+__device__ uint64_t canon(uint64_t s, const GroupDev &g, int *stab)
+{
+  uint64_t m = s;
+  int st = 0;
+  if (g.dim == 2) {
+    const int nf = g.censym ? 2 : 1;
+    for (int f = 0; f < nf; ++f) {
+      const uint64_t s0 = f ? (~s & g.mask) : s;
+      if (g.bc == 0) {
+        uint64_t t = s0;
+        for (int k = 0; k < g.L; ++k) {
+          const uint64_t r = dev::brev(t, g.L);
+          m = t < m ? t : m;
+          m = r < m ? r : m;
+          st += (t == s) + (r == s);
+          t = dev::rotr(t, g.L, 1);
+        }
+      } else {
+        const uint64_t r = dev::brev(s0, g.L);
+        m = s0 < m ? s0 : m;
+        m = r < m ? r : m;
+        st += (s0 == s) + (r == s);
+      }
+    }
+  } else {
+    const int nrot = g.rotate ? 2 : 1, nf = g.censym ? 2 : 1;
+    uint64_t sr = s;
+    for (int ir = 0; ir < nrot; ++ir) {
+      for (int f = 0; f < nf; ++f) {
+        const uint64_t sf = f ? (~sr & g.mask) : sr;
+        for (int m1 = 0; m1 < 2; ++m1) {
+          const uint64_t sm1 = m1 ? rows_mirror(sf, g) : sf;
+          for (int m2 = 0; m2 < 2; ++m2) {
+            uint64_t t = m2 ? rows_reverse(sm1, g) : sm1;
+            for (int a = 0; a < g.s2; ++a) {
+              uint64_t u = t;
+              for (int b = 0; b < g.s1; ++b) {
+                m = u < m ? u : m;
+                st += (u == s);
+                u = dev::rotr(u, g.L, g.s2);
+              }
+              t = row_rot1(t, g);
+            }
+          }
+        }
+      }
+      sr = layer_rot90(sr, g);
+    }
+  }
+  if (stab) *stab = st;
+  return m;
+}
+
+// bitmap[w] bit j = (state 64w+j is the minimum of its orbit); wcount[w] = popcount
+__global__ void k_canon_bits(GroupDev g, uint64_t nstate, uint64_t *bitmap, uint32_t *wcount,
+                             uint64_t nwords)
+{
+  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords;
+       w += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t bits = 0;
+    const uint64_t s0 = w << 6;
+    for (int j = 0; j < 64; ++j) {
+      const uint64_t s = s0 + j;
+      if (s >= nstate) break;
+      if (canon(s, g, nullptr) == s) bits |= 1ull << j;
+    }
+    bitmap[w] = bits;
+    wcount[w] = (uint32_t)pop(bits);
+  }
+}
+
+// Block-local exclusive scan over 4096-entry chunks of wcount, plus chunk sums.
+constexpr int SCAN_CHUNK = 4096;
+constexpr int SCAN_THREADS = 256;
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_chunk_sum(const uint32_t *in, uint64_t n,
+                                                            uint64_t *sums)
+{
+  __shared__ uint64_t red[SCAN_THREADS];
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_CHUNK;
+  uint64_t acc = 0;
+  for (int i = threadIdx.x; i < SCAN_CHUNK; i += SCAN_THREADS)
+    if (base + i < n) acc += in[base + i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = SCAN_THREADS / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_chunk_scan(const uint32_t *in, uint64_t n,
+                                                             const uint64_t *offs,
+                                                             uint64_t *out)
+{
+  // each thread scans 16 consecutive entries; then a block scan of thread sums
+  __shared__ uint64_t ts[SCAN_THREADS];
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_CHUNK + threadIdx.x * 16;
+  uint64_t loc[16];
+  uint64_t acc = 0;
+  for (int i = 0; i < 16; ++i) {
+    loc[i] = acc;
+    if (base + i < n) acc += in[base + i];
+  }
+  ts[threadIdx.x] = acc;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of ts
+  for (int o = 1; o < SCAN_THREADS; o <<= 1) {
+    uint64_t v = threadIdx.x >= o ? ts[threadIdx.x - o] : 0;
+    __syncthreads();
+    ts[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const uint64_t pre = offs[blockIdx.x] + (threadIdx.x ? ts[threadIdx.x - 1] : 0);
+  for (int i = 0; i < 16; ++i)
+    if (base + i < n) out[base + i] = pre + loc[i];
+}
+
+__global__ void k_write_reps(GroupDev g, const uint64_t *bitmap, const uint64_t *wpre,
+                             uint64_t nwords, uint64_t *reps, uint16_t *dups)
+{
+  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords;
+       w += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t bits = bitmap[w];
+    uint64_t idx = wpre[w];
+    while (bits) {
+      const int j = __ffsll((unsigned long long)bits) - 1;
+      bits &= bits - 1;
+      const uint64_t s = (w << 6) | j;
+      int st = 0;
+      canon(s, g, &st);
+      reps[idx] = s;
+      dups[idx] = (uint16_t)(g.order / st);
+      ++idx;
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t rank_of(uint64_t rep, const uint64_t *bitmap,
+                                            const uint64_t *wpre)
+{
+  const uint64_t w = rep >> 6;
+  const uint64_t below = bitmap[w] & ((1ull << (rep & 63)) - 1);
+  return (uint32_t)(wpre[w] + pop(below));
+}
+
+__global__ void k_build_map(GroupDev g, uint64_t nstate, const uint64_t *bitmap,
+                            const uint64_t *wpre, uint32_t *map)
+{
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < nstate;
+       s += (uint64_t)gridDim.x * blockDim.x)
+    map[s] = rank_of(canon(s, g, nullptr), bitmap, wpre);
+}
+
+__global__ void k_build_cmap(GroupDev g, int n, const uint32_t *cols, uint64_t Q,
+                             const uint64_t *bitmap, const uint64_t *wpre, uint32_t *cmap)
+{
+  const uint64_t total = Q << n;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t q = i >> n, b = i & ((1ull << n) - 1);
+    const uint64_t s = ((uint64_t)cols[q] << n) | b;
+    cmap[i] = rank_of(canon(s, g, nullptr), bitmap, wpre);
+  }
+}
+
+GroupDev to_dev(const GroupSpec &gs)
+{
+  GroupDev g{};
+  g.dim = gs.dim;
+  g.L = gs.L;
+  g.s1 = gs.s1;
+  g.s2 = gs.s2;
+  g.bc = gs.bc;
+  g.censym = gs.censym;
+  g.rotate = gs.rotate;
+  g.mask = (1ull << gs.L) - 1;
+  if (gs.dim == 3) {
+    uint64_t c1 = 0, cr1 = 0;
+    for (int r = 0; r < gs.s1; ++r) {
+      c1 |= 1ull << (r * gs.s2);
+      cr1 |= ((1ull << (gs.s2 - 1)) - 1) << (r * gs.s2);
+    }
+    g.cm1 = c1;
+    g.cmr1 = cr1;
+    g.rowmask = (1ull << gs.s2) - 1;
+    g.order = gs.s1 * gs.s2 * 4 * (gs.censym ? 2 : 1) * (gs.rotate ? 2 : 1);
+  } else {
+    g.order = (gs.bc == 0 ? 2 * gs.L : 2) * (gs.censym ? 2 : 1);
+  }
+  return g;
+}
+
+unsigned grid_for(uint64_t n, int threads)
+{
+  uint64_t b = (n + threads - 1) / threads;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+// Exclusive prefix of the word counts -> wpre (device), returns the total.
+uint64_t scan_words(const uint32_t *wcount, uint64_t nwords, uint64_t *wpre, hipStream_t s)
+{
+  const uint64_t nchunk = (nwords + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  DevBuf sums(nchunk * sizeof(uint64_t)), offs(nchunk * sizeof(uint64_t));
+  hipLaunchKernelGGL(k_chunk_sum, dim3((unsigned)nchunk), dim3(SCAN_THREADS), 0, s, wcount,
+                     nwords, sums.as<uint64_t>());
+  TM_LAUNCH_CHECK();
+  std::vector<uint64_t> h(nchunk), o(nchunk);
+  TM_HIP(hipMemcpyAsync(h.data(), sums.get(), nchunk * 8, hipMemcpyDeviceToHost, s));
+  TM_HIP(hipStreamSynchronize(s));
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < nchunk; ++i) { o[i] = acc; acc += h[i]; }
+  TM_HIP(hipMemcpyAsync(offs.get(), o.data(), nchunk * 8, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_chunk_scan, dim3((unsigned)nchunk), dim3(SCAN_THREADS), 0, s, wcount,
+                     nwords, offs.as<uint64_t>(), wpre);
+  TM_LAUNCH_CHECK();
+  TM_HIP(hipStreamSynchronize(s));
+  return acc;
+}
+
+struct OrbitScratch {
+  DevBuf bitmap, wcount, wpre;
+  uint64_t nwords = 0;
+};
+
+uint64_t orbit_bitmap(const GroupDev &g, OrbitScratch &sc, hipStream_t s)
+{
+  const uint64_t nstate = 1ull << g.L;
+  sc.nwords = (nstate + 63) / 64;
+  sc.bitmap.alloc(sc.nwords * 8);
+  sc.wcount.alloc(sc.nwords * 4);
+  sc.wpre.alloc(sc.nwords * 8);
+  hipLaunchKernelGGL(k_canon_bits, dim3(grid_for(sc.nwords, 256)), dim3(256), 0, s, g, nstate,
+                     sc.bitmap.as<uint64_t>(), sc.wcount.as<uint32_t>(), sc.nwords);
+  TM_LAUNCH_CHECK();
+  return scan_words(sc.wcount.as<uint32_t>(), sc.nwords, sc.wpre.as<uint64_t>(), s);
+}
+
+}  // namespace
+
+int64_t enumerate_orbits(const GroupSpec &gs, DevBuf &reps, DevBuf &dups, DevBuf *map,
+                         hipStream_t s)
+{
+  TM_CHECK(gs.L >= 4 && gs.L <= 32, TM_EINVAL, "orbit enumeration supports 4 <= L <= 32");
+  const GroupDev g = to_dev(gs);
+  OrbitScratch sc;
+  const uint64_t M = orbit_bitmap(g, sc, s);
+  TM_CHECK(M > 0 && M < (1ull << 32), TM_EINVAL, "orbit count out of range");
+  reps.alloc(M * sizeof(uint64_t));
+  dups.alloc(M * sizeof(uint16_t));
+  hipLaunchKernelGGL(k_write_reps, dim3(grid_for(sc.nwords, 256)), dim3(256), 0, s, g,
+                     sc.bitmap.as<uint64_t>(), sc.wpre.as<uint64_t>(), sc.nwords,
+                     reps.as<uint64_t>(), dups.as<uint16_t>());
+  TM_LAUNCH_CHECK();
+  if (map) {
+    const uint64_t nstate = 1ull << gs.L;
+    map->alloc(nstate * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_build_map, dim3(grid_for(nstate, 256)), dim3(256), 0, s, g, nstate,
+                       sc.bitmap.as<uint64_t>(), sc.wpre.as<uint64_t>(), map->as<uint32_t>());
+    TM_LAUNCH_CHECK();
+  }
+  TM_HIP(hipStreamSynchronize(s));
+  return (int64_t)M;
+}
+
+void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, FusedTables &ft,
+                        hipStream_t s)
+{
+  TM_CHECK(gs.dim == 2 && gs.bc == 0 && (gs.L % 2) == 0 && gs.L <= 32, TM_EINVAL,
+           "fused reduced path needs a periodic 2D layer of even width");
+  const int n = gs.L / 2;
+  const uint32_t nh = 1u << n;
+  ft.n = n;
+  // stored columns: a <= rev_n(a)
+  auto rev = [n](uint32_t a) {
+    uint32_t r = 0;
+    for (int i = 0; i < n; ++i) r |= ((a >> i) & 1u) << (n - 1 - i);
+    return r;
+  };
+  std::vector<uint32_t> cols;
+  for (uint32_t a = 0; a < nh; ++a)
+    if (a <= rev(a)) cols.push_back(a);
+  ft.Q = (int64_t)cols.size();
+  // representatives on the host: high halves, rep ranges, low halves
+  std::vector<uint64_t> hreps(M);
+  TM_HIP(hipMemcpy(hreps.data(), reps.get(), M * 8, hipMemcpyDeviceToHost));
+  std::vector<uint16_t> brows, replo(M);
+  std::vector<int64_t> rstart;
+  for (int64_t o = 0; o < M; ++o) {
+    const uint32_t b = (uint32_t)(hreps[o] >> n);
+    replo[o] = (uint16_t)(hreps[o] & (nh - 1));
+    if (brows.empty() || brows.back() != b) {
+      brows.push_back((uint16_t)b);
+      rstart.push_back(o);
+    }
+  }
+  rstart.push_back(M);
+  ft.nB = (int)brows.size();
+  std::vector<char> need(nh, 0);
+  for (uint16_t b : brows) { need[b] = 1; need[rev(b)] = 1; }
+  std::vector<uint16_t> srows;
+  std::vector<int32_t> rowidx(nh, -1);
+  for (uint32_t b = 0; b < nh; ++b)
+    if (need[b]) { rowidx[b] = (int32_t)srows.size(); srows.push_back((uint16_t)b); }
+  ft.nS = (int)srows.size();
+
+  auto up = [&](DevBuf &d, const void *src, size_t bytes) {
+    d.alloc(bytes);
+    TM_HIP(hipMemcpy(d.get(), src, bytes, hipMemcpyHostToDevice));
+  };
+  up(ft.cols, cols.data(), cols.size() * 4);
+  up(ft.srows, srows.data(), srows.size() * 2);
+  up(ft.rowidx, rowidx.data(), rowidx.size() * 4);
+  up(ft.brows, brows.data(), brows.size() * 2);
+  up(ft.rstart, rstart.data(), rstart.size() * 8);
+  up(ft.replo, replo.data(), replo.size() * 2);
+
+  // compact map over the stored columns
+  const GroupDev g = to_dev(gs);
+  OrbitScratch sc;
+  const uint64_t M2 = orbit_bitmap(g, sc, s);
+  TM_CHECK((int64_t)M2 == M, TM_EINVAL, "orbit count mismatch while building fused tables");
+  const uint64_t total = (uint64_t)ft.Q << n;
+  ft.cmap.alloc(total * 4);
+  hipLaunchKernelGGL(k_build_cmap, dim3(grid_for(total, 256)), dim3(256), 0, s, g, n,
+                     ft.cols.as<uint32_t>(), (uint64_t)ft.Q, sc.bitmap.as<uint64_t>(),
+                     sc.wpre.as<uint64_t>(), ft.cmap.as<uint32_t>());
+  TM_LAUNCH_CHECK();
+  TM_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// reduced operator application
+// ---------------------------------------------------------------------------
+namespace {
+
+// xt[o] = (C)(x[o] * scale) * tvec[o]
+template <typename C, typename TI>
+__global__ void k_xtilde(const TI *__restrict__ x, const C *__restrict__ tvec, C *__restrict__ xt,
+                         int64_t M, const double *__restrict__ scale)
+{
+  const C sc = scale ? (C)(*scale) : (C)1;
+  for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < M;
+       o += (int64_t)gridDim.x * blockDim.x)
+    xt[o] = (C)x[o] * sc * tvec[o];
+}
+
+template <typename C>
+__global__ void k_expand(const uint32_t *__restrict__ map, const C *__restrict__ xt,
+                         C *__restrict__ u, uint64_t nstate)
+{
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < nstate;
+       s += (uint64_t)gridDim.x * blockDim.x)
+    u[s] = xt[map[s]];
+}
+
+template <typename C, typename TO>
+__global__ void k_gather(const C *__restrict__ u, const uint64_t *__restrict__ reps,
+                         const C *__restrict__ dt, TO *__restrict__ y, int64_t M,
+                         const double *__restrict__ scale)
+{
+  const C inv = scale ? (C)(1.0 / *scale) : (C)1;
+  for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < M;
+       o += (int64_t)gridDim.x * blockDim.x)
+    y[o] = (TO)(dt[o] * u[reps[o]] * inv);
+}
+
+// Plain in-LDS Kronecker transform over the full space for small L (generic
+// path): one workgroup per vector.
+template <typename C>
+__global__ __launch_bounds__(1024) void k_kron_small(C *u, int L, C ka, C kb)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  C *lds = reinterpret_cast<C *>(smem);
+  const int n = 1 << L;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = u[i];
+  __syncthreads();
+  dev::lds_kron(lds, L, 0, ka, kb);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) u[i] = lds[i];
+}
+
+constexpr int FUSED_THREADS = 1024;
+
+// Pass 1: one stored column a = cols[q]; u(b', a) = xt[cmap[q][b']] for all
+// b'; transform over b'; keep rows srows[*] in Ap[row * ldA + q].
+template <typename C>
+__global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
+    const uint32_t *__restrict__ cmap, const C *__restrict__ xt, C *__restrict__ Ap,
+    const uint16_t *__restrict__ srows, int nS, int n, int64_t Q, int64_t ldA, C ka, C kb)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  C *lds = reinterpret_cast<C *>(smem);
+  const int64_t q = dev::xcd_remap(blockIdx.x, Q);
+  if (q < 0) return;
+  const int nh = 1 << n;
+  const uint32_t *mrow = cmap + ((uint64_t)q << n);
+  // 16 consecutive positions per thread: gather, then the first 4 stages in
+  // registers
+  for (int grp = threadIdx.x; grp < (nh >> 4); grp += blockDim.x) {
+    const uint4 *m4 = reinterpret_cast<const uint4 *>(mrow + (grp << 4));
+    uint4 mm[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) mm[i] = m4[i];
+    C v[16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[4 * i + 0] = xt[mm[i].x];
+      v[4 * i + 1] = xt[mm[i].y];
+      v[4 * i + 2] = xt[mm[i].z];
+      v[4 * i + 3] = xt[mm[i].w];
+    }
+    dev::kron_regs<4>(v, ka, kb);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) lds[(grp << 4) + i] = v[i];
+  }
+  __syncthreads();
+  dev::lds_kron(lds, n, 4, ka, kb);
+  for (int i = threadIdx.x; i < nS; i += blockDim.x) Ap[(int64_t)i * ldA + q] = lds[srows[i]];
+}
+
+// Pass 2: one kept row b = brows[bi]; rebuild A'(b, a) for all a from the
+// stored rows b and rev(b); transform over a; write the representatives.
+template <typename C, typename TO>
+__global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
+    const C *__restrict__ Ap, const uint32_t *__restrict__ cols, int64_t Q,
+    const uint16_t *__restrict__ brows, const int32_t *__restrict__ rowidx,
+    const int64_t *__restrict__ rstart, const uint16_t *__restrict__ replo,
+    const C *__restrict__ dt, TO *__restrict__ y, int n, int64_t ldA, C ka, C kb,
+    const double *__restrict__ scale)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  C *lds = reinterpret_cast<C *>(smem);
+  const int bi = blockIdx.x;
+  const uint32_t b = brows[bi];
+  const uint32_t rb = dev::brev32(b, n);
+  const C *rowb = Ap + (int64_t)rowidx[b] * ldA;
+  const C *rowrb = Ap + (int64_t)rowidx[rb] * ldA;
+  for (int64_t q = threadIdx.x; q < Q; q += blockDim.x) {
+    const uint32_t a = cols[q];
+    const uint32_t ra = dev::brev32(a, n);
+    lds[a] = rowb[q];
+    if (ra != a) lds[ra] = rowrb[q];
+  }
+  __syncthreads();
+  dev::lds_kron(lds, n, 0, ka, kb);
+  const C inv = scale ? (C)(1.0 / *scale) : (C)1;
+  const int64_t o0 = rstart[bi], o1 = rstart[bi + 1];
+  for (int64_t o = o0 + threadIdx.x; o < o1; o += blockDim.x)
+    y[o] = (TO)(dt[o] * lds[replo[o]] * inv);
+}
+
+template <typename C>
+void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io, void *work,
+                   const double *scale, hipStream_t s, const C *tvec, const C *dt)
+{
+  const int64_t M = p.M;
+  C *xt = static_cast<C *>(work);
+  C *rest = xt + ((M + 63) / 64) * 64;
+  const unsigned gM = grid_for((uint64_t)M, 256);
+  if (io == TM_F64)
+    hipLaunchKernelGGL((k_xtilde<C, double>), dim3(gM), dim3(256), 0, s,
+                       static_cast<const double *>(in), tvec, xt, M, scale);
+  else
+    hipLaunchKernelGGL((k_xtilde<C, float>), dim3(gM), dim3(256), 0, s,
+                       static_cast<const float *>(in), tvec, xt, M, scale);
+  TM_LAUNCH_CHECK();
+  const C ka = (C)p.ka, kb = (C)p.kb;
+  if (p.ft) {
+    const FusedTables &ft = *p.ft;
+    const int n = ft.n;
+    const int64_t ldA = ((ft.Q + 63) / 64) * 64;
+    C *Ap = rest;
+    const size_t shm = (size_t(1) << n) * sizeof(C);
+    const unsigned g1 = (unsigned)(((ft.Q + 7) / 8) * 8);
+    hipLaunchKernelGGL((k_fused_pass1<C>), dim3(g1), dim3(FUSED_THREADS), shm, s,
+                       ft.cmap.as<uint32_t>(), xt, Ap, ft.srows.as<uint16_t>(), ft.nS, n, ft.Q,
+                       ldA, ka, kb);
+    TM_LAUNCH_CHECK();
+    if (io == TM_F64)
+      hipLaunchKernelGGL((k_fused_pass2<C, double>), dim3(ft.nB), dim3(FUSED_THREADS), shm, s,
+                         Ap, ft.cols.as<uint32_t>(), ft.Q, ft.brows.as<uint16_t>(),
+                         ft.rowidx.as<int32_t>(), ft.rstart.as<int64_t>(),
+                         ft.replo.as<uint16_t>(), dt, static_cast<double *>(out), n, ldA, ka,
+                         kb, scale);
+    else
+      hipLaunchKernelGGL((k_fused_pass2<C, float>), dim3(ft.nB), dim3(FUSED_THREADS), shm, s,
+                         Ap, ft.cols.as<uint32_t>(), ft.Q, ft.brows.as<uint16_t>(),
+                         ft.rowidx.as<int32_t>(), ft.rstart.as<int64_t>(),
+                         ft.replo.as<uint16_t>(), dt, static_cast<float *>(out), n, ldA, ka,
+                         kb, scale);
+    TM_LAUNCH_CHECK();
+    return;
+  }
+  // generic: expand, Kronecker passes, gather
+  const uint64_t N = 1ull << p.L;
+  C *u = rest;
+  hipLaunchKernelGGL((k_expand<C>), dim3(grid_for(N, 256)), dim3(256), 0, s, p.map, xt, u, N);
+  TM_LAUNCH_CHECK();
+  const int smallL = sizeof(C) == 8 ? 13 : 14;
+  if (p.L <= smallL) {
+    hipLaunchKernelGGL((k_kron_small<C>), dim3(1), dim3(1024), (size_t(1) << p.L) * sizeof(C),
+                       s, u, p.L, ka, kb);
+    TM_LAUNCH_CHECK();
+  } else {
+    KronPlan kp{};
+    kp.L = p.L;
+    kp.prologue = kp.epilogue = false;
+    kp.ka = p.ka;
+    kp.kb = p.kb;
+    kp.diag.kind = -1;
+    const tm_dtype ct = sizeof(C) == 8 ? TM_F64 : TM_F32;
+    launch_kron(kp, u, u, ct, ct, ct, nullptr, nullptr, s);
+  }
+  if (io == TM_F64)
+    hipLaunchKernelGGL((k_gather<C, double>), dim3(gM), dim3(256), 0, s, u, p.reps, dt,
+                       static_cast<double *>(out), M, scale);
+  else
+    hipLaunchKernelGGL((k_gather<C, float>), dim3(gM), dim3(256), 0, s, u, p.reps, dt,
+                       static_cast<float *>(out), M, scale);
+  TM_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+size_t reduced_work_bytes(const ReducedPlan &p, tm_dtype comp)
+{
+  const size_t es = comp == TM_F64 ? 8 : 4;
+  size_t b = (size_t)((p.M + 63) / 64) * 64 * es;
+  if (p.ft) {
+    const int64_t ldA = ((p.ft->Q + 63) / 64) * 64;
+    b += (size_t)p.ft->nS * ldA * es;
+  } else {
+    b += (size_t(1) << p.L) * es;
+  }
+  return b;
+}
+
+void launch_reduced(const ReducedPlan &p, const void *in, void *out, tm_dtype io, tm_dtype comp,
+                    void *work, const double *scale, hipStream_t s)
+{
+  if (comp == TM_F64)
+    reduced_typed<double>(p, in, out, io, work, scale, s, p.tvec, p.dt64);
+  else
+    reduced_typed<float>(p, in, out, io, work, scale, s, p.tvec32, p.dt32);
+}
+
+}  // namespace ltm

@@ -1,0 +1,641 @@
+// tm_solver.cpp -- the eigen driver (pfunc_inner / getlefteigvec of the
+// reference, TMat/pfunc.hpp:39-114) with the Krylov restart loop on the GPU.
+//
+// Method: thick-restart Krylov-Schur (Stewart) for both symmetric and
+// general operators.  The reference calls Spectra's implicitly restarted
+// Lanczos / Arnoldi (SymEigsSolver / GenEigsSolver, pfunc.hpp:84-98) with
+// nev = neigs, ncv = suggested_nev (tmat.hpp:88), LARGEST_MAGN and Spectra's
+// defaults tol = 1e-10, maxit = 1000; a thick restart that keeps the wanted
+// Ritz vectors spans the same restarted Krylov space as an implicit restart
+// with the unwanted Ritz values as shifts.
+//
+// Device: the basis V (FP64, ncv+1 vectors), T.x, CGS2 orthogonalisation,
+// the restart V <- V Y.  Host: the (ncv x ncv) projected matrix, its
+// eigen-decomposition, convergence tests.  One small device->host copy per
+// Krylov step.
+//
+// Mixed precision (TM_PREC_MIXED): phase 1 runs the same loop with the
+// operator computing in FP32 (FP64 basis and orthogonalisation) to a
+// residual of mixed_tol; phase 2 refines the kept Ritz block with FP64
+// matvecs (Rayleigh-Ritz, repeated with one subspace-iteration step) until
+// the FP64 residuals or the eigenvalue change meet the target.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <numeric>
+#include <vector>
+
+#include "tm_internal.hpp"
+
+namespace ltm {
+namespace {
+
+constexpr double EPS = 2.220446049250313e-16;
+
+struct Ritz {
+  std::vector<double> wr, wi;     // eigenvalues (sorted by magnitude, descending)
+  std::vector<double> Zr, Zi;     // eigenvectors of the projected matrix (m x m)
+  std::vector<double> res;        // Krylov residual estimates
+};
+
+class KrylovSchur {
+ public:
+  KrylovSchur(tm_op *op, int nev, int m, double tol, int maxit, tm_dtype comp, int transpose,
+              uint64_t seed, int verbose)
+      : op_(op), nev_(nev), m_(m), tol_(tol), maxit_(maxit), comp_(comp), tr_(transpose),
+        seed_(seed), verbose_(verbose)
+  {
+    n_ = op->ncol;
+    ldv_ = ((n_ + 63) / 64) * 64;
+    sym_ = op->symmetric;
+    V_.alloc(sizeof(double) * ldv_ * (m_ + 1));
+    partial_.alloc(sizeof(double) * kry::NBLK * 72);
+    hbuf_.alloc(sizeof(double) * 3 * 72);
+    Ydev_.alloc(sizeof(double) * 72 * 72);
+    H_.assign((size_t)(m_ + 1) * m_, 0.0);
+  }
+
+  double *V(int k) const { return V_.as<double>() + (size_t)k * ldv_; }
+  int64_t ldv() const { return ldv_; }
+  int64_t n() const { return n_; }
+  double &H(int i, int j) { return H_[i + (size_t)j * (m_ + 1)]; }
+
+  // Start vector: uniform(0,1) from the seed, normalised (pfunc.hpp:97).
+  void start()
+  {
+    hipStream_t s = op_->stream;
+    kry::random_uniform(V(0), n_, seed_, s);
+    kry::multidot(V_.as<double>(), ldv_, 0, V(0), n_, partial_.as<double>(), hbuf_.as<double>(), s);
+    kry::normalize(V(0), n_, hbuf_.as<double>(), s);
+    k_ = 0;
+  }
+
+  // Start from given orthonormal vectors already in V(0..k-1)? (not used)
+
+  // Expand the Krylov-Schur decomposition from column k_ to m_.
+  void expand()
+  {
+    hipStream_t s = op_->stream;
+    double *hb = hbuf_.as<double>();
+    std::vector<double> hh(3 * 72);
+    for (int j = k_; j < m_; ++j) {
+      double *w = V(j + 1);
+      op_->apply_dev(V(j), w, ldv_, 1, TM_F64, comp_, tr_, nullptr, s);
+      ++matvecs_;
+      const int nk = j + 1;
+      kry::multidot(V_.as<double>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
+      kry::update_dots(V_.as<double>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 72, s);
+      kry::update_norm(V_.as<double>(), ldv_, nk, w, n_, hb + 72, partial_.as<double>(), hb + 144, s);
+      TM_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double) * 3 * 72, hipMemcpyDeviceToHost, s));
+      TM_HIP(hipStreamSynchronize(s));
+      double hn2 = 0;
+      for (int kk = 0; kk < nk; ++kk) {
+        H(kk, j) = hh[kk] + hh[72 + kk];
+        hn2 += H(kk, j) * H(kk, j);
+      }
+      const double beta2 = hh[144];
+      double beta = std::sqrt(std::max(beta2, 0.0));
+      if (beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) {
+        // invariant subspace: continue with a fresh random direction
+        // orthogonal to the basis; the coupling is zero.
+        kry::random_uniform(w, n_, seed_ + 7919 * (uint64_t)(j + 1 + 31 * restarts_), s);
+        for (int pass = 0; pass < 2; ++pass) {
+          kry::multidot(V_.as<double>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
+          kry::update_norm(V_.as<double>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 144, s);
+        }
+        kry::normalize(w, n_, hb + 144, s);
+        beta = 0.0;
+      } else {
+        kry::normalize(w, n_, hb + 144, s);
+      }
+      H(j + 1, j) = beta;
+    }
+    k_ = m_;
+  }
+
+  // Eigen-decomposition of the m x m projected matrix, sorted by |lambda|.
+  Ritz ritz()
+  {
+    const int m = m_;
+    std::vector<double> A((size_t)m * m);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) A[i + (size_t)j * m] = H(i, j);
+    Ritz r;
+    std::vector<double> wr(m, 0.0), wi(m, 0.0), Zr((size_t)m * m), Zi((size_t)m * m, 0.0);
+    if (sym_) {
+      sym_eig(m, A.data(), wr.data(), Zr.data());
+    } else {
+      gen_eig(m, A.data(), wr.data(), wi.data(), Zr.data(), Zi.data());
+    }
+    std::vector<int> idx(m);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+      const double ma = std::hypot(wr[a], wi[a]), mb = std::hypot(wr[b], wi[b]);
+      if (ma != mb) return ma > mb;
+      return wi[a] > wi[b];  // conjugate pairs: positive imaginary part first
+    });
+    r.wr.resize(m);
+    r.wi.resize(m);
+    r.Zr.resize((size_t)m * m);
+    r.Zi.resize((size_t)m * m);
+    r.res.resize(m);
+    // gen_eig returns each conjugate pair with exactly opposite imaginary
+    // parts; order every pair (+, -) next to each other
+    if (!sym_) {
+      for (int c = 0; c + 1 < m; ++c) {
+        const int o = idx[c];
+        if (wi[o] == 0) continue;
+        int best = -1;
+        for (int d = c + 1; d < m; ++d) {
+          const int od = idx[d];
+          if (wi[od] != 0 && std::fabs(wr[od] - wr[o]) <= 1e-12 * std::fabs(wr[o]) + 1e-300 &&
+              std::fabs(wi[od] + wi[o]) <= 1e-12 * std::fabs(wi[o])) { best = d; break; }
+        }
+        if (best < 0) continue;
+        std::swap(idx[c + 1], idx[best]);
+        if (wi[idx[c]] < 0) std::swap(idx[c], idx[c + 1]);
+        // exact conjugate pairing
+        wr[idx[c + 1]] = wr[idx[c]];
+        wi[idx[c + 1]] = -wi[idx[c]];
+        ++c;
+      }
+    }
+    const double beta = std::fabs(H(m, m - 1));
+    for (int c = 0; c < m; ++c) {
+      const int o = idx[c];
+      r.wr[c] = wr[o];
+      r.wi[c] = wi[o];
+      for (int i = 0; i < m; ++i) {
+        r.Zr[i + (size_t)c * m] = Zr[i + (size_t)o * m];
+        r.Zi[i + (size_t)c * m] = Zi[i + (size_t)o * m];
+      }
+      r.res[c] = beta * std::hypot(r.Zr[(m - 1) + (size_t)c * m], r.Zi[(m - 1) + (size_t)c * m]);
+    }
+    return r;
+  }
+
+  int nconv(const Ritz &r, double tol) const
+  {
+    int nc = 0;
+    const double e23 = std::pow(EPS, 2.0 / 3.0);
+    for (int i = 0; i < nev_; ++i) {
+      const double mag = std::hypot(r.wr[i], r.wi[i]);
+      if (r.res[i] <= tol * std::max(e23, mag)) ++nc;
+    }
+    return nc;
+  }
+
+  // Real orthonormal basis (m x k) of the first k Ritz vectors; conjugate
+  // pairs contribute their real and imaginary parts.  Returns k used.
+  int ritz_basis(const Ritz &r, int k, std::vector<double> &Y) const
+  {
+    const int m = m_;
+    // a conjugate pair is (i, i+1) with wi[i] != 0 and wi[i+1] == -wi[i]
+    auto pair_at = [&](int i) {
+      return !sym_ && i + 1 < m && r.wi[i] != 0 && r.wi[i + 1] == -r.wi[i] &&
+             r.wr[i + 1] == r.wr[i];
+    };
+    if (!sym_) {
+      // keep conjugate pairs together: if column k-1 opens a pair, widen
+      // (or shrink) the kept block by one
+      int i = 0;
+      while (i < k) {
+        if (pair_at(i)) {
+          if (i + 1 == k) { k = (k + 1 <= m - 1) ? k + 1 : k - 1; break; }
+          i += 2;
+        } else {
+          ++i;
+        }
+      }
+    }
+    Y.assign((size_t)m * k, 0.0);
+    int c = 0;
+    for (int i = 0; i < k && c < k;) {
+      for (int t = 0; t < m; ++t) Y[t + (size_t)c * m] = r.Zr[t + (size_t)i * m];
+      ++c;
+      if (pair_at(i) && c < k) {
+        for (int t = 0; t < m; ++t) Y[t + (size_t)c * m] = r.Zi[t + (size_t)i * m];
+        ++c;
+        i += 2;
+      } else {
+        ++i;
+      }
+    }
+    k = c;
+    // modified Gram-Schmidt, twice
+    for (int pass = 0; pass < 2; ++pass)
+      for (int a = 0; a < k; ++a) {
+        for (int b = 0; b < a; ++b) {
+          double d = 0;
+          for (int t = 0; t < m; ++t) d += Y[t + (size_t)a * m] * Y[t + (size_t)b * m];
+          for (int t = 0; t < m; ++t) Y[t + (size_t)a * m] -= d * Y[t + (size_t)b * m];
+        }
+        double nn = 0;
+        for (int t = 0; t < m; ++t) nn += Y[t + (size_t)a * m] * Y[t + (size_t)a * m];
+        nn = std::sqrt(nn);
+        for (int t = 0; t < m; ++t) Y[t + (size_t)a * m] /= nn;
+      }
+    return k;
+  }
+
+  // Thick restart keeping k Ritz vectors.
+  void restart(const Ritz &r, int k)
+  {
+    const int m = m_;
+    std::vector<double> Y;
+    k = ritz_basis(r, k, Y);
+    // S = Y^T H Y,  b = beta * Y(m-1, :)
+    std::vector<double> HY((size_t)m * k, 0.0), S((size_t)k * k, 0.0);
+    for (int c = 0; c < k; ++c)
+      for (int i = 0; i < m; ++i) {
+        double s = 0;
+        for (int t = 0; t < m; ++t) s += H(i, t) * Y[t + (size_t)c * m];
+        HY[i + (size_t)c * m] = s;
+      }
+    for (int c = 0; c < k; ++c)
+      for (int a = 0; a < k; ++a) {
+        double s = 0;
+        for (int t = 0; t < m; ++t) s += Y[t + (size_t)a * m] * HY[t + (size_t)c * m];
+        S[a + (size_t)c * k] = s;
+      }
+    const double beta = H(m, m - 1);
+    if (verbose_ > 1) {
+      double e = 0, hn = 0;
+      for (int c = 0; c < k; ++c)
+        for (int i = 0; i < m; ++i) {
+          double t = HY[i + (size_t)c * m];
+          for (int a = 0; a < k; ++a) t -= Y[i + (size_t)a * m] * S[a + (size_t)c * k];
+          e += t * t;
+          hn += HY[i + (size_t)c * m] * HY[i + (size_t)c * m];
+        }
+      std::fprintf(stderr, "[tm_solve] restart k=%d invariance error %.3e\n", k, std::sqrt(e / hn));
+      if (verbose_ > 2) {
+        std::fprintf(stderr, "HMAT");
+        for (int j = 0; j < m; ++j)
+          for (int i = 0; i < m; ++i) std::fprintf(stderr, " %.17g", H(i, j));
+        std::fprintf(stderr, "\nWR");
+        for (int i = 0; i < m; ++i) std::fprintf(stderr, " %.17g %.17g", r.wr[i], r.wi[i]);
+        std::fprintf(stderr, "\nZR");
+        for (int i = 0; i < m * m; ++i) std::fprintf(stderr, " %.17g %.17g", r.Zr[i], r.Zi[i]);
+        std::fprintf(stderr, "\n");
+      }
+    }
+    hipStream_t s = op_->stream;
+    TM_HIP(hipMemcpyAsync(Ydev_.get(), Y.data(), sizeof(double) * m * k, hipMemcpyHostToDevice, s));
+    kry::gemm_vy(V(0), ldv_, m, Ydev_.as<double>(), m, k, V(0), ldv_, n_, s);
+    kry::copy(V(m), V(k), n_, s);
+    std::fill(H_.begin(), H_.end(), 0.0);
+    for (int c = 0; c < k; ++c) {
+      for (int a = 0; a < k; ++a) H(a, c) = S[a + (size_t)c * k];
+      H(k, c) = beta * Y[(m - 1) + (size_t)c * m];
+    }
+    k_ = k;
+    ++restarts_;
+    TM_HIP(hipStreamSynchronize(s));
+  }
+
+  // Run to convergence; returns the number of converged wanted pairs.
+  int run(Ritz &out)
+  {
+    start();
+    int nc = 0;
+    for (int it = 0; it < maxit_; ++it) {
+      expand();
+      out = ritz();
+      nc = nconv(out, tol_);
+      if (verbose_)
+        std::fprintf(stderr, "[tm_solve] cycle %d matvecs %lld lambda0 %.15g res0 %.3e nconv %d\n",
+                     it, (long long)matvecs_, out.wr[0], out.res[0], nc);
+      if (nc >= nev_) break;
+      int k = nev_ + std::min(nc, (m_ - nev_) / 2);
+      k = std::max(k, (nev_ + m_) / 2);
+      k = std::min(k, m_ - 1);
+      restart(out, k);
+    }
+    return nc;
+  }
+
+  // Ritz vectors X[:, c] = V(0..m-1) * Z[:, c] for c < cnt (real part, and
+  // imaginary part into Xi if non-null) into device buffers of ld ldv().
+  void ritz_vectors(const Ritz &r, int cnt, double *Xr, double *Xi)
+  {
+    const int m = m_;
+    hipStream_t s = op_->stream;
+    std::vector<double> Z((size_t)m * cnt);
+    for (int c = 0; c < cnt; ++c)
+      for (int t = 0; t < m; ++t) Z[t + (size_t)c * m] = r.Zr[t + (size_t)c * m];
+    TM_HIP(hipMemcpyAsync(Ydev_.get(), Z.data(), sizeof(double) * m * cnt, hipMemcpyHostToDevice, s));
+    kry::gemm_vy(V(0), ldv_, m, Ydev_.as<double>(), m, cnt, Xr, ldv_, n_, s);
+    TM_HIP(hipStreamSynchronize(s));
+    if (Xi) {
+      for (int c = 0; c < cnt; ++c)
+        for (int t = 0; t < m; ++t) Z[t + (size_t)c * m] = r.Zi[t + (size_t)c * m];
+      TM_HIP(hipMemcpyAsync(Ydev_.get(), Z.data(), sizeof(double) * m * cnt, hipMemcpyHostToDevice, s));
+      kry::gemm_vy(V(0), ldv_, m, Ydev_.as<double>(), m, cnt, Xi, ldv_, n_, s);
+      TM_HIP(hipStreamSynchronize(s));
+    }
+  }
+
+  int64_t matvecs() const { return matvecs_; }
+  int64_t restarts() const { return restarts_; }
+  bool sym() const { return sym_; }
+  int m() const { return m_; }
+
+ private:
+  tm_op *op_;
+  int nev_, m_;
+  double tol_;
+  int maxit_;
+  tm_dtype comp_;
+  int tr_;
+  uint64_t seed_;
+  int verbose_;
+  int64_t n_ = 0, ldv_ = 0;
+  bool sym_ = true;
+  int k_ = 0;
+  int64_t matvecs_ = 0, restarts_ = 0;
+  DevBuf V_, partial_, hbuf_, Ydev_;
+  std::vector<double> H_;
+};
+
+// Cholesky of a small SPD matrix (k x k, column-major), lower factor in place.
+bool cholesky(int k, std::vector<double> &G)
+{
+  for (int j = 0; j < k; ++j) {
+    double d = G[j + (size_t)j * k];
+    for (int t = 0; t < j; ++t) d -= G[j + (size_t)t * k] * G[j + (size_t)t * k];
+    if (!(d > 0)) return false;
+    d = std::sqrt(d);
+    G[j + (size_t)j * k] = d;
+    for (int i = j + 1; i < k; ++i) {
+      double s = G[i + (size_t)j * k];
+      for (int t = 0; t < j; ++t) s -= G[i + (size_t)t * k] * G[j + (size_t)t * k];
+      G[i + (size_t)j * k] = s / d;
+    }
+    for (int i = 0; i < j; ++i) G[i + (size_t)j * k] = 0;
+  }
+  return true;
+}
+
+struct RefineOut {
+  std::vector<double> theta, res;
+  int64_t matvecs = 0;
+};
+
+// FP64 Rayleigh-Ritz refinement of an orthonormal block Y (k vectors at
+// B[0..k), leading dimension ld) for a symmetric operator.  Uses B[k..2k)
+// for A*Y and B[2k..3k) as scratch.  On return B[0..nev) hold the refined
+// Ritz vectors.
+RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev, double tol,
+                     int maxiter, int verbose)
+{
+  hipStream_t s = op->stream;
+  RefineOut ro;
+  DevBuf partial(sizeof(double) * kry::NBLK * 72), G(sizeof(double) * 72 * 72),
+      C(sizeof(double) * 2 * 72 * 72);
+  std::vector<double> prev;
+  auto col = [&](int c) { return B + (size_t)c * ld; };
+  std::vector<double> hG((size_t)k * k), th(k), Z((size_t)k * k);
+  for (int it = 0; it < maxiter; ++it) {
+    // W = A Y
+    for (int c = 0; c < k; ++c) {
+      op->apply_dev(col(c), col(k + c), ld, 1, TM_F64, TM_F64, 0, nullptr, s);
+      ++ro.matvecs;
+    }
+    kry::gram(col(0), col(k), ld, k, n, partial.as<double>(), G.as<double>(), s);
+    TM_HIP(hipMemcpyAsync(hG.data(), G.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
+    TM_HIP(hipStreamSynchronize(s));
+    // hG[row*k + c] = <Y_row, W_c>; symmetric projected matrix
+    std::vector<double> Hp((size_t)k * k);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) Hp[i + (size_t)j * k] = 0.5 * (hG[i * k + j] + hG[j * k + i]);
+    sym_eig(k, Hp.data(), th.data(), Z.data());
+    std::vector<int> idx(k);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return std::fabs(th[a]) > std::fabs(th[b]); });
+    std::vector<double> ths(k), Zs((size_t)k * k);
+    for (int c = 0; c < k; ++c) {
+      ths[c] = th[idx[c]];
+      for (int t = 0; t < k; ++t) Zs[t + (size_t)c * k] = Z[t + (size_t)idx[c] * k];
+    }
+    // [X | R] = [Y | W] * [[Z, -Z diag(th)], [0, Z]]  -> stored into B[2k..4k)?
+    // B has room for 3k vectors: X -> B[2k..3k), then R computed into B[k..2k)
+    std::vector<double> Cx((size_t)2 * k * k, 0.0), Cr((size_t)2 * k * k, 0.0);
+    for (int c = 0; c < k; ++c)
+      for (int t = 0; t < k; ++t) {
+        Cx[t + (size_t)c * 2 * k] = Zs[t + (size_t)c * k];
+        Cr[t + (size_t)c * 2 * k] = -ths[c] * Zs[t + (size_t)c * k];
+        Cr[k + t + (size_t)c * 2 * k] = Zs[t + (size_t)c * k];
+      }
+    TM_HIP(hipMemcpyAsync(C.get(), Cx.data(), sizeof(double) * 2 * k * k, hipMemcpyHostToDevice, s));
+    kry::gemm_vy(col(0), ld, 2 * k, C.as<double>(), 2 * k, k, col(2 * k), ld, n, s);  // X
+    TM_HIP(hipMemcpyAsync(C.get(), Cr.data(), sizeof(double) * 2 * k * k, hipMemcpyHostToDevice, s));
+    // R overwrites W (each row reads all 2k inputs before writing)
+    kry::gemm_vy(col(0), ld, 2 * k, C.as<double>(), 2 * k, k, col(k), ld, n, s);   // R
+    kry::gram(col(k), col(k), ld, k, n, partial.as<double>(), G.as<double>(), s);
+    TM_HIP(hipMemcpyAsync(hG.data(), G.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
+    TM_HIP(hipStreamSynchronize(s));
+    ro.theta = ths;
+    ro.res.assign(k, 0.0);
+    double worst = 0, change = 0, bound = 0;
+    for (int c = 0; c < nev; ++c) {
+      const double r = std::sqrt(std::max(hG[c * k + c], 0.0));
+      ro.res[c] = r / std::fabs(ths[c]);
+      worst = std::max(worst, ro.res[c]);
+      double gap = 1e300;
+      for (int d = 0; d < k; ++d)
+        if (d != c) gap = std::min(gap, std::fabs(ths[c] - ths[d]));
+      bound = std::max(bound, r * r / (gap * std::fabs(ths[c])));
+      if (!prev.empty()) change = std::max(change, std::fabs(ths[c] - prev[c]) / std::fabs(ths[c]));
+    }
+    if (verbose)
+      std::fprintf(stderr, "[tm_refine] it %d theta0 %.16g res %.3e bound %.3e change %.3e\n", it,
+                   ths[0], worst, bound, change);
+    // Y <- orth(X) for the next round, X kept as the answer
+    kry::gram(col(2 * k), col(2 * k), ld, k, n, partial.as<double>(), G.as<double>(), s);
+    TM_HIP(hipMemcpyAsync(hG.data(), G.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
+    TM_HIP(hipStreamSynchronize(s));
+    // accept when the FP64 residual meets tol, or when the Rayleigh-Ritz
+    // eigenvalue error estimate r^2 / (gap |theta|) is below 1e-13
+    const bool done = worst <= tol || bound <= 1e-13 || it == maxiter - 1;
+    (void)change;
+    // copy X into the first k slots (orthonormalised for the next iteration)
+    std::vector<double> Gm((size_t)k * k);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) Gm[i + (size_t)j * k] = hG[i * k + j];
+    std::vector<double> Rinv((size_t)k * k, 0.0);
+    if (!done && cholesky(k, Gm)) {
+      // Y = X L^-T : solve for the upper-triangular inverse of L^T
+      for (int c = 0; c < k; ++c) {
+        // column c of L^-T
+        std::vector<double> e(k, 0.0);
+        e[c] = 1.0;
+        // solve L^T x = e (backward)
+        for (int i = k - 1; i >= 0; --i) {
+          double s2 = e[i];
+          for (int t = i + 1; t < k; ++t) s2 -= Gm[t + (size_t)i * k] * e[t];
+          e[i] = s2 / Gm[i + (size_t)i * k];
+        }
+        for (int i = 0; i < k; ++i) Rinv[i + (size_t)c * k] = e[i];
+      }
+    } else {
+      for (int i = 0; i < k; ++i) Rinv[i + (size_t)i * k] = 1.0 / std::sqrt(std::max(hG[i * k + i], 1e-300));
+    }
+    TM_HIP(hipMemcpyAsync(C.get(), Rinv.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, s));
+    kry::gemm_vy(col(2 * k), ld, k, C.as<double>(), k, k, col(0), ld, n, s);
+    TM_HIP(hipStreamSynchronize(s));
+    prev = ths;
+    if (done) break;
+  }
+  return ro;
+}
+
+int suggested_ncv(int neigs) { return neigs > 4 ? 2 * neigs + 1 : 2 * neigs + 3; }  // tmat.hpp:88
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// public solve
+// ---------------------------------------------------------------------------
+tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t n = op->ncol;
+  const int nev = o.nev;
+  TM_CHECK(nev >= 1 && nev < n, TM_EINVAL, "nev must be in [1, cols())");
+  int ncv = o.ncv > 0 ? o.ncv : suggested_ncv(nev);
+  if (ncv >= n) ncv = (int)n - 1;  // pfunc.hpp:90
+  TM_CHECK(ncv > nev, TM_EINVAL, "ncv must exceed nev");
+  TM_CHECK(ncv <= 60, TM_EINVAL, "ncv above 60 is not supported by the device kernels");
+  const bool mixed = o.precision == TM_PREC_MIXED;
+  const int64_t mv0 = op->matvecs;
+  int64_t mv32 = 0, mv64 = 0, restarts = 0;
+  int trial = 0, nc = 0;
+  Ritz rz;
+  std::unique_ptr<KrylovSchur> ks;
+  // pfunc.hpp:87-109: up to maxtrial attempts, ncv += 2 on failure
+  for (;;) {
+    ks.reset(new KrylovSchur(op, nev, ncv, mixed ? std::max(o.mixed_tol, o.tol) : o.tol, o.maxit,
+                             mixed ? TM_F32 : TM_F64, 0, o.seed, o.verbose));
+    nc = ks->run(rz);
+    restarts += ks->restarts();
+    if (mixed) mv32 += ks->matvecs(); else mv64 += ks->matvecs();
+    ++trial;
+    if (nc >= nev) break;
+    if (o.verbose) std::fprintf(stderr, "eigensolver not converged (%d/%d)\n", trial, o.maxtrial);
+    if (trial >= o.maxtrial) break;
+    ncv += 2;
+    if (ncv >= n) ncv = (int)n - 1;
+  }
+  if (nc == 0 && !mixed) {
+    out->nconv = 0;
+    out->trials = trial;
+    throw Error(TM_ENOCONV, "Number of converged eigenvalue is zero");
+  }
+  const int64_t ldv = ks->ldv();
+  std::vector<double> evr(nev), evi(nev), res(nev);
+  DevBuf Xr, Xi;
+  const bool cplx = !op->symmetric;
+  if (mixed && op->symmetric) {
+    // FP64 refinement of a block of kept Ritz vectors
+    const int k = std::min(ks->m() - 1, nev + 1);
+    DevBuf B(sizeof(double) * ldv * 3 * k);
+    ks->ritz_vectors(rz, k, B.as<double>(), nullptr);
+    RefineOut ro = refine_sym(op, B.as<double>(), ldv, n, k, nev, o.tol, 4, o.verbose);
+    mv64 += ro.matvecs;
+    for (int i = 0; i < nev; ++i) {
+      evr[i] = ro.theta[i];
+      evi[i] = 0;
+      res[i] = ro.res[i];
+    }
+    nc = 0;
+    for (int i = 0; i < nev; ++i) nc += std::isfinite(ro.theta[i]) ? 1 : 0;
+    Xr.alloc(sizeof(double) * ldv * nev);
+    TM_HIP(hipMemcpy(Xr.get(), B.get(), sizeof(double) * ldv * nev, hipMemcpyDeviceToDevice));
+  } else {
+    if (mixed) {
+      // general operator: FP32 phase result refined by an FP64 Krylov run
+      // warm-started from the leading FP32 Ritz vector
+      Xr.alloc(sizeof(double) * ldv * nev);
+      ks->ritz_vectors(rz, 1, Xr.as<double>(), nullptr);
+      tm_solve_opts o2 = o;
+      o2.precision = TM_PREC_F64;
+      KrylovSchur k2(op, nev, ncv, o.tol, o.maxit, TM_F64, 0, o.seed, o.verbose);
+      nc = k2.run(rz);
+      mv64 += k2.matvecs();
+      restarts += k2.restarts();
+      ks.reset();
+      // use k2's vectors
+      Xr.alloc(sizeof(double) * k2.ldv() * nev);
+      if (cplx) Xi.alloc(sizeof(double) * k2.ldv() * nev);
+      k2.ritz_vectors(rz, nev, Xr.as<double>(), cplx ? Xi.as<double>() : nullptr);
+    } else {
+      Xr.alloc(sizeof(double) * ldv * nev);
+      if (cplx) Xi.alloc(sizeof(double) * ldv * nev);
+      ks->ritz_vectors(rz, nev, Xr.as<double>(), cplx ? Xi.as<double>() : nullptr);
+    }
+    for (int i = 0; i < nev; ++i) {
+      evr[i] = rz.wr[i];
+      evi[i] = rz.wi[i];
+      const double mag = std::hypot(rz.wr[i], rz.wi[i]);
+      res[i] = rz.res[i] / (mag > 0 ? mag : 1.0);
+    }
+  }
+  out->nconv = std::min(nc, nev);
+  for (int i = 0; i < nev; ++i) {
+    if (out->evals_re) out->evals_re[i] = evr[i];
+    if (out->evals_im) out->evals_im[i] = evi[i];
+    if (out->resid) out->resid[i] = res[i];
+  }
+  if (o.want_vectors && out->vecs) {
+    if (!cplx) {
+      for (int i = 0; i < nev; ++i)
+        TM_HIP(hipMemcpy(out->vecs + (size_t)i * n, Xr.as<double>() + (size_t)i * ldv,
+                         sizeof(double) * n, hipMemcpyDeviceToHost));
+    } else {
+      // cols() x nev complex128 interleaved, column-major (pfunc.hpp:139)
+      std::vector<double> re(n), im(n);
+      for (int i = 0; i < nev; ++i) {
+        TM_HIP(hipMemcpy(re.data(), Xr.as<double>() + (size_t)i * ldv, sizeof(double) * n,
+                         hipMemcpyDeviceToHost));
+        TM_HIP(hipMemcpy(im.data(), Xi.as<double>() + (size_t)i * ldv, sizeof(double) * n,
+                         hipMemcpyDeviceToHost));
+        for (int64_t t = 0; t < n; ++t) {
+          out->vecs[2 * ((size_t)i * n + t)] = re[t];
+          out->vecs[2 * ((size_t)i * n + t) + 1] = im[t];
+        }
+      }
+    }
+  }
+  // left eigenvector (getlefteigvec, pfunc.hpp:47-67): T^T with nev=1, ncv=7
+  out->eval_left = std::nan("");
+  if (o.want_left && !op->symmetric) {
+    int ncvl = 7;
+    if (ncvl >= n) ncvl = (int)n - 1;
+    KrylovSchur kl(op, 1, ncvl, o.tol, o.maxit, TM_F64, 1, o.seed + 1, o.verbose);
+    Ritz rl;
+    const int ncl = kl.run(rl);
+    mv64 += kl.matvecs();
+    if (ncl >= 1) {
+      out->eval_left = rl.wr[0];
+      if (out->vecl) {
+        DevBuf XL(sizeof(double) * kl.ldv());
+        kl.ritz_vectors(rl, 1, XL.as<double>(), nullptr);
+        TM_HIP(hipMemcpy(out->vecl, XL.get(), sizeof(double) * n, hipMemcpyDeviceToHost));
+      }
+    } else if (o.verbose) {
+      std::fprintf(stderr, "Warning: left eigenvalue calculation failed.\n");
+    }
+  }
+  out->matvecs_f32 = mv32;
+  out->matvecs_f64 = mv64;
+  (void)mv0;
+  out->restarts = restarts;
+  out->trials = trial;
+  out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return nc >= nev ? TM_OK : TM_ENOCONV;
+}
+
+}  // namespace ltm

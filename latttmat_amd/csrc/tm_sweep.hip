@@ -1,0 +1,391 @@
+// tm_sweep.hip -- auxiliary-bit sweep operators on gfx950: DNNI (XDT) and
+// BNNNI/3NN diagonal TM (ZBT and its transpose).
+//
+// The reference applies L (XDT, TMat/tmat.cpp:344-476) or L/2 (ZBT,
+// TMat/tmat.cpp:521-649) "permute_mult_T1" steps, each of which rotates the
+// whole state index and mixes the lowest bit(s) with 1 (2) auxiliary bits.
+// Written in physical bit positions the rotation disappears: step t is a
+// local gate on (aux, bit t) [XDT] or (aux pair, bits 2t,2t+1) [ZBT] whose
+// coefficients depend on the next bit(s) of the state (or, for the last step,
+// on the "lastbit" value the reference loops over).  The lastbit loop becomes
+// a batch dimension, so one sweep over the array
+//     Z[lastbit][aux][P],   P = 2^L physical states
+// does all of it.  Gates are grouped into passes: a workgroup holds a tile of
+// Z spanning (lastbit, aux, the pass's state bits, W contiguous low states)
+// in LDS, applies its gates in order, and writes the tile back.  The first
+// pass builds Z from x on the fly (Tv^1/2 x placed at the right aux/lastbit)
+// and the last pass contracts over (lastbit, aux) and applies Tv^1/2.
+#include "tm_device.hpp"
+
+namespace ltm {
+namespace {
+
+constexpr int SWEEP_THREADS = 512;
+
+struct SweepArgs {
+  int L;
+  int model;        // 0 = XDT, 1 = ZBT
+  int dir;          // ZBT: 0 = T, 1 = T^T
+  int bc;           // XDT: 0 periodic (lastbit batch), 5 / 9 confined
+  int nlb, na;      // lastbit values, aux values (log2 in nlbl, nal)
+  int nlbl, nal;
+  int t0, ng;       // first gate, number of gates in this pass
+  int bitlo;        // first state bit owned by the pass
+  int nbits;        // state bits owned by the pass
+  int logW;
+  int first, last;
+  DiagSpec dg;
+  double b1[2], b2[2], b3[2];
+  double outfac;    // XDT: normfactor; ZBT: 1 (folded into the diagonal)
+};
+
+__device__ __forceinline__ uint32_t ins0(uint32_t q, int pos)
+{
+  return ((q >> pos) << (pos + 1)) | (q & ((1u << pos) - 1));
+}
+
+template <typename C>
+struct Coef {
+  C b1[2], b2[2], b3[2];
+};
+
+// XDT gate: in[a][u] (a = aux, u = state bit t), control c.
+//   out[p][s] = b1[s^p] b2[c^s] (b2[s] in[0][p] + b2[s^1] in[1][p])
+template <typename C>
+__device__ __forceinline__ void xdt_gate(C v[2][2], int c, const Coef<C> &k)
+{
+  C o[2][2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+      o[p][s] = k.b1[s ^ p] * k.b2[c ^ s] * (k.b2[s] * v[0][p] + k.b2[s ^ 1] * v[1][p]);
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) v[p][s] = o[p][s];
+}
+
+// ZBT gate: in[aq][u] (aq = aux pair (al,ar), u = (bit 2t+1, bit 2t)),
+// controls (fl, fr).  Output aux p = (pl,pr) takes the old bit pair, the
+// new bit pair is s = (sl, sr).
+template <typename C, int DIR>
+__device__ __forceinline__ void zbt_gate(C v[4][4], int fl, int fr, const Coef<C> &k)
+{
+  C o[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int pl = p >> 1, pr = p & 1;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int sl = s >> 1, sr = s & 1;
+      C w;
+      if (DIR == 0)
+        w = k.b1[sr ^ pl] * k.b2[sr ^ fr] * k.b2[sl ^ fl] * k.b3[sr ^ pr] * k.b3[sl ^ pl];
+      else
+        w = k.b1[sl ^ fr] * k.b1[sl ^ pr] * k.b2[sr ^ fr] * k.b2[sl ^ fl] * k.b3[sr ^ pr] *
+            k.b3[sl ^ pl];
+      C acc = 0;
+#pragma unroll
+      for (int aq = 0; aq < 4; ++aq) {
+        const int al = aq >> 1, ar = aq & 1;
+        C c;
+        if (DIR == 0) c = k.b1[sr ^ al] * k.b2[sr ^ ar] * k.b2[sl ^ al];
+        else c = k.b2[sr ^ ar] * k.b2[sl ^ al];
+        acc += v[aq][p] * c;
+      }
+      o[p][s] = w * acc;
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) v[p][s] = o[p][s];
+}
+
+template <int MODEL, int DIR, typename C, typename TI, typename TO>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restrict__ x,
+                                                               const C *__restrict__ zin,
+                                                               C *__restrict__ zout,
+                                                               TO *__restrict__ y, SweepArgs a,
+                                                               const C *__restrict__ tab,
+                                                               const double *__restrict__ scale)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tileLog = a.nlbl + a.nal + a.nbits + a.logW;
+  const int T = 1 << tileLog;
+  C *lds = reinterpret_cast<C *>(smem);
+  C *stab = lds + T;
+  if (a.first || a.last)
+    for (int i = threadIdx.x; i < 192; i += blockDim.x) stab[i] = tab[i];
+  Coef<C> k;
+  for (int i = 0; i < 2; ++i) {
+    k.b1[i] = (C)a.b1[i];
+    k.b2[i] = (C)a.b2[i];
+    k.b3[i] = (C)a.b3[i];
+  }
+  const int L = a.L;
+  const uint64_t N = 1ull << L;
+  const uint64_t tile = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  const uint64_t innerBlocks = (1ull << a.bitlo) >> a.logW;
+  const uint64_t o = tile / innerBlocks, ib = tile % innerBlocks;
+  const uint64_t base = (o << (a.bitlo + a.nbits)) | (ib << a.logW);
+  const int W = 1 << a.logW;
+  const int pmask = (1 << a.nbits) - 1;
+  const int posAux = a.logW + a.nbits;
+  const int posLb = posAux + a.nal;
+  C sin = 1, sout = 1;
+  if (scale) {
+    if (a.first) sin = (C)(*scale);
+    if (a.last) sout = (C)(1.0 / *scale);
+  }
+  __syncthreads();
+
+  // ---- load / build the tile ----
+  for (int e = threadIdx.x; e < T; e += blockDim.x) {
+    const int w = e & (W - 1);
+    const int pl = (e >> a.logW) & pmask;
+    const int aux = (e >> posAux) & (a.na - 1);
+    const int lb = e >> posLb;
+    const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
+    C v;
+    if (a.first) {
+      v = 0;
+      if (MODEL == 0) {
+        if (a.bc == 0) {
+          const int auxw = (int)((P >> (L - 1)) & 1);
+          if ((int)(P & 1) == lb && aux == auxw)
+            v = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P);
+        } else {
+          // confined ends, TMat/tmat.cpp:440-458
+          const int top = (int)(P >> (L - 1));
+          const int want = a.bc == 5 ? 0 : 1;
+          if (aux == want) {
+            const int low = (int)(P & 1) ^ (a.bc == 5 ? 0 : 1);
+            v = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P) * k.b2[low] * k.b2[top];
+          }
+        }
+      } else {
+        const int auxw = (int)((P >> (L - 2)) & 3);
+        if ((int)(P & 3) == lb && aux == auxw)
+          v = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P);
+      }
+    } else {
+      v = zin[((uint64_t)lb * a.na + aux) * N + P];
+    }
+    lds[e] = v;
+  }
+  __syncthreads();
+
+  // ---- gates ----
+  for (int g = 0; g < a.ng; ++g) {
+    const int t = a.t0 + g;
+    if (MODEL == 0) {
+      const int jb = a.logW + (t - a.bitlo);  // tile position of state bit t
+      const int nitems = T >> 2;
+      for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
+        // insert zeros at the two varying positions (ascending)
+        const uint32_t b0 = ins0(ins0((uint32_t)q, jb), posAux);
+        const int lb = (int)(b0 >> posLb);
+        const int w = b0 & (W - 1);
+        const int pl = (b0 >> a.logW) & pmask;
+        const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
+        int c;
+        if (t + 1 < L) c = (int)((P >> (t + 1)) & 1);
+        else c = (a.bc == 0) ? lb : 0;
+        C v[2][2];
+#pragma unroll
+        for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+          for (int u = 0; u < 2; ++u) v[ax][u] = lds[b0 | (ax << posAux) | (u << jb)];
+        xdt_gate(v, c, k);
+#pragma unroll
+        for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+          for (int u = 0; u < 2; ++u) lds[b0 | (ax << posAux) | (u << jb)] = v[ax][u];
+      }
+    } else {
+      const int jb = a.logW + (2 * t - a.bitlo);  // tile position of bit 2t
+      const int nitems = T >> 4;
+      for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
+        uint32_t b0 = ins0((uint32_t)q, jb);
+        b0 = ins0(b0, jb + 1);
+        b0 = ins0(b0, posAux);
+        b0 = ins0(b0, posAux + 1);
+        const int lb = (int)(b0 >> posLb);
+        const int w = b0 & (W - 1);
+        const int pl = (b0 >> a.logW) & pmask;
+        const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
+        int fl, fr;
+        if (2 * t + 2 < L) {
+          fr = (int)((P >> (2 * t + 2)) & 1);
+          fl = (int)((P >> (2 * t + 3)) & 1);
+        } else {
+          fl = lb >> 1;
+          fr = lb & 1;
+        }
+        C v[4][4];
+#pragma unroll
+        for (int ax = 0; ax < 4; ++ax)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) v[ax][u] = lds[b0 | (ax << posAux) | (u << jb)];
+        zbt_gate<C, DIR>(v, fl, fr, k);
+#pragma unroll
+        for (int ax = 0; ax < 4; ++ax)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) lds[b0 | (ax << posAux) | (u << jb)] = v[ax][u];
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- store ----
+  if (!a.last) {
+    for (int e = threadIdx.x; e < T; e += blockDim.x) {
+      const int w = e & (W - 1);
+      const int pl = (e >> a.logW) & pmask;
+      const int aux = (e >> posAux) & (a.na - 1);
+      const int lb = e >> posLb;
+      const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
+      zout[((uint64_t)lb * a.na + aux) * N + P] = lds[e];
+    }
+  } else {
+    const int NP = 1 << (a.nbits + a.logW);
+    const int nz = a.nlb * a.na;
+    const C fac = (C)a.outfac * sout;
+    for (int e = threadIdx.x; e < NP; e += blockDim.x) {
+      const int w = e & (W - 1);
+      const int pl = e >> a.logW;
+      const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
+      C acc = 0;
+      for (int z = 0; z < nz; ++z) acc += lds[(z << posAux) | e];
+      y[P] = (TO)(fac * dev::diag_value<C>(a.dg, stab, P) * acc);
+    }
+  }
+}
+
+struct SPass { int t0, ng, bitlo, nbits, logW; };
+
+std::vector<SPass> plan_sweep(int model, int L, int bc, tm_dtype comp)
+{
+  const bool big = g_lds_per_block >= (size_t)(136 * 1024);
+  const int tl = comp == TM_F64 ? (big ? 14 : 13) : (big ? 15 : 14);
+  const int wl = comp == TM_F64 ? 3 : 4;
+  const int fixed = model == 0 ? (bc == 0 ? 2 : 1) : 4;  // log2(nlb * na)
+  const int bpg = model == 0 ? 1 : 2;                      // state bits per gate
+  const int ngates = model == 0 ? L : L / 2;
+  std::vector<SPass> ps;
+  int t = 0;
+  while (t < ngates) {
+    const int bitlo = t * bpg;
+    int budget = tl - fixed;
+    int logW = 0;
+    if (bitlo > 0) {
+      logW = wl < bitlo ? wl : bitlo;
+      budget -= logW;
+    }
+    int ng = budget / bpg;
+    if (ng > ngates - t) ng = ngates - t;
+    if (ng < 1) ng = 1;
+    const int nbits = ng * bpg;
+    // widen W with the leftover budget on the last pass
+    if (bitlo > 0) {
+      int extra = tl - fixed - nbits - logW;
+      while (extra > 0 && logW < bitlo) { ++logW; --extra; }
+    }
+    ps.push_back({t, ng, bitlo, nbits, logW});
+    t += ng;
+  }
+  return ps;
+}
+
+template <int MODEL, int DIR, typename C, typename TI, typename TO>
+void run_sweep_pass(const SPass &p, SweepArgs a, const void *x, const C *zin, C *zout, void *y,
+                    const C *tab, const double *scale, hipStream_t s)
+{
+  a.t0 = p.t0;
+  a.ng = p.ng;
+  a.bitlo = p.bitlo;
+  a.nbits = p.nbits;
+  a.logW = p.logW;
+  const int tileLog = a.nlbl + a.nal + p.nbits + p.logW;
+  const uint64_t ntile = 1ull << (a.L - p.nbits - p.logW);
+  dim3 grid;
+  if (ntile <= 65535) grid = dim3((unsigned)ntile);
+  else grid = dim3(65536, (unsigned)(ntile / 65536));
+  const size_t shm = ((size_t(1) << tileLog) + 192) * sizeof(C);
+  hipLaunchKernelGGL((k_sweep_pass<MODEL, DIR, C, TI, TO>), grid, dim3(SWEEP_THREADS), shm, s,
+                     static_cast<const TI *>(x), zin, zout, static_cast<TO *>(y), a, tab, scale);
+  TM_LAUNCH_CHECK();
+}
+
+template <int MODEL, int DIR, typename C>
+void sweep_typed(const SweepPlan &plan, const void *in, void *out, tm_dtype io, void *work,
+                 const double *scale, hipStream_t s, const C *tab)
+{
+  SweepArgs a{};
+  a.L = plan.L;
+  a.model = MODEL;
+  a.dir = DIR;
+  a.bc = plan.bc;
+  if (MODEL == 0) {
+    a.nlb = plan.bc == 0 ? 2 : 1;
+    a.na = 2;
+  } else {
+    a.nlb = 4;
+    a.na = 4;
+  }
+  a.nlbl = a.nlb == 1 ? 0 : (a.nlb == 2 ? 1 : 2);
+  a.nal = a.na == 2 ? 1 : 2;
+  a.dg = plan.diag;
+  for (int i = 0; i < 2; ++i) {
+    a.b1[i] = plan.b1[i];
+    a.b2[i] = plan.b2[i];
+    a.b3[i] = plan.b3[i];
+  }
+  a.outfac = MODEL == 0 ? plan.nf : 1.0;
+  auto ps = plan_sweep(MODEL, plan.L, plan.bc, sizeof(C) == 8 ? TM_F64 : TM_F32);
+  const uint64_t N = 1ull << plan.L;
+  C *z0 = static_cast<C *>(work);
+  C *z1 = z0 + (uint64_t)a.nlb * a.na * N;
+  const int np = (int)ps.size();
+  C *zin = z0, *zout = z0;  // passes work in place
+  (void)z1;
+  for (int i = 0; i < np; ++i) {
+    a.first = (i == 0);
+    a.last = (i == np - 1);
+    const double *sc = (a.first || a.last) ? scale : nullptr;
+    if (io == TM_F64)
+      run_sweep_pass<MODEL, DIR, C, double, double>(ps[i], a, in, zin, zout, out, tab, sc, s);
+    else
+      run_sweep_pass<MODEL, DIR, C, float, float>(ps[i], a, in, zin, zout, out, tab, sc, s);
+  }
+}
+
+}  // namespace
+
+size_t sweep_work_bytes(int kind, int L, tm_dtype comp)
+{
+  const size_t es = comp == TM_F64 ? 8 : 4;
+  const size_t mult = kind == TM_KIND_ZBT ? 16 : 4;
+  return mult * (size_t(1) << L) * es;
+}
+
+void launch_sweep(const SweepPlan &plan, const void *in, void *out, tm_dtype io, tm_dtype comp,
+                  int transpose, void *work, const double *scale, hipStream_t s)
+{
+  if (plan.kind == TM_KIND_XDT) {
+    if (comp == TM_F64) sweep_typed<0, 0, double>(plan, in, out, io, work, scale, s, plan.tab64);
+    else sweep_typed<0, 0, float>(plan, in, out, io, work, scale, s, plan.tab32);
+  } else {
+    if (comp == TM_F64) {
+      if (transpose) sweep_typed<1, 1, double>(plan, in, out, io, work, scale, s, plan.tab64);
+      else sweep_typed<1, 0, double>(plan, in, out, io, work, scale, s, plan.tab64);
+    } else {
+      if (transpose) sweep_typed<1, 1, float>(plan, in, out, io, work, scale, s, plan.tab32);
+      else sweep_typed<1, 0, float>(plan, in, out, io, work, scale, s, plan.tab32);
+    }
+  }
+}
+
+}  // namespace ltm

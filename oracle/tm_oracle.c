@@ -1,0 +1,853 @@
+/*
+ * tm_oracle.c -- CPU restatement of the LattTMat transfer-matrix operators.
+ *
+ * TEST INFRASTRUCTURE ONLY (see tm_oracle.h): the checker for the HIP product
+ * path, never part of it.  Every function cites the reference file:line whose
+ * algorithm it restates; loop structure and floating-point operation order
+ * follow the reference so that results agree to rounding.
+ */
+#include "tm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define POP(x) __builtin_popcountll(x)
+
+int or_num_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* bit primitives                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* TMat/utility.hpp:91-94: rotate right by d within n bits */
+uint64_t or_offset_state(uint64_t s, int n, int d)
+{
+  return (s >> d) | ((s & ((1ULL << d) - 1)) << (n - d));
+}
+
+/* TMat/utility.hpp:99-102: rotate left by d within n bits */
+uint64_t or_lffset_state(uint64_t s, int n, int d)
+{
+  return ((s << d) & ((1ULL << n) - 1)) | (s >> (n - d));
+}
+
+/* TMat/utility.cpp:17-31: reverse the low n bits */
+uint64_t or_bitreversal(uint64_t s, int n)
+{
+  s = ((s >> 1) & 0x5555555555555555ULL) | ((s & 0x5555555555555555ULL) << 1);
+  s = ((s >> 2) & 0x3333333333333333ULL) | ((s & 0x3333333333333333ULL) << 2);
+  s = ((s >> 4) & 0x0f0f0f0f0f0f0f0fULL) | ((s & 0x0f0f0f0f0f0f0f0fULL) << 4);
+  s = ((s >> 8) & 0x00ff00ff00ff00ffULL) | ((s & 0x00ff00ff00ff00ffULL) << 8);
+  s = ((s >> 16) & 0x0000ffff0000ffffULL) | ((s & 0x0000ffff0000ffffULL) << 16);
+  s = (s >> 32) | (s << 32);
+  return s >> (64 - n);
+}
+
+/* TMat/utility.hpp:83-86 */
+int or_spinbzexp(uint64_t a, uint64_t b, int n)
+{
+  return 2 * POP(a ^ b) - n;
+}
+
+/* TMat/utility.hpp:36-55: a[0]^(-b) by repeated squaring of a[0] or a[1] */
+double or_powdi(const double a[2], int b)
+{
+  double result = 1.0;
+  const double *p = a;
+  if (b < 0) b = -b; else ++p;
+  double t = *p;
+  while (b) {
+    if (b & 1) result *= t;
+    t *= t;
+    b >>= 1;
+  }
+  return result;
+}
+
+/* TMat/utility.cpp:52-63: rotate each of the s1 rows (s2 bits) right by d */
+uint64_t or_rffset3d(uint64_t s, int s1, int s2, int d)
+{
+  uint64_t mask = (1ULL << s2) - 1, r = 0;
+  for (int rp = 0; rp < s1; ++rp) {
+    uint64_t t = or_offset_state(s & mask, s2, d);
+    r |= t << (rp * s2);
+    s >>= s2;
+  }
+  return r;
+}
+
+/* TMat/utility.cpp:68-73: rotate the rows down by d (whole-word rotate by d*s2) */
+uint64_t or_dffset3d(uint64_t s, int s1, int s2, int d)
+{
+  uint64_t mask = (1ULL << (s2 * d)) - 1, r = s >> (s2 * d);
+  r |= (s & mask) << ((s1 - d) * s2);
+  return r;
+}
+
+/* TMat/utility.cpp:78-91: rotate an n x n layer by 90 degrees */
+uint64_t or_rotate3d(uint64_t s, int n)
+{
+  uint64_t r = 0;
+  for (int rp = 0; rp < n; ++rp)
+    for (int rq = 0; rq < n; ++rq) {
+      int loc = rp * n + rq;
+      if (s & (1ULL << loc)) r |= 1ULL << ((rq + 1) * n - (rp + 1));
+    }
+  return r;
+}
+
+/* TMat/utility.cpp:97-120: which=0 reverses each row, which=1 reverses row order */
+uint64_t or_reflect3d(uint64_t s, int s1, int s2, int which)
+{
+  uint64_t mask = (1ULL << s2) - 1, r = 0;
+  if (!which) {
+    for (int rp = 0; rp < s1; ++rp) {
+      uint64_t t = or_bitreversal(s & mask, s2);
+      r |= t << (rp * s2);
+      s >>= s2;
+    }
+  } else {
+    for (int rp = 0; rp < s1; ++rp) {
+      r <<= s2;
+      r |= s & mask;
+      s >>= s2;
+    }
+  }
+  return r;
+}
+
+/* TMat/utility.hpp:125-128 */
+static inline double bzexpn(int nspin, int nanti, double J)
+{
+  return J * (2.0 * nanti - nspin);
+}
+
+/* ------------------------------------------------------------------------ */
+/* genTvec variants                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* TMat/tmat.hpp:208-310 (XATMatVecProd2D::genTvec) */
+void or_tvec_xat(int L, int bc, double betaJ, double kappa, double h, double *V)
+{
+  uint64_t nstate = 1ULL << L, mask = nstate - 1;
+  double J1 = -betaJ, J2 = betaJ * kappa, hJ = -betaJ * h;
+#pragma omp parallel for schedule(static)
+  for (int64_t is = 0; is < (int64_t)nstate; ++is) {
+    uint64_t s = (uint64_t)is, s1s, s2s;
+    int n1, n2, n0;
+    double b1, b2, b0;
+    switch (bc) {
+      case 0:
+        s1s = or_offset_state(s, L, 1);
+        s2s = or_offset_state(s, L, 2);
+        n1 = POP(s ^ s1s); n2 = POP(s ^ s2s); n0 = POP(s);
+        b1 = bzexpn(L, n1, J1); b2 = bzexpn(L, n2, J2); b0 = bzexpn(L, n0, hJ);
+        break;
+      case 1:
+        s1s = s >> 1; s2s = s >> 2;
+        n1 = POP((s & (mask >> 1)) ^ s1s); n2 = POP((s & (mask >> 2)) ^ s2s); n0 = POP(s);
+        b1 = bzexpn(L - 1, n1, J1); b2 = bzexpn(L - 2, n2, J2); b0 = bzexpn(L, n0, hJ);
+        break;
+      case 5:
+      case 9:
+        s1s = (bc == 5) ? (s << 1) : ((s << 1) | 1);
+        s2s = s >> 1;
+        n1 = POP(s ^ s1s); n2 = POP((s1s & mask) ^ s2s); n0 = POP(s);
+        b1 = bzexpn(L + 1, n1, J1); b2 = bzexpn(L, n2, J2); b0 = bzexpn(L, n0, hJ);
+        break;
+      case 13:
+      case 17:
+        s1s = (bc == 13) ? (s << 1) : ((s << 1) | 1);
+        s2s = (bc == 13) ? (s << 2) : ((s << 2) | 3);
+        n1 = POP(s ^ s1s); n2 = POP(s ^ s2s); n0 = POP(s);
+        b1 = bzexpn(L + 1, n1, J1); b2 = bzexpn(L + 2, n2, J2); b0 = bzexpn(L, n0, hJ);
+        break;
+      default:
+        b1 = b2 = b0 = NAN;
+    }
+    V[s] = exp(b1 + b2 + b0);
+  }
+}
+
+/* TMat/tmat.hpp:437-504 (XDTMatVecProd2D::genTvec) */
+void or_tvec_xdt(int L, int bc, double betaJ, double h, double *V)
+{
+  uint64_t nstate = 1ULL << L;
+  double J1 = -betaJ, hJ = -betaJ * h;
+#pragma omp parallel for schedule(static)
+  for (int64_t is = 0; is < (int64_t)nstate; ++is) {
+    uint64_t s = (uint64_t)is;
+    int n1, n0 = POP(s);
+    double b1;
+    switch (bc) {
+      case 0: n1 = POP(s ^ or_offset_state(s, L, 1)); b1 = bzexpn(L, n1, J1); break;
+      case 1: n1 = POP((s & ((nstate - 1) >> 1)) ^ (s >> 1)); b1 = bzexpn(L - 1, n1, J1); break;
+      case 5: n1 = POP((s << 1) ^ s); b1 = bzexpn(L + 1, n1, J1); break;
+      case 9: n1 = POP(((s << 1) | 1) ^ s); b1 = bzexpn(L + 1, n1, J1); break;
+      default: b1 = NAN;
+    }
+    V[s] = exp(b1 + bzexpn(L, n0, hJ));
+  }
+}
+
+/* TMat/tmat.hpp:537-556 (ZBTMatVecProd2D::genTvec) */
+void or_tvec_zbt(int L, double betaJ, double kappa2, double h, double *V)
+{
+  uint64_t nstate = 1ULL << L;
+  double J1 = -betaJ, J3 = kappa2 * betaJ, hJ = -betaJ * h;
+#pragma omp parallel for schedule(static)
+  for (int64_t is = 0; is < (int64_t)nstate; ++is) {
+    uint64_t s = (uint64_t)is;
+    int n1 = POP(s ^ or_offset_state(s, L, 1));
+    int n2 = POP(s ^ or_offset_state(s, L, 2));
+    int n0 = POP(s);
+    V[s] = exp(bzexpn(L, n1, J1) + bzexpn(L, n2, J3) + bzexpn(L, n0, hJ));
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* orbit enumeration                                                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { uint64_t *w; } bitset;
+static int bs_init(bitset *b, int nbits)
+{
+  uint64_t nchunk = nbits <= 6 ? 1 : (1ULL << (nbits - 6));
+  b->w = (uint64_t *)calloc(nchunk, sizeof(uint64_t));
+  return b->w != NULL;
+}
+static inline int bs_get(const bitset *b, uint64_t s) { return (b->w[s >> 6] >> (s & 63)) & 1; }
+static inline void bs_set(bitset *b, uint64_t s) { b->w[s >> 6] |= 1ULL << (s & 63); }
+
+static void push_rep(uint64_t *reps, uint16_t *dups, int64_t idx, uint64_t s)
+{
+  if (reps) reps[idx] = s;
+  if (dups) dups[idx] = 0;
+}
+
+/* TMat/tmatreduce.cpp:19-111 (genX2DDuplist) */
+int64_t or_orbits_x2d(int L, int bc, int censym, uint64_t *reps, uint16_t *dups)
+{
+  uint64_t nstate = 1ULL << L, mask = nstate - 1;
+  bitset buf;
+  if (!bs_init(&buf, L)) return -1;
+  int64_t m = 0;
+  for (uint64_t now = 0; now < nstate; ++now) {
+    if (bs_get(&buf, now)) continue;
+    push_rep(reps, dups, m, now);
+    uint16_t d = 0;
+    if (bc == 0) {
+      uint64_t nx;
+      for (nx = now; !bs_get(&buf, nx); nx = (nx >> 1) | ((nx & 1) << (L - 1))) {
+        ++d; bs_set(&buf, nx);
+        if (censym) {
+          uint64_t fm = ~nx & mask;
+          if (!bs_get(&buf, fm)) { ++d; bs_set(&buf, fm); }
+        }
+      }
+      for (nx = or_bitreversal(now, L); !bs_get(&buf, nx); nx = (nx >> 1) | ((nx & 1) << (L - 1))) {
+        ++d; bs_set(&buf, nx);
+        if (censym) {
+          uint64_t fm = ~nx & mask;
+          if (!bs_get(&buf, fm)) { ++d; bs_set(&buf, fm); }
+        }
+      }
+    } else {
+      bs_set(&buf, now); ++d;
+      uint64_t nx = or_bitreversal(now, L);
+      if (nx != now) { ++d; bs_set(&buf, nx); }
+      if (censym) {
+        uint64_t fm = ~now & mask;
+        if (!bs_get(&buf, fm)) { ++d; bs_set(&buf, fm); }
+        fm = ~nx & mask;
+        if (!bs_get(&buf, fm)) { ++d; bs_set(&buf, fm); }
+      }
+    }
+    if (dups) dups[m] = d;
+    ++m;
+  }
+  free(buf.w);
+  return m;
+}
+
+/* TMat/tmat3d.cpp:17-98 (genX3DDuplist) */
+int64_t or_orbits_x3d(int s1, int s2, int censym, int rotate, uint64_t *reps, uint16_t *dups)
+{
+  int size = s1 * s2;
+  uint64_t nstate = 1ULL << size, mask = nstate - 1;
+  bitset buf;
+  if (!bs_init(&buf, size)) return -1;
+  int64_t m = 0;
+  for (uint64_t now = 0; now < nstate; ++now) {
+    if (bs_get(&buf, now)) continue;
+    push_rep(reps, dups, m, now);
+    uint16_t d = 0;
+    uint64_t nr = now;
+    int rot_status = 0;
+    do {
+      uint64_t nc = nr;
+      int cen_status = 0;
+      do {
+        uint64_t nm1 = nc;
+        int m1_status = 0;
+        do {
+          uint64_t nm2 = nm1;
+          int m2_status = 0;
+          do {
+            for (uint64_t o1 = nm2; !bs_get(&buf, o1); o1 = or_rffset3d(o1, s1, s2, 1))
+              for (uint64_t o2 = o1; !bs_get(&buf, o2); o2 = or_dffset3d(o2, s1, s2, 1)) {
+                bs_set(&buf, o2);
+                ++d;
+              }
+            if (0 == m2_status) { m2_status = 1; nm2 = or_reflect3d(nm2, s1, s2, 1); }
+            else break;
+          } while (!bs_get(&buf, nm2));
+          if (0 == m1_status) { m1_status = 1; nm1 = or_reflect3d(nm1, s1, s2, 0); }
+          else break;
+        } while (!bs_get(&buf, nm1));
+        if (0 == cen_status && censym) { cen_status = 1; nc = ~nc & mask; }
+        else break;
+      } while (!bs_get(&buf, nc));
+      if (0 == rot_status && rotate) { rot_status = 1; nr = or_rotate3d(nr, s1); }
+      else break;
+    } while (!bs_get(&buf, nr));
+    if (dups) dups[m] = d;
+    ++m;
+  }
+  free(buf.w);
+  return m;
+}
+
+/* ------------------------------------------------------------------------ */
+/* operator construction                                                     */
+/* ------------------------------------------------------------------------ */
+
+static double nf_x2d(int L, double betaJ, double kappa, double *nfexp)
+{
+  /* TMat/tmat.cpp:48-55, TMat/tmatreduce.cpp:349-359 */
+  double e = (kappa < 0.5) ? L * (kappa - 2.0) : -L * (kappa + 1.0);
+  if (nfexp) *nfexp = e;
+  return exp(betaJ * e);
+}
+
+static double nf_x3d(int size, double betaJ, double kappa, double *nfexp)
+{
+  /* TMat/tmat3d.cpp:117-127 */
+  double e = (kappa < 0.5) ? size * (kappa - 3.0) : -size * (kappa + 2.0);
+  if (nfexp) *nfexp = e;
+  return exp(betaJ * e);
+}
+
+/* Energy exponent of a 3D layer, TMat/tmat3d.cpp:188-199 (without Jnormf) */
+static double e3d_half(uint64_t s, int s1, int s2, double betaJ, double kappa, double h)
+{
+  int size = s1 * s2;
+  double hb = 0.5 * betaJ;
+  double J1 = -hb, J2 = hb * kappa, hJ = -hb * h;
+  int n1a = POP(s ^ or_rffset3d(s, s1, s2, 1));
+  int n1b = POP(s ^ or_dffset3d(s, s1, s2, 1));
+  int n2 = POP(s ^ or_rffset3d(s, s1, s2, 2));
+  int n0 = POP(s);
+  return 2 * J1 * (n1a + n1b - size) + J2 * (2 * n2 - size) + hJ * (2 * n0 - size);
+}
+
+or_op *or_op_create(int kind, int size1, int size2, int bc,
+                    double betaJ, double kappa, double kappa2, double h)
+{
+  or_op *op = (or_op *)calloc(1, sizeof(or_op));
+  if (!op) return NULL;
+  op->kind = kind;
+  op->betaJ = betaJ; op->kappa = kappa; op->kappa2 = kappa2; op->h = h;
+  op->bc = bc;
+  op->bzwj1[0] = exp(betaJ); op->bzwj1[1] = 1.0 / op->bzwj1[0];
+  op->bzwj2[0] = exp(-betaJ * kappa); op->bzwj2[1] = 1.0 / op->bzwj2[0];
+  op->bzwj3[0] = exp(-betaJ * kappa2); op->bzwj3[1] = 1.0 / op->bzwj3[0];
+  switch (kind) {
+    case OR_XAT: {
+      /* TMat/tmat.cpp:39-56 */
+      int L = size1;
+      if (L < 2 || L > 30) goto fail;
+      if (!(bc == 0 || bc == 1 || bc == 5 || bc == 9 || bc == 13 || bc == 17)) goto fail;
+      op->L = L; op->nstate = op->ncol = 1LL << L;
+      op->nf = nf_x2d(L, betaJ, kappa, NULL);
+      op->tvec = (double *)malloc(sizeof(double) * op->ncol);
+      if (!op->tvec) goto fail;
+      or_tvec_xat(L, bc, 0.5 * betaJ, kappa, h, op->tvec);
+      break;
+    }
+    case OR_XDT: {
+      /* TMat/tmat.cpp:322-341 */
+      int L = size1;
+      if (L < 3 || L > 30 || !(bc == 0 || bc == 5 || bc == 9)) goto fail;
+      op->L = L; op->nstate = op->ncol = 1LL << L;
+      op->nf = (kappa < 0.5) ? exp(L * betaJ * 2.0 * (kappa - 1.0)) : exp(-L * betaJ * 2.0 * kappa);
+      op->tvec = (double *)malloc(sizeof(double) * op->ncol);
+      if (!op->tvec) goto fail;
+      or_tvec_xdt(L, bc, 0.5 * betaJ, h, op->tvec);
+      break;
+    }
+    case OR_ZBT: {
+      /* TMat/tmat.cpp:480-518 */
+      int L = size1;
+      if (L < 4 || L > 30 || (L & 1)) goto fail;
+      op->L = L; op->nstate = op->ncol = 1LL << L;
+      double nfe;
+      if (kappa + kappa2 < 0.5) nfe = L * 2.0 * (kappa + kappa2 - 1.0);
+      else if (kappa < 0.5 && kappa2 - kappa < 0.5) nfe = -L;
+      else if (kappa2 < 2 * kappa) nfe = -L * 2.0 * kappa;
+      else nfe = L * 2.0 * (kappa - kappa2);
+      double half = exp(0.5 * betaJ * nfe);
+      op->nf = half * half;
+      op->tvec = (double *)malloc(sizeof(double) * op->ncol);
+      if (!op->tvec) goto fail;
+      or_tvec_zbt(L, 0.5 * betaJ, kappa2, h, op->tvec);
+      for (int64_t i = 0; i < op->ncol; ++i) op->tvec[i] *= half;
+      break;
+    }
+    case OR_XAM2D: {
+      /* TMat/tmatreduce.cpp:338-365 and genTvec :367-404 */
+      int L = size1;
+      if (L < 2 || L > 30) goto fail;
+      int extend = bc;
+      op->bc = extend & 1;
+      op->L = L; op->nstate = 1LL << L;
+      op->nleadbit = (L < 10 || op->bc) ? 1 : 2 + (int)log2((double)L);
+      op->censym = (h == 0 && (extend & 2) == 0);
+      double nfe;
+      op->nf = nf_x2d(L, betaJ, kappa, &nfe);
+      int64_t m = or_orbits_x2d(L, op->bc, op->censym, NULL, NULL);
+      if (m <= 0) goto fail;
+      op->ncol = m;
+      op->reps = (uint64_t *)malloc(sizeof(uint64_t) * m);
+      op->dups = (uint16_t *)malloc(sizeof(uint16_t) * m);
+      op->tvec = (double *)malloc(sizeof(double) * m);
+      if (!op->reps || !op->dups || !op->tvec) goto fail;
+      or_orbits_x2d(L, op->bc, op->censym, op->reps, op->dups);
+      double hb = 0.5 * betaJ;
+      double J1 = -hb, J2 = hb * kappa, hJ = -hb * h, Jn = hb * nfe;
+      uint64_t mask = op->nstate - 1;
+#pragma omp parallel for schedule(static)
+      for (int64_t rp = 0; rp < m; ++rp) {
+        uint64_t s = op->reps[rp];
+        int n1, n2, n0 = POP(s);
+        if (!op->bc) {
+          n1 = POP(s ^ or_offset_state(s, L, 1));
+          n2 = POP(s ^ or_offset_state(s, L, 2));
+        } else {
+          n1 = POP((s & (mask >> 1)) ^ (s >> 1));
+          n2 = POP((s & (mask >> 2)) ^ (s >> 2));
+        }
+        double e = J1 * (2 * n1 - L) + J2 * (2 * n2 - L) + hJ * (2 * n0 - L) + Jn;
+        op->tvec[rp] = exp(e) / sqrt((double)op->dups[rp]);
+      }
+      break;
+    }
+    case OR_XAM3D: {
+      /* TMat/tmat3d.cpp:102-134 and genTvec :179-201 */
+      int size = size1 * size2;
+      if (size1 < 2 || size2 < 3 || size > 30) goto fail;
+      op->size1 = size1; op->size2 = size2; op->L = size;
+      op->bc = bc & 1;
+      op->nstate = 1LL << size;
+      op->nleadbit = size < 10 ? 1 : 2 + (int)log2((double)size);
+      op->censym = (h == 0);
+      op->ising = (kappa == 0);
+      op->rotate = op->ising && size1 == size2;
+      double nfe;
+      op->nf = nf_x3d(size, betaJ, kappa, &nfe);
+      int64_t m = or_orbits_x3d(size1, size2, op->censym, op->rotate, NULL, NULL);
+      if (m <= 0) goto fail;
+      op->ncol = m;
+      op->reps = (uint64_t *)malloc(sizeof(uint64_t) * m);
+      op->dups = (uint16_t *)malloc(sizeof(uint16_t) * m);
+      op->tvec = (double *)malloc(sizeof(double) * m);
+      if (!op->reps || !op->dups || !op->tvec) goto fail;
+      or_orbits_x3d(size1, size2, op->censym, op->rotate, op->reps, op->dups);
+      double Jn = 0.5 * betaJ * nfe;
+#pragma omp parallel for schedule(static)
+      for (int64_t rp = 0; rp < m; ++rp) {
+        double e = e3d_half(op->reps[rp], size1, size2, betaJ, kappa, h) + Jn;
+        op->tvec[rp] = exp(e) / sqrt((double)op->dups[rp]);
+      }
+      break;
+    }
+    case OR_XAT3D: {
+      /* Full-space counterpart of XAMMatVecProd3D: same layer energies
+       * (TMat/tmat3d.cpp:188-199), same normfactor (:117-127), Th = (x)K. */
+      int size = size1 * size2;
+      if (size1 < 2 || size2 < 3 || size > 30) goto fail;
+      op->size1 = size1; op->size2 = size2; op->L = size;
+      op->nstate = op->ncol = 1LL << size;
+      double nfe;
+      op->nf = nf_x3d(size, betaJ, kappa, &nfe);
+      op->tvec = (double *)malloc(sizeof(double) * op->ncol);
+      if (!op->tvec) goto fail;
+      double Jn = 0.5 * betaJ * nfe;
+#pragma omp parallel for schedule(static)
+      for (int64_t s = 0; s < op->ncol; ++s)
+        op->tvec[s] = exp(e3d_half((uint64_t)s, size1, size2, betaJ, kappa, h) + Jn);
+      break;
+    }
+    default:
+      goto fail;
+  }
+  return op;
+fail:
+  or_op_destroy(op);
+  return NULL;
+}
+
+void or_op_destroy(or_op *op)
+{
+  if (!op) return;
+  free(op->tvec);
+  free(op->reps);
+  free(op->dups);
+  free(op);
+}
+
+int64_t or_op_ncol(const or_op *op) { return op->ncol; }
+double or_op_normfactor(const or_op *op) { return op->nf; }
+const double *or_op_tvec(const or_op *op) { return op->tvec; }
+const uint64_t *or_op_reps(const or_op *op) { return op->reps; }
+const uint16_t *or_op_dups(const or_op *op) { return op->dups; }
+
+/* ------------------------------------------------------------------------ */
+/* operator application                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* In-place Kronecker butterfly on nbits low bits: TMat/tmat.cpp:80-93,
+ * TMat/tmatreduce.cpp:546-563, TMat/tmat3d.cpp:323-340. */
+static void butterfly_low(double *y, int64_t n, int nbits, const double bz[2])
+{
+  for (int it = 0; it < nbits; ++it) {
+    uint64_t ms = 1ULL << it;
+#pragma omp parallel for schedule(static)
+    for (int64_t rp = 0; rp < n; ++rp) {
+      if ((uint64_t)rp & ms) {
+        double t = y[rp];
+        y[rp] = bz[0] * t + bz[1] * y[rp ^ ms];
+        y[rp ^ ms] = bz[1] * t + bz[0] * y[rp ^ ms];
+      }
+    }
+  }
+}
+
+/* TMat/tmat.cpp:71-102 (XATMatVecProd2D::perform_op) */
+static void apply_xat(const or_op *op, const double *x, double *y)
+{
+  int64_t n = op->nstate;
+#pragma omp parallel for schedule(static)
+  for (int64_t rp = 0; rp < n; ++rp) y[rp] = op->tvec[rp] * x[rp];
+  butterfly_low(y, n, op->L, op->bzwj1);
+#pragma omp parallel for schedule(static)
+  for (int64_t rp = 0; rp < n; ++rp) y[rp] = op->nf * op->tvec[rp] * y[rp];
+}
+
+/* Full 3D: same structure as XAT with sqrt(nf) folded into tvec. */
+static void apply_xat3d(const or_op *op, const double *x, double *y)
+{
+  int64_t n = op->nstate;
+#pragma omp parallel for schedule(static)
+  for (int64_t rp = 0; rp < n; ++rp) y[rp] = op->tvec[rp] * x[rp];
+  butterfly_low(y, n, op->L, op->bzwj1);
+#pragma omp parallel for schedule(static)
+  for (int64_t rp = 0; rp < n; ++rp) y[rp] = op->tvec[rp] * y[rp];
+}
+
+/* TMat/tmat.cpp:344-359 (XDT permute_mult_T1) and :361-376 (_last) */
+static void xdt_step(const or_op *op, const double *xi, double *yo, int last, uint64_t lastbit)
+{
+  int L = op->L;
+  uint64_t N = (uint64_t)op->nstate, nb = N << 1, mask = N - 1, maskb = N - 2;
+  const double *b1 = op->bzwj1, *b2 = op->bzwj2;
+#pragma omp parallel for schedule(static)
+  for (int64_t irpb = 0; irpb < (int64_t)nb; ++irpb) {
+    uint64_t rpb = (uint64_t)irpb;
+    uint64_t auxp = rpb >> L;
+    uint64_t rp = rpb & mask;
+    uint64_t rpp = (auxp << L) | or_offset_state(rp, L, 1);
+    uint64_t rq = (rp & maskb) | auxp;
+    uint64_t sbit = rp & 1;
+    uint64_t nb2 = last ? lastbit : ((rq >> 1) & 1);
+    yo[rpp] = b1[sbit ^ auxp] * b2[nb2 ^ sbit] * (b2[sbit] * xi[rq] + b2[sbit ^ 1] * xi[rq | N]);
+  }
+}
+
+/* TMat/tmat.cpp:379-476 (XDTMatVecProd2D::perform_op) */
+static void apply_xdt(const or_op *op, const double *x, double *y)
+{
+  int L = op->L;
+  uint64_t N = (uint64_t)op->nstate, nc = N >> 1;
+  double *t1 = (double *)malloc(sizeof(double) * 2 * N);
+  double *t2 = (double *)malloc(sizeof(double) * 2 * N);
+  const double *tv = op->tvec;
+  if (op->bc == 0) {
+    for (uint64_t lastbit = 0; lastbit < 2; ++lastbit) {
+      double *p1 = t1, *p2 = t2, *sw;
+#pragma omp parallel for schedule(static)
+      for (int64_t irp = 0; irp < (int64_t)N; ++irp) {
+        uint64_t rp = (uint64_t)irp;
+        uint64_t rpb = ((rp & nc) << 1) | rp;
+        p1[rpb] = ((rp & 1) == lastbit) ? tv[rp] * x[rp] : 0.0;
+        p1[rpb ^ N] = 0.0;
+      }
+      for (int it = 0; it < L - 1; ++it) {
+        xdt_step(op, p1, p2, 0, 0);
+        sw = p1; p1 = p2; p2 = sw;
+      }
+      xdt_step(op, p1, p2, 1, lastbit);
+#pragma omp parallel for schedule(static)
+      for (int64_t irp = 0; irp < (int64_t)N; ++irp) {
+        uint64_t rp = (uint64_t)irp;
+        double v = op->nf * tv[rp] * (p2[rp] + p2[rp ^ N]);
+        if (lastbit == 0) y[rp] = v; else y[rp] += v;
+      }
+    }
+  } else {
+    double *p1 = t1, *p2 = t2, *sw;
+    const double *b2 = op->bzwj2;
+#pragma omp parallel for schedule(static)
+    for (int64_t irp = 0; irp < (int64_t)N; ++irp) {
+      uint64_t rp = (uint64_t)irp;
+      if (op->bc == 5) {
+        p1[rp] = tv[rp] * x[rp] * b2[rp & 1] * b2[rp >> (L - 1)];
+        p1[rp | N] = 0.0;
+      } else {
+        p1[rp | N] = tv[rp] * x[rp] * b2[(rp & 1) ^ 1] * b2[rp >> (L - 1)];
+        p1[rp] = 0.0;
+      }
+    }
+    for (int it = 0; it < L - 1; ++it) {
+      xdt_step(op, p1, p2, 0, 0);
+      sw = p1; p1 = p2; p2 = sw;
+    }
+    xdt_step(op, p1, p2, 1, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t irp = 0; irp < (int64_t)N; ++irp) {
+      uint64_t rp = (uint64_t)irp;
+      y[rp] = op->nf * tv[rp] * (p2[rp] + p2[rp | N]);
+    }
+  }
+  free(t1);
+  free(t2);
+}
+
+/* TMat/tmat.cpp:521-553 (ZBT permute_mult_T1) and :556-588 (_last) */
+static void zbt_step(const or_op *op, const double *xi, double *yo, int last, uint64_t lastbit, int dir)
+{
+  int L = op->L;
+  uint64_t N = (uint64_t)op->nstate, nb = N << 2, mask = N - 1, maskb = N - 4;
+  const double *b1 = op->bzwj1, *b2 = op->bzwj2, *b3 = op->bzwj3;
+#pragma omp parallel for schedule(static)
+  for (int64_t irpb = 0; irpb < (int64_t)nb; ++irpb) {
+    uint64_t rpb = (uint64_t)irpb;
+    uint64_t auxp = rpb >> L;
+    uint64_t rp = rpb & mask;
+    uint64_t rpp = (auxp << L) | or_offset_state(rp, L, 2);
+    uint64_t rq = (rp & maskb) | auxp;
+    uint64_t sl = (rp >> 1) & 1, sr = rp & 1, pl = auxp >> 1, pr = auxp & 1;
+    uint64_t fl = last ? (lastbit >> 1) : ((rp >> 3) & 1);
+    uint64_t fr = last ? (lastbit & 1) : ((rp >> 2) & 1);
+    double w;
+    if (!dir)
+      w = b1[sr ^ pl] * b2[sr ^ fr] * b2[sl ^ fl] * b3[sr ^ pr] * b3[sl ^ pl];
+    else
+      w = b1[sl ^ fr] * b1[sl ^ pr] * b2[sr ^ fr] * b2[sl ^ fl] * b3[sr ^ pr] * b3[sl ^ pl];
+    double acc = 0.0;
+    for (uint64_t aq = 0; aq < 4; ++aq) {
+      uint64_t al = aq >> 1, ar = aq & 1;
+      uint64_t rqq = (aq << L) | rq;
+      if (!dir) acc += xi[rqq] * b1[sr ^ al] * b2[sr ^ ar] * b2[sl ^ al];
+      else acc += xi[rqq] * b2[sr ^ ar] * b2[sl ^ al];
+    }
+    w *= acc;
+    yo[rpp] = w;
+  }
+}
+
+/* TMat/tmat.cpp:596-649 (ZBTMatVecProd2D::perform_op_internal) */
+static void apply_zbt(const or_op *op, const double *x, double *y, int dir)
+{
+  int L = op->L;
+  uint64_t N = (uint64_t)op->nstate, nc = (N >> 1) | (N >> 2);
+  double *t1 = (double *)malloc(sizeof(double) * 4 * N);
+  double *t2 = (double *)malloc(sizeof(double) * 4 * N);
+  const double *tv = op->tvec;
+  for (uint64_t lastbit = 0; lastbit < 4; ++lastbit) {
+    double *p1 = t1, *p2 = t2, *sw;
+#pragma omp parallel for schedule(static)
+    for (int64_t irp = 0; irp < (int64_t)N; ++irp) {
+      uint64_t rp = (uint64_t)irp;
+      uint64_t rpb = ((rp & nc) << 2) | rp;
+      p1[rpb] = ((rp & 3) == lastbit) ? tv[rp] * x[rp] : 0.0;
+      p1[rpb ^ N] = 0.0;
+      p1[rpb ^ (N << 1)] = 0.0;
+      p1[rpb ^ (nc << 2)] = 0.0;
+    }
+    for (int it = 0; it < L / 2 - 1; ++it) {
+      zbt_step(op, p1, p2, 0, 0, dir);
+      sw = p1; p1 = p2; p2 = sw;
+    }
+    zbt_step(op, p1, p2, 1, lastbit, dir);
+#pragma omp parallel for schedule(static)
+    for (int64_t irp = 0; irp < (int64_t)N; ++irp) {
+      uint64_t rp = (uint64_t)irp;
+      double v = tv[rp] * (p2[rp] + p2[rp ^ N] + p2[rp ^ (N << 1)] + p2[rp ^ (nc << 2)]);
+      if (lastbit == 0) y[rp] = v; else y[rp] += v;
+    }
+  }
+  free(t1);
+  free(t2);
+}
+
+/* TMat/tmatreduce.cpp:406-529 (XAMMatVecProd2D::perform_op) */
+static void apply_xam2d(const or_op *op, const double *x, double *y)
+{
+  int L = op->L, nlb = op->nleadbit;
+  uint64_t N = (uint64_t)op->nstate;
+  uint64_t nloop = 1ULL << nlb, vecsize = N >> nlb;
+  uint64_t maskb = vecsize - 1, maskc = nloop - 1, maskbinv = ~maskb & (N - 1);
+  int nsub = L - nlb;
+  int64_t m = op->ncol;
+  double *yt = (double *)malloc(sizeof(double) * vecsize);
+  const uint64_t *idx = op->reps;
+  const double *tv = op->tvec;
+  int bc = op->bc;
+  if (op->censym) {
+    nloop >>= 1;
+    for (uint64_t rdup = 0; rdup < nloop; ++rdup) {
+      uint64_t rh = rdup << nsub, rr = or_bitreversal(rdup, nlb);
+#pragma omp parallel for schedule(dynamic, 4096)
+      for (int64_t rp = 0; rp < m; ++rp) {
+        double d = tv[rp] * x[rp];
+        uint64_t now = idx[rp], nx = now;
+        do {
+          if ((nx & maskbinv) == rh) yt[nx & maskb] = d;
+          else if ((~nx & maskbinv) == rh) yt[~nx & maskb] = d;
+          if ((nx & maskc) == rr) yt[or_bitreversal(nx, L) & maskb] = d;
+          else if ((~nx & maskc) == rr) yt[or_bitreversal(~nx, L) & maskb] = d;
+          if (bc) break;
+          nx = (nx >> 1) | ((nx & 1) << (L - 1));
+        } while (nx != now);
+      }
+      butterfly_low(yt, (int64_t)vecsize, nsub, op->bzwj1);
+#pragma omp parallel for schedule(static)
+      for (int64_t rp = 0; rp < m; ++rp) {
+        uint64_t now = idx[rp];
+        double d = or_powdi(op->bzwj1, or_spinbzexp(now >> nsub, rdup, nlb));
+        double v = d * yt[now & maskb] + yt[~now & maskb] / d;
+        if (rdup) y[rp] += v; else y[rp] = v;
+      }
+    }
+  } else {
+    for (uint64_t rdup = 0; rdup < nloop; ++rdup) {
+      uint64_t rh = rdup << nsub, rr = or_bitreversal(rdup, nlb);
+#pragma omp parallel for schedule(dynamic, 4096)
+      for (int64_t rp = 0; rp < m; ++rp) {
+        double d = tv[rp] * x[rp];
+        uint64_t now = idx[rp], nx = now;
+        do {
+          if ((nx & maskbinv) == rh) yt[nx & maskb] = d;
+          if ((nx & maskc) == rr) yt[or_bitreversal(nx, L) & maskb] = d;
+          if (bc) break;
+          nx = (nx >> 1) | ((nx & 1) << (L - 1));
+        } while (nx != now);
+      }
+      butterfly_low(yt, (int64_t)vecsize, nsub, op->bzwj1);
+#pragma omp parallel for schedule(static)
+      for (int64_t rp = 0; rp < m; ++rp) {
+        uint64_t now = idx[rp];
+        double d = or_powdi(op->bzwj1, or_spinbzexp(now >> nsub, rdup, nlb));
+        double v = d * yt[now & maskb];
+        if (rdup) y[rp] += v; else y[rp] = v;
+      }
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t rp = 0; rp < m; ++rp) y[rp] *= op->dups[rp] * tv[rp];
+  free(yt);
+}
+
+/* TMat/tmat3d.cpp:203-321 (XAMMatVecProd3D::perform_op) */
+static void apply_xam3d(const or_op *op, const double *x, double *y)
+{
+  int s1 = op->size1, s2 = op->size2, size = op->L, nlb = op->nleadbit;
+  uint64_t N = (uint64_t)op->nstate, mask = N - 1;
+  uint64_t nloop = 1ULL << nlb, vecsize = N >> nlb;
+  uint64_t maskb = vecsize - 1, maskbinv = ~maskb & (N - 1);
+  int nsub = size - nlb;
+  int64_t m = op->ncol;
+  double *yt = (double *)malloc(sizeof(double) * vecsize);
+  if (op->censym) nloop >>= 1;
+  int ncen = op->censym ? 2 : 1, nrot = op->rotate ? 2 : 1;
+  for (uint64_t rdup = 0; rdup < nloop; ++rdup) {
+    uint64_t rh = rdup << nsub;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t rp = 0; rp < m; ++rp) {
+      double d = op->tvec[rp] * x[rp];
+      uint64_t nr = op->reps[rp];
+      for (int ir = 0; ir < nrot; ++ir) {
+        uint64_t nc = nr;
+        for (int ic = 0; ic < ncen; ++ic) {
+          uint64_t nm1 = nc;
+          for (int i1 = 0; i1 < 2; ++i1) {
+            uint64_t nm2 = nm1;
+            for (int i2 = 0; i2 < 2; ++i2) {
+              uint64_t o1 = nm2;
+              do {
+                uint64_t o2 = o1;
+                do {
+                  if ((o2 & maskbinv) == rh) yt[o2 & maskb] = d;
+                  o2 = or_dffset3d(o2, s1, s2, 1);
+                } while (o2 != o1);
+                o1 = or_rffset3d(o1, s1, s2, 1);
+              } while (o1 != nm2);
+              nm2 = or_reflect3d(nm2, s1, s2, 1);
+            }
+            nm1 = or_reflect3d(nm1, s1, s2, 0);
+          }
+          nc = ~nc & mask;
+        }
+        nr = or_rotate3d(nr, s1);
+      }
+    }
+    butterfly_low(yt, (int64_t)vecsize, nsub, op->bzwj1);
+#pragma omp parallel for schedule(static)
+    for (int64_t rp = 0; rp < m; ++rp) {
+      uint64_t now = op->reps[rp];
+      double d = or_powdi(op->bzwj1, or_spinbzexp(now >> nsub, rdup, nlb));
+      double v;
+      if (op->censym) v = d * yt[now & maskb] + yt[~now & maskb] / d;
+      else v = d * yt[now & maskb];
+      if (rdup) y[rp] += v; else y[rp] = v;
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t rp = 0; rp < m; ++rp) y[rp] *= op->dups[rp] * op->tvec[rp];
+  free(yt);
+}
+
+void or_op_apply(const or_op *op, const double *x, double *y, int transpose)
+{
+  switch (op->kind) {
+    case OR_XAT: apply_xat(op, x, y); break;
+    case OR_XAT3D: apply_xat3d(op, x, y); break;
+    case OR_XDT: apply_xdt(op, x, y); break;
+    case OR_ZBT: apply_zbt(op, x, y, transpose ? 1 : 0); break;
+    case OR_XAM2D: apply_xam2d(op, x, y); break;
+    case OR_XAM3D: apply_xam3d(op, x, y); break;
+    default: break;
+  }
+}

@@ -1,0 +1,112 @@
+// CPU implementations of the Krylov vector kernels and a dense-matrix
+// operator, for unit tests of the eigen driver without a GPU.  Test
+// infrastructure only.
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <vector>
+
+#include "../../latttmat_amd/csrc/tm_internal.hpp"
+
+namespace ltm {
+size_t g_lds_per_block = 160 * 1024;
+[[noreturn]] void throw_hip(hipError_t, const char *expr, const char *, int)
+{
+  throw Error(TM_EHIP, expr);
+}
+void DevBuf::alloc(size_t bytes)
+{
+  release();
+  p_ = bytes ? std::calloc(bytes, 1) : nullptr;
+  n_ = bytes;
+}
+void DevBuf::release()
+{
+  std::free(p_);
+  p_ = nullptr;
+  n_ = 0;
+}
+namespace kry {
+void multidot(const double *V, int64_t ldv, int j, const double *w, int64_t n, double *, double *out, hipStream_t)
+{
+  for (int k = 0; k <= j; ++k) out[k] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < j; ++k) out[k] += V[k * ldv + i] * w[i];
+    out[j] += w[i] * w[i];
+  }
+}
+void update_dots(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h, double *, double *out, hipStream_t)
+{
+  std::vector<double> hh(h, h + j);
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < j; ++k) w[i] -= hh[k] * V[k * ldv + i];
+  multidot(V, ldv, j, w, n, nullptr, out, nullptr);
+}
+void update_norm(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h, double *, double *out, hipStream_t)
+{
+  std::vector<double> hh(h, h + j);
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < j; ++k) w[i] -= hh[k] * V[k * ldv + i];
+  out[0] = 0;
+  for (int64_t i = 0; i < n; ++i) out[0] += w[i] * w[i];
+}
+void normalize(double *w, int64_t n, const double *nrm2, hipStream_t)
+{
+  const double inv = 1.0 / std::sqrt(nrm2[0]);
+  for (int64_t i = 0; i < n; ++i) w[i] *= inv;
+}
+void gemm_vy(const double *Vin, int64_t ldv, int m, const double *Y, int ldy, int k, double *Vout, int64_t ldo, int64_t n, hipStream_t)
+{
+  std::vector<double> row(m);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int r = 0; r < m; ++r) row[r] = Vin[r * ldv + i];
+    for (int c = 0; c < k; ++c) {
+      double s = 0;
+      for (int r = 0; r < m; ++r) s += row[r] * Y[r + (int64_t)c * ldy];
+      Vout[c * ldo + i] = s;
+    }
+  }
+}
+void random_uniform(double *v, int64_t n, uint64_t seed, hipStream_t)
+{
+  uint64_t x = seed * 0x9e3779b97f4a7c15ull + 1;
+  for (int64_t i = 0; i < n; ++i) {
+    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+    v[i] = ((double)(x >> 11) + 0.5) / 9007199254740992.0;
+  }
+}
+void copy(const double *a, double *b, int64_t n, hipStream_t) { for (int64_t i = 0; i < n; ++i) b[i] = a[i]; }
+void to_f32(const double *a, float *b, int64_t n, hipStream_t) { for (int64_t i = 0; i < n; ++i) b[i] = (float)a[i]; }
+void fill(double *a, double v, int64_t n, hipStream_t) { for (int64_t i = 0; i < n; ++i) a[i] = v; }
+void pow2_scale(const double *, int64_t, double *, double *scale, hipStream_t) { scale[0] = 1.0; }
+void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double *, double *G, hipStream_t)
+{
+  for (int r = 0; r < k; ++r)
+    for (int c = 0; c < k; ++c) {
+      double s = 0;
+      for (int64_t i = 0; i < n; ++i) s += A[r * ld + i] * B[c * ld + i];
+      G[r * k + c] = s;
+    }
+}
+}  // namespace kry
+}  // namespace ltm
+
+// dense operator: op->tab64 holds the n x n column-major matrix
+tm_op::~tm_op() = default;
+void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype, tm_dtype comp, int transpose,
+                      const double *, hipStream_t)
+{
+  const double *A = tab64.as<double>();
+  const int64_t n = ncol;
+  for (int v = 0; v < nvec; ++v) {
+    const double *xv = static_cast<const double *>(x) + v * ld;
+    double *yv = static_cast<double *>(y) + v * ld;
+    for (int64_t i = 0; i < n; ++i) {
+      double s = 0;
+      for (int64_t j = 0; j < n; ++j) s += (transpose ? A[j + i * n] : A[i + j * n]) * xv[j];
+      // FP32 compute: round the result to single precision
+      yv[i] = comp == TM_F32 ? (double)(float)s : s;
+    }
+    ++matvecs;
+  }
+}

@@ -1,0 +1,19 @@
+// Host-only stand-in for the handful of HIP runtime calls made by the
+// eigen driver (tm_solver.cpp) and the dense eigensolvers, so that the
+// driver logic can be unit-tested with g++ on a machine without a GPU.
+// Memory is plain host memory; "device" kernels are provided by
+// fake_backend.cpp.  Test infrastructure only: never linked into
+// liblatttmat.so.
+#pragma once
+#include <cstddef>
+#include <cstring>
+typedef int hipError_t;
+typedef void *hipStream_t;
+enum { hipSuccess = 0, hipErrorOutOfMemory = 2 };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
+inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return 0; }
+inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) { std::memmove(d, s, n); return 0; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
+inline hipError_t hipGetLastError() { return 0; }
+inline const char *hipGetErrorString(hipError_t) { return "fake"; }
+struct dim3 { unsigned x, y, z; dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {} };

@@ -1,0 +1,163 @@
+"""Dense transfer matrices written directly from the lattice Hamiltonians.
+
+These are independent of the operator algorithms (butterflies, sweeps,
+orbit scatter/gather): each entry T(a, b) is the Boltzmann weight of the two
+layers a, b summed from the bond definitions.  Spin convention of the
+reference: bit i of a state is site i, a set bit is a down spin
+(readme.md:100), so sum_i s_i s'_i = L - 2 popcount(a ^ b).
+
+Hamiltonian sources in the reference (the same couplings as the operators):
+  ANNNI perp-TM  DumpConfig/dumpreduce.cpp:26-45
+  DNNI           DumpConfig/dumpreduce.cpp:495-514, TMat/tmat.cpp:713-734
+  BNNNI/3NN      DumpConfig/dumpreduce.cpp:374-407 (diagonal TM)
+  3D ANNNI       TMat/tmat3d.cpp:188-199 (layer), Th = prod K over sites
+"""
+import numpy as np
+
+
+def pc(x):
+    return bin(int(x)).count("1")
+
+
+def rotr(s, n, d):
+    return (s >> d) | ((s & ((1 << d) - 1)) << (n - d))
+
+
+def rotl(s, n, d):
+    return ((s << d) & ((1 << n) - 1)) | (s >> (n - d))
+
+
+def cross_nn(a, b, L):
+    """sum_i s_i s'_i"""
+    return L - 2 * pc(a ^ b)
+
+
+def pairsum(x, L):
+    return L - 2 * pc(x)
+
+
+def xat_dense(L, bJ, k, h):
+    """Symmetrised ANNNI perpendicular TM (without the normfactor)."""
+    N = 1 << L
+    e = np.array([bJ * pairsum(s ^ rotr(s, L, 1), L) - k * bJ * pairsum(s ^ rotr(s, L, 2), L)
+                  + h * bJ * pairsum(s, L) for s in range(N)])
+    T = np.empty((N, N))
+    for a in range(N):
+        for b in range(N):
+            T[a, b] = np.exp(0.5 * e[a] + 0.5 * e[b] + bJ * cross_nn(a, b, L))
+    return T
+
+
+def xdt_dense(L, bJ, k, h):
+    """Symmetrised DNNI TM: NN in-layer + field on the layer, NN and the two
+    diagonal NNN (-kappa) across layers."""
+    N = 1 << L
+    e = np.array([bJ * pairsum(s ^ rotr(s, L, 1), L) + h * bJ * pairsum(s, L) for s in range(N)])
+    T = np.empty((N, N))
+    for a in range(N):
+        for b in range(N):
+            c = (bJ * cross_nn(a, b, L) - k * bJ * pairsum(a ^ rotr(b, L, 1), L)
+                 - k * bJ * pairsum(a ^ rotl(b, L, 1), L))
+            T[a, b] = np.exp(0.5 * e[a] + 0.5 * e[b] + c)
+    return T
+
+
+def zbt_dense(L, bJ, k, k2, h):
+    """BNNNI + 3NN on the diagonal TM (row index = layer a, column = b)."""
+    N = 1 << L
+    e = np.array([h * bJ * pairsum(s, L) + bJ * pairsum(s ^ rotr(s, L, 1), L)
+                  - k2 * bJ * pairsum(s ^ rotr(s, L, 2), L) for s in range(N)])
+
+    def cross(ra, rb):
+        t = (-k2 * bJ * pairsum(ra ^ rb, L) - k * bJ * pairsum(ra ^ rotr(rb, L, 2), L)
+             - k * bJ * pairsum(ra ^ rotl(rb, L, 2), L))
+        # zig-zag nearest neighbours between the two diagonal layers; the
+        # rotated copy keeps bit L (= the top bit again) so that the last pair
+        # wraps around the ring
+        rb2 = (rb << 1) | (rb >> (L - 1))
+        nanti = 0
+        for _ in range(L // 2):
+            nanti += (ra ^ rb2) & 1
+            rb2 >>= 2
+            nanti += (ra ^ rb2) & 1
+            ra >>= 2
+        return t + bJ * (L - 2 * nanti)
+
+    T = np.empty((N, N))
+    for a in range(N):
+        for b in range(N):
+            T[a, b] = np.exp(0.5 * e[a] + 0.5 * e[b] + cross(a, b))
+    return T
+
+
+def layer3d_energy(s, s1, s2, bJ, k, h):
+    L = s1 * s2
+    m = (1 << s2) - 1
+
+    def rff(x, d):
+        r = 0
+        for i in range(s1):
+            r |= rotr((x >> (i * s2)) & m, s2, d) << (i * s2)
+        return r
+    nn = pairsum(s ^ rff(s, 1), L) + pairsum(s ^ rotr(s, L, s2), L)
+    return bJ * nn - k * bJ * pairsum(s ^ rff(s, 2), L) + h * bJ * pairsum(s, L)
+
+
+def xat3d_dense(s1, s2, bJ, k, h):
+    L = s1 * s2
+    N = 1 << L
+    e = np.array([layer3d_energy(s, s1, s2, bJ, k, h) for s in range(N)])
+    T = np.empty((N, N))
+    for a in range(N):
+        for b in range(N):
+            T[a, b] = np.exp(0.5 * e[a] + 0.5 * e[b] + bJ * cross_nn(a, b, L))
+    return T
+
+
+def kaufman(L, K):
+    """lambda0, lambda1 of the 2D Ising perpendicular TM (kappa=0, h=0),
+    periodic width L, coupling K = betaJ (Kaufman 1949)."""
+    Ks = np.arctanh(np.exp(-2 * K))
+
+    def gam(k):
+        c = (np.cosh(2 * Ks) * np.cosh(2 * K) - np.sinh(2 * Ks) * np.sinh(2 * K)
+             * np.cos(np.pi * k / L))
+        return np.arccosh(c)
+    pre = (2 * np.sinh(2 * K)) ** (L / 2)
+    odd = sum(gam(k) for k in range(1, 2 * L, 2))
+    even = sum(gam(k) for k in range(2, 2 * L, 2))
+    l0 = pre * np.exp(0.5 * odd)
+    l1 = pre * np.exp(0.5 * (2 * (K - Ks) + even))
+    return l0, l1
+
+
+def orbit_closure_2d(L, bc, censym):
+    """Brute-force orbits: reps (minima, ascending) and sizes."""
+    N = 1 << L
+    mask = N - 1
+    seen = np.zeros(N, bool)
+    reps, dups = [], []
+
+    def rev(s):
+        return int(format(s, f"0{L}b")[::-1], 2)
+    for s in range(N):
+        if seen[s]:
+            continue
+        orb = set()
+        frontier = [s]
+        while frontier:
+            t = frontier.pop()
+            if t in orb:
+                continue
+            orb.add(t)
+            nxt = [rev(t)]
+            if bc == 0:
+                nxt.append(rotr(t, L, 1))
+            if censym:
+                nxt.append(~t & mask)
+            frontier.extend(nxt)
+        for t in orb:
+            seen[t] = True
+        reps.append(min(orb))
+        dups.append(len(orb))
+    return np.array(reps, np.uint64), np.array(dups, np.uint16)

@@ -1,0 +1,92 @@
+"""Eigen driver (latttmat_amd/csrc/tm_solver.cpp + tm_dense.cpp) on the CPU.
+
+tests/native builds the driver with g++ against a host stand-in for the
+few HIP runtime calls it makes and a dense-matrix operator, so the restart
+logic, the conjugate-pair handling, the mixed-precision refinement and the
+retry/left-vector semantics of pfunc_inner (TMat/pfunc.hpp:39-114) are
+checked here without a GPU, against LAPACK (numpy)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NATIVE = os.path.join(HERE, "native")
+EXE = os.path.join(NATIVE, "driver_test")
+
+
+@pytest.fixture(scope="module")
+def driver():
+    subprocess.run(["make", "-s", "-C", NATIVE], check=True, capture_output=True)
+    return EXE
+
+
+def run(driver, A, sym, nev, ncv=0, prec=0, left=0, tmp=None):
+    n = A.shape[0]
+    path = os.path.join(tmp, "m.txt")
+    with open(path, "w") as f:
+        f.write(f"{n} {int(sym)} {nev} {ncv} {prec} {left}\n")
+        f.write(" ".join("%.17g" % v for v in A.flatten(order="F")))
+    r = subprocess.run([driver, path], capture_output=True, text=True, timeout=120)
+    out = {"evals": [], "res": []}
+    for line in r.stdout.splitlines():
+        p = line.split()
+        if p[0] == "status":
+            out["status"] = int(p[1])
+            out["nconv"] = int(p[3])
+            out["mv64"], out["mv32"] = int(p[5]), int(p[6])
+        elif p[0] == "eval":
+            out["evals"].append(complex(float(p[1]), float(p[2])))
+            out["res"].append(float(p[3]))
+        elif p[0] == "left":
+            out["left"] = float(p[1])
+    return out
+
+
+def test_symmetric_xat(driver, oracle, tmp_path):
+    D = oracle.OracleOp(oracle.XAT, 9, 0, 0, 0.5, 0.6, 0.0, 0.0).dense()
+    D = 0.5 * (D + D.T)
+    w = np.sort(np.linalg.eigvalsh(D))[::-1]
+    for nev in (1, 2, 4):
+        o = run(driver, D, True, nev, tmp=str(tmp_path))
+        assert o["status"] == 0 and o["nconv"] == nev
+        got = np.array([e.real for e in o["evals"]])
+        assert np.all(np.abs(got / w[:nev] - 1) < 1e-12), (got, w[:nev])
+
+
+def test_nonsymmetric_zbt_pairs_and_left(driver, oracle, tmp_path):
+    D = oracle.OracleOp(oracle.ZBT, 10, 0, 0, 0.5, 0.5, 0.3, 0.0).dense()
+    w = np.linalg.eigvals(D)
+    w = w[np.argsort(-np.abs(w))]
+    o = run(driver, D, False, 2, left=1, tmp=str(tmp_path))
+    assert o["status"] == 0
+    assert abs(o["evals"][0] / w[0] - 1) < 1e-12
+    # lambda1 is one of a complex-conjugate pair (modulated phase)
+    assert abs(w[1].imag) > 1
+    assert min(abs(o["evals"][1] - w[1]), abs(o["evals"][1] - w[2])) / abs(w[1]) < 1e-9
+    # left eigenvalue of T^T equals lambda0 (residual tol 1e-10)
+    assert abs(o["left"] / w[0].real - 1) < 1e-10
+
+
+def test_mixed_precision_refinement(driver, oracle, tmp_path):
+    # operator results rounded to FP32 in phase 1; FP64 Rayleigh-Ritz after
+    D = oracle.OracleOp(oracle.XAM2D, 12, 0, 2, 0.4, 0.5, 0.0, 0.0).dense()
+    D = 0.5 * (D + D.T)
+    w = np.sort(np.linalg.eigvalsh(D))[::-1]
+    o = run(driver, D, True, 4, prec=1, tmp=str(tmp_path))
+    assert o["mv32"] > 0 and o["mv64"] > 0
+    got = np.array([e.real for e in o["evals"]])
+    assert abs(got[0] / w[0] - 1) <= 1e-12
+    assert np.all(np.abs(got / w[:4] - 1) <= 1e-11)
+
+
+def test_small_matrix_clamps_ncv(driver, tmp_path):
+    # ncv >= cols() is clamped to cols()-1 (TMat/pfunc.hpp:90)
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((6, 6))
+    A = A @ A.T
+    o = run(driver, A, True, 2, tmp=str(tmp_path))
+    w = np.sort(np.linalg.eigvalsh(A))[::-1]
+    got = np.array([e.real for e in o["evals"]])
+    assert np.all(np.abs(got / w[:2] - 1) < 1e-12)

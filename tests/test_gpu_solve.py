@@ -1,0 +1,86 @@
+"""GPU eigensolves against dense LAPACK, the Kaufman closed form and the
+full-vs-reduced identity.  Eigenvalue bar: 1e-12 relative."""
+import numpy as np
+import pytest
+
+from tests import physics
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lt():
+    import latttmat_amd as lt
+    return lt
+
+
+def test_xat_l12_dense(lt):
+    # config 1: A, axialdir 1, L=12, betaJ=0.5, kappa1=0.6, h=0, neigs=1
+    op = lt.TransferMatrix("A", 2, 1, 12, J=0.5, kappa=0.6)
+    r = lt.solve(op, nev=1, want_vectors=True)
+    print("matvecs", r.matvecs_f64, "restarts", r.restarts, flush=True)
+    T = physics.xat_dense(12, 0.5, 0.6, 0.0) if False else None  # dense 4096^2 built below
+    from oracle import oracle as O
+    oop = O.OracleOp(O.XAT, 12, 0, 0, 0.5, 0.6, 0.0, 0.0)
+    D = oop.dense()
+    w = np.linalg.eigvalsh(0.5 * (D + D.T))
+    lam0 = w[-1]
+    assert abs(r.evals[0] / lam0 - 1) < 1e-12
+    v = r.vecs[:, 0]
+    res = np.linalg.norm(D @ v - r.evals[0] * v) / abs(r.evals[0])
+    assert res < 1e-9
+
+
+@pytest.mark.parametrize("Lw,K", [(8, 0.5), (12, 0.3), (14, 0.44)])
+def test_kaufman(lt, Lw, K):
+    op = lt.TransferMatrix("A", 2, 1, Lw, J=K, kappa=0.0)
+    r = lt.solve(op, nev=2)
+    l0, l1 = physics.kaufman(Lw, K)
+    nf = op.getnormfactor()
+    got = np.sort(np.abs(r.evals))[::-1] / nf
+    print(Lw, K, got, l0, l1, flush=True)
+    assert abs(got[0] / l0 - 1) < 1e-12
+    assert abs(got[1] / l1 - 1) < 1e-11
+
+
+def test_reduced_equals_full(lt):
+    full = lt.TransferMatrix("A", 2, 1, 16, J=0.4, kappa=0.5)
+    red = lt.TransferMatrix("A", 2, 5, 16, J=0.4, kappa=0.5, neigs=1)
+    rf = lt.solve(full, nev=1)
+    rr = lt.solve(red, nev=1)
+    assert abs(rr.evals[0] / rf.evals[0] - 1) < 1e-12
+
+
+def test_zbt_nonsym_dense(lt):
+    op = lt.TransferMatrix("B", 2, 3, 10, J=0.5, kappa=0.5, kappa2=0.3)
+    r = lt.solve(op, nev=2, want_left=True, want_vectors=True)
+    from oracle import oracle as O
+    D = O.OracleOp(O.ZBT, 10, 0, 0, 0.5, 0.5, 0.3, 0.0).dense()
+    w = np.linalg.eigvals(D)
+    w = w[np.argsort(-np.abs(w))]
+    print("gpu", r.evals, "dense", w[:3], flush=True)
+    assert abs(r.evals[0] / w[0] - 1) < 1e-12
+    assert abs(abs(r.evals[1]) / abs(w[1]) - 1) < 1e-10
+    assert abs(r.eval_left / w[0].real - 1) < 1e-12
+    # left vector is the right Perron vector of T^T
+    lvec = r.vecl
+    assert np.linalg.norm(D.T @ lvec - r.eval_left * lvec) / abs(r.eval_left) < 1e-8
+
+
+def test_mixed_precision_reduced(lt):
+    op = lt.TransferMatrix("A", 2, 5, 20, J=0.4, kappa=0.5, neigs=4)
+    r64 = lt.solve(op, nev=4, precision=lt.TM_PREC_F64)
+    rmx = lt.solve(op, nev=4, precision=lt.TM_PREC_MIXED)
+    print("f64", r64.evals, r64.matvecs_f64, "mixed", rmx.evals, rmx.matvecs_f32,
+          rmx.matvecs_f64, flush=True)
+    rel = np.abs(rmx.evals - r64.evals) / np.abs(r64.evals)
+    assert rel[0] <= 1e-12
+    assert np.all(rel <= 1e-11)
+
+
+def test_xdt_dense(lt):
+    op = lt.TransferMatrix("D", 2, 1, 10, J=0.6, kappa=0.7, h=0.1)
+    r = lt.solve(op, nev=1)
+    T = physics.xdt_dense(10, 0.6, 0.7, 0.1) * op.getnormfactor()
+    lam0 = np.linalg.eigvalsh(T)[-1]
+    assert abs(r.evals[0] / lam0 - 1) < 1e-12
