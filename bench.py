@@ -1,0 +1,195 @@
+"""Benchmark: eigensolves/s of the symmetry-reduced 2D ANNNI transfer matrix.
+
+Workload (the configuration the headline metric is quoted on): ANNNI,
+perpendicular TM reduced by translations + reflections (axialdir=5), width
+L=28 (2^28 layer states folded into 4,806,078 orbits; spin-flip folding is off
+because neigs>1), betaJ=0.4, kappa1=0.5, h=0, neigs=4.  One "step" is one
+complete eigensolve to the leading 4 eigenvalues with the chosen method
+(FP32 T.x, FP64 Krylov basis/orthogonalisation, FP64 Rayleigh-Ritz
+refinement); the plain-FP64 eigensolve is timed beside it and the chosen
+method's lambda0 must agree with it to 1e-12 relative for the line to be
+valid.  Inputs (orbit tables, diagonals) are resident in HBM before timing.
+
+Also measured: the T.x operator alone (mean of timed device applies, HIP
+events on the operator's stream) against the HBM roofline, and the CPU
+oracle (a restatement of the reference algorithm) on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--L", type=int, default=28)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tx-reps", type=int, default=20)
+    ap.add_argument("--verbose", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of this "
+              "path is design-only in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import latttmat_amd as lt
+
+    L, bJ, kappa, neigs = args.L, 0.4, 0.5, 4
+    t0 = time.perf_counter()
+    op = lt.TransferMatrix("A", 2, 5, L, J=bJ, kappa=kappa, h=0.0, neigs=neigs)
+    setup_s = time.perf_counter() - t0
+    M = op.cols()
+    log(f"[bench] operator XAM L={L} M={M} setup {setup_s:.2f}s")
+
+    def solve(prec):
+        return lt.solve(op, nev=neigs, precision=prec, seed=1)
+
+    # warmup (also the reference values for the accuracy gate)
+    ref64 = solve(lt.TM_PREC_F64)
+    for _ in range(args.warmup):
+        solve(lt.TM_PREC_MIXED)
+    log(f"[bench] warmup done; fp64 lambda0 {ref64.evals[0]!r} matvecs {ref64.matvecs_f64}")
+    if args.verbose:
+        lt.solve(op, nev=neigs, precision=lt.TM_PREC_MIXED, seed=1, verbose=True)
+
+    # timed: chosen method
+    ts = time.perf_counter()
+    res = [solve(lt.TM_PREC_MIXED) for _ in range(args.steps)]
+    t_mixed = time.perf_counter() - ts
+    log(f"[bench] mixed: {args.steps} solves in {t_mixed:.3f}s")
+    # timed: plain FP64
+    ts = time.perf_counter()
+    res64 = [solve(lt.TM_PREC_F64) for _ in range(args.steps)]
+    t_f64 = time.perf_counter() - ts
+    log(f"[bench] fp64: {args.steps} solves in {t_f64:.3f}s")
+
+    r, r64 = res[-1], res64[-1]
+    lam = np.sort(np.abs(r.evals))[::-1]
+    lam64 = np.sort(np.abs(r64.evals))[::-1]
+    err_l0 = abs(lam[0] - lam64[0]) / lam64[0]
+    err_l1 = abs(lam[1] - lam64[1]) / lam64[1]
+    xi = 1.0 / math.log(lam[0] / lam[1])
+    xi64 = 1.0 / math.log(lam64[0] / lam64[1])
+    valid = bool(err_l0 <= 1e-12)
+
+    # T.x alone
+    ms32 = op.time_apply(lt.TM_F64, lt.TM_F32, warmup=3, reps=args.tx_reps)
+    ms64 = op.time_apply(lt.TM_F64, lt.TM_F64, warmup=3, reps=args.tx_reps)
+    N = 1 << L
+    # algorithmic bytes per T.x: one read + write of the expanded space and
+    # the reduced vectors and tables (8-byte rep + 1-byte size)
+    alg32 = N * (4 + 4) + M * (4 + 4 + 9)
+    alg64 = N * (8 + 8) + M * (8 + 8 + 9)
+    gbs32 = alg32 / (ms32 * 1e-3) / 1e9
+    gbs64 = alg64 / (ms64 * 1e-3) / 1e9
+    log(f"[bench] T.x fp32 {ms32:.3f} ms ({gbs32:.0f} GB/s alg), fp64 {ms64:.3f} ms ({gbs64:.0f} GB/s alg)")
+
+    # FP32 operator error level on a random unit vector
+    xr = np.random.default_rng(3).uniform(0, 1, M)
+    xr /= np.linalg.norm(xr)
+    y64 = op.apply_prec(xr, lt.TM_F64)
+    y32 = op.apply_prec(xr, lt.TM_F32)
+    tx_err = float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64))
+    log(f"[bench] FP32 T.x relative error {tx_err:.2e}")
+
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "xam_l28_traffic.json")
+    if os.path.exists(tp):
+        try:
+            traffic = json.load(open(tp)).get("fp32_bytes_per_tx")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(L, bJ, kappa, r64.matvecs_f64)
+
+    value = args.steps / t_mixed
+    out = {
+        "metric": "eigensolves/s at |dlambda0|/lambda0<=1e-12, chosen vs plain FP64; T.x achieved HBM GB/s",
+        "value": round(value, 4),
+        "unit": "eigensolves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_mixed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 T.x + fp64 Krylov/refinement",
+        "data": "synthetic (model parameters; seeded uniform(0,1) start vector)",
+        "config": {
+            "workload": "2D ANNNI perpendicular-TM, symmetry-reduced (axialdir=5), L=28, "
+                        "betaJ=0.4, kappa1=0.5, h=0, neigs=4",
+            "states": N, "orbits": M, "ncv": 11, "tol": 1e-10,
+        },
+        "valid": valid,
+        "plain_fp64": {
+            "value": round(args.steps / t_f64, 4), "ms_per_step": round(1e3 * t_f64 / args.steps, 3),
+            "matvecs": int(r64.matvecs_f64),
+        },
+        "speedup_vs_fp64": round((args.steps / t_mixed) / (args.steps / t_f64), 3),
+        "chosen_matvecs": {"fp32": int(r.matvecs_f32), "fp64": int(r.matvecs_f64)},
+        "accuracy_vs_fp64": {"lambda0": err_l0, "lambda1": err_l1,
+                             "xi": abs(xi - xi64) / xi64, "xi_value": xi},
+        "tx": {"fp32_ms": round(ms32, 4), "fp64_ms": round(ms64, 4), "fp32_rel_err": tx_err,
+               "fp32_alg_GBs": round(gbs32, 1), "fp64_alg_GBs": round(gbs64, 1)},
+        "roofline": {
+            "bound": "hbm", "kernel": "reduced T.x (xtilde + pass1 + pass2), FP32 compute",
+            "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+        "setup_s": round(setup_s, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(L, bJ, kappa, nmatvec):
+    """Time the CPU oracle (restatement of XAMMatVecProd2D::perform_op,
+    TMat/tmatreduce.cpp:406-529) on the host cores: orbit enumeration once,
+    then one T.x at the benchmark size; eigensolves/s is estimated as
+    1 / (t_Tx * number of T.x the plain FP64 eigensolve needed)."""
+    import numpy as np
+
+    from oracle import oracle as O
+    if not os.path.exists(O._LIB_PATH):
+        O.build()
+    t0 = time.perf_counter()
+    oop = O.OracleOp(O.XAM2D, L, 0, 2, bJ, kappa, 0.0, 0.0)
+    t_setup = time.perf_counter() - t0
+    x = np.random.default_rng(1).uniform(0, 1, oop.n)
+    t0 = time.perf_counter()
+    oop.apply(x)
+    t_tx = time.perf_counter() - t0
+    log(f"[bench] cpu oracle: setup {t_setup:.1f}s, one T.x {t_tx:.2f}s on {O.num_threads()} threads")
+    return {
+        "value": round(1.0 / (t_tx * nmatvec), 6),
+        "unit": "eigensolves/s",
+        "cores": int(O.num_threads()),
+        "kind": "port",
+        "sample": f"one oracle T.x at L={L} ({t_tx:.2f}s) x {nmatvec} T.x per FP64 eigensolve; "
+                  f"orbit setup {t_setup:.1f}s excluded",
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -140,8 +140,14 @@ typedef struct {
   uint64_t     seed;
   int          want_left;   /* also solve T^T for the left vector (pfunc.hpp:47-67) */
   int          want_vectors;/* copy right (and left) vectors to the host          */
-  double       mixed_tol;   /* residual target of the FP32 phase (default 1e-6)  */
+  double       mixed_tol;   /* residual target of the FP32 phase (default 2e-7;
+                               it also stops at the FP32 noise floor)     */
   int          verbose;
+  int          refine;      /* mixed precision, FP64 phase: 1 = block
+                               Rayleigh-Ritz on the FP32 Ritz vectors, accepted
+                               when r^2/(gap |theta|) <= 1e-13 (symmetric only);
+                               2 = Krylov-Schur warm-started from the FP32 Ritz
+                               vectors, run to tol; 0 = 1 if symmetric else 2 */
 } tm_solve_opts;
 
 /* Results.  Eigenvalues are those of the operator as applied (the reference

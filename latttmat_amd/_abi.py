@@ -46,7 +46,7 @@ class SolveOpts(C.Structure):
         ("nev", C.c_int), ("ncv", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
         ("maxtrial", C.c_int), ("precision", C.c_int), ("seed", C.c_uint64),
         ("want_left", C.c_int), ("want_vectors", C.c_int), ("mixed_tol", C.c_double),
-        ("verbose", C.c_int),
+        ("verbose", C.c_int), ("refine", C.c_int),
     ]
 
 
@@ -259,14 +259,14 @@ class SolveResult:
 
 
 def solve(op: TransferMatrix, nev=1, ncv=0, tol=1e-10, maxit=1000, precision=TM_PREC_F64,
-          seed=1, want_left=False, want_vectors=False, mixed_tol=1e-6, verbose=False,
-          maxtrial=3, raise_on_noconv=True) -> SolveResult:
+          seed=1, want_left=False, want_vectors=False, mixed_tol=2e-7, verbose=False,
+          maxtrial=3, raise_on_noconv=True, refine=0) -> SolveResult:
     o = SolveOpts()
     lib().tm_solve_opts_default(C.byref(o))
     o.nev, o.ncv, o.tol, o.maxit, o.maxtrial = nev, ncv, tol, maxit, maxtrial
     o.precision, o.seed = precision, seed
     o.want_left, o.want_vectors = int(want_left), int(want_vectors)
-    o.mixed_tol, o.verbose = mixed_tol, int(verbose)
+    o.mixed_tol, o.verbose, o.refine = mixed_tol, int(verbose), int(refine)
     n = op.cols()
     er, ei, rs = np.zeros(nev), np.zeros(nev), np.zeros(nev)
     sym = op.is_symmetric()

@@ -192,8 +192,9 @@ void tm_solve_opts_default(tm_solve_opts *o)
   o->seed = 1;
   o->want_left = 0;
   o->want_vectors = 0;
-  o->mixed_tol = 1e-6;
+  o->mixed_tol = 2e-7;
   o->verbose = 0;
+  o->refine = 0;
 }
 
 tm_status tm_solve(tm_op *op, const tm_solve_opts *opts, tm_solve_out *out)

@@ -112,6 +112,12 @@ __device__ __forceinline__ C diag_value(const DiagSpec &g, const C *tab, uint64_
   return tab[i.i1] * tab[64 + i.i2] * tab[128 + i.i0];
 }
 
+// LDS swizzle for 8-byte elements: XOR the low 5 index bits with the next 5.
+// A thread that owns 16 consecutive elements (or 16 elements strided by 16)
+// then touches distinct banks across its wavefront.  Involution, identity
+// for index < 32.
+__device__ __forceinline__ int sw(int e) { return e ^ ((e >> 5) & 31); }
+
 // One radix-2^R Kronecker stage group on R register values: v[i] indexed by
 // R bits, K = [[a, b], [b, a]] applied on each of the R bits.
 template <int R, typename C>
@@ -133,7 +139,7 @@ __device__ __forceinline__ void kron_regs(C *v, C a, C b)
 
 // Full Kronecker transform of an LDS array of 2^nbits elements (bits
 // [bit0, nbits) still to do), using every thread of the block.  Element
-// index p maps to lds[p].  Ends with a block barrier.
+// index p is stored at lds[sw(p)].  Ends with a block barrier.
 template <int R, typename C>
 __device__ __forceinline__ void lds_kron_group(C *lds, int nbits, int bg, C a, C b)
 {
@@ -144,10 +150,10 @@ __device__ __forceinline__ void lds_kron_group(C *lds, int nbits, int bg, C a, C
     const int base = (hi << (bg + R)) | lo;
     C v[1 << R];
 #pragma unroll
-    for (int i = 0; i < (1 << R); ++i) v[i] = lds[base | (i << bg)];
+    for (int i = 0; i < (1 << R); ++i) v[i] = lds[sw(base | (i << bg))];
     kron_regs<R>(v, a, b);
 #pragma unroll
-    for (int i = 0; i < (1 << R); ++i) lds[base | (i << bg)] = v[i];
+    for (int i = 0; i < (1 << R); ++i) lds[sw(base | (i << bg))] = v[i];
   }
 }
 

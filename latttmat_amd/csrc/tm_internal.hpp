@@ -156,19 +156,23 @@ struct GroupSpec {
 // sizes), returns M.  If map != nullptr it receives map[s] = orbit index of s.
 int64_t enumerate_orbits(const GroupSpec &g, DevBuf &reps, DevBuf &dups, DevBuf *map,
                          hipStream_t s);
-// Fused-path tables (2D periodic, L even).
+// Fused-path tables (2D periodic, L = 2n even).  A state s is split into
+// its even bits E and odd bits O (n bits each); a rotation of s by two sites
+// rotates E and O together, so the transform over O only has to be done for
+// columns E that are necklace representatives (minimal rotations).
 struct FusedTables {
   int n;                 // L/2
-  int64_t Q;             // stored columns (a <= rev a)
-  int nS;                // stored rows (B_R u rev B_R)
-  int nB;                // |B_R|
-  DevBuf cols;           // uint32[Q]   column value a of stored column q
-  DevBuf cmap;           // uint32[Q << n] orbit index of state (cols[q] << n | b')
-  DevBuf srows;          // uint16[nS]  row values b (ascending)
-  DevBuf rowidx;         // int32[2^n]  index of b in srows or -1
-  DevBuf brows;          // uint16[nB]  B_R values (ascending)
-  DevBuf rstart;         // int64[nB+1] rep range of each B_R row
-  DevBuf replo;          // uint16[M]   low n bits of each rep
+  int nNk;               // number of n-bit necklaces
+  int64_t ldA;           // leading dimension of the pass-1 output (nNk*16, padded)
+  DevBuf NR;             // uint16[nNk]   necklace representatives, ascending
+  DevBuf period;         // uint8[nNk]    rotation period of each representative
+  DevBuf nkw;            // uint16[2^n]   necklace index of every n-bit word
+  DevBuf rotw;           // uint8[2^n]    word = rotl(NR[nkw], rotw)
+  DevBuf slotO;          // uint16[nNk*16] word stored in (necklace, slot), 0xFFFF unused
+  DevBuf cmap;           // uint32[nNk << n] orbit of the state (E = NR[c], O = O')
+  DevBuf ostart;         // int64[nNk+1]  orbit range whose evaluation point has O = NR[r]
+  DevBuf olist;          // uint32[M]     orbit indices grouped by r
+  DevBuf Elist;          // uint16[M]     E part of each orbit's evaluation point
 };
 void build_fused_tables(const GroupSpec &g, const DevBuf &reps, int64_t M, FusedTables &ft,
                         hipStream_t s);
@@ -191,7 +195,7 @@ size_t reduced_work_bytes(const ReducedPlan &plan, tm_dtype comp);
 
 // ---- Krylov vector kernels (tm_krylov.hip) ----
 namespace kry {
-constexpr int NBLK = 256;   // partial-sum blocks of the reductions
+constexpr int NBLK = 1024;  // partial-sum blocks of the reductions
 // out[k] = <V_k, w> for k < j, out[j] = <w,w>; partial buffer of NBLK*(j+1)
 void multidot(const double *V, int64_t ldv, int j, const double *w, int64_t n,
               double *partial, double *out, hipStream_t s);

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(KRON_THREADS) void k_kron_pass(
     const uint64_t g = base | ((uint64_t)(e >> logW) << b0) | (uint64_t)(e & (W - 1));
     C v = (C)in[g] * sin;
     if (flags & F_PRO) v *= dev::diag_value<C>(dg, stab, g);
-    lds[e] = v;
+    lds[dev::sw(e)] = v;
   }
   __syncthreads();
 
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(KRON_THREADS) void k_kron_pass(
 
   for (int e = threadIdx.x; e < T; e += blockDim.x) {
     const uint64_t g = base | ((uint64_t)(e >> logW) << b0) | (uint64_t)(e & (W - 1));
-    C v = lds[e] * sout;
+    C v = lds[dev::sw(e)] * sout;
     if (flags & F_EPI) v *= dev::diag_value<C>(dg, stab, g);
     out[g] = (TO)v;
   }

@@ -23,14 +23,19 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
-// Block-reduce `cnt` values acc[0..cnt) and write to out[0..cnt).
-__device__ __forceinline__ void block_reduce_write(double *acc, int cnt, double *out)
+// Block-reduce the first `cnt` of the J values acc[] (compile-time indexed,
+// so the array stays in registers) and write them to out[0..cnt).
+template <int J>
+__device__ __forceinline__ void block_reduce_write(const double (&acc)[J], int cnt, double *out)
 {
-  __shared__ double red[THREADS / 64][MAXJ + 1];
+  __shared__ double red[THREADS / 64][J];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int k = 0; k < cnt; ++k) {
-    const double v = wave_sum(acc[k]);
-    if (lane == 0) red[wid][k] = v;
+#pragma unroll
+  for (int k = 0; k < J; ++k) {
+    if (k < cnt) {
+      const double v = wave_sum(acc[k]);
+      if (lane == 0) red[wid][k] = v;
+    }
   }
   __syncthreads();
   for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
@@ -40,7 +45,7 @@ __device__ __forceinline__ void block_reduce_write(double *acc, int cnt, double 
   }
 }
 
-// acc registers: one per dot product, limited to MAXJ+1
+// out slots: <V_k, w> for k < j, then <w, w> in slot j
 template <int J>
 __global__ __launch_bounds__(THREADS) void k_multidot(const double *__restrict__ V, int64_t ldv,
                                                       int j, const double *__restrict__ w,
@@ -57,14 +62,14 @@ __global__ __launch_bounds__(THREADS) void k_multidot(const double *__restrict__
       if (k < j) acc[k] += V[k * ldv + i] * wi;
     acc[J] += wi * wi;
   }
-  // move <w,w> to slot j
-  double tmp[J + 1];
+  // move <w,w> into slot j
 #pragma unroll
-  for (int k = 0; k <= J; ++k) tmp[k] = (k < j) ? acc[k] : 0.0;
-  tmp[j < J ? j : J] = acc[J];
-  block_reduce_write(tmp, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
+  for (int k = 0; k < J; ++k)
+    if (k == j) acc[k] = acc[J];
+  block_reduce_write<J + 1>(acc, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
 }
 
+// w -= sum_k h[k] V_k; then (dots=1) slots <V_k, w>, <w,w> or (dots=0) slot 0 = <w,w>
 template <int J>
 __global__ __launch_bounds__(THREADS) void k_update_dots(const double *__restrict__ V,
                                                          int64_t ldv, int j,
@@ -72,9 +77,9 @@ __global__ __launch_bounds__(THREADS) void k_update_dots(const double *__restric
                                                          const double *__restrict__ h,
                                                          double *__restrict__ partial, int dots)
 {
-  double hk[J];
-#pragma unroll
-  for (int k = 0; k < J; ++k) hk[k] = k < j ? h[k] : 0.0;
+  __shared__ double hk[J];
+  for (int k = threadIdx.x; k < J; k += blockDim.x) hk[k] = k < j ? h[k] : 0.0;
+  __syncthreads();
   double acc[J + 1];
 #pragma unroll
   for (int k = 0; k <= J; ++k) acc[k] = 0;
@@ -96,30 +101,28 @@ __global__ __launch_bounds__(THREADS) void k_update_dots(const double *__restric
     }
     acc[J] += wi * wi;
   }
-  double tmp[J + 1];
-  int cnt;
   if (dots) {
 #pragma unroll
-    for (int k = 0; k <= J; ++k) tmp[k] = (k < j) ? acc[k] : 0.0;
-    tmp[j < J ? j : J] = acc[J];
-    cnt = j + 1;
+    for (int k = 0; k < J; ++k)
+      if (k == j) acc[k] = acc[J];
+    block_reduce_write<J + 1>(acc, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
   } else {
-    tmp[0] = acc[J];
-#pragma unroll
-    for (int k = 1; k <= J; ++k) tmp[k] = 0;
-    cnt = 1;
+    acc[0] = acc[J];
+    block_reduce_write<J + 1>(acc, 1, partial + (int64_t)blockIdx.x);
   }
-  block_reduce_write(tmp, cnt, partial + (int64_t)blockIdx.x * cnt);
 }
 
-__global__ void k_reduce_partials(const double *__restrict__ partial, int nblk, int cnt,
-                                  double *__restrict__ out)
+// One wavefront per output slot: lane l adds partials l, l+64, ... in a fixed
+// order, then a fixed shuffle tree -> deterministic.
+__global__ __launch_bounds__(64) void k_reduce_partials(const double *__restrict__ partial,
+                                                        int nblk, int cnt,
+                                                        double *__restrict__ out)
 {
-  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-    double s = 0;
-    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * cnt + k];
-    out[k] = s;
-  }
+  const int k = blockIdx.x;
+  double s = 0;
+  for (int b = threadIdx.x; b < nblk; b += 64) s += partial[(int64_t)b * cnt + k];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) out[k] = s;
 }
 
 __global__ void k_normalize(double *__restrict__ w, int64_t n, const double *__restrict__ nrm2)
@@ -241,7 +244,7 @@ __global__ __launch_bounds__(THREADS) void k_gram_row(const double *__restrict__
     for (int c = 0; c < J; ++c)
       if (c < k) acc[c] += a * B[c * ld + i];
   }
-  block_reduce_write(acc, k, partial + (int64_t)blockIdx.x * k);
+  block_reduce_write<J>(acc, k, partial + (int64_t)blockIdx.x * k);
 }
 
 unsigned nblocks(int64_t n)
@@ -269,7 +272,7 @@ void multidot(const double *V, int64_t ldv, int j, const double *w, int64_t n, d
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
   const unsigned nb = nblocks(n);
   TM_JDISPATCH(k_multidot, j, dim3(nb), V, ldv, j, w, n, partial);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
   TM_LAUNCH_CHECK();
 }
 
@@ -279,7 +282,7 @@ void update_dots(const double *V, int64_t ldv, int j, double *w, int64_t n, cons
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
   const unsigned nb = nblocks(n);
   TM_JDISPATCH(k_update_dots, j, dim3(nb), V, ldv, j, w, n, h, partial, 1);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
   TM_LAUNCH_CHECK();
 }
 
@@ -359,7 +362,7 @@ void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double
   for (int row = 0; row < k; ++row) {
     TM_JDISPATCH(k_gram_row, k, dim3(nb), A, B, ld, k, n, row, partial);
     // G[row + c*k] = <A_row, B_c>
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, (int)nb, k,
+    hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(64), 0, s, partial, (int)nb, k,
                        G + row * k);
     TM_LAUNCH_CHECK();
   }

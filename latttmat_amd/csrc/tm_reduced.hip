@@ -14,14 +14,19 @@
 //   y_o = d_o Tvec_o [ (x)K  u ](rep_o),   u(s) = Tvec_{o(s)} x_{o(s)}.
 // Generic path: expand u over the full space with a state->orbit map, run
 // the Kronecker passes, gather at the representatives.
-// Fused path (2D periodic, L = 2n even): uses two symmetries of u,
-//   rotation by n:      u(b, a) = u(a, b)           (state = b<<n | a)
-//   reversal + rot n:   u(b, a) = u(rev b, rev a)
-// so that the transform over the high half of every state is done on
-// contiguous rows of the map, only rows b that occur as the high half of a
-// representative (or its reversal) are kept, and only columns a <= rev(a)
-// are computed.  Pass 2 transforms the kept rows over the low half and
-// writes y at the representatives.  The expanded 2^L vector never exists.
+// Fused path (2D periodic, L = 2n even).  Split every state into its even
+// bits E and odd bits O, s = interleave(E, O).  The full transform is
+// (x)K over E times (x)K over O, and a rotation of s by two sites rotates E
+// and O together, so u(E, O) = u(rotl E, rotl O) and therefore
+//   A(E, O) = sum_O' K(O, O') u(E, O')  satisfies  A(rotl E, rotl O) = A(E, O).
+// Pass 1 computes A only for columns E that are necklace representatives
+// (1,182 of 16,384 at L=28), gathering u from a column-major state->orbit map,
+// and stores each column grouped by row necklace (the <= n rotations of a
+// row share one 64-byte group).  Pass 2 rebuilds, for every row necklace
+// representative O_r, the full E-column A(., O_r) from those groups,
+// transforms over E and writes y at one point of every orbit (each orbit has
+// an image whose O part is a necklace representative).  The expanded 2^L
+// vector never exists.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -233,14 +238,26 @@ __global__ void k_build_map(GroupDev g, uint64_t nstate, const uint64_t *bitmap,
     map[s] = rank_of(canon(s, g, nullptr), bitmap, wpre);
 }
 
-__global__ void k_build_cmap(GroupDev g, int n, const uint32_t *cols, uint64_t Q,
+// spread the low 16 bits of x to the even bit positions
+__device__ __forceinline__ uint64_t spread16(uint32_t x)
+{
+  uint64_t v = x & 0xFFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFull;
+  v = (v | (v << 4)) & 0x0F0F0F0Full;
+  v = (v | (v << 2)) & 0x33333333ull;
+  v = (v | (v << 1)) & 0x55555555ull;
+  return v;
+}
+
+__global__ void k_build_cmap(GroupDev g, int n, const uint16_t *NR, uint64_t nNk,
                              const uint64_t *bitmap, const uint64_t *wpre, uint32_t *cmap)
 {
-  const uint64_t total = Q << n;
+  const uint64_t total = nNk << n;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t q = i >> n, b = i & ((1ull << n) - 1);
-    const uint64_t s = ((uint64_t)cols[q] << n) | b;
+    const uint64_t c = i >> n;
+    const uint32_t O = (uint32_t)(i & ((1ull << n) - 1));
+    const uint64_t s = spread16(NR[c]) | (spread16(O) << 1);
     cmap[i] = rank_of(canon(s, g, nullptr), bitmap, wpre);
   }
 }
@@ -349,64 +366,93 @@ int64_t enumerate_orbits(const GroupSpec &gs, DevBuf &reps, DevBuf &dups, DevBuf
 void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, FusedTables &ft,
                         hipStream_t s)
 {
-  TM_CHECK(gs.dim == 2 && gs.bc == 0 && (gs.L % 2) == 0 && gs.L <= 32, TM_EINVAL,
-           "fused reduced path needs a periodic 2D layer of even width");
+  TM_CHECK(gs.dim == 2 && gs.bc == 0 && (gs.L % 2) == 0 && gs.L >= 8 && gs.L <= 30, TM_EINVAL,
+           "fused reduced path needs a periodic 2D layer of even width 8..30");
   const int n = gs.L / 2;
-  const uint32_t nh = 1u << n;
+  const uint32_t nw = 1u << n, wmask = nw - 1;
   ft.n = n;
-  // stored columns: a <= rev_n(a)
-  auto rev = [n](uint32_t a) {
-    uint32_t r = 0;
-    for (int i = 0; i < n; ++i) r |= ((a >> i) & 1u) << (n - 1 - i);
-    return r;
+  auto rotl = [&](uint32_t w, int d) -> uint32_t {
+    d %= n;
+    return d ? (((w << d) & wmask) | (w >> (n - d))) : w;
   };
-  std::vector<uint32_t> cols;
-  for (uint32_t a = 0; a < nh; ++a)
-    if (a <= rev(a)) cols.push_back(a);
-  ft.Q = (int64_t)cols.size();
-  // representatives on the host: high halves, rep ranges, low halves
-  std::vector<uint64_t> hreps(M);
-  TM_HIP(hipMemcpy(hreps.data(), reps.get(), M * 8, hipMemcpyDeviceToHost));
-  std::vector<uint16_t> brows, replo(M);
-  std::vector<int64_t> rstart;
-  for (int64_t o = 0; o < M; ++o) {
-    const uint32_t b = (uint32_t)(hreps[o] >> n);
-    replo[o] = (uint16_t)(hreps[o] & (nh - 1));
-    if (brows.empty() || brows.back() != b) {
-      brows.push_back((uint16_t)b);
-      rstart.push_back(o);
+  // necklaces of n-bit words
+  std::vector<uint16_t> nk(nw), NR;
+  std::vector<uint8_t> rot(nw), per;
+  std::vector<int32_t> repidx(nw, -1);
+  for (uint32_t w = 0; w < nw; ++w) {
+    uint32_t m = w;
+    for (int j = 1; j < n; ++j) m = std::min(m, rotl(w, j));
+    if (m == w) {
+      repidx[w] = (int32_t)NR.size();
+      NR.push_back((uint16_t)w);
+      int p = 1;
+      while (rotl(w, p) != w) ++p;
+      per.push_back((uint8_t)p);
     }
   }
-  rstart.push_back(M);
-  ft.nB = (int)brows.size();
-  std::vector<char> need(nh, 0);
-  for (uint16_t b : brows) { need[b] = 1; need[rev(b)] = 1; }
-  std::vector<uint16_t> srows;
-  std::vector<int32_t> rowidx(nh, -1);
-  for (uint32_t b = 0; b < nh; ++b)
-    if (need[b]) { rowidx[b] = (int32_t)srows.size(); srows.push_back((uint16_t)b); }
-  ft.nS = (int)srows.size();
-
+  for (uint32_t w = 0; w < nw; ++w) {
+    uint32_t m = w;
+    for (int j = 1; j < n; ++j) m = std::min(m, rotl(w, j));
+    const int r = repidx[m];
+    nk[w] = (uint16_t)r;
+    int k = 0;
+    while (rotl(m, k) != w) ++k;
+    rot[w] = (uint8_t)k;
+  }
+  ft.nNk = (int)NR.size();
+  ft.ldA = (((int64_t)ft.nNk * 16 + 63) / 64) * 64;
+  std::vector<uint16_t> slot((size_t)ft.nNk * 16, 0xFFFF);
+  for (int r = 0; r < ft.nNk; ++r)
+    for (int j = 0; j < per[r]; ++j) slot[(size_t)r * 16 + j] = (uint16_t)rotl(NR[r], j);
+  // evaluation point of every orbit: rotate the representative so that its
+  // odd-bit part becomes a necklace representative
+  std::vector<uint64_t> hreps(M);
+  TM_HIP(hipMemcpy(hreps.data(), reps.get(), M * 8, hipMemcpyDeviceToHost));
+  auto compact = [&](uint64_t v) {  // bits 0,2,4,... -> 0,1,2,...
+    uint32_t r = 0;
+    for (int i = 0; i < n; ++i) r |= (uint32_t)((v >> (2 * i)) & 1) << i;
+    return r;
+  };
+  std::vector<int64_t> cnt(ft.nNk + 1, 0);
+  std::vector<uint16_t> pr(M), pe(M);
+  for (int64_t o = 0; o < M; ++o) {
+    const uint32_t E = compact(hreps[o]), O = compact(hreps[o] >> 1);
+    const int r = nk[O], k = rot[O];
+    pr[o] = (uint16_t)r;
+    pe[o] = (uint16_t)rotl(E, n - k);  // rotate E right by k
+    ++cnt[r + 1];
+  }
+  for (int r = 0; r < ft.nNk; ++r) cnt[r + 1] += cnt[r];
+  std::vector<uint32_t> olist(M);
+  std::vector<uint16_t> elist(M);
+  std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+  for (int64_t o = 0; o < M; ++o) {
+    const int64_t i = pos[pr[o]]++;
+    olist[i] = (uint32_t)o;
+    elist[i] = pe[o];
+  }
   auto up = [&](DevBuf &d, const void *src, size_t bytes) {
     d.alloc(bytes);
     TM_HIP(hipMemcpy(d.get(), src, bytes, hipMemcpyHostToDevice));
   };
-  up(ft.cols, cols.data(), cols.size() * 4);
-  up(ft.srows, srows.data(), srows.size() * 2);
-  up(ft.rowidx, rowidx.data(), rowidx.size() * 4);
-  up(ft.brows, brows.data(), brows.size() * 2);
-  up(ft.rstart, rstart.data(), rstart.size() * 8);
-  up(ft.replo, replo.data(), replo.size() * 2);
+  up(ft.NR, NR.data(), NR.size() * 2);
+  up(ft.period, per.data(), per.size());
+  up(ft.nkw, nk.data(), nk.size() * 2);
+  up(ft.rotw, rot.data(), rot.size());
+  up(ft.slotO, slot.data(), slot.size() * 2);
+  up(ft.ostart, cnt.data(), cnt.size() * 8);
+  up(ft.olist, olist.data(), olist.size() * 4);
+  up(ft.Elist, elist.data(), elist.size() * 2);
 
-  // compact map over the stored columns
+  // column-major state->orbit map over the necklace columns
   const GroupDev g = to_dev(gs);
   OrbitScratch sc;
   const uint64_t M2 = orbit_bitmap(g, sc, s);
   TM_CHECK((int64_t)M2 == M, TM_EINVAL, "orbit count mismatch while building fused tables");
-  const uint64_t total = (uint64_t)ft.Q << n;
+  const uint64_t total = (uint64_t)ft.nNk << n;
   ft.cmap.alloc(total * 4);
   hipLaunchKernelGGL(k_build_cmap, dim3(grid_for(total, 256)), dim3(256), 0, s, g, n,
-                     ft.cols.as<uint32_t>(), (uint64_t)ft.Q, sc.bitmap.as<uint64_t>(),
+                     ft.NR.as<uint16_t>(), (uint64_t)ft.nNk, sc.bitmap.as<uint64_t>(),
                      sc.wpre.as<uint64_t>(), ft.cmap.as<uint32_t>());
   TM_LAUNCH_CHECK();
   TM_HIP(hipStreamSynchronize(s));
@@ -456,29 +502,27 @@ __global__ __launch_bounds__(1024) void k_kron_small(C *u, int L, C ka, C kb)
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
   const int n = 1 << L;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = u[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[dev::sw(i)] = u[i];
   __syncthreads();
   dev::lds_kron(lds, L, 0, ka, kb);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) u[i] = lds[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) u[i] = lds[dev::sw(i)];
 }
 
 constexpr int FUSED_THREADS = 1024;
 
-// Pass 1: one stored column a = cols[q]; u(b', a) = xt[cmap[q][b']] for all
-// b'; transform over b'; keep rows srows[*] in Ap[row * ldA + q].
+// Pass 1: one necklace column E = NR[c]; u(E, O') = xt[cmap[c][O']] for all
+// O'; transform over O; store A(E, O) grouped as [row necklace][slot].
 template <typename C>
 __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
     const uint32_t *__restrict__ cmap, const C *__restrict__ xt, C *__restrict__ Ap,
-    const uint16_t *__restrict__ srows, int nS, int n, int64_t Q, int64_t ldA, C ka, C kb)
+    const uint16_t *__restrict__ slotO, int n, int nNk, int64_t ldA, C ka, C kb)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
-  const int64_t q = dev::xcd_remap(blockIdx.x, Q);
-  if (q < 0) return;
+  const int c = blockIdx.x;
   const int nh = 1 << n;
-  const uint32_t *mrow = cmap + ((uint64_t)q << n);
-  // 16 consecutive positions per thread: gather, then the first 4 stages in
-  // registers
+  const uint32_t *mrow = cmap + ((uint64_t)c << n);
+  // 16 consecutive O' per thread: gather, then the first 4 stages in registers
   for (int grp = threadIdx.x; grp < (nh >> 4); grp += blockDim.x) {
     const uint4 *m4 = reinterpret_cast<const uint4 *>(mrow + (grp << 4));
     uint4 mm[4];
@@ -494,42 +538,61 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
     }
     dev::kron_regs<4>(v, ka, kb);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) lds[(grp << 4) + i] = v[i];
+    for (int i = 0; i < 16; ++i) lds[dev::sw((grp << 4) + i)] = v[i];
   }
   __syncthreads();
   dev::lds_kron(lds, n, 4, ka, kb);
-  for (int i = threadIdx.x; i < nS; i += blockDim.x) Ap[(int64_t)i * ldA + q] = lds[srows[i]];
+  C *out = Ap + (int64_t)c * ldA;
+  const int ns = nNk * 16;
+  for (int t = threadIdx.x; t < ns; t += blockDim.x) {
+    const uint16_t O = slotO[t];
+    out[t] = O != 0xFFFF ? lds[dev::sw(O)] : (C)0;
+  }
 }
 
-// Pass 2: one kept row b = brows[bi]; rebuild A'(b, a) for all a from the
-// stored rows b and rev(b); transform over a; write the representatives.
+// Pass 2: one row necklace O_r = NR[r]; rebuild A(E, O_r) for every E from
+// A(E, O_r) = A(NR[nk E], rotr_k O_r), E = rotl_k NR[nk E]; transform over E;
+// write y at the orbits whose evaluation point lies on this row.
 template <typename C, typename TO>
 __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
-    const C *__restrict__ Ap, const uint32_t *__restrict__ cols, int64_t Q,
-    const uint16_t *__restrict__ brows, const int32_t *__restrict__ rowidx,
-    const int64_t *__restrict__ rstart, const uint16_t *__restrict__ replo,
-    const C *__restrict__ dt, TO *__restrict__ y, int n, int64_t ldA, C ka, C kb,
-    const double *__restrict__ scale)
+    const C *__restrict__ Ap, const uint16_t *__restrict__ NR, const uint8_t *__restrict__ period,
+    int nNk, const int64_t *__restrict__ ostart, const uint32_t *__restrict__ olist,
+    const uint16_t *__restrict__ Elist, const C *__restrict__ dt, TO *__restrict__ y, int n,
+    int64_t ldA, C ka, C kb, const double *__restrict__ scale)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
-  const int bi = blockIdx.x;
-  const uint32_t b = brows[bi];
-  const uint32_t rb = dev::brev32(b, n);
-  const C *rowb = Ap + (int64_t)rowidx[b] * ldA;
-  const C *rowrb = Ap + (int64_t)rowidx[rb] * ldA;
-  for (int64_t q = threadIdx.x; q < Q; q += blockDim.x) {
-    const uint32_t a = cols[q];
-    const uint32_t ra = dev::brev32(a, n);
-    lds[a] = rowb[q];
-    if (ra != a) lds[ra] = rowrb[q];
+  const int r = blockIdx.x;
+  const int pr = period[r];
+  const uint32_t wmask = (1u << n) - 1;
+  // one necklace column per thread: a 16-slot group holds A(NR[c], rotations
+  // of O_r); every rotation E = rotl_k NR[c] reads slot (n - k) mod p_r
+  const C *grp0 = Ap + (int64_t)r * 16;
+  for (int c = threadIdx.x; c < nNk; c += blockDim.x) {
+    const C *g = grp0 + (int64_t)c * ldA;
+    C v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = g[j];
+    const uint32_t w = NR[c];
+    const int pc = period[c];
+    for (int k = 0; k < pc; ++k) {
+      const uint32_t E = k ? (((w << k) & wmask) | (w >> (n - k))) : w;
+      const int j = (n - k) % pr;
+      C val = v[0];
+#pragma unroll
+      for (int t = 1; t < 16; ++t)
+        if (t == j) val = v[t];
+      lds[dev::sw(E)] = val;
+    }
   }
   __syncthreads();
   dev::lds_kron(lds, n, 0, ka, kb);
   const C inv = scale ? (C)(1.0 / *scale) : (C)1;
-  const int64_t o0 = rstart[bi], o1 = rstart[bi + 1];
-  for (int64_t o = o0 + threadIdx.x; o < o1; o += blockDim.x)
-    y[o] = (TO)(dt[o] * lds[replo[o]] * inv);
+  const int64_t i0 = ostart[r], i1 = ostart[r + 1];
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const uint32_t o = olist[i];
+    y[o] = (TO)(dt[o] * lds[dev::sw(Elist[i])] * inv);
+  }
 }
 
 template <typename C>
@@ -551,26 +614,22 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
   if (p.ft) {
     const FusedTables &ft = *p.ft;
     const int n = ft.n;
-    const int64_t ldA = ((ft.Q + 63) / 64) * 64;
     C *Ap = rest;
     const size_t shm = (size_t(1) << n) * sizeof(C);
-    const unsigned g1 = (unsigned)(((ft.Q + 7) / 8) * 8);
-    hipLaunchKernelGGL((k_fused_pass1<C>), dim3(g1), dim3(FUSED_THREADS), shm, s,
-                       ft.cmap.as<uint32_t>(), xt, Ap, ft.srows.as<uint16_t>(), ft.nS, n, ft.Q,
-                       ldA, ka, kb);
+    hipLaunchKernelGGL((k_fused_pass1<C>), dim3(ft.nNk), dim3(FUSED_THREADS), shm, s,
+                       ft.cmap.as<uint32_t>(), xt, Ap, ft.slotO.as<uint16_t>(), n, ft.nNk,
+                       ft.ldA, ka, kb);
     TM_LAUNCH_CHECK();
     if (io == TM_F64)
-      hipLaunchKernelGGL((k_fused_pass2<C, double>), dim3(ft.nB), dim3(FUSED_THREADS), shm, s,
-                         Ap, ft.cols.as<uint32_t>(), ft.Q, ft.brows.as<uint16_t>(),
-                         ft.rowidx.as<int32_t>(), ft.rstart.as<int64_t>(),
-                         ft.replo.as<uint16_t>(), dt, static_cast<double *>(out), n, ldA, ka,
-                         kb, scale);
+      hipLaunchKernelGGL((k_fused_pass2<C, double>), dim3(ft.nNk), dim3(FUSED_THREADS), shm, s,
+                         Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(), ft.nNk,
+                         ft.ostart.as<int64_t>(), ft.olist.as<uint32_t>(), ft.Elist.as<uint16_t>(),
+                         dt, static_cast<double *>(out), n, ft.ldA, ka, kb, scale);
     else
-      hipLaunchKernelGGL((k_fused_pass2<C, float>), dim3(ft.nB), dim3(FUSED_THREADS), shm, s,
-                         Ap, ft.cols.as<uint32_t>(), ft.Q, ft.brows.as<uint16_t>(),
-                         ft.rowidx.as<int32_t>(), ft.rstart.as<int64_t>(),
-                         ft.replo.as<uint16_t>(), dt, static_cast<float *>(out), n, ldA, ka,
-                         kb, scale);
+      hipLaunchKernelGGL((k_fused_pass2<C, float>), dim3(ft.nNk), dim3(FUSED_THREADS), shm, s,
+                         Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(), ft.nNk,
+                         ft.ostart.as<int64_t>(), ft.olist.as<uint32_t>(), ft.Elist.as<uint16_t>(),
+                         dt, static_cast<float *>(out), n, ft.ldA, ka, kb, scale);
     TM_LAUNCH_CHECK();
     return;
   }
@@ -610,8 +669,7 @@ size_t reduced_work_bytes(const ReducedPlan &p, tm_dtype comp)
   const size_t es = comp == TM_F64 ? 8 : 4;
   size_t b = (size_t)((p.M + 63) / 64) * 64 * es;
   if (p.ft) {
-    const int64_t ldA = ((p.ft->Q + 63) / 64) * 64;
-    b += (size_t)p.ft->nS * ldA * es;
+    b += (size_t)p.ft->nNk * p.ft->ldA * es;
   } else {
     b += (size_t(1) << p.L) * es;
   }

@@ -72,7 +72,15 @@ class KrylovSchur {
     k_ = 0;
   }
 
-  // Start from given orthonormal vectors already in V(0..k-1)? (not used)
+  // Start from a given device vector (normalised here).
+  void start_from(const double *v0)
+  {
+    hipStream_t s = op_->stream;
+    kry::copy(v0, V(0), n_, s);
+    kry::multidot(V_.as<double>(), ldv_, 0, V(0), n_, partial_.as<double>(), hbuf_.as<double>(), s);
+    kry::normalize(V(0), n_, hbuf_.as<double>(), s);
+    k_ = 0;
+  }
 
   // Expand the Krylov-Schur decomposition from column k_ to m_.
   void expand()
@@ -85,17 +93,28 @@ class KrylovSchur {
       op_->apply_dev(V(j), w, ldv_, 1, TM_F64, comp_, tr_, nullptr, s);
       ++matvecs_;
       const int nk = j + 1;
+      // classical Gram-Schmidt with one DGKS correction (only when the first
+      // projection removed more than half of the norm^2)
       kry::multidot(V_.as<double>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
       kry::update_dots(V_.as<double>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 72, s);
-      kry::update_norm(V_.as<double>(), ldv_, nk, w, n_, hb + 72, partial_.as<double>(), hb + 144, s);
-      TM_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double) * 3 * 72, hipMemcpyDeviceToHost, s));
+      TM_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double) * 2 * 72, hipMemcpyDeviceToHost, s));
       TM_HIP(hipStreamSynchronize(s));
-      double hn2 = 0;
-      for (int kk = 0; kk < nk; ++kk) {
-        H(kk, j) = hh[kk] + hh[72 + kk];
-        hn2 += H(kk, j) * H(kk, j);
+      const double w0 = hh[nk], w1 = hh[72 + nk];
+      const double *nrm_dev = hb + 72 + nk;
+      double beta2 = w1;
+      const bool reorth = !(w1 > 0.5 * w0);
+      for (int kk = 0; kk < nk; ++kk) H(kk, j) = hh[kk];
+      if (reorth) {
+        kry::update_norm(V_.as<double>(), ldv_, nk, w, n_, hb + 72, partial_.as<double>(), hb + 144, s);
+        TM_HIP(hipMemcpyAsync(hh.data() + 144, hb + 144, sizeof(double), hipMemcpyDeviceToHost, s));
+        TM_HIP(hipStreamSynchronize(s));
+        for (int kk = 0; kk < nk; ++kk) H(kk, j) += hh[72 + kk];
+        beta2 = hh[144];
+        nrm_dev = hb + 144;
+        ++reorths_;
       }
-      const double beta2 = hh[144];
+      double hn2 = 0;
+      for (int kk = 0; kk < nk; ++kk) hn2 += H(kk, j) * H(kk, j);
       double beta = std::sqrt(std::max(beta2, 0.0));
       if (beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) {
         // invariant subspace: continue with a fresh random direction
@@ -108,7 +127,7 @@ class KrylovSchur {
         kry::normalize(w, n_, hb + 144, s);
         beta = 0.0;
       } else {
-        kry::normalize(w, n_, hb + 144, s);
+        kry::normalize(w, n_, nrm_dev, s);
       }
       H(j + 1, j) = beta;
     }
@@ -297,18 +316,29 @@ class KrylovSchur {
   }
 
   // Run to convergence; returns the number of converged wanted pairs.
-  int run(Ritz &out)
+  int run(Ritz &out, const double *v0 = nullptr)
   {
-    start();
+    if (v0) start_from(v0); else start();
     int nc = 0;
+    double best = 1e300;
     for (int it = 0; it < maxit_; ++it) {
       expand();
       out = ritz();
       nc = nconv(out, tol_);
+      double worst = 0;
+      for (int i = 0; i < nev_; ++i)
+        worst = std::max(worst, out.res[i] / std::max(std::hypot(out.wr[i], out.wi[i]), 1e-300));
       if (verbose_)
-        std::fprintf(stderr, "[tm_solve] cycle %d matvecs %lld lambda0 %.15g res0 %.3e nconv %d\n",
-                     it, (long long)matvecs_, out.wr[0], out.res[0], nc);
+        std::fprintf(stderr, "[tm_solve] cycle %d matvecs %lld lambda0 %.15g res0 %.3e worst %.3e nconv %d\n",
+                     it, (long long)matvecs_, out.wr[0], out.res[0], worst, nc);
       if (nc >= nev_) break;
+      // reduced-precision phase: stop once the residual has reached the
+      // noise floor of the FP32 operator (no 30% improvement in a cycle)
+      if (stall_stop_ && worst < 1e-4 && worst > 0.7 * best) {
+        stalled_ = true;
+        break;
+      }
+      best = std::min(best, worst);
       int k = nev_ + std::min(nc, (m_ - nev_) / 2);
       k = std::max(k, (nev_ + m_) / 2);
       k = std::min(k, m_ - 1);
@@ -340,6 +370,9 @@ class KrylovSchur {
 
   int64_t matvecs() const { return matvecs_; }
   int64_t restarts() const { return restarts_; }
+  void set_stall_stop(bool v) { stall_stop_ = v; }
+  int64_t reorths() const { return reorths_; }
+  bool stalled() const { return stalled_; }
   bool sym() const { return sym_; }
   int m() const { return m_; }
 
@@ -356,6 +389,8 @@ class KrylovSchur {
   bool sym_ = true;
   int k_ = 0;
   int64_t matvecs_ = 0, restarts_ = 0;
+  bool stall_stop_ = false, stalled_ = false;
+  int64_t reorths_ = 0;
   DevBuf V_, partial_, hbuf_, Ydev_;
   std::vector<double> H_;
 };
@@ -381,47 +416,56 @@ bool cholesky(int k, std::vector<double> &G)
 
 struct RefineOut {
   std::vector<double> theta, res;
+  double bound = 0;
   int64_t matvecs = 0;
 };
 
 // FP64 Rayleigh-Ritz refinement of an orthonormal block Y (k vectors at
-// B[0..k), leading dimension ld) for a symmetric operator.  Uses B[k..2k)
-// for A*Y and B[2k..3k) as scratch.  On return B[0..nev) hold the refined
-// Ritz vectors.
+// B[0..k), leading dimension ld) for a symmetric operator.  B must hold 3k
+// vectors: [Y | W = A Y | X = Y Z].  Each iteration: W = A Y, Ritz pairs of
+// Y^T W, residuals R = W Z - X diag(theta) (into the W slots); accepted when
+// the FP64 residual meets tol or the Rayleigh-Ritz error bound
+// r^2 / (gap |theta|) (gap to the Krylov Ritz values outside the block) is
+// below 1e-13 for every wanted pair; otherwise Y <- orth(A X) = orth(R + X theta)
+// (one subspace-iteration step) and repeat.  On return B[0..nev) hold the
+// refined, normalised Ritz vectors.
 RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev, double tol,
-                     int maxiter, int verbose)
+                     int maxiter, int verbose, const std::vector<double> &outside)
 {
   hipStream_t s = op->stream;
   RefineOut ro;
   DevBuf partial(sizeof(double) * kry::NBLK * 72), G(sizeof(double) * 72 * 72),
       C(sizeof(double) * 2 * 72 * 72);
-  std::vector<double> prev;
   auto col = [&](int c) { return B + (size_t)c * ld; };
   std::vector<double> hG((size_t)k * k), th(k), Z((size_t)k * k);
+  auto upload = [&](const std::vector<double> &h) {
+    TM_HIP(hipMemcpyAsync(C.get(), h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s));
+  };
+  auto gram_host = [&](int a, int b) {  // hG[r*k + c] = <col(a+r), col(b+c)>
+    kry::gram(col(a), col(b), ld, k, n, partial.as<double>(), G.as<double>(), s);
+    TM_HIP(hipMemcpyAsync(hG.data(), G.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
+    TM_HIP(hipStreamSynchronize(s));
+  };
   for (int it = 0; it < maxiter; ++it) {
-    // W = A Y
     for (int c = 0; c < k; ++c) {
       op->apply_dev(col(c), col(k + c), ld, 1, TM_F64, TM_F64, 0, nullptr, s);
       ++ro.matvecs;
     }
-    kry::gram(col(0), col(k), ld, k, n, partial.as<double>(), G.as<double>(), s);
-    TM_HIP(hipMemcpyAsync(hG.data(), G.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
-    TM_HIP(hipStreamSynchronize(s));
-    // hG[row*k + c] = <Y_row, W_c>; symmetric projected matrix
+    gram_host(0, k);
     std::vector<double> Hp((size_t)k * k);
     for (int i = 0; i < k; ++i)
       for (int j = 0; j < k; ++j) Hp[i + (size_t)j * k] = 0.5 * (hG[i * k + j] + hG[j * k + i]);
     sym_eig(k, Hp.data(), th.data(), Z.data());
     std::vector<int> idx(k);
     std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return std::fabs(th[a]) > std::fabs(th[b]); });
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int b) { return std::fabs(th[a]) > std::fabs(th[b]); });
     std::vector<double> ths(k), Zs((size_t)k * k);
     for (int c = 0; c < k; ++c) {
       ths[c] = th[idx[c]];
       for (int t = 0; t < k; ++t) Zs[t + (size_t)c * k] = Z[t + (size_t)idx[c] * k];
     }
-    // [X | R] = [Y | W] * [[Z, -Z diag(th)], [0, Z]]  -> stored into B[2k..4k)?
-    // B has room for 3k vectors: X -> B[2k..3k), then R computed into B[k..2k)
+    // X = [Y|W] [Z; 0] -> slots 2k..3k ;  R = [Y|W] [-Z theta; Z] -> slots k..2k
     std::vector<double> Cx((size_t)2 * k * k, 0.0), Cr((size_t)2 * k * k, 0.0);
     for (int c = 0; c < k; ++c)
       for (int t = 0; t < k; ++t) {
@@ -429,50 +473,53 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
         Cr[t + (size_t)c * 2 * k] = -ths[c] * Zs[t + (size_t)c * k];
         Cr[k + t + (size_t)c * 2 * k] = Zs[t + (size_t)c * k];
       }
-    TM_HIP(hipMemcpyAsync(C.get(), Cx.data(), sizeof(double) * 2 * k * k, hipMemcpyHostToDevice, s));
-    kry::gemm_vy(col(0), ld, 2 * k, C.as<double>(), 2 * k, k, col(2 * k), ld, n, s);  // X
-    TM_HIP(hipMemcpyAsync(C.get(), Cr.data(), sizeof(double) * 2 * k * k, hipMemcpyHostToDevice, s));
-    // R overwrites W (each row reads all 2k inputs before writing)
-    kry::gemm_vy(col(0), ld, 2 * k, C.as<double>(), 2 * k, k, col(k), ld, n, s);   // R
-    kry::gram(col(k), col(k), ld, k, n, partial.as<double>(), G.as<double>(), s);
-    TM_HIP(hipMemcpyAsync(hG.data(), G.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
-    TM_HIP(hipStreamSynchronize(s));
+    upload(Cx);
+    kry::gemm_vy(col(0), ld, 2 * k, C.as<double>(), 2 * k, k, col(2 * k), ld, n, s);
+    upload(Cr);
+    kry::gemm_vy(col(0), ld, 2 * k, C.as<double>(), 2 * k, k, col(k), ld, n, s);
+    gram_host(k, k);
     ro.theta = ths;
     ro.res.assign(k, 0.0);
-    double worst = 0, change = 0, bound = 0;
+    double worst = 0, bound = 0;
+    std::vector<double> bnd(nev);
     for (int c = 0; c < nev; ++c) {
       const double r = std::sqrt(std::max(hG[c * k + c], 0.0));
       ro.res[c] = r / std::fabs(ths[c]);
       worst = std::max(worst, ro.res[c]);
+      // error bound r^2 / (gap |theta|), gap to the spectrum outside the block
       double gap = 1e300;
-      for (int d = 0; d < k; ++d)
-        if (d != c) gap = std::min(gap, std::fabs(ths[c] - ths[d]));
-      bound = std::max(bound, r * r / (gap * std::fabs(ths[c])));
-      if (!prev.empty()) change = std::max(change, std::fabs(ths[c] - prev[c]) / std::fabs(ths[c]));
+      for (double t : outside) gap = std::min(gap, std::fabs(std::fabs(ths[c]) - std::fabs(t)));
+      if (outside.empty())
+        for (int d = 0; d < k; ++d)
+          if (d != c) gap = std::min(gap, std::fabs(ths[c] - ths[d]));
+      bnd[c] = r * r / (gap * std::fabs(ths[c]));
+      bound = std::max(bound, bnd[c]);
     }
-    if (verbose)
-      std::fprintf(stderr, "[tm_refine] it %d theta0 %.16g res %.3e bound %.3e change %.3e\n", it,
-                   ths[0], worst, bound, change);
-    // Y <- orth(X) for the next round, X kept as the answer
-    kry::gram(col(2 * k), col(2 * k), ld, k, n, partial.as<double>(), G.as<double>(), s);
-    TM_HIP(hipMemcpyAsync(hG.data(), G.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
-    TM_HIP(hipStreamSynchronize(s));
-    // accept when the FP64 residual meets tol, or when the Rayleigh-Ritz
-    // eigenvalue error estimate r^2 / (gap |theta|) is below 1e-13
+    if (verbose) {
+      std::fprintf(stderr, "[tm_refine] it %d theta0 %.16g res %.3e bound", it, ths[0], worst);
+      for (int c = 0; c < nev; ++c) std::fprintf(stderr, " %.2e", bnd[c]);
+      std::fprintf(stderr, "\n");
+    }
+    ro.bound = bound;
     const bool done = worst <= tol || bound <= 1e-13 || it == maxiter - 1;
-    (void)change;
-    // copy X into the first k slots (orthonormalised for the next iteration)
+    // next block: done -> normalised X ; else orth(A X) = orth(R + X theta)
+    std::vector<double> Cn((size_t)2 * k * k, 0.0);
+    for (int c = 0; c < k; ++c) {
+      if (!done) Cn[c + (size_t)c * 2 * k] = 1.0;            // R
+      Cn[k + c + (size_t)c * 2 * k] = done ? 1.0 : ths[c];   // X
+    }
+    upload(Cn);
+    kry::gemm_vy(col(k), ld, 2 * k, C.as<double>(), 2 * k, k, col(0), ld, n, s);
+    gram_host(0, 0);
     std::vector<double> Gm((size_t)k * k);
     for (int i = 0; i < k; ++i)
       for (int j = 0; j < k; ++j) Gm[i + (size_t)j * k] = hG[i * k + j];
     std::vector<double> Rinv((size_t)k * k, 0.0);
     if (!done && cholesky(k, Gm)) {
-      // Y = X L^-T : solve for the upper-triangular inverse of L^T
+      // Y = X' L^-T (orthonormal)
       for (int c = 0; c < k; ++c) {
-        // column c of L^-T
         std::vector<double> e(k, 0.0);
         e[c] = 1.0;
-        // solve L^T x = e (backward)
         for (int i = k - 1; i >= 0; --i) {
           double s2 = e[i];
           for (int t = i + 1; t < k; ++t) s2 -= Gm[t + (size_t)i * k] * e[t];
@@ -481,12 +528,12 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
         for (int i = 0; i < k; ++i) Rinv[i + (size_t)c * k] = e[i];
       }
     } else {
-      for (int i = 0; i < k; ++i) Rinv[i + (size_t)i * k] = 1.0 / std::sqrt(std::max(hG[i * k + i], 1e-300));
+      for (int i = 0; i < k; ++i)
+        Rinv[i + (size_t)i * k] = 1.0 / std::sqrt(std::max(hG[i * k + i], 1e-300));
     }
-    TM_HIP(hipMemcpyAsync(C.get(), Rinv.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, s));
-    kry::gemm_vy(col(2 * k), ld, k, C.as<double>(), k, k, col(0), ld, n, s);
+    upload(Rinv);
+    kry::gemm_vy(col(0), ld, k, C.as<double>(), k, k, col(0), ld, n, s);
     TM_HIP(hipStreamSynchronize(s));
-    prev = ths;
     if (done) break;
   }
   return ro;
@@ -519,7 +566,9 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   for (;;) {
     ks.reset(new KrylovSchur(op, nev, ncv, mixed ? std::max(o.mixed_tol, o.tol) : o.tol, o.maxit,
                              mixed ? TM_F32 : TM_F64, 0, o.seed, o.verbose));
+    if (mixed) ks->set_stall_stop(true);
     nc = ks->run(rz);
+    if (mixed && ks->stalled()) nc = nev;  // FP32 phase ends at its noise floor
     restarts += ks->restarts();
     if (mixed) mv32 += ks->matvecs(); else mv64 += ks->matvecs();
     ++trial;
@@ -538,12 +587,15 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   std::vector<double> evr(nev), evi(nev), res(nev);
   DevBuf Xr, Xi;
   const bool cplx = !op->symmetric;
-  if (mixed && op->symmetric) {
-    // FP64 refinement of a block of kept Ritz vectors
+  const int refine = o.refine != 0 ? o.refine : (op->symmetric ? 1 : 2);
+  if (mixed && op->symmetric && refine == 1) {
+    // FP64 block Rayleigh-Ritz refinement of the kept FP32 Ritz vectors
     const int k = std::min(ks->m() - 1, nev + 1);
     DevBuf B(sizeof(double) * ldv * 3 * k);
     ks->ritz_vectors(rz, k, B.as<double>(), nullptr);
-    RefineOut ro = refine_sym(op, B.as<double>(), ldv, n, k, nev, o.tol, 4, o.verbose);
+    std::vector<double> outside;
+    for (int d = k; d < ks->m(); ++d) outside.push_back(rz.wr[d]);
+    RefineOut ro = refine_sym(op, B.as<double>(), ldv, n, k, nev, o.tol, 4, o.verbose, outside);
     mv64 += ro.matvecs;
     for (int i = 0; i < nev; ++i) {
       evr[i] = ro.theta[i];
@@ -555,27 +607,27 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
     Xr.alloc(sizeof(double) * ldv * nev);
     TM_HIP(hipMemcpy(Xr.get(), B.get(), sizeof(double) * ldv * nev, hipMemcpyDeviceToDevice));
   } else {
+    std::unique_ptr<KrylovSchur> k2;
+    KrylovSchur *fin = ks.get();
     if (mixed) {
-      // general operator: FP32 phase result refined by an FP64 Krylov run
-      // warm-started from the leading FP32 Ritz vector
-      Xr.alloc(sizeof(double) * ldv * nev);
-      ks->ritz_vectors(rz, 1, Xr.as<double>(), nullptr);
-      tm_solve_opts o2 = o;
-      o2.precision = TM_PREC_F64;
-      KrylovSchur k2(op, nev, ncv, o.tol, o.maxit, TM_F64, 0, o.seed, o.verbose);
-      nc = k2.run(rz);
-      mv64 += k2.matvecs();
-      restarts += k2.restarts();
-      ks.reset();
-      // use k2's vectors
-      Xr.alloc(sizeof(double) * k2.ldv() * nev);
-      if (cplx) Xi.alloc(sizeof(double) * k2.ldv() * nev);
-      k2.ritz_vectors(rz, nev, Xr.as<double>(), cplx ? Xi.as<double>() : nullptr);
-    } else {
-      Xr.alloc(sizeof(double) * ldv * nev);
-      if (cplx) Xi.alloc(sizeof(double) * ldv * nev);
-      ks->ritz_vectors(rz, nev, Xr.as<double>(), cplx ? Xi.as<double>() : nullptr);
+      // FP64 Krylov-Schur warm-started from the sum of the wanted FP32 Ritz
+      // vectors (real parts): the FP64 run converges to tol in a cycle or two
+      DevBuf X0(sizeof(double) * ldv * nev), v0(sizeof(double) * ldv);
+      ks->ritz_vectors(rz, nev, X0.as<double>(), nullptr);
+      std::vector<double> ones((size_t)nev * 1, 1.0);
+      DevBuf Y1(sizeof(double) * nev);
+      TM_HIP(hipMemcpy(Y1.get(), ones.data(), sizeof(double) * nev, hipMemcpyHostToDevice));
+      kry::gemm_vy(X0.as<double>(), ldv, nev, Y1.as<double>(), nev, 1, v0.as<double>(), ldv, n,
+                   op->stream);
+      k2.reset(new KrylovSchur(op, nev, ncv, o.tol, o.maxit, TM_F64, 0, o.seed, o.verbose));
+      nc = k2->run(rz, v0.as<double>());
+      mv64 += k2->matvecs();
+      restarts += k2->restarts();
+      fin = k2.get();
     }
+    Xr.alloc(sizeof(double) * fin->ldv() * nev);
+    if (cplx) Xi.alloc(sizeof(double) * fin->ldv() * nev);
+    fin->ritz_vectors(rz, nev, Xr.as<double>(), cplx ? Xi.as<double>() : nullptr);
     for (int i = 0; i < nev; ++i) {
       evr[i] = rz.wr[i];
       evi[i] = rz.wi[i];

@@ -61,7 +61,9 @@ def test_zbt_nonsym_dense(lt):
     print("gpu", r.evals, "dense", w[:3], flush=True)
     assert abs(r.evals[0] / w[0] - 1) < 1e-12
     assert abs(abs(r.evals[1]) / abs(w[1]) - 1) < 1e-10
-    assert abs(r.eval_left / w[0].real - 1) < 1e-12
+    # the left solve stops at residual tol 1e-10 (pfunc.hpp:52 uses the same
+    # defaults); for a non-symmetric matrix the eigenvalue error is first order
+    assert abs(r.eval_left / w[0].real - 1) < 1e-10
     # left vector is the right Perron vector of T^T
     lvec = r.vecl
     assert np.linalg.norm(D.T @ lvec - r.eval_left * lvec) / abs(r.eval_left) < 1e-8
