@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tx-reps", type=int, default=20)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--profile-tx", action="store_true",
+                    help="only a calibration copy and a few T.x applies (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of this "
@@ -58,6 +60,10 @@ def main():
     setup_s = time.perf_counter() - t0
     M = op.cols()
     log(f"[bench] operator XAM L={L} M={M} setup {setup_s:.2f}s")
+
+    if args.profile_tx:
+        profile_tx(lt, op)
+        return
 
     def solve(prec):
         return lt.solve(op, nev=neigs, precision=prec, seed=1)
@@ -161,6 +167,21 @@ def main():
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(out), flush=True)
+
+
+def profile_tx(lt, op):
+    """Counter-run workload: a plain device copy of known size (the
+    calibration for FETCH_SIZE / WRITE_SIZE), then 5 FP32 and 5 FP64 T.x."""
+    import torch
+    nbytes = 1 << 29  # 512 MiB read + 512 MiB written
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    op.time_apply(lt.TM_F64, lt.TM_F32, warmup=0, reps=5)
+    op.time_apply(lt.TM_F64, lt.TM_F64, warmup=0, reps=5)
+    log(f"[bench] profile-tx done: copy {nbytes} B x3, T.x fp32 x5, fp64 x5")
 
 
 def cpu_baseline(L, bJ, kappa, nmatvec):

@@ -1,0 +1,197 @@
+"""The `tmat` program: the positional input parser (TMat/read_input.cpp:14-92)
+on the CPU, and end-to-end runs on the GPU whose output files
+(X_val.dat, X_vec.dat, X_vecl.dat, X_phys.dat; TMat/pfunc.hpp:116-199) are
+checked against the oracle's dense matrices and closed forms."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TMAT = os.path.join(ROOT, "latttmat_amd", "tmat")
+
+INPUT = """modeltype = {model} # model type. A: ANNNI; B: BNNNI and 3NN; D: DNNI
+dim = {dim}       # dimensionality. 2 or 3
+axialdir = {axialdir}  # indices of the type of boundary condition (see source)
+neigs = {neigs}     # number of eigenvalues been calculated
+savevec = {savevec} # options (1 or 0) to save vectors or computing layer magnetication (or not)
+outfile = {out}  # prefix for output files
+size = {size}        # system width L
+J = {J}          # inverse temperature beta * J
+kappa = {kappa}    # kappa1 and kappa2 values
+h = {h}            # external field
+"""
+
+
+def write_input(tmp, **kw):
+    d = dict(model="A", dim=2, axialdir=1, neigs=1, savevec="0 0", out="T", size=8, J=0.5,
+             kappa="0.6 0", h=0)
+    d.update(kw)
+    path = os.path.join(tmp, "input.dat")
+    with open(path, "w") as f:
+        f.write(INPUT.format(**d))
+    return path
+
+
+def run(args, cwd, timeout=120):
+    if not os.path.exists(TMAT):
+        pytest.fail(f"{TMAT} is not built; run __graft_entry__.build()")
+    return subprocess.run([TMAT, *args], cwd=cwd, capture_output=True, text=True, timeout=timeout)
+
+
+# ---------------------------------------------------------------- CPU ----
+
+def test_parse_ok_and_echo(tmp_path):
+    p = write_input(str(tmp_path), model="B", axialdir=3, kappa="0.5 0.3", size=10, h=0.25)
+    r = run(["--check-input", p], str(tmp_path))
+    assert r.returncode == 0, r.stdout
+    lines = r.stdout.splitlines()
+    assert lines[0] == f"Reading input from file {p}"
+    assert lines[1] == "     MODEL : 2D_B3"
+    assert lines[2] == "Save & flg : T 0 0"
+    assert lines[3] == "      size : 10"
+    assert lines[4] == "     betaJ : 0.5"
+    assert lines[5] == "     kappa : 0.5"
+    assert lines[6] == "    kappa2 : 0.3"
+    assert lines[7] == "         h : 0.25"
+
+
+def test_parse_3d_sizes(tmp_path):
+    p = write_input(str(tmp_path), dim=3, size="4 5")
+    r = run(["--check-input", p], str(tmp_path))
+    assert r.returncode == 0
+    assert "      size : 4 5" in r.stdout.splitlines()
+
+
+def test_parse_kappa2_defaults_to_zero(tmp_path):
+    p = write_input(str(tmp_path), model="B", axialdir=3, kappa="0.7")
+    r = run(["--check-input", p], str(tmp_path))
+    assert r.returncode == 0
+    assert "    kappa2 : 0" in r.stdout.splitlines()
+
+
+@pytest.mark.parametrize("kw,code,msg", [
+    (dict(dim=4), 4, "Invalid dim 4"),
+    (dict(model="C"), 4, "Invalid model C"),
+    (dict(size=3), 4, "Invalid size (<4) 3"),
+    (dict(J="abc"), 3, "Error reading input file"),
+])
+def test_parse_errors(tmp_path, kw, code, msg):
+    p = write_input(str(tmp_path), **kw)
+    r = run(["--check-input", p], str(tmp_path))
+    assert r.returncode == code
+    assert msg in r.stdout
+
+
+def test_missing_file(tmp_path):
+    r = run(["--check-input", str(tmp_path / "nope.dat")], str(tmp_path))
+    assert r.returncode == 2
+    assert "Can't open" in r.stdout
+    # without --check-input the program exits 0 like the reference's main()
+    r = run([str(tmp_path / "nope.dat")], str(tmp_path))
+    assert r.returncode == 0
+
+
+def test_default_input_name(tmp_path):
+    write_input(str(tmp_path))
+    r = run(["--check-input"], str(tmp_path))
+    assert r.returncode == 0
+    assert "Reading input from file input.dat" in r.stdout
+
+
+# ---------------------------------------------------------------- GPU ----
+
+def moments_full(q2, L, nodes_bc=None):
+    s = np.arange(len(q2), dtype=np.uint64)
+    pc = np.array([bin(int(v)).count("1") for v in s], dtype=np.float64)
+    m = 2 * pc - L
+    mom = [float(np.sum(m ** k * q2)) / L for k in range(1, 5)]
+    return mom
+
+
+@pytest.mark.gpu
+def test_cli_xat_outputs(tmp_path, oracle):
+    L = 10
+    p = write_input(str(tmp_path), size=L, neigs=2, savevec="1 1")
+    r = run([p], str(tmp_path))
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    D = oracle.OracleOp(oracle.XAT, L, 0, 0, 0.5, 0.6, 0.0, 0.0)
+    T = D.dense()
+    w, V = np.linalg.eigh(0.5 * (T + T.T))
+    nf = D.normfactor
+    vals = np.loadtxt(tmp_path / "T_val.dat")
+    assert vals.shape == (2,)
+    np.testing.assert_allclose(vals, w[::-1][:2] / nf, rtol=1e-12)
+    raw = open(tmp_path / "T_val.dat").read()
+    assert not raw.endswith("\n")
+    vec = np.fromfile(tmp_path / "T_vec.dat", dtype=np.float64)
+    assert vec.size == 2 * (1 << L)
+    v0 = vec[: 1 << L]
+    assert abs(abs(v0 @ V[:, -1]) - 1) < 1e-10
+    phys = open(tmp_path / "T_phys.dat").read().split("\n")
+    mom = [float(x) for x in phys[0].split("\t")]
+    ref = moments_full(V[:, -1] ** 2, L)
+    np.testing.assert_allclose(mom, ref, rtol=1e-8, atol=1e-10)
+    assert "Number of iteration:" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cli_reduced_phys_equals_full(tmp_path):
+    """Moments and node counts are invariant under the reduction's group, so
+    the reduced operator's X_phys.dat equals the full operator's."""
+    L = 14
+    d1, d2 = tmp_path / "full", tmp_path / "red"
+    d1.mkdir()
+    d2.mkdir()
+    p1 = write_input(str(d1), size=L, axialdir=1, savevec="0 1", out="F", J=0.3, kappa="0.4 0")
+    p2 = write_input(str(d2), size=L, axialdir=5, savevec="0 1", out="R", J=0.3, kappa="0.4 0")
+    r1 = run([p1], str(d1))
+    r2 = run([p2], str(d2))
+    assert r1.returncode == 0 and r2.returncode == 0
+    f = np.array([[float(x) for x in ln.split("\t")]
+                  for ln in open(d1 / "F_phys.dat").read().split("\n")], dtype=object)
+    g = np.array([[float(x) for x in ln.split("\t")]
+                  for ln in open(d2 / "R_phys.dat").read().split("\n")], dtype=object)
+    print(f, g, flush=True)
+    np.testing.assert_allclose(np.array(f[0], float), np.array(g[0], float), rtol=1e-8, atol=1e-9)
+    np.testing.assert_allclose(np.array(f[1], float), np.array(g[1], float), rtol=1e-8, atol=1e-9)
+    vf = np.loadtxt(d1 / "F_val.dat")
+    vr = np.loadtxt(d2 / "R_val.dat")
+    assert abs(float(vr) / float(vf) - 1) < 1e-12
+
+
+@pytest.mark.gpu
+def test_cli_zbt_left_and_vecs(tmp_path, oracle):
+    L = 8
+    p = write_input(str(tmp_path), model="B", axialdir=3, size=L, kappa="0.5 0.3", savevec="1 1",
+                    out="Z")
+    r = run([p], str(tmp_path))
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    D = oracle.OracleOp(oracle.ZBT, L, 0, 0, 0.5, 0.5, 0.3, 0.0)
+    T = D.dense()
+    w, VR = np.linalg.eig(T)
+    i0 = np.argmax(np.abs(w))
+    lam = w[i0].real
+    raw = open(tmp_path / "Z_val.dat").read()
+    assert raw.startswith("(") and raw.endswith(")")
+    re_, im_ = raw.strip("()").split(",")
+    assert abs(float(re_) / (lam / D.normfactor) - 1) < 1e-12 and float(im_) == 0.0
+    vr = np.fromfile(tmp_path / "Z_vec.dat", dtype=np.float64)
+    vl = np.fromfile(tmp_path / "Z_vecl.dat", dtype=np.float64)
+    assert vr.size == vl.size == 1 << L
+    assert np.linalg.norm(T @ vr - lam * vr) / abs(lam) < 1e-8
+    assert np.linalg.norm(T.T @ vl - lam * vl) / abs(lam) < 1e-8
+    chk = [ln for ln in r.stdout.splitlines() if ln.startswith("Eigensolver check: ")]
+    assert len(chk) == 1 and abs(float(chk[0].split()[-1]) - 1) < 1e-9
+    # non-symmetric moments: weights ql*qr normalised by their sum, no second line
+    phys = open(tmp_path / "Z_phys.dat").read()
+    assert "\n" not in phys
+    mom = [float(x) for x in phys.split("\t")]
+    wgt = vl * vr
+    s = np.arange(1 << L)
+    m = 2 * np.array([bin(int(v)).count("1") for v in s], float) - L
+    ref = [float(np.sum(m ** k * wgt) / (L * wgt.sum())) for k in range(1, 5)]
+    np.testing.assert_allclose(mom, ref, rtol=1e-8, atol=1e-10)
