@@ -36,6 +36,8 @@ class TransferMatrixT;
 
 class TransferMatrix {
  public:
+  using Scalar = double;  // element type of perform_op's vectors
+  using MatVecProd_T = TransferMatrixT;
   explicit TransferMatrix(const tm_params &p)
   {
     check(tm_op_create(&p, &op_));
@@ -64,6 +66,7 @@ class TransferMatrix {
 // TMat/tmat.cpp:652-662).  It borrows the operator it was made from.
 class TransferMatrixT {
  public:
+  using Scalar = double;
   explicit TransferMatrixT(const TransferMatrix &t) : op_(t.handle()) {}
   int64_t rows() const { return tm_op_rows(op_); }
   int64_t cols() const { return tm_op_cols(op_); }

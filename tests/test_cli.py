@@ -137,29 +137,50 @@ def test_cli_xat_outputs(tmp_path, oracle):
     assert "Number of iteration:" in r.stdout
 
 
+def read_phys(path):
+    return [[float(x) for x in ln.split("\t")] for ln in open(path).read().split("\n")]
+
+
 @pytest.mark.gpu
-def test_cli_reduced_phys_equals_full(tmp_path):
-    """Moments and node counts are invariant under the reduction's group, so
-    the reduced operator's X_phys.dat equals the full operator's."""
+def test_cli_reduced_phys(tmp_path, oracle):
+    """The reduced operator's X_phys.dat weights each orbit by the
+    magnetisation of its representative (TMat/tmatreduce.cpp:565-613), checked
+    against the oracle's orbit basis and dense Perron vector.  Even moments and
+    node counts are invariant under the group (spin flip included), so they
+    also equal the full operator's."""
     L = 14
     d1, d2 = tmp_path / "full", tmp_path / "red"
     d1.mkdir()
     d2.mkdir()
     p1 = write_input(str(d1), size=L, axialdir=1, savevec="0 1", out="F", J=0.3, kappa="0.4 0")
-    p2 = write_input(str(d2), size=L, axialdir=5, savevec="0 1", out="R", J=0.3, kappa="0.4 0")
+    p2 = write_input(str(d2), size=L, axialdir=5, savevec="1 1", out="R", J=0.3, kappa="0.4 0")
     r1 = run([p1], str(d1))
     r2 = run([p2], str(d2))
     assert r1.returncode == 0 and r2.returncode == 0
-    f = np.array([[float(x) for x in ln.split("\t")]
-                  for ln in open(d1 / "F_phys.dat").read().split("\n")], dtype=object)
-    g = np.array([[float(x) for x in ln.split("\t")]
-                  for ln in open(d2 / "R_phys.dat").read().split("\n")], dtype=object)
+    f = read_phys(d1 / "F_phys.dat")
+    g = read_phys(d2 / "R_phys.dat")
     print(f, g, flush=True)
-    np.testing.assert_allclose(np.array(f[0], float), np.array(g[0], float), rtol=1e-8, atol=1e-9)
-    np.testing.assert_allclose(np.array(f[1], float), np.array(g[1], float), rtol=1e-8, atol=1e-9)
+    # oracle: reduced basis with spin-flip folding (neigs=1 -> censym on)
+    oop = oracle.OracleOp(oracle.XAM2D, L, 0, 0, 0.3, 0.4, 0.0, 0.0)
+    T = oop.dense()
+    w, V = np.linalg.eigh(0.5 * (T + T.T))
+    q = V[:, -1]
+    reps = oop.reps().astype(np.uint64)
+    m = 2.0 * np.array([bin(int(s)).count("1") for s in reps]) - L
+    mom = [float(np.sum(m ** k * q * q)) / L for k in range(1, 5)]
+    sh = (reps >> np.uint64(1)) | ((reps & np.uint64(1)) << np.uint64(L - 1))
+    nd = np.array([bin(int(a ^ b)).count("1") for a, b in zip(reps, sh)], float)
+    node = [float(np.sum(nd * q * q)), float(np.sum(nd * nd * q * q))]
+    np.testing.assert_allclose(g[0], mom, rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(g[1], node, rtol=1e-8)
+    np.testing.assert_allclose([f[0][1], f[0][3]], [g[0][1], g[0][3]], rtol=1e-8)
+    np.testing.assert_allclose(f[1], g[1], rtol=1e-8)
+    assert abs(f[0][0]) < 1e-9 and abs(f[0][2]) < 1e-8  # full vector is flip-even
+    vr = np.fromfile(d2 / "R_vec.dat", dtype=np.float64)
+    assert vr.size == len(reps) and abs(abs(vr @ q) - 1) < 1e-10
     vf = np.loadtxt(d1 / "F_val.dat")
-    vr = np.loadtxt(d2 / "R_val.dat")
-    assert abs(float(vr) / float(vf) - 1) < 1e-12
+    vv = np.loadtxt(d2 / "R_val.dat")
+    assert abs(float(vv) / float(vf) - 1) < 1e-12
 
 
 @pytest.mark.gpu
