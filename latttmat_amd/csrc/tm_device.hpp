@@ -12,6 +12,29 @@
 namespace ltm {
 namespace dev {
 
+// Forward-mode dual number (value, derivative with respect to one model
+// parameter).  The operator kernels are templates over their arithmetic
+// type; instantiated with Dual they apply T and dT/dtheta in one pass.
+struct Dual {
+  double v, d;
+  Dual() = default;
+  __host__ __device__ constexpr Dual(double a) : v(a), d(0.0) {}
+  __host__ __device__ constexpr Dual(double a, double b) : v(a), d(b) {}
+};
+__host__ __device__ __forceinline__ Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__host__ __device__ __forceinline__ Dual operator*(Dual a, Dual b)
+{
+  return Dual(a.v * b.v, a.d * b.v + a.v * b.d);
+}
+__host__ __device__ __forceinline__ Dual &operator+=(Dual &a, Dual b) { return a = a + b; }
+__host__ __device__ __forceinline__ Dual &operator*=(Dual &a, Dual b) { return a = a * b; }
+
+// a coefficient of arithmetic type C from its value and derivative
+template <typename C>
+__host__ __device__ __forceinline__ C mk(double v, double) { return (C)v; }
+template <>
+__host__ __device__ __forceinline__ Dual mk<Dual>(double v, double d) { return Dual(v, d); }
+
 __device__ __forceinline__ int pop(uint64_t x) { return __popcll(x); }
 
 // rotate right by d within n bits (TMat/utility.hpp:91-94)

@@ -120,6 +120,11 @@ struct KronPlan {
   const float *tab32;       // device, 192 floats
   double ka, kb;
 };
+// Dual-number apply: out (2 doubles per state: T in, dT/dtheta in) from a
+// real input; tab_dual holds 192 (value, derivative) pairs and dka/dkb the
+// derivatives of ka/kb.
+void launch_kron_dual(const KronPlan &plan, const double *in, void *out, const void *tab_dual,
+                      double dka, double dkb, hipStream_t s);
 void launch_kron(const KronPlan &plan, const void *in, void *out, tm_dtype io_in,
                  tm_dtype io_out, tm_dtype comp, void *work, const double *scale_in,
                  hipStream_t s);
@@ -135,13 +140,16 @@ struct SweepPlan {
   const float *tab32;
   double b1[2], b2[2], b3[2];
   double nf;          // XDT output factor (ZBT folds it into the diagonal)
+  double db1[2], db2[2], db3[2], dnf;  // derivatives (dual-number apply only)
 };
+void launch_sweep_dual(const SweepPlan &plan, const double *in, void *out, int transpose,
+                       void *work, const void *tab_dual, hipStream_t s);
+size_t sweep_dual_work_bytes(int kind, int L);
 void launch_sweep(const SweepPlan &plan, const void *in, void *out, tm_dtype io,
                   tm_dtype comp, int transpose, void *work, const double *scale_in,
                   hipStream_t s);
 size_t sweep_work_bytes(int kind, int L, tm_dtype comp);
 
-// Grad kernels (dual numbers through the passes); see tm_grad in tm_ops.cpp.
 
 // ---- orbit enumeration (tm_reduced.hip) ----
 struct GroupSpec {
@@ -156,6 +164,9 @@ struct GroupSpec {
 // sizes), returns M.  If map != nullptr it receives map[s] = orbit index of s.
 int64_t enumerate_orbits(const GroupSpec &g, DevBuf &reps, DevBuf &dups, DevBuf *map,
                          hipStream_t s);
+// xf[s] = xr[orbit(s)] / sqrt(orbit size) over all 2^L states (device).
+void expand_orbits(const GroupSpec &g, const uint64_t *reps, const uint16_t *dups, int64_t M,
+                   const double *xr, double *xf, hipStream_t s);
 // Fused-path tables (2D periodic, L = 2n even).  A state s is split into
 // its even bits E and odd bits O (n bits each); a rotation of s by two sites
 // rotates E and O together, so the transform over O only has to be done for

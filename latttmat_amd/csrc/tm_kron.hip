@@ -77,19 +77,17 @@ __global__ __launch_bounds__(KRON_THREADS) void k_kron_pass(
 
 struct Pass { int b0, nb, logW; };
 
-// 2^13 FP64 / 2^14 FP32 tiles (64 KB + table) need the gfx950 LDS size;
-// fall back to half-size tiles on a smaller LDS.
-int tile_log_for(tm_dtype comp)
-{
-  const bool big = g_lds_per_block >= (size_t)(96 * 1024);
-  return comp == TM_F64 ? (big ? 13 : 12) : (big ? 14 : 13);
-}
+// Tiles of 64 KB (2^13 FP64, 2^14 FP32, 2^12 dual elements) plus the table
+// need the gfx950 LDS size; fall back to half-size tiles on a smaller LDS.
+// Passes after the first move 64-byte contiguous runs.
+int log2_es(int es) { return es == 4 ? 2 : (es == 8 ? 3 : 4); }
 
-std::vector<Pass> plan_passes(int L, tm_dtype comp)
+std::vector<Pass> plan_passes(int L, int es)
 {
   std::vector<Pass> ps;
-  const int tl = tile_log_for(comp);
-  const int wl = comp == TM_F64 ? 3 : 4;  // 64-byte contiguous runs
+  const bool big = g_lds_per_block >= (size_t)(96 * 1024);
+  const int tl = (big ? 16 : 15) - log2_es(es);
+  const int wl = 6 - log2_es(es);
   int nb0 = L < tl ? L : tl;
   ps.push_back({0, nb0, 0});
   int b = nb0;
@@ -105,7 +103,7 @@ std::vector<Pass> plan_passes(int L, tm_dtype comp)
 
 template <typename C, typename TI, typename TO>
 void run_pass(const Pass &p, int L, const void *in, void *out, int flags, const KronPlan &plan,
-              const C *tab, const double *scale, hipStream_t s)
+              const C *tab, C ka, C kb, const double *scale, hipStream_t s)
 {
   const uint64_t ntile = 1ull << (L - p.nb - p.logW);
   dim3 grid;
@@ -114,7 +112,7 @@ void run_pass(const Pass &p, int L, const void *in, void *out, int flags, const 
   const size_t shm = ((size_t(1) << (p.nb + p.logW)) + 192) * sizeof(C);
   hipLaunchKernelGGL((k_kron_pass<C, TI, TO>), grid, dim3(KRON_THREADS), shm, s,
                      static_cast<const TI *>(in), static_cast<TO *>(out), p.b0, p.nb, p.logW,
-                     flags, plan.diag, tab, (C)plan.ka, (C)plan.kb, scale);
+                     flags, plan.diag, tab, ka, kb, scale);
   TM_LAUNCH_CHECK();
 }
 
@@ -123,8 +121,9 @@ void kron_typed(const KronPlan &plan, const void *in, void *out, tm_dtype io_in,
                 void *work, const double *scale, hipStream_t s, const C *tab)
 {
   const int L = plan.L;
-  auto ps = plan_passes(L, sizeof(C) == 8 ? TM_F64 : TM_F32);
+  auto ps = plan_passes(L, (int)sizeof(C));
   const int np = (int)ps.size();
+  const C ka = (C)plan.ka, kb = (C)plan.kb;
   // Intermediate buffer: the output itself when it has the compute type,
   // else the workspace.
   const bool out_is_c = (sizeof(C) == 8) == (io_out == TM_F64);
@@ -141,18 +140,18 @@ void kron_typed(const KronPlan &plan, const void *in, void *out, tm_dtype io_in,
     const bool dst_io = (i == np - 1);
     const double *sc = (src_io || dst_io) ? scale : nullptr;
     if (src_io && dst_io) {
-      if (io_in == TM_F64 && io_out == TM_F64) run_pass<C, double, double>(ps[i], L, src, dst, flags, plan, tab, sc, s);
-      else if (io_in == TM_F64) run_pass<C, double, float>(ps[i], L, src, dst, flags, plan, tab, sc, s);
-      else if (io_out == TM_F64) run_pass<C, float, double>(ps[i], L, src, dst, flags, plan, tab, sc, s);
-      else run_pass<C, float, float>(ps[i], L, src, dst, flags, plan, tab, sc, s);
+      if (io_in == TM_F64 && io_out == TM_F64) run_pass<C, double, double>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
+      else if (io_in == TM_F64) run_pass<C, double, float>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
+      else if (io_out == TM_F64) run_pass<C, float, double>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
+      else run_pass<C, float, float>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
     } else if (src_io) {
-      if (io_in == TM_F64) run_pass<C, double, C>(ps[i], L, src, dst, flags, plan, tab, sc, s);
-      else run_pass<C, float, C>(ps[i], L, src, dst, flags, plan, tab, sc, s);
+      if (io_in == TM_F64) run_pass<C, double, C>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
+      else run_pass<C, float, C>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
     } else if (dst_io) {
-      if (io_out == TM_F64) run_pass<C, C, double>(ps[i], L, src, dst, flags, plan, tab, sc, s);
-      else run_pass<C, C, float>(ps[i], L, src, dst, flags, plan, tab, sc, s);
+      if (io_out == TM_F64) run_pass<C, C, double>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
+      else run_pass<C, C, float>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
     } else {
-      run_pass<C, C, C>(ps[i], L, src, dst, flags, plan, tab, nullptr, s);
+      run_pass<C, C, C>(ps[i], L, src, dst, flags, plan, tab, ka, kb, nullptr, s);
     }
   }
 }
@@ -162,6 +161,25 @@ void kron_typed(const KronPlan &plan, const void *in, void *out, tm_dtype io_in,
 size_t kron_work_bytes(int L, tm_dtype comp)
 {
   return comp == TM_F32 ? (sizeof(float) << L) : 0;
+}
+
+void launch_kron_dual(const KronPlan &plan, const double *in, void *out, const void *tab_dual,
+                      double dka, double dkb, hipStream_t s)
+{
+  using dev::Dual;
+  const int L = plan.L;
+  auto ps = plan_passes(L, (int)sizeof(Dual));
+  const int np = (int)ps.size();
+  const Dual ka(plan.ka, dka), kb(plan.kb, dkb);
+  const Dual *tab = static_cast<const Dual *>(tab_dual);
+  Dual *o = static_cast<Dual *>(out);
+  for (int i = 0; i < np; ++i) {
+    int flags = 0;
+    if (i == 0 && plan.prologue) flags |= F_PRO;
+    if (i == np - 1 && plan.epilogue) flags |= F_EPI;
+    if (i == 0) run_pass<Dual, double, Dual>(ps[i], L, in, o, flags, plan, tab, ka, kb, nullptr, s);
+    else run_pass<Dual, Dual, Dual>(ps[i], L, o, o, flags, plan, tab, ka, kb, nullptr, s);
+  }
 }
 
 void launch_kron(const KronPlan &plan, const void *in, void *out, tm_dtype io_in, tm_dtype io_out,

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <vector>
 
+#include "tm_consts.hpp"
 #include "tm_internal.hpp"
 
 namespace ltm {
@@ -63,39 +64,29 @@ void upload_tables(tm_op *op, const std::vector<double> &t)
   TM_HIP(hipMemcpy(op->tab32.get(), f.data(), 192 * sizeof(float), hipMemcpyHostToDevice));
 }
 
-// normfactor of the X-type 2D operators (TMat/tmat.cpp:48-55,
-// TMat/tmatreduce.cpp:349-359); exponent returned in *e (units of betaJ)
+// normfactor of the X-type operators; exponent (units of betaJ) in *e
 double nf_x2d(int L, double bJ, double k, double *e)
 {
-  *e = (k < 0.5) ? L * (k - 2.0) : -L * (k + 1.0);
+  *e = nfexp_x2d(L, k);
   return std::exp(bJ * *e);
 }
-// 3D (TMat/tmat3d.cpp:117-127)
 double nf_x3d(int size, double bJ, double k, double *e)
 {
-  *e = (k < 0.5) ? size * (k - 3.0) : -size * (k + 2.0);
+  *e = nfexp_x3d(size, k);
   return std::exp(bJ * *e);
 }
 
-// Tv^1/2 tables of XAT (genTvec with betaJ/2, TMat/tmat.hpp:208-310),
-// sqrt(nf) folded into the first table.
-std::vector<double> tables_xat(int L, int bc, double bJ, double k, double h, double sqnf)
+// Upload the constants of a full-space operator: tables, Boltzmann factors,
+// normfactor.
+void apply_consts(tm_op *op, const OpConsts<double> &c)
 {
-  const double J1 = -0.5 * bJ, J2 = 0.5 * bJ * k, hJ = -0.5 * bJ * h;
-  int c1 = L, c2 = L;
-  switch (bc) {
-    case 1: c1 = L - 1; c2 = L - 2; break;
-    case 5: case 9: c1 = L + 1; c2 = L; break;
-    case 13: case 17: c1 = L + 1; c2 = L + 2; break;
-    default: break;
+  upload_tables(op, std::vector<double>(c.tab, c.tab + 192));
+  for (int i = 0; i < 2; ++i) {
+    op->bzwj1[i] = c.b1[i];
+    op->bzwj2[i] = c.b2[i];
+    op->bzwj3[i] = c.b3[i];
   }
-  std::vector<double> t(192, 0.0);
-  for (int n = 0; n < 64; ++n) {
-    t[n] = sqnf * std::exp(J1 * (2.0 * n - c1));
-    t[64 + n] = std::exp(J2 * (2.0 * n - c2));
-    t[128 + n] = std::exp(hJ * (2.0 * n - L));
-  }
-  return t;
+  op->nf = c.nf;
 }
 
 double energy_half_x2d(uint64_t s, int L, int obc, double bJ, double k, double h)
@@ -139,13 +130,13 @@ double energy_half_x3d(uint64_t s, int s1, int s2, double bJ, double k, double h
 
 void set_bz(tm_op *op)
 {
-  const double bJ = op->p.betaJ;
-  op->bzwj1[0] = std::exp(bJ);
-  op->bzwj1[1] = 1.0 / op->bzwj1[0];
-  op->bzwj2[0] = std::exp(-bJ * op->p.kappa);
-  op->bzwj2[1] = 1.0 / op->bzwj2[0];
-  op->bzwj3[0] = std::exp(-bJ * op->p.kappa2);
-  op->bzwj3[1] = 1.0 / op->bzwj3[0];
+  OpConsts<double> c;
+  consts_bz(c, op->p.betaJ, op->p.kappa, op->p.kappa2);
+  for (int i = 0; i < 2; ++i) {
+    op->bzwj1[i] = c.b1[i];
+    op->bzwj2[i] = c.b2[i];
+    op->bzwj3[i] = c.b3[i];
+  }
 }
 
 void init_device()
@@ -220,10 +211,8 @@ tm_op *make_op(const tm_params &p)
     op->L = L;
     op->bc = bc;
     op->nstate = op->ncol = int64_t(1) << L;
-    double e;
-    op->nf = nf_x2d(L, bJ, k, &e);  // TMat/tmat.cpp:48-55
     op->diag = DiagSpec{bc, L, 0, 0, 0, 0, 0, 0};
-    upload_tables(op.get(), tables_xat(L, bc, bJ, k, h, std::sqrt(op->nf)));
+    apply_consts(op.get(), consts_xat<double>(L, bc, bJ, k, h));  // TMat/tmat.cpp:39-56
   };
   auto make_xam = [&](int L, int ext) {
     TM_CHECK(L >= 4 && L <= 32, TM_EINVAL, "XAM width must be in [4, 32]");
@@ -244,20 +233,9 @@ tm_op *make_op(const tm_params &p)
     op->L = L;
     op->bc = bc;
     op->nstate = op->ncol = int64_t(1) << L;
-    // TMat/tmat.cpp:333-340
-    op->nf = (k < 0.5) ? std::exp(L * bJ * 2.0 * (k - 1.0)) : std::exp(-L * bJ * 2.0 * k);
     const int dk = bc == 0 ? DIAG_XDT_PBC : (bc == 5 ? DIAG_XDT_C5 : DIAG_XDT_C9);
     op->diag = DiagSpec{dk, L, 0, 0, 0, 0, 0, 0};
-    // genTvec with betaJ/2 (TMat/tmat.hpp:437-504)
-    const double J1 = -0.5 * bJ, hJ = -0.5 * bJ * h;
-    const int c1 = bc == 0 ? L : L + 1;
-    std::vector<double> t(192, 0.0);
-    for (int n = 0; n < 64; ++n) {
-      t[n] = std::exp(J1 * (2.0 * n - c1));
-      t[64 + n] = 1.0;
-      t[128 + n] = std::exp(hJ * (2.0 * n - L));
-    }
-    upload_tables(op.get(), t);
+    apply_consts(op.get(), consts_xdt<double>(L, bc, bJ, k, h));  // TMat/tmat.cpp:322-341
   };
   auto make_zbt = [&](int L) {
     TM_CHECK(L >= 4 && L <= 30 && (L % 2) == 0, TM_EINVAL,
@@ -266,24 +244,8 @@ tm_op *make_op(const tm_params &p)
     op->symmetric = false;
     op->L = L;
     op->nstate = op->ncol = int64_t(1) << L;
-    const double k2 = p.kappa2;
-    double nfe;  // TMat/tmat.cpp:491-511
-    if (k + k2 < 0.5) nfe = L * 2.0 * (k + k2 - 1.0);
-    else if (k < 0.5 && k2 - k < 0.5) nfe = -L;
-    else if (k2 < 2 * k) nfe = -L * 2.0 * k;
-    else nfe = L * 2.0 * (k - k2);
-    const double half = std::exp(0.5 * bJ * nfe);
-    op->nf = half * half;
     op->diag = DiagSpec{DIAG_ZBT, L, 0, 0, 0, 0, 0, 0};
-    // genTvec with betaJ/2 (TMat/tmat.hpp:537-556), halfnormfactor folded
-    const double J1 = -0.5 * bJ, J3 = 0.5 * k2 * bJ, hJ = -0.5 * bJ * h;
-    std::vector<double> t(192, 0.0);
-    for (int n = 0; n < 64; ++n) {
-      t[n] = half * std::exp(J1 * (2.0 * n - L));
-      t[64 + n] = std::exp(J3 * (2.0 * n - L));
-      t[128 + n] = std::exp(hJ * (2.0 * n - L));
-    }
-    upload_tables(op.get(), t);
+    apply_consts(op.get(), consts_zbt<double>(L, bJ, k, p.kappa2, h));  // TMat/tmat.cpp:480-518
   };
 
   if (p.model == 'A' && p.dim == 2) {
@@ -318,14 +280,7 @@ tm_op *make_op(const tm_params &p)
       op->ncol = op->nstate;
       op->diag = DiagSpec{DIAG_X3D, size, s1, s2, colmask(s1, s2, 1), colmask(s1, s2, s2 - 1),
                           colmask(s1, s2, 2), colmask(s1, s2, s2 - 2)};
-      const double hb = 0.5 * bJ, Jn = hb * e;
-      std::vector<double> t(192, 0.0);
-      for (int n = 0; n < 64; ++n) {
-        t[n] = std::exp(2 * (-hb) * (n - size) + Jn);
-        t[64 + n] = std::exp(hb * k * (2.0 * n - size));
-        t[128 + n] = std::exp(-hb * h * (2.0 * n - size));
-      }
-      upload_tables(op.get(), t);
+      apply_consts(op.get(), consts_xat3d<double>(size, bJ, k, h));
     } else {
       op->kind = TM_KIND_XAM3D;
       op->censym = (h == 0);                    // TMat/tmat3d.cpp:105
@@ -359,38 +314,6 @@ tm_op *make_op(const tm_params &p)
   const double one = 1.0;
   TM_HIP(hipMemcpy(op->one.get(), &one, sizeof one, hipMemcpyHostToDevice));
   return op.release();
-}
-
-void dlnnf(const tm_op *op, double out[4])
-{
-  // derivative of ln(normfactor) w.r.t. (betaJ, kappa, kappa2, h)
-  const double bJ = op->p.betaJ, k = op->p.kappa, k2 = op->p.kappa2;
-  const int L = op->L;
-  for (int i = 0; i < 4; ++i) out[i] = 0;
-  switch (op->kind) {
-    case TM_KIND_XAT:
-    case TM_KIND_XAM2D:
-      if (k < 0.5) { out[0] = L * (k - 2.0); out[1] = L * bJ; }
-      else { out[0] = -L * (k + 1.0); out[1] = -L * bJ; }
-      break;
-    case TM_KIND_XAM3D:
-    case TM_KIND_XAT3D:
-      if (k < 0.5) { out[0] = L * (k - 3.0); out[1] = L * bJ; }
-      else { out[0] = -L * (k + 2.0); out[1] = -L * bJ; }
-      break;
-    case TM_KIND_XDT:
-      if (k < 0.5) { out[0] = L * 2.0 * (k - 1.0); out[1] = L * bJ * 2.0; }
-      else { out[0] = -L * 2.0 * k; out[1] = -L * bJ * 2.0; }
-      break;
-    case TM_KIND_ZBT: {
-      if (k + k2 < 0.5) { out[0] = L * 2.0 * (k + k2 - 1.0); out[1] = out[2] = 2.0 * L * bJ; }
-      else if (k < 0.5 && k2 - k < 0.5) { out[0] = -L; }
-      else if (k2 < 2 * k) { out[0] = -L * 2.0 * k; out[1] = -2.0 * L * bJ; }
-      else { out[0] = L * 2.0 * (k - k2); out[1] = 2.0 * L * bJ; out[2] = -2.0 * L * bJ; }
-      break;
-    }
-    default: break;
-  }
 }
 
 }  // namespace ltm

@@ -338,6 +338,39 @@ uint64_t orbit_bitmap(const GroupDev &g, OrbitScratch &sc, hipStream_t s)
 
 }  // namespace
 
+namespace {
+// Expanded-space vector of a reduced one: xf[s] = xr[o(s)] / sqrt(|o(s)|),
+// with o(s) found by canonicalising s and a binary search of the ascending
+// representatives.  Inner products and the transfer matrix carry over:
+// <xf, T yf> = <xr, T_red yr>.
+__global__ void k_expand_orbits(GroupDev g, uint64_t nstate, const uint64_t *__restrict__ reps,
+                                const uint16_t *__restrict__ dups, int64_t M,
+                                const double *__restrict__ xr, double *__restrict__ xf)
+{
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < nstate;
+       s += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t c = canon(s, g, nullptr);
+    int64_t lo = 0, hi = M - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (reps[mid] < c) lo = mid + 1;
+      else hi = mid;
+    }
+    xf[s] = xr[lo] * rsqrt((double)dups[lo]);
+  }
+}
+}  // namespace
+
+void expand_orbits(const GroupSpec &gs, const uint64_t *reps, const uint16_t *dups, int64_t M,
+                   const double *xr, double *xf, hipStream_t s)
+{
+  const GroupDev g = to_dev(gs);
+  const uint64_t nstate = 1ull << gs.L;
+  hipLaunchKernelGGL(k_expand_orbits, dim3(grid_for(nstate, 256)), dim3(256), 0, s, g, nstate,
+                     reps, dups, M, xr, xf);
+  TM_LAUNCH_CHECK();
+}
+
 int64_t enumerate_orbits(const GroupSpec &gs, DevBuf &reps, DevBuf &dups, DevBuf *map,
                          hipStream_t s)
 {

@@ -37,6 +37,7 @@ struct SweepArgs {
   DiagSpec dg;
   double b1[2], b2[2], b3[2];
   double outfac;    // XDT: normfactor; ZBT: 1 (folded into the diagonal)
+  double db1[2], db2[2], db3[2], doutfac;  // derivatives (dual-number kernels only)
 };
 
 __device__ __forceinline__ uint32_t ins0(uint32_t q, int pos)
@@ -120,9 +121,9 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restri
     for (int i = threadIdx.x; i < 192; i += blockDim.x) stab[i] = tab[i];
   Coef<C> k;
   for (int i = 0; i < 2; ++i) {
-    k.b1[i] = (C)a.b1[i];
-    k.b2[i] = (C)a.b2[i];
-    k.b3[i] = (C)a.b3[i];
+    k.b1[i] = dev::mk<C>(a.b1[i], a.db1[i]);
+    k.b2[i] = dev::mk<C>(a.b2[i], a.db2[i]);
+    k.b3[i] = dev::mk<C>(a.b3[i], a.db3[i]);
   }
   const int L = a.L;
   const uint64_t N = 1ull << L;
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restri
   } else {
     const int NP = 1 << (a.nbits + a.logW);
     const int nz = a.nlb * a.na;
-    const C fac = (C)a.outfac * sout;
+    const C fac = dev::mk<C>(a.outfac, a.doutfac) * sout;
     for (int e = threadIdx.x; e < NP; e += blockDim.x) {
       const int w = e & (W - 1);
       const int pl = e >> a.logW;
@@ -266,11 +267,14 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restri
 
 struct SPass { int t0, ng, bitlo, nbits, logW; };
 
-std::vector<SPass> plan_sweep(int model, int L, int bc, tm_dtype comp)
+// Tiles of 128 KB (2^14 FP64, 2^15 FP32, 2^13 dual elements) on the gfx950
+// LDS, half that on a smaller one; 64-byte contiguous runs.
+std::vector<SPass> plan_sweep(int model, int L, int bc, int es)
 {
   const bool big = g_lds_per_block >= (size_t)(136 * 1024);
-  const int tl = comp == TM_F64 ? (big ? 14 : 13) : (big ? 15 : 14);
-  const int wl = comp == TM_F64 ? 3 : 4;
+  const int les = es == 4 ? 2 : (es == 8 ? 3 : 4);
+  const int tl = (big ? 17 : 16) - les;
+  const int wl = 6 - les;
   const int fixed = model == 0 ? (bc == 0 ? 2 : 1) : 4;  // log2(nlb * na)
   const int bpg = model == 0 ? 1 : 2;                      // state bits per gate
   const int ngates = model == 0 ? L : L / 2;
@@ -319,9 +323,8 @@ void run_sweep_pass(const SPass &p, SweepArgs a, const void *x, const C *zin, C 
   TM_LAUNCH_CHECK();
 }
 
-template <int MODEL, int DIR, typename C>
-void sweep_typed(const SweepPlan &plan, const void *in, void *out, tm_dtype io, void *work,
-                 const double *scale, hipStream_t s, const C *tab)
+template <int MODEL, int DIR>
+SweepArgs make_args(const SweepPlan &plan)
 {
   SweepArgs a{};
   a.L = plan.L;
@@ -342,24 +345,39 @@ void sweep_typed(const SweepPlan &plan, const void *in, void *out, tm_dtype io, 
     a.b1[i] = plan.b1[i];
     a.b2[i] = plan.b2[i];
     a.b3[i] = plan.b3[i];
+    a.db1[i] = plan.db1[i];
+    a.db2[i] = plan.db2[i];
+    a.db3[i] = plan.db3[i];
   }
   a.outfac = MODEL == 0 ? plan.nf : 1.0;
-  auto ps = plan_sweep(MODEL, plan.L, plan.bc, sizeof(C) == 8 ? TM_F64 : TM_F32);
-  const uint64_t N = 1ull << plan.L;
-  C *z0 = static_cast<C *>(work);
-  C *z1 = z0 + (uint64_t)a.nlb * a.na * N;
+  a.doutfac = MODEL == 0 ? plan.dnf : 0.0;
+  return a;
+}
+
+// TI/TO: vector types in/out (TI = TO = the io type, or double -> Dual for
+// the dual-number kernels).  The passes work in place in the aux array.
+template <int MODEL, int DIR, typename C, typename TI, typename TO>
+void sweep_run(const SweepPlan &plan, const void *in, void *out, void *work, const double *scale,
+               hipStream_t s, const C *tab)
+{
+  SweepArgs a = make_args<MODEL, DIR>(plan);
+  auto ps = plan_sweep(MODEL, plan.L, plan.bc, (int)sizeof(C));
+  C *z = static_cast<C *>(work);
   const int np = (int)ps.size();
-  C *zin = z0, *zout = z0;  // passes work in place
-  (void)z1;
   for (int i = 0; i < np; ++i) {
     a.first = (i == 0);
     a.last = (i == np - 1);
     const double *sc = (a.first || a.last) ? scale : nullptr;
-    if (io == TM_F64)
-      run_sweep_pass<MODEL, DIR, C, double, double>(ps[i], a, in, zin, zout, out, tab, sc, s);
-    else
-      run_sweep_pass<MODEL, DIR, C, float, float>(ps[i], a, in, zin, zout, out, tab, sc, s);
+    run_sweep_pass<MODEL, DIR, C, TI, TO>(ps[i], a, in, z, z, out, tab, sc, s);
   }
+}
+
+template <int MODEL, int DIR, typename C>
+void sweep_typed(const SweepPlan &plan, const void *in, void *out, tm_dtype io, void *work,
+                 const double *scale, hipStream_t s, const C *tab)
+{
+  if (io == TM_F64) sweep_run<MODEL, DIR, C, double, double>(plan, in, out, work, scale, s, tab);
+  else sweep_run<MODEL, DIR, C, float, float>(plan, in, out, work, scale, s, tab);
 }
 
 }  // namespace
@@ -369,6 +387,26 @@ size_t sweep_work_bytes(int kind, int L, tm_dtype comp)
   const size_t es = comp == TM_F64 ? 8 : 4;
   const size_t mult = kind == TM_KIND_ZBT ? 16 : 4;
   return mult * (size_t(1) << L) * es;
+}
+
+size_t sweep_dual_work_bytes(int kind, int L)
+{
+  const size_t es = sizeof(dev::Dual);
+  const size_t mult = kind == TM_KIND_ZBT ? 16 : 4;
+  return mult * (size_t(1) << L) * es;
+}
+
+void launch_sweep_dual(const SweepPlan &plan, const double *in, void *out, int transpose,
+                       void *work, const void *tab_dual, hipStream_t s)
+{
+  using dev::Dual;
+  const Dual *tab = static_cast<const Dual *>(tab_dual);
+  if (plan.kind == TM_KIND_XDT)
+    sweep_run<0, 0, Dual, double, Dual>(plan, in, out, work, nullptr, s, tab);
+  else if (transpose)
+    sweep_run<1, 1, Dual, double, Dual>(plan, in, out, work, nullptr, s, tab);
+  else
+    sweep_run<1, 0, Dual, double, Dual>(plan, in, out, work, nullptr, s, tab);
 }
 
 void launch_sweep(const SweepPlan &plan, const void *in, void *out, tm_dtype io, tm_dtype comp,
