@@ -42,8 +42,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tx-reps", type=int, default=20)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--workers", type=int, default=4, help="host threads of the scan batch")
     ap.add_argument("--profile-tx", action="store_true",
                     help="only a calibration copy and a few T.x applies (for counter runs)")
+    ap.add_argument("--workload", default="reduced", choices=("reduced", "scan", "3d"),
+                    help="reduced: the headline line; scan: 96-instance BNNNI batch; "
+                         "3d: 5x5 3D layer, reduced vs full space")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of this "
@@ -53,6 +57,11 @@ def main():
     import numpy as np
 
     import latttmat_amd as lt
+
+    if args.workload == "scan":
+        return bench_scan(lt, args)
+    if args.workload == "3d":
+        return bench_3d(lt, args)
 
     L, bJ, kappa, neigs = args.L, 0.4, 0.5, 4
     t0 = time.perf_counter()
@@ -165,6 +174,126 @@ def main():
         },
         "cpu_baseline": cpu,
         "setup_s": round(setup_s, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def scan_instances():
+    """96 BNNNI/3NN instances: instance i takes grid point i mod 54 of
+    betaJ 0.2..1.0 (step 0.1) x kappa1 {0.2, 0.5, 0.8} x kappa2 {0, 0.5}
+    (lexicographic) and width (14, 16, 18, 20, 22)[i mod 5]; h = 0."""
+    import itertools
+    bJs = [round(0.2 + 0.1 * i, 10) for i in range(9)]
+    grid = list(itertools.product(bJs, (0.2, 0.5, 0.8), (0.0, 0.5)))
+    return [(grid[i % 54], (14, 16, 18, 20, 22)[i % 5]) for i in range(96)]
+
+
+def bench_scan(lt, args):
+    """Batched parameter scan: 96 ZBT eigensolves (neigs 2) of uneven widths.
+    Timed: one by one in FP64, the batch in FP64, the batch in mixed
+    precision.  value = instances/s of the mixed batch."""
+    import numpy as np
+    inst = scan_instances()
+    t0 = time.perf_counter()
+    ops = [lt.TransferMatrix("B", 2, 3, L, J=bJ, kappa=k1, kappa2=k2) for (bJ, k1, k2), L in inst]
+    setup = time.perf_counter() - t0
+    states = sum(op.cols() for op in ops)
+    log(f"[scan] {len(ops)} operators, {states} states in total, setup {setup:.2f}s")
+    # warm-up: one of each width
+    seen = set()
+    for op, (_, L) in zip(ops, inst):
+        if L not in seen:
+            seen.add(L)
+            lt.solve(op, nev=2)
+            lt.solve(op, nev=2, precision=lt.TM_PREC_MIXED)
+    ts = time.perf_counter()
+    seq = [lt.solve(op, nev=2) for op in ops]
+    t_seq = time.perf_counter() - ts
+    log(f"[scan] sequential fp64 {t_seq:.3f}s")
+    ts = time.perf_counter()
+    b64 = lt.solve_batch(ops, nev=2, nworkers=args.workers)
+    t_b64 = time.perf_counter() - ts
+    log(f"[scan] batch fp64 {t_b64:.3f}s")
+    ts = time.perf_counter()
+    bmx = lt.solve_batch(ops, nev=2, precision=lt.TM_PREC_MIXED, nworkers=args.workers)
+    t_bmx = time.perf_counter() - ts
+    log(f"[scan] batch mixed {t_bmx:.3f}s")
+    err = max(abs(m.evals[0] / f.evals[0] - 1) for m, f in zip(bmx, seq))
+    err_b = max(abs(m.evals[0] / f.evals[0] - 1) for m, f in zip(b64, seq))
+    mv = dict(seq_f64=int(sum(r.matvecs_f64 for r in seq)),
+              mixed_f32=int(sum(r.matvecs_f32 for r in bmx)),
+              mixed_f64=int(sum(r.matvecs_f64 for r in bmx)))
+    per_L = {}
+    for r, (_, L) in zip(seq, inst):
+        per_L.setdefault(L, []).append(r.seconds)
+    out = {
+        "metric": "BNNNI parameter scan: eigensolves/s over 96 instances (batch, mixed precision)",
+        "value": round(len(ops) / t_bmx, 3), "unit": "eigensolves/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "fp32 T.x + fp64 Krylov/refinement",
+        "data": "synthetic (model parameters)",
+        "config": {"workload": "BNNNI/3NN diagonal TM (B, axialdir 3), 96 instances, "
+                               "L in {14,16,18,20,22}, betaJ 0.2..1.0 x kappa1 {0.2,0.5,0.8} x "
+                               "kappa2 {0,0.5}, h=0, neigs=2", "states_total": int(states),
+                   "workers": args.workers},
+        "sequential_fp64_per_s": round(len(ops) / t_seq, 3),
+        "batch_fp64_per_s": round(len(ops) / t_b64, 3),
+        "speedup_batch_mixed_vs_sequential_fp64": round(t_seq / t_bmx, 3),
+        "max_rel_err_lambda0_mixed_vs_fp64": err, "max_rel_err_lambda0_batch_vs_seq": err_b,
+        "matvecs": mv,
+        "mean_solve_s_by_width": {str(L): round(float(np.mean(v)), 5) for L, v in sorted(per_L.items())},
+        "all_converged": bool(all(r.nconv >= 2 for r in bmx + b64 + seq)),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_3d(lt, args):
+    """3D ANNNI 5x5 layer, betaJ 0.25, kappa 0.4, h 0, neigs 2: the
+    reference's reduced operator (170,440 orbits, FP64) against the new
+    full-space operator (2^25 states) in FP64 and mixed precision with a
+    20-vector Krylov basis; lambda0 must agree to 1e-12."""
+    import numpy as np
+    kw = dict(J=0.25, kappa=0.4, neigs=2)
+    t0 = time.perf_counter()
+    red = lt.TransferMatrix("A", 3, 1, (5, 5), **kw)
+    full = lt.TransferMatrix("A", 3, 1, (5, 5), flags=lt.TM_FLAG_FULLSPACE, **kw)
+    log(f"[3d] reduced M={red.cols()} full N={full.cols()} setup {time.perf_counter() - t0:.2f}s")
+    lt.solve(red, nev=2)
+    lt.solve(full, nev=2, ncv=20, precision=lt.TM_PREC_MIXED)
+    reps = max(1, args.steps)
+    def timed(fn):
+        ts = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        return r, (time.perf_counter() - ts) / reps
+    rr, t_red = timed(lambda: lt.solve(red, nev=2))
+    rf, t_f64 = timed(lambda: lt.solve(full, nev=2, ncv=20))
+    rm, t_mix = timed(lambda: lt.solve(full, nev=2, ncv=20, precision=lt.TM_PREC_MIXED))
+    ms32 = full.time_apply(lt.TM_F64, lt.TM_F32, warmup=2, reps=10)
+    ms64 = full.time_apply(lt.TM_F64, lt.TM_F64, warmup=2, reps=10)
+    N = full.cols()
+    gbs32 = N * 8 / (ms32 * 1e-3) / 1e9
+    gbs64 = N * 16 / (ms64 * 1e-3) / 1e9
+    l0r, l0f, l0m = rr.evals[0], rf.evals[0], rm.evals[0]
+    lam_f = np.sort(np.abs(rf.evals))[::-1]
+    lam_m = np.sort(np.abs(rm.evals))[::-1]
+    out = {
+        "metric": "3D 5x5 ANNNI eigensolves/s (full 2^25 mixed, 20-vector basis)",
+        "value": round(1.0 / t_mix, 4), "unit": "eigensolves/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "fp32 T.x + fp64 Krylov/refinement",
+        "data": "synthetic (model parameters)",
+        "config": {"workload": "3D ANNNI 5x5 periodic layer, betaJ 0.25, kappa 0.4, h 0, neigs 2",
+                   "full_states": int(N), "reduced_orbits": int(red.cols()), "ncv_full": 20},
+        "reduced_fp64_s": round(t_red, 4), "full_fp64_s": round(t_f64, 4),
+        "full_mixed_s": round(t_mix, 4),
+        "lambda0": {"reduced": l0r, "full_fp64": l0f, "full_mixed": l0m},
+        "rel_err_lambda0": {"full_fp64_vs_reduced": abs(l0f / l0r - 1),
+                            "full_mixed_vs_full_fp64": abs(l0m / l0f - 1)},
+        "rel_err_lambda1_mixed_vs_fp64": abs(lam_m[1] / lam_f[1] - 1),
+        "matvecs": {"reduced": rr.matvecs_f64, "full_fp64": rf.matvecs_f64,
+                    "full_mixed_f32": rm.matvecs_f32, "full_mixed_f64": rm.matvecs_f64},
+        "tx_full": {"fp32_ms": round(ms32, 4), "fp64_ms": round(ms64, 4),
+                    "fp32_GBs": round(gbs32, 1), "fp64_GBs": round(gbs64, 1),
+                    "fp32_frac_hbm": round(gbs32 / HBM_PEAK_GBS, 4)},
     }
     print(json.dumps(out), flush=True)
 

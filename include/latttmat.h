@@ -129,7 +129,7 @@ tm_status tm_op_orbits(const tm_op *op, uint64_t *reps, uint16_t *dups);
 
 /* Options of the eigen driver.  Defaults (tm_solve_opts_default): nev=1,
  * ncv=0 (=suggested_nev), tol=1e-10, maxit=1000, maxtrial=3 (pfunc.hpp:87),
- * precision FP64, seed 1, want_left=0. */
+ * precision FP64, seed 1, want_left=0, use_graph=1. */
 typedef struct {
   int          nev;
   int          ncv;
@@ -148,6 +148,9 @@ typedef struct {
                                when r^2/(gap |theta|) <= 1e-13 (symmetric only);
                                2 = Krylov-Schur warm-started from the FP32 Ritz
                                vectors, run to tol; 0 = 1 if symmetric else 2 */
+  int          use_graph;   /* 1 (default): restart cycles after the first
+                               are captured once into a hipGraph and
+                               replayed; 0: launched eagerly            */
 } tm_solve_opts;
 
 /* Results.  Eigenvalues are those of the operator as applied (the reference
@@ -168,11 +171,23 @@ typedef struct {
   int64_t  restarts;
   int      trials;
   double   seconds;         /* wall time of the solve (setup excluded)        */
+  int64_t  graph_cycles;    /* restart cycles replayed from a hipGraph        */
 } tm_solve_out;
 
 void      tm_solve_opts_default(tm_solve_opts *o);
 /* pfunc_inner's solve + retry loop (TMat/pfunc.hpp:87-114) on the GPU. */
 tm_status tm_solve(tm_op *op, const tm_solve_opts *opts, tm_solve_out *out);
+
+/* Solve n independent operators (any kinds and sizes) concurrently:
+ * instances are taken largest first by nworkers host threads (0 = 4), each
+ * driving the instance's own HIP stream, so small solves run beside the
+ * large ones; every instance's restart cycles are graph-replayed as in
+ * tm_solve.  opts[i] / outs[i] as for tm_solve.  Returns the first non-OK
+ * status (every instance is attempted; tm_last_error names the instance).
+ * No reference counterpart (the reference solves one parameter point per
+ * run, TMat/main.cpp:31-58). */
+tm_status tm_solve_batch(tm_op *const *ops, int n, const tm_solve_opts *opts,
+                         tm_solve_out *outs, int nworkers);
 
 /* Hellmann-Feynman derivative of the leading eigenvalue of the operator as
  * applied, with respect to (betaJ, kappa, kappa2, h):

@@ -25,10 +25,11 @@ from ._abi import (  # noqa: F401
     exported_symbols,
     lib,
     solve,
+    solve_batch,
 )
 
 __all__ = [
     "LIB_PATH", "TM_F32", "TM_F64", "TM_FLAG_FULLSPACE", "TM_FLAG_GENERIC_REDUCED",
     "TM_PREC_F64", "TM_PREC_MIXED", "TMError", "TransferMatrix", "SolveResult",
-    "exported_symbols", "lib", "solve",
+    "exported_symbols", "lib", "solve", "solve_batch",
 ]

@@ -23,7 +23,8 @@ EXPORTS = [
     "tm_op_is_symmetric", "tm_op_kind", "tm_op_suggested_ncv", "tm_op_layer_size",
     "tm_op_matvec_count", "tm_op_apply", "tm_op_apply_dev", "tm_op_apply_prec",
     "tm_op_time_apply", "tm_op_orbits",
-    "tm_solve_opts_default", "tm_solve", "tm_grad_hf", "tm_last_error", "tm_version",
+    "tm_solve_opts_default", "tm_solve", "tm_solve_batch", "tm_grad_hf", "tm_last_error",
+    "tm_version",
 ]
 
 
@@ -46,7 +47,7 @@ class SolveOpts(C.Structure):
         ("nev", C.c_int), ("ncv", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
         ("maxtrial", C.c_int), ("precision", C.c_int), ("seed", C.c_uint64),
         ("want_left", C.c_int), ("want_vectors", C.c_int), ("mixed_tol", C.c_double),
-        ("verbose", C.c_int), ("refine", C.c_int),
+        ("verbose", C.c_int), ("refine", C.c_int), ("use_graph", C.c_int),
     ]
 
 
@@ -57,6 +58,7 @@ class SolveOut(C.Structure):
         ("vecs", C.POINTER(C.c_double)), ("vecl", C.POINTER(C.c_double)),
         ("eval_left", C.c_double), ("matvecs_f64", C.c_int64), ("matvecs_f32", C.c_int64),
         ("restarts", C.c_int64), ("trials", C.c_int), ("seconds", C.c_double),
+        ("graph_cycles", C.c_int64),
     ]
 
 
@@ -107,6 +109,8 @@ def lib():
         L.tm_solve_opts_default.argtypes = [C.POINTER(SolveOpts)]
         L.tm_solve.restype = i32
         L.tm_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(SolveOut)]
+        L.tm_solve_batch.restype = i32
+        L.tm_solve_batch.argtypes = [vp, i32, C.POINTER(SolveOpts), C.POINTER(SolveOut), i32]
         L.tm_grad_hf.restype = i32
         L.tm_grad_hf.argtypes = [vp, pd, pd, pd, pd]
         L.tm_last_error.restype = C.c_char_p
@@ -258,35 +262,79 @@ class SolveResult:
     extra: dict = field(default_factory=dict)
 
 
-def solve(op: TransferMatrix, nev=1, ncv=0, tol=1e-10, maxit=1000, precision=TM_PREC_F64,
-          seed=1, want_left=False, want_vectors=False, mixed_tol=2e-7, verbose=False,
-          maxtrial=3, raise_on_noconv=True, refine=0) -> SolveResult:
+def _make_opts(nev, ncv, tol, maxit, precision, seed, want_left, want_vectors, mixed_tol,
+               verbose, maxtrial, refine, use_graph):
     o = SolveOpts()
     lib().tm_solve_opts_default(C.byref(o))
     o.nev, o.ncv, o.tol, o.maxit, o.maxtrial = nev, ncv, tol, maxit, maxtrial
     o.precision, o.seed = precision, seed
     o.want_left, o.want_vectors = int(want_left), int(want_vectors)
     o.mixed_tol, o.verbose, o.refine = mixed_tol, int(verbose), int(refine)
-    n = op.cols()
-    er, ei, rs = np.zeros(nev), np.zeros(nev), np.zeros(nev)
-    sym = op.is_symmetric()
-    vecs = None
-    if want_vectors:
-        vecs = np.zeros(nev * n if sym else 2 * nev * n)
-    vecl = np.zeros(n) if (want_left and not sym and want_vectors) else None
-    out = SolveOut()
-    out.evals_re, out.evals_im, out.resid = _pd(er), _pd(ei), _pd(rs)
-    out.vecs = _pd(vecs) if vecs is not None else None
-    out.vecl = _pd(vecl) if vecl is not None else None
-    st = lib().tm_solve(op._h, C.byref(o), C.byref(out))
+    o.use_graph = int(use_graph)
+    return o
+
+
+class _OutBufs:
+    """Host buffers behind one SolveOut."""
+
+    def __init__(self, op, nev, want_left, want_vectors):
+        n = op.cols()
+        self.n, self.nev = n, nev
+        self.sym = op.is_symmetric()
+        self.er, self.ei, self.rs = np.zeros(nev), np.zeros(nev), np.zeros(nev)
+        self.vecs = None
+        if want_vectors:
+            self.vecs = np.zeros(nev * n if self.sym else 2 * nev * n)
+        self.vecl = np.zeros(n) if (want_left and not self.sym and want_vectors) else None
+        out = SolveOut()
+        out.evals_re, out.evals_im, out.resid = _pd(self.er), _pd(self.ei), _pd(self.rs)
+        out.vecs = _pd(self.vecs) if self.vecs is not None else None
+        out.vecl = _pd(self.vecl) if self.vecl is not None else None
+        self.out = out
+
+    def result(self, st):
+        out, n, nev = self.out, self.n, self.nev
+        evals = self.er + 1j * self.ei if not self.sym else self.er
+        vecs = self.vecs
+        if vecs is not None:
+            if self.sym:
+                vecs = vecs.reshape(nev, n).T.copy()
+            else:
+                v = vecs.reshape(nev, n, 2)
+                vecs = (v[..., 0] + 1j * v[..., 1]).T.copy()
+        return SolveResult(st, out.nconv, evals, self.rs, vecs, self.vecl, out.eval_left,
+                           out.matvecs_f64, out.matvecs_f32, out.restarts, out.trials,
+                           out.seconds, {"graph_cycles": int(out.graph_cycles)})
+
+
+def solve(op: TransferMatrix, nev=1, ncv=0, tol=1e-10, maxit=1000, precision=TM_PREC_F64,
+          seed=1, want_left=False, want_vectors=False, mixed_tol=2e-7, verbose=False,
+          maxtrial=3, raise_on_noconv=True, refine=0, use_graph=True) -> SolveResult:
+    o = _make_opts(nev, ncv, tol, maxit, precision, seed, want_left, want_vectors, mixed_tol,
+                   verbose, maxtrial, refine, use_graph)
+    b = _OutBufs(op, nev, want_left, want_vectors)
+    st = lib().tm_solve(op._h, C.byref(o), C.byref(b.out))
     if st not in (TM_OK, TM_ENOCONV) or (st == TM_ENOCONV and raise_on_noconv):
         _check(st)
-    evals = er + 1j * ei if not sym else er
-    if vecs is not None:
-        if sym:
-            vecs = vecs.reshape(nev, n).T.copy()
-        else:
-            v = vecs.reshape(nev, n, 2)
-            vecs = (v[..., 0] + 1j * v[..., 1]).T.copy()
-    return SolveResult(st, out.nconv, evals, rs, vecs, vecl, out.eval_left, out.matvecs_f64,
-                       out.matvecs_f32, out.restarts, out.trials, out.seconds)
+    return b.result(st)
+
+
+def solve_batch(ops, nev=1, ncv=0, tol=1e-10, maxit=1000, precision=TM_PREC_F64, seed=1,
+                want_left=False, want_vectors=False, mixed_tol=2e-7, maxtrial=3, refine=0,
+                use_graph=True, nworkers=4, raise_on_error=True):
+    """tm_solve_batch over a list of operators with one set of options."""
+    n = len(ops)
+    o = _make_opts(nev, ncv, tol, maxit, precision, seed, want_left, want_vectors, mixed_tol,
+                   False, maxtrial, refine, use_graph)
+    opts = (SolveOpts * n)(*([o] * n))
+    bufs = [_OutBufs(op, nev, want_left, want_vectors) for op in ops]
+    outs = (SolveOut * n)(*[b.out for b in bufs])
+    hs = (C.c_void_p * n)(*[op._h for op in ops])
+    st = lib().tm_solve_batch(hs, n, opts, outs, nworkers)
+    if st != TM_OK and raise_on_error:
+        _check(st)
+    res = []
+    for i, b in enumerate(bufs):
+        b.out = outs[i]
+        res.append(b.result(TM_OK if outs[i].nconv >= nev else TM_ENOCONV))
+    return res

@@ -1,8 +1,14 @@
 // tm_api.cpp -- the C ABI (include/latttmat.h).  Every entry point maps
 // C++ exceptions to a tm_status and a thread-local message.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <numeric>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "tm_internal.hpp"
 
@@ -173,8 +179,8 @@ tm_status tm_op_orbits(const tm_op *op, uint64_t *reps, uint16_t *dups)
   if (op->kind != TM_KIND_XAM2D && op->kind != TM_KIND_XAM3D)
     return fail(TM_EINVAL, "operator has no orbit tables (not a reduced operator)");
   return guarded([&] {
-    TM_HIP(hipMemcpy(reps, op->reps.get(), sizeof(uint64_t) * op->ncol, hipMemcpyDeviceToHost));
-    TM_HIP(hipMemcpy(dups, op->dups.get(), sizeof(uint16_t) * op->ncol, hipMemcpyDeviceToHost));
+    ltm::copy_sync(reps, op->reps.get(), sizeof(uint64_t) * op->ncol, hipMemcpyDeviceToHost, op->stream);
+    ltm::copy_sync(dups, op->dups.get(), sizeof(uint16_t) * op->ncol, hipMemcpyDeviceToHost, op->stream);
     return TM_OK;
   });
 }
@@ -194,6 +200,7 @@ void tm_solve_opts_default(tm_solve_opts *o)
   o->want_vectors = 0;
   o->mixed_tol = 2e-7;
   o->verbose = 0;
+  o->use_graph = 1;
   o->refine = 0;
 }
 
@@ -207,6 +214,46 @@ tm_status tm_solve(tm_op *op, const tm_solve_opts *opts, tm_solve_out *out)
   });
 }
 
+tm_status tm_solve_batch(tm_op *const *ops, int n, const tm_solve_opts *opts, tm_solve_out *outs,
+                         int nworkers)
+{
+  if (n < 0 || (n > 0 && (!ops || !opts || !outs))) return fail(TM_EINVAL, "null argument");
+  for (int i = 0; i < n; ++i)
+    if (!ops[i]) return fail(TM_EINVAL, "null operator in batch");
+  if (n == 0) return TM_OK;
+  // largest first: the long solves start at once, the short ones fill in
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return ops[a]->ncol > ops[b]->ncol; });
+  const int nw = std::max(1, std::min(nworkers > 0 ? nworkers : 4, n));
+  std::atomic<int> next{0};
+  std::mutex mu;
+  tm_status first = TM_OK;
+  std::string first_msg;
+  auto worker = [&] {
+    for (;;) {
+      const int q = next.fetch_add(1);
+      if (q >= n) break;
+      const int i = order[q];
+      const tm_status st = tm_solve(ops[i], &opts[i], &outs[i]);
+      if (st != TM_OK) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (first == TM_OK) {
+          first = st;
+          first_msg = "instance " + std::to_string(i) + ": " + g_err;
+        }
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int w = 1; w < nw; ++w) th.emplace_back(worker);
+  worker();
+  for (auto &t : th) t.join();
+  if (first != TM_OK) g_err = first_msg;
+  return first;
+}
+
 tm_status tm_grad_hf(const tm_op *cop, const double *l, const double *r, double dlam[4],
                      double dlnnf[4])
 {
@@ -215,8 +262,8 @@ tm_status tm_grad_hf(const tm_op *cop, const double *l, const double *r, double 
   return guarded([&] {
     const size_t bytes = sizeof(double) * op->ncol;
     ltm::DevBuf dl(bytes), dr(bytes);
-    TM_HIP(hipMemcpy(dl.get(), l, bytes, hipMemcpyHostToDevice));
-    TM_HIP(hipMemcpy(dr.get(), r, bytes, hipMemcpyHostToDevice));
+    ltm::copy_sync(dl.get(), l, bytes, hipMemcpyHostToDevice, op->stream);
+    ltm::copy_sync(dr.get(), r, bytes, hipMemcpyHostToDevice, op->stream);
     double num[5];
     ltm::grad_numerators(op, dl.as<double>(), dr.as<double>(), num);
     // num[4] = <l, r>

@@ -49,6 +49,24 @@ struct Error : std::runtime_error {
 #define TM_LAUNCH_CHECK() TM_HIP(hipGetLastError())
 
 // ---------------------------------------------------------------------------
+// Stream-ordered copy that returns once the data has arrived.  Operator
+// streams are non-blocking, so nothing here uses the legacy null stream
+// (whose implicit synchronisation would break graph capture on other
+// threads).
+void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
+
+// Graph capture and device allocation exclude each other across threads:
+// a capture holds this lock exclusively (and marks its thread), allocations
+// and frees hold it shared.  An allocation on the capturing thread throws.
+struct CaptureGuard {
+  CaptureGuard();
+  ~CaptureGuard();
+  CaptureGuard(const CaptureGuard &) = delete;
+  CaptureGuard &operator=(const CaptureGuard &) = delete;
+};
+// hipGraphExecDestroy under the shared side of the same lock
+void destroy_graph_exec(hipGraphExec_t g);
+
 // device buffer (RAII)
 // ---------------------------------------------------------------------------
 class DevBuf {
@@ -216,6 +234,15 @@ void update_dots(const double *V, int64_t ldv, int j, double *w, int64_t n,
 // w -= sum_k h[k] V_k; out[0] = <w,w>
 void update_norm(const double *V, int64_t ldv, int j, double *w, int64_t n,
                  const double *h, double *partial, double *out, hipStream_t s);
+// if !(w1[0] > w0[0]/2): w -= sum_k h[k] V_k, out[0] = <w,w>; else nothing
+void update_norm_cond(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h,
+                      double *partial, double *out, const double *w0, const double *w1,
+                      hipStream_t s);
+// finish Krylov step j on the device: projected-matrix column j (+ subdiag)
+// into Hd (ld ldh), nrm[0] = beta^2, flags[0] += correction, flags[1] = first
+// breakdown step (must start < 0)
+void step_finish(int j, const double *hb1, const double *hb2, const double *hb3, double *Hd,
+                 int ldh, double *nrm, double *flags, hipStream_t s);
 // v = w * (1/sqrt(nrm2[0]))
 void normalize(double *w, int64_t n, const double *nrm2, hipStream_t s);
 // Vout[:, c] = sum_r Vin[:, r] * Y[r + c*ldy], r < m, c < k  (Vout may alias Vin)

@@ -125,6 +125,77 @@ __global__ __launch_bounds__(64) void k_reduce_partials(const double *__restrict
   if (threadIdx.x == 0) out[k] = s;
 }
 
+// DGKS correction only when the first projection removed more than half of
+// the norm^2 (w1 <= w0 / 2, or not finite); every block reads the same two
+// device values, so the whole grid takes the same branch.
+__device__ __forceinline__ bool need_reorth(const double *w0p, const double *w1p)
+{
+  return !(w1p[0] > 0.5 * w0p[0]);
+}
+
+template <int J>
+__global__ __launch_bounds__(THREADS) void k_update_norm_cond(const double *__restrict__ V,
+                                                              int64_t ldv, int j,
+                                                              double *__restrict__ w, int64_t n,
+                                                              const double *__restrict__ h,
+                                                              double *__restrict__ partial,
+                                                              const double *w0p, const double *w1p)
+{
+  if (!need_reorth(w0p, w1p)) return;
+  __shared__ double hk[J];
+  for (int k = threadIdx.x; k < J; k += blockDim.x) hk[k] = k < j ? h[k] : 0.0;
+  __syncthreads();
+  double acc[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * THREADS) {
+    double wi = w[i];
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k < j) wi -= hk[k] * V[k * ldv + i];
+    w[i] = wi;
+    acc[0] += wi * wi;
+  }
+  block_reduce_write<1>(acc, 1, partial + (int64_t)blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void k_reduce_partials_cond(const double *__restrict__ partial,
+                                                             int nblk, double *__restrict__ out,
+                                                             const double *w0p, const double *w1p)
+{
+  if (!need_reorth(w0p, w1p)) return;
+  double s = 0;
+  for (int b = threadIdx.x; b < nblk; b += 64) s += partial[b];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+// Close Krylov step j on the device (one thread): column j of the projected
+// matrix (ld ldh) = first projection + correction, the subdiagonal
+// beta = ||w||, nrm[0] = beta^2 for the normalisation, flags[0] counts
+// corrections, flags[1] records the first step whose beta fell below
+// 1e-13 ||h|| (an invariant subspace; the host redoes the cycle from there).
+__global__ void k_step_finish(int j, const double *__restrict__ hb1,
+                              const double *__restrict__ hb2, const double *__restrict__ hb3,
+                              double *__restrict__ Hd, int ldh, double *__restrict__ nrm,
+                              double *__restrict__ flags)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int nk = j + 1;
+  const bool re = need_reorth(hb1 + nk, hb2 + nk);
+  double hn2 = 0;
+  for (int k = 0; k < nk; ++k) {
+    const double hk = hb1[k] + (re ? hb2[k] : 0.0);
+    Hd[k + (int64_t)j * ldh] = hk;
+    hn2 += hk * hk;
+  }
+  const double beta2 = re ? hb3[0] : hb2[nk];
+  const double beta = sqrt(fmax(beta2, 0.0));
+  Hd[nk + (int64_t)j * ldh] = beta;
+  nrm[0] = beta2;
+  if (re) flags[0] += 1.0;
+  if ((beta <= 1e-13 * sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
+}
+
 __global__ void k_normalize(double *__restrict__ w, int64_t n, const double *__restrict__ nrm2)
 {
   const double inv = 1.0 / sqrt(nrm2[0]);
@@ -293,6 +364,26 @@ void update_norm(const double *V, int64_t ldv, int j, double *w, int64_t n, cons
   const unsigned nb = nblocks(n);
   TM_JDISPATCH(k_update_dots, j, dim3(nb), V, ldv, j, w, n, h, partial, 0);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, (int)nb, 1, out);
+  TM_LAUNCH_CHECK();
+}
+
+void update_norm_cond(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h,
+                      double *partial, double *out, const double *w0, const double *w1,
+                      hipStream_t s)
+{
+  TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
+  const unsigned nb = nblocks(n);
+  TM_JDISPATCH(k_update_norm_cond, j, dim3(nb), V, ldv, j, w, n, h, partial, w0, w1);
+  hipLaunchKernelGGL(k_reduce_partials_cond, dim3(1), dim3(64), 0, s, partial, (int)nb, out, w0,
+                     w1);
+  TM_LAUNCH_CHECK();
+}
+
+void step_finish(int j, const double *hb1, const double *hb2, const double *hb3, double *Hd,
+                 int ldh, double *nrm, double *flags, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(64), 0, s, j, hb1, hb2, hb3, Hd, ldh, nrm,
+                     flags);
   TM_LAUNCH_CHECK();
 }
 

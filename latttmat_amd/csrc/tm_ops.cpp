@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
+#include <shared_mutex>
 #include <vector>
 
 #include "tm_consts.hpp"
@@ -20,10 +22,41 @@ size_t g_lds_per_block = 64 * 1024;
   throw Error(e == hipErrorOutOfMemory ? TM_ENOMEM : TM_EHIP, buf);
 }
 
+namespace {
+std::shared_mutex g_capture_mu;
+thread_local bool t_capturing = false;
+}  // namespace
+
+CaptureGuard::CaptureGuard()
+{
+  g_capture_mu.lock();
+  t_capturing = true;
+}
+CaptureGuard::~CaptureGuard()
+{
+  t_capturing = false;
+  g_capture_mu.unlock();
+}
+
+void destroy_graph_exec(hipGraphExec_t g)
+{
+  if (!g) return;
+  std::shared_lock<std::shared_mutex> lk(g_capture_mu);
+  (void)hipGraphExecDestroy(g);
+}
+
+void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s)
+{
+  TM_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
+  TM_HIP(hipStreamSynchronize(s));
+}
+
 void DevBuf::alloc(size_t bytes)
 {
   release();
   if (bytes == 0) return;
+  if (t_capturing) throw Error(TM_EHIP, "device allocation during graph capture");
+  std::shared_lock<std::shared_mutex> lk(g_capture_mu);
   hipError_t e = hipMalloc(&p_, bytes);
   if (e != hipSuccess) {
     p_ = nullptr;
@@ -37,7 +70,14 @@ void DevBuf::alloc(size_t bytes)
 
 void DevBuf::release()
 {
-  if (p_) (void)hipFree(p_);
+  if (p_) {
+    if (t_capturing) {
+      (void)hipFree(p_);
+    } else {
+      std::shared_lock<std::shared_mutex> lk(g_capture_mu);
+      (void)hipFree(p_);
+    }
+  }
   p_ = nullptr;
   n_ = 0;
 }
@@ -60,8 +100,8 @@ void upload_tables(tm_op *op, const std::vector<double> &t)
   std::vector<float> f(t.begin(), t.end());
   op->tab64.alloc(192 * sizeof(double));
   op->tab32.alloc(192 * sizeof(float));
-  TM_HIP(hipMemcpy(op->tab64.get(), t.data(), 192 * sizeof(double), hipMemcpyHostToDevice));
-  TM_HIP(hipMemcpy(op->tab32.get(), f.data(), 192 * sizeof(float), hipMemcpyHostToDevice));
+  copy_sync(op->tab64.get(), t.data(), 192 * sizeof(double), hipMemcpyHostToDevice, op->stream);
+  copy_sync(op->tab32.get(), f.data(), 192 * sizeof(float), hipMemcpyHostToDevice, op->stream);
 }
 
 // normfactor of the X-type operators; exponent (units of betaJ) in *e
@@ -166,8 +206,8 @@ void build_reduced(tm_op *op, const GroupSpec &g, double nfexp, bool want_fused)
   op->ncol = M;
   std::vector<uint64_t> reps(M);
   std::vector<uint16_t> dups(M);
-  TM_HIP(hipMemcpy(reps.data(), op->reps.get(), M * 8, hipMemcpyDeviceToHost));
-  TM_HIP(hipMemcpy(dups.data(), op->dups.get(), M * 2, hipMemcpyDeviceToHost));
+  copy_sync(reps.data(), op->reps.get(), M * 8, hipMemcpyDeviceToHost, op->stream);
+  copy_sync(dups.data(), op->dups.get(), M * 2, hipMemcpyDeviceToHost, op->stream);
   std::vector<double> tv(M), dt(M);
   const double Jn = 0.5 * op->p.betaJ * nfexp;
   for (int64_t o = 0; o < M; ++o) {
@@ -182,10 +222,10 @@ void build_reduced(tm_op *op, const GroupSpec &g, double nfexp, bool want_fused)
   op->dt64.alloc(M * 8);
   op->tvec32.alloc(M * 4);
   op->dt32.alloc(M * 4);
-  TM_HIP(hipMemcpy(op->tvec.get(), tv.data(), M * 8, hipMemcpyHostToDevice));
-  TM_HIP(hipMemcpy(op->dt64.get(), dt.data(), M * 8, hipMemcpyHostToDevice));
-  TM_HIP(hipMemcpy(op->tvec32.get(), tv32.data(), M * 4, hipMemcpyHostToDevice));
-  TM_HIP(hipMemcpy(op->dt32.get(), dt32.data(), M * 4, hipMemcpyHostToDevice));
+  copy_sync(op->tvec.get(), tv.data(), M * 8, hipMemcpyHostToDevice, op->stream);
+  copy_sync(op->dt64.get(), dt.data(), M * 8, hipMemcpyHostToDevice, op->stream);
+  copy_sync(op->tvec32.get(), tv32.data(), M * 4, hipMemcpyHostToDevice, op->stream);
+  copy_sync(op->dt32.get(), dt32.data(), M * 4, hipMemcpyHostToDevice, op->stream);
   if (fused) {
     op->ft.reset(new FusedTables());
     build_fused_tables(g, op->reps, M, *op->ft, op->stream);
@@ -201,6 +241,9 @@ tm_op *make_op(const tm_params &p)
            "invalid model (expected A, B or D)");
   TM_CHECK(p.dim == 2 || p.dim == 3, TM_EINVAL, "invalid dim (expected 2 or 3)");
   std::unique_ptr<tm_op> op(new tm_op());
+  // one stream per operator (default priority): its applies and solves are
+  // stream-ordered, and independent operators run concurrently
+  TM_HIP(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
   op->p = p;
   set_bz(op.get());
   const int multieigs = p.neigs > 1 ? 2 : 0;  // TMat/pfunc.cpp:22
@@ -312,13 +355,16 @@ tm_op *make_op(const tm_params &p)
   }
   op->one.alloc(sizeof(double));
   const double one = 1.0;
-  TM_HIP(hipMemcpy(op->one.get(), &one, sizeof one, hipMemcpyHostToDevice));
+  copy_sync(op->one.get(), &one, sizeof one, hipMemcpyHostToDevice, op->stream);
   return op.release();
 }
 
 }  // namespace ltm
 
-tm_op::~tm_op() = default;
+tm_op::~tm_op()
+{
+  if (stream) (void)hipStreamDestroy(stream);
+}
 
 void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
                       int transpose, const double *scale_in, hipStream_t s)

@@ -440,7 +440,7 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   // evaluation point of every orbit: rotate the representative so that its
   // odd-bit part becomes a necklace representative
   std::vector<uint64_t> hreps(M);
-  TM_HIP(hipMemcpy(hreps.data(), reps.get(), M * 8, hipMemcpyDeviceToHost));
+  copy_sync(hreps.data(), reps.get(), M * 8, hipMemcpyDeviceToHost, s);
   auto compact = [&](uint64_t v) {  // bits 0,2,4,... -> 0,1,2,...
     uint32_t r = 0;
     for (int i = 0; i < n; ++i) r |= (uint32_t)((v >> (2 * i)) & 1) << i;
@@ -466,7 +466,7 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   }
   auto up = [&](DevBuf &d, const void *src, size_t bytes) {
     d.alloc(bytes);
-    TM_HIP(hipMemcpy(d.get(), src, bytes, hipMemcpyHostToDevice));
+    copy_sync(d.get(), src, bytes, hipMemcpyHostToDevice, s);
   };
   up(ft.NR, NR.data(), NR.size() * 2);
   up(ft.period, per.data(), per.size());

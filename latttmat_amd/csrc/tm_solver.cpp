@@ -55,7 +55,17 @@ class KrylovSchur {
     hbuf_.alloc(sizeof(double) * 3 * 72);
     Ydev_.alloc(sizeof(double) * 72 * 72);
     H_.assign((size_t)(m_ + 1) * m_, 0.0);
+    Hd_.alloc(sizeof(double) * ((size_t)(m_ + 1) * m_ + 8));
   }
+
+  ~KrylovSchur()
+  {
+    for (auto &g : graphs_) destroy_graph_exec(g.exec);
+  }
+  KrylovSchur(const KrylovSchur &) = delete;
+  KrylovSchur &operator=(const KrylovSchur &) = delete;
+  void set_graph(bool on) { use_graph_ = on; }
+  int64_t graph_cycles() const { return graph_cycles_; }
 
   double *V(int k) const { return V_.as<double>() + (size_t)k * ldv_; }
   int64_t ldv() const { return ldv_; }
@@ -85,10 +95,117 @@ class KrylovSchur {
   // Expand the Krylov-Schur decomposition from column k_ to m_.
   void expand()
   {
+    if (device_steps_) expand_device();
+    else expand_host(k_);
+  }
+
+  // The whole cycle without host synchronisation: each step's correction
+  // decision, projected-matrix column and norm stay on the device
+  // (kry::step_finish); the host reads the m x m block once at the end.  An
+  // invariant subspace found at step jb is redone from jb by the host-driven
+  // loop below (V(0..jb) are untouched by the steps after it).
+  void expand_device()
+  {
+    hipStream_t s = op_->stream;
+    const int ldh = m_ + 1;
+    double *Hd = Hd_.as<double>();
+    const int k0 = k_;
+    // Cycles after the first (k0 > 0, buffers already allocated) are
+    // captured once per k0 into a graph and replayed.
+    hipGraphExec_t ex = nullptr;
+    if (use_graph_ && k0 > 0) {
+      for (const auto &g : graphs_)
+        if (g.k0 == k0 && g.work == op_->work.get()) ex = g.exec;
+      if (!ex) {
+        ex = capture_cycle(k0);
+        if (ex) graphs_.push_back({k0, op_->work.get(), ex});
+        else use_graph_ = false;
+      }
+    }
+    if (ex) {
+      TM_HIP(hipGraphLaunch(ex, s));
+      op_->matvecs += m_ - k0;
+      ++graph_cycles_;
+    } else {
+      issue_cycle(k0);
+    }
+    matvecs_ += m_ - k0;
+    std::vector<double> hd((size_t)ldh * m_ + 2);
+    TM_HIP(hipMemcpyAsync(hd.data(), Hd, sizeof(double) * hd.size(), hipMemcpyDeviceToHost, s));
+    TM_HIP(hipStreamSynchronize(s));
+    const double *fl = hd.data() + (size_t)ldh * m_;
+    reorths_ += (int64_t)fl[0];
+    const int jb = fl[1] < 0 ? m_ : (int)fl[1];
+    for (int j = k0; j < jb; ++j)
+      for (int i = 0; i <= j + 1; ++i) H(i, j) = hd[i + (size_t)j * ldh];
+    if (jb < m_) {
+      matvecs_ -= m_ - jb;
+      expand_host(jb);
+    }
+    k_ = m_;
+  }
+
+  // The kernels of one cycle from column k0 (no host synchronisation, no
+  // allocation once the operator's workspace exists).
+  void issue_cycle(int k0)
+  {
+    hipStream_t s = op_->stream;
+    double *hb = hbuf_.as<double>();
+    const int ldh = m_ + 1;
+    double *Hd = Hd_.as<double>();
+    double *flags = Hd + (size_t)ldh * m_;
+    double *nrm = flags + 2;
+    kry::fill(flags, 0.0, 1, s);
+    kry::fill(flags + 1, -1.0, 1, s);
+    for (int j = k0; j < m_; ++j) {
+      double *w = V(j + 1);
+      op_->apply_dev(V(j), w, ldv_, 1, TM_F64, comp_, tr_, nullptr, s);
+      const int nk = j + 1;
+      kry::multidot(V_.as<double>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
+      kry::update_dots(V_.as<double>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 72, s);
+      kry::update_norm_cond(V_.as<double>(), ldv_, nk, w, n_, hb + 72, partial_.as<double>(),
+                            hb + 144, hb + nk, hb + 72 + nk, s);
+      kry::step_finish(j, hb, hb + 72, hb + 144, Hd, ldh, nrm, flags, s);
+      kry::normalize(w, n_, nrm, s);
+    }
+  }
+
+  // Record issue_cycle(k0) into an executable graph; nullptr if the runtime
+  // refuses (the caller then stays eager).
+  hipGraphExec_t capture_cycle(int k0)
+  {
+    CaptureGuard guard;  // no allocation or free on any thread meanwhile
+    hipStream_t s = op_->stream;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    const int64_t mv = op_->matvecs;
+    bool ok = true;
+    try {
+      issue_cycle(k0);
+    } catch (...) {
+      ok = false;
+    }
+    op_->matvecs = mv;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    hipGraphExec_t ex = nullptr;
+    if (ok && e == hipSuccess && g) {
+      if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
+    }
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return ex;
+  }
+
+  // Host-driven expansion from column j0 (one device->host copy per step).
+  void expand_host(int j0)
+  {
     hipStream_t s = op_->stream;
     double *hb = hbuf_.as<double>();
     std::vector<double> hh(3 * 72);
-    for (int j = k_; j < m_; ++j) {
+    for (int j = j0; j < m_; ++j) {
       double *w = V(j + 1);
       op_->apply_dev(V(j), w, ldv_, 1, TM_F64, comp_, tr_, nullptr, s);
       ++matvecs_;
@@ -386,6 +503,16 @@ class KrylovSchur {
   uint64_t seed_;
   int verbose_;
   int64_t n_ = 0, ldv_ = 0;
+  bool device_steps_ = true;
+  bool use_graph_ = true;
+  struct CycleGraph {
+    int k0;
+    void *work;
+    hipGraphExec_t exec;
+  };
+  std::vector<CycleGraph> graphs_;
+  int64_t graph_cycles_ = 0;
+  DevBuf Hd_;
   bool sym_ = true;
   int k_ = 0;
   int64_t matvecs_ = 0, restarts_ = 0;
@@ -558,7 +685,7 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   TM_CHECK(ncv <= 60, TM_EINVAL, "ncv above 60 is not supported by the device kernels");
   const bool mixed = o.precision == TM_PREC_MIXED;
   const int64_t mv0 = op->matvecs;
-  int64_t mv32 = 0, mv64 = 0, restarts = 0;
+  int64_t mv32 = 0, mv64 = 0, restarts = 0, gcycles = 0;
   int trial = 0, nc = 0;
   Ritz rz;
   std::unique_ptr<KrylovSchur> ks;
@@ -567,9 +694,11 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
     ks.reset(new KrylovSchur(op, nev, ncv, mixed ? std::max(o.mixed_tol, o.tol) : o.tol, o.maxit,
                              mixed ? TM_F32 : TM_F64, 0, o.seed, o.verbose));
     if (mixed) ks->set_stall_stop(true);
+    ks->set_graph(o.use_graph != 0);
     nc = ks->run(rz);
     if (mixed && ks->stalled()) nc = nev;  // FP32 phase ends at its noise floor
     restarts += ks->restarts();
+    gcycles += ks->graph_cycles();
     if (mixed) mv32 += ks->matvecs(); else mv64 += ks->matvecs();
     ++trial;
     if (nc >= nev) break;
@@ -605,7 +734,7 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
     nc = 0;
     for (int i = 0; i < nev; ++i) nc += std::isfinite(ro.theta[i]) ? 1 : 0;
     Xr.alloc(sizeof(double) * ldv * nev);
-    TM_HIP(hipMemcpy(Xr.get(), B.get(), sizeof(double) * ldv * nev, hipMemcpyDeviceToDevice));
+    copy_sync(Xr.get(), B.get(), sizeof(double) * ldv * nev, hipMemcpyDeviceToDevice, op->stream);
   } else {
     std::unique_ptr<KrylovSchur> k2;
     KrylovSchur *fin = ks.get();
@@ -616,13 +745,15 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
       ks->ritz_vectors(rz, nev, X0.as<double>(), nullptr);
       std::vector<double> ones((size_t)nev * 1, 1.0);
       DevBuf Y1(sizeof(double) * nev);
-      TM_HIP(hipMemcpy(Y1.get(), ones.data(), sizeof(double) * nev, hipMemcpyHostToDevice));
+      copy_sync(Y1.get(), ones.data(), sizeof(double) * nev, hipMemcpyHostToDevice, op->stream);
       kry::gemm_vy(X0.as<double>(), ldv, nev, Y1.as<double>(), nev, 1, v0.as<double>(), ldv, n,
                    op->stream);
       k2.reset(new KrylovSchur(op, nev, ncv, o.tol, o.maxit, TM_F64, 0, o.seed, o.verbose));
+      k2->set_graph(o.use_graph != 0);
       nc = k2->run(rz, v0.as<double>());
       mv64 += k2->matvecs();
       restarts += k2->restarts();
+      gcycles += k2->graph_cycles();
       fin = k2.get();
     }
     Xr.alloc(sizeof(double) * fin->ldv() * nev);
@@ -644,16 +775,14 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   if (o.want_vectors && out->vecs) {
     if (!cplx) {
       for (int i = 0; i < nev; ++i)
-        TM_HIP(hipMemcpy(out->vecs + (size_t)i * n, Xr.as<double>() + (size_t)i * ldv,
-                         sizeof(double) * n, hipMemcpyDeviceToHost));
+        copy_sync(out->vecs + (size_t)i * n, Xr.as<double>() + (size_t)i * ldv,
+                         sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
     } else {
       // cols() x nev complex128 interleaved, column-major (pfunc.hpp:139)
       std::vector<double> re(n), im(n);
       for (int i = 0; i < nev; ++i) {
-        TM_HIP(hipMemcpy(re.data(), Xr.as<double>() + (size_t)i * ldv, sizeof(double) * n,
-                         hipMemcpyDeviceToHost));
-        TM_HIP(hipMemcpy(im.data(), Xi.as<double>() + (size_t)i * ldv, sizeof(double) * n,
-                         hipMemcpyDeviceToHost));
+        copy_sync(re.data(), Xr.as<double>() + (size_t)i * ldv, sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
+        copy_sync(im.data(), Xi.as<double>() + (size_t)i * ldv, sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
         for (int64_t t = 0; t < n; ++t) {
           out->vecs[2 * ((size_t)i * n + t)] = re[t];
           out->vecs[2 * ((size_t)i * n + t) + 1] = im[t];
@@ -667,15 +796,17 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
     int ncvl = 7;
     if (ncvl >= n) ncvl = (int)n - 1;
     KrylovSchur kl(op, 1, ncvl, o.tol, o.maxit, TM_F64, 1, o.seed + 1, o.verbose);
+    kl.set_graph(o.use_graph != 0);
     Ritz rl;
     const int ncl = kl.run(rl);
     mv64 += kl.matvecs();
+    gcycles += kl.graph_cycles();
     if (ncl >= 1) {
       out->eval_left = rl.wr[0];
       if (out->vecl) {
         DevBuf XL(sizeof(double) * kl.ldv());
         kl.ritz_vectors(rl, 1, XL.as<double>(), nullptr);
-        TM_HIP(hipMemcpy(out->vecl, XL.get(), sizeof(double) * n, hipMemcpyDeviceToHost));
+        copy_sync(out->vecl, XL.get(), sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
       }
     } else if (o.verbose) {
       std::fprintf(stderr, "Warning: left eigenvalue calculation failed.\n");
@@ -685,6 +816,7 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   out->matvecs_f64 = mv64;
   (void)mv0;
   out->restarts = restarts;
+  out->graph_cycles = gcycles;
   out->trials = trial;
   out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return nc >= nev ? TM_OK : TM_ENOCONV;

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "../../latttmat_amd/csrc/tm_internal.hpp"
@@ -25,6 +26,13 @@ void DevBuf::release()
   std::free(p_);
   p_ = nullptr;
   n_ = 0;
+}
+CaptureGuard::CaptureGuard() {}
+CaptureGuard::~CaptureGuard() {}
+void destroy_graph_exec(hipGraphExec_t) {}
+void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind, hipStream_t)
+{
+  std::memmove(dst, src, bytes);
 }
 namespace kry {
 void multidot(const double *V, int64_t ldv, int j, const double *w, int64_t n, double *, double *out, hipStream_t)
@@ -49,6 +57,30 @@ void update_norm(const double *V, int64_t ldv, int j, double *w, int64_t n, cons
     for (int k = 0; k < j; ++k) w[i] -= hh[k] * V[k * ldv + i];
   out[0] = 0;
   for (int64_t i = 0; i < n; ++i) out[0] += w[i] * w[i];
+}
+void update_norm_cond(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h,
+                      double *, double *out, const double *w0, const double *w1, hipStream_t)
+{
+  if (w1[0] > 0.5 * w0[0]) return;
+  update_norm(V, ldv, j, w, n, h, nullptr, out, nullptr);
+}
+void step_finish(int j, const double *hb1, const double *hb2, const double *hb3, double *Hd,
+                 int ldh, double *nrm, double *flags, hipStream_t)
+{
+  const int nk = j + 1;
+  const bool re = !(hb2[nk] > 0.5 * hb1[nk]);
+  double hn2 = 0;
+  for (int k = 0; k < nk; ++k) {
+    const double hk = hb1[k] + (re ? hb2[k] : 0.0);
+    Hd[k + (int64_t)j * ldh] = hk;
+    hn2 += hk * hk;
+  }
+  const double beta2 = re ? hb3[0] : hb2[nk];
+  const double beta = std::sqrt(std::fmax(beta2, 0.0));
+  Hd[nk + (int64_t)j * ldh] = beta;
+  nrm[0] = beta2;
+  if (re) flags[0] += 1.0;
+  if ((beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
 }
 void normalize(double *w, int64_t n, const double *nrm2, hipStream_t)
 {

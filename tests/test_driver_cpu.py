@@ -90,3 +90,17 @@ def test_small_matrix_clamps_ncv(driver, tmp_path):
     w = np.sort(np.linalg.eigvalsh(A))[::-1]
     got = np.array([e.real for e in o["evals"]])
     assert np.all(np.abs(got / w[:2] - 1) < 1e-12)
+
+
+def test_invariant_subspace_breakdown(driver, tmp_path):
+    # A start vector that lives in a 3-dimensional invariant subspace cannot
+    # fill an ncv=7 basis: the device-side step flags the breakdown and the
+    # host redoes the cycle from there with a fresh orthogonal direction.
+    n = 40
+    evals = np.repeat([5.0, 3.0, 1.0], [1, 1, n - 2])
+    Q = np.linalg.qr(np.random.default_rng(7).standard_normal((n, n)))[0]
+    A = (Q * evals) @ Q.T
+    o = run(driver, A, True, 2, tmp=str(tmp_path))
+    got = np.array([e.real for e in o["evals"]])
+    assert o["status"] == 0
+    assert np.all(np.abs(got / np.array([5.0, 3.0]) - 1) < 1e-12), got
