@@ -49,10 +49,10 @@ struct Error : std::runtime_error {
 #define TM_LAUNCH_CHECK() TM_HIP(hipGetLastError())
 
 // ---------------------------------------------------------------------------
-// Stream-ordered copy that returns once the data has arrived.  Operator
-// streams are non-blocking, so nothing here uses the legacy null stream
-// (whose implicit synchronisation would break graph capture on other
-// threads).
+// Stream-ordered copy that returns once the data has arrived.  The library
+// never uses the legacy null stream, whose implicit synchronisation with
+// every blocking stream would break a graph capture running on another
+// thread.
 void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
 
 // Graph capture and device allocation exclude each other across threads:

@@ -125,6 +125,127 @@ __global__ __launch_bounds__(64) void k_reduce_partials(const double *__restrict
   if (threadIdx.x == 0) out[k] = s;
 }
 
+// ---- 16-byte (two-double) variants: V columns and w 16-byte aligned, ldv
+// even.  Lane p handles elements 2p and 2p+1; block 0 thread 0 adds an odd
+// tail element.  Same two-level deterministic reductions.
+template <int J>
+__global__ __launch_bounds__(THREADS) void k_multidot2(const double *__restrict__ V, int64_t ldv,
+                                                       int j, const double *__restrict__ w,
+                                                       int64_t n, double *__restrict__ partial)
+{
+  const double2 *__restrict__ V2 = reinterpret_cast<const double2 *>(V);
+  const double2 *__restrict__ w2 = reinterpret_cast<const double2 *>(w);
+  const int64_t ld2 = ldv >> 1, n2 = n >> 1;
+  double acc[J + 1];
+#pragma unroll
+  for (int k = 0; k <= J; ++k) acc[k] = 0;
+  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < n2;
+       p += (int64_t)gridDim.x * THREADS) {
+    const double2 wi = w2[p];
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k < j) {
+        const double2 v = V2[k * ld2 + p];
+        acc[k] += v.x * wi.x + v.y * wi.y;
+      }
+    acc[J] += wi.x * wi.x + wi.y * wi.y;
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const double wi = w[n - 1];
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k < j) acc[k] += V[k * ldv + n - 1] * wi;
+    acc[J] += wi * wi;
+  }
+#pragma unroll
+  for (int k = 0; k < J; ++k)
+    if (k == j) acc[k] = acc[J];
+  block_reduce_write<J + 1>(acc, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
+}
+
+// w -= sum_k h[k] V_k; then (dots=1) slots <V_k, w>, <w,w> or (dots=0) slot 0 = <w,w>
+template <int J>
+__global__ __launch_bounds__(THREADS) void k_update_dots2(const double *__restrict__ V,
+                                                          int64_t ldv, int j,
+                                                          double *__restrict__ w, int64_t n,
+                                                          const double *__restrict__ h,
+                                                          double *__restrict__ partial, int dots,
+                                                          const double *w0p, const double *w1p)
+{
+  // conditional form (w0p != null): only when the first projection removed
+  // more than half of the norm^2; the whole grid takes the same branch
+  if (w0p && (w1p[0] > 0.5 * w0p[0])) return;
+  __shared__ double hk[J];
+  for (int k = threadIdx.x; k < J; k += blockDim.x) hk[k] = k < j ? h[k] : 0.0;
+  __syncthreads();
+  const double2 *__restrict__ V2 = reinterpret_cast<const double2 *>(V);
+  double2 *__restrict__ w2 = reinterpret_cast<double2 *>(w);
+  const int64_t ld2 = ldv >> 1, n2 = n >> 1;
+  double acc[J + 1];
+#pragma unroll
+  for (int k = 0; k <= J; ++k) acc[k] = 0;
+  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < n2;
+       p += (int64_t)gridDim.x * THREADS) {
+    double2 vk[J];
+    double2 wi = w2[p];
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k < j) {
+        vk[k] = V2[k * ld2 + p];
+        wi.x -= hk[k] * vk[k].x;
+        wi.y -= hk[k] * vk[k].y;
+      }
+    w2[p] = wi;
+    if (dots) {
+#pragma unroll
+      for (int k = 0; k < J; ++k)
+        if (k < j) acc[k] += vk[k].x * wi.x + vk[k].y * wi.y;
+    }
+    acc[J] += wi.x * wi.x + wi.y * wi.y;
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    double vk[J];
+    double wi = w[n - 1];
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k < j) {
+        vk[k] = V[k * ldv + n - 1];
+        wi -= hk[k] * vk[k];
+      }
+    w[n - 1] = wi;
+    if (dots) {
+#pragma unroll
+      for (int k = 0; k < J; ++k)
+        if (k < j) acc[k] += vk[k] * wi;
+    }
+    acc[J] += wi * wi;
+  }
+  if (dots) {
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k == j) acc[k] = acc[J];
+    block_reduce_write<J + 1>(acc, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
+  } else {
+    acc[0] = acc[J];
+    block_reduce_write<J + 1>(acc, 1, partial + (int64_t)blockIdx.x);
+  }
+}
+
+__global__ void k_normalize2(double *__restrict__ w, int64_t n, const double *__restrict__ nrm2)
+{
+  const double inv = 1.0 / sqrt(nrm2[0]);
+  double2 *__restrict__ w2 = reinterpret_cast<double2 *>(w);
+  const int64_t n2 = n >> 1;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n2;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = w2[p];
+    v.x *= inv;
+    v.y *= inv;
+    w2[p] = v;
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) w[n - 1] *= inv;
+}
+
 // DGKS correction only when the first projection removed more than half of
 // the norm^2 (w1 <= w0 / 2, or not finite); every block reads the same two
 // device values, so the whole grid takes the same branch.
@@ -337,12 +458,20 @@ unsigned nblocks(int64_t n)
     TM_LAUNCH_CHECK();                                                                   \
   } while (0)
 
+bool pairs_ok(const void *a, const void *b, int64_t ld)
+{
+  return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0 &&
+         (ld & 1) == 0;
+}
+
 void multidot(const double *V, int64_t ldv, int j, const double *w, int64_t n, double *partial,
               double *out, hipStream_t s)
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
-  const unsigned nb = nblocks(n);
-  TM_JDISPATCH(k_multidot, j, dim3(nb), V, ldv, j, w, n, partial);
+  const bool v2 = pairs_ok(V, w, ldv);
+  const unsigned nb = v2 ? nblocks(n / 2) : nblocks(n);
+  if (v2) TM_JDISPATCH(k_multidot2, j, dim3(nb), V, ldv, j, w, n, partial);
+  else TM_JDISPATCH(k_multidot, j, dim3(nb), V, ldv, j, w, n, partial);
   hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
   TM_LAUNCH_CHECK();
 }
@@ -351,8 +480,12 @@ void update_dots(const double *V, int64_t ldv, int j, double *w, int64_t n, cons
                  double *partial, double *out, hipStream_t s)
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
-  const unsigned nb = nblocks(n);
-  TM_JDISPATCH(k_update_dots, j, dim3(nb), V, ldv, j, w, n, h, partial, 1);
+  const bool v2 = pairs_ok(V, w, ldv);
+  const unsigned nb = v2 ? nblocks(n / 2) : nblocks(n);
+  if (v2)
+    TM_JDISPATCH(k_update_dots2, j, dim3(nb), V, ldv, j, w, n, h, partial, 1, nullptr, nullptr);
+  else
+    TM_JDISPATCH(k_update_dots, j, dim3(nb), V, ldv, j, w, n, h, partial, 1);
   hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
   TM_LAUNCH_CHECK();
 }
@@ -361,8 +494,12 @@ void update_norm(const double *V, int64_t ldv, int j, double *w, int64_t n, cons
                  double *partial, double *out, hipStream_t s)
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
-  const unsigned nb = nblocks(n);
-  TM_JDISPATCH(k_update_dots, j, dim3(nb), V, ldv, j, w, n, h, partial, 0);
+  const bool v2 = pairs_ok(V, w, ldv);
+  const unsigned nb = v2 ? nblocks(n / 2) : nblocks(n);
+  if (v2)
+    TM_JDISPATCH(k_update_dots2, j, dim3(nb), V, ldv, j, w, n, h, partial, 0, nullptr, nullptr);
+  else
+    TM_JDISPATCH(k_update_dots, j, dim3(nb), V, ldv, j, w, n, h, partial, 0);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, (int)nb, 1, out);
   TM_LAUNCH_CHECK();
 }
@@ -372,8 +509,12 @@ void update_norm_cond(const double *V, int64_t ldv, int j, double *w, int64_t n,
                       hipStream_t s)
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
-  const unsigned nb = nblocks(n);
-  TM_JDISPATCH(k_update_norm_cond, j, dim3(nb), V, ldv, j, w, n, h, partial, w0, w1);
+  const bool v2 = pairs_ok(V, w, ldv);
+  const unsigned nb = v2 ? nblocks(n / 2) : nblocks(n);
+  if (v2)
+    TM_JDISPATCH(k_update_dots2, j, dim3(nb), V, ldv, j, w, n, h, partial, 0, w0, w1);
+  else
+    TM_JDISPATCH(k_update_norm_cond, j, dim3(nb), V, ldv, j, w, n, h, partial, w0, w1);
   hipLaunchKernelGGL(k_reduce_partials_cond, dim3(1), dim3(64), 0, s, partial, (int)nb, out, w0,
                      w1);
   TM_LAUNCH_CHECK();
@@ -389,7 +530,10 @@ void step_finish(int j, const double *hb1, const double *hb2, const double *hb3,
 
 void normalize(double *w, int64_t n, const double *nrm2, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_normalize, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, w, n, nrm2);
+  if ((reinterpret_cast<uintptr_t>(w) & 15) == 0)
+    hipLaunchKernelGGL(k_normalize2, dim3(nblocks(n / 2) * 4), dim3(THREADS), 0, s, w, n, nrm2);
+  else
+    hipLaunchKernelGGL(k_normalize, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, w, n, nrm2);
   TM_LAUNCH_CHECK();
 }
 

@@ -241,9 +241,11 @@ tm_op *make_op(const tm_params &p)
            "invalid model (expected A, B or D)");
   TM_CHECK(p.dim == 2 || p.dim == 3, TM_EINVAL, "invalid dim (expected 2 or 3)");
   std::unique_ptr<tm_op> op(new tm_op());
-  // one stream per operator (default priority): its applies and solves are
-  // stream-ordered, and independent operators run concurrently
-  TM_HIP(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+  // one stream per operator (default priority, blocking): its applies and
+  // solves are stream-ordered, and independent operators run concurrently.
+  // Nothing in the library uses the legacy null stream, so its implicit
+  // synchronisation never reaches a stream that another thread captures.
+  TM_HIP(hipStreamCreate(&op->stream));
   op->p = p;
   set_bz(op.get());
   const int multieigs = p.neigs > 1 ? 2 : 0;  // TMat/pfunc.cpp:22
