@@ -154,14 +154,15 @@ tm_status tm_op_time_apply(const tm_op *cop, tm_dtype io, tm_dtype compute, int 
     ltm::kry::random_uniform(tmp.as<double>(), n, 12345, op->stream);
     if (io == TM_F64) ltm::kry::copy(tmp.as<double>(), x.as<double>(), n, op->stream);
     else ltm::kry::to_f32(tmp.as<double>(), x.as<float>(), n, op->stream);
+    // the eigensolver's T.x: vectors in the operator's internal order
     for (int i = 0; i < warmup; ++i)
-      op->apply_dev(x.get(), y.get(), n, 1, io, compute, transpose, nullptr, op->stream);
+      op->apply_internal(x.get(), y.get(), n, 1, io, compute, transpose, nullptr, op->stream);
     hipEvent_t e0, e1;
     TM_HIP(hipEventCreate(&e0));
     TM_HIP(hipEventCreate(&e1));
     TM_HIP(hipEventRecord(e0, op->stream));
     for (int i = 0; i < reps; ++i)
-      op->apply_dev(x.get(), y.get(), n, 1, io, compute, transpose, nullptr, op->stream);
+      op->apply_internal(x.get(), y.get(), n, 1, io, compute, transpose, nullptr, op->stream);
     TM_HIP(hipEventRecord(e1, op->stream));
     TM_HIP(hipEventSynchronize(e1));
     float ms = 0;

@@ -195,9 +195,11 @@ __device__ void lds_kron(C *lds, int nbits, int bit0, C a, C b)
   }
 }
 
-// XCD-aware block remap: hardware dispatches workgroup i to XCD i % 8; give
-// every XCD a contiguous range of logical block ids so neighbouring tiles
-// share that XCD's L2.  Returns a logical id in [0, nlogical) or -1.
+// XCD-aware block remap: hardware dispatches workgroup i to XCD i % 8.
+// Launch 8 * ceil(n / 8) workgroups; workgroup i takes logical block
+// (i % 8) * ceil(n / 8) + i / 8, so every XCD walks a contiguous range of
+// logical blocks (neighbours share that XCD's L2).  A bijection onto
+// [0, 8 * ceil(n / 8)); returns -1 for the padding ids >= n.
 __device__ __forceinline__ int64_t xcd_remap(int64_t i, int64_t nlogical)
 {
   const int64_t per = (nlogical + 7) / 8;

@@ -199,10 +199,19 @@ struct FusedTables {
   DevBuf rotw;           // uint8[2^n]    word = rotl(NR[nkw], rotw)
   DevBuf slotO;          // uint16[nNk*16] word stored in (necklace, slot), 0xFFFF unused
   DevBuf cmap;           // uint32[nNk << n] orbit of the state (E = NR[c], O = O')
-  DevBuf ostart;         // int64[nNk+1]  orbit range whose evaluation point has O = NR[r]
-  DevBuf olist;          // uint32[M]     orbit indices grouped by r
+  DevBuf ostart;         // int64[nNk+1]  internal-index range of the orbits whose
+                         //               evaluation point has O = NR[r]
+  DevBuf olist;          // uint32[M]     internal index -> orbit (ascending-rep order)
   DevBuf Elist;          // uint16[M]     E part of each orbit's evaluation point
+  std::vector<uint32_t> perm;  // host copy of olist
 };
+// The fused operator numbers orbits internally by row (olist order), so
+// that pass 2 writes contiguously; vectors cross the API in the reference's
+// ascending-representative order: xi[i] = x[perm[i]], y[perm[i]] = yi[i].
+void permute_to_internal(const void *x, void *xi, const uint32_t *perm, int64_t M, tm_dtype io,
+                         hipStream_t s);
+void permute_to_external(const void *yi, void *y, const uint32_t *perm, int64_t M, tm_dtype io,
+                         hipStream_t s);
 void build_fused_tables(const GroupSpec &g, const DevBuf &reps, int64_t M, FusedTables &ft,
                         hipStream_t s);
 
@@ -243,6 +252,22 @@ void update_norm_cond(const double *V, int64_t ldv, int j, double *w, int64_t n,
 // breakdown step (must start < 0)
 void step_finish(int j, const double *hb1, const double *hb2, const double *hb3, double *Hd,
                  int ldh, double *nrm, double *flags, hipStream_t s);
+// close Krylov step j from the two projections: projected-matrix column j
+// (+ subdiagonal) into Hd (ld ldh), nrm[0] = beta^2 (Pythagoras after a
+// correction), flags as for step_finish (also set when the correction removed
+// more than half of the norm again)
+void step_close(int j, const double *hb1, const double *hb2, double *Hd, int ldh, double *nrm,
+                double *flags, hipStream_t s);
+// w <- (w - c V h2) / sqrt(nrm2[0]), c = 1 iff force or !(w1 > w0 / 2)
+void update_normalize(const double *V, int64_t ldv, int j, double *w, int64_t n,
+                      const double *h2, const double *w0, const double *w1, const double *nrm2,
+                      int force, hipStream_t s);
+// Lanczos step of a symmetric operator (j above the cycle's first step):
+// hL[0..j] = (0.., beta_{j-1} = Hd[j, j-1], alpha = hA[0])
+void lanczos_coeffs(int j, const double *hA, const double *Hd, int ldh, double *hL, hipStream_t s);
+// column j = hL + h2, beta^2 = |w1|^2 - |h2|^2 (hb2[j+1] = |w1|^2); flags as step_close
+void step_close_lanczos(int j, const double *hL, const double *hb2, double *Hd, int ldh,
+                        double *nrm, double *flags, hipStream_t s);
 // v = w * (1/sqrt(nrm2[0]))
 void normalize(double *w, int64_t n, const double *nrm2, hipStream_t s);
 // Vout[:, c] = sum_r Vin[:, r] * Y[r + c*ldy], r < m, c < k  (Vout may alias Vin)
@@ -292,10 +317,17 @@ struct tm_op {
   // reduced operators
   ltm::DevBuf reps, dups, map, tvec, dt64, tvec32, dt32;
   std::unique_ptr<ltm::FusedTables> ft;
+  // internal orbit numbering (fused reduced operator); empty = identity.
+  // perm[i] = reference (ascending-representative) index of internal i.
+  std::vector<uint32_t> perm;
   // workspaces
-  ltm::DevBuf work, one, hx, hy;
+  ltm::DevBuf work, one, hx, hy, px, py;
 
   virtual ~tm_op();
+  // T.x on vectors in the internal order (the eigensolver's basis)
+  void apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
+                      int transpose, const double *scale_in, hipStream_t s);
+  // T.x on vectors in the reference order (the API)
   void apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
                  int transpose, const double *scale_in, hipStream_t s);
 };

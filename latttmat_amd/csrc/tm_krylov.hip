@@ -246,6 +246,142 @@ __global__ void k_normalize2(double *__restrict__ w, int64_t n, const double *__
   if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) w[n - 1] *= inv;
 }
 
+// Close a Krylov step in one sweep: w <- (w - c V h2) / beta, with c = 1
+// when the DGKS correction is due (w1 <= w0 / 2) and beta^2 = nrm2[0] as set
+// by k_step_finish (w1 - |h2|^2 after a correction, w1 otherwise).
+template <int J>
+__global__ __launch_bounds__(THREADS) void k_update_normalize(const double *__restrict__ V,
+                                                              int64_t ldv, int j,
+                                                              double *__restrict__ w, int64_t n,
+                                                              const double *__restrict__ h2,
+                                                              const double *w0p, const double *w1p,
+                                                              const double *__restrict__ nrm2,
+                                                              int force)
+{
+  const bool re = force || !(w1p[0] > 0.5 * w0p[0]);
+  const double inv = 1.0 / sqrt(nrm2[0]);
+  __shared__ double hk[J];
+  for (int k = threadIdx.x; k < J; k += blockDim.x) hk[k] = (re && k < j) ? h2[k] : 0.0;
+  __syncthreads();
+  const int jj = re ? j : 0;
+  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * THREADS) {
+    double wi = w[i];
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k < jj) wi -= hk[k] * V[k * ldv + i];
+    w[i] = wi * inv;
+  }
+}
+
+template <int J>
+__global__ __launch_bounds__(THREADS) void k_update_normalize2(const double *__restrict__ V,
+                                                               int64_t ldv, int j,
+                                                               double *__restrict__ w, int64_t n,
+                                                               const double *__restrict__ h2,
+                                                               const double *w0p,
+                                                               const double *w1p,
+                                                               const double *__restrict__ nrm2,
+                                                               int force)
+{
+  const bool re = force || !(w1p[0] > 0.5 * w0p[0]);
+  const double inv = 1.0 / sqrt(nrm2[0]);
+  __shared__ double hk[J];
+  for (int k = threadIdx.x; k < J; k += blockDim.x) hk[k] = (re && k < j) ? h2[k] : 0.0;
+  __syncthreads();
+  const int jj = re ? j : 0;
+  const double2 *__restrict__ V2 = reinterpret_cast<const double2 *>(V);
+  double2 *__restrict__ w2 = reinterpret_cast<double2 *>(w);
+  const int64_t ld2 = ldv >> 1, n2 = n >> 1;
+  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < n2;
+       p += (int64_t)gridDim.x * THREADS) {
+    double2 wi = w2[p];
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k < jj) {
+        const double2 v = V2[k * ld2 + p];
+        wi.x -= hk[k] * v.x;
+        wi.y -= hk[k] * v.y;
+      }
+    wi.x *= inv;
+    wi.y *= inv;
+    w2[p] = wi;
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    double wi = w[n - 1];
+    for (int k = 0; k < jj; ++k) wi -= hk[k] * V[k * ldv + n - 1];
+    w[n - 1] = wi * inv;
+  }
+}
+
+// One thread closes step j from the two projections alone: column j of the
+// projected matrix = h1 + c h2 (c as above), beta^2 = w1 - |h2|^2 after a
+// correction (Pythagoras; V orthonormal) or w1; nrm[0] = beta^2.  flags[0]
+// counts corrections; flags[1] records the first step j whose beta is below
+// 1e-13 |h| or whose correction removed more than half of w1 again (a
+// numerically invariant subspace: the host redoes the cycle from j with
+// explicit norms).
+__global__ void k_step_close(int j, const double *__restrict__ hb1,
+                             const double *__restrict__ hb2, double *__restrict__ Hd, int ldh,
+                             double *__restrict__ nrm, double *__restrict__ flags)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int nk = j + 1;
+  const double w0 = hb1[nk], w1 = hb2[nk];
+  const bool re = !(w1 > 0.5 * w0);
+  double hn2 = 0, h22 = 0;
+  for (int k = 0; k < nk; ++k) {
+    const double hk = hb1[k] + (re ? hb2[k] : 0.0);
+    Hd[k + (int64_t)j * ldh] = hk;
+    hn2 += hk * hk;
+    h22 += hb2[k] * hb2[k];
+  }
+  const double beta2 = re ? w1 - h22 : w1;
+  const double beta = sqrt(fmax(beta2, 0.0));
+  Hd[nk + (int64_t)j * ldh] = beta;
+  nrm[0] = beta2;
+  if (re) flags[0] += 1.0;
+  const bool unsafe = re && !(h22 <= 0.5 * w1);
+  if ((unsafe || beta <= 1e-13 * sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
+}
+
+// Symmetric operator, step j > first step of the cycle: T v_j couples to
+// v_{j-1}, v_j, v_{j+1} only.  hL[0..j] = (0, .., 0, beta_{j-1}, alpha) with
+// beta_{j-1} = Hd[j, j-1] and alpha = <v_j, T v_j> (hA[0]).
+__global__ void k_lanczos_coeffs(int j, const double *__restrict__ hA,
+                                 const double *__restrict__ Hd, int ldh, double *__restrict__ hL)
+{
+  for (int k = threadIdx.x; k <= j; k += blockDim.x)
+    hL[k] = k == j ? hA[0] : (k == j - 1 ? Hd[j + (int64_t)(j - 1) * ldh] : 0.0);
+}
+
+// Close a Lanczos step: column j = hL + h2 (the full reorthogonalisation pass
+// always runs), beta^2 = |w1|^2 - |h2|^2 (Pythagoras).  flags as k_step_close;
+// unsafe when the reorthogonalisation removed more than half of |w1|^2.
+__global__ void k_step_close_lanczos(int j, const double *__restrict__ hL,
+                                     const double *__restrict__ hb2, double *__restrict__ Hd,
+                                     int ldh, double *__restrict__ nrm,
+                                     double *__restrict__ flags)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int nk = j + 1;
+  const double w1 = hb2[nk];
+  double hn2 = 0, h22 = 0;
+  for (int k = 0; k < nk; ++k) {
+    const double hk = hL[k] + hb2[k];
+    Hd[k + (int64_t)j * ldh] = hk;
+    hn2 += hk * hk;
+    h22 += hb2[k] * hb2[k];
+  }
+  const double beta2 = w1 - h22;
+  const double beta = sqrt(fmax(beta2, 0.0));
+  Hd[nk + (int64_t)j * ldh] = beta;
+  nrm[0] = beta2;
+  flags[0] += 1.0;
+  const bool unsafe = !(h22 <= 0.5 * w1);
+  if ((unsafe || beta <= 1e-13 * sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
+}
+
 // DGKS correction only when the first projection removed more than half of
 // the norm^2 (w1 <= w0 / 2, or not finite); every block reads the same two
 // device values, so the whole grid takes the same branch.
@@ -517,6 +653,40 @@ void update_norm_cond(const double *V, int64_t ldv, int j, double *w, int64_t n,
     TM_JDISPATCH(k_update_norm_cond, j, dim3(nb), V, ldv, j, w, n, h, partial, w0, w1);
   hipLaunchKernelGGL(k_reduce_partials_cond, dim3(1), dim3(64), 0, s, partial, (int)nb, out, w0,
                      w1);
+  TM_LAUNCH_CHECK();
+}
+
+void step_close(int j, const double *hb1, const double *hb2, double *Hd, int ldh, double *nrm,
+                double *flags, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_step_close, dim3(1), dim3(64), 0, s, j, hb1, hb2, Hd, ldh, nrm, flags);
+  TM_LAUNCH_CHECK();
+}
+
+void update_normalize(const double *V, int64_t ldv, int j, double *w, int64_t n,
+                      const double *h2, const double *w0, const double *w1, const double *nrm2,
+                      int force, hipStream_t s)
+{
+  TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
+  if (pairs_ok(V, w, ldv))
+    TM_JDISPATCH(k_update_normalize2, j, dim3(nblocks(n / 2)), V, ldv, j, w, n, h2, w0, w1, nrm2,
+                 force);
+  else
+    TM_JDISPATCH(k_update_normalize, j, dim3(nblocks(n)), V, ldv, j, w, n, h2, w0, w1, nrm2,
+                 force);
+}
+
+void lanczos_coeffs(int j, const double *hA, const double *Hd, int ldh, double *hL, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_lanczos_coeffs, dim3(1), dim3(64), 0, s, j, hA, Hd, ldh, hL);
+  TM_LAUNCH_CHECK();
+}
+
+void step_close_lanczos(int j, const double *hL, const double *hb2, double *Hd, int ldh,
+                        double *nrm, double *flags, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_step_close_lanczos, dim3(1), dim3(64), 0, s, j, hL, hb2, Hd, ldh, nrm,
+                     flags);
   TM_LAUNCH_CHECK();
 }
 

@@ -217,6 +217,19 @@ void build_reduced(tm_op *op, const GroupSpec &g, double nfexp, bool want_fused)
     tv[o] = std::exp(e + Jn) / std::sqrt((double)dups[o]);
     dt[o] = dups[o] * tv[o];
   }
+  if (fused) {
+    op->ft.reset(new FusedTables());
+    build_fused_tables(g, op->reps, M, *op->ft, op->stream);
+    // the fused kernels work in the internal (row) orbit order
+    op->perm = op->ft->perm;
+    std::vector<double> a(M), b(M);
+    for (int64_t i = 0; i < M; ++i) {
+      a[i] = tv[op->perm[i]];
+      b[i] = dt[op->perm[i]];
+    }
+    tv.swap(a);
+    dt.swap(b);
+  }
   std::vector<float> tv32(tv.begin(), tv.end()), dt32(dt.begin(), dt.end());
   op->tvec.alloc(M * 8);
   op->dt64.alloc(M * 8);
@@ -226,10 +239,6 @@ void build_reduced(tm_op *op, const GroupSpec &g, double nfexp, bool want_fused)
   copy_sync(op->dt64.get(), dt.data(), M * 8, hipMemcpyHostToDevice, op->stream);
   copy_sync(op->tvec32.get(), tv32.data(), M * 4, hipMemcpyHostToDevice, op->stream);
   copy_sync(op->dt32.get(), dt32.data(), M * 4, hipMemcpyHostToDevice, op->stream);
-  if (fused) {
-    op->ft.reset(new FusedTables());
-    build_fused_tables(g, op->reps, M, *op->ft, op->stream);
-  }
 }
 
 }  // namespace
@@ -370,6 +379,28 @@ tm_op::~tm_op()
 
 void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
                       int transpose, const double *scale_in, hipStream_t s)
+{
+  using namespace ltm;
+  if (perm.empty()) {
+    apply_internal(x, y, ld, nvec, io, comp, transpose, scale_in, s);
+    return;
+  }
+  const size_t es = io == TM_F64 ? 8 : 4;
+  px.ensure(es * ncol);
+  py.ensure(es * ncol);
+  const uint32_t *pd = ft->olist.as<uint32_t>();
+  for (int v = 0; v < nvec; ++v) {
+    const void *xv = static_cast<const char *>(x) + (size_t)v * ld * es;
+    void *yv = static_cast<char *>(y) + (size_t)v * ld * es;
+    permute_to_internal(xv, px.get(), pd, ncol, io, s);
+    apply_internal(px.get(), py.get(), ncol, 1, io, comp, transpose,
+                   scale_in ? scale_in + v : nullptr, s);
+    permute_to_external(py.get(), yv, pd, ncol, io, s);
+  }
+}
+
+void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype io,
+                           tm_dtype comp, int transpose, const double *scale_in, hipStream_t s)
 {
   using namespace ltm;
   const size_t es = io == TM_F64 ? 8 : 4;

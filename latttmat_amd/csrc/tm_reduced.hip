@@ -396,6 +396,15 @@ int64_t enumerate_orbits(const GroupSpec &gs, DevBuf &reps, DevBuf &dups, DevBuf
   return (int64_t)M;
 }
 
+namespace {
+__global__ void k_remap_u32(uint32_t *__restrict__ a, const uint32_t *__restrict__ q, uint64_t n)
+{
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    a[i] = q[a[i]];
+}
+}  // namespace
+
 void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, FusedTables &ft,
                         hipStream_t s)
 {
@@ -459,11 +468,18 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   std::vector<uint32_t> olist(M);
   std::vector<uint16_t> elist(M);
   std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-  for (int64_t o = 0; o < M; ++o) {
-    const int64_t i = pos[pr[o]]++;
-    olist[i] = (uint32_t)o;
-    elist[i] = pe[o];
-  }
+  for (int64_t o = 0; o < M; ++o) olist[pos[pr[o]]++] = (uint32_t)o;
+  // Within a row, order the orbits by the necklace class (then rotation) of
+  // their E part: pass 1 of column c gathers, from every row, only orbits
+  // whose E part is a rotation of NR[c], which are then one contiguous run.
+  for (int r = 0; r < ft.nNk; ++r)
+    std::sort(olist.begin() + cnt[r], olist.begin() + cnt[r + 1], [&](uint32_t a, uint32_t b) {
+      const uint32_t ea = pe[a], eb = pe[b];
+      if (nk[ea] != nk[eb]) return nk[ea] < nk[eb];
+      if (rot[ea] != rot[eb]) return rot[ea] < rot[eb];
+      return a < b;
+    });
+  for (int64_t i = 0; i < M; ++i) elist[i] = pe[olist[i]];
   auto up = [&](DevBuf &d, const void *src, size_t bytes) {
     d.alloc(bytes);
     copy_sync(d.get(), src, bytes, hipMemcpyHostToDevice, s);
@@ -476,6 +492,12 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   up(ft.ostart, cnt.data(), cnt.size() * 8);
   up(ft.olist, olist.data(), olist.size() * 4);
   up(ft.Elist, elist.data(), elist.size() * 2);
+  // internal orbit numbering = position in olist (rows contiguous)
+  std::vector<uint32_t> inv(M);
+  for (int64_t i = 0; i < M; ++i) inv[olist[i]] = (uint32_t)i;
+  DevBuf invd;
+  up(invd, inv.data(), inv.size() * 4);
+  ft.perm = std::move(olist);
 
   // column-major state->orbit map over the necklace columns
   const GroupDev g = to_dev(gs);
@@ -487,6 +509,9 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   hipLaunchKernelGGL(k_build_cmap, dim3(grid_for(total, 256)), dim3(256), 0, s, g, n,
                      ft.NR.as<uint16_t>(), (uint64_t)ft.nNk, sc.bitmap.as<uint64_t>(),
                      sc.wpre.as<uint64_t>(), ft.cmap.as<uint32_t>());
+  TM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_remap_u32, dim3(grid_for(total, 256)), dim3(256), 0, s,
+                     ft.cmap.as<uint32_t>(), invd.as<uint32_t>(), total);
   TM_LAUNCH_CHECK();
   TM_HIP(hipStreamSynchronize(s));
 }
@@ -589,13 +614,15 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
 template <typename C, typename TO>
 __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
     const C *__restrict__ Ap, const uint16_t *__restrict__ NR, const uint8_t *__restrict__ period,
-    int nNk, const int64_t *__restrict__ ostart, const uint32_t *__restrict__ olist,
-    const uint16_t *__restrict__ Elist, const C *__restrict__ dt, TO *__restrict__ y, int n,
+    int nNk, const int64_t *__restrict__ ostart, const uint16_t *__restrict__ Elist, const C *__restrict__ dt, TO *__restrict__ y, int n,
     int64_t ldA, C ka, C kb, const double *__restrict__ scale)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
-  const int r = blockIdx.x;
+  // consecutive rows read neighbouring 16-slot groups of every column (the
+  // same cache lines): keep them on one XCD
+  const int r = (int)dev::xcd_remap(blockIdx.x, nNk);
+  if (r < 0) return;
   const int pr = period[r];
   const uint32_t wmask = (1u << n) - 1;
   // one necklace column per thread: a 16-slot group holds A(NR[c], rotations
@@ -621,11 +648,10 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
   __syncthreads();
   dev::lds_kron(lds, n, 0, ka, kb);
   const C inv = scale ? (C)(1.0 / *scale) : (C)1;
+  // internal numbering: the orbits of this row are [ostart[r], ostart[r+1])
   const int64_t i0 = ostart[r], i1 = ostart[r + 1];
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    const uint32_t o = olist[i];
-    y[o] = (TO)(dt[o] * lds[dev::sw(Elist[i])] * inv);
-  }
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+    y[i] = (TO)(dt[i] * lds[dev::sw(Elist[i])] * inv);
 }
 
 template <typename C>
@@ -653,16 +679,17 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
                        ft.cmap.as<uint32_t>(), xt, Ap, ft.slotO.as<uint16_t>(), n, ft.nNk,
                        ft.ldA, ka, kb);
     TM_LAUNCH_CHECK();
+    const unsigned g2 = 8u * (unsigned)((ft.nNk + 7) / 8);
     if (io == TM_F64)
-      hipLaunchKernelGGL((k_fused_pass2<C, double>), dim3(ft.nNk), dim3(FUSED_THREADS), shm, s,
+      hipLaunchKernelGGL((k_fused_pass2<C, double>), dim3(g2), dim3(FUSED_THREADS), shm, s,
                          Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(), ft.nNk,
-                         ft.ostart.as<int64_t>(), ft.olist.as<uint32_t>(), ft.Elist.as<uint16_t>(),
-                         dt, static_cast<double *>(out), n, ft.ldA, ka, kb, scale);
+                         ft.ostart.as<int64_t>(), ft.Elist.as<uint16_t>(), dt,
+                         static_cast<double *>(out), n, ft.ldA, ka, kb, scale);
     else
-      hipLaunchKernelGGL((k_fused_pass2<C, float>), dim3(ft.nNk), dim3(FUSED_THREADS), shm, s,
+      hipLaunchKernelGGL((k_fused_pass2<C, float>), dim3(g2), dim3(FUSED_THREADS), shm, s,
                          Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(), ft.nNk,
-                         ft.ostart.as<int64_t>(), ft.olist.as<uint32_t>(), ft.Elist.as<uint16_t>(),
-                         dt, static_cast<float *>(out), n, ft.ldA, ka, kb, scale);
+                         ft.ostart.as<int64_t>(), ft.Elist.as<uint16_t>(), dt,
+                         static_cast<float *>(out), n, ft.ldA, ka, kb, scale);
     TM_LAUNCH_CHECK();
     return;
   }
@@ -716,6 +743,51 @@ void launch_reduced(const ReducedPlan &p, const void *in, void *out, tm_dtype io
     reduced_typed<double>(p, in, out, io, work, scale, s, p.tvec, p.dt64);
   else
     reduced_typed<float>(p, in, out, io, work, scale, s, p.tvec32, p.dt32);
+}
+
+namespace {
+template <typename T>
+__global__ void k_perm_gather(const T *__restrict__ x, const uint32_t *__restrict__ perm,
+                              T *__restrict__ xi, int64_t M)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M;
+       i += (int64_t)gridDim.x * blockDim.x)
+    xi[i] = x[perm[i]];
+}
+template <typename T>
+__global__ void k_perm_scatter(const T *__restrict__ yi, const uint32_t *__restrict__ perm,
+                               T *__restrict__ y, int64_t M)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[perm[i]] = yi[i];
+}
+}  // namespace
+
+void permute_to_internal(const void *x, void *xi, const uint32_t *perm, int64_t M, tm_dtype io,
+                         hipStream_t s)
+{
+  const unsigned g = grid_for((uint64_t)M, 256);
+  if (io == TM_F64)
+    hipLaunchKernelGGL(k_perm_gather<double>, dim3(g), dim3(256), 0, s,
+                       static_cast<const double *>(x), perm, static_cast<double *>(xi), M);
+  else
+    hipLaunchKernelGGL(k_perm_gather<float>, dim3(g), dim3(256), 0, s,
+                       static_cast<const float *>(x), perm, static_cast<float *>(xi), M);
+  TM_LAUNCH_CHECK();
+}
+
+void permute_to_external(const void *yi, void *y, const uint32_t *perm, int64_t M, tm_dtype io,
+                         hipStream_t s)
+{
+  const unsigned g = grid_for((uint64_t)M, 256);
+  if (io == TM_F64)
+    hipLaunchKernelGGL(k_perm_scatter<double>, dim3(g), dim3(256), 0, s,
+                       static_cast<const double *>(yi), perm, static_cast<double *>(y), M);
+  else
+    hipLaunchKernelGGL(k_perm_scatter<float>, dim3(g), dim3(256), 0, s,
+                       static_cast<const float *>(yi), perm, static_cast<float *>(y), M);
+  TM_LAUNCH_CHECK();
 }
 
 }  // namespace ltm

@@ -159,14 +159,27 @@ class KrylovSchur {
     kry::fill(flags + 1, -1.0, 1, s);
     for (int j = k0; j < m_; ++j) {
       double *w = V(j + 1);
-      op_->apply_dev(V(j), w, ldv_, 1, TM_F64, comp_, tr_, nullptr, s);
+      op_->apply_internal(V(j), w, ldv_, 1, TM_F64, comp_, tr_, nullptr, s);
       const int nk = j + 1;
+      if (sym_ && j > k0) {
+        // Lanczos step: subtract alpha v_j + beta_{j-1} v_{j-1} inside the
+        // reorthogonalisation sweep, then one full correction sweep that also
+        // normalises (two sweeps over V instead of three)
+        kry::multidot(V(j), ldv_, 1, w, n_, partial_.as<double>(), hb, s);
+        kry::lanczos_coeffs(j, hb, Hd, ldh, hb + 144, s);
+        kry::update_dots(V_.as<double>(), ldv_, nk, w, n_, hb + 144, partial_.as<double>(),
+                         hb + 72, s);
+        kry::step_close_lanczos(j, hb + 144, hb + 72, Hd, ldh, nrm, flags, s);
+        kry::update_normalize(V_.as<double>(), ldv_, nk, w, n_, hb + 72, hb, hb, nrm, 1, s);
+        continue;
+      }
       kry::multidot(V_.as<double>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
       kry::update_dots(V_.as<double>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 72, s);
-      kry::update_norm_cond(V_.as<double>(), ldv_, nk, w, n_, hb + 72, partial_.as<double>(),
-                            hb + 144, hb + nk, hb + 72 + nk, s);
-      kry::step_finish(j, hb, hb + 72, hb + 144, Hd, ldh, nrm, flags, s);
-      kry::normalize(w, n_, nrm, s);
+      // DGKS correction and normalisation in one sweep, the norm from the
+      // projections (an unsafe correction sends the step to the host loop)
+      kry::step_close(j, hb, hb + 72, Hd, ldh, nrm, flags, s);
+      kry::update_normalize(V_.as<double>(), ldv_, nk, w, n_, hb + 72, hb + nk, hb + 72 + nk, nrm,
+                            0, s);
     }
   }
 
@@ -207,7 +220,7 @@ class KrylovSchur {
     std::vector<double> hh(3 * 72);
     for (int j = j0; j < m_; ++j) {
       double *w = V(j + 1);
-      op_->apply_dev(V(j), w, ldv_, 1, TM_F64, comp_, tr_, nullptr, s);
+      op_->apply_internal(V(j), w, ldv_, 1, TM_F64, comp_, tr_, nullptr, s);
       ++matvecs_;
       const int nk = j + 1;
       // classical Gram-Schmidt with one DGKS correction (only when the first
@@ -575,7 +588,7 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
   };
   for (int it = 0; it < maxiter; ++it) {
     for (int c = 0; c < k; ++c) {
-      op->apply_dev(col(c), col(k + c), ld, 1, TM_F64, TM_F64, 0, nullptr, s);
+      op->apply_internal(col(c), col(k + c), ld, 1, TM_F64, TM_F64, 0, nullptr, s);
       ++ro.matvecs;
     }
     gram_host(0, k);
@@ -774,9 +787,19 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   }
   if (o.want_vectors && out->vecs) {
     if (!cplx) {
-      for (int i = 0; i < nev; ++i)
-        copy_sync(out->vecs + (size_t)i * n, Xr.as<double>() + (size_t)i * ldv,
-                         sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
+      std::vector<double> tmp(op->perm.empty() ? 0 : n);
+      for (int i = 0; i < nev; ++i) {
+        double *dst = out->vecs + (size_t)i * n;
+        if (op->perm.empty()) {
+          copy_sync(dst, Xr.as<double>() + (size_t)i * ldv, sizeof(double) * n,
+                    hipMemcpyDeviceToHost, op->stream);
+        } else {
+          // internal orbit order -> ascending representatives
+          copy_sync(tmp.data(), Xr.as<double>() + (size_t)i * ldv, sizeof(double) * n,
+                    hipMemcpyDeviceToHost, op->stream);
+          for (int64_t t = 0; t < n; ++t) dst[op->perm[t]] = tmp[t];
+        }
+      }
     } else {
       // cols() x nev complex128 interleaved, column-major (pfunc.hpp:139)
       std::vector<double> re(n), im(n);

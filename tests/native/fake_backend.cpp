@@ -82,6 +82,64 @@ void step_finish(int j, const double *hb1, const double *hb2, const double *hb3,
   if (re) flags[0] += 1.0;
   if ((beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
 }
+void step_close(int j, const double *hb1, const double *hb2, double *Hd, int ldh, double *nrm,
+                double *flags, hipStream_t)
+{
+  const int nk = j + 1;
+  const double w0 = hb1[nk], w1 = hb2[nk];
+  const bool re = !(w1 > 0.5 * w0);
+  double hn2 = 0, h22 = 0;
+  for (int k = 0; k < nk; ++k) {
+    const double hk = hb1[k] + (re ? hb2[k] : 0.0);
+    Hd[k + (int64_t)j * ldh] = hk;
+    hn2 += hk * hk;
+    h22 += hb2[k] * hb2[k];
+  }
+  const double beta2 = re ? w1 - h22 : w1;
+  const double beta = std::sqrt(std::fmax(beta2, 0.0));
+  Hd[nk + (int64_t)j * ldh] = beta;
+  nrm[0] = beta2;
+  if (re) flags[0] += 1.0;
+  const bool unsafe = re && !(h22 <= 0.5 * w1);
+  if ((unsafe || beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
+}
+void lanczos_coeffs(int j, const double *hA, const double *Hd, int ldh, double *hL, hipStream_t)
+{
+  for (int k = 0; k <= j; ++k) hL[k] = k == j ? hA[0] : (k == j - 1 ? Hd[j + (int64_t)(j - 1) * ldh] : 0.0);
+}
+void step_close_lanczos(int j, const double *hL, const double *hb2, double *Hd, int ldh, double *nrm,
+                        double *flags, hipStream_t)
+{
+  const int nk = j + 1;
+  const double w1 = hb2[nk];
+  double hn2 = 0, h22 = 0;
+  for (int k = 0; k < nk; ++k) {
+    const double hk = hL[k] + hb2[k];
+    Hd[k + (int64_t)j * ldh] = hk;
+    hn2 += hk * hk;
+    h22 += hb2[k] * hb2[k];
+  }
+  const double beta2 = w1 - h22;
+  const double beta = std::sqrt(std::fmax(beta2, 0.0));
+  Hd[nk + (int64_t)j * ldh] = beta;
+  nrm[0] = beta2;
+  flags[0] += 1.0;
+  const bool unsafe = !(h22 <= 0.5 * w1);
+  if ((unsafe || beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
+}
+void update_normalize(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h2,
+                      const double *w0, const double *w1, const double *nrm2, int force, hipStream_t)
+{
+  const bool re = force || !(w1[0] > 0.5 * w0[0]);
+  std::vector<double> hh(h2, h2 + j);
+  const double inv = 1.0 / std::sqrt(nrm2[0]);
+  for (int64_t i = 0; i < n; ++i) {
+    double wi = w[i];
+    if (re)
+      for (int k = 0; k < j; ++k) wi -= hh[k] * V[k * ldv + i];
+    w[i] = wi * inv;
+  }
+}
 void normalize(double *w, int64_t n, const double *nrm2, hipStream_t)
 {
   const double inv = 1.0 / std::sqrt(nrm2[0]);
@@ -125,7 +183,12 @@ void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double
 
 // dense operator: op->tab64 holds the n x n column-major matrix
 tm_op::~tm_op() = default;
-void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype, tm_dtype comp, int transpose,
+void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp, int transpose,
+                      const double *sc, hipStream_t s)
+{
+  apply_internal(x, y, ld, nvec, io, comp, transpose, sc, s);
+}
+void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype, tm_dtype comp, int transpose,
                       const double *, hipStream_t)
 {
   const double *A = tab64.as<double>();
