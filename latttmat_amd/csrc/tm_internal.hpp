@@ -67,6 +67,22 @@ struct CaptureGuard {
 // hipGraphExecDestroy under the shared side of the same lock
 void destroy_graph_exec(hipGraphExec_t g);
 
+// page-locked host buffer (RAII) for the solver's small per-cycle transfers
+class PinnedBuf {
+ public:
+  PinnedBuf() = default;
+  explicit PinnedBuf(size_t bytes) { alloc(bytes); }
+  ~PinnedBuf() { release(); }
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  void alloc(size_t bytes);
+  void release();
+  template <typename T> T *as() const { return static_cast<T *>(p_); }
+
+ private:
+  void *p_ = nullptr;
+};
+
 // device buffer (RAII)
 // ---------------------------------------------------------------------------
 class DevBuf {

@@ -585,13 +585,18 @@ unsigned nblocks(int64_t n)
 
 }  // namespace
 
-#define TM_JDISPATCH(KERNEL, jj, grid, ...)                                              \
-  do {                                                                                   \
-    if ((jj) <= 8) hipLaunchKernelGGL(KERNEL<8>, grid, dim3(THREADS), 0, s, __VA_ARGS__);  \
+// Basis-size classes of the fused kernels: the register arrays are sized by
+// the template argument, so keep it close to the actual count.
+#define TM_JDISPATCH(KERNEL, jj, grid, ...)                                                  \
+  do {                                                                                       \
+    if ((jj) <= 4) hipLaunchKernelGGL(KERNEL<4>, grid, dim3(THREADS), 0, s, __VA_ARGS__);     \
+    else if ((jj) <= 8) hipLaunchKernelGGL(KERNEL<8>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
+    else if ((jj) <= 12) hipLaunchKernelGGL(KERNEL<12>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
     else if ((jj) <= 16) hipLaunchKernelGGL(KERNEL<16>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
+    else if ((jj) <= 24) hipLaunchKernelGGL(KERNEL<24>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
     else if ((jj) <= 32) hipLaunchKernelGGL(KERNEL<32>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
-    else hipLaunchKernelGGL(KERNEL<64>, grid, dim3(THREADS), 0, s, __VA_ARGS__);          \
-    TM_LAUNCH_CHECK();                                                                   \
+    else hipLaunchKernelGGL(KERNEL<64>, grid, dim3(THREADS), 0, s, __VA_ARGS__);              \
+    TM_LAUNCH_CHECK();                                                                       \
   } while (0)
 
 bool pairs_ok(const void *a, const void *b, int64_t ld)

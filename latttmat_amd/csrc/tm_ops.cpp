@@ -51,6 +51,31 @@ void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hip
   TM_HIP(hipStreamSynchronize(s));
 }
 
+void PinnedBuf::alloc(size_t bytes)
+{
+  release();
+  if (bytes == 0) return;
+  if (t_capturing) throw Error(TM_EHIP, "host allocation during graph capture");
+  std::shared_lock<std::shared_mutex> lk(g_capture_mu);
+  if (hipHostMalloc(&p_, bytes, hipHostMallocDefault) != hipSuccess) {
+    p_ = nullptr;
+    (void)hipGetLastError();
+    throw Error(TM_ENOMEM, "pinned host allocation failed");
+  }
+}
+
+void PinnedBuf::release()
+{
+  if (!p_) return;
+  if (t_capturing) {
+    (void)hipHostFree(p_);
+  } else {
+    std::shared_lock<std::shared_mutex> lk(g_capture_mu);
+    (void)hipHostFree(p_);
+  }
+  p_ = nullptr;
+}
+
 void DevBuf::alloc(size_t bytes)
 {
   release();

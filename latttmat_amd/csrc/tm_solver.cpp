@@ -56,6 +56,7 @@ class KrylovSchur {
     Ydev_.alloc(sizeof(double) * 72 * 72);
     H_.assign((size_t)(m_ + 1) * m_, 0.0);
     Hd_.alloc(sizeof(double) * ((size_t)(m_ + 1) * m_ + 8));
+    stage_.alloc(sizeof(double) * 72 * 72);
   }
 
   ~KrylovSchur()
@@ -130,10 +131,11 @@ class KrylovSchur {
       issue_cycle(k0);
     }
     matvecs_ += m_ - k0;
-    std::vector<double> hd((size_t)ldh * m_ + 2);
-    TM_HIP(hipMemcpyAsync(hd.data(), Hd, sizeof(double) * hd.size(), hipMemcpyDeviceToHost, s));
+    double *hd = stage_.as<double>();  // (m+1) m + 2 <= 72 * 72 doubles
+    TM_HIP(hipMemcpyAsync(hd, Hd, sizeof(double) * ((size_t)ldh * m_ + 2), hipMemcpyDeviceToHost,
+                          s));
     TM_HIP(hipStreamSynchronize(s));
-    const double *fl = hd.data() + (size_t)ldh * m_;
+    const double *fl = hd + (size_t)ldh * m_;
     reorths_ += (int64_t)fl[0];
     const int jb = fl[1] < 0 ? m_ : (int)fl[1];
     for (int j = k0; j < jb; ++j)
@@ -432,7 +434,9 @@ class KrylovSchur {
       }
     }
     hipStream_t s = op_->stream;
-    TM_HIP(hipMemcpyAsync(Ydev_.get(), Y.data(), sizeof(double) * m * k, hipMemcpyHostToDevice, s));
+    std::copy(Y.begin(), Y.begin() + (size_t)m * k, stage_.as<double>());
+    TM_HIP(hipMemcpyAsync(Ydev_.get(), stage_.as<double>(), sizeof(double) * m * k,
+                          hipMemcpyHostToDevice, s));
     kry::gemm_vy(V(0), ldv_, m, Ydev_.as<double>(), m, k, V(0), ldv_, n_, s);
     kry::copy(V(m), V(k), n_, s);
     std::fill(H_.begin(), H_.end(), 0.0);
@@ -526,6 +530,7 @@ class KrylovSchur {
   std::vector<CycleGraph> graphs_;
   int64_t graph_cycles_ = 0;
   DevBuf Hd_;
+  PinnedBuf stage_;
   bool sym_ = true;
   int k_ = 0;
   int64_t matvecs_ = 0, restarts_ = 0;

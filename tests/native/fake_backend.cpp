@@ -15,6 +15,16 @@ size_t g_lds_per_block = 160 * 1024;
 {
   throw Error(TM_EHIP, expr);
 }
+void PinnedBuf::alloc(size_t bytes)
+{
+  release();
+  p_ = bytes ? std::malloc(bytes) : nullptr;
+}
+void PinnedBuf::release()
+{
+  std::free(p_);
+  p_ = nullptr;
+}
 void DevBuf::alloc(size_t bytes)
 {
   release();

@@ -7,6 +7,7 @@
 #pragma once
 #include <cstddef>
 #include <cstring>
+#include <cstdlib>
 typedef int hipError_t;
 typedef void *hipStream_t;
 enum { hipSuccess = 0, hipErrorOutOfMemory = 2 };
@@ -27,3 +28,6 @@ inline hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t, void *, voi
 inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return 1; }
 inline hipError_t hipGraphDestroy(hipGraph_t) { return 0; }
 inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return 0; }
+enum { hipHostMallocDefault = 0 };
+inline hipError_t hipHostMalloc(void **p, size_t n, unsigned) { *p = std::malloc(n); return *p ? 0 : 2; }
+inline hipError_t hipHostFree(void *p) { std::free(p); return 0; }
