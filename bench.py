@@ -45,7 +45,7 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="host threads of the scan batch")
     ap.add_argument("--profile-tx", action="store_true",
                     help="only a calibration copy and a few T.x applies (for counter runs)")
-    ap.add_argument("--workload", default="reduced", choices=("reduced", "scan", "3d"),
+    ap.add_argument("--workload", default="reduced", choices=("reduced", "scan", "3d", "grad"),
                     help="reduced: the headline line; scan: 96-instance BNNNI batch; "
                          "3d: 5x5 3D layer, reduced vs full space")
     args = ap.parse_args()
@@ -62,6 +62,8 @@ def main():
         return bench_scan(lt, args)
     if args.workload == "3d":
         return bench_3d(lt, args)
+    if args.workload == "grad":
+        return bench_grad(lt, args)
 
     L, bJ, kappa, neigs = args.L, 0.4, 0.5, 4
     t0 = time.perf_counter()
@@ -242,6 +244,16 @@ def bench_scan(lt, args):
     per_L = {}
     for r, (_, L) in zip(seq, inst):
         per_L.setdefault(L, []).append(r.seconds)
+    # one T.x of the first instance of each width (FP64 vectors; FP64 and
+    # FP32 arithmetic), HIP events on the operator's stream
+    tx = {}
+    for op, (_, L) in zip(ops, inst):
+        if str(L) in tx:
+            continue
+        m64 = op.time_apply(lt.TM_F64, lt.TM_F64, warmup=2, reps=10)
+        m32 = op.time_apply(lt.TM_F64, lt.TM_F32, warmup=2, reps=10)
+        tx[str(L)] = {"fp64_ms": round(m64, 4), "fp32_ms": round(m32, 4),
+                      "fp64_GBs_vec": round(op.cols() * 16 / (m64 * 1e-3) / 1e9, 1)}
     out = {
         "metric": "BNNNI parameter scan: eigensolves/s over 96 instances (batch, mixed precision)",
         "value": round(len(ops) / t_bmx, 3), "unit": "eigensolves/s", "n_gpus": 1,
@@ -257,6 +269,7 @@ def bench_scan(lt, args):
         "max_rel_err_lambda0_mixed_vs_fp64": err, "max_rel_err_lambda0_batch_vs_seq": err_b,
         "matvecs": mv,
         "mean_solve_s_by_width": {str(L): round(float(np.mean(v)), 5) for L, v in sorted(per_L.items())},
+        "tx_by_width": tx,
         "all_converged": bool(all(r.nconv >= 2 for r in bmx + b64 + seq)),
     }
     print(json.dumps(out), flush=True)
@@ -311,6 +324,56 @@ def bench_3d(lt, args):
         "tx_full": {"fp32_ms": round(ms32, 4), "fp64_ms": round(ms64, 4),
                     "fp32_GBs": round(gbs32, 1), "fp64_GBs": round(gbs64, 1),
                     "fp32_frac_hbm": round(gbs32 / HBM_PEAK_GBS, 4)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_grad(lt, args):
+    """DNNI perpendicular-TM (D, axialdir 1), L=22, betaJ 0.6, kappa1 0.7,
+    h 0.1: gradient of f = -ln(lambda0)/L with respect to (betaJ, kappa1, h)
+    by Hellmann-Feynman (one eigensolve + tm_grad_hf) against central finite
+    differences (step 1e-5, six more eigensolves), FP64."""
+    import numpy as np
+    L, th0 = 22, np.array([0.6, 0.7, 0.0, 0.1])
+
+    def make(th):
+        return lt.TransferMatrix("D", 2, 1, L, J=th[0], kappa=th[1], kappa2=th[2], h=th[3])
+
+    def f_of(th):
+        op = make(th)
+        r = lt.solve(op, nev=1, tol=1e-12)
+        return -np.log(r.evals[0] / op.getnormfactor()) / L
+    op = make(th0)
+    lt.solve(op, nev=1, tol=1e-12, want_vectors=True)  # warm-up
+    ts = time.perf_counter()
+    r = lt.solve(op, nev=1, tol=1e-12, want_vectors=True)
+    t_solve = time.perf_counter() - ts
+    v = r.vecs[:, 0]
+    ts = time.perf_counter()
+    dlam, dlnnf = op.grad_hf(v, v)
+    t_grad = time.perf_counter() - ts
+    lam = r.evals[0]
+    g_hf = -(dlam / lam - dlnnf) / L
+    ts = time.perf_counter()
+    g_fd = np.zeros(4)
+    for i in (0, 1, 3):
+        tp, tm = th0.copy(), th0.copy()
+        tp[i] += 1e-5
+        tm[i] -= 1e-5
+        g_fd[i] = (f_of(tp) - f_of(tm)) / 2e-5
+    t_fd = time.perf_counter() - ts
+    rel = [abs(g_hf[i] - g_fd[i]) / abs(g_fd[i]) for i in (0, 1, 3)]
+    out = {
+        "metric": "DNNI L=22 free-energy gradient: Hellmann-Feynman vs central finite differences",
+        "value": round(1.0 / (t_solve + t_grad), 4), "unit": "gradients/s (solve + HF)",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic (model parameters)",
+        "config": {"workload": "DNNI perpendicular-TM (D, axialdir 1), L=22, betaJ 0.6, kappa1 0.7, "
+                               "h 0.1, neigs 1, f = -ln(lambda0)/L", "states": int(op.cols())},
+        "grad_hf": {"betaJ": g_hf[0], "kappa1": g_hf[1], "kappa2": g_hf[2], "h": g_hf[3]},
+        "grad_fd": {"betaJ": g_fd[0], "kappa1": g_fd[1], "h": g_fd[3]},
+        "rel_err_hf_vs_fd": {"betaJ": rel[0], "kappa1": rel[1], "h": rel[2]},
+        "seconds": {"solve": round(t_solve, 4), "hf_gradient": round(t_grad, 4),
+                    "fd_six_solves": round(t_fd, 3)},
     }
     print(json.dumps(out), flush=True)
 

@@ -46,11 +46,26 @@ __global__ __launch_bounds__(KRON_THREADS) void k_kron_pass(
   }
   __syncthreads();
 
-  for (int e = threadIdx.x; e < T; e += blockDim.x) {
-    const uint64_t g = base | ((uint64_t)(e >> logW) << b0) | (uint64_t)(e & (W - 1));
-    C v = (C)in[g] * sin;
-    if (flags & F_PRO) v *= dev::diag_value<C>(dg, stab, g);
-    lds[dev::sw(e)] = v;
+  // UNR independent loads per thread before the LDS stores (several HBM
+  // requests in flight per wave)
+  constexpr int UNR = 8;
+  for (int e0 = threadIdx.x; e0 < T; e0 += blockDim.x * UNR) {
+    C v[UNR];
+#pragma unroll
+    for (int uu = 0; uu < UNR; ++uu) {
+      const int e = e0 + uu * blockDim.x;
+      v[uu] = 0;
+      if (e < T) {
+        const uint64_t g = base | ((uint64_t)(e >> logW) << b0) | (uint64_t)(e & (W - 1));
+        v[uu] = (C)in[g] * sin;
+        if (flags & F_PRO) v[uu] *= dev::diag_value<C>(dg, stab, g);
+      }
+    }
+#pragma unroll
+    for (int uu = 0; uu < UNR; ++uu) {
+      const int e = e0 + uu * blockDim.x;
+      if (e < T) lds[dev::sw(e)] = v[uu];
+    }
   }
   __syncthreads();
 

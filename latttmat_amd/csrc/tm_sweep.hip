@@ -69,33 +69,73 @@ __device__ __forceinline__ void xdt_gate(C v[2][2], int c, const Coef<C> &k)
 
 // ZBT gate: in[aq][u] (aq = aux pair (al,ar), u = (bit 2t+1, bit 2t)),
 // controls (fl, fr).  Output aux p = (pl,pr) takes the old bit pair, the
-// new bit pair is s = (sl, sr).
+// new bit pair is s = (sl, sr):
+//   out[p][s] = A[p][s] F_{fl,fr}[s] sum_aq Cm[s][aq] in[aq][p]
+// with the control-independent factors A, Cm tabulated once per thread
+// (ZCoef) and F built from two selects per item.
+//   T   (DIR 0): A = b1[sr^pl] b3[sr^pr] b3[sl^pl], Cm = b1[sr^al] b2[sr^ar] b2[sl^al],
+//                F = b2[sr^fr] b2[sl^fl]
+//   T^T (DIR 1): A = b1[sl^pr] b3[sr^pr] b3[sl^pl], Cm = b2[sr^ar] b2[sl^al],
+//                F = b1[sl^fr] b2[sr^fr] b2[sl^fl]
+template <typename C>
+struct ZCoef {
+  C A[4][4], Cm[4][4];
+  C b1[2], b2[2];
+};
+
 template <typename C, int DIR>
-__device__ __forceinline__ void zbt_gate(C v[4][4], int fl, int fr, const Coef<C> &k)
+__device__ __forceinline__ ZCoef<C> make_zcoef(const Coef<C> &k)
 {
+  ZCoef<C> z;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int sl = s >> 1, sr = s & 1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int ql = q >> 1, qr = q & 1;  // q = p (for A) or aq (for Cm)
+      if (DIR == 0) {
+        z.A[q][s] = k.b1[sr ^ ql] * k.b3[sr ^ qr] * k.b3[sl ^ ql];
+        z.Cm[s][q] = k.b1[sr ^ ql] * k.b2[sr ^ qr] * k.b2[sl ^ ql];
+      } else {
+        z.A[q][s] = k.b1[sl ^ qr] * k.b3[sr ^ qr] * k.b3[sl ^ ql];
+        z.Cm[s][q] = k.b2[sr ^ qr] * k.b2[sl ^ ql];
+      }
+    }
+  }
+  for (int i = 0; i < 2; ++i) {
+    z.b1[i] = k.b1[i];
+    z.b2[i] = k.b2[i];
+  }
+  return z;
+}
+
+template <typename C, int DIR>
+__device__ __forceinline__ void zbt_gate(C v[4][4], int fl, int fr, const ZCoef<C> &z)
+{
+  // F[s] for this item's controls
+  const C ur0 = fr ? z.b2[1] : z.b2[0], ur1 = fr ? z.b2[0] : z.b2[1];  // b2[sr ^ fr]
+  const C ul0 = fl ? z.b2[1] : z.b2[0], ul1 = fl ? z.b2[0] : z.b2[1];  // b2[sl ^ fl]
+  C F[4];
+  F[0] = ul0 * ur0;
+  F[1] = ul0 * ur1;
+  F[2] = ul1 * ur0;
+  F[3] = ul1 * ur1;
+  if (DIR == 1) {
+    const C g0 = fr ? z.b1[1] : z.b1[0], g1 = fr ? z.b1[0] : z.b1[1];  // b1[sl ^ fr]
+    F[0] *= g0;
+    F[1] *= g0;
+    F[2] *= g1;
+    F[3] *= g1;
+  }
   C o[4][4];
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
-    const int pl = p >> 1, pr = p & 1;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const int sl = s >> 1, sr = s & 1;
-      C w;
-      if (DIR == 0)
-        w = k.b1[sr ^ pl] * k.b2[sr ^ fr] * k.b2[sl ^ fl] * k.b3[sr ^ pr] * k.b3[sl ^ pl];
-      else
-        w = k.b1[sl ^ fr] * k.b1[sl ^ pr] * k.b2[sr ^ fr] * k.b2[sl ^ fl] * k.b3[sr ^ pr] *
-            k.b3[sl ^ pl];
-      C acc = 0;
+      C acc = z.Cm[s][0] * v[0][p];
 #pragma unroll
-      for (int aq = 0; aq < 4; ++aq) {
-        const int al = aq >> 1, ar = aq & 1;
-        C c;
-        if (DIR == 0) c = k.b1[sr ^ al] * k.b2[sr ^ ar] * k.b2[sl ^ al];
-        else c = k.b2[sr ^ ar] * k.b2[sl ^ al];
-        acc += v[aq][p] * c;
-      }
-      o[p][s] = w * acc;
+      for (int aq = 1; aq < 4; ++aq) acc += z.Cm[s][aq] * v[aq][p];
+      o[p][s] = (z.A[p][s] * F[s]) * acc;
     }
   }
 #pragma unroll
@@ -125,8 +165,8 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restri
     k.b2[i] = dev::mk<C>(a.b2[i], a.db2[i]);
     k.b3[i] = dev::mk<C>(a.b3[i], a.db3[i]);
   }
+  const ZCoef<C> zk = make_zcoef<C, DIR>(k);
   const int L = a.L;
-  const uint64_t N = 1ull << L;
   const uint64_t tile = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
   const uint64_t innerBlocks = (1ull << a.bitlo) >> a.logW;
   const uint64_t o = tile / innerBlocks, ib = tile % innerBlocks;
@@ -135,6 +175,15 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restri
   const int pmask = (1 << a.nbits) - 1;
   const int posAux = a.logW + a.nbits;
   const int posLb = posAux + a.nal;
+  // The aux array is state-major in HBM: Z[(P << lgz) | z], z = (lastbit,
+  // aux), so the 2^lgz values of a state are contiguous and a tile's runs
+  // are W * 2^lgz elements long.  The load and store loops walk the tile in
+  // that order; the LDS tile keeps (aux, lastbit) in its high bits and XORs
+  // them into the low bits (swz) so that those walks stay conflict-free.
+  const int lgz = a.nlbl + a.nal;
+  const int nz = 1 << lgz;
+  const int zx = (1 << (lgz < posAux ? lgz : posAux)) - 1;
+  auto swz = [&](int e) { return e ^ ((e >> posAux) & zx); };
   C sin = 1, sout = 1;
   if (scale) {
     if (a.first) sin = (C)(*scale);
@@ -143,38 +192,57 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restri
   __syncthreads();
 
   // ---- load / build the tile ----
-  for (int e = threadIdx.x; e < T; e += blockDim.x) {
-    const int w = e & (W - 1);
-    const int pl = (e >> a.logW) & pmask;
-    const int aux = (e >> posAux) & (a.na - 1);
-    const int lb = e >> posLb;
-    const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
-    C v;
-    if (a.first) {
-      v = 0;
-      if (MODEL == 0) {
-        if (a.bc == 0) {
-          const int auxw = (int)((P >> (L - 1)) & 1);
-          if ((int)(P & 1) == lb && aux == auxw)
-            v = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P);
-        } else {
-          // confined ends, TMat/tmat.cpp:440-458
-          const int top = (int)(P >> (L - 1));
-          const int want = a.bc == 5 ? 0 : 1;
-          if (aux == want) {
-            const int low = (int)(P & 1) ^ (a.bc == 5 ? 0 : 1);
-            v = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P) * k.b2[low] * k.b2[top];
+  // UNR independent loads per thread are issued before the LDS stores, so
+  // a wave keeps several HBM requests in flight (one workgroup per CU).
+  constexpr int UNR = 8;
+  for (int g0 = threadIdx.x; g0 < T; g0 += blockDim.x * UNR) {
+    C v[UNR];
+#pragma unroll
+    for (int uu = 0; uu < UNR; ++uu) {
+      const int g = g0 + uu * blockDim.x;
+      v[uu] = 0;
+      if (g >= T) continue;
+      const int z = g & (nz - 1);
+      const int rest = g >> lgz;
+      const int w = rest & (W - 1);
+      const int pl = rest >> a.logW;
+      const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
+      if (a.first) {
+        const int aux = z & (a.na - 1);
+        const int lb = z >> a.nal;
+        if (MODEL == 0) {
+          if (a.bc == 0) {
+            const int auxw = (int)((P >> (L - 1)) & 1);
+            if ((int)(P & 1) == lb && aux == auxw)
+              v[uu] = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P);
+          } else {
+            // confined ends, TMat/tmat.cpp:440-458
+            const int top = (int)(P >> (L - 1));
+            const int want = a.bc == 5 ? 0 : 1;
+            if (aux == want) {
+              const int low = (int)(P & 1) ^ (a.bc == 5 ? 0 : 1);
+              v[uu] = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P) * k.b2[low] * k.b2[top];
+            }
           }
+        } else {
+          const int auxw = (int)((P >> (L - 2)) & 3);
+          if ((int)(P & 3) == lb && aux == auxw)
+            v[uu] = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P);
         }
       } else {
-        const int auxw = (int)((P >> (L - 2)) & 3);
-        if ((int)(P & 3) == lb && aux == auxw)
-          v = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P);
+        v[uu] = zin[(P << lgz) | (uint64_t)z];
       }
-    } else {
-      v = zin[((uint64_t)lb * a.na + aux) * N + P];
     }
-    lds[e] = v;
+#pragma unroll
+    for (int uu = 0; uu < UNR; ++uu) {
+      const int g = g0 + uu * blockDim.x;
+      if (g >= T) continue;
+      const int z = g & (nz - 1);
+      const int rest = g >> lgz;
+      const int w = rest & (W - 1);
+      const int pl = rest >> a.logW;
+      lds[swz(w | (pl << a.logW) | (z << posAux))] = v[uu];
+    }
   }
   __syncthreads();
 
@@ -198,12 +266,12 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restri
 #pragma unroll
         for (int ax = 0; ax < 2; ++ax)
 #pragma unroll
-          for (int u = 0; u < 2; ++u) v[ax][u] = lds[b0 | (ax << posAux) | (u << jb)];
+          for (int u = 0; u < 2; ++u) v[ax][u] = lds[swz(b0 | (ax << posAux) | (u << jb))];
         xdt_gate(v, c, k);
 #pragma unroll
         for (int ax = 0; ax < 2; ++ax)
 #pragma unroll
-          for (int u = 0; u < 2; ++u) lds[b0 | (ax << posAux) | (u << jb)] = v[ax][u];
+          for (int u = 0; u < 2; ++u) lds[swz(b0 | (ax << posAux) | (u << jb))] = v[ax][u];
       }
     } else {
       const int jb = a.logW + (2 * t - a.bitlo);  // tile position of bit 2t
@@ -229,12 +297,12 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restri
 #pragma unroll
         for (int ax = 0; ax < 4; ++ax)
 #pragma unroll
-          for (int u = 0; u < 4; ++u) v[ax][u] = lds[b0 | (ax << posAux) | (u << jb)];
-        zbt_gate<C, DIR>(v, fl, fr, k);
+          for (int u = 0; u < 4; ++u) v[ax][u] = lds[swz(b0 | (ax << posAux) | (u << jb))];
+        zbt_gate<C, DIR>(v, fl, fr, zk);
 #pragma unroll
         for (int ax = 0; ax < 4; ++ax)
 #pragma unroll
-          for (int u = 0; u < 4; ++u) lds[b0 | (ax << posAux) | (u << jb)] = v[ax][u];
+          for (int u = 0; u < 4; ++u) lds[swz(b0 | (ax << posAux) | (u << jb))] = v[ax][u];
       }
     }
     __syncthreads();
@@ -242,24 +310,24 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restri
 
   // ---- store ----
   if (!a.last) {
-    for (int e = threadIdx.x; e < T; e += blockDim.x) {
-      const int w = e & (W - 1);
-      const int pl = (e >> a.logW) & pmask;
-      const int aux = (e >> posAux) & (a.na - 1);
-      const int lb = e >> posLb;
+    for (int g = threadIdx.x; g < T; g += blockDim.x) {
+      const int z = g & (nz - 1);
+      const int rest = g >> lgz;
+      const int w = rest & (W - 1);
+      const int pl = rest >> a.logW;
+      const int e = w | (pl << a.logW) | (z << posAux);
       const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
-      zout[((uint64_t)lb * a.na + aux) * N + P] = lds[e];
+      zout[(P << lgz) | (uint64_t)z] = lds[swz(e)];
     }
   } else {
     const int NP = 1 << (a.nbits + a.logW);
-    const int nz = a.nlb * a.na;
     const C fac = dev::mk<C>(a.outfac, a.doutfac) * sout;
     for (int e = threadIdx.x; e < NP; e += blockDim.x) {
       const int w = e & (W - 1);
       const int pl = e >> a.logW;
       const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
       C acc = 0;
-      for (int z = 0; z < nz; ++z) acc += lds[(z << posAux) | e];
+      for (int z = 0; z < nz; ++z) acc += lds[swz((z << posAux) | e)];
       y[P] = (TO)(fac * dev::diag_value<C>(a.dg, stab, P) * acc);
     }
   }
@@ -268,7 +336,8 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restri
 struct SPass { int t0, ng, bitlo, nbits, logW; };
 
 // Tiles of 128 KB (2^14 FP64, 2^15 FP32, 2^13 dual elements) on the gfx950
-// LDS, half that on a smaller one; 64-byte contiguous runs.
+// LDS, half that on a smaller one; passes after the first take W >= 64 bytes
+// of consecutive states (each state's 2^lgz aux values are contiguous too).
 std::vector<SPass> plan_sweep(int model, int L, int bc, int es)
 {
   const bool big = g_lds_per_block >= (size_t)(136 * 1024);
