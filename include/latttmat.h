@@ -71,6 +71,9 @@ typedef struct {
 /* Force the generic expand/butterfly/gather kernels for reduced operators
  * (disables the two-pass fused kernel; used for cross-checking). */
 #define TM_FLAG_GENERIC_REDUCED 2
+/* Keep the default tile sizes: no timing of candidate tile sizes on the first
+ * apply of each (kernel, width, precision). */
+#define TM_FLAG_NO_AUTOTUNE 4
 
 typedef struct tm_op tm_op;
 

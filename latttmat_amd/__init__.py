@@ -17,6 +17,7 @@ from ._abi import (  # noqa: F401
     TM_F64,
     TM_FLAG_FULLSPACE,
     TM_FLAG_GENERIC_REDUCED,
+    TM_FLAG_NO_AUTOTUNE,
     TM_PREC_F64,
     TM_PREC_MIXED,
     TMError,
@@ -29,7 +30,7 @@ from ._abi import (  # noqa: F401
 )
 
 __all__ = [
-    "LIB_PATH", "TM_F32", "TM_F64", "TM_FLAG_FULLSPACE", "TM_FLAG_GENERIC_REDUCED",
+    "LIB_PATH", "TM_F32", "TM_F64", "TM_FLAG_FULLSPACE", "TM_FLAG_GENERIC_REDUCED", "TM_FLAG_NO_AUTOTUNE",
     "TM_PREC_F64", "TM_PREC_MIXED", "TMError", "TransferMatrix", "SolveResult",
     "exported_symbols", "lib", "solve", "solve_batch",
 ]

@@ -15,6 +15,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -49,6 +50,15 @@ struct Error : std::runtime_error {
 #define TM_LAUNCH_CHECK() TM_HIP(hipGetLastError())
 
 // ---------------------------------------------------------------------------
+// Autotuner for tile sizes.  For a (kernel, L, element size) key, the first
+// call outside graph capture with tune == true runs every candidate tile
+// exponent through run(t) on stream s (one warm-up, then three timed with
+// HIP events), caches the fastest and returns it; later calls with tune ==
+// true return the cached choice.  With tune == false, or during a capture
+// before a choice exists, returns cands[0] (the LDS-size default).
+int autotune_tile(uint32_t kernel, int L, int es, const std::vector<int> &cands, bool tune,
+                  hipStream_t s, const std::function<void(int)> &run);
+
 // Stream-ordered copy that returns once the data has arrived.  The library
 // never uses the legacy null stream, whose implicit synchronisation with
 // every blocking stream would break a graph capture running on another
@@ -153,6 +163,7 @@ struct KronPlan {
   const double *tab64;      // device, 192 doubles
   const float *tab32;       // device, 192 floats
   double ka, kb;
+  bool tune = false;        // let the autotuner pick the tile size
 };
 // Dual-number apply: out (2 doubles per state: T in, dT/dtheta in) from a
 // real input; tab_dual holds 192 (value, derivative) pairs and dka/dkb the
@@ -175,6 +186,7 @@ struct SweepPlan {
   double b1[2], b2[2], b3[2];
   double nf;          // XDT output factor (ZBT folds it into the diagonal)
   double db1[2], db2[2], db3[2], dnf;  // derivatives (dual-number apply only)
+  bool tune = false;  // let the autotuner pick the tile size
 };
 void launch_sweep_dual(const SweepPlan &plan, const double *in, void *out, int transpose,
                        void *work, const void *tab_dual, hipStream_t s);

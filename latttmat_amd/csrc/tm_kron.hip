@@ -97,11 +97,23 @@ struct Pass { int b0, nb, logW; };
 // Passes after the first move 64-byte contiguous runs.
 int log2_es(int es) { return es == 4 ? 2 : (es == 8 ? 3 : 4); }
 
-std::vector<Pass> plan_passes(int L, int es)
+// default tile exponent, and the autotuner's candidates (default first)
+int kron_tile_default(int es)
+{
+  const bool big = g_lds_per_block >= (size_t)(96 * 1024);
+  return (big ? 16 : 15) - log2_es(es);
+}
+std::vector<int> kron_tile_candidates(int es)
+{
+  const int d = kron_tile_default(es);
+  std::vector<int> c{d, d - 1};
+  if (((size_t(1) << (d + 1)) + 192) * (size_t)es <= g_lds_per_block) c.push_back(d + 1);
+  return c;
+}
+
+std::vector<Pass> plan_passes(int L, int es, int tl)
 {
   std::vector<Pass> ps;
-  const bool big = g_lds_per_block >= (size_t)(96 * 1024);
-  const int tl = (big ? 16 : 15) - log2_es(es);
   const int wl = 6 - log2_es(es);
   int nb0 = L < tl ? L : tl;
   ps.push_back({0, nb0, 0});
@@ -133,10 +145,10 @@ void run_pass(const Pass &p, int L, const void *in, void *out, int flags, const 
 
 template <typename C>
 void kron_typed(const KronPlan &plan, const void *in, void *out, tm_dtype io_in, tm_dtype io_out,
-                void *work, const double *scale, hipStream_t s, const C *tab)
+                void *work, const double *scale, hipStream_t s, const C *tab, int tl)
 {
   const int L = plan.L;
-  auto ps = plan_passes(L, (int)sizeof(C));
+  auto ps = plan_passes(L, (int)sizeof(C), tl);
   const int np = (int)ps.size();
   const C ka = (C)plan.ka, kb = (C)plan.kb;
   // Intermediate buffer: the output itself when it has the compute type,
@@ -183,27 +195,36 @@ void launch_kron_dual(const KronPlan &plan, const double *in, void *out, const v
 {
   using dev::Dual;
   const int L = plan.L;
-  auto ps = plan_passes(L, (int)sizeof(Dual));
-  const int np = (int)ps.size();
   const Dual ka(plan.ka, dka), kb(plan.kb, dkb);
   const Dual *tab = static_cast<const Dual *>(tab_dual);
   Dual *o = static_cast<Dual *>(out);
-  for (int i = 0; i < np; ++i) {
-    int flags = 0;
-    if (i == 0 && plan.prologue) flags |= F_PRO;
-    if (i == np - 1 && plan.epilogue) flags |= F_EPI;
-    if (i == 0) run_pass<Dual, double, Dual>(ps[i], L, in, o, flags, plan, tab, ka, kb, nullptr, s);
-    else run_pass<Dual, Dual, Dual>(ps[i], L, o, o, flags, plan, tab, ka, kb, nullptr, s);
-  }
+  auto run = [&](int tl) {
+    auto ps = plan_passes(L, (int)sizeof(Dual), tl);
+    const int np = (int)ps.size();
+    for (int i = 0; i < np; ++i) {
+      int flags = 0;
+      if (i == 0 && plan.prologue) flags |= F_PRO;
+      if (i == np - 1 && plan.epilogue) flags |= F_EPI;
+      if (i == 0) run_pass<Dual, double, Dual>(ps[i], L, in, o, flags, plan, tab, ka, kb, nullptr, s);
+      else run_pass<Dual, Dual, Dual>(ps[i], L, o, o, flags, plan, tab, ka, kb, nullptr, s);
+    }
+  };
+  const int es = (int)sizeof(Dual);
+  run(autotune_tile(1, L, es, kron_tile_candidates(es), plan.tune, s, run));
 }
 
 void launch_kron(const KronPlan &plan, const void *in, void *out, tm_dtype io_in, tm_dtype io_out,
                  tm_dtype comp, void *work, const double *scale, hipStream_t s)
 {
-  if (comp == TM_F64)
-    kron_typed<double>(plan, in, out, io_in, io_out, work, scale, s, plan.tab64);
-  else
-    kron_typed<float>(plan, in, out, io_in, io_out, work, scale, s, plan.tab32);
+  const int es = comp == TM_F64 ? 8 : 4;
+  auto run = [&](int tl) {
+    if (comp == TM_F64)
+      kron_typed<double>(plan, in, out, io_in, io_out, work, scale, s, plan.tab64, tl);
+    else
+      kron_typed<float>(plan, in, out, io_in, io_out, work, scale, s, plan.tab32, tl);
+  };
+  // timing candidates rewrites out from in, so only out-of-place applies tune
+  run(autotune_tile(0, plan.L, es, kron_tile_candidates(es), plan.tune && in != out, s, run));
 }
 
 }  // namespace ltm

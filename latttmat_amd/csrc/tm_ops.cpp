@@ -3,6 +3,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <limits>
+#include <map>
+#include <tuple>
 #include <mutex>
 #include <shared_mutex>
 #include <vector>
@@ -36,6 +39,46 @@ CaptureGuard::~CaptureGuard()
 {
   t_capturing = false;
   g_capture_mu.unlock();
+}
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<uint32_t, int, int>, int> g_tune;
+}  // namespace
+
+int autotune_tile(uint32_t kernel, int L, int es, const std::vector<int> &cands, bool tune,
+                  hipStream_t s, const std::function<void(int)> &run)
+{
+  if (!tune || cands.size() < 2) return cands[0];
+  const auto key = std::make_tuple(kernel, L, es);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) return it->second;
+  }
+  if (t_capturing) return cands[0];
+  hipEvent_t e0, e1;
+  TM_HIP(hipEventCreate(&e0));
+  TM_HIP(hipEventCreate(&e1));
+  int best = cands[0];
+  float best_ms = std::numeric_limits<float>::max();
+  for (int t : cands) {
+    run(t);
+    TM_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < 3; ++r) run(t);
+    TM_HIP(hipEventRecord(e1, s));
+    TM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    TM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return g_tune.emplace(key, best).first->second;
 }
 
 void destroy_graph_exec(hipGraphExec_t g)
@@ -443,6 +486,7 @@ void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtyp
         kp.tab32 = tab32.as<float>();
         kp.ka = bzwj1[0];
         kp.kb = bzwj1[1];
+        kp.tune = !(p.flags & TM_FLAG_NO_AUTOTUNE);
         work.ensure(kron_work_bytes(L, comp));
         launch_kron(kp, xv, yv, io, io, comp, work.get(), scale_in, s);
         break;
@@ -462,6 +506,7 @@ void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtyp
           sp.b3[i] = bzwj3[i];
         }
         sp.nf = nf;
+        sp.tune = !(p.flags & TM_FLAG_NO_AUTOTUNE);
         work.ensure(sweep_work_bytes(kind, L, comp));
         launch_sweep(sp, xv, yv, io, comp, kind == TM_KIND_ZBT ? transpose : 0, work.get(),
                      scale_in, s);

@@ -338,11 +338,18 @@ struct SPass { int t0, ng, bitlo, nbits, logW; };
 // Tiles of 128 KB (2^14 FP64, 2^15 FP32, 2^13 dual elements) on the gfx950
 // LDS, half that on a smaller one; passes after the first take W >= 64 bytes
 // of consecutive states (each state's 2^lgz aux values are contiguous too).
-std::vector<SPass> plan_sweep(int model, int L, int bc, int es)
+int sweep_log_es(int es) { return es == 4 ? 2 : (es == 8 ? 3 : 4); }
+// default tile exponent first, then the autotuner's other candidate
+std::vector<int> sweep_tile_candidates(int es)
 {
   const bool big = g_lds_per_block >= (size_t)(136 * 1024);
-  const int les = es == 4 ? 2 : (es == 8 ? 3 : 4);
-  const int tl = (big ? 17 : 16) - les;
+  const int d = (big ? 17 : 16) - sweep_log_es(es);
+  return {d, d - 1};
+}
+
+std::vector<SPass> plan_sweep(int model, int L, int bc, int es, int tl)
+{
+  const int les = sweep_log_es(es);
   const int wl = 6 - les;
   const int fixed = model == 0 ? (bc == 0 ? 2 : 1) : 4;  // log2(nlb * na)
   const int bpg = model == 0 ? 1 : 2;                      // state bits per gate
@@ -426,11 +433,11 @@ SweepArgs make_args(const SweepPlan &plan)
 // TI/TO: vector types in/out (TI = TO = the io type, or double -> Dual for
 // the dual-number kernels).  The passes work in place in the aux array.
 template <int MODEL, int DIR, typename C, typename TI, typename TO>
-void sweep_run(const SweepPlan &plan, const void *in, void *out, void *work, const double *scale,
-               hipStream_t s, const C *tab)
+void sweep_run_tl(const SweepPlan &plan, const void *in, void *out, void *work,
+                  const double *scale, hipStream_t s, const C *tab, int tl)
 {
   SweepArgs a = make_args<MODEL, DIR>(plan);
-  auto ps = plan_sweep(MODEL, plan.L, plan.bc, (int)sizeof(C));
+  auto ps = plan_sweep(MODEL, plan.L, plan.bc, (int)sizeof(C), tl);
   C *z = static_cast<C *>(work);
   const int np = (int)ps.size();
   for (int i = 0; i < np; ++i) {
@@ -439,6 +446,18 @@ void sweep_run(const SweepPlan &plan, const void *in, void *out, void *work, con
     const double *sc = (a.first || a.last) ? scale : nullptr;
     run_sweep_pass<MODEL, DIR, C, TI, TO>(ps[i], a, in, z, z, out, tab, sc, s);
   }
+}
+
+template <int MODEL, int DIR, typename C, typename TI, typename TO>
+void sweep_run(const SweepPlan &plan, const void *in, void *out, void *work, const double *scale,
+               hipStream_t s, const C *tab)
+{
+  auto run = [&](int tl) {
+    sweep_run_tl<MODEL, DIR, C, TI, TO>(plan, in, out, work, scale, s, tab, tl);
+  };
+  const int es = (int)sizeof(C);
+  const uint32_t kid = 16 + 4 * MODEL + 2 * DIR + (plan.bc != 0 ? 1 : 0);
+  run(autotune_tile(kid, plan.L, es, sweep_tile_candidates(es), plan.tune, s, run));
 }
 
 template <int MODEL, int DIR, typename C>
