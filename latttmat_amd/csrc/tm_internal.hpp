@@ -149,6 +149,7 @@ struct DiagSpec {
 // Largest dynamic LDS allocation per workgroup on the current device (bytes);
 // set when the first operator is created.  Tile sizes are chosen from it.
 extern size_t g_lds_per_block;
+extern int g_num_cu;  // compute units of the device (set at first operator creation)
 
 // ---------------------------------------------------------------------------
 // launchers (defined in the .hip files)
@@ -226,7 +227,8 @@ struct FusedTables {
   DevBuf nkw;            // uint16[2^n]   necklace index of every n-bit word
   DevBuf rotw;           // uint8[2^n]    word = rotl(NR[nkw], rotw)
   DevBuf slotO;          // uint16[nNk*16] word stored in (necklace, slot), 0xFFFF unused
-  DevBuf cmap;           // uint32[nNk << n] orbit of the state (E = NR[c], O = O')
+  DevBuf nkl;            // uint16[2^n]   n-bit words necklace-major (class, rotation)
+  DevBuf cmap;           // uint32[nNk << n] orbit of the state (E = NR[c], O = nkl[j])
   DevBuf ostart;         // int64[nNk+1]  internal-index range of the orbits whose
                          //               evaluation point has O = NR[r]
   DevBuf olist;          // uint32[M]     internal index -> orbit (ascending-rep order)

@@ -16,6 +16,7 @@
 namespace ltm {
 
 size_t g_lds_per_block = 64 * 1024;
+int g_num_cu = 256;
 
 [[noreturn]] void throw_hip(hipError_t e, const char *expr, const char *file, int line)
 {
@@ -262,6 +263,7 @@ void init_device()
   hipDeviceProp_t prop;
   TM_HIP(hipGetDeviceProperties(&prop, dev));
   g_lds_per_block = prop.sharedMemPerBlock;
+  g_num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   done = true;
 }
 

@@ -250,13 +250,14 @@ __device__ __forceinline__ uint64_t spread16(uint32_t x)
 }
 
 __global__ void k_build_cmap(GroupDev g, int n, const uint16_t *NR, uint64_t nNk,
-                             const uint64_t *bitmap, const uint64_t *wpre, uint32_t *cmap)
+                             const uint16_t *order, const uint64_t *bitmap, const uint64_t *wpre,
+                             uint32_t *cmap)
 {
   const uint64_t total = nNk << n;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t c = i >> n;
-    const uint32_t O = (uint32_t)(i & ((1ull << n) - 1));
+    const uint32_t O = order[i & ((1ull << n) - 1)];
     const uint64_t s = spread16(NR[c]) | (spread16(O) << 1);
     cmap[i] = rank_of(canon(s, g, nullptr), bitmap, wpre);
   }
@@ -443,6 +444,12 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   }
   ft.nNk = (int)NR.size();
   ft.ldA = (((int64_t)ft.nNk * 16 + 63) / 64) * 64;
+  // column order of cmap: O' words necklace-major (class, then rotation), so
+  // that a wavefront's gathers fall into a few per-row clusters
+  std::vector<uint16_t> order;
+  order.reserve(nw);
+  for (int q = 0; q < ft.nNk; ++q)
+    for (int k = 0; k < per[q]; ++k) order.push_back((uint16_t)rotl(NR[q], k));
   std::vector<uint16_t> slot((size_t)ft.nNk * 16, 0xFFFF);
   for (int r = 0; r < ft.nNk; ++r)
     for (int j = 0; j < per[r]; ++j) slot[(size_t)r * 16 + j] = (uint16_t)rotl(NR[r], j);
@@ -492,6 +499,7 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   up(ft.ostart, cnt.data(), cnt.size() * 8);
   up(ft.olist, olist.data(), olist.size() * 4);
   up(ft.Elist, elist.data(), elist.size() * 2);
+  up(ft.nkl, order.data(), order.size() * 2);
   // internal orbit numbering = position in olist (rows contiguous)
   std::vector<uint32_t> inv(M);
   for (int64_t i = 0; i < M; ++i) inv[olist[i]] = (uint32_t)i;
@@ -507,8 +515,8 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   const uint64_t total = (uint64_t)ft.nNk << n;
   ft.cmap.alloc(total * 4);
   hipLaunchKernelGGL(k_build_cmap, dim3(grid_for(total, 256)), dim3(256), 0, s, g, n,
-                     ft.NR.as<uint16_t>(), (uint64_t)ft.nNk, sc.bitmap.as<uint64_t>(),
-                     sc.wpre.as<uint64_t>(), ft.cmap.as<uint32_t>());
+                     ft.NR.as<uint16_t>(), (uint64_t)ft.nNk, ft.nkl.as<uint16_t>(),
+                     sc.bitmap.as<uint64_t>(), sc.wpre.as<uint64_t>(), ft.cmap.as<uint32_t>());
   TM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_remap_u32, dim3(grid_for(total, 256)), dim3(256), 0, s,
                      ft.cmap.as<uint32_t>(), invd.as<uint32_t>(), total);
@@ -568,43 +576,57 @@ __global__ __launch_bounds__(1024) void k_kron_small(C *u, int L, C ka, C kb)
 
 constexpr int FUSED_THREADS = 1024;
 
-// Pass 1: one necklace column E = NR[c]; u(E, O') = xt[cmap[c][O']] for all
-// O'; transform over O; store A(E, O) grouped as [row necklace][slot].
+// Pass 1: one necklace column E = NR[c]; u(E, O') = xt[cmap[c][j]] with
+// O' = nkl[j] for all O'; transform over O; store A(E, O) grouped as
+// [row necklace][slot].
 template <typename C>
 __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
-    const uint32_t *__restrict__ cmap, const C *__restrict__ xt, C *__restrict__ Ap,
-    const uint16_t *__restrict__ slotO, int n, int nNk, int64_t ldA, C ka, C kb)
+    const uint32_t *__restrict__ cmap, const uint16_t *__restrict__ nkl, const C *__restrict__ xt,
+    C *__restrict__ Ap, const uint16_t *__restrict__ slotO, int n, int nNk, int64_t ldA, C ka,
+    C kb)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
   const int c = blockIdx.x;
   const int nh = 1 << n;
   const uint32_t *mrow = cmap + ((uint64_t)c << n);
-  // 16 consecutive O' per thread: gather, then the first 4 stages in registers
-  for (int grp = threadIdx.x; grp < (nh >> 4); grp += blockDim.x) {
-    const uint4 *m4 = reinterpret_cast<const uint4 *>(mrow + (grp << 4));
-    uint4 mm[4];
+  // cmap columns are in necklace-major O' order (nkl): consecutive lanes
+  // gather from a few contiguous runs of x~; UNR gathers in flight per thread
+  constexpr int UNG = 8;
+  for (int j0 = threadIdx.x; j0 < nh; j0 += blockDim.x * UNG) {
+    uint32_t idx[UNG];
+    uint16_t O[UNG];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) mm[i] = m4[i];
-    C v[16];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      v[4 * i + 0] = xt[mm[i].x];
-      v[4 * i + 1] = xt[mm[i].y];
-      v[4 * i + 2] = xt[mm[i].z];
-      v[4 * i + 3] = xt[mm[i].w];
+    for (int u = 0; u < UNG; ++u) {
+      const int j = j0 + u * blockDim.x;
+      idx[u] = j < nh ? mrow[j] : 0u;
+      O[u] = j < nh ? nkl[j] : (uint16_t)0;
     }
-    dev::kron_regs<4>(v, ka, kb);
+    C v[UNG];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) lds[dev::sw((grp << 4) + i)] = v[i];
+    for (int u = 0; u < UNG; ++u) v[u] = xt[idx[u]];
+#pragma unroll
+    for (int u = 0; u < UNG; ++u)
+      if (j0 + u * (int)blockDim.x < nh) lds[dev::sw(O[u])] = v[u];
   }
   __syncthreads();
-  dev::lds_kron(lds, n, 4, ka, kb);
+  dev::lds_kron(lds, n, 0, ka, kb);
+  // output: UNR slot-table loads in flight per thread before the LDS reads
   C *out = Ap + (int64_t)c * ldA;
   const int ns = nNk * 16;
-  for (int t = threadIdx.x; t < ns; t += blockDim.x) {
-    const uint16_t O = slotO[t];
-    out[t] = O != 0xFFFF ? lds[dev::sw(O)] : (C)0;
+  constexpr int UNR = 8;
+  for (int t0 = threadIdx.x; t0 < ns; t0 += blockDim.x * UNR) {
+    uint16_t O[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int t = t0 + u * blockDim.x;
+      O[u] = t < ns ? slotO[t] : (uint16_t)0xFFFF;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int t = t0 + u * blockDim.x;
+      if (t < ns) out[t] = O[u] != 0xFFFF ? lds[dev::sw(O[u])] : (C)0;
+    }
   }
 }
 
@@ -648,10 +670,25 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
   __syncthreads();
   dev::lds_kron(lds, n, 0, ka, kb);
   const C inv = scale ? (C)(1.0 / *scale) : (C)1;
-  // internal numbering: the orbits of this row are [ostart[r], ostart[r+1])
+  // internal numbering: the orbits of this row are [ostart[r], ostart[r+1]);
+  // UNR index and weight loads in flight per thread before the LDS reads
   const int64_t i0 = ostart[r], i1 = ostart[r + 1];
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
-    y[i] = (TO)(dt[i] * lds[dev::sw(Elist[i])] * inv);
+  constexpr int UNR = 4;
+  for (int64_t b = i0 + threadIdx.x; b < i1; b += (int64_t)blockDim.x * UNR) {
+    uint16_t E[UNR];
+    C d[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int64_t i = b + (int64_t)u * blockDim.x;
+      E[u] = i < i1 ? Elist[i] : (uint16_t)0;
+      d[u] = i < i1 ? dt[i] : (C)0;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int64_t i = b + (int64_t)u * blockDim.x;
+      if (i < i1) y[i] = (TO)(d[u] * lds[dev::sw(E[u])] * inv);
+    }
+  }
 }
 
 template <typename C>
@@ -676,8 +713,8 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
     C *Ap = rest;
     const size_t shm = (size_t(1) << n) * sizeof(C);
     hipLaunchKernelGGL((k_fused_pass1<C>), dim3(ft.nNk), dim3(FUSED_THREADS), shm, s,
-                       ft.cmap.as<uint32_t>(), xt, Ap, ft.slotO.as<uint16_t>(), n, ft.nNk,
-                       ft.ldA, ka, kb);
+                       ft.cmap.as<uint32_t>(), ft.nkl.as<uint16_t>(), xt, Ap,
+                       ft.slotO.as<uint16_t>(), n, ft.nNk, ft.ldA, ka, kb);
     TM_LAUNCH_CHECK();
     const unsigned g2 = 8u * (unsigned)((ft.nNk + 7) / 8);
     if (io == TM_F64)

@@ -11,6 +11,7 @@
 
 namespace ltm {
 size_t g_lds_per_block = 160 * 1024;
+int g_num_cu = 256;
 [[noreturn]] void throw_hip(hipError_t, const char *expr, const char *, int)
 {
   throw Error(TM_EHIP, expr);
