@@ -45,7 +45,7 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="host threads of the scan batch")
     ap.add_argument("--profile-tx", action="store_true",
                     help="only a calibration copy and a few T.x applies (for counter runs)")
-    ap.add_argument("--workload", default="reduced", choices=("reduced", "scan", "3d", "grad"),
+    ap.add_argument("--workload", default="reduced", choices=("reduced", "scan", "3d", "grad", "small"),
                     help="reduced: the headline line; scan: 96-instance BNNNI batch; "
                          "3d: 5x5 3D layer, reduced vs full space")
     args = ap.parse_args()
@@ -64,6 +64,8 @@ def main():
         return bench_3d(lt, args)
     if args.workload == "grad":
         return bench_grad(lt, args)
+    if args.workload == "small":
+        return bench_small(lt, args)
 
     L, bJ, kappa, neigs = args.L, 0.4, 0.5, 4
     t0 = time.perf_counter()
@@ -325,6 +327,40 @@ def bench_3d(lt, args):
                     "fp32_GBs": round(gbs32, 1), "fp64_GBs": round(gbs64, 1),
                     "fp32_frac_hbm": round(gbs32 / HBM_PEAK_GBS, 4)},
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_small(lt, args):
+    """Plain method, small: 2D ANNNI perpendicular-TM full (A, axialdir 1),
+    L=12 (4096 states), betaJ 0.5, kappa1 0.6, h 0, neigs 1, FP64, right
+    vector returned (the left vector of a symmetric T is the right one,
+    pfunc.hpp:39-43).  Checked against dense LAPACK on the oracle's matrix."""
+    import numpy as np
+    op = lt.TransferMatrix("A", 2, 1, 12, J=0.5, kappa=0.6)
+    reps = max(50, args.steps)
+    for _ in range(5):
+        lt.solve(op, nev=1, want_vectors=True)
+    ts = time.perf_counter()
+    for _ in range(reps):
+        r = lt.solve(op, nev=1, want_vectors=True)
+    dt = (time.perf_counter() - ts) / reps
+    out = {
+        "metric": "XAT L=12 eigensolves/s (FP64, with the eigenvector)",
+        "value": round(1.0 / dt, 2), "unit": "eigensolves/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic (model parameters)",
+        "config": {"workload": "2D ANNNI perpendicular-TM full (A, axialdir 1), L=12, betaJ 0.5, "
+                               "kappa1 0.6, h 0, neigs 1", "states": int(op.cols())},
+        "ms_per_solve": round(dt * 1e3, 4), "matvecs": int(r.matvecs_f64),
+        "restarts": int(r.restarts), "graph_cycles": int(r.extra["graph_cycles"]),
+        "lambda0_over_nf": float(r.evals[0] / op.getnormfactor()),
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        if not os.path.exists(O._LIB_PATH):
+            O.build()
+        D = O.OracleOp(O.XAT, 12, 0, 0, 0.5, 0.6, 0.0, 0.0).dense()
+        w = np.linalg.eigvalsh(0.5 * (D + D.T))
+        out["rel_err_vs_dense_lapack"] = abs(r.evals[0] / w[-1] - 1)
     print(json.dumps(out), flush=True)
 
 

@@ -180,8 +180,10 @@ __device__ __forceinline__ void lds_kron_group(C *lds, int nbits, int bg, C a, C
   }
 }
 
+// Inlined into every caller: as a called function its registers would be
+// outside the caller's register budget (and cost the call ABI).
 template <typename C>
-__device__ void lds_kron(C *lds, int nbits, int bit0, C a, C b)
+__device__ __forceinline__ void lds_kron(C *lds, int nbits, int bit0, C a, C b)
 {
   for (int bg = bit0; bg < nbits; bg += 4) {
     const int r = nbits - bg < 4 ? nbits - bg : 4;

@@ -263,7 +263,7 @@ size_t reduced_work_bytes(const ReducedPlan &plan, tm_dtype comp);
 
 // ---- Krylov vector kernels (tm_krylov.hip) ----
 namespace kry {
-constexpr int NBLK = 1024;  // partial-sum blocks of the reductions
+constexpr int NBLK = 2048;  // partial-sum blocks of the reductions
 // out[k] = <V_k, w> for k < j, out[j] = <w,w>; partial buffer of NBLK*(j+1)
 void multidot(const double *V, int64_t ldv, int j, const double *w, int64_t n,
               double *partial, double *out, hipStream_t s);

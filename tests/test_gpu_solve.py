@@ -19,7 +19,6 @@ def test_xat_l12_dense(lt):
     op = lt.TransferMatrix("A", 2, 1, 12, J=0.5, kappa=0.6)
     r = lt.solve(op, nev=1, want_vectors=True)
     print("matvecs", r.matvecs_f64, "restarts", r.restarts, flush=True)
-    T = physics.xat_dense(12, 0.5, 0.6, 0.0) if False else None  # dense 4096^2 built below
     from oracle import oracle as O
     oop = O.OracleOp(O.XAT, 12, 0, 0, 0.5, 0.6, 0.0, 0.0)
     D = oop.dense()
