@@ -132,7 +132,7 @@ tm_status tm_op_orbits(const tm_op *op, uint64_t *reps, uint16_t *dups);
 
 /* Options of the eigen driver.  Defaults (tm_solve_opts_default): nev=1,
  * ncv=0 (=suggested_nev), tol=1e-10, maxit=1000, maxtrial=3 (pfunc.hpp:87),
- * precision FP64, seed 1, want_left=0, use_graph=1. */
+ * precision FP64, seed 1, want_left=0, use_graph=1, basis_f32=1. */
 typedef struct {
   int          nev;
   int          ncv;
@@ -154,6 +154,10 @@ typedef struct {
   int          use_graph;   /* 1 (default): restart cycles after the first
                                are captured once into a hipGraph and
                                replayed; 0: launched eagerly            */
+  int          basis_f32;   /* mixed precision, FP32 phase: 1 (default) stores
+                               the Krylov basis in FP32 (half the bytes of
+                               every orthogonalisation sweep; products and
+                               sums still in FP64); 0 stores it in FP64   */
 } tm_solve_opts;
 
 /* Results.  Eigenvalues are those of the operator as applied (the reference

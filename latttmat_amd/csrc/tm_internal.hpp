@@ -262,19 +262,31 @@ void launch_reduced(const ReducedPlan &plan, const void *in, void *out, tm_dtype
 size_t reduced_work_bytes(const ReducedPlan &plan, tm_dtype comp);
 
 // ---- Krylov vector kernels (tm_krylov.hip) ----
+// Vector arguments of type T are basis vectors stored as double or float
+// (instantiated for both); every product and sum is accumulated in FP64 and
+// the scalar results (h, out, nrm) are double.
 namespace kry {
 constexpr int NBLK = 2048;  // partial-sum blocks of the reductions
 // out[k] = <V_k, w> for k < j, out[j] = <w,w>; partial buffer of NBLK*(j+1)
-void multidot(const double *V, int64_t ldv, int j, const double *w, int64_t n,
-              double *partial, double *out, hipStream_t s);
+template <typename T>
+void multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *partial,
+              double *out, hipStream_t s);
 // w -= sum_k h[k] V_k (k<j); then out[k] = <V_k, w>, out[j] = <w,w>
-void update_dots(const double *V, int64_t ldv, int j, double *w, int64_t n,
-                 const double *h, double *partial, double *out, hipStream_t s);
+template <typename T>
+void update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
+                 double *partial, double *out, hipStream_t s);
+// update_dots for h zero outside its last two slots (a Lanczos step): w is
+// updated from V_{j-2}, V_{j-1} only, then all dots as update_dots
+template <typename T>
+void update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
+                         double *partial, double *out, hipStream_t s);
 // w -= sum_k h[k] V_k; out[0] = <w,w>
-void update_norm(const double *V, int64_t ldv, int j, double *w, int64_t n,
-                 const double *h, double *partial, double *out, hipStream_t s);
+template <typename T>
+void update_norm(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
+                 double *partial, double *out, hipStream_t s);
 // if !(w1[0] > w0[0]/2): w -= sum_k h[k] V_k, out[0] = <w,w>; else nothing
-void update_norm_cond(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h,
+template <typename T>
+void update_norm_cond(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
                       double *partial, double *out, const double *w0, const double *w1,
                       hipStream_t s);
 // finish Krylov step j on the device: projected-matrix column j (+ subdiag)
@@ -289,9 +301,10 @@ void step_finish(int j, const double *hb1, const double *hb2, const double *hb3,
 void step_close(int j, const double *hb1, const double *hb2, double *Hd, int ldh, double *nrm,
                 double *flags, hipStream_t s);
 // w <- (w - c V h2) / sqrt(nrm2[0]), c = 1 iff force or !(w1 > w0 / 2)
-void update_normalize(const double *V, int64_t ldv, int j, double *w, int64_t n,
-                      const double *h2, const double *w0, const double *w1, const double *nrm2,
-                      int force, hipStream_t s);
+template <typename T>
+void update_normalize(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h2,
+                      const double *w0, const double *w1, const double *nrm2, int force,
+                      hipStream_t s);
 // Lanczos step of a symmetric operator (j above the cycle's first step):
 // hL[0..j] = (0.., beta_{j-1} = Hd[j, j-1], alpha = hA[0])
 void lanczos_coeffs(int j, const double *hA, const double *Hd, int ldh, double *hL, hipStream_t s);
@@ -299,12 +312,19 @@ void lanczos_coeffs(int j, const double *hA, const double *Hd, int ldh, double *
 void step_close_lanczos(int j, const double *hL, const double *hb2, double *Hd, int ldh,
                         double *nrm, double *flags, hipStream_t s);
 // v = w * (1/sqrt(nrm2[0]))
-void normalize(double *w, int64_t n, const double *nrm2, hipStream_t s);
-// Vout[:, c] = sum_r Vin[:, r] * Y[r + c*ldy], r < m, c < k  (Vout may alias Vin)
-void gemm_vy(const double *Vin, int64_t ldv, int m, const double *Y, int ldy, int k,
-             double *Vout, int64_t ldo, int64_t n, hipStream_t s);
-void random_uniform(double *v, int64_t n, uint64_t seed, hipStream_t s);
-void copy(const double *a, double *b, int64_t n, hipStream_t s);
+template <typename T>
+void normalize(T *w, int64_t n, const double *nrm2, hipStream_t s);
+// Vout[:, c] = sum_r Vin[:, r] * Y[r + c*ldy], r < m, c < k  (Vout may alias
+// Vin when both have one element type).  Instantiated for (double, double),
+// (float, float) and (float, double).
+template <typename Ti, typename To>
+void gemm_vy(const Ti *Vin, int64_t ldv, int m, const double *Y, int ldy, int k, To *Vout,
+             int64_t ldo, int64_t n, hipStream_t s);
+template <typename T>
+void random_uniform(T *v, int64_t n, uint64_t seed, hipStream_t s);
+// b = a with element conversion (all four double/float pairs)
+template <typename Ta, typename Tb>
+void copy(const Ta *a, Tb *b, int64_t n, hipStream_t s);
 void to_f32(const double *a, float *b, int64_t n, hipStream_t s);
 void fill(double *a, double val, int64_t n, hipStream_t s);
 // max|x| -> 2^-ceil(log2 max) into scale (device double)

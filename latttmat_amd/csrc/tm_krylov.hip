@@ -1,4 +1,6 @@
-// tm_krylov.hip -- FP64 Krylov-basis kernels for the device eigensolver.
+// tm_krylov.hip -- Krylov-basis kernels for the device eigensolver (basis
+// stored in FP64, or in FP32 for the FP32 phase of a mixed solve; FP64
+// arithmetic throughout).
 //
 // The basis V is column-major (vector k at V + k*ldv).  Every reduction is
 // two-level and deterministic: a fixed grid of NBLK workgroups writes
@@ -7,6 +9,9 @@
 // Orthogonalisation is classical Gram-Schmidt applied twice (CGS2): the
 // update kernel subtracts V h and, in the same sweep over the data, already
 // accumulates the dot products needed by the next correction.
+#include <algorithm>
+#include <type_traits>
+
 #include "tm_device.hpp"
 
 namespace ltm {
@@ -45,10 +50,35 @@ __device__ __forceinline__ void block_reduce_write(const double (&acc)[J], int c
   }
 }
 
+// Basis vectors are stored as T = double (FP64 phase) or float (the FP32
+// phase of the mixed-precision solve); every product and sum is formed in
+// FP64 either way.  The packed kernels move 16 bytes per lane and element
+// group: Pk<T>::P elements (2 doubles or 4 floats); they need V, w 16-byte
+// aligned and ldv a multiple of P, and block 0 thread 0 takes the n % P tail.
+template <typename T> struct Pk;
+template <> struct Pk<double> {
+  typedef double V __attribute__((ext_vector_type(2)));
+  static constexpr int P = 2;
+};
+template <> struct Pk<float> {
+  typedef float V __attribute__((ext_vector_type(4)));
+  static constexpr int P = 4;
+};
+
+// sum_e a[e] b[e] in FP64
+template <typename PV, int P>
+__device__ __forceinline__ double pdot(const PV &a, const PV &b)
+{
+  double d = (double)a[0] * (double)b[0];
+#pragma unroll
+  for (int e = 1; e < P; ++e) d += (double)a[e] * (double)b[e];
+  return d;
+}
+
 // out slots: <V_k, w> for k < j, then <w, w> in slot j
-template <int J>
-__global__ __launch_bounds__(THREADS) void k_multidot(const double *__restrict__ V, int64_t ldv,
-                                                      int j, const double *__restrict__ w,
+template <int J, typename T>
+__global__ __launch_bounds__(THREADS) void k_multidot(const T *__restrict__ V, int64_t ldv,
+                                                      int j, const T *__restrict__ w,
                                                       int64_t n, double *__restrict__ partial)
 {
   double acc[J + 1];
@@ -59,7 +89,7 @@ __global__ __launch_bounds__(THREADS) void k_multidot(const double *__restrict__
     const double wi = w[i];
 #pragma unroll
     for (int k = 0; k < J; ++k)
-      if (k < j) acc[k] += V[k * ldv + i] * wi;
+      if (k < j) acc[k] += (double)V[k * ldv + i] * wi;
     acc[J] += wi * wi;
   }
   // move <w,w> into slot j
@@ -70,10 +100,11 @@ __global__ __launch_bounds__(THREADS) void k_multidot(const double *__restrict__
 }
 
 // w -= sum_k h[k] V_k; then (dots=1) slots <V_k, w>, <w,w> or (dots=0) slot 0 = <w,w>
-template <int J>
-__global__ __launch_bounds__(THREADS) void k_update_dots(const double *__restrict__ V,
+// (the products use w as stored)
+template <int J, typename T>
+__global__ __launch_bounds__(THREADS) void k_update_dots(const T *__restrict__ V,
                                                          int64_t ldv, int j,
-                                                         double *__restrict__ w, int64_t n,
+                                                         T *__restrict__ w, int64_t n,
                                                          const double *__restrict__ h,
                                                          double *__restrict__ partial, int dots)
 {
@@ -93,7 +124,9 @@ __global__ __launch_bounds__(THREADS) void k_update_dots(const double *__restric
         vk[k] = V[k * ldv + i];
         wi -= hk[k] * vk[k];
       }
-    w[i] = wi;
+    const T ws = (T)wi;
+    w[i] = ws;
+    wi = ws;
     if (dots) {
 #pragma unroll
       for (int k = 0; k < J; ++k)
@@ -125,101 +158,109 @@ __global__ __launch_bounds__(64) void k_reduce_partials(const double *__restrict
   if (threadIdx.x == 0) out[k] = s;
 }
 
-// ---- 16-byte (two-double) variants: V columns and w 16-byte aligned, ldv
-// even.  Lane p handles elements 2p and 2p+1; block 0 thread 0 adds an odd
-// tail element.  Same two-level deterministic reductions.
-template <int J>
-__global__ __launch_bounds__(THREADS) void k_multidot2(const double *__restrict__ V, int64_t ldv,
-                                                       int j, const double *__restrict__ w,
-                                                       int64_t n, double *__restrict__ partial)
+// ---- packed variants of the same kernels (same two-level reductions)
+template <int J, typename T>
+__global__ __launch_bounds__(THREADS) void k_multidot_pk(const T *__restrict__ V, int64_t ldv,
+                                                         int j, const T *__restrict__ w,
+                                                         int64_t n, double *__restrict__ partial)
 {
-  const double2 *__restrict__ V2 = reinterpret_cast<const double2 *>(V);
-  const double2 *__restrict__ w2 = reinterpret_cast<const double2 *>(w);
-  const int64_t ld2 = ldv >> 1, n2 = n >> 1;
+  using PV = typename Pk<T>::V;
+  constexpr int P = Pk<T>::P;
+  const PV *__restrict__ Vp = reinterpret_cast<const PV *>(V);
+  const PV *__restrict__ wp = reinterpret_cast<const PV *>(w);
+  const int64_t ldp = ldv / P, np = n / P;
   double acc[J + 1];
 #pragma unroll
   for (int k = 0; k <= J; ++k) acc[k] = 0;
-  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < n2;
+  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < np;
        p += (int64_t)gridDim.x * THREADS) {
-    const double2 wi = w2[p];
+    const PV wi = wp[p];
 #pragma unroll
     for (int k = 0; k < J; ++k)
-      if (k < j) {
-        const double2 v = V2[k * ld2 + p];
-        acc[k] += v.x * wi.x + v.y * wi.y;
-      }
-    acc[J] += wi.x * wi.x + wi.y * wi.y;
+      if (k < j) acc[k] += pdot<PV, P>(Vp[k * ldp + p], wi);
+    acc[J] += pdot<PV, P>(wi, wi);
   }
-  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
-    const double wi = w[n - 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int64_t i = np * P; i < n; ++i) {
+      const double wi = w[i];
 #pragma unroll
-    for (int k = 0; k < J; ++k)
-      if (k < j) acc[k] += V[k * ldv + n - 1] * wi;
-    acc[J] += wi * wi;
-  }
+      for (int k = 0; k < J; ++k)
+        if (k < j) acc[k] += (double)V[k * ldv + i] * wi;
+      acc[J] += wi * wi;
+    }
 #pragma unroll
   for (int k = 0; k < J; ++k)
     if (k == j) acc[k] = acc[J];
   block_reduce_write<J + 1>(acc, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
 }
 
-// w -= sum_k h[k] V_k; then (dots=1) slots <V_k, w>, <w,w> or (dots=0) slot 0 = <w,w>
-template <int J>
-__global__ __launch_bounds__(THREADS) void k_update_dots2(const double *__restrict__ V,
-                                                          int64_t ldv, int j,
-                                                          double *__restrict__ w, int64_t n,
-                                                          const double *__restrict__ h,
-                                                          double *__restrict__ partial, int dots,
-                                                          const double *w0p, const double *w1p)
+template <int J, typename T>
+__global__ __launch_bounds__(THREADS) void k_update_dots_pk(const T *__restrict__ V,
+                                                            int64_t ldv, int j,
+                                                            T *__restrict__ w, int64_t n,
+                                                            const double *__restrict__ h,
+                                                            double *__restrict__ partial, int dots,
+                                                            const double *w0p, const double *w1p)
 {
   // conditional form (w0p != null): only when the first projection removed
   // more than half of the norm^2; the whole grid takes the same branch
   if (w0p && (w1p[0] > 0.5 * w0p[0])) return;
+  using PV = typename Pk<T>::V;
+  constexpr int P = Pk<T>::P;
   __shared__ double hk[J];
   for (int k = threadIdx.x; k < J; k += blockDim.x) hk[k] = k < j ? h[k] : 0.0;
   __syncthreads();
-  const double2 *__restrict__ V2 = reinterpret_cast<const double2 *>(V);
-  double2 *__restrict__ w2 = reinterpret_cast<double2 *>(w);
-  const int64_t ld2 = ldv >> 1, n2 = n >> 1;
+  const PV *__restrict__ Vp = reinterpret_cast<const PV *>(V);
+  PV *__restrict__ wp = reinterpret_cast<PV *>(w);
+  const int64_t ldp = ldv / P, np = n / P;
   double acc[J + 1];
 #pragma unroll
   for (int k = 0; k <= J; ++k) acc[k] = 0;
-  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < n2;
+  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < np;
        p += (int64_t)gridDim.x * THREADS) {
-    double2 vk[J];
-    double2 wi = w2[p];
+    PV vk[J];
+    const PV wi = wp[p];
+    double wd[P];
+#pragma unroll
+    for (int e = 0; e < P; ++e) wd[e] = wi[e];
 #pragma unroll
     for (int k = 0; k < J; ++k)
       if (k < j) {
-        vk[k] = V2[k * ld2 + p];
-        wi.x -= hk[k] * vk[k].x;
-        wi.y -= hk[k] * vk[k].y;
+        vk[k] = Vp[k * ldp + p];
+#pragma unroll
+        for (int e = 0; e < P; ++e) wd[e] -= hk[k] * (double)vk[k][e];
       }
-    w2[p] = wi;
+    PV wo;
+#pragma unroll
+    for (int e = 0; e < P; ++e) wo[e] = (T)wd[e];
+    wp[p] = wo;
     if (dots) {
 #pragma unroll
       for (int k = 0; k < J; ++k)
-        if (k < j) acc[k] += vk[k].x * wi.x + vk[k].y * wi.y;
+        if (k < j) acc[k] += pdot<PV, P>(vk[k], wo);
     }
-    acc[J] += wi.x * wi.x + wi.y * wi.y;
+    acc[J] += pdot<PV, P>(wo, wo);
   }
-  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
-    double vk[J];
-    double wi = w[n - 1];
-#pragma unroll
-    for (int k = 0; k < J; ++k)
-      if (k < j) {
-        vk[k] = V[k * ldv + n - 1];
-        wi -= hk[k] * vk[k];
-      }
-    w[n - 1] = wi;
-    if (dots) {
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int64_t i = np * P; i < n; ++i) {
+      double vk[J];
+      double wi = w[i];
 #pragma unroll
       for (int k = 0; k < J; ++k)
-        if (k < j) acc[k] += vk[k] * wi;
+        if (k < j) {
+          vk[k] = V[k * ldv + i];
+          wi -= hk[k] * vk[k];
+        }
+      const T ws = (T)wi;
+      w[i] = ws;
+      wi = ws;
+      if (dots) {
+#pragma unroll
+        for (int k = 0; k < J; ++k)
+          if (k < j) acc[k] += vk[k] * wi;
+      }
+      acc[J] += wi * wi;
     }
-    acc[J] += wi * wi;
-  }
   if (dots) {
 #pragma unroll
     for (int k = 0; k < J; ++k)
@@ -231,28 +272,94 @@ __global__ __launch_bounds__(THREADS) void k_update_dots2(const double *__restri
   }
 }
 
-__global__ void k_normalize2(double *__restrict__ w, int64_t n, const double *__restrict__ nrm2)
+// Lanczos form of k_update_dots_pk (dots = 1): h is zero except in its last
+// two slots (beta_{j-1}, alpha), so w is updated from the last two basis
+// vectors first and the other vectors are streamed straight into the dot
+// products (no register array of J vectors).  Same results as the general
+// kernel: the skipped terms subtract exact zeros.
+template <int J, typename T>
+__global__ __launch_bounds__(THREADS) void k_lanczos_dots_pk(const T *__restrict__ V,
+                                                             int64_t ldv, int j,
+                                                             T *__restrict__ w, int64_t n,
+                                                             const double *__restrict__ h,
+                                                             double *__restrict__ partial)
 {
-  const double inv = 1.0 / sqrt(nrm2[0]);
-  double2 *__restrict__ w2 = reinterpret_cast<double2 *>(w);
-  const int64_t n2 = n >> 1;
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n2;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    double2 v = w2[p];
-    v.x *= inv;
-    v.y *= inv;
-    w2[p] = v;
+  using PV = typename Pk<T>::V;
+  constexpr int P = Pk<T>::P;
+  const double ha = j >= 2 ? h[j - 2] : 0.0, hb = h[j - 1];
+  const PV *__restrict__ Vp = reinterpret_cast<const PV *>(V);
+  PV *__restrict__ wp = reinterpret_cast<PV *>(w);
+  const int64_t ldp = ldv / P, np = n / P;
+  double acc[J + 1];
+#pragma unroll
+  for (int k = 0; k <= J; ++k) acc[k] = 0;
+  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < np;
+       p += (int64_t)gridDim.x * THREADS) {
+    const PV wi = wp[p];
+    const PV va = j >= 2 ? Vp[(j - 2) * ldp + p] : PV{};
+    const PV vb = Vp[(j - 1) * ldp + p];
+    PV wo;
+#pragma unroll
+    for (int e = 0; e < P; ++e) {
+      double x = wi[e];
+      if (j >= 2) x -= ha * (double)va[e];
+      x -= hb * (double)vb[e];
+      wo[e] = (T)x;
+    }
+    wp[p] = wo;
+#pragma unroll
+    for (int k = 0; k < J; ++k) {
+      if (k < j - 2) acc[k] += pdot<PV, P>(Vp[k * ldp + p], wo);
+      else if (k == j - 2) acc[k] += pdot<PV, P>(va, wo);
+      else if (k == j - 1) acc[k] += pdot<PV, P>(vb, wo);
+    }
+    acc[J] += pdot<PV, P>(wo, wo);
   }
-  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) w[n - 1] *= inv;
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int64_t i = np * P; i < n; ++i) {
+      double x = w[i];
+      if (j >= 2) x -= ha * (double)V[(j - 2) * ldv + i];
+      x -= hb * (double)V[(j - 1) * ldv + i];
+      const T ws = (T)x;
+      w[i] = ws;
+      x = ws;
+#pragma unroll
+      for (int k = 0; k < J; ++k)
+        if (k < j) acc[k] += (double)V[k * ldv + i] * x;
+      acc[J] += x * x;
+    }
+#pragma unroll
+  for (int k = 0; k < J; ++k)
+    if (k == j) acc[k] = acc[J];
+  block_reduce_write<J + 1>(acc, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
+}
+
+template <typename T>
+__global__ void k_normalize_pk(T *__restrict__ w, int64_t n, const double *__restrict__ nrm2)
+{
+  using PV = typename Pk<T>::V;
+  constexpr int P = Pk<T>::P;
+  const double inv = 1.0 / sqrt(nrm2[0]);
+  PV *__restrict__ wp = reinterpret_cast<PV *>(w);
+  const int64_t np = n / P;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    PV v = wp[p];
+#pragma unroll
+    for (int e = 0; e < P; ++e) v[e] = (T)((double)v[e] * inv);
+    wp[p] = v;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int64_t i = np * P; i < n; ++i) w[i] = (T)((double)w[i] * inv);
 }
 
 // Close a Krylov step in one sweep: w <- (w - c V h2) / beta, with c = 1
 // when the DGKS correction is due (w1 <= w0 / 2) and beta^2 = nrm2[0] as set
 // by k_step_finish (w1 - |h2|^2 after a correction, w1 otherwise).
-template <int J>
-__global__ __launch_bounds__(THREADS) void k_update_normalize(const double *__restrict__ V,
+template <int J, typename T>
+__global__ __launch_bounds__(THREADS) void k_update_normalize(const T *__restrict__ V,
                                                               int64_t ldv, int j,
-                                                              double *__restrict__ w, int64_t n,
+                                                              T *__restrict__ w, int64_t n,
                                                               const double *__restrict__ h2,
                                                               const double *w0p, const double *w1p,
                                                               const double *__restrict__ nrm2,
@@ -269,49 +376,56 @@ __global__ __launch_bounds__(THREADS) void k_update_normalize(const double *__re
     double wi = w[i];
 #pragma unroll
     for (int k = 0; k < J; ++k)
-      if (k < jj) wi -= hk[k] * V[k * ldv + i];
-    w[i] = wi * inv;
+      if (k < jj) wi -= hk[k] * (double)V[k * ldv + i];
+    w[i] = (T)(wi * inv);
   }
 }
 
-template <int J>
-__global__ __launch_bounds__(THREADS) void k_update_normalize2(const double *__restrict__ V,
-                                                               int64_t ldv, int j,
-                                                               double *__restrict__ w, int64_t n,
-                                                               const double *__restrict__ h2,
-                                                               const double *w0p,
-                                                               const double *w1p,
-                                                               const double *__restrict__ nrm2,
-                                                               int force)
+template <int J, typename T>
+__global__ __launch_bounds__(THREADS) void k_update_normalize_pk(const T *__restrict__ V,
+                                                                 int64_t ldv, int j,
+                                                                 T *__restrict__ w, int64_t n,
+                                                                 const double *__restrict__ h2,
+                                                                 const double *w0p,
+                                                                 const double *w1p,
+                                                                 const double *__restrict__ nrm2,
+                                                                 int force)
 {
+  using PV = typename Pk<T>::V;
+  constexpr int P = Pk<T>::P;
   const bool re = force || !(w1p[0] > 0.5 * w0p[0]);
   const double inv = 1.0 / sqrt(nrm2[0]);
   __shared__ double hk[J];
   for (int k = threadIdx.x; k < J; k += blockDim.x) hk[k] = (re && k < j) ? h2[k] : 0.0;
   __syncthreads();
   const int jj = re ? j : 0;
-  const double2 *__restrict__ V2 = reinterpret_cast<const double2 *>(V);
-  double2 *__restrict__ w2 = reinterpret_cast<double2 *>(w);
-  const int64_t ld2 = ldv >> 1, n2 = n >> 1;
-  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < n2;
+  const PV *__restrict__ Vp = reinterpret_cast<const PV *>(V);
+  PV *__restrict__ wp = reinterpret_cast<PV *>(w);
+  const int64_t ldp = ldv / P, np = n / P;
+  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < np;
        p += (int64_t)gridDim.x * THREADS) {
-    double2 wi = w2[p];
+    const PV wi = wp[p];
+    double wd[P];
+#pragma unroll
+    for (int e = 0; e < P; ++e) wd[e] = wi[e];
 #pragma unroll
     for (int k = 0; k < J; ++k)
       if (k < jj) {
-        const double2 v = V2[k * ld2 + p];
-        wi.x -= hk[k] * v.x;
-        wi.y -= hk[k] * v.y;
+        const PV v = Vp[k * ldp + p];
+#pragma unroll
+        for (int e = 0; e < P; ++e) wd[e] -= hk[k] * (double)v[e];
       }
-    wi.x *= inv;
-    wi.y *= inv;
-    w2[p] = wi;
+    PV wo;
+#pragma unroll
+    for (int e = 0; e < P; ++e) wo[e] = (T)(wd[e] * inv);
+    wp[p] = wo;
   }
-  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
-    double wi = w[n - 1];
-    for (int k = 0; k < jj; ++k) wi -= hk[k] * V[k * ldv + n - 1];
-    w[n - 1] = wi * inv;
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int64_t i = np * P; i < n; ++i) {
+      double wi = w[i];
+      for (int k = 0; k < jj; ++k) wi -= hk[k] * (double)V[k * ldv + i];
+      w[i] = (T)(wi * inv);
+    }
 }
 
 // One thread closes step j from the two projections alone: column j of the
@@ -390,10 +504,10 @@ __device__ __forceinline__ bool need_reorth(const double *w0p, const double *w1p
   return !(w1p[0] > 0.5 * w0p[0]);
 }
 
-template <int J>
-__global__ __launch_bounds__(THREADS) void k_update_norm_cond(const double *__restrict__ V,
+template <int J, typename T>
+__global__ __launch_bounds__(THREADS) void k_update_norm_cond(const T *__restrict__ V,
                                                               int64_t ldv, int j,
-                                                              double *__restrict__ w, int64_t n,
+                                                              T *__restrict__ w, int64_t n,
                                                               const double *__restrict__ h,
                                                               double *__restrict__ partial,
                                                               const double *w0p, const double *w1p)
@@ -408,8 +522,10 @@ __global__ __launch_bounds__(THREADS) void k_update_norm_cond(const double *__re
     double wi = w[i];
 #pragma unroll
     for (int k = 0; k < J; ++k)
-      if (k < j) wi -= hk[k] * V[k * ldv + i];
-    w[i] = wi;
+      if (k < j) wi -= hk[k] * (double)V[k * ldv + i];
+    const T ws = (T)wi;
+    w[i] = ws;
+    wi = ws;
     acc[0] += wi * wi;
   }
   block_reduce_write<1>(acc, 1, partial + (int64_t)blockIdx.x);
@@ -453,18 +569,19 @@ __global__ void k_step_finish(int j, const double *__restrict__ hb1,
   if ((beta <= 1e-13 * sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
 }
 
-__global__ void k_normalize(double *__restrict__ w, int64_t n, const double *__restrict__ nrm2)
+template <typename T>
+__global__ void k_normalize(T *__restrict__ w, int64_t n, const double *__restrict__ nrm2)
 {
   const double inv = 1.0 / sqrt(nrm2[0]);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    w[i] *= inv;
+    w[i] = (T)((double)w[i] * inv);
 }
 
-template <int M>
-__global__ __launch_bounds__(THREADS) void k_gemm_vy(const double *__restrict__ Vin, int64_t ldv,
+template <int M, typename Ti, typename To>
+__global__ __launch_bounds__(THREADS) void k_gemm_vy(const Ti *__restrict__ Vin, int64_t ldv,
                                                      int m, const double *__restrict__ Y,
-                                                     int ldy, int k, double *Vout, int64_t ldo,
+                                                     int ldy, int k, To *Vout, int64_t ldo,
                                                      int64_t n)
 {
   __shared__ double sY[M * M];
@@ -477,13 +594,13 @@ __global__ __launch_bounds__(THREADS) void k_gemm_vy(const double *__restrict__ 
        i += (int64_t)gridDim.x * THREADS) {
     double v[M];
 #pragma unroll
-    for (int r = 0; r < M; ++r) v[r] = r < m ? Vin[r * ldv + i] : 0.0;
+    for (int r = 0; r < M; ++r) v[r] = r < m ? (double)Vin[r * ldv + i] : 0.0;
     // all reads of row i happen before any write of row i (Vout may alias)
     for (int c = 0; c < k; ++c) {
       double s = 0;
 #pragma unroll
       for (int r = 0; r < M; ++r) s += v[r] * sY[r + c * M];
-      Vout[c * ldo + i] = s;
+      Vout[c * ldo + i] = (To)s;
     }
   }
 }
@@ -497,20 +614,22 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t z)
 }
 
 // uniform(0,1) from a counter-based hash of (seed, index)
-__global__ void k_random(double *v, int64_t n, uint64_t seed)
+template <typename T>
+__global__ void k_random(T *v, int64_t n, uint64_t seed)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t r = splitmix64(splitmix64(seed) ^ (uint64_t)i);
-    v[i] = ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    v[i] = (T)(((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0));
   }
 }
 
-__global__ void k_copy(const double *a, double *b, int64_t n)
+template <typename Ta, typename Tb>
+__global__ void k_copy(const Ta *a, Tb *b, int64_t n)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = a[i];
+    b[i] = (Tb)a[i];
 }
 
 __global__ void k_to_f32(const double *a, float *b, int64_t n)
@@ -556,7 +675,7 @@ __global__ void k_pow2_scale(const double *partial, int nblk, double *scale)
   }
 }
 
-template <int J>
+template <int J, typename T = double>
 __global__ __launch_bounds__(THREADS) void k_gram_row(const double *__restrict__ A,
                                                       const double *__restrict__ B, int64_t ld,
                                                       int k, int64_t n, int row,
@@ -575,6 +694,46 @@ __global__ __launch_bounds__(THREADS) void k_gram_row(const double *__restrict__
   block_reduce_write<J>(acc, k, partial + (int64_t)blockIdx.x * k);
 }
 
+// Rows [r0, r0 + RB) of the Gram matrix in one sweep: each vector of A's
+// row block and of B is read once.  Partial slot r * J + c.
+template <int RB, int J>
+__global__ __launch_bounds__(THREADS) void k_gram_block(const double *__restrict__ A,
+                                                        const double *__restrict__ B, int64_t ld,
+                                                        int k, int64_t n, int r0, int nr,
+                                                        double *__restrict__ partial)
+{
+  double acc[RB * J];
+#pragma unroll
+  for (int t = 0; t < RB * J; ++t) acc[t] = 0;
+  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * THREADS) {
+    double a[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) a[r] = r < nr ? A[(r0 + r) * ld + i] : 0.0;
+#pragma unroll
+    for (int c = 0; c < J; ++c)
+      if (c < k) {
+        const double b = B[c * ld + i];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) acc[r * J + c] += a[r] * b;
+      }
+  }
+  block_reduce_write<RB * J>(acc, RB * J, partial + (int64_t)blockIdx.x * (RB * J));
+}
+
+// out[r * k + c] = sum over blocks of slot r * J + c, for r < nr, c < k (one
+// wavefront per output, fixed order)
+__global__ __launch_bounds__(64) void k_reduce_gram(const double *__restrict__ partial, int nblk,
+                                                    int stride, int J, int k,
+                                                    double *__restrict__ out)
+{
+  const int r = blockIdx.x / k, c = blockIdx.x % k;
+  double s = 0;
+  for (int b = threadIdx.x; b < nblk; b += 64) s += partial[(int64_t)b * stride + r * J + c];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) out[r * k + c] = s;
+}
+
 unsigned nblocks(int64_t n)
 {
   int64_t b = (n + THREADS - 1) / THREADS;
@@ -586,76 +745,109 @@ unsigned nblocks(int64_t n)
 }  // namespace
 
 // Basis-size classes of the fused kernels: the register arrays are sized by
-// the template argument, so keep it close to the actual count.
-#define TM_JDISPATCH(KERNEL, jj, grid, ...)                                                  \
-  do {                                                                                       \
-    if ((jj) <= 4) hipLaunchKernelGGL(KERNEL<4>, grid, dim3(THREADS), 0, s, __VA_ARGS__);     \
-    else if ((jj) <= 8) hipLaunchKernelGGL(KERNEL<8>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
-    else if ((jj) <= 12) hipLaunchKernelGGL(KERNEL<12>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
-    else if ((jj) <= 16) hipLaunchKernelGGL(KERNEL<16>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
-    else if ((jj) <= 24) hipLaunchKernelGGL(KERNEL<24>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
-    else if ((jj) <= 32) hipLaunchKernelGGL(KERNEL<32>, grid, dim3(THREADS), 0, s, __VA_ARGS__); \
-    else hipLaunchKernelGGL(KERNEL<64>, grid, dim3(THREADS), 0, s, __VA_ARGS__);              \
-    TM_LAUNCH_CHECK();                                                                       \
+// the template argument, so keep it close to the actual count.  T is the
+// vector element type.
+#define TM_JDISPATCH(KERNEL, T, jj, grid, ...)                                                 \
+  do {                                                                                         \
+    if ((jj) <= 4) hipLaunchKernelGGL((KERNEL<4, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__);  \
+    else if ((jj) <= 8) hipLaunchKernelGGL((KERNEL<8, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__); \
+    else if ((jj) <= 12) hipLaunchKernelGGL((KERNEL<12, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__); \
+    else if ((jj) <= 16) hipLaunchKernelGGL((KERNEL<16, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__); \
+    else if ((jj) <= 24) hipLaunchKernelGGL((KERNEL<24, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__); \
+    else if ((jj) <= 32) hipLaunchKernelGGL((KERNEL<32, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__); \
+    else hipLaunchKernelGGL((KERNEL<64, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__);          \
+    TM_LAUNCH_CHECK();                                                                         \
   } while (0)
 
-bool pairs_ok(const void *a, const void *b, int64_t ld)
+namespace {
+// the packed kernels apply: 16-byte aligned vectors, ld a multiple of the
+// pack width
+template <typename T>
+bool packs_ok(const void *a, const void *b, int64_t ld)
 {
   return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0 &&
-         (ld & 1) == 0;
+         (ld % Pk<T>::P) == 0;
 }
+// grid of a packed (or scalar) sweep over n elements
+template <typename T>
+unsigned sweep_blocks(bool packed, int64_t n)
+{
+  return packed ? nblocks(n / Pk<T>::P) : nblocks(n);
+}
+}  // namespace
 
-void multidot(const double *V, int64_t ldv, int j, const double *w, int64_t n, double *partial,
+template <typename T>
+void multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *partial,
               double *out, hipStream_t s)
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
-  const bool v2 = pairs_ok(V, w, ldv);
-  const unsigned nb = v2 ? nblocks(n / 2) : nblocks(n);
-  if (v2) TM_JDISPATCH(k_multidot2, j, dim3(nb), V, ldv, j, w, n, partial);
-  else TM_JDISPATCH(k_multidot, j, dim3(nb), V, ldv, j, w, n, partial);
+  const bool pk = packs_ok<T>(V, w, ldv);
+  const unsigned nb = sweep_blocks<T>(pk, n);
+  if (pk) TM_JDISPATCH(k_multidot_pk, T, j, dim3(nb), V, ldv, j, w, n, partial);
+  else TM_JDISPATCH(k_multidot, T, j, dim3(nb), V, ldv, j, w, n, partial);
   hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
   TM_LAUNCH_CHECK();
 }
 
-void update_dots(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h,
+template <typename T>
+void update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
                  double *partial, double *out, hipStream_t s)
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
-  const bool v2 = pairs_ok(V, w, ldv);
-  const unsigned nb = v2 ? nblocks(n / 2) : nblocks(n);
-  if (v2)
-    TM_JDISPATCH(k_update_dots2, j, dim3(nb), V, ldv, j, w, n, h, partial, 1, nullptr, nullptr);
+  const bool pk = packs_ok<T>(V, w, ldv);
+  const unsigned nb = sweep_blocks<T>(pk, n);
+  if (pk)
+    TM_JDISPATCH(k_update_dots_pk, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 1, nullptr,
+                 nullptr);
   else
-    TM_JDISPATCH(k_update_dots, j, dim3(nb), V, ldv, j, w, n, h, partial, 1);
+    TM_JDISPATCH(k_update_dots, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 1);
   hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
   TM_LAUNCH_CHECK();
 }
 
-void update_norm(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h,
+template <typename T>
+void update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
+                         double *partial, double *out, hipStream_t s)
+{
+  TM_CHECK(j >= 1 && j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
+  const bool pk = packs_ok<T>(V, w, ldv);
+  const unsigned nb = sweep_blocks<T>(pk, n);
+  if (pk)
+    TM_JDISPATCH(k_lanczos_dots_pk, T, j, dim3(nb), V, ldv, j, w, n, h, partial);
+  else
+    TM_JDISPATCH(k_update_dots, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 1);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
+  TM_LAUNCH_CHECK();
+}
+
+template <typename T>
+void update_norm(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
                  double *partial, double *out, hipStream_t s)
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
-  const bool v2 = pairs_ok(V, w, ldv);
-  const unsigned nb = v2 ? nblocks(n / 2) : nblocks(n);
-  if (v2)
-    TM_JDISPATCH(k_update_dots2, j, dim3(nb), V, ldv, j, w, n, h, partial, 0, nullptr, nullptr);
+  const bool pk = packs_ok<T>(V, w, ldv);
+  const unsigned nb = sweep_blocks<T>(pk, n);
+  if (pk)
+    TM_JDISPATCH(k_update_dots_pk, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 0, nullptr,
+                 nullptr);
   else
-    TM_JDISPATCH(k_update_dots, j, dim3(nb), V, ldv, j, w, n, h, partial, 0);
+    TM_JDISPATCH(k_update_dots, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 0);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, (int)nb, 1, out);
   TM_LAUNCH_CHECK();
 }
 
-void update_norm_cond(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h,
+template <typename T>
+void update_norm_cond(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
                       double *partial, double *out, const double *w0, const double *w1,
                       hipStream_t s)
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
-  const bool v2 = pairs_ok(V, w, ldv);
-  const unsigned nb = v2 ? nblocks(n / 2) : nblocks(n);
-  if (v2)
-    TM_JDISPATCH(k_update_dots2, j, dim3(nb), V, ldv, j, w, n, h, partial, 0, w0, w1);
+  const bool pk = packs_ok<T>(V, w, ldv);
+  const unsigned nb = sweep_blocks<T>(pk, n);
+  if (pk)
+    TM_JDISPATCH(k_update_dots_pk, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 0, w0, w1);
   else
-    TM_JDISPATCH(k_update_norm_cond, j, dim3(nb), V, ldv, j, w, n, h, partial, w0, w1);
+    TM_JDISPATCH(k_update_norm_cond, T, j, dim3(nb), V, ldv, j, w, n, h, partial, w0, w1);
   hipLaunchKernelGGL(k_reduce_partials_cond, dim3(1), dim3(64), 0, s, partial, (int)nb, out, w0,
                      w1);
   TM_LAUNCH_CHECK();
@@ -668,17 +860,19 @@ void step_close(int j, const double *hb1, const double *hb2, double *Hd, int ldh
   TM_LAUNCH_CHECK();
 }
 
-void update_normalize(const double *V, int64_t ldv, int j, double *w, int64_t n,
-                      const double *h2, const double *w0, const double *w1, const double *nrm2,
-                      int force, hipStream_t s)
+template <typename T>
+void update_normalize(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h2,
+                      const double *w0, const double *w1, const double *nrm2, int force,
+                      hipStream_t s)
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
-  if (pairs_ok(V, w, ldv))
-    TM_JDISPATCH(k_update_normalize2, j, dim3(nblocks(n / 2)), V, ldv, j, w, n, h2, w0, w1, nrm2,
+  const bool pk = packs_ok<T>(V, w, ldv);
+  const unsigned nb = sweep_blocks<T>(pk, n);
+  if (pk)
+    TM_JDISPATCH(k_update_normalize_pk, T, j, dim3(nb), V, ldv, j, w, n, h2, w0, w1, nrm2,
                  force);
   else
-    TM_JDISPATCH(k_update_normalize, j, dim3(nblocks(n)), V, ldv, j, w, n, h2, w0, w1, nrm2,
-                 force);
+    TM_JDISPATCH(k_update_normalize, T, j, dim3(nb), V, ldv, j, w, n, h2, w0, w1, nrm2, force);
 }
 
 void lanczos_coeffs(int j, const double *hA, const double *Hd, int ldh, double *hL, hipStream_t s)
@@ -703,52 +897,55 @@ void step_finish(int j, const double *hb1, const double *hb2, const double *hb3,
   TM_LAUNCH_CHECK();
 }
 
-void normalize(double *w, int64_t n, const double *nrm2, hipStream_t s)
+template <typename T>
+void normalize(T *w, int64_t n, const double *nrm2, hipStream_t s)
 {
   if ((reinterpret_cast<uintptr_t>(w) & 15) == 0)
-    hipLaunchKernelGGL(k_normalize2, dim3(nblocks(n / 2) * 4), dim3(THREADS), 0, s, w, n, nrm2);
+    hipLaunchKernelGGL(k_normalize_pk<T>, dim3(nblocks(n / Pk<T>::P) * 4), dim3(THREADS), 0, s,
+                       w, n, nrm2);
   else
-    hipLaunchKernelGGL(k_normalize, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, w, n, nrm2);
+    hipLaunchKernelGGL(k_normalize<T>, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, w, n, nrm2);
   TM_LAUNCH_CHECK();
 }
 
-void gemm_vy(const double *Vin, int64_t ldv, int m, const double *Y, int ldy, int k,
-             double *Vout, int64_t ldo, int64_t n, hipStream_t s)
+template <typename Ti, typename To>
+void gemm_vy(const Ti *Vin, int64_t ldv, int m, const double *Y, int ldy, int k, To *Vout,
+             int64_t ldo, int64_t n, hipStream_t s)
 {
   TM_CHECK(m <= MAXJ && k <= m, TM_EINVAL, "restart block too large");
+  TM_CHECK(sizeof(Ti) == sizeof(To) || (const void *)Vin != (const void *)Vout, TM_EINVAL,
+           "in-place basis rotation needs one element type");
   const unsigned nb = nblocks(n) * 4;
   if (m <= 8)
-    hipLaunchKernelGGL(k_gemm_vy<8>, dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y, ldy, k, Vout,
-                       ldo, n);
+    hipLaunchKernelGGL((k_gemm_vy<8, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y, ldy,
+                       k, Vout, ldo, n);
   else if (m <= 16)
-    hipLaunchKernelGGL(k_gemm_vy<16>, dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y, ldy, k,
-                       Vout, ldo, n);
+    hipLaunchKernelGGL((k_gemm_vy<16, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y,
+                       ldy, k, Vout, ldo, n);
   else if (m <= 32)
-    hipLaunchKernelGGL(k_gemm_vy<32>, dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y, ldy, k,
-                       Vout, ldo, n);
+    hipLaunchKernelGGL((k_gemm_vy<32, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y,
+                       ldy, k, Vout, ldo, n);
   else
-    hipLaunchKernelGGL(k_gemm_vy<64>, dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y, ldy, k,
-                       Vout, ldo, n);
+    hipLaunchKernelGGL((k_gemm_vy<64, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y,
+                       ldy, k, Vout, ldo, n);
   TM_LAUNCH_CHECK();
 }
 
-void random_uniform(double *v, int64_t n, uint64_t seed, hipStream_t s)
+template <typename T>
+void random_uniform(T *v, int64_t n, uint64_t seed, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_random, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, v, n, seed);
+  hipLaunchKernelGGL(k_random<T>, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, v, n, seed);
   TM_LAUNCH_CHECK();
 }
 
-void copy(const double *a, double *b, int64_t n, hipStream_t s)
+template <typename Ta, typename Tb>
+void copy(const Ta *a, Tb *b, int64_t n, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_copy, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, a, b, n);
+  hipLaunchKernelGGL((k_copy<Ta, Tb>), dim3(nblocks(n) * 4), dim3(THREADS), 0, s, a, b, n);
   TM_LAUNCH_CHECK();
 }
 
-void to_f32(const double *a, float *b, int64_t n, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n) * 4), dim3(THREADS), 0, s, a, b, n);
-  TM_LAUNCH_CHECK();
-}
+void to_f32(const double *a, float *b, int64_t n, hipStream_t s) { copy(a, b, n, s); }
 
 void fill(double *a, double val, int64_t n, hipStream_t s)
 {
@@ -769,14 +966,65 @@ void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double
 {
   TM_CHECK(k <= MAXJ, TM_EINVAL, "gram block too large");
   const unsigned nb = nblocks(n);
-  for (int row = 0; row < k; ++row) {
-    TM_JDISPATCH(k_gram_row, k, dim3(nb), A, B, ld, k, n, row, partial);
-    // G[row + c*k] = <A_row, B_c>
-    hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(64), 0, s, partial, (int)nb, k,
-                       G + row * k);
-    TM_LAUNCH_CHECK();
+  // row blocks of 4 (k <= 8) or 2 (k <= 16): each sweep reads RB + k vectors
+  auto blocked = [&](auto rb_tag, auto j_tag) {
+    constexpr int RB = decltype(rb_tag)::value, J = decltype(j_tag)::value;
+    for (int r0 = 0; r0 < k; r0 += RB) {
+      const int nr = std::min(RB, k - r0);
+      hipLaunchKernelGGL((k_gram_block<RB, J>), dim3(nb), dim3(THREADS), 0, s, A, B, ld, k, n, r0,
+                         nr, partial);
+      hipLaunchKernelGGL(k_reduce_gram, dim3(nr * k), dim3(64), 0, s, partial, (int)nb, RB * J, J,
+                         k, G + r0 * k);
+      TM_LAUNCH_CHECK();
+    }
+  };
+  if (k <= 4) {
+    blocked(std::integral_constant<int, 4>(), std::integral_constant<int, 4>());
+  } else if (k <= 8) {
+    blocked(std::integral_constant<int, 4>(), std::integral_constant<int, 8>());
+  } else if (k <= 16) {
+    blocked(std::integral_constant<int, 2>(), std::integral_constant<int, 16>());
+  } else {
+    for (int row = 0; row < k; ++row) {
+      TM_JDISPATCH(k_gram_row, double, k, dim3(nb), A, B, ld, k, n, row, partial);
+      // G[row + c*k] = <A_row, B_c>
+      hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(64), 0, s, partial, (int)nb, k,
+                         G + row * k);
+      TM_LAUNCH_CHECK();
+    }
   }
 }
+
+#define TM_KRY_INST(T)                                                                          \
+  template void multidot<T>(const T *, int64_t, int, const T *, int64_t, double *, double *,   \
+                            hipStream_t);                                                      \
+  template void update_dots<T>(const T *, int64_t, int, T *, int64_t, const double *, double *, \
+                               double *, hipStream_t);                                         \
+  template void update_dots_lanczos<T>(const T *, int64_t, int, T *, int64_t, const double *,   \
+                                       double *, double *, hipStream_t);                       \
+  template void update_norm<T>(const T *, int64_t, int, T *, int64_t, const double *, double *, \
+                               double *, hipStream_t);                                         \
+  template void update_norm_cond<T>(const T *, int64_t, int, T *, int64_t, const double *,      \
+                                    double *, double *, const double *, const double *,        \
+                                    hipStream_t);                                              \
+  template void update_normalize<T>(const T *, int64_t, int, T *, int64_t, const double *,      \
+                                    const double *, const double *, const double *, int,       \
+                                    hipStream_t);                                              \
+  template void normalize<T>(T *, int64_t, const double *, hipStream_t);                        \
+  template void random_uniform<T>(T *, int64_t, uint64_t, hipStream_t);
+TM_KRY_INST(double)
+TM_KRY_INST(float)
+#undef TM_KRY_INST
+template void gemm_vy<double, double>(const double *, int64_t, int, const double *, int, int,
+                                      double *, int64_t, int64_t, hipStream_t);
+template void gemm_vy<float, float>(const float *, int64_t, int, const double *, int, int,
+                                    float *, int64_t, int64_t, hipStream_t);
+template void gemm_vy<float, double>(const float *, int64_t, int, const double *, int, int,
+                                     double *, int64_t, int64_t, hipStream_t);
+template void copy<double, double>(const double *, double *, int64_t, hipStream_t);
+template void copy<float, float>(const float *, float *, int64_t, hipStream_t);
+template void copy<double, float>(const double *, float *, int64_t, hipStream_t);
+template void copy<float, double>(const float *, double *, int64_t, hipStream_t);
 
 }  // namespace kry
 }  // namespace ltm

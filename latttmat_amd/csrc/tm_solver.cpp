@@ -9,16 +9,17 @@
 // Ritz vectors spans the same restarted Krylov space as an implicit restart
 // with the unwanted Ritz values as shifts.
 //
-// Device: the basis V (FP64, ncv+1 vectors), T.x, CGS2 orthogonalisation,
-// the restart V <- V Y.  Host: the (ncv x ncv) projected matrix, its
+// Device: the basis V (ncv+1 vectors), T.x, CGS2 orthogonalisation, the
+// restart V <- V Y.  Host: the (ncv x ncv) projected matrix, its
 // eigen-decomposition, convergence tests.  One small device->host copy per
-// Krylov step.
+// restart cycle.
 //
 // Mixed precision (TM_PREC_MIXED): phase 1 runs the same loop with the
-// operator computing in FP32 (FP64 basis and orthogonalisation) to a
-// residual of mixed_tol; phase 2 refines the kept Ritz block with FP64
+// operator computing in FP32 and, by default, the basis stored in FP32
+// (products and sums still accumulated in FP64) to a residual of mixed_tol
+// or the FP32 noise floor; phase 2 refines the kept Ritz block with FP64
 // matvecs (Rayleigh-Ritz, repeated with one subspace-iteration step) until
-// the FP64 residuals or the eigenvalue change meet the target.
+// the FP64 residuals or the eigenvalue error bound meet the target.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -40,7 +41,29 @@ struct Ritz {
   std::vector<double> res;        // Krylov residual estimates
 };
 
-class KrylovSchur {
+// What the driver needs of a Krylov-Schur run, whatever the basis storage.
+class KrylovRun {
+ public:
+  virtual ~KrylovRun() = default;
+  virtual void set_graph(bool on) = 0;
+  virtual void set_stall_stop(bool v) = 0;
+  // run to convergence from a random start (v0 null) or from the FP64
+  // device vector v0; returns the number of converged wanted pairs
+  virtual int run(Ritz &out, const double *v0) = 0;
+  // FP64 Ritz vectors (real parts; imaginary parts into Xi if non-null)
+  virtual void ritz_vectors(const Ritz &r, int cnt, double *Xr, double *Xi) = 0;
+  virtual int64_t graph_cycles() const = 0;
+  virtual int64_t ldv() const = 0;
+  virtual int64_t matvecs() const = 0;
+  virtual int64_t restarts() const = 0;
+  virtual bool stalled() const = 0;
+  virtual int m() const = 0;
+};
+
+// T = element type of the stored basis (double, or float for the FP32
+// phase of a mixed solve; the operator then reads and writes FP32 vectors).
+template <typename T>
+class KrylovSchur final : public KrylovRun {
  public:
   KrylovSchur(tm_op *op, int nev, int m, double tol, int maxit, tm_dtype comp, int transpose,
               uint64_t seed, int verbose)
@@ -50,7 +73,7 @@ class KrylovSchur {
     n_ = op->ncol;
     ldv_ = ((n_ + 63) / 64) * 64;
     sym_ = op->symmetric;
-    V_.alloc(sizeof(double) * ldv_ * (m_ + 1));
+    V_.alloc(sizeof(T) * ldv_ * (m_ + 1));
     partial_.alloc(sizeof(double) * kry::NBLK * 72);
     hbuf_.alloc(sizeof(double) * 3 * 72);
     Ydev_.alloc(sizeof(double) * 72 * 72);
@@ -65,11 +88,11 @@ class KrylovSchur {
   }
   KrylovSchur(const KrylovSchur &) = delete;
   KrylovSchur &operator=(const KrylovSchur &) = delete;
-  void set_graph(bool on) { use_graph_ = on; }
-  int64_t graph_cycles() const { return graph_cycles_; }
+  void set_graph(bool on) override { use_graph_ = on; }
+  int64_t graph_cycles() const override { return graph_cycles_; }
 
-  double *V(int k) const { return V_.as<double>() + (size_t)k * ldv_; }
-  int64_t ldv() const { return ldv_; }
+  T *V(int k) const { return V_.as<T>() + (size_t)k * ldv_; }
+  int64_t ldv() const override { return ldv_; }
   int64_t n() const { return n_; }
   double &H(int i, int j) { return H_[i + (size_t)j * (m_ + 1)]; }
 
@@ -78,17 +101,17 @@ class KrylovSchur {
   {
     hipStream_t s = op_->stream;
     kry::random_uniform(V(0), n_, seed_, s);
-    kry::multidot(V_.as<double>(), ldv_, 0, V(0), n_, partial_.as<double>(), hbuf_.as<double>(), s);
+    kry::multidot(V_.as<T>(), ldv_, 0, V(0), n_, partial_.as<double>(), hbuf_.as<double>(), s);
     kry::normalize(V(0), n_, hbuf_.as<double>(), s);
     k_ = 0;
   }
 
-  // Start from a given device vector (normalised here).
+  // Start from a given FP64 device vector (normalised here).
   void start_from(const double *v0)
   {
     hipStream_t s = op_->stream;
     kry::copy(v0, V(0), n_, s);
-    kry::multidot(V_.as<double>(), ldv_, 0, V(0), n_, partial_.as<double>(), hbuf_.as<double>(), s);
+    kry::multidot(V_.as<T>(), ldv_, 0, V(0), n_, partial_.as<double>(), hbuf_.as<double>(), s);
     kry::normalize(V(0), n_, hbuf_.as<double>(), s);
     k_ = 0;
   }
@@ -160,8 +183,8 @@ class KrylovSchur {
     kry::fill(flags, 0.0, 1, s);
     kry::fill(flags + 1, -1.0, 1, s);
     for (int j = k0; j < m_; ++j) {
-      double *w = V(j + 1);
-      op_->apply_internal(V(j), w, ldv_, 1, TM_F64, comp_, tr_, nullptr, s);
+      T *w = V(j + 1);
+      op_->apply_internal(V(j), w, ldv_, 1, io_, comp_, tr_, nullptr, s);
       const int nk = j + 1;
       if (sym_ && j > k0) {
         // Lanczos step: subtract alpha v_j + beta_{j-1} v_{j-1} inside the
@@ -169,18 +192,18 @@ class KrylovSchur {
         // normalises (two sweeps over V instead of three)
         kry::multidot(V(j), ldv_, 1, w, n_, partial_.as<double>(), hb, s);
         kry::lanczos_coeffs(j, hb, Hd, ldh, hb + 144, s);
-        kry::update_dots(V_.as<double>(), ldv_, nk, w, n_, hb + 144, partial_.as<double>(),
-                         hb + 72, s);
+        kry::update_dots_lanczos(V_.as<T>(), ldv_, nk, w, n_, hb + 144, partial_.as<double>(),
+                                 hb + 72, s);
         kry::step_close_lanczos(j, hb + 144, hb + 72, Hd, ldh, nrm, flags, s);
-        kry::update_normalize(V_.as<double>(), ldv_, nk, w, n_, hb + 72, hb, hb, nrm, 1, s);
+        kry::update_normalize(V_.as<T>(), ldv_, nk, w, n_, hb + 72, hb, hb, nrm, 1, s);
         continue;
       }
-      kry::multidot(V_.as<double>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
-      kry::update_dots(V_.as<double>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 72, s);
+      kry::multidot(V_.as<T>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
+      kry::update_dots(V_.as<T>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 72, s);
       // DGKS correction and normalisation in one sweep, the norm from the
       // projections (an unsafe correction sends the step to the host loop)
       kry::step_close(j, hb, hb + 72, Hd, ldh, nrm, flags, s);
-      kry::update_normalize(V_.as<double>(), ldv_, nk, w, n_, hb + 72, hb + nk, hb + 72 + nk, nrm,
+      kry::update_normalize(V_.as<T>(), ldv_, nk, w, n_, hb + 72, hb + nk, hb + 72 + nk, nrm,
                             0, s);
     }
   }
@@ -221,14 +244,14 @@ class KrylovSchur {
     double *hb = hbuf_.as<double>();
     std::vector<double> hh(3 * 72);
     for (int j = j0; j < m_; ++j) {
-      double *w = V(j + 1);
-      op_->apply_internal(V(j), w, ldv_, 1, TM_F64, comp_, tr_, nullptr, s);
+      T *w = V(j + 1);
+      op_->apply_internal(V(j), w, ldv_, 1, io_, comp_, tr_, nullptr, s);
       ++matvecs_;
       const int nk = j + 1;
       // classical Gram-Schmidt with one DGKS correction (only when the first
       // projection removed more than half of the norm^2)
-      kry::multidot(V_.as<double>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
-      kry::update_dots(V_.as<double>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 72, s);
+      kry::multidot(V_.as<T>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
+      kry::update_dots(V_.as<T>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 72, s);
       TM_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double) * 2 * 72, hipMemcpyDeviceToHost, s));
       TM_HIP(hipStreamSynchronize(s));
       const double w0 = hh[nk], w1 = hh[72 + nk];
@@ -237,7 +260,7 @@ class KrylovSchur {
       const bool reorth = !(w1 > 0.5 * w0);
       for (int kk = 0; kk < nk; ++kk) H(kk, j) = hh[kk];
       if (reorth) {
-        kry::update_norm(V_.as<double>(), ldv_, nk, w, n_, hb + 72, partial_.as<double>(), hb + 144, s);
+        kry::update_norm(V_.as<T>(), ldv_, nk, w, n_, hb + 72, partial_.as<double>(), hb + 144, s);
         TM_HIP(hipMemcpyAsync(hh.data() + 144, hb + 144, sizeof(double), hipMemcpyDeviceToHost, s));
         TM_HIP(hipStreamSynchronize(s));
         for (int kk = 0; kk < nk; ++kk) H(kk, j) += hh[72 + kk];
@@ -253,8 +276,8 @@ class KrylovSchur {
         // orthogonal to the basis; the coupling is zero.
         kry::random_uniform(w, n_, seed_ + 7919 * (uint64_t)(j + 1 + 31 * restarts_), s);
         for (int pass = 0; pass < 2; ++pass) {
-          kry::multidot(V_.as<double>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
-          kry::update_norm(V_.as<double>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 144, s);
+          kry::multidot(V_.as<T>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
+          kry::update_norm(V_.as<T>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 144, s);
         }
         kry::normalize(w, n_, hb + 144, s);
         beta = 0.0;
@@ -450,7 +473,7 @@ class KrylovSchur {
   }
 
   // Run to convergence; returns the number of converged wanted pairs.
-  int run(Ritz &out, const double *v0 = nullptr)
+  int run(Ritz &out, const double *v0) override
   {
     if (v0) start_from(v0); else start();
     int nc = 0;
@@ -483,7 +506,7 @@ class KrylovSchur {
 
   // Ritz vectors X[:, c] = V(0..m-1) * Z[:, c] for c < cnt (real part, and
   // imaginary part into Xi if non-null) into device buffers of ld ldv().
-  void ritz_vectors(const Ritz &r, int cnt, double *Xr, double *Xi)
+  void ritz_vectors(const Ritz &r, int cnt, double *Xr, double *Xi) override
   {
     const int m = m_;
     hipStream_t s = op_->stream;
@@ -502,13 +525,13 @@ class KrylovSchur {
     }
   }
 
-  int64_t matvecs() const { return matvecs_; }
-  int64_t restarts() const { return restarts_; }
-  void set_stall_stop(bool v) { stall_stop_ = v; }
+  int64_t matvecs() const override { return matvecs_; }
+  int64_t restarts() const override { return restarts_; }
+  void set_stall_stop(bool v) override { stall_stop_ = v; }
   int64_t reorths() const { return reorths_; }
-  bool stalled() const { return stalled_; }
+  bool stalled() const override { return stalled_; }
   bool sym() const { return sym_; }
-  int m() const { return m_; }
+  int m() const override { return m_; }
 
  private:
   tm_op *op_;
@@ -516,6 +539,8 @@ class KrylovSchur {
   double tol_;
   int maxit_;
   tm_dtype comp_;
+  // vectors handed to the operator are the stored basis columns
+  const tm_dtype io_ = sizeof(T) == 4 ? TM_F32 : TM_F64;
   int tr_;
   uint64_t seed_;
   int verbose_;
@@ -565,8 +590,9 @@ struct RefineOut {
   int64_t matvecs = 0;
 };
 
-// FP64 Rayleigh-Ritz refinement of an orthonormal block Y (k vectors at
-// B[0..k), leading dimension ld) for a symmetric operator.  B must hold 3k
+// FP64 Rayleigh-Ritz refinement of a block Y (k vectors at B[0..k), leading
+// dimension ld; orthonormalised first unless input_orthonormal) for a
+// symmetric operator.  B must hold 3k
 // vectors: [Y | W = A Y | X = Y Z].  Each iteration: W = A Y, Ritz pairs of
 // Y^T W, residuals R = W Z - X diag(theta) (into the W slots); accepted when
 // the FP64 residual meets tol or the Rayleigh-Ritz error bound
@@ -575,7 +601,8 @@ struct RefineOut {
 // (one subspace-iteration step) and repeat.  On return B[0..nev) hold the
 // refined, normalised Ritz vectors.
 RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev, double tol,
-                     int maxiter, int verbose, const std::vector<double> &outside)
+                     int maxiter, int verbose, const std::vector<double> &outside,
+                     bool input_orthonormal)
 {
   hipStream_t s = op->stream;
   RefineOut ro;
@@ -591,6 +618,37 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
     TM_HIP(hipMemcpyAsync(hG.data(), G.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
     TM_HIP(hipStreamSynchronize(s));
   };
+  // B[0..k) <- B[0..k) L^-T with G = L L^T its Gram matrix (Cholesky QR:
+  // orthonormal columns), or (full = false, or G not positive definite) each
+  // column normalised
+  auto orth0 = [&](bool full) {
+    gram_host(0, 0);
+    std::vector<double> Gm((size_t)k * k);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) Gm[i + (size_t)j * k] = hG[i * k + j];
+    std::vector<double> Rinv((size_t)k * k, 0.0);
+    if (full && cholesky(k, Gm)) {
+      for (int c = 0; c < k; ++c) {
+        std::vector<double> e(k, 0.0);
+        e[c] = 1.0;
+        for (int i = k - 1; i >= 0; --i) {
+          double s2 = e[i];
+          for (int t = i + 1; t < k; ++t) s2 -= Gm[t + (size_t)i * k] * e[t];
+          e[i] = s2 / Gm[i + (size_t)i * k];
+        }
+        for (int i = 0; i < k; ++i) Rinv[i + (size_t)c * k] = e[i];
+      }
+    } else {
+      for (int i = 0; i < k; ++i)
+        Rinv[i + (size_t)i * k] = 1.0 / std::sqrt(std::max(hG[i * k + i], 1e-300));
+    }
+    upload(Rinv);
+    kry::gemm_vy(col(0), ld, k, C.as<double>(), k, k, col(0), ld, n, s);
+    TM_HIP(hipStreamSynchronize(s));
+  };
+  // Ritz vectors of an FP32-stored basis are orthonormal only to FP32
+  // rounding; the Rayleigh-Ritz step below needs them orthonormal in FP64
+  if (!input_orthonormal) orth0(true);
   for (int it = 0; it < maxiter; ++it) {
     for (int c = 0; c < k; ++c) {
       op->apply_internal(col(c), col(k + c), ld, 1, TM_F64, TM_F64, 0, nullptr, s);
@@ -655,30 +713,7 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
     }
     upload(Cn);
     kry::gemm_vy(col(k), ld, 2 * k, C.as<double>(), 2 * k, k, col(0), ld, n, s);
-    gram_host(0, 0);
-    std::vector<double> Gm((size_t)k * k);
-    for (int i = 0; i < k; ++i)
-      for (int j = 0; j < k; ++j) Gm[i + (size_t)j * k] = hG[i * k + j];
-    std::vector<double> Rinv((size_t)k * k, 0.0);
-    if (!done && cholesky(k, Gm)) {
-      // Y = X' L^-T (orthonormal)
-      for (int c = 0; c < k; ++c) {
-        std::vector<double> e(k, 0.0);
-        e[c] = 1.0;
-        for (int i = k - 1; i >= 0; --i) {
-          double s2 = e[i];
-          for (int t = i + 1; t < k; ++t) s2 -= Gm[t + (size_t)i * k] * e[t];
-          e[i] = s2 / Gm[i + (size_t)i * k];
-        }
-        for (int i = 0; i < k; ++i) Rinv[i + (size_t)c * k] = e[i];
-      }
-    } else {
-      for (int i = 0; i < k; ++i)
-        Rinv[i + (size_t)i * k] = 1.0 / std::sqrt(std::max(hG[i * k + i], 1e-300));
-    }
-    upload(Rinv);
-    kry::gemm_vy(col(0), ld, k, C.as<double>(), k, k, col(0), ld, n, s);
-    TM_HIP(hipStreamSynchronize(s));
+    orth0(!done);
     if (done) break;
   }
   return ro;
@@ -706,14 +741,18 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   int64_t mv32 = 0, mv64 = 0, restarts = 0, gcycles = 0;
   int trial = 0, nc = 0;
   Ritz rz;
-  std::unique_ptr<KrylovSchur> ks;
+  std::unique_ptr<KrylovRun> ks;
   // pfunc.hpp:87-109: up to maxtrial attempts, ncv += 2 on failure
   for (;;) {
-    ks.reset(new KrylovSchur(op, nev, ncv, mixed ? std::max(o.mixed_tol, o.tol) : o.tol, o.maxit,
-                             mixed ? TM_F32 : TM_F64, 0, o.seed, o.verbose));
+    const double tol1 = mixed ? std::max(o.mixed_tol, o.tol) : o.tol;
+    if (mixed && o.basis_f32)
+      ks.reset(new KrylovSchur<float>(op, nev, ncv, tol1, o.maxit, TM_F32, 0, o.seed, o.verbose));
+    else
+      ks.reset(new KrylovSchur<double>(op, nev, ncv, tol1, o.maxit, mixed ? TM_F32 : TM_F64, 0,
+                                       o.seed, o.verbose));
     if (mixed) ks->set_stall_stop(true);
     ks->set_graph(o.use_graph != 0);
-    nc = ks->run(rz);
+    nc = ks->run(rz, nullptr);
     if (mixed && ks->stalled()) nc = nev;  // FP32 phase ends at its noise floor
     restarts += ks->restarts();
     gcycles += ks->graph_cycles();
@@ -742,7 +781,8 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
     ks->ritz_vectors(rz, k, B.as<double>(), nullptr);
     std::vector<double> outside;
     for (int d = k; d < ks->m(); ++d) outside.push_back(rz.wr[d]);
-    RefineOut ro = refine_sym(op, B.as<double>(), ldv, n, k, nev, o.tol, 4, o.verbose, outside);
+    RefineOut ro = refine_sym(op, B.as<double>(), ldv, n, k, nev, o.tol, 4, o.verbose, outside,
+                              !o.basis_f32);
     mv64 += ro.matvecs;
     for (int i = 0; i < nev; ++i) {
       evr[i] = ro.theta[i];
@@ -754,8 +794,8 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
     Xr.alloc(sizeof(double) * ldv * nev);
     copy_sync(Xr.get(), B.get(), sizeof(double) * ldv * nev, hipMemcpyDeviceToDevice, op->stream);
   } else {
-    std::unique_ptr<KrylovSchur> k2;
-    KrylovSchur *fin = ks.get();
+    std::unique_ptr<KrylovRun> k2;
+    KrylovRun *fin = ks.get();
     if (mixed) {
       // FP64 Krylov-Schur warm-started from the sum of the wanted FP32 Ritz
       // vectors (real parts): the FP64 run converges to tol in a cycle or two
@@ -766,7 +806,7 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
       copy_sync(Y1.get(), ones.data(), sizeof(double) * nev, hipMemcpyHostToDevice, op->stream);
       kry::gemm_vy(X0.as<double>(), ldv, nev, Y1.as<double>(), nev, 1, v0.as<double>(), ldv, n,
                    op->stream);
-      k2.reset(new KrylovSchur(op, nev, ncv, o.tol, o.maxit, TM_F64, 0, o.seed, o.verbose));
+      k2.reset(new KrylovSchur<double>(op, nev, ncv, o.tol, o.maxit, TM_F64, 0, o.seed, o.verbose));
       k2->set_graph(o.use_graph != 0);
       nc = k2->run(rz, v0.as<double>());
       mv64 += k2->matvecs();
@@ -823,10 +863,10 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   if (o.want_left && !op->symmetric) {
     int ncvl = 7;
     if (ncvl >= n) ncvl = (int)n - 1;
-    KrylovSchur kl(op, 1, ncvl, o.tol, o.maxit, TM_F64, 1, o.seed + 1, o.verbose);
+    KrylovSchur<double> kl(op, 1, ncvl, o.tol, o.maxit, TM_F64, 1, o.seed + 1, o.verbose);
     kl.set_graph(o.use_graph != 0);
     Ritz rl;
-    const int ncl = kl.run(rl);
+    const int ncl = kl.run(rl, nullptr);
     mv64 += kl.matvecs();
     gcycles += kl.graph_cycles();
     if (ncl >= 1) {

@@ -12,7 +12,9 @@ int main(int argc, char **argv)
 {
   FILE *f = std::fopen(argv[1], "r");
   int n, sym, nev, ncv, prec, left;
-  if (std::fscanf(f, "%d %d %d %d %d %d", &n, &sym, &nev, &ncv, &prec, &left) != 6) return 2;
+  int basis32 = 0;
+  if (std::fscanf(f, "%d %d %d %d %d %d %d", &n, &sym, &nev, &ncv, &prec, &left, &basis32) != 7)
+    return 2;
   tm_op op;
   op.ncol = n;
   op.symmetric = sym;
@@ -23,7 +25,7 @@ int main(int argc, char **argv)
   std::memset(&o, 0, sizeof o);
   o.nev = nev; o.ncv = ncv; o.tol = 1e-10; o.maxit = 1000; o.maxtrial = 3;
   o.precision = (tm_precision)prec; o.seed = 1; o.want_left = left; o.want_vectors = 1;
-  o.mixed_tol = 1e-6; o.verbose = argc > 2 ? 3 : 0; o.refine = 0;
+  o.mixed_tol = 1e-6; o.verbose = argc > 2 ? 3 : 0; o.refine = 0; o.basis_f32 = basis32;
   std::vector<double> er(nev), ei(nev), rs(nev), vecs((size_t)2 * n * nev), vl(n);
   tm_solve_out out;
   std::memset(&out, 0, sizeof out);

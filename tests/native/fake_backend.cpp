@@ -46,30 +46,46 @@ void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind, hipStrea
   std::memmove(dst, src, bytes);
 }
 namespace kry {
-void multidot(const double *V, int64_t ldv, int j, const double *w, int64_t n, double *, double *out, hipStream_t)
+template <typename T>
+void multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *, double *out, hipStream_t)
 {
   for (int k = 0; k <= j; ++k) out[k] = 0;
   for (int64_t i = 0; i < n; ++i) {
-    for (int k = 0; k < j; ++k) out[k] += V[k * ldv + i] * w[i];
-    out[j] += w[i] * w[i];
+    for (int k = 0; k < j; ++k) out[k] += (double)V[k * ldv + i] * (double)w[i];
+    out[j] += (double)w[i] * (double)w[i];
   }
 }
-void update_dots(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h, double *, double *out, hipStream_t)
+template <typename T>
+void update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h, double *, double *out, hipStream_t)
 {
   std::vector<double> hh(h, h + j);
-  for (int64_t i = 0; i < n; ++i)
-    for (int k = 0; k < j; ++k) w[i] -= hh[k] * V[k * ldv + i];
+  for (int64_t i = 0; i < n; ++i) {
+    double wi = w[i];
+    for (int k = 0; k < j; ++k) wi -= hh[k] * (double)V[k * ldv + i];
+    w[i] = (T)wi;
+  }
   multidot(V, ldv, j, w, n, nullptr, out, nullptr);
 }
-void update_norm(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h, double *, double *out, hipStream_t)
+template <typename T>
+void update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h, double *p,
+                         double *out, hipStream_t st)
+{
+  update_dots(V, ldv, j, w, n, h, p, out, st);
+}
+template <typename T>
+void update_norm(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h, double *, double *out, hipStream_t)
 {
   std::vector<double> hh(h, h + j);
-  for (int64_t i = 0; i < n; ++i)
-    for (int k = 0; k < j; ++k) w[i] -= hh[k] * V[k * ldv + i];
+  for (int64_t i = 0; i < n; ++i) {
+    double wi = w[i];
+    for (int k = 0; k < j; ++k) wi -= hh[k] * (double)V[k * ldv + i];
+    w[i] = (T)wi;
+  }
   out[0] = 0;
-  for (int64_t i = 0; i < n; ++i) out[0] += w[i] * w[i];
+  for (int64_t i = 0; i < n; ++i) out[0] += (double)w[i] * (double)w[i];
 }
-void update_norm_cond(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h,
+template <typename T>
+void update_norm_cond(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
                       double *, double *out, const double *w0, const double *w1, hipStream_t)
 {
   if (w1[0] > 0.5 * w0[0]) return;
@@ -138,7 +154,8 @@ void step_close_lanczos(int j, const double *hL, const double *hb2, double *Hd, 
   const bool unsafe = !(h22 <= 0.5 * w1);
   if ((unsafe || beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
 }
-void update_normalize(const double *V, int64_t ldv, int j, double *w, int64_t n, const double *h2,
+template <typename T>
+void update_normalize(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h2,
                       const double *w0, const double *w1, const double *nrm2, int force, hipStream_t)
 {
   const bool re = force || !(w1[0] > 0.5 * w0[0]);
@@ -147,16 +164,18 @@ void update_normalize(const double *V, int64_t ldv, int j, double *w, int64_t n,
   for (int64_t i = 0; i < n; ++i) {
     double wi = w[i];
     if (re)
-      for (int k = 0; k < j; ++k) wi -= hh[k] * V[k * ldv + i];
-    w[i] = wi * inv;
+      for (int k = 0; k < j; ++k) wi -= hh[k] * (double)V[k * ldv + i];
+    w[i] = (T)(wi * inv);
   }
 }
-void normalize(double *w, int64_t n, const double *nrm2, hipStream_t)
+template <typename T>
+void normalize(T *w, int64_t n, const double *nrm2, hipStream_t)
 {
   const double inv = 1.0 / std::sqrt(nrm2[0]);
-  for (int64_t i = 0; i < n; ++i) w[i] *= inv;
+  for (int64_t i = 0; i < n; ++i) w[i] = (T)((double)w[i] * inv);
 }
-void gemm_vy(const double *Vin, int64_t ldv, int m, const double *Y, int ldy, int k, double *Vout, int64_t ldo, int64_t n, hipStream_t)
+template <typename Ti, typename To>
+void gemm_vy(const Ti *Vin, int64_t ldv, int m, const double *Y, int ldy, int k, To *Vout, int64_t ldo, int64_t n, hipStream_t)
 {
   std::vector<double> row(m);
   for (int64_t i = 0; i < n; ++i) {
@@ -164,19 +183,21 @@ void gemm_vy(const double *Vin, int64_t ldv, int m, const double *Y, int ldy, in
     for (int c = 0; c < k; ++c) {
       double s = 0;
       for (int r = 0; r < m; ++r) s += row[r] * Y[r + (int64_t)c * ldy];
-      Vout[c * ldo + i] = s;
+      Vout[c * ldo + i] = (To)s;
     }
   }
 }
-void random_uniform(double *v, int64_t n, uint64_t seed, hipStream_t)
+template <typename T>
+void random_uniform(T *v, int64_t n, uint64_t seed, hipStream_t)
 {
   uint64_t x = seed * 0x9e3779b97f4a7c15ull + 1;
   for (int64_t i = 0; i < n; ++i) {
     x ^= x << 13; x ^= x >> 7; x ^= x << 17;
-    v[i] = ((double)(x >> 11) + 0.5) / 9007199254740992.0;
+    v[i] = (T)(((double)(x >> 11) + 0.5) / 9007199254740992.0);
   }
 }
-void copy(const double *a, double *b, int64_t n, hipStream_t) { for (int64_t i = 0; i < n; ++i) b[i] = a[i]; }
+template <typename Ta, typename Tb>
+void copy(const Ta *a, Tb *b, int64_t n, hipStream_t) { for (int64_t i = 0; i < n; ++i) b[i] = (Tb)a[i]; }
 void to_f32(const double *a, float *b, int64_t n, hipStream_t) { for (int64_t i = 0; i < n; ++i) b[i] = (float)a[i]; }
 void fill(double *a, double v, int64_t n, hipStream_t) { for (int64_t i = 0; i < n; ++i) a[i] = v; }
 void pow2_scale(const double *, int64_t, double *, double *scale, hipStream_t) { scale[0] = 1.0; }
@@ -189,6 +210,35 @@ void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double
       G[r * k + c] = s;
     }
 }
+#define FAKE_KRY_INST(T)                                                                        \
+  template void multidot<T>(const T *, int64_t, int, const T *, int64_t, double *, double *,   \
+                            hipStream_t);                                                      \
+  template void update_dots<T>(const T *, int64_t, int, T *, int64_t, const double *, double *, \
+                               double *, hipStream_t);                                         \
+  template void update_norm<T>(const T *, int64_t, int, T *, int64_t, const double *, double *, \
+                               double *, hipStream_t);                                         \
+  template void update_dots_lanczos<T>(const T *, int64_t, int, T *, int64_t, const double *,   \
+                                       double *, double *, hipStream_t);                       \
+  template void update_norm_cond<T>(const T *, int64_t, int, T *, int64_t, const double *,      \
+                                    double *, double *, const double *, const double *,        \
+                                    hipStream_t);                                              \
+  template void update_normalize<T>(const T *, int64_t, int, T *, int64_t, const double *,      \
+                                    const double *, const double *, const double *, int,       \
+                                    hipStream_t);                                              \
+  template void normalize<T>(T *, int64_t, const double *, hipStream_t);                        \
+  template void random_uniform<T>(T *, int64_t, uint64_t, hipStream_t);
+FAKE_KRY_INST(double)
+FAKE_KRY_INST(float)
+template void gemm_vy<double, double>(const double *, int64_t, int, const double *, int, int,
+                                      double *, int64_t, int64_t, hipStream_t);
+template void gemm_vy<float, float>(const float *, int64_t, int, const double *, int, int,
+                                    float *, int64_t, int64_t, hipStream_t);
+template void gemm_vy<float, double>(const float *, int64_t, int, const double *, int, int,
+                                     double *, int64_t, int64_t, hipStream_t);
+template void copy<double, double>(const double *, double *, int64_t, hipStream_t);
+template void copy<float, float>(const float *, float *, int64_t, hipStream_t);
+template void copy<double, float>(const double *, float *, int64_t, hipStream_t);
+template void copy<float, double>(const float *, double *, int64_t, hipStream_t);
 }  // namespace kry
 }  // namespace ltm
 
@@ -199,19 +249,23 @@ void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io,
 {
   apply_internal(x, y, ld, nvec, io, comp, transpose, sc, s);
 }
-void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype, tm_dtype comp, int transpose,
+void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp, int transpose,
                       const double *, hipStream_t)
 {
   const double *A = tab64.as<double>();
   const int64_t n = ncol;
+  std::vector<double> xv(n);
   for (int v = 0; v < nvec; ++v) {
-    const double *xv = static_cast<const double *>(x) + v * ld;
-    double *yv = static_cast<double *>(y) + v * ld;
+    for (int64_t j = 0; j < n; ++j)
+      xv[j] = io == TM_F32 ? (double)static_cast<const float *>(x)[v * ld + j]
+                           : static_cast<const double *>(x)[v * ld + j];
     for (int64_t i = 0; i < n; ++i) {
       double s = 0;
       for (int64_t j = 0; j < n; ++j) s += (transpose ? A[j + i * n] : A[i + j * n]) * xv[j];
       // FP32 compute: round the result to single precision
-      yv[i] = comp == TM_F32 ? (double)(float)s : s;
+      if (comp == TM_F32) s = (double)(float)s;
+      if (io == TM_F32) static_cast<float *>(y)[v * ld + i] = (float)s;
+      else static_cast<double *>(y)[v * ld + i] = s;
     }
     ++matvecs;
   }

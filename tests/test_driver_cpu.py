@@ -22,11 +22,11 @@ def driver():
     return EXE
 
 
-def run(driver, A, sym, nev, ncv=0, prec=0, left=0, tmp=None):
+def run(driver, A, sym, nev, ncv=0, prec=0, left=0, tmp=None, basis32=0):
     n = A.shape[0]
     path = os.path.join(tmp, "m.txt")
     with open(path, "w") as f:
-        f.write(f"{n} {int(sym)} {nev} {ncv} {prec} {left}\n")
+        f.write(f"{n} {int(sym)} {nev} {ncv} {prec} {left} {basis32}\n")
         f.write(" ".join("%.17g" % v for v in A.flatten(order="F")))
     r = subprocess.run([driver, path], capture_output=True, text=True, timeout=120)
     out = {"evals": [], "res": []}
@@ -69,12 +69,14 @@ def test_nonsymmetric_zbt_pairs_and_left(driver, oracle, tmp_path):
     assert abs(o["left"] / w[0].real - 1) < 1e-10
 
 
-def test_mixed_precision_refinement(driver, oracle, tmp_path):
-    # operator results rounded to FP32 in phase 1; FP64 Rayleigh-Ritz after
+@pytest.mark.parametrize("basis32", [0, 1])
+def test_mixed_precision_refinement(driver, oracle, tmp_path, basis32):
+    # operator results rounded to FP32 in phase 1 (basis stored in FP64 or
+    # FP32); FP64 Rayleigh-Ritz after
     D = oracle.OracleOp(oracle.XAM2D, 12, 0, 2, 0.4, 0.5, 0.0, 0.0).dense()
     D = 0.5 * (D + D.T)
     w = np.sort(np.linalg.eigvalsh(D))[::-1]
-    o = run(driver, D, True, 4, prec=1, tmp=str(tmp_path))
+    o = run(driver, D, True, 4, prec=1, tmp=str(tmp_path), basis32=basis32)
     assert o["mv32"] > 0 and o["mv64"] > 0
     got = np.array([e.real for e in o["evals"]])
     assert abs(got[0] / w[0] - 1) <= 1e-12

@@ -140,3 +140,54 @@ def test_xat3d_full(L, oracle, s1, s2):
     oop = oracle.OracleOp(oracle.XAT3D, s1, s2, 0, 0.25, 0.4, 0.0, 0.05)
     _check_apply(L, oracle, op, oop)
     _check_f32(L, oracle, op, oop)
+
+
+def _check_f32_io(lt, op, oop, transpose=False):
+    """FP32 vectors in and out of the device apply (what the FP32 phase of a
+    mixed solve hands the operator), against the oracle on the same rounded
+    input."""
+    import torch
+    n = op.cols()
+    x32 = _vec(n, 11).astype(np.float32)
+    xd = torch.from_numpy(x32).to("cuda")
+    yd = torch.zeros(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    op.apply_dev(xd.data_ptr(), yd.data_ptr(), n, 1, lt.TM_F32, lt.TM_F32, transpose)
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy().astype(np.float64)
+    yr = oop.apply(x32.astype(np.float64), transpose)
+    err = _rel(y, yr)
+    print(f"  {op.kind()} FP32 io n={n} tr={transpose} rel={err:.2e}", flush=True)
+    assert err <= TOL32, err
+
+
+@pytest.mark.parametrize("case", ["xat14", "xat13_bc", "xdt12", "zbt12", "xam16", "xam16_generic",
+                                  "xam3d_34", "xat3d_34"])
+def test_apply_dev_f32_io(L, oracle, case):
+    O = oracle
+    if case == "xat14":
+        op = L.TransferMatrix("A", 2, 1, 14, J=0.5, kappa=0.6)
+        oop = O.OracleOp(O.XAT, 14, 0, 0, 0.5, 0.6, 0.0, 0.0)
+    elif case == "xat13_bc":
+        op = L.TransferMatrix("A", 2, 9, 13, J=0.45, kappa=0.3, h=0.2)
+        oop = O.OracleOp(O.XAT, 13, 0, 1, 0.45, 0.3, 0.0, 0.2)
+    elif case == "xdt12":
+        op = L.TransferMatrix("D", 2, 1, 12, J=0.4, kappa=0.3, h=0.1)
+        oop = O.OracleOp(O.XDT, 12, 0, 0, 0.4, 0.3, 0.0, 0.1)
+    elif case == "zbt12":
+        op = L.TransferMatrix("B", 2, 3, 12, J=0.3, kappa=0.5, kappa2=0.2)
+        oop = O.OracleOp(O.ZBT, 12, 0, 0, 0.3, 0.5, 0.2, 0.0)
+    elif case in ("xam16", "xam16_generic"):
+        flags = L.TM_FLAG_GENERIC_REDUCED if case.endswith("generic") else 0
+        op = L.TransferMatrix("A", 2, 5, 16, J=0.4, kappa=0.5, neigs=4, flags=flags)
+        oop = O.OracleOp(O.XAM2D, 16, 0, 2, 0.4, 0.5, 0.0, 0.0)
+    elif case == "xam3d_34":
+        op = L.TransferMatrix("A", 3, 1, (3, 4), J=0.25, kappa=0.4)
+        oop = O.OracleOp(O.XAM3D, 3, 4, 0, 0.25, 0.4, 0.0, 0.0)
+    else:
+        op = L.TransferMatrix("A", 3, 1, (3, 4), J=0.25, kappa=0.4, h=0.05,
+                              flags=L.TM_FLAG_FULLSPACE)
+        oop = O.OracleOp(O.XAT3D, 3, 4, 0, 0.25, 0.4, 0.0, 0.05)
+    _check_f32_io(L, op, oop)
+    if case == "zbt12":
+        _check_f32_io(L, op, oop, transpose=True)
