@@ -239,6 +239,9 @@ tm_status tm_solve_batch(tm_op *const *ops, int n, const tm_solve_opts *opts, tm
       if (q >= n) break;
       const int i = order[q];
       const tm_status st = tm_solve(ops[i], &opts[i], &outs[i]);
+      // a batch solves each instance once: free its solver buffers now
+      // rather than holding every instance's basis until the ops go
+      ops[i]->scache.reset();
       if (st != TM_OK) {
         std::lock_guard<std::mutex> lk(mu);
         if (first == TM_OK) {

@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <array>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -86,11 +87,13 @@ class PinnedBuf {
   PinnedBuf(const PinnedBuf &) = delete;
   PinnedBuf &operator=(const PinnedBuf &) = delete;
   void alloc(size_t bytes);
+  void ensure(size_t bytes) { if (bytes > n_) alloc(bytes); }
   void release();
   template <typename T> T *as() const { return static_cast<T *>(p_); }
 
  private:
   void *p_ = nullptr;
+  size_t n_ = 0;
 };
 
 // device buffer (RAII)
@@ -117,6 +120,28 @@ class DevBuf {
  private:
   void *p_ = nullptr;
   size_t n_ = 0;
+};
+
+// Device and pinned buffers plus captured restart-cycle graphs that the
+// eigen driver keeps on an operator between solves (slot numbers are
+// assigned by the driver).  A graph is keyed by every pointer and size its
+// launches embed, so a key match means identical kernel arguments.
+struct SolverCache {
+  static constexpr int NDEV = 24, NPIN = 4;
+  DevBuf dev[NDEV];
+  PinnedBuf pinned[NPIN];
+  struct Graph {
+    std::array<uint64_t, 12> key;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+  SolverCache() = default;
+  SolverCache(const SolverCache &) = delete;
+  SolverCache &operator=(const SolverCache &) = delete;
+  ~SolverCache()
+  {
+    for (auto &g : graphs) destroy_graph_exec(g.exec);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -332,6 +357,13 @@ void pow2_scale(const double *x, int64_t n, double *partial, double *scale, hipS
 // G[i*k + j] = <A_i, B_j>  (A, B are k column vectors)
 void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double *partial,
           double *G, hipStream_t s);
+// G[i*nv + j] = <U_i, U_j> for the nv column vectors of U (one sweep for nv <= 10)
+void gram_sym(const double *U, int64_t ld, int nv, int64_t n, double *partial, double *G,
+              hipStream_t s);
+// out[c] = || sum_{r < nu} Cm[r + c*ldc] U_r ||^2 for c < nc, without forming
+// the combinations (one sweep per 8 columns)
+void combine_norms(const double *U, int64_t ld, int nu, const double *Cm, int ldc, int nc,
+                   int64_t n, double *partial, double *out, hipStream_t s);
 }  // namespace kry
 
 // ---------------------------------------------------------------------------
@@ -372,6 +404,13 @@ struct tm_op {
   std::vector<uint32_t> perm;
   // workspaces
   ltm::DevBuf work, one, hx, hy, px, py;
+  // eigen-driver buffers and graphs kept between solves (created on demand)
+  std::unique_ptr<ltm::SolverCache> scache;
+  ltm::SolverCache &solver_cache()
+  {
+    if (!scache) scache.reset(new ltm::SolverCache);
+    return *scache;
+  }
 
   virtual ~tm_op();
   // T.x on vectors in the internal order (the eigensolver's basis)

@@ -734,6 +734,86 @@ __global__ __launch_bounds__(64) void k_reduce_gram(const double *__restrict__ p
   if (threadIdx.x == 0) out[r * k + c] = s;
 }
 
+// Gram matrix of NV <= 10 vectors U_0..U_{nv-1} (columns at U + i*ld) in one
+// sweep: packed lower triangle, slot a(a+1)/2 + b for b <= a.
+template <int NV>
+__global__ __launch_bounds__(THREADS) void k_gram_tri(const double *__restrict__ U, int64_t ld,
+                                                      int nv, int64_t n,
+                                                      double *__restrict__ partial)
+{
+  constexpr int NT = NV * (NV + 1) / 2;
+  double acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = 0;
+  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * THREADS) {
+    double u[NV];
+#pragma unroll
+    for (int a = 0; a < NV; ++a) u[a] = a < nv ? U[a * ld + i] : 0.0;
+#pragma unroll
+    for (int a = 0; a < NV; ++a)
+#pragma unroll
+      for (int b = 0; b <= a; ++b) acc[a * (a + 1) / 2 + b] += u[a] * u[b];
+  }
+  block_reduce_write<NT>(acc, NT, partial + (int64_t)blockIdx.x * NT);
+}
+
+// G[a * nv + b] = G[b * nv + a] = sum over blocks of triangle slot (a, b)
+__global__ __launch_bounds__(64) void k_reduce_tri(const double *__restrict__ partial, int nblk,
+                                                   int nt, int nv, double *__restrict__ G)
+{
+  const int t = blockIdx.x;
+  int a = 0;
+  while ((a + 1) * (a + 2) / 2 <= t) ++a;
+  const int b = t - a * (a + 1) / 2;
+  double s = 0;
+  for (int q = threadIdx.x; q < nblk; q += 64) s += partial[(int64_t)q * nt + t];
+  s = wave_sum(s);
+  if (threadIdx.x == 0 && a < nv) {
+    G[a * nv + b] = s;
+    G[b * nv + a] = s;
+  }
+}
+
+// out[c0 + c] = || sum_r C[r + (c0 + c) * ldc] U_r ||^2 for c < nc <= 8, r < nu
+// <= NU: combinations of the columns of U and their squared norms in one
+// sweep, without writing the combinations.
+template <int NU>
+__global__ __launch_bounds__(THREADS) void k_combine_norms(const double *__restrict__ U,
+                                                           int64_t ld, int nu,
+                                                           const double *__restrict__ Cm, int ldc,
+                                                           int c0, int nc, int64_t n,
+                                                           double *__restrict__ partial)
+{
+  constexpr int NC = 8;
+  __shared__ double sC[NU * NC];
+  for (int t = threadIdx.x; t < NU * NC; t += blockDim.x) {
+    const int r = t % NU, c = t / NU;
+    sC[t] = (r < nu && c < nc) ? Cm[r + (int64_t)(c0 + c) * ldc] : 0.0;
+  }
+  __syncthreads();
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0;
+  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * THREADS) {
+    // keep the coefficients in LDS: without this barrier the compiler hoists
+    // all NU * NC of them into registers
+    __asm__ volatile("" ::: "memory");
+    double u[NU];
+#pragma unroll
+    for (int r = 0; r < NU; ++r) u[r] = r < nu ? U[r * ld + i] : 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      double x = 0;
+#pragma unroll
+      for (int r = 0; r < NU; ++r) x += sC[r + c * NU] * u[r];
+      acc[c] += x * x;
+    }
+  }
+  block_reduce_write<NC>(acc, NC, partial + (int64_t)blockIdx.x * NC);
+}
+
 unsigned nblocks(int64_t n)
 {
   int64_t b = (n + THREADS - 1) / THREADS;
@@ -992,6 +1072,49 @@ void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double
                          G + row * k);
       TM_LAUNCH_CHECK();
     }
+  }
+}
+
+void gram_sym(const double *U, int64_t ld, int nv, int64_t n, double *partial, double *G,
+              hipStream_t s)
+{
+  TM_CHECK(nv >= 1 && nv <= MAXJ, TM_EINVAL, "gram block too large");
+  if (nv > 10) {
+    gram(U, U, ld, nv, n, partial, G, s);
+    return;
+  }
+  const unsigned nb = nblocks(n);
+  const int nt = nv <= 6 ? 21 : 55;
+  if (nv <= 6)
+    hipLaunchKernelGGL(k_gram_tri<6>, dim3(nb), dim3(THREADS), 0, s, U, ld, nv, n, partial);
+  else
+    hipLaunchKernelGGL(k_gram_tri<10>, dim3(nb), dim3(THREADS), 0, s, U, ld, nv, n, partial);
+  hipLaunchKernelGGL(k_reduce_tri, dim3(nt), dim3(64), 0, s, partial, (int)nb, nt, nv, G);
+  TM_LAUNCH_CHECK();
+}
+
+void combine_norms(const double *U, int64_t ld, int nu, const double *Cm, int ldc, int nc,
+                   int64_t n, double *partial, double *out, hipStream_t s)
+{
+  TM_CHECK(nu >= 1 && nu <= MAXJ && nc >= 1, TM_EINVAL, "combination block too large");
+  const unsigned nb = nblocks(n);
+  for (int c0 = 0; c0 < nc; c0 += 8) {
+    const int m = std::min(8, nc - c0);
+    if (nu <= 8)
+      hipLaunchKernelGGL(k_combine_norms<8>, dim3(nb), dim3(THREADS), 0, s, U, ld, nu, Cm, ldc, c0,
+                         m, n, partial);
+    else if (nu <= 16)
+      hipLaunchKernelGGL(k_combine_norms<16>, dim3(nb), dim3(THREADS), 0, s, U, ld, nu, Cm, ldc,
+                         c0, m, n, partial);
+    else if (nu <= 32)
+      hipLaunchKernelGGL(k_combine_norms<32>, dim3(nb), dim3(THREADS), 0, s, U, ld, nu, Cm, ldc,
+                         c0, m, n, partial);
+    else
+      hipLaunchKernelGGL(k_combine_norms<64>, dim3(nb), dim3(THREADS), 0, s, U, ld, nu, Cm, ldc,
+                         c0, m, n, partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(m), dim3(64), 0, s, partial, (int)nb, 8,
+                       out + c0);
+    TM_LAUNCH_CHECK();
   }
 }
 

@@ -106,6 +106,7 @@ void PinnedBuf::alloc(size_t bytes)
     (void)hipGetLastError();
     throw Error(TM_ENOMEM, "pinned host allocation failed");
   }
+  n_ = bytes;
 }
 
 void PinnedBuf::release()
@@ -118,6 +119,7 @@ void PinnedBuf::release()
     (void)hipHostFree(p_);
   }
   p_ = nullptr;
+  n_ = 0;
 }
 
 void DevBuf::alloc(size_t bytes)

@@ -35,6 +35,15 @@ namespace {
 
 constexpr double EPS = 2.220446049250313e-16;
 
+// Operator-cache slots: KrylovSchur role r uses dev[SLOT_ROLE * r + 0..4]
+// and pinned[r] (r = 0 main run, 1 FP64 continuation of a mixed run, 2 left
+// vector); the refinement block and its scratch, and the Ritz vectors handed
+// to the caller, have slots of their own.
+constexpr int SLOT_ROLE = 6;
+constexpr int SLOT_REFINE_B = 18, SLOT_REFINE_PARTIAL = 19, SLOT_REFINE_G = 20,
+              SLOT_REFINE_C = 21, SLOT_XR = 22, SLOT_XI = 23;
+static_assert(SLOT_XI < SolverCache::NDEV, "operator cache slots");
+
 struct Ritz {
   std::vector<double> wr, wi;     // eigenvalues (sorted by magnitude, descending)
   std::vector<double> Zr, Zi;     // eigenvectors of the projected matrix (m x m)
@@ -65,27 +74,29 @@ class KrylovRun {
 template <typename T>
 class KrylovSchur final : public KrylovRun {
  public:
+  // role (0, 1, 2) selects the operator's cached buffers, so that runs that
+  // are alive at the same time (FP32 phase and FP64 continuation, right and
+  // left solves) never share them.
   KrylovSchur(tm_op *op, int nev, int m, double tol, int maxit, tm_dtype comp, int transpose,
-              uint64_t seed, int verbose)
+              uint64_t seed, int verbose, int role)
       : op_(op), nev_(nev), m_(m), tol_(tol), maxit_(maxit), comp_(comp), tr_(transpose),
-        seed_(seed), verbose_(verbose)
+        seed_(seed), verbose_(verbose), cache_(op->solver_cache()),
+        V_(cache_.dev[SLOT_ROLE * role + 0]), partial_(cache_.dev[SLOT_ROLE * role + 1]),
+        hbuf_(cache_.dev[SLOT_ROLE * role + 2]), Ydev_(cache_.dev[SLOT_ROLE * role + 3]),
+        Hd_(cache_.dev[SLOT_ROLE * role + 4]), stage_(cache_.pinned[role])
   {
     n_ = op->ncol;
     ldv_ = ((n_ + 63) / 64) * 64;
     sym_ = op->symmetric;
-    V_.alloc(sizeof(T) * ldv_ * (m_ + 1));
-    partial_.alloc(sizeof(double) * kry::NBLK * 72);
-    hbuf_.alloc(sizeof(double) * 3 * 72);
-    Ydev_.alloc(sizeof(double) * 72 * 72);
+    V_.ensure(sizeof(T) * ldv_ * (m_ + 1));
+    partial_.ensure(sizeof(double) * kry::NBLK * 72);
+    hbuf_.ensure(sizeof(double) * 3 * 72);
+    Ydev_.ensure(sizeof(double) * 72 * 72);
     H_.assign((size_t)(m_ + 1) * m_, 0.0);
-    Hd_.alloc(sizeof(double) * ((size_t)(m_ + 1) * m_ + 8));
-    stage_.alloc(sizeof(double) * 72 * 72);
+    Hd_.ensure(sizeof(double) * ((size_t)(m_ + 1) * m_ + 8));
+    stage_.ensure(sizeof(double) * 72 * 72);
   }
 
-  ~KrylovSchur()
-  {
-    for (auto &g : graphs_) destroy_graph_exec(g.exec);
-  }
   KrylovSchur(const KrylovSchur &) = delete;
   KrylovSchur &operator=(const KrylovSchur &) = delete;
   void set_graph(bool on) override { use_graph_ = on; }
@@ -136,13 +147,20 @@ class KrylovSchur final : public KrylovRun {
     const int k0 = k_;
     // Cycles after the first (k0 > 0, buffers already allocated) are
     // captured once per k0 into a graph and replayed.
+    // The graph is kept on the operator and reused by later solves whose
+    // launches would carry the same arguments.
     hipGraphExec_t ex = nullptr;
     if (use_graph_ && k0 > 0) {
-      for (const auto &g : graphs_)
-        if (g.k0 == k0 && g.work == op_->work.get()) ex = g.exec;
+      const std::array<uint64_t, 12> key{
+          (uint64_t)sizeof(T), (uint64_t)comp_, (uint64_t)tr_, (uint64_t)m_, (uint64_t)k0,
+          (uint64_t)n_, (uint64_t)ldv_, (uint64_t)(uintptr_t)V_.get(),
+          (uint64_t)(uintptr_t)partial_.get(), (uint64_t)(uintptr_t)hbuf_.get(),
+          (uint64_t)(uintptr_t)Hd_.get(), (uint64_t)(uintptr_t)op_->work.get()};
+      for (const auto &g : cache_.graphs)
+        if (g.key == key) ex = g.exec;
       if (!ex) {
         ex = capture_cycle(k0);
-        if (ex) graphs_.push_back({k0, op_->work.get(), ex});
+        if (ex) cache_.graphs.push_back({key, ex});
         else use_graph_ = false;
       }
     }
@@ -547,21 +565,15 @@ class KrylovSchur final : public KrylovRun {
   int64_t n_ = 0, ldv_ = 0;
   bool device_steps_ = true;
   bool use_graph_ = true;
-  struct CycleGraph {
-    int k0;
-    void *work;
-    hipGraphExec_t exec;
-  };
-  std::vector<CycleGraph> graphs_;
   int64_t graph_cycles_ = 0;
-  DevBuf Hd_;
-  PinnedBuf stage_;
+  SolverCache &cache_;
+  DevBuf &V_, &partial_, &hbuf_, &Ydev_, &Hd_;
+  PinnedBuf &stage_;
   bool sym_ = true;
   int k_ = 0;
   int64_t matvecs_ = 0, restarts_ = 0;
   bool stall_stop_ = false, stalled_ = false;
   int64_t reorths_ = 0;
-  DevBuf V_, partial_, hbuf_, Ydev_;
   std::vector<double> H_;
 };
 
@@ -588,105 +600,113 @@ struct RefineOut {
   std::vector<double> theta, res;
   double bound = 0;
   int64_t matvecs = 0;
+  bool failed = false;
 };
 
-// FP64 Rayleigh-Ritz refinement of a block Y (k vectors at B[0..k), leading
-// dimension ld; orthonormalised first unless input_orthonormal) for a
-// symmetric operator.  B must hold 3k
-// vectors: [Y | W = A Y | X = Y Z].  Each iteration: W = A Y, Ritz pairs of
-// Y^T W, residuals R = W Z - X diag(theta) (into the W slots); accepted when
-// the FP64 residual meets tol or the Rayleigh-Ritz error bound
-// r^2 / (gap |theta|) (gap to the Krylov Ritz values outside the block) is
-// below 1e-13 for every wanted pair; otherwise Y <- orth(A X) = orth(R + X theta)
-// (one subspace-iteration step) and repeat.  On return B[0..nev) hold the
-// refined, normalised Ritz vectors.
+// FP64 Rayleigh-Ritz refinement of the block Y = B[0..k) (leading dimension
+// ld; any basis of the subspace, e.g. Ritz vectors of an FP32-stored Krylov
+// basis) for a symmetric operator.  B holds 2k vectors [Y | W].  Each
+// iteration: W = A Y (k FP64 matvecs); the Gram matrix of [Y | W] in one
+// sweep; on the host, with Y^T Y = L L^T (Cholesky), the Ritz pairs
+// (theta, q) of L^-1 sym(Y^T W) L^-T and Z = L^-T Q, so that X = Y Z has
+// orthonormal columns; then the residual norms ||W z - theta Y z|| in one
+// sweep that forms the sums explicitly (no cancellation) without storing
+// them.  Accepted when every wanted residual meets tol or the Rayleigh-Ritz
+// error bound r^2 / (gap |theta|) (gap to the Krylov Ritz values outside the
+// block) is below 1e-13; otherwise Y <- W Z diag(1/theta) (= A X scaled, one
+// subspace-iteration step) and repeat.  On return B[0..nev) hold X.
+// failed = true if Y^T Y is not positive definite (a rank-deficient block).
 RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev, double tol,
-                     int maxiter, int verbose, const std::vector<double> &outside,
-                     bool input_orthonormal)
+                     int maxiter, int verbose, const std::vector<double> &outside)
 {
   hipStream_t s = op->stream;
   RefineOut ro;
-  DevBuf partial(sizeof(double) * kry::NBLK * 72), G(sizeof(double) * 72 * 72),
-      C(sizeof(double) * 2 * 72 * 72);
+  const int k2 = 2 * k;
+  SolverCache &cache = op->solver_cache();
+  DevBuf &partial = cache.dev[SLOT_REFINE_PARTIAL], &G = cache.dev[SLOT_REFINE_G],
+         &C = cache.dev[SLOT_REFINE_C];
+  PinnedBuf &stage = cache.pinned[3];
+  partial.ensure(sizeof(double) * kry::NBLK * 72);
+  G.ensure(sizeof(double) * ((size_t)k2 * k2 + k));
+  C.ensure(sizeof(double) * ((size_t)k2 * k + (size_t)k * k));
+  stage.ensure(sizeof(double) * ((size_t)k2 * k2 + (size_t)k2 * k + (size_t)k * k + k));
+  double *hG = stage.as<double>();               // Gram matrix read back
+  double *hCr = hG + (size_t)k2 * k2;            // residual combinations (k2 x k)
+  double *hZ = hCr + (size_t)k2 * k;             // next-block / output combinations
+  double *hR = hZ + (size_t)k * k;               // residual norms^2 read back
+  double *dCr = C.as<double>(), *dZ = dCr + (size_t)k2 * k;
+  double *dR = G.as<double>() + (size_t)k2 * k2;
   auto col = [&](int c) { return B + (size_t)c * ld; };
-  std::vector<double> hG((size_t)k * k), th(k), Z((size_t)k * k);
-  auto upload = [&](const std::vector<double> &h) {
-    TM_HIP(hipMemcpyAsync(C.get(), h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s));
-  };
-  auto gram_host = [&](int a, int b) {  // hG[r*k + c] = <col(a+r), col(b+c)>
-    kry::gram(col(a), col(b), ld, k, n, partial.as<double>(), G.as<double>(), s);
-    TM_HIP(hipMemcpyAsync(hG.data(), G.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
-    TM_HIP(hipStreamSynchronize(s));
-  };
-  // B[0..k) <- B[0..k) L^-T with G = L L^T its Gram matrix (Cholesky QR:
-  // orthonormal columns), or (full = false, or G not positive definite) each
-  // column normalised
-  auto orth0 = [&](bool full) {
-    gram_host(0, 0);
-    std::vector<double> Gm((size_t)k * k);
-    for (int i = 0; i < k; ++i)
-      for (int j = 0; j < k; ++j) Gm[i + (size_t)j * k] = hG[i * k + j];
-    std::vector<double> Rinv((size_t)k * k, 0.0);
-    if (full && cholesky(k, Gm)) {
-      for (int c = 0; c < k; ++c) {
-        std::vector<double> e(k, 0.0);
-        e[c] = 1.0;
-        for (int i = k - 1; i >= 0; --i) {
-          double s2 = e[i];
-          for (int t = i + 1; t < k; ++t) s2 -= Gm[t + (size_t)i * k] * e[t];
-          e[i] = s2 / Gm[i + (size_t)i * k];
-        }
-        for (int i = 0; i < k; ++i) Rinv[i + (size_t)c * k] = e[i];
-      }
-    } else {
-      for (int i = 0; i < k; ++i)
-        Rinv[i + (size_t)i * k] = 1.0 / std::sqrt(std::max(hG[i * k + i], 1e-300));
-    }
-    upload(Rinv);
-    kry::gemm_vy(col(0), ld, k, C.as<double>(), k, k, col(0), ld, n, s);
-    TM_HIP(hipStreamSynchronize(s));
-  };
-  // Ritz vectors of an FP32-stored basis are orthonormal only to FP32
-  // rounding; the Rayleigh-Ritz step below needs them orthonormal in FP64
-  if (!input_orthonormal) orth0(true);
   for (int it = 0; it < maxiter; ++it) {
     for (int c = 0; c < k; ++c) {
       op->apply_internal(col(c), col(k + c), ld, 1, TM_F64, TM_F64, 0, nullptr, s);
       ++ro.matvecs;
     }
-    gram_host(0, k);
-    std::vector<double> Hp((size_t)k * k);
+    kry::gram_sym(col(0), ld, k2, n, partial.as<double>(), G.as<double>(), s);
+    TM_HIP(hipMemcpyAsync(hG, G.get(), sizeof(double) * k2 * k2, hipMemcpyDeviceToHost, s));
+    TM_HIP(hipStreamSynchronize(s));
+    // L = chol(Y^T Y), Li = L^-1, M = Li sym(Y^T W) Li^T
+    std::vector<double> Lf((size_t)k * k), Li((size_t)k * k, 0.0), M((size_t)k * k);
     for (int i = 0; i < k; ++i)
-      for (int j = 0; j < k; ++j) Hp[i + (size_t)j * k] = 0.5 * (hG[i * k + j] + hG[j * k + i]);
-    sym_eig(k, Hp.data(), th.data(), Z.data());
+      for (int j = 0; j < k; ++j) Lf[i + (size_t)j * k] = hG[i * k2 + j];
+    if (!cholesky(k, Lf)) {
+      ro.failed = true;
+      return ro;
+    }
+    for (int c = 0; c < k; ++c)  // forward substitution, column c of Li
+      for (int i = c; i < k; ++i) {
+        double t = i == c ? 1.0 : 0.0;
+        for (int q = c; q < i; ++q) t -= Lf[i + (size_t)q * k] * Li[q + (size_t)c * k];
+        Li[i + (size_t)c * k] = t / Lf[i + (size_t)i * k];
+      }
+    std::vector<double> S((size_t)k * k), LS((size_t)k * k, 0.0);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j)
+        S[i + (size_t)j * k] = 0.5 * (hG[i * k2 + k + j] + hG[j * k2 + k + i]);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) {
+        double t = 0;
+        for (int q = 0; q <= i; ++q) t += Li[i + (size_t)q * k] * S[q + (size_t)j * k];
+        LS[i + (size_t)j * k] = t;
+      }
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) {
+        double t = 0;
+        for (int q = 0; q <= j; ++q) t += LS[i + (size_t)q * k] * Li[j + (size_t)q * k];
+        M[i + (size_t)j * k] = t;
+      }
+    std::vector<double> th(k), Q((size_t)k * k);
+    sym_eig(k, M.data(), th.data(), Q.data());
     std::vector<int> idx(k);
     std::iota(idx.begin(), idx.end(), 0);
     std::stable_sort(idx.begin(), idx.end(),
                      [&](int a, int b) { return std::fabs(th[a]) > std::fabs(th[b]); });
-    std::vector<double> ths(k), Zs((size_t)k * k);
+    // Z = Li^T Q (sorted)
+    std::vector<double> ths(k), Z((size_t)k * k);
     for (int c = 0; c < k; ++c) {
       ths[c] = th[idx[c]];
-      for (int t = 0; t < k; ++t) Zs[t + (size_t)c * k] = Z[t + (size_t)idx[c] * k];
+      for (int t = 0; t < k; ++t) {
+        double z = 0;
+        for (int q = t; q < k; ++q) z += Li[q + (size_t)t * k] * Q[q + (size_t)idx[c] * k];
+        Z[t + (size_t)c * k] = z;
+      }
     }
-    // X = [Y|W] [Z; 0] -> slots 2k..3k ;  R = [Y|W] [-Z theta; Z] -> slots k..2k
-    std::vector<double> Cx((size_t)2 * k * k, 0.0), Cr((size_t)2 * k * k, 0.0);
+    // residual r_c = [Y | W] [-theta_c z_c; z_c]: norms of the wanted ones
     for (int c = 0; c < k; ++c)
       for (int t = 0; t < k; ++t) {
-        Cx[t + (size_t)c * 2 * k] = Zs[t + (size_t)c * k];
-        Cr[t + (size_t)c * 2 * k] = -ths[c] * Zs[t + (size_t)c * k];
-        Cr[k + t + (size_t)c * 2 * k] = Zs[t + (size_t)c * k];
+        hCr[t + (size_t)c * k2] = -ths[c] * Z[t + (size_t)c * k];
+        hCr[k + t + (size_t)c * k2] = Z[t + (size_t)c * k];
       }
-    upload(Cx);
-    kry::gemm_vy(col(0), ld, 2 * k, C.as<double>(), 2 * k, k, col(2 * k), ld, n, s);
-    upload(Cr);
-    kry::gemm_vy(col(0), ld, 2 * k, C.as<double>(), 2 * k, k, col(k), ld, n, s);
-    gram_host(k, k);
+    TM_HIP(hipMemcpyAsync(dCr, hCr, sizeof(double) * k2 * nev, hipMemcpyHostToDevice, s));
+    kry::combine_norms(col(0), ld, k2, dCr, k2, nev, n, partial.as<double>(), dR, s);
+    TM_HIP(hipMemcpyAsync(hR, dR, sizeof(double) * nev, hipMemcpyDeviceToHost, s));
+    TM_HIP(hipStreamSynchronize(s));
     ro.theta = ths;
     ro.res.assign(k, 0.0);
     double worst = 0, bound = 0;
     std::vector<double> bnd(nev);
     for (int c = 0; c < nev; ++c) {
-      const double r = std::sqrt(std::max(hG[c * k + c], 0.0));
+      const double r = std::sqrt(std::max(hR[c], 0.0));
       ro.res[c] = r / std::fabs(ths[c]);
       worst = std::max(worst, ro.res[c]);
       // error bound r^2 / (gap |theta|), gap to the spectrum outside the block
@@ -705,16 +725,22 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
     }
     ro.bound = bound;
     const bool done = worst <= tol || bound <= 1e-13 || it == maxiter - 1;
-    // next block: done -> normalised X ; else orth(A X) = orth(R + X theta)
-    std::vector<double> Cn((size_t)2 * k * k, 0.0);
-    for (int c = 0; c < k; ++c) {
-      if (!done) Cn[c + (size_t)c * 2 * k] = 1.0;            // R
-      Cn[k + c + (size_t)c * 2 * k] = done ? 1.0 : ths[c];   // X
+    if (done) {
+      // X = Y Z[:, :nev] in place (every row is read before it is written)
+      for (int c = 0; c < nev; ++c)
+        for (int t = 0; t < k; ++t) hZ[t + (size_t)c * k] = Z[t + (size_t)c * k];
+      TM_HIP(hipMemcpyAsync(dZ, hZ, sizeof(double) * k * nev, hipMemcpyHostToDevice, s));
+      kry::gemm_vy(col(0), ld, k, dZ, k, nev, col(0), ld, n, s);
+      TM_HIP(hipStreamSynchronize(s));
+      break;
     }
-    upload(Cn);
-    kry::gemm_vy(col(k), ld, 2 * k, C.as<double>(), 2 * k, k, col(0), ld, n, s);
-    orth0(!done);
-    if (done) break;
+    // next block Y = W Z diag(1/theta) (close to orthonormal)
+    for (int c = 0; c < k; ++c)
+      for (int t = 0; t < k; ++t)
+        hZ[t + (size_t)c * k] = Z[t + (size_t)c * k] / (ths[c] != 0 ? ths[c] : 1.0);
+    TM_HIP(hipMemcpyAsync(dZ, hZ, sizeof(double) * k * k, hipMemcpyHostToDevice, s));
+    kry::gemm_vy(col(k), ld, k, dZ, k, k, col(0), ld, n, s);
+    TM_HIP(hipStreamSynchronize(s));
   }
   return ro;
 }
@@ -746,10 +772,11 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   for (;;) {
     const double tol1 = mixed ? std::max(o.mixed_tol, o.tol) : o.tol;
     if (mixed && o.basis_f32)
-      ks.reset(new KrylovSchur<float>(op, nev, ncv, tol1, o.maxit, TM_F32, 0, o.seed, o.verbose));
+      ks.reset(new KrylovSchur<float>(op, nev, ncv, tol1, o.maxit, TM_F32, 0, o.seed, o.verbose,
+                                      0));
     else
       ks.reset(new KrylovSchur<double>(op, nev, ncv, tol1, o.maxit, mixed ? TM_F32 : TM_F64, 0,
-                                       o.seed, o.verbose));
+                                       o.seed, o.verbose, 0));
     if (mixed) ks->set_stall_stop(true);
     ks->set_graph(o.use_graph != 0);
     nc = ks->run(rz, nullptr);
@@ -771,42 +798,54 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   }
   const int64_t ldv = ks->ldv();
   std::vector<double> evr(nev), evi(nev), res(nev);
-  DevBuf Xr, Xi;
+  SolverCache &cache = op->solver_cache();
+  DevBuf &Xr = cache.dev[SLOT_XR], &Xi = cache.dev[SLOT_XI];
   const bool cplx = !op->symmetric;
   const int refine = o.refine != 0 ? o.refine : (op->symmetric ? 1 : 2);
+  const double *xr = nullptr, *xi = nullptr;  // device Ritz vectors (ld ldv)
+  bool refined = false;
   if (mixed && op->symmetric && refine == 1) {
     // FP64 block Rayleigh-Ritz refinement of the kept FP32 Ritz vectors
     const int k = std::min(ks->m() - 1, nev + 1);
-    DevBuf B(sizeof(double) * ldv * 3 * k);
+    DevBuf &B = cache.dev[SLOT_REFINE_B];
+    B.ensure(sizeof(double) * ldv * 2 * k);
     ks->ritz_vectors(rz, k, B.as<double>(), nullptr);
     std::vector<double> outside;
     for (int d = k; d < ks->m(); ++d) outside.push_back(rz.wr[d]);
-    RefineOut ro = refine_sym(op, B.as<double>(), ldv, n, k, nev, o.tol, 4, o.verbose, outside,
-                              !o.basis_f32);
+    RefineOut ro = refine_sym(op, B.as<double>(), ldv, n, k, nev, o.tol, 4, o.verbose, outside);
     mv64 += ro.matvecs;
-    for (int i = 0; i < nev; ++i) {
-      evr[i] = ro.theta[i];
-      evi[i] = 0;
-      res[i] = ro.res[i];
+    if (!ro.failed) {
+      for (int i = 0; i < nev; ++i) {
+        evr[i] = ro.theta[i];
+        evi[i] = 0;
+        res[i] = ro.res[i];
+      }
+      nc = 0;
+      for (int i = 0; i < nev; ++i) nc += std::isfinite(ro.theta[i]) ? 1 : 0;
+      xr = B.as<double>();
+      refined = true;
+    } else if (o.verbose) {
+      std::fprintf(stderr, "[tm_refine] rank-deficient block; FP64 Krylov-Schur instead\n");
     }
-    nc = 0;
-    for (int i = 0; i < nev; ++i) nc += std::isfinite(ro.theta[i]) ? 1 : 0;
-    Xr.alloc(sizeof(double) * ldv * nev);
-    copy_sync(Xr.get(), B.get(), sizeof(double) * ldv * nev, hipMemcpyDeviceToDevice, op->stream);
-  } else {
+  }
+  if (!refined) {
     std::unique_ptr<KrylovRun> k2;
     KrylovRun *fin = ks.get();
     if (mixed) {
       // FP64 Krylov-Schur warm-started from the sum of the wanted FP32 Ritz
       // vectors (real parts): the FP64 run converges to tol in a cycle or two
-      DevBuf X0(sizeof(double) * ldv * nev), v0(sizeof(double) * ldv);
+      DevBuf &X0 = cache.dev[SLOT_REFINE_B], &v0 = cache.dev[SLOT_REFINE_C];
+      X0.ensure(sizeof(double) * ldv * nev);
+      v0.ensure(sizeof(double) * ldv);
       ks->ritz_vectors(rz, nev, X0.as<double>(), nullptr);
       std::vector<double> ones((size_t)nev * 1, 1.0);
-      DevBuf Y1(sizeof(double) * nev);
+      DevBuf &Y1 = cache.dev[SLOT_REFINE_G];
+      Y1.ensure(sizeof(double) * nev);
       copy_sync(Y1.get(), ones.data(), sizeof(double) * nev, hipMemcpyHostToDevice, op->stream);
       kry::gemm_vy(X0.as<double>(), ldv, nev, Y1.as<double>(), nev, 1, v0.as<double>(), ldv, n,
                    op->stream);
-      k2.reset(new KrylovSchur<double>(op, nev, ncv, o.tol, o.maxit, TM_F64, 0, o.seed, o.verbose));
+      k2.reset(new KrylovSchur<double>(op, nev, ncv, o.tol, o.maxit, TM_F64, 0, o.seed, o.verbose,
+                                       1));
       k2->set_graph(o.use_graph != 0);
       nc = k2->run(rz, v0.as<double>());
       mv64 += k2->matvecs();
@@ -814,9 +853,11 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
       gcycles += k2->graph_cycles();
       fin = k2.get();
     }
-    Xr.alloc(sizeof(double) * fin->ldv() * nev);
-    if (cplx) Xi.alloc(sizeof(double) * fin->ldv() * nev);
+    Xr.ensure(sizeof(double) * fin->ldv() * nev);
+    if (cplx) Xi.ensure(sizeof(double) * fin->ldv() * nev);
     fin->ritz_vectors(rz, nev, Xr.as<double>(), cplx ? Xi.as<double>() : nullptr);
+    xr = Xr.as<double>();
+    xi = cplx ? Xi.as<double>() : nullptr;
     for (int i = 0; i < nev; ++i) {
       evr[i] = rz.wr[i];
       evi[i] = rz.wi[i];
@@ -836,11 +877,11 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
       for (int i = 0; i < nev; ++i) {
         double *dst = out->vecs + (size_t)i * n;
         if (op->perm.empty()) {
-          copy_sync(dst, Xr.as<double>() + (size_t)i * ldv, sizeof(double) * n,
+          copy_sync(dst, xr + (size_t)i * ldv, sizeof(double) * n,
                     hipMemcpyDeviceToHost, op->stream);
         } else {
           // internal orbit order -> ascending representatives
-          copy_sync(tmp.data(), Xr.as<double>() + (size_t)i * ldv, sizeof(double) * n,
+          copy_sync(tmp.data(), xr + (size_t)i * ldv, sizeof(double) * n,
                     hipMemcpyDeviceToHost, op->stream);
           for (int64_t t = 0; t < n; ++t) dst[op->perm[t]] = tmp[t];
         }
@@ -849,8 +890,8 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
       // cols() x nev complex128 interleaved, column-major (pfunc.hpp:139)
       std::vector<double> re(n), im(n);
       for (int i = 0; i < nev; ++i) {
-        copy_sync(re.data(), Xr.as<double>() + (size_t)i * ldv, sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
-        copy_sync(im.data(), Xi.as<double>() + (size_t)i * ldv, sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
+        copy_sync(re.data(), xr + (size_t)i * ldv, sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
+        copy_sync(im.data(), xi + (size_t)i * ldv, sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
         for (int64_t t = 0; t < n; ++t) {
           out->vecs[2 * ((size_t)i * n + t)] = re[t];
           out->vecs[2 * ((size_t)i * n + t) + 1] = im[t];
@@ -863,7 +904,7 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   if (o.want_left && !op->symmetric) {
     int ncvl = 7;
     if (ncvl >= n) ncvl = (int)n - 1;
-    KrylovSchur<double> kl(op, 1, ncvl, o.tol, o.maxit, TM_F64, 1, o.seed + 1, o.verbose);
+    KrylovSchur<double> kl(op, 1, ncvl, o.tol, o.maxit, TM_F64, 1, o.seed + 1, o.verbose, 2);
     kl.set_graph(o.use_graph != 0);
     Ritz rl;
     const int ncl = kl.run(rl, nullptr);
@@ -872,7 +913,8 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
     if (ncl >= 1) {
       out->eval_left = rl.wr[0];
       if (out->vecl) {
-        DevBuf XL(sizeof(double) * kl.ldv());
+        DevBuf &XL = cache.dev[SLOT_XI];  // the right vectors are on the host by now
+        XL.ensure(sizeof(double) * kl.ldv());
         kl.ritz_vectors(rl, 1, XL.as<double>(), nullptr);
         copy_sync(out->vecl, XL.get(), sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
       }

@@ -20,11 +20,13 @@ void PinnedBuf::alloc(size_t bytes)
 {
   release();
   p_ = bytes ? std::malloc(bytes) : nullptr;
+  n_ = p_ ? bytes : 0;
 }
 void PinnedBuf::release()
 {
   std::free(p_);
   p_ = nullptr;
+  n_ = 0;
 }
 void DevBuf::alloc(size_t bytes)
 {
@@ -209,6 +211,23 @@ void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double
       for (int64_t i = 0; i < n; ++i) s += A[r * ld + i] * B[c * ld + i];
       G[r * k + c] = s;
     }
+}
+void gram_sym(const double *U, int64_t ld, int nv, int64_t n, double *p, double *G, hipStream_t st)
+{
+  gram(U, U, ld, nv, n, p, G, st);
+}
+void combine_norms(const double *U, int64_t ld, int nu, const double *Cm, int ldc, int nc, int64_t n,
+                   double *, double *out, hipStream_t)
+{
+  for (int c = 0; c < nc; ++c) {
+    double s = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      double x = 0;
+      for (int r = 0; r < nu; ++r) x += Cm[r + (int64_t)c * ldc] * U[r * ld + i];
+      s += x * x;
+    }
+    out[c] = s;
+  }
 }
 #define FAKE_KRY_INST(T)                                                                        \
   template void multidot<T>(const T *, int64_t, int, const T *, int64_t, double *, double *,   \
