@@ -587,7 +587,11 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
-  const int c = blockIdx.x;
+  // within a row necklace the orbits are ordered by their E necklace, so
+  // columns c, c+1, ... gather neighbouring runs of x~: keep consecutive
+  // columns on one XCD so that they share its L2 lines
+  const int c = (int)dev::xcd_remap(blockIdx.x, nNk);
+  if (c < 0) return;
   const int nh = 1 << n;
   const uint32_t *mrow = cmap + ((uint64_t)c << n);
   // cmap columns are in necklace-major O' order (nkl): consecutive lanes
@@ -712,7 +716,8 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
     const int n = ft.n;
     C *Ap = rest;
     const size_t shm = (size_t(1) << n) * sizeof(C);
-    hipLaunchKernelGGL((k_fused_pass1<C>), dim3(ft.nNk), dim3(FUSED_THREADS), shm, s,
+    const unsigned g1 = 8u * (unsigned)((ft.nNk + 7) / 8);
+    hipLaunchKernelGGL((k_fused_pass1<C>), dim3(g1), dim3(FUSED_THREADS), shm, s,
                        ft.cmap.as<uint32_t>(), ft.nkl.as<uint16_t>(), xt, Ap,
                        ft.slotO.as<uint16_t>(), n, ft.nNk, ft.ldA, ka, kb);
     TM_LAUNCH_CHECK();
