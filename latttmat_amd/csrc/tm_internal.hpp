@@ -292,50 +292,50 @@ size_t reduced_work_bytes(const ReducedPlan &plan, tm_dtype comp);
 // the scalar results (h, out, nrm) are double.
 namespace kry {
 constexpr int NBLK = 2048;  // partial-sum blocks of the reductions
-// out[k] = <V_k, w> for k < j, out[j] = <w,w>; partial buffer of NBLK*(j+1)
+// out[k] = <V_k, w> for k < j, out[j] = <w,w>; partial buffer of NBLK*(j+1).
+// The sweep functions below return the number of partial blocks they wrote;
+// with reduce = false they leave the final sum to one of the reduce_*
+// functions (which also close the Krylov step in the same launch).
 template <typename T>
-void multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *partial,
-              double *out, hipStream_t s);
+unsigned multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *partial,
+                  double *out, hipStream_t s, bool reduce = true);
 // w -= sum_k h[k] V_k (k<j); then out[k] = <V_k, w>, out[j] = <w,w>
 template <typename T>
-void update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
-                 double *partial, double *out, hipStream_t s);
+unsigned update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
+                     double *partial, double *out, hipStream_t s, bool reduce = true);
 // update_dots for h zero outside its last two slots (a Lanczos step): w is
 // updated from V_{j-2}, V_{j-1} only, then all dots as update_dots
 template <typename T>
-void update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
-                         double *partial, double *out, hipStream_t s);
+unsigned update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
+                             double *partial, double *out, hipStream_t s, bool reduce = true);
 // w -= sum_k h[k] V_k; out[0] = <w,w>
 template <typename T>
 void update_norm(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
                  double *partial, double *out, hipStream_t s);
-// if !(w1[0] > w0[0]/2): w -= sum_k h[k] V_k, out[0] = <w,w>; else nothing
-template <typename T>
-void update_norm_cond(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
-                      double *partial, double *out, const double *w0, const double *w1,
-                      hipStream_t s);
-// finish Krylov step j on the device: projected-matrix column j (+ subdiag)
-// into Hd (ld ldh), nrm[0] = beta^2, flags[0] += correction, flags[1] = first
-// breakdown step (must start < 0)
-void step_finish(int j, const double *hb1, const double *hb2, const double *hb3, double *Hd,
-                 int ldh, double *nrm, double *flags, hipStream_t s);
-// close Krylov step j from the two projections: projected-matrix column j
-// (+ subdiagonal) into Hd (ld ldh), nrm[0] = beta^2 (Pythagoras after a
-// correction), flags as for step_finish (also set when the correction removed
-// more than half of the norm again)
-void step_close(int j, const double *hb1, const double *hb2, double *Hd, int ldh, double *nrm,
-                double *flags, hipStream_t s);
+// Sum the nb partial blocks of cnt slots into out, then close Krylov step j
+// from the two projections hb1 (first) and out (second): projected-matrix
+// column j (+ subdiagonal) into Hd (ld ldh), nrm[0] = beta^2 (Pythagoras after
+// a correction), flags[0] += correction, flags[1] = first step whose beta fell
+// below 1e-13 |h| or whose correction removed more than half of the norm
+// again (must start < 0)
+void reduce_step_close(const double *partial, unsigned nb, int cnt, double *out, int j,
+                       const double *hb1, double *Hd, int ldh, double *nrm, double *flags,
+                       hipStream_t s);
+// Sum into out (out[0] = alpha = <v_j, T v_j>), then the Lanczos coefficients
+// of step j: hL[0..j] = (0.., beta_{j-1} = Hd[j, j-1], alpha)
+void reduce_lanczos_coeffs(const double *partial, unsigned nb, int cnt, double *out, int j,
+                           double *Hd, int ldh, double *hL, hipStream_t s);
+// Sum into out (the projections h2 of the three-term residual and its
+// norm^2), then close the Lanczos step: column j = hL + h2, beta^2 =
+// |w1|^2 - |h2|^2; flags as reduce_step_close
+void reduce_step_close_lanczos(const double *partial, unsigned nb, int cnt, double *out, int j,
+                               const double *hL, double *Hd, int ldh, double *nrm, double *flags,
+                               hipStream_t s);
 // w <- (w - c V h2) / sqrt(nrm2[0]), c = 1 iff force or !(w1 > w0 / 2)
 template <typename T>
 void update_normalize(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h2,
                       const double *w0, const double *w1, const double *nrm2, int force,
                       hipStream_t s);
-// Lanczos step of a symmetric operator (j above the cycle's first step):
-// hL[0..j] = (0.., beta_{j-1} = Hd[j, j-1], alpha = hA[0])
-void lanczos_coeffs(int j, const double *hA, const double *Hd, int ldh, double *hL, hipStream_t s);
-// column j = hL + h2, beta^2 = |w1|^2 - |h2|^2 (hb2[j+1] = |w1|^2); flags as step_close
-void step_close_lanczos(int j, const double *hL, const double *hb2, double *Hd, int ldh,
-                        double *nrm, double *flags, hipStream_t s);
 // v = w * (1/sqrt(nrm2[0]))
 template <typename T>
 void normalize(T *w, int64_t n, const double *nrm2, hipStream_t s);

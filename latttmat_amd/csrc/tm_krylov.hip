@@ -355,7 +355,7 @@ __global__ void k_normalize_pk(T *__restrict__ w, int64_t n, const double *__res
 
 // Close a Krylov step in one sweep: w <- (w - c V h2) / beta, with c = 1
 // when the DGKS correction is due (w1 <= w0 / 2) and beta^2 = nrm2[0] as set
-// by k_step_finish (w1 - |h2|^2 after a correction, w1 otherwise).
+// by the step close (w1 - |h2|^2 after a correction, w1 otherwise).
 template <int J, typename T>
 __global__ __launch_bounds__(THREADS) void k_update_normalize(const T *__restrict__ V,
                                                               int64_t ldv, int j,
@@ -435,11 +435,10 @@ __global__ __launch_bounds__(THREADS) void k_update_normalize_pk(const T *__rest
 // 1e-13 |h| or whose correction removed more than half of w1 again (a
 // numerically invariant subspace: the host redoes the cycle from j with
 // explicit norms).
-__global__ void k_step_close(int j, const double *__restrict__ hb1,
-                             const double *__restrict__ hb2, double *__restrict__ Hd, int ldh,
-                             double *__restrict__ nrm, double *__restrict__ flags)
+__device__ void close_body(int j, const double *__restrict__ hb1,
+                           const double *__restrict__ hb2, double *__restrict__ Hd, int ldh,
+                           double *__restrict__ nrm, double *__restrict__ flags)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int nk = j + 1;
   const double w0 = hb1[nk], w1 = hb2[nk];
   const bool re = !(w1 > 0.5 * w0);
@@ -462,22 +461,20 @@ __global__ void k_step_close(int j, const double *__restrict__ hb1,
 // Symmetric operator, step j > first step of the cycle: T v_j couples to
 // v_{j-1}, v_j, v_{j+1} only.  hL[0..j] = (0, .., 0, beta_{j-1}, alpha) with
 // beta_{j-1} = Hd[j, j-1] and alpha = <v_j, T v_j> (hA[0]).
-__global__ void k_lanczos_coeffs(int j, const double *__restrict__ hA,
-                                 const double *__restrict__ Hd, int ldh, double *__restrict__ hL)
+__device__ void coeffs_body(int j, const double *__restrict__ hA, const double *__restrict__ Hd,
+                            int ldh, double *__restrict__ hL)
 {
-  for (int k = threadIdx.x; k <= j; k += blockDim.x)
+  for (int k = 0; k <= j; ++k)
     hL[k] = k == j ? hA[0] : (k == j - 1 ? Hd[j + (int64_t)(j - 1) * ldh] : 0.0);
 }
 
 // Close a Lanczos step: column j = hL + h2 (the full reorthogonalisation pass
-// always runs), beta^2 = |w1|^2 - |h2|^2 (Pythagoras).  flags as k_step_close;
+// always runs), beta^2 = |w1|^2 - |h2|^2 (Pythagoras).  flags as close_body;
 // unsafe when the reorthogonalisation removed more than half of |w1|^2.
-__global__ void k_step_close_lanczos(int j, const double *__restrict__ hL,
-                                     const double *__restrict__ hb2, double *__restrict__ Hd,
-                                     int ldh, double *__restrict__ nrm,
-                                     double *__restrict__ flags)
+__device__ void close_lanczos_body(int j, const double *__restrict__ hL,
+                                   const double *__restrict__ hb2, double *__restrict__ Hd,
+                                   int ldh, double *__restrict__ nrm, double *__restrict__ flags)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int nk = j + 1;
   const double w1 = hb2[nk];
   double hn2 = 0, h22 = 0;
@@ -496,77 +493,31 @@ __global__ void k_step_close_lanczos(int j, const double *__restrict__ hL,
   if ((unsafe || beta <= 1e-13 * sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
 }
 
-// DGKS correction only when the first projection removed more than half of
-// the norm^2 (w1 <= w0 / 2, or not finite); every block reads the same two
-// device values, so the whole grid takes the same branch.
-__device__ __forceinline__ bool need_reorth(const double *w0p, const double *w1p)
+// Partial sums of slots 0..cnt-1 (one wavefront per slot, the same order as
+// k_reduce_partials) into out[], then thread 0 closes the step from them:
+// EPI 1 = Lanczos coefficients (hA = out), 2 = close_body (hb1 = a,
+// hb2 = out), 3 = the Lanczos close (hL = a, hb2 = out).  Saves the launch
+// of a one-thread kernel per reduction.
+template <int EPI>
+__global__ __launch_bounds__(1024) void k_reduce_epi(const double *__restrict__ partial, int nblk,
+                                                     int cnt, double *__restrict__ out, int j,
+                                                     double *__restrict__ a,
+                                                     double *__restrict__ Hd, int ldh,
+                                                     double *__restrict__ nrm,
+                                                     double *__restrict__ flags)
 {
-  return !(w1p[0] > 0.5 * w0p[0]);
-}
-
-template <int J, typename T>
-__global__ __launch_bounds__(THREADS) void k_update_norm_cond(const T *__restrict__ V,
-                                                              int64_t ldv, int j,
-                                                              T *__restrict__ w, int64_t n,
-                                                              const double *__restrict__ h,
-                                                              double *__restrict__ partial,
-                                                              const double *w0p, const double *w1p)
-{
-  if (!need_reorth(w0p, w1p)) return;
-  __shared__ double hk[J];
-  for (int k = threadIdx.x; k < J; k += blockDim.x) hk[k] = k < j ? h[k] : 0.0;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int k = wid; k < cnt; k += nw) {
+    double s = 0;
+    for (int b = lane; b < nblk; b += 64) s += partial[(int64_t)b * cnt + k];
+    s = wave_sum(s);
+    if (lane == 0) out[k] = s;
+  }
   __syncthreads();
-  double acc[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * THREADS) {
-    double wi = w[i];
-#pragma unroll
-    for (int k = 0; k < J; ++k)
-      if (k < j) wi -= hk[k] * (double)V[k * ldv + i];
-    const T ws = (T)wi;
-    w[i] = ws;
-    wi = ws;
-    acc[0] += wi * wi;
-  }
-  block_reduce_write<1>(acc, 1, partial + (int64_t)blockIdx.x);
-}
-
-__global__ __launch_bounds__(64) void k_reduce_partials_cond(const double *__restrict__ partial,
-                                                             int nblk, double *__restrict__ out,
-                                                             const double *w0p, const double *w1p)
-{
-  if (!need_reorth(w0p, w1p)) return;
-  double s = 0;
-  for (int b = threadIdx.x; b < nblk; b += 64) s += partial[b];
-  s = wave_sum(s);
-  if (threadIdx.x == 0) out[0] = s;
-}
-
-// Close Krylov step j on the device (one thread): column j of the projected
-// matrix (ld ldh) = first projection + correction, the subdiagonal
-// beta = ||w||, nrm[0] = beta^2 for the normalisation, flags[0] counts
-// corrections, flags[1] records the first step whose beta fell below
-// 1e-13 ||h|| (an invariant subspace; the host redoes the cycle from there).
-__global__ void k_step_finish(int j, const double *__restrict__ hb1,
-                              const double *__restrict__ hb2, const double *__restrict__ hb3,
-                              double *__restrict__ Hd, int ldh, double *__restrict__ nrm,
-                              double *__restrict__ flags)
-{
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int nk = j + 1;
-  const bool re = need_reorth(hb1 + nk, hb2 + nk);
-  double hn2 = 0;
-  for (int k = 0; k < nk; ++k) {
-    const double hk = hb1[k] + (re ? hb2[k] : 0.0);
-    Hd[k + (int64_t)j * ldh] = hk;
-    hn2 += hk * hk;
-  }
-  const double beta2 = re ? hb3[0] : hb2[nk];
-  const double beta = sqrt(fmax(beta2, 0.0));
-  Hd[nk + (int64_t)j * ldh] = beta;
-  nrm[0] = beta2;
-  if (re) flags[0] += 1.0;
-  if ((beta <= 1e-13 * sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
+  if (threadIdx.x != 0) return;
+  if (EPI == 1) coeffs_body(j, out, Hd, ldh, a);
+  if (EPI == 2) close_body(j, a, out, Hd, ldh, nrm, flags);
+  if (EPI == 3) close_lanczos_body(j, a, out, Hd, ldh, nrm, flags);
 }
 
 template <typename T>
@@ -857,21 +808,25 @@ unsigned sweep_blocks(bool packed, int64_t n)
 }  // namespace
 
 template <typename T>
-void multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *partial,
-              double *out, hipStream_t s)
+unsigned multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *partial,
+                  double *out, hipStream_t s, bool reduce)
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
   const bool pk = packs_ok<T>(V, w, ldv);
   const unsigned nb = sweep_blocks<T>(pk, n);
   if (pk) TM_JDISPATCH(k_multidot_pk, T, j, dim3(nb), V, ldv, j, w, n, partial);
   else TM_JDISPATCH(k_multidot, T, j, dim3(nb), V, ldv, j, w, n, partial);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
-  TM_LAUNCH_CHECK();
+  if (reduce) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1,
+                       out);
+    TM_LAUNCH_CHECK();
+  }
+  return nb;
 }
 
 template <typename T>
-void update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
-                 double *partial, double *out, hipStream_t s)
+unsigned update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
+                     double *partial, double *out, hipStream_t s, bool reduce)
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
   const bool pk = packs_ok<T>(V, w, ldv);
@@ -881,13 +836,17 @@ void update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *
                  nullptr);
   else
     TM_JDISPATCH(k_update_dots, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 1);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
-  TM_LAUNCH_CHECK();
+  if (reduce) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1,
+                       out);
+    TM_LAUNCH_CHECK();
+  }
+  return nb;
 }
 
 template <typename T>
-void update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
-                         double *partial, double *out, hipStream_t s)
+unsigned update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
+                             double *partial, double *out, hipStream_t s, bool reduce)
 {
   TM_CHECK(j >= 1 && j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
   const bool pk = packs_ok<T>(V, w, ldv);
@@ -896,8 +855,12 @@ void update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, const 
     TM_JDISPATCH(k_lanczos_dots_pk, T, j, dim3(nb), V, ldv, j, w, n, h, partial);
   else
     TM_JDISPATCH(k_update_dots, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 1);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1, out);
-  TM_LAUNCH_CHECK();
+  if (reduce) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1,
+                       out);
+    TM_LAUNCH_CHECK();
+  }
+  return nb;
 }
 
 template <typename T>
@@ -916,27 +879,12 @@ void update_norm(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *
   TM_LAUNCH_CHECK();
 }
 
-template <typename T>
-void update_norm_cond(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
-                      double *partial, double *out, const double *w0, const double *w1,
-                      hipStream_t s)
+void reduce_step_close(const double *partial, unsigned nb, int cnt, double *out, int j,
+                       const double *hb1, double *Hd, int ldh, double *nrm, double *flags,
+                       hipStream_t s)
 {
-  TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
-  const bool pk = packs_ok<T>(V, w, ldv);
-  const unsigned nb = sweep_blocks<T>(pk, n);
-  if (pk)
-    TM_JDISPATCH(k_update_dots_pk, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 0, w0, w1);
-  else
-    TM_JDISPATCH(k_update_norm_cond, T, j, dim3(nb), V, ldv, j, w, n, h, partial, w0, w1);
-  hipLaunchKernelGGL(k_reduce_partials_cond, dim3(1), dim3(64), 0, s, partial, (int)nb, out, w0,
-                     w1);
-  TM_LAUNCH_CHECK();
-}
-
-void step_close(int j, const double *hb1, const double *hb2, double *Hd, int ldh, double *nrm,
-                double *flags, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_step_close, dim3(1), dim3(64), 0, s, j, hb1, hb2, Hd, ldh, nrm, flags);
+  hipLaunchKernelGGL(k_reduce_epi<2>, dim3(1), dim3(1024), 0, s, partial, (int)nb, cnt, out, j,
+                     const_cast<double *>(hb1), Hd, ldh, nrm, flags);
   TM_LAUNCH_CHECK();
 }
 
@@ -955,25 +903,20 @@ void update_normalize(const T *V, int64_t ldv, int j, T *w, int64_t n, const dou
     TM_JDISPATCH(k_update_normalize, T, j, dim3(nb), V, ldv, j, w, n, h2, w0, w1, nrm2, force);
 }
 
-void lanczos_coeffs(int j, const double *hA, const double *Hd, int ldh, double *hL, hipStream_t s)
+void reduce_lanczos_coeffs(const double *partial, unsigned nb, int cnt, double *out, int j,
+                           double *Hd, int ldh, double *hL, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_lanczos_coeffs, dim3(1), dim3(64), 0, s, j, hA, Hd, ldh, hL);
+  hipLaunchKernelGGL(k_reduce_epi<1>, dim3(1), dim3(1024), 0, s, partial, (int)nb, cnt, out, j, hL,
+                     Hd, ldh, nullptr, nullptr);
   TM_LAUNCH_CHECK();
 }
 
-void step_close_lanczos(int j, const double *hL, const double *hb2, double *Hd, int ldh,
-                        double *nrm, double *flags, hipStream_t s)
+void reduce_step_close_lanczos(const double *partial, unsigned nb, int cnt, double *out, int j,
+                               const double *hL, double *Hd, int ldh, double *nrm, double *flags,
+                               hipStream_t s)
 {
-  hipLaunchKernelGGL(k_step_close_lanczos, dim3(1), dim3(64), 0, s, j, hL, hb2, Hd, ldh, nrm,
-                     flags);
-  TM_LAUNCH_CHECK();
-}
-
-void step_finish(int j, const double *hb1, const double *hb2, const double *hb3, double *Hd,
-                 int ldh, double *nrm, double *flags, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(64), 0, s, j, hb1, hb2, hb3, Hd, ldh, nrm,
-                     flags);
+  hipLaunchKernelGGL(k_reduce_epi<3>, dim3(1), dim3(1024), 0, s, partial, (int)nb, cnt, out, j,
+                     const_cast<double *>(hL), Hd, ldh, nrm, flags);
   TM_LAUNCH_CHECK();
 }
 
@@ -1119,17 +1062,15 @@ void combine_norms(const double *U, int64_t ld, int nu, const double *Cm, int ld
 }
 
 #define TM_KRY_INST(T)                                                                          \
-  template void multidot<T>(const T *, int64_t, int, const T *, int64_t, double *, double *,   \
-                            hipStream_t);                                                      \
-  template void update_dots<T>(const T *, int64_t, int, T *, int64_t, const double *, double *, \
-                               double *, hipStream_t);                                         \
-  template void update_dots_lanczos<T>(const T *, int64_t, int, T *, int64_t, const double *,   \
-                                       double *, double *, hipStream_t);                       \
+  template unsigned multidot<T>(const T *, int64_t, int, const T *, int64_t, double *,         \
+                                double *, hipStream_t, bool);                                  \
+  template unsigned update_dots<T>(const T *, int64_t, int, T *, int64_t, const double *,       \
+                                   double *, double *, hipStream_t, bool);                     \
+  template unsigned update_dots_lanczos<T>(const T *, int64_t, int, T *, int64_t,               \
+                                           const double *, double *, double *, hipStream_t,    \
+                                           bool);                                              \
   template void update_norm<T>(const T *, int64_t, int, T *, int64_t, const double *, double *, \
                                double *, hipStream_t);                                         \
-  template void update_norm_cond<T>(const T *, int64_t, int, T *, int64_t, const double *,      \
-                                    double *, double *, const double *, const double *,        \
-                                    hipStream_t);                                              \
   template void update_normalize<T>(const T *, int64_t, int, T *, int64_t, const double *,      \
                                     const double *, const double *, const double *, int,       \
                                     hipStream_t);                                              \

@@ -136,7 +136,7 @@ class KrylovSchur final : public KrylovRun {
 
   // The whole cycle without host synchronisation: each step's correction
   // decision, projected-matrix column and norm stay on the device
-  // (kry::step_finish); the host reads the m x m block once at the end.  An
+  // (kry::reduce_step_close*); the host reads the m x m block once at the end.  An
   // invariant subspace found at step jb is redone from jb by the host-driven
   // loop below (V(0..jb) are untouched by the steps after it).
   void expand_device()
@@ -208,19 +208,24 @@ class KrylovSchur final : public KrylovRun {
         // Lanczos step: subtract alpha v_j + beta_{j-1} v_{j-1} inside the
         // reorthogonalisation sweep, then one full correction sweep that also
         // normalises (two sweeps over V instead of three)
-        kry::multidot(V(j), ldv_, 1, w, n_, partial_.as<double>(), hb, s);
-        kry::lanczos_coeffs(j, hb, Hd, ldh, hb + 144, s);
-        kry::update_dots_lanczos(V_.as<T>(), ldv_, nk, w, n_, hb + 144, partial_.as<double>(),
-                                 hb + 72, s);
-        kry::step_close_lanczos(j, hb + 144, hb + 72, Hd, ldh, nrm, flags, s);
+        // (each reduction also closes its part of the step in the same launch)
+        double *part = partial_.as<double>();
+        unsigned nb = kry::multidot(V(j), ldv_, 1, w, n_, part, hb, s, false);
+        kry::reduce_lanczos_coeffs(part, nb, 2, hb, j, Hd, ldh, hb + 144, s);
+        nb = kry::update_dots_lanczos(V_.as<T>(), ldv_, nk, w, n_, hb + 144, part, hb + 72, s,
+                                      false);
+        kry::reduce_step_close_lanczos(part, nb, nk + 1, hb + 72, j, hb + 144, Hd, ldh, nrm, flags,
+                                       s);
         kry::update_normalize(V_.as<T>(), ldv_, nk, w, n_, hb + 72, hb, hb, nrm, 1, s);
         continue;
       }
-      kry::multidot(V_.as<T>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
-      kry::update_dots(V_.as<T>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 72, s);
+      double *part = partial_.as<double>();
+      kry::multidot(V_.as<T>(), ldv_, nk, w, n_, part, hb, s);
+      const unsigned nb = kry::update_dots(V_.as<T>(), ldv_, nk, w, n_, hb, part, hb + 72, s,
+                                           false);
       // DGKS correction and normalisation in one sweep, the norm from the
       // projections (an unsafe correction sends the step to the host loop)
-      kry::step_close(j, hb, hb + 72, Hd, ldh, nrm, flags, s);
+      kry::reduce_step_close(part, nb, nk + 1, hb + 72, j, hb, Hd, ldh, nrm, flags, s);
       kry::update_normalize(V_.as<T>(), ldv_, nk, w, n_, hb + 72, hb + nk, hb + 72 + nk, nrm,
                             0, s);
     }

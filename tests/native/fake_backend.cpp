@@ -49,16 +49,19 @@ void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind, hipStrea
 }
 namespace kry {
 template <typename T>
-void multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *, double *out, hipStream_t)
+unsigned multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *, double *out,
+                  hipStream_t, bool)
 {
   for (int k = 0; k <= j; ++k) out[k] = 0;
   for (int64_t i = 0; i < n; ++i) {
     for (int k = 0; k < j; ++k) out[k] += (double)V[k * ldv + i] * (double)w[i];
     out[j] += (double)w[i] * (double)w[i];
   }
+  return 1;
 }
 template <typename T>
-void update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h, double *, double *out, hipStream_t)
+unsigned update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h, double *,
+                     double *out, hipStream_t, bool)
 {
   std::vector<double> hh(h, h + j);
   for (int64_t i = 0; i < n; ++i) {
@@ -66,13 +69,13 @@ void update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *
     for (int k = 0; k < j; ++k) wi -= hh[k] * (double)V[k * ldv + i];
     w[i] = (T)wi;
   }
-  multidot(V, ldv, j, w, n, nullptr, out, nullptr);
+  return multidot(V, ldv, j, w, n, nullptr, out, nullptr, true);
 }
 template <typename T>
-void update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h, double *p,
-                         double *out, hipStream_t st)
+unsigned update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
+                             double *p, double *out, hipStream_t st, bool r)
 {
-  update_dots(V, ldv, j, w, n, h, p, out, st);
+  return update_dots(V, ldv, j, w, n, h, p, out, st, r);
 }
 template <typename T>
 void update_norm(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h, double *, double *out, hipStream_t)
@@ -86,34 +89,12 @@ void update_norm(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *
   out[0] = 0;
   for (int64_t i = 0; i < n; ++i) out[0] += (double)w[i] * (double)w[i];
 }
-template <typename T>
-void update_norm_cond(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
-                      double *, double *out, const double *w0, const double *w1, hipStream_t)
+// The sweeps above already leave the full sums in out (nothing to reduce):
+// these run only the step-closing arithmetic of the device kernels.
+void reduce_step_close(const double *, unsigned, int, double *out, int j, const double *hb1,
+                       double *Hd, int ldh, double *nrm, double *flags, hipStream_t)
 {
-  if (w1[0] > 0.5 * w0[0]) return;
-  update_norm(V, ldv, j, w, n, h, nullptr, out, nullptr);
-}
-void step_finish(int j, const double *hb1, const double *hb2, const double *hb3, double *Hd,
-                 int ldh, double *nrm, double *flags, hipStream_t)
-{
-  const int nk = j + 1;
-  const bool re = !(hb2[nk] > 0.5 * hb1[nk]);
-  double hn2 = 0;
-  for (int k = 0; k < nk; ++k) {
-    const double hk = hb1[k] + (re ? hb2[k] : 0.0);
-    Hd[k + (int64_t)j * ldh] = hk;
-    hn2 += hk * hk;
-  }
-  const double beta2 = re ? hb3[0] : hb2[nk];
-  const double beta = std::sqrt(std::fmax(beta2, 0.0));
-  Hd[nk + (int64_t)j * ldh] = beta;
-  nrm[0] = beta2;
-  if (re) flags[0] += 1.0;
-  if ((beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
-}
-void step_close(int j, const double *hb1, const double *hb2, double *Hd, int ldh, double *nrm,
-                double *flags, hipStream_t)
-{
+  const double *hb2 = out;
   const int nk = j + 1;
   const double w0 = hb1[nk], w1 = hb2[nk];
   const bool re = !(w1 > 0.5 * w0);
@@ -132,13 +113,15 @@ void step_close(int j, const double *hb1, const double *hb2, double *Hd, int ldh
   const bool unsafe = re && !(h22 <= 0.5 * w1);
   if ((unsafe || beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
 }
-void lanczos_coeffs(int j, const double *hA, const double *Hd, int ldh, double *hL, hipStream_t)
+void reduce_lanczos_coeffs(const double *, unsigned, int, double *out, int j, double *Hd, int ldh,
+                           double *hL, hipStream_t)
 {
-  for (int k = 0; k <= j; ++k) hL[k] = k == j ? hA[0] : (k == j - 1 ? Hd[j + (int64_t)(j - 1) * ldh] : 0.0);
+  for (int k = 0; k <= j; ++k) hL[k] = k == j ? out[0] : (k == j - 1 ? Hd[j + (int64_t)(j - 1) * ldh] : 0.0);
 }
-void step_close_lanczos(int j, const double *hL, const double *hb2, double *Hd, int ldh, double *nrm,
-                        double *flags, hipStream_t)
+void reduce_step_close_lanczos(const double *, unsigned, int, double *out, int j, const double *hL,
+                               double *Hd, int ldh, double *nrm, double *flags, hipStream_t)
 {
+  const double *hb2 = out;
   const int nk = j + 1;
   const double w1 = hb2[nk];
   double hn2 = 0, h22 = 0;
@@ -230,17 +213,15 @@ void combine_norms(const double *U, int64_t ld, int nu, const double *Cm, int ld
   }
 }
 #define FAKE_KRY_INST(T)                                                                        \
-  template void multidot<T>(const T *, int64_t, int, const T *, int64_t, double *, double *,   \
-                            hipStream_t);                                                      \
-  template void update_dots<T>(const T *, int64_t, int, T *, int64_t, const double *, double *, \
-                               double *, hipStream_t);                                         \
+  template unsigned multidot<T>(const T *, int64_t, int, const T *, int64_t, double *,         \
+                                double *, hipStream_t, bool);                                  \
+  template unsigned update_dots<T>(const T *, int64_t, int, T *, int64_t, const double *,       \
+                                   double *, double *, hipStream_t, bool);                     \
   template void update_norm<T>(const T *, int64_t, int, T *, int64_t, const double *, double *, \
                                double *, hipStream_t);                                         \
-  template void update_dots_lanczos<T>(const T *, int64_t, int, T *, int64_t, const double *,   \
-                                       double *, double *, hipStream_t);                       \
-  template void update_norm_cond<T>(const T *, int64_t, int, T *, int64_t, const double *,      \
-                                    double *, double *, const double *, const double *,        \
-                                    hipStream_t);                                              \
+  template unsigned update_dots_lanczos<T>(const T *, int64_t, int, T *, int64_t,               \
+                                           const double *, double *, double *, hipStream_t,    \
+                                           bool);                                              \
   template void update_normalize<T>(const T *, int64_t, int, T *, int64_t, const double *,      \
                                     const double *, const double *, const double *, int,       \
                                     hipStream_t);                                              \
