@@ -142,9 +142,11 @@ __device__ __forceinline__ C diag_value(const DiagSpec &g, const C *tab, uint64_
 __device__ __forceinline__ int sw(int e) { return e ^ ((e >> 5) & 31); }
 
 // One radix-2^R Kronecker stage group on R register values: v[i] indexed by
-// R bits, K = [[a, b], [b, a]] applied on each of the R bits.
+// R bits, K' = [[1, r], [r, 1]] applied on each of the R bits (one FMA per
+// output).  K = [[a, b], [b, a]] = a K' with r = b / a: the caller applies
+// the factor a^(bits transformed) once per element.
 template <int R, typename C>
-__device__ __forceinline__ void kron_regs(C *v, C a, C b)
+__device__ __forceinline__ void kron_regs(C *v, C r)
 {
 #pragma unroll
   for (int j = 0; j < R; ++j) {
@@ -153,18 +155,19 @@ __device__ __forceinline__ void kron_regs(C *v, C a, C b)
     for (int i = 0; i < (1 << R); ++i) {
       if (!(i & m)) {
         C x0 = v[i], x1 = v[i | m];
-        v[i] = a * x0 + b * x1;
-        v[i | m] = b * x0 + a * x1;
+        v[i] = x0 + r * x1;
+        v[i | m] = x1 + r * x0;
       }
     }
   }
 }
 
-// Full Kronecker transform of an LDS array of 2^nbits elements (bits
-// [bit0, nbits) still to do), using every thread of the block.  Element
-// index p is stored at lds[sw(p)].  Ends with a block barrier.
+// Transform of an LDS array of 2^nbits elements by K' = [[1, r], [r, 1]] on
+// bits [bit0, nbits), using every thread of the block (the caller scales by
+// a^(nbits - bit0) for K = a K').  Element index p is stored at lds[sw(p)].
+// Ends with a block barrier.
 template <int R, typename C>
-__device__ __forceinline__ void lds_kron_group(C *lds, int nbits, int bg, C a, C b)
+__device__ __forceinline__ void lds_kron_group(C *lds, int nbits, int bg, C r)
 {
   const int nitems = 1 << (nbits - R);
   for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
@@ -174,7 +177,7 @@ __device__ __forceinline__ void lds_kron_group(C *lds, int nbits, int bg, C a, C
     C v[1 << R];
 #pragma unroll
     for (int i = 0; i < (1 << R); ++i) v[i] = lds[sw(base | (i << bg))];
-    kron_regs<R>(v, a, b);
+    kron_regs<R>(v, r);
 #pragma unroll
     for (int i = 0; i < (1 << R); ++i) lds[sw(base | (i << bg))] = v[i];
   }
@@ -183,18 +186,34 @@ __device__ __forceinline__ void lds_kron_group(C *lds, int nbits, int bg, C a, C
 // Inlined into every caller: as a called function its registers would be
 // outside the caller's register budget (and cost the call ABI).
 template <typename C>
-__device__ __forceinline__ void lds_kron(C *lds, int nbits, int bit0, C a, C b)
+__device__ __forceinline__ void lds_kron(C *lds, int nbits, int bit0, C r)
 {
   for (int bg = bit0; bg < nbits; bg += 4) {
-    const int r = nbits - bg < 4 ? nbits - bg : 4;
-    switch (r) {
-      case 1: lds_kron_group<1>(lds, nbits, bg, a, b); break;
-      case 2: lds_kron_group<2>(lds, nbits, bg, a, b); break;
-      case 3: lds_kron_group<3>(lds, nbits, bg, a, b); break;
-      default: lds_kron_group<4>(lds, nbits, bg, a, b); break;
+    const int rr = nbits - bg < 4 ? nbits - bg : 4;
+    switch (rr) {
+      case 1: lds_kron_group<1>(lds, nbits, bg, r); break;
+      case 2: lds_kron_group<2>(lds, nbits, bg, r); break;
+      case 3: lds_kron_group<3>(lds, nbits, bg, r); break;
+      default: lds_kron_group<4>(lds, nbits, bg, r); break;
     }
     __syncthreads();
   }
+}
+
+// r = b / a and f = a^nb (with derivatives for dual arithmetic) of the
+// butterfly K = [[a, b], [b, a]] = a [[1, r], [r, 1]] over nb bits.
+template <typename C>
+struct KronCoef {
+  C r, f;
+};
+template <typename C>
+inline KronCoef<C> kron_coef(double a, double da, double b, double db, int nb)
+{
+  const double rv = b / a, rd = (db * a - b * da) / (a * a);
+  double fv = 1.0;
+  for (int i = 0; i < nb; ++i) fv *= a;
+  const double fd = nb > 0 ? nb * (fv / a) * da : 0.0;
+  return {mk<C>(rv, rd), mk<C>(fv, fd)};
 }
 
 // XCD-aware block remap: hardware dispatches workgroup i to XCD i % 8.

@@ -23,7 +23,7 @@ enum { F_PRO = 1, F_EPI = 2, F_SIN = 4, F_SOUT = 8 };
 template <typename C, typename TI, typename TO>
 __global__ __launch_bounds__(KRON_THREADS) void k_kron_pass(
     const TI *__restrict__ in, TO *__restrict__ out, int b0, int nb, int logW, int flags,
-    DiagSpec dg, const C *__restrict__ tab, C ka, C kb, const double *__restrict__ scale)
+    DiagSpec dg, const C *__restrict__ tab, C r, C f, const double *__restrict__ scale)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tileLog = nb + logW;
@@ -72,19 +72,21 @@ __global__ __launch_bounds__(KRON_THREADS) void k_kron_pass(
   // nb stages: LDS index of element (p, w) is p * W + w, so bit j of p is
   // bit (j + logW) of the LDS index; the w bits ride along as lo bits.
   for (int bg = 0; bg < nb; bg += 4) {
-    const int r = nb - bg < 4 ? nb - bg : 4;
-    switch (r) {
-      case 1: dev::lds_kron_group<1>(lds, tileLog, bg + logW, ka, kb); break;
-      case 2: dev::lds_kron_group<2>(lds, tileLog, bg + logW, ka, kb); break;
-      case 3: dev::lds_kron_group<3>(lds, tileLog, bg + logW, ka, kb); break;
-      default: dev::lds_kron_group<4>(lds, tileLog, bg + logW, ka, kb); break;
+    const int rr = nb - bg < 4 ? nb - bg : 4;
+    switch (rr) {
+      case 1: dev::lds_kron_group<1>(lds, tileLog, bg + logW, r); break;
+      case 2: dev::lds_kron_group<2>(lds, tileLog, bg + logW, r); break;
+      case 3: dev::lds_kron_group<3>(lds, tileLog, bg + logW, r); break;
+      default: dev::lds_kron_group<4>(lds, tileLog, bg + logW, r); break;
     }
     __syncthreads();
   }
 
+  // f = a^nb of K = a [[1, r], [r, 1]] over this pass's bits
+  const C fs = f * sout;
   for (int e = threadIdx.x; e < T; e += blockDim.x) {
     const uint64_t g = base | ((uint64_t)(e >> logW) << b0) | (uint64_t)(e & (W - 1));
-    C v = lds[dev::sw(e)] * sout;
+    C v = lds[dev::sw(e)] * fs;
     if (flags & F_EPI) v *= dev::diag_value<C>(dg, stab, g);
     out[g] = (TO)v;
   }
@@ -130,8 +132,11 @@ std::vector<Pass> plan_passes(int L, int es, int tl)
 
 template <typename C, typename TI, typename TO>
 void run_pass(const Pass &p, int L, const void *in, void *out, int flags, const KronPlan &plan,
-              const C *tab, C ka, C kb, const double *scale, hipStream_t s)
+              const C *tab, double dka, double dkb, const double *scale, hipStream_t s)
 {
+  // K = ka [[1, r], [r, 1]]: the pass applies r in its butterflies and ka^nb
+  // on store (derivatives dka, dkb for dual arithmetic)
+  const auto kc = dev::kron_coef<C>(plan.ka, dka, plan.kb, dkb, p.nb);
   const uint64_t ntile = 1ull << (L - p.nb - p.logW);
   dim3 grid;
   if (ntile <= 65535) grid = dim3((unsigned)ntile, 1, 1);
@@ -139,7 +144,7 @@ void run_pass(const Pass &p, int L, const void *in, void *out, int flags, const 
   const size_t shm = ((size_t(1) << (p.nb + p.logW)) + 192) * sizeof(C);
   hipLaunchKernelGGL((k_kron_pass<C, TI, TO>), grid, dim3(KRON_THREADS), shm, s,
                      static_cast<const TI *>(in), static_cast<TO *>(out), p.b0, p.nb, p.logW,
-                     flags, plan.diag, tab, ka, kb, scale);
+                     flags, plan.diag, tab, kc.r, kc.f, scale);
   TM_LAUNCH_CHECK();
 }
 
@@ -150,7 +155,6 @@ void kron_typed(const KronPlan &plan, const void *in, void *out, tm_dtype io_in,
   const int L = plan.L;
   auto ps = plan_passes(L, (int)sizeof(C), tl);
   const int np = (int)ps.size();
-  const C ka = (C)plan.ka, kb = (C)plan.kb;
   // Intermediate buffer: the output itself when it has the compute type,
   // else the workspace.
   const bool out_is_c = (sizeof(C) == 8) == (io_out == TM_F64);
@@ -167,18 +171,18 @@ void kron_typed(const KronPlan &plan, const void *in, void *out, tm_dtype io_in,
     const bool dst_io = (i == np - 1);
     const double *sc = (src_io || dst_io) ? scale : nullptr;
     if (src_io && dst_io) {
-      if (io_in == TM_F64 && io_out == TM_F64) run_pass<C, double, double>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
-      else if (io_in == TM_F64) run_pass<C, double, float>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
-      else if (io_out == TM_F64) run_pass<C, float, double>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
-      else run_pass<C, float, float>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
+      if (io_in == TM_F64 && io_out == TM_F64) run_pass<C, double, double>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
+      else if (io_in == TM_F64) run_pass<C, double, float>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
+      else if (io_out == TM_F64) run_pass<C, float, double>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
+      else run_pass<C, float, float>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
     } else if (src_io) {
-      if (io_in == TM_F64) run_pass<C, double, C>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
-      else run_pass<C, float, C>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
+      if (io_in == TM_F64) run_pass<C, double, C>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
+      else run_pass<C, float, C>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
     } else if (dst_io) {
-      if (io_out == TM_F64) run_pass<C, C, double>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
-      else run_pass<C, C, float>(ps[i], L, src, dst, flags, plan, tab, ka, kb, sc, s);
+      if (io_out == TM_F64) run_pass<C, C, double>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
+      else run_pass<C, C, float>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
     } else {
-      run_pass<C, C, C>(ps[i], L, src, dst, flags, plan, tab, ka, kb, nullptr, s);
+      run_pass<C, C, C>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, nullptr, s);
     }
   }
 }
@@ -195,7 +199,6 @@ void launch_kron_dual(const KronPlan &plan, const double *in, void *out, const v
 {
   using dev::Dual;
   const int L = plan.L;
-  const Dual ka(plan.ka, dka), kb(plan.kb, dkb);
   const Dual *tab = static_cast<const Dual *>(tab_dual);
   Dual *o = static_cast<Dual *>(out);
   auto run = [&](int tl) {
@@ -205,8 +208,8 @@ void launch_kron_dual(const KronPlan &plan, const double *in, void *out, const v
       int flags = 0;
       if (i == 0 && plan.prologue) flags |= F_PRO;
       if (i == np - 1 && plan.epilogue) flags |= F_EPI;
-      if (i == 0) run_pass<Dual, double, Dual>(ps[i], L, in, o, flags, plan, tab, ka, kb, nullptr, s);
-      else run_pass<Dual, Dual, Dual>(ps[i], L, o, o, flags, plan, tab, ka, kb, nullptr, s);
+      if (i == 0) run_pass<Dual, double, Dual>(ps[i], L, in, o, flags, plan, tab, dka, dkb, nullptr, s);
+      else run_pass<Dual, Dual, Dual>(ps[i], L, o, o, flags, plan, tab, dka, dkb, nullptr, s);
     }
   };
   const int es = (int)sizeof(Dual);

@@ -561,17 +561,17 @@ __global__ void k_gather(const C *__restrict__ u, const uint64_t *__restrict__ r
 }
 
 // Plain in-LDS Kronecker transform over the full space for small L (generic
-// path): one workgroup per vector.
+// path): one workgroup per vector; K = a [[1, r], [r, 1]], f = a^L.
 template <typename C>
-__global__ __launch_bounds__(1024) void k_kron_small(C *u, int L, C ka, C kb)
+__global__ __launch_bounds__(1024) void k_kron_small(C *u, int L, C r, C f)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
   const int n = 1 << L;
   for (int i = threadIdx.x; i < n; i += blockDim.x) lds[dev::sw(i)] = u[i];
   __syncthreads();
-  dev::lds_kron(lds, L, 0, ka, kb);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) u[i] = lds[dev::sw(i)];
+  dev::lds_kron(lds, L, 0, r);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) u[i] = lds[dev::sw(i)] * f;
 }
 
 constexpr int FUSED_THREADS = 1024;
@@ -582,8 +582,8 @@ constexpr int FUSED_THREADS = 1024;
 template <typename C>
 __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
     const uint32_t *__restrict__ cmap, const uint16_t *__restrict__ nkl, const C *__restrict__ xt,
-    C *__restrict__ Ap, const uint16_t *__restrict__ slotO, int n, int nNk, int64_t ldA, C ka,
-    C kb)
+    C *__restrict__ Ap, const uint16_t *__restrict__ slotO, int n, int nNk, int64_t ldA, C kr,
+    C kf)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
@@ -614,8 +614,9 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
       if (j0 + u * (int)blockDim.x < nh) lds[dev::sw(O[u])] = v[u];
   }
   __syncthreads();
-  dev::lds_kron(lds, n, 0, ka, kb);
-  // output: UNR slot-table loads in flight per thread before the LDS reads
+  dev::lds_kron(lds, n, 0, kr);
+  // output (scaled by kf = a^n): UNR slot-table loads in flight per thread
+  // before the LDS reads
   C *out = Ap + (int64_t)c * ldA;
   const int ns = nNk * 16;
   constexpr int UNR = 8;
@@ -629,7 +630,7 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       const int t = t0 + u * blockDim.x;
-      if (t < ns) out[t] = O[u] != 0xFFFF ? lds[dev::sw(O[u])] : (C)0;
+      if (t < ns) out[t] = O[u] != 0xFFFF ? lds[dev::sw(O[u])] * kf : (C)0;
     }
   }
 }
@@ -641,7 +642,7 @@ template <typename C, typename TO>
 __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
     const C *__restrict__ Ap, const uint16_t *__restrict__ NR, const uint8_t *__restrict__ period,
     int nNk, const int64_t *__restrict__ ostart, const uint16_t *__restrict__ Elist, const C *__restrict__ dt, TO *__restrict__ y, int n,
-    int64_t ldA, C ka, C kb, const double *__restrict__ scale)
+    int64_t ldA, C kr, C kf, const double *__restrict__ scale)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
@@ -672,8 +673,9 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
     }
   }
   __syncthreads();
-  dev::lds_kron(lds, n, 0, ka, kb);
-  const C inv = scale ? (C)(1.0 / *scale) : (C)1;
+  dev::lds_kron(lds, n, 0, kr);
+  // 1 / scale of the input, and kf = a^n of this pass's butterflies
+  const C inv = (scale ? (C)(1.0 / *scale) : (C)1) * kf;
   // internal numbering: the orbits of this row are [ostart[r], ostart[r+1]);
   // UNR index and weight loads in flight per thread before the LDS reads
   const int64_t i0 = ostart[r], i1 = ostart[r + 1];
@@ -710,7 +712,8 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
     hipLaunchKernelGGL((k_xtilde<C, float>), dim3(gM), dim3(256), 0, s,
                        static_cast<const float *>(in), tvec, xt, M, scale);
   TM_LAUNCH_CHECK();
-  const C ka = (C)p.ka, kb = (C)p.kb;
+  // K = ka [[1, r], [r, 1]]: butterflies with r, the factor ka^(bits) on output
+  const C kr = (C)(p.kb / p.ka);
   if (p.ft) {
     const FusedTables &ft = *p.ft;
     const int n = ft.n;
@@ -719,19 +722,22 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
     const unsigned g1 = 8u * (unsigned)((ft.nNk + 7) / 8);
     hipLaunchKernelGGL((k_fused_pass1<C>), dim3(g1), dim3(FUSED_THREADS), shm, s,
                        ft.cmap.as<uint32_t>(), ft.nkl.as<uint16_t>(), xt, Ap,
-                       ft.slotO.as<uint16_t>(), n, ft.nNk, ft.ldA, ka, kb);
+                       ft.slotO.as<uint16_t>(), n, ft.nNk, ft.ldA, kr,
+                       dev::kron_coef<C>(p.ka, 0, p.kb, 0, n).f);
     TM_LAUNCH_CHECK();
     const unsigned g2 = 8u * (unsigned)((ft.nNk + 7) / 8);
     if (io == TM_F64)
       hipLaunchKernelGGL((k_fused_pass2<C, double>), dim3(g2), dim3(FUSED_THREADS), shm, s,
                          Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(), ft.nNk,
                          ft.ostart.as<int64_t>(), ft.Elist.as<uint16_t>(), dt,
-                         static_cast<double *>(out), n, ft.ldA, ka, kb, scale);
+                         static_cast<double *>(out), n, ft.ldA, kr,
+                         dev::kron_coef<C>(p.ka, 0, p.kb, 0, n).f, scale);
     else
       hipLaunchKernelGGL((k_fused_pass2<C, float>), dim3(g2), dim3(FUSED_THREADS), shm, s,
                          Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(), ft.nNk,
                          ft.ostart.as<int64_t>(), ft.Elist.as<uint16_t>(), dt,
-                         static_cast<float *>(out), n, ft.ldA, ka, kb, scale);
+                         static_cast<float *>(out), n, ft.ldA, kr,
+                         dev::kron_coef<C>(p.ka, 0, p.kb, 0, n).f, scale);
     TM_LAUNCH_CHECK();
     return;
   }
@@ -743,7 +749,7 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
   const int smallL = sizeof(C) == 8 ? 13 : 14;
   if (p.L <= smallL) {
     hipLaunchKernelGGL((k_kron_small<C>), dim3(1), dim3(1024), (size_t(1) << p.L) * sizeof(C),
-                       s, u, p.L, ka, kb);
+                       s, u, p.L, kr, dev::kron_coef<C>(p.ka, 0, p.kb, 0, p.L).f);
     TM_LAUNCH_CHECK();
   } else {
     KronPlan kp{};
