@@ -251,7 +251,8 @@ struct FusedTables {
   DevBuf period;         // uint8[nNk]    rotation period of each representative
   DevBuf nkw;            // uint16[2^n]   necklace index of every n-bit word
   DevBuf rotw;           // uint8[2^n]    word = rotl(NR[nkw], rotw)
-  DevBuf slotO;          // uint16[nNk*16] word stored in (necklace, slot), 0xFFFF unused
+  DevBuf slotO;          // uint16[nNk*16] O word of (row necklace r, slot k):
+                         //   rotl_{(n-k) mod p_r} NR[r] for k < n, 0xFFFF unused
   DevBuf nkl;            // uint16[2^n]   n-bit words necklace-major (class, rotation)
   DevBuf cmap;           // uint32[nNk << n] orbit of the state (E = NR[c], O = nkl[j])
   DevBuf ostart;         // int64[nNk+1]  internal-index range of the orbits whose

@@ -450,9 +450,13 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   order.reserve(nw);
   for (int q = 0; q < ft.nNk; ++q)
     for (int k = 0; k < per[q]; ++k) order.push_back((uint16_t)rotl(NR[q], k));
+  // slot k of row r holds the O word that pass 2 needs for the rotation
+  // E = rotl_k(NR[c]) of any column: A(NR[c], rotr_k NR[r]) = A(NR[c],
+  // rotl_{(n-k) mod p_r} NR[r]), k < n (k >= p_c is never read)
   std::vector<uint16_t> slot((size_t)ft.nNk * 16, 0xFFFF);
   for (int r = 0; r < ft.nNk; ++r)
-    for (int j = 0; j < per[r]; ++j) slot[(size_t)r * 16 + j] = (uint16_t)rotl(NR[r], j);
+    for (int k = 0; k < n; ++k)
+      slot[(size_t)r * 16 + k] = (uint16_t)rotl(NR[r], (n - k) % per[r]);
   // evaluation point of every orbit: rotate the representative so that its
   // odd-bit part becomes a necklace representative
   std::vector<uint64_t> hreps(M);
@@ -650,10 +654,9 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
   // same cache lines): keep them on one XCD
   const int r = (int)dev::xcd_remap(blockIdx.x, nNk);
   if (r < 0) return;
-  const int pr = period[r];
   const uint32_t wmask = (1u << n) - 1;
-  // one necklace column per thread: a 16-slot group holds A(NR[c], rotations
-  // of O_r); every rotation E = rotl_k NR[c] reads slot (n - k) mod p_r
+  // one necklace column per thread: slot k of the 16-slot group holds the
+  // value for the rotation E = rotl_k NR[c] (the slot table is ordered so)
   const C *grp0 = Ap + (int64_t)r * 16;
   for (int c = threadIdx.x; c < nNk; c += blockDim.x) {
     const C *g = grp0 + (int64_t)c * ldA;
@@ -662,14 +665,12 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
     for (int j = 0; j < 16; ++j) v[j] = g[j];
     const uint32_t w = NR[c];
     const int pc = period[c];
-    for (int k = 0; k < pc; ++k) {
-      const uint32_t E = k ? (((w << k) & wmask) | (w >> (n - k))) : w;
-      const int j = (n - k) % pr;
-      C val = v[0];
+    uint32_t E = w;
 #pragma unroll
-      for (int t = 1; t < 16; ++t)
-        if (t == j) val = v[t];
-      lds[dev::sw(E)] = val;
+    for (int k = 0; k < 16; ++k) {
+      if (k >= pc) break;
+      lds[dev::sw(E)] = v[k];
+      E = ((E << 1) & wmask) | (E >> (n - 1));
     }
   }
   __syncthreads();
