@@ -139,7 +139,7 @@ __device__ __forceinline__ C diag_value(const DiagSpec &g, const C *tab, uint64_
 // A thread that owns 16 consecutive elements (or 16 elements strided by 16)
 // then touches distinct banks across its wavefront.  Involution, identity
 // for index < 32.
-__device__ __forceinline__ int sw(int e) { return e ^ ((e >> 5) & 31); }
+__host__ __device__ __forceinline__ int sw(int e) { return e ^ ((e >> 5) & 31); }
 
 // One radix-2^R Kronecker stage group on R register values: v[i] indexed by
 // R bits, K' = [[1, r], [r, 1]] applied on each of the R bits (one FMA per
@@ -183,21 +183,49 @@ __device__ __forceinline__ void lds_kron_group(C *lds, int nbits, int bg, C r)
   }
 }
 
+// lds_kron_group with the bit group BG known at compile time: sw is linear
+// over XOR, so every address is sw(base) ^ sw(i << BG) -- one XOR with an
+// immediate per element instead of recomputing the swizzle.
+template <int R, int BG, typename C>
+__device__ __forceinline__ void lds_kron_group_c(C *lds, int nbits, C r)
+{
+  const int nitems = 1 << (nbits - R);
+  for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
+    const int lo = q & ((1 << BG) - 1);
+    const int hi = q >> BG;
+    const int sb = sw((hi << (BG + R)) | lo);
+    C v[1 << R];
+#pragma unroll
+    for (int i = 0; i < (1 << R); ++i) v[i] = lds[sb ^ sw(i << BG)];
+    kron_regs<R>(v, r);
+#pragma unroll
+    for (int i = 0; i < (1 << R); ++i) lds[sb ^ sw(i << BG)] = v[i];
+  }
+}
+
+template <int BG, typename C>
+__device__ __forceinline__ void lds_kron_step(C *lds, int nbits, C r)
+{
+  if (nbits <= BG) return;
+  switch (nbits - BG) {
+    case 1: lds_kron_group_c<1, BG>(lds, nbits, r); break;
+    case 2: lds_kron_group_c<2, BG>(lds, nbits, r); break;
+    case 3: lds_kron_group_c<3, BG>(lds, nbits, r); break;
+    default: lds_kron_group_c<4, BG>(lds, nbits, r); break;
+  }
+  __syncthreads();
+}
+
+// Transform of all nbits <= 16 bits of an LDS array (see lds_kron_group).
 // Inlined into every caller: as a called function its registers would be
 // outside the caller's register budget (and cost the call ABI).
 template <typename C>
-__device__ __forceinline__ void lds_kron(C *lds, int nbits, int bit0, C r)
+__device__ __forceinline__ void lds_kron(C *lds, int nbits, C r)
 {
-  for (int bg = bit0; bg < nbits; bg += 4) {
-    const int rr = nbits - bg < 4 ? nbits - bg : 4;
-    switch (rr) {
-      case 1: lds_kron_group<1>(lds, nbits, bg, r); break;
-      case 2: lds_kron_group<2>(lds, nbits, bg, r); break;
-      case 3: lds_kron_group<3>(lds, nbits, bg, r); break;
-      default: lds_kron_group<4>(lds, nbits, bg, r); break;
-    }
-    __syncthreads();
-  }
+  lds_kron_step<0>(lds, nbits, r);
+  lds_kron_step<4>(lds, nbits, r);
+  lds_kron_step<8>(lds, nbits, r);
+  lds_kron_step<12>(lds, nbits, r);
 }
 
 // r = b / a and f = a^nb (with derivatives for dual arithmetic) of the

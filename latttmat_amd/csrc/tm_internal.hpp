@@ -251,14 +251,16 @@ struct FusedTables {
   DevBuf period;         // uint8[nNk]    rotation period of each representative
   DevBuf nkw;            // uint16[2^n]   necklace index of every n-bit word
   DevBuf rotw;           // uint8[2^n]    word = rotl(NR[nkw], rotw)
-  DevBuf slotO;          // uint16[nNk*16] O word of (row necklace r, slot k):
-                         //   rotl_{(n-k) mod p_r} NR[r] for k < n, 0xFFFF unused
-  DevBuf nkl;            // uint16[2^n]   n-bit words necklace-major (class, rotation)
+  DevBuf slotO;          // uint16[nNk*16] LDS index sw(O) of (row necklace r, slot k),
+                         //   O = rotl_{(n-k) mod p_r} NR[r] for k < n; 0xFFFF unused
+  DevBuf nkl;            // uint16[2^n]   LDS index sw(O) of the n-bit words O in
+                         //               necklace-major order (class, rotation)
   DevBuf cmap;           // uint32[nNk << n] orbit of the state (E = NR[c], O = nkl[j])
   DevBuf ostart;         // int64[nNk+1]  internal-index range of the orbits whose
                          //               evaluation point has O = NR[r]
   DevBuf olist;          // uint32[M]     internal index -> orbit (ascending-rep order)
-  DevBuf Elist;          // uint16[M]     E part of each orbit's evaluation point
+  DevBuf Elist;          // uint16[M]     LDS index sw(E) of the E part of each orbit's
+                         //               evaluation point
   std::vector<uint32_t> perm;  // host copy of olist
 };
 // The fused operator numbers orbits internally by row (olist order), so

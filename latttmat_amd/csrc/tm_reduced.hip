@@ -499,9 +499,13 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   up(ft.period, per.data(), per.size());
   up(ft.nkw, nk.data(), nk.size() * 2);
   up(ft.rotw, rot.data(), rot.size());
+  // the pass kernels use these words only as LDS indices: store them swizzled
+  for (auto &w : slot)
+    if (w != 0xFFFF) w = (uint16_t)dev::sw(w);
   up(ft.slotO, slot.data(), slot.size() * 2);
   up(ft.ostart, cnt.data(), cnt.size() * 8);
   up(ft.olist, olist.data(), olist.size() * 4);
+  for (auto &w : elist) w = (uint16_t)dev::sw(w);
   up(ft.Elist, elist.data(), elist.size() * 2);
   up(ft.nkl, order.data(), order.size() * 2);
   // internal orbit numbering = position in olist (rows contiguous)
@@ -526,6 +530,9 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
                      ft.cmap.as<uint32_t>(), invd.as<uint32_t>(), total);
   TM_LAUNCH_CHECK();
   TM_HIP(hipStreamSynchronize(s));
+  // the map is built from the plain words; pass 1 needs only their LDS index
+  for (auto &w : order) w = (uint16_t)dev::sw(w);
+  copy_sync(ft.nkl.get(), order.data(), order.size() * 2, hipMemcpyHostToDevice, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -574,14 +581,14 @@ __global__ __launch_bounds__(1024) void k_kron_small(C *u, int L, C r, C f)
   const int n = 1 << L;
   for (int i = threadIdx.x; i < n; i += blockDim.x) lds[dev::sw(i)] = u[i];
   __syncthreads();
-  dev::lds_kron(lds, L, 0, r);
+  dev::lds_kron(lds, L, r);
   for (int i = threadIdx.x; i < n; i += blockDim.x) u[i] = lds[dev::sw(i)] * f;
 }
 
 constexpr int FUSED_THREADS = 1024;
 
 // Pass 1: one necklace column E = NR[c]; u(E, O') = xt[cmap[c][j]] with
-// O' = nkl[j] for all O'; transform over O; store A(E, O) grouped as
+// O' over all words (LDS index nkl[j] = sw(O')); transform over O; store A(E, O) grouped as
 // [row necklace][slot].
 template <typename C>
 __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
@@ -615,10 +622,10 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
     for (int u = 0; u < UNG; ++u) v[u] = xt[idx[u]];
 #pragma unroll
     for (int u = 0; u < UNG; ++u)
-      if (j0 + u * (int)blockDim.x < nh) lds[dev::sw(O[u])] = v[u];
+      if (j0 + u * (int)blockDim.x < nh) lds[O[u]] = v[u];
   }
   __syncthreads();
-  dev::lds_kron(lds, n, 0, kr);
+  dev::lds_kron(lds, n, kr);
   // output (scaled by kf = a^n): UNR slot-table loads in flight per thread
   // before the LDS reads
   C *out = Ap + (int64_t)c * ldA;
@@ -634,7 +641,7 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       const int t = t0 + u * blockDim.x;
-      if (t < ns) out[t] = O[u] != 0xFFFF ? lds[dev::sw(O[u])] * kf : (C)0;
+      if (t < ns) out[t] = O[u] != 0xFFFF ? lds[O[u]] * kf : (C)0;
     }
   }
 }
@@ -674,7 +681,7 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
     }
   }
   __syncthreads();
-  dev::lds_kron(lds, n, 0, kr);
+  dev::lds_kron(lds, n, kr);
   // 1 / scale of the input, and kf = a^n of this pass's butterflies
   const C inv = (scale ? (C)(1.0 / *scale) : (C)1) * kf;
   // internal numbering: the orbits of this row are [ostart[r], ostart[r+1]);
@@ -693,7 +700,7 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       const int64_t i = b + (int64_t)u * blockDim.x;
-      if (i < i1) y[i] = (TO)(d[u] * lds[dev::sw(E[u])] * inv);
+      if (i < i1) y[i] = (TO)(d[u] * lds[E[u]] * inv);
     }
   }
 }
