@@ -607,7 +607,7 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
   const uint32_t *mrow = cmap + ((uint64_t)c << n);
   // cmap columns are in necklace-major O' order (nkl): consecutive lanes
   // gather from a few contiguous runs of x~; UNR gathers in flight per thread
-  constexpr int UNG = 8;
+  constexpr int UNG = 16;
   for (int j0 = threadIdx.x; j0 < nh; j0 += blockDim.x * UNG) {
     uint32_t idx[UNG];
     uint16_t O[UNG];
