@@ -65,6 +65,11 @@ template <> struct Pk<float> {
   static constexpr int P = 4;
 };
 
+// Waves per SIMD the dot sweeps are compiled for: the compiler keeps every
+// basis load of an element group in flight, so registers grow with J; cap
+// them where the loads still fit without spilling.
+constexpr int sweep_waves(int J) { return J <= 8 ? 8 : (J <= 12 ? 6 : 1); }
+
 // sum_e a[e] b[e] in FP64
 template <typename PV, int P>
 __device__ __forceinline__ double pdot(const PV &a, const PV &b)
@@ -278,7 +283,7 @@ __global__ __launch_bounds__(THREADS) void k_update_dots_pk(const T *__restrict_
 // products (no register array of J vectors).  Same results as the general
 // kernel: the skipped terms subtract exact zeros.
 template <int J, typename T>
-__global__ __launch_bounds__(THREADS) void k_lanczos_dots_pk(const T *__restrict__ V,
+__global__ __launch_bounds__(THREADS, sweep_waves(J)) void k_lanczos_dots_pk(const T *__restrict__ V,
                                                              int64_t ldv, int j,
                                                              T *__restrict__ w, int64_t n,
                                                              const double *__restrict__ h,

@@ -492,7 +492,8 @@ class KrylovSchur final : public KrylovRun {
     }
     k_ = k;
     ++restarts_;
-    TM_HIP(hipStreamSynchronize(s));
+    // no synchronisation: the next cycle is ordered after V Y on the stream,
+    // and the staging buffer is next written by a stream-ordered read-back
   }
 
   // Run to convergence; returns the number of converged wanted pairs.
