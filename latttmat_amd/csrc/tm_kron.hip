@@ -17,11 +17,14 @@
 namespace ltm {
 namespace {
 
-constexpr int KRON_THREADS = 512;
+// Threads per workgroup: 512, or 1024 for the largest tile (one workgroup
+// per CU then still keeps 16 wavefronts resident).  A tile candidate encodes
+// the tile exponent in bits 0-7 and the 1024-thread choice in bit 8.
+constexpr int KRON_WIDE = 256;
 enum { F_PRO = 1, F_EPI = 2, F_SIN = 4, F_SOUT = 8 };
 
-template <typename C, typename TI, typename TO>
-__global__ __launch_bounds__(KRON_THREADS) void k_kron_pass(
+template <typename C, typename TI, typename TO, int NT>
+__global__ __launch_bounds__(NT) void k_kron_pass(
     const TI *__restrict__ in, TO *__restrict__ out, int b0, int nb, int logW, int flags,
     DiagSpec dg, const C *__restrict__ tab, C r, C f, const double *__restrict__ scale)
 {
@@ -109,7 +112,10 @@ std::vector<int> kron_tile_candidates(int es)
 {
   const int d = kron_tile_default(es);
   std::vector<int> c{d, d - 1};
-  if (((size_t(1) << (d + 1)) + 192) * (size_t)es <= g_lds_per_block) c.push_back(d + 1);
+  if (((size_t(1) << (d + 1)) + 192) * (size_t)es <= g_lds_per_block) {
+    c.push_back(d + 1);
+    c.push_back((d + 1) | KRON_WIDE);
+  }
   return c;
 }
 
@@ -132,7 +138,8 @@ std::vector<Pass> plan_passes(int L, int es, int tl)
 
 template <typename C, typename TI, typename TO>
 void run_pass(const Pass &p, int L, const void *in, void *out, int flags, const KronPlan &plan,
-              const C *tab, double dka, double dkb, const double *scale, hipStream_t s)
+              const C *tab, double dka, double dkb, const double *scale, hipStream_t s,
+              bool wide)
 {
   // K = ka [[1, r], [r, 1]]: the pass applies r in its butterflies and ka^nb
   // on store (derivatives dka, dkb for dual arithmetic)
@@ -142,18 +149,24 @@ void run_pass(const Pass &p, int L, const void *in, void *out, int flags, const 
   if (ntile <= 65535) grid = dim3((unsigned)ntile, 1, 1);
   else grid = dim3(65536, (unsigned)(ntile / 65536), 1);
   const size_t shm = ((size_t(1) << (p.nb + p.logW)) + 192) * sizeof(C);
-  hipLaunchKernelGGL((k_kron_pass<C, TI, TO>), grid, dim3(KRON_THREADS), shm, s,
-                     static_cast<const TI *>(in), static_cast<TO *>(out), p.b0, p.nb, p.logW,
-                     flags, plan.diag, tab, kc.r, kc.f, scale);
+  if (wide)
+    hipLaunchKernelGGL((k_kron_pass<C, TI, TO, 1024>), grid, dim3(1024), shm, s,
+                       static_cast<const TI *>(in), static_cast<TO *>(out), p.b0, p.nb, p.logW,
+                       flags, plan.diag, tab, kc.r, kc.f, scale);
+  else
+    hipLaunchKernelGGL((k_kron_pass<C, TI, TO, 512>), grid, dim3(512), shm, s,
+                       static_cast<const TI *>(in), static_cast<TO *>(out), p.b0, p.nb, p.logW,
+                       flags, plan.diag, tab, kc.r, kc.f, scale);
   TM_LAUNCH_CHECK();
 }
 
 template <typename C>
 void kron_typed(const KronPlan &plan, const void *in, void *out, tm_dtype io_in, tm_dtype io_out,
-                void *work, const double *scale, hipStream_t s, const C *tab, int tl)
+                void *work, const double *scale, hipStream_t s, const C *tab, int cand)
 {
   const int L = plan.L;
-  auto ps = plan_passes(L, (int)sizeof(C), tl);
+  const bool wide = (cand & KRON_WIDE) != 0;
+  auto ps = plan_passes(L, (int)sizeof(C), cand & 0xFF);
   const int np = (int)ps.size();
   // Intermediate buffer: the output itself when it has the compute type,
   // else the workspace.
@@ -171,18 +184,18 @@ void kron_typed(const KronPlan &plan, const void *in, void *out, tm_dtype io_in,
     const bool dst_io = (i == np - 1);
     const double *sc = (src_io || dst_io) ? scale : nullptr;
     if (src_io && dst_io) {
-      if (io_in == TM_F64 && io_out == TM_F64) run_pass<C, double, double>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
-      else if (io_in == TM_F64) run_pass<C, double, float>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
-      else if (io_out == TM_F64) run_pass<C, float, double>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
-      else run_pass<C, float, float>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
+      if (io_in == TM_F64 && io_out == TM_F64) run_pass<C, double, double>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s, wide);
+      else if (io_in == TM_F64) run_pass<C, double, float>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s, wide);
+      else if (io_out == TM_F64) run_pass<C, float, double>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s, wide);
+      else run_pass<C, float, float>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s, wide);
     } else if (src_io) {
-      if (io_in == TM_F64) run_pass<C, double, C>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
-      else run_pass<C, float, C>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
+      if (io_in == TM_F64) run_pass<C, double, C>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s, wide);
+      else run_pass<C, float, C>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s, wide);
     } else if (dst_io) {
-      if (io_out == TM_F64) run_pass<C, C, double>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
-      else run_pass<C, C, float>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s);
+      if (io_out == TM_F64) run_pass<C, C, double>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s, wide);
+      else run_pass<C, C, float>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, sc, s, wide);
     } else {
-      run_pass<C, C, C>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, nullptr, s);
+      run_pass<C, C, C>(ps[i], L, src, dst, flags, plan, tab, 0.0, 0.0, nullptr, s, wide);
     }
   }
 }
@@ -201,15 +214,16 @@ void launch_kron_dual(const KronPlan &plan, const double *in, void *out, const v
   const int L = plan.L;
   const Dual *tab = static_cast<const Dual *>(tab_dual);
   Dual *o = static_cast<Dual *>(out);
-  auto run = [&](int tl) {
-    auto ps = plan_passes(L, (int)sizeof(Dual), tl);
+  auto run = [&](int cand) {
+    const bool wide = (cand & KRON_WIDE) != 0;
+    auto ps = plan_passes(L, (int)sizeof(Dual), cand & 0xFF);
     const int np = (int)ps.size();
     for (int i = 0; i < np; ++i) {
       int flags = 0;
       if (i == 0 && plan.prologue) flags |= F_PRO;
       if (i == np - 1 && plan.epilogue) flags |= F_EPI;
-      if (i == 0) run_pass<Dual, double, Dual>(ps[i], L, in, o, flags, plan, tab, dka, dkb, nullptr, s);
-      else run_pass<Dual, Dual, Dual>(ps[i], L, o, o, flags, plan, tab, dka, dkb, nullptr, s);
+      if (i == 0) run_pass<Dual, double, Dual>(ps[i], L, in, o, flags, plan, tab, dka, dkb, nullptr, s, wide);
+      else run_pass<Dual, Dual, Dual>(ps[i], L, o, o, flags, plan, tab, dka, dkb, nullptr, s, wide);
     }
   };
   const int es = (int)sizeof(Dual);
@@ -220,11 +234,11 @@ void launch_kron(const KronPlan &plan, const void *in, void *out, tm_dtype io_in
                  tm_dtype comp, void *work, const double *scale, hipStream_t s)
 {
   const int es = comp == TM_F64 ? 8 : 4;
-  auto run = [&](int tl) {
+  auto run = [&](int cand) {
     if (comp == TM_F64)
-      kron_typed<double>(plan, in, out, io_in, io_out, work, scale, s, plan.tab64, tl);
+      kron_typed<double>(plan, in, out, io_in, io_out, work, scale, s, plan.tab64, cand);
     else
-      kron_typed<float>(plan, in, out, io_in, io_out, work, scale, s, plan.tab32, tl);
+      kron_typed<float>(plan, in, out, io_in, io_out, work, scale, s, plan.tab32, cand);
   };
   // timing candidates rewrites out from in, so only out-of-place applies tune
   run(autotune_tile(0, plan.L, es, kron_tile_candidates(es), plan.tune && in != out, s, run));
