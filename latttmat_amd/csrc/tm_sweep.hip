@@ -50,6 +50,14 @@ struct Coef {
   C b1[2], b2[2], b3[2];
 };
 
+// b[i] for a run-time bit i as a select (a dynamically indexed private
+// array would be placed in scratch memory)
+template <typename C>
+__device__ __forceinline__ C pick(const C (&b)[2], int i)
+{
+  return i ? b[1] : b[0];
+}
+
 // XDT gate: in[a][u] (a = aux, u = state bit t), control c.
 //   out[p][s] = b1[s^p] b2[c^s] (b2[s] in[0][p] + b2[s^1] in[1][p])
 template <typename C>
@@ -60,7 +68,7 @@ __device__ __forceinline__ void xdt_gate(C v[2][2], int c, const Coef<C> &k)
   for (int p = 0; p < 2; ++p)
 #pragma unroll
     for (int s = 0; s < 2; ++s)
-      o[p][s] = k.b1[s ^ p] * k.b2[c ^ s] * (k.b2[s] * v[0][p] + k.b2[s ^ 1] * v[1][p]);
+      o[p][s] = k.b1[s ^ p] * pick(k.b2, c ^ s) * (k.b2[s] * v[0][p] + k.b2[s ^ 1] * v[1][p]);
 #pragma unroll
   for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -221,7 +229,8 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restri
             const int want = a.bc == 5 ? 0 : 1;
             if (aux == want) {
               const int low = (int)(P & 1) ^ (a.bc == 5 ? 0 : 1);
-              v[uu] = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P) * k.b2[low] * k.b2[top];
+              v[uu] = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P) * pick(k.b2, low) *
+                      pick(k.b2, top);
             }
           }
         } else {
