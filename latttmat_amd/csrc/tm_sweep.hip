@@ -20,7 +20,11 @@
 namespace ltm {
 namespace {
 
-constexpr int SWEEP_THREADS = 512;
+// Workgroup size: 512 threads, or 1024 (one 128 KB tile per CU then still
+// keeps 16 wavefronts resident) -- an autotuner choice for FP64/FP32; a tile
+// candidate encodes the tile exponent in bits 0-7 and the 1024-thread choice
+// in bit 8.
+constexpr int SWEEP_WIDE = 256;
 
 struct SweepArgs {
   int L;
@@ -152,8 +156,8 @@ __device__ __forceinline__ void zbt_gate(C v[4][4], int fl, int fr, const ZCoef<
     for (int s = 0; s < 4; ++s) v[p][s] = o[p][s];
 }
 
-template <int MODEL, int DIR, typename C, typename TI, typename TO>
-__global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_pass(const TI *__restrict__ x,
+template <int MODEL, int DIR, typename C, typename TI, typename TO, int NT>
+__global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
                                                                const C *__restrict__ zin,
                                                                C *__restrict__ zout,
                                                                TO *__restrict__ y, SweepArgs a,
@@ -353,6 +357,7 @@ std::vector<int> sweep_tile_candidates(int es)
 {
   const bool big = g_lds_per_block >= (size_t)(136 * 1024);
   const int d = (big ? 17 : 16) - sweep_log_es(es);
+  if (es <= 8) return {d, d - 1, d | SWEEP_WIDE};
   return {d, d - 1};
 }
 
@@ -390,7 +395,7 @@ std::vector<SPass> plan_sweep(int model, int L, int bc, int es, int tl)
 
 template <int MODEL, int DIR, typename C, typename TI, typename TO>
 void run_sweep_pass(const SPass &p, SweepArgs a, const void *x, const C *zin, C *zout, void *y,
-                    const C *tab, const double *scale, hipStream_t s)
+                    const C *tab, const double *scale, hipStream_t s, bool wide)
 {
   a.t0 = p.t0;
   a.ng = p.ng;
@@ -403,7 +408,16 @@ void run_sweep_pass(const SPass &p, SweepArgs a, const void *x, const C *zin, C 
   if (ntile <= 65535) grid = dim3((unsigned)ntile);
   else grid = dim3(65536, (unsigned)(ntile / 65536));
   const size_t shm = ((size_t(1) << tileLog) + 192) * sizeof(C);
-  hipLaunchKernelGGL((k_sweep_pass<MODEL, DIR, C, TI, TO>), grid, dim3(SWEEP_THREADS), shm, s,
+  if constexpr (sizeof(C) <= 8) {
+    if (wide) {
+      hipLaunchKernelGGL((k_sweep_pass<MODEL, DIR, C, TI, TO, 1024>), grid, dim3(1024), shm, s,
+                         static_cast<const TI *>(x), zin, zout, static_cast<TO *>(y), a, tab,
+                         scale);
+      TM_LAUNCH_CHECK();
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_sweep_pass<MODEL, DIR, C, TI, TO, 512>), grid, dim3(512), shm, s,
                      static_cast<const TI *>(x), zin, zout, static_cast<TO *>(y), a, tab, scale);
   TM_LAUNCH_CHECK();
 }
@@ -443,17 +457,18 @@ SweepArgs make_args(const SweepPlan &plan)
 // the dual-number kernels).  The passes work in place in the aux array.
 template <int MODEL, int DIR, typename C, typename TI, typename TO>
 void sweep_run_tl(const SweepPlan &plan, const void *in, void *out, void *work,
-                  const double *scale, hipStream_t s, const C *tab, int tl)
+                  const double *scale, hipStream_t s, const C *tab, int cand)
 {
   SweepArgs a = make_args<MODEL, DIR>(plan);
-  auto ps = plan_sweep(MODEL, plan.L, plan.bc, (int)sizeof(C), tl);
+  const bool wide = (cand & SWEEP_WIDE) != 0;
+  auto ps = plan_sweep(MODEL, plan.L, plan.bc, (int)sizeof(C), cand & 0xFF);
   C *z = static_cast<C *>(work);
   const int np = (int)ps.size();
   for (int i = 0; i < np; ++i) {
     a.first = (i == 0);
     a.last = (i == np - 1);
     const double *sc = (a.first || a.last) ? scale : nullptr;
-    run_sweep_pass<MODEL, DIR, C, TI, TO>(ps[i], a, in, z, z, out, tab, sc, s);
+    run_sweep_pass<MODEL, DIR, C, TI, TO>(ps[i], a, in, z, z, out, tab, sc, s, wide);
   }
 }
 
