@@ -132,7 +132,7 @@ tm_status tm_op_orbits(const tm_op *op, uint64_t *reps, uint16_t *dups);
 
 /* Options of the eigen driver.  Defaults (tm_solve_opts_default): nev=1,
  * ncv=0 (=suggested_nev), tol=1e-10, maxit=1000, maxtrial=3 (pfunc.hpp:87),
- * precision FP64, seed 1, want_left=0, use_graph=1, basis_f32=1. */
+ * precision FP64, seed 1, want_left=0, use_graph=1, basis_f32=1, refine_extra=1. */
 typedef struct {
   int          nev;
   int          ncv;
@@ -158,6 +158,11 @@ typedef struct {
                                the Krylov basis in FP32 (half the bytes of
                                every orthogonalisation sweep; products and
                                sums still in FP64); 0 stores it in FP64   */
+  int          refine_extra;/* refine 1: the FP64 Rayleigh-Ritz block holds
+                               nev + refine_extra Ritz vectors (default 1);
+                               a larger block widens the gap in the error
+                               bound, a smaller one saves FP64 T.x; a block
+                               whose bound is not met is iterated          */
 } tm_solve_opts;
 
 /* Results.  Eigenvalues are those of the operator as applied (the reference

@@ -203,6 +203,7 @@ void tm_solve_opts_default(tm_solve_opts *o)
   o->verbose = 0;
   o->use_graph = 1;
   o->basis_f32 = 1;
+  o->refine_extra = 1;
   o->refine = 0;
 }
 

@@ -28,6 +28,24 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
+// Sum of p[b * stride] over b = lane, lane + 64, ... < nblk into lane 0 of
+// the wavefront: four independent accumulators per lane (loads in flight),
+// combined in a fixed order, then a fixed shuffle tree -- deterministic.
+__device__ __forceinline__ double lane_sum(const double *__restrict__ p, int nblk, int64_t stride,
+                                           int lane)
+{
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int b = lane;
+  for (; b + 192 < nblk; b += 256) {
+    s0 += p[(int64_t)b * stride];
+    s1 += p[(int64_t)(b + 64) * stride];
+    s2 += p[(int64_t)(b + 128) * stride];
+    s3 += p[(int64_t)(b + 192) * stride];
+  }
+  for (; b < nblk; b += 64) s0 += p[(int64_t)b * stride];
+  return wave_sum((s0 + s1) + (s2 + s3));
+}
+
 // Block-reduce the first `cnt` of the J values acc[] (compile-time indexed,
 // so the array stays in registers) and write them to out[0..cnt).
 template <int J>
@@ -157,9 +175,7 @@ __global__ __launch_bounds__(64) void k_reduce_partials(const double *__restrict
                                                         double *__restrict__ out)
 {
   const int k = blockIdx.x;
-  double s = 0;
-  for (int b = threadIdx.x; b < nblk; b += 64) s += partial[(int64_t)b * cnt + k];
-  s = wave_sum(s);
+  const double s = lane_sum(partial + k, nblk, cnt, threadIdx.x);
   if (threadIdx.x == 0) out[k] = s;
 }
 
@@ -513,9 +529,7 @@ __global__ __launch_bounds__(1024) void k_reduce_epi(const double *__restrict__ 
 {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int k = wid; k < cnt; k += nw) {
-    double s = 0;
-    for (int b = lane; b < nblk; b += 64) s += partial[(int64_t)b * cnt + k];
-    s = wave_sum(s);
+    const double s = lane_sum(partial + k, nblk, cnt, lane);
     if (lane == 0) out[k] = s;
   }
   __syncthreads();
@@ -684,9 +698,7 @@ __global__ __launch_bounds__(64) void k_reduce_gram(const double *__restrict__ p
                                                     double *__restrict__ out)
 {
   const int r = blockIdx.x / k, c = blockIdx.x % k;
-  double s = 0;
-  for (int b = threadIdx.x; b < nblk; b += 64) s += partial[(int64_t)b * stride + r * J + c];
-  s = wave_sum(s);
+  const double s = lane_sum(partial + r * J + c, nblk, stride, threadIdx.x);
   if (threadIdx.x == 0) out[r * k + c] = s;
 }
 
@@ -722,9 +734,7 @@ __global__ __launch_bounds__(64) void k_reduce_tri(const double *__restrict__ pa
   int a = 0;
   while ((a + 1) * (a + 2) / 2 <= t) ++a;
   const int b = t - a * (a + 1) / 2;
-  double s = 0;
-  for (int q = threadIdx.x; q < nblk; q += 64) s += partial[(int64_t)q * nt + t];
-  s = wave_sum(s);
+  const double s = lane_sum(partial + t, nblk, nt, threadIdx.x);
   if (threadIdx.x == 0 && a < nv) {
     G[a * nv + b] = s;
     G[b * nv + a] = s;

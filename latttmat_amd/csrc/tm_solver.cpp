@@ -812,7 +812,7 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   bool refined = false;
   if (mixed && op->symmetric && refine == 1) {
     // FP64 block Rayleigh-Ritz refinement of the kept FP32 Ritz vectors
-    const int k = std::min(ks->m() - 1, nev + 1);
+    const int k = std::min(ks->m() - 1, nev + std::max(0, o.refine_extra));
     DevBuf &B = cache.dev[SLOT_REFINE_B];
     B.ensure(sizeof(double) * ldv * 2 * k);
     ks->ritz_vectors(rz, k, B.as<double>(), nullptr);

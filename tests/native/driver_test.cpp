@@ -25,7 +25,7 @@ int main(int argc, char **argv)
   std::memset(&o, 0, sizeof o);
   o.nev = nev; o.ncv = ncv; o.tol = 1e-10; o.maxit = 1000; o.maxtrial = 3;
   o.precision = (tm_precision)prec; o.seed = 1; o.want_left = left; o.want_vectors = 1;
-  o.mixed_tol = 1e-6; o.verbose = argc > 2 ? 3 : 0; o.refine = 0; o.basis_f32 = basis32;
+  o.mixed_tol = 1e-6; o.verbose = argc > 2 ? 3 : 0; o.refine = 0; o.basis_f32 = basis32; o.refine_extra = 1;
   std::vector<double> er(nev), ei(nev), rs(nev), vecs((size_t)2 * n * nev), vl(n);
   tm_solve_out out;
   std::memset(&out, 0, sizeof out);

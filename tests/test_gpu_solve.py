@@ -79,6 +79,22 @@ def test_mixed_precision_reduced(lt):
     assert np.all(rel <= 1e-11)
 
 
+@pytest.mark.parametrize("extra,basis32", [(0, True), (2, True), (1, False)])
+def test_mixed_refine_block_and_basis(lt, extra, basis32):
+    # FP64 Rayleigh-Ritz block of nev + extra FP32 Ritz vectors, FP32- or
+    # FP64-stored basis in the FP32 phase: same eigenvalues and vectors as FP64
+    op = lt.TransferMatrix("A", 2, 5, 20, J=0.4, kappa=0.5, neigs=4)
+    r64 = lt.solve(op, nev=4, precision=lt.TM_PREC_F64, tol=1e-13, want_vectors=True)
+    rmx = lt.solve(op, nev=4, ncv=13, precision=lt.TM_PREC_MIXED, refine_extra=extra,
+                   basis_f32=basis32, want_vectors=True)
+    print("extra", extra, "basis32", basis32, "mv", rmx.matvecs_f32, rmx.matvecs_f64, flush=True)
+    rel = np.abs(rmx.evals - r64.evals) / np.abs(r64.evals)
+    assert np.all(rel <= 1e-12), rel
+    for c in range(4):
+        ov = abs(rmx.vecs[:, c] @ r64.vecs[:, c])
+        assert abs(ov - 1) < 1e-9, (c, ov)
+
+
 def test_xdt_dense(lt):
     op = lt.TransferMatrix("D", 2, 1, 10, J=0.6, kappa=0.7, h=0.1)
     r = lt.solve(op, nev=1)
