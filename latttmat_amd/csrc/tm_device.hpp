@@ -58,74 +58,87 @@ __device__ __forceinline__ uint32_t brev32(uint32_t s, int n)
 }
 
 // 3D row rotation right by 1 / 2 inside every row of s2 bits
-// (TMat/utility.cpp:52-63); whole-layer row shift is rotr(s, L, s2)
-// (TMat/utility.cpp:68-73).
-__device__ __forceinline__ uint64_t rff1(uint64_t s, const DiagSpec &g)
-{
-  return ((s >> 1) & g.cmr1) | ((s & g.cm1) << (g.s2 - 1));
-}
-__device__ __forceinline__ uint64_t rff2(uint64_t s, const DiagSpec &g)
-{
-  return ((s >> 2) & g.cmr2) | ((s & g.cm2) << (g.s2 - 2));
-}
+// (TMat/utility.cpp:52-63; in diag_index_w below); whole-layer row shift is
+// rotr(s, L, s2) (TMat/utility.cpp:68-73).
 
 // Popcount indices (i1, i2, i0) of a state for the diagonal tables.  Each
 // branch reproduces the corresponding genTvec of the reference; the table
-// entries carry the coupling constants and constant offsets.
+// entries carry the coupling constants and constant offsets.  W is the state
+// word: 32-bit arithmetic (half the VALU work) when the width allows it.
 struct Idx3 { int i1, i2, i0; };
 
-__device__ __forceinline__ Idx3 diag_index(const DiagSpec &g, uint64_t s)
+__device__ __forceinline__ int popw(uint32_t x) { return __popc(x); }
+__device__ __forceinline__ int popw(uint64_t x) { return __popcll(x); }
+
+template <typename W>
+__device__ __forceinline__ W rotrw(W s, int n, int d)
+{
+  return (s >> d) | ((s & ((W(1) << d) - 1)) << (n - d));
+}
+
+template <typename W>
+__device__ __forceinline__ Idx3 diag_index_w(const DiagSpec &g, W s)
 {
   const int L = g.L;
-  const uint64_t mask = (L >= 64) ? ~0ull : ((1ull << L) - 1);
+  const W mask = (L >= (int)(8 * sizeof(W))) ? ~W(0) : ((W(1) << L) - 1);
   Idx3 r;
-  r.i0 = pop(s);
+  r.i0 = popw(s);
   switch (g.kind) {
     case DIAG_X2D_PBC:
     case DIAG_ZBT:
-      r.i1 = pop(s ^ rotr(s, L, 1));
-      r.i2 = pop(s ^ rotr(s, L, 2));
+      r.i1 = popw(W(s ^ rotrw<W>(s, L, 1)));
+      r.i2 = popw(W(s ^ rotrw<W>(s, L, 2)));
       break;
     case DIAG_X2D_OBC:
-      r.i1 = pop((s & (mask >> 1)) ^ (s >> 1));
-      r.i2 = pop((s & (mask >> 2)) ^ (s >> 2));
+      r.i1 = popw(W((s & (mask >> 1)) ^ (s >> 1)));
+      r.i2 = popw(W((s & (mask >> 2)) ^ (s >> 2)));
       break;
     case DIAG_X2D_C5:
     case DIAG_X2D_C9: {
-      uint64_t a = (s << 1) | (g.kind == DIAG_X2D_C9 ? 1ull : 0ull);
-      r.i1 = pop(s ^ a);
-      r.i2 = pop((a & mask) ^ (s >> 1));
+      const W a = W(s << 1) | (g.kind == DIAG_X2D_C9 ? W(1) : W(0));
+      r.i1 = popw(W(s ^ a));
+      r.i2 = popw(W((a & mask) ^ (s >> 1)));
       break;
     }
     case DIAG_X2D_C13:
     case DIAG_X2D_C17: {
-      bool one = g.kind == DIAG_X2D_C17;
-      uint64_t a = (s << 1) | (one ? 1ull : 0ull);
-      uint64_t b = (s << 2) | (one ? 3ull : 0ull);
-      r.i1 = pop(s ^ a);
-      r.i2 = pop(s ^ b);
+      const bool one = g.kind == DIAG_X2D_C17;
+      const W a = W(s << 1) | (one ? W(1) : W(0));
+      const W b = W(s << 2) | (one ? W(3) : W(0));
+      r.i1 = popw(W(s ^ a));
+      r.i2 = popw(W(s ^ b));
       break;
     }
-    case DIAG_X3D:
-      r.i1 = pop(s ^ rff1(s, g)) + pop(s ^ rotr(s, L, g.s2));
-      r.i2 = pop(s ^ rff2(s, g));
+    case DIAG_X3D: {
+      const W f1 = ((s >> 1) & W(g.cmr1)) | ((s & W(g.cm1)) << (g.s2 - 1));
+      const W f2 = ((s >> 2) & W(g.cmr2)) | ((s & W(g.cm2)) << (g.s2 - 2));
+      r.i1 = popw(W(s ^ f1)) + popw(W(s ^ rotrw<W>(s, L, g.s2)));
+      r.i2 = popw(W(s ^ f2));
       break;
+    }
     case DIAG_XDT_PBC:
-      r.i1 = pop(s ^ rotr(s, L, 1));
+      r.i1 = popw(W(s ^ rotrw<W>(s, L, 1)));
       r.i2 = 0;
       break;
     case DIAG_XDT_C5:
-      r.i1 = pop((s << 1) ^ s);
+      r.i1 = popw(W(W(s << 1) ^ s));
       r.i2 = 0;
       break;
     case DIAG_XDT_C9:
-      r.i1 = pop(((s << 1) | 1ull) ^ s);
+      r.i1 = popw(W((W(s << 1) | W(1)) ^ s));
       r.i2 = 0;
       break;
     default:
       r.i1 = r.i2 = 0;
   }
   return r;
+}
+
+// 32-bit words up to L = 30: the open-boundary branches shift left by up to
+// two bits without masking and count bits L and L+1 as well
+__device__ __forceinline__ Idx3 diag_index(const DiagSpec &g, uint64_t s)
+{
+  return g.L <= 30 ? diag_index_w<uint32_t>(g, (uint32_t)s) : diag_index_w<uint64_t>(g, s);
 }
 
 template <typename C>
