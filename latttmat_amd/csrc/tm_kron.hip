@@ -74,15 +74,20 @@ __global__ __launch_bounds__(NT) void k_kron_pass(
 
   // nb stages: LDS index of element (p, w) is p * W + w, so bit j of p is
   // bit (j + logW) of the LDS index; the w bits ride along as lo bits.
-  for (int bg = 0; bg < nb; bg += 4) {
-    const int rr = nb - bg < 4 ? nb - bg : 4;
-    switch (rr) {
-      case 1: dev::lds_kron_group<1>(lds, tileLog, bg + logW, r); break;
-      case 2: dev::lds_kron_group<2>(lds, tileLog, bg + logW, r); break;
-      case 3: dev::lds_kron_group<3>(lds, tileLog, bg + logW, r); break;
-      default: dev::lds_kron_group<4>(lds, tileLog, bg + logW, r); break;
+  if (logW == 0) {
+    // contiguous tile (the first pass): bit groups at compile-time positions
+    dev::lds_kron(lds, nb, r);
+  } else {
+    for (int bg = 0; bg < nb; bg += 4) {
+      const int rr = nb - bg < 4 ? nb - bg : 4;
+      switch (rr) {
+        case 1: dev::lds_kron_group<1>(lds, tileLog, bg + logW, r); break;
+        case 2: dev::lds_kron_group<2>(lds, tileLog, bg + logW, r); break;
+        case 3: dev::lds_kron_group<3>(lds, tileLog, bg + logW, r); break;
+        default: dev::lds_kron_group<4>(lds, tileLog, bg + logW, r); break;
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 
   // f = a^nb of K = a [[1, r], [r, 1]] over this pass's bits
