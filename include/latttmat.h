@@ -47,7 +47,9 @@ typedef enum {
   TM_KIND_ZBT = 3,      /* TMat/tmat.cpp:480-662      BNNNI/3NN diagonal TM     */
   TM_KIND_XAM2D = 4,    /* TMat/tmatreduce.cpp:338-563 ANNNI reduced            */
   TM_KIND_XAM3D = 5,    /* TMat/tmat3d.cpp:102-340    3D ANNNI reduced          */
-  TM_KIND_XAT3D = 6     /* full 2^(s1*s2) 3D ANNNI (no reference counterpart)   */
+  TM_KIND_XAT3D = 6,    /* full 2^(s1*s2) 3D ANNNI (no reference counterpart)   */
+  TM_KIND_XDM = 7,      /* TMat/tmatreduce.cpp:1024-1400 DNNI reduced           */
+  TM_KIND_ZBM = 8       /* TMat/tmatreduce.cpp:1403-1868 BNNNI/3NN diagonal reduced */
 } tm_kind;
 
 /* Model parameters: the tmat input-file keys (TMat/read_input.cpp:29-60). */

@@ -16,7 +16,8 @@ TM_PREC_F64, TM_PREC_MIXED = 0, 1
 TM_FLAG_FULLSPACE = 1
 TM_FLAG_GENERIC_REDUCED = 2
 TM_FLAG_NO_AUTOTUNE = 4
-KIND_NAMES = {1: "XAT", 2: "XDT", 3: "ZBT", 4: "XAM2D", 5: "XAM3D", 6: "XAT3D"}
+KIND_NAMES = {1: "XAT", 2: "XDT", 3: "ZBT", 4: "XAM2D", 5: "XAM3D", 6: "XAT3D", 7: "XDM",
+              8: "ZBM"}
 
 # every function the header declares (checked by the CPU tests)
 EXPORTS = [

@@ -177,7 +177,7 @@ tm_status tm_op_time_apply(const tm_op *cop, tm_dtype io, tm_dtype compute, int 
 tm_status tm_op_orbits(const tm_op *op, uint64_t *reps, uint16_t *dups)
 {
   if (!op || !reps || !dups) return fail(TM_EINVAL, "null argument");
-  if (op->kind != TM_KIND_XAM2D && op->kind != TM_KIND_XAM3D)
+  if (!ltm::is_reduced(op->kind))
     return fail(TM_EINVAL, "operator has no orbit tables (not a reduced operator)");
   return guarded([&] {
     ltm::copy_sync(reps, op->reps.get(), sizeof(uint64_t) * op->ncol, hipMemcpyDeviceToHost, op->stream);

@@ -80,6 +80,9 @@ OpConsts<S> consts_for(const tm_op *op, S bJ, S k, S k2, S h)
     case TM_KIND_XAM2D: return consts_xat<S>(op->L, 0, bJ, k, h);
     case TM_KIND_XDT: return consts_xdt<S>(op->L, op->bc, bJ, k, h);
     case TM_KIND_ZBT: return consts_zbt<S>(op->L, bJ, k, k2, h);
+    // the reduced circuits carry the periodic full-space constants
+    case TM_KIND_XDM: return consts_xdt<S>(op->L, 0, bJ, k, h);
+    case TM_KIND_ZBM: return consts_zbt<S>(op->L, bJ, k, k2, h);
     case TM_KIND_XAT3D:
     case TM_KIND_XAM3D: return consts_xat3d<S>(op->L, bJ, k, h);
     default: throw Error(TM_EINVAL, "unknown operator kind");
@@ -159,18 +162,22 @@ void full_numerators(const tm_op *cop, tm_op *op, const double *l, const double 
 
 void grad_numerators(tm_op *op, const double *l, const double *r, double num[5])
 {
-  if (op->kind != TM_KIND_XAM2D && op->kind != TM_KIND_XAM3D) {
+  if (!is_reduced(op->kind)) {
     full_numerators(op, op, l, r, num);
     return;
   }
   // reduced: the same numerators on the expanded full-space vectors
+  // (<P l, dT P r> = <l, P^T dT P r>, <P l, P r> = <l, r>)
   tm_params q = op->p;
   if (op->kind == TM_KIND_XAM2D) q.axialdir = op->bc ? 9 : 1;
+  else if (op->kind == TM_KIND_XDM) q.axialdir = 1;
+  else if (op->kind == TM_KIND_ZBM) q.axialdir = 3;
   else q.flags |= TM_FLAG_FULLSPACE;
   q.flags &= ~TM_FLAG_GENERIC_REDUCED;
   std::unique_ptr<tm_op> full(make_op(q));
-  const GroupSpec g{op->kind == TM_KIND_XAM2D ? 2 : 3, op->L, op->s1, op->s2,
-                    op->kind == TM_KIND_XAM2D ? op->bc : 0, op->censym, op->rotate};
+  const bool two = op->kind != TM_KIND_XAM3D;
+  const GroupSpec g{two ? 2 : 3, op->L, op->s1, op->s2, op->kind == TM_KIND_XAM2D ? op->bc : 0,
+                    op->censym, op->rotate, op->kind == TM_KIND_ZBM ? 1 : 0};
   const size_t bytes = sizeof(double) * (size_t)full->ncol;
   DevBuf lf(bytes), rf(bytes);
   expand_orbits(g, op->reps.as<uint64_t>(), op->dups.as<uint16_t>(), op->ncol, l,

@@ -231,6 +231,9 @@ struct GroupSpec {
   int bc;           // 2D: 0 periodic, 1 open
   int censym;       // include global spin flip
   int rotate;       // 3D: include 90-degree rotation
+  int zgroup;       // 2D periodic: translations by two sites and the
+                    // reflection i -> -i (genZ2DDuplist, TMat/tmatreduce.cpp:242-334)
+                    // instead of all translations and i -> L-1-i
 };
 // Enumerate orbits on the device.  Fills reps (ascending) and dups (orbit
 // sizes), returns M.  If map != nullptr it receives map[s] = orbit index of s.
@@ -284,6 +287,10 @@ struct ReducedPlan {
   const uint32_t *map;    // device [2^L] (generic path) or null
   const FusedTables *ft;  // fused path tables or null
   double ka, kb;
+  // full-space operator between expansion and gather (XDM, ZBM); null = the
+  // Kronecker product of K = [[ka, kb], [kb, ka]] (XAM)
+  const SweepPlan *sweep;
+  int transpose;
 };
 void launch_reduced(const ReducedPlan &plan, const void *in, void *out, tm_dtype io,
                     tm_dtype comp, void *work, const double *scale_in, hipStream_t s);
@@ -386,6 +393,7 @@ void gen_eig(int n, const double *A, double *wr, double *wi, double *Zr, double 
 struct tm_op {
   tm_params p{};
   tm_kind kind{};
+  tm_kind fullkind{};   // XDM, ZBM: the full-space circuit (XDT, ZBT) they reduce
   int L = 0, s1 = 0, s2 = 0;
   int bc = 0;           // boundary code (XAT/XDT) or extend flag (XAM)
   int censym = 0, rotate = 0, nleadbit = 0;
@@ -416,6 +424,8 @@ struct tm_op {
   }
 
   virtual ~tm_op();
+  // gate-circuit plan of XDT/ZBT (or of the circuit a reduced XDM/ZBM wraps)
+  ltm::SweepPlan sweep_plan() const;
   // T.x on vectors in the internal order (the eigensolver's basis)
   void apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
                       int transpose, const double *scale_in, hipStream_t s);
@@ -425,6 +435,11 @@ struct tm_op {
 };
 
 namespace ltm {
+// Operators over orbit representatives (the reduced matrices).
+inline bool is_reduced(tm_kind k)
+{
+  return k == TM_KIND_XAM2D || k == TM_KIND_XAM3D || k == TM_KIND_XDM || k == TM_KIND_ZBM;
+}
 // Construct an operator from parameters (pfunc dispatch); throws Error.
 tm_op *make_op(const tm_params &p);
 // Hellmann-Feynman numerators: num[i] = <l, dT/dtheta_i r> for theta =

@@ -269,11 +269,15 @@ void init_device()
   done = true;
 }
 
-// Reduced 2D/3D operators: orbit tables + Tvec on the device.
-void build_reduced(tm_op *op, const GroupSpec &g, double nfexp, bool want_fused)
+// Reduced 2D/3D operators: orbit tables + Tvec on the device.  With
+// diag_in_full the full-space operator applied between expansion and gather
+// carries the layer diagonal and the normfactor itself (XDM, ZBM), so
+// Tvec = 1/sqrt(dup) and dup*Tvec = sqrt(dup).
+void build_reduced(tm_op *op, const GroupSpec &g, double nfexp, bool want_fused,
+                   bool diag_in_full = false)
 {
-  const bool fused = want_fused && g.dim == 2 && g.bc == 0 && (g.L % 2) == 0 && g.L >= 8 &&
-                     ((size_t(1) << (g.L / 2)) * sizeof(double) <= g_lds_per_block);
+  const bool fused = want_fused && !g.zgroup && g.dim == 2 && g.bc == 0 && (g.L % 2) == 0 &&
+                     g.L >= 8 && ((size_t(1) << (g.L / 2)) * sizeof(double) <= g_lds_per_block);
   const int64_t M = enumerate_orbits(g, op->reps, op->dups, fused ? nullptr : &op->map, op->stream);
   op->ncol = M;
   std::vector<uint64_t> reps(M);
@@ -283,6 +287,11 @@ void build_reduced(tm_op *op, const GroupSpec &g, double nfexp, bool want_fused)
   std::vector<double> tv(M), dt(M);
   const double Jn = 0.5 * op->p.betaJ * nfexp;
   for (int64_t o = 0; o < M; ++o) {
+    if (diag_in_full) {
+      tv[o] = 1.0 / std::sqrt((double)dups[o]);
+      dt[o] = std::sqrt((double)dups[o]);
+      continue;
+    }
     double e;
     if (g.dim == 2) e = energy_half_x2d(reps[o], g.L, g.bc, op->p.betaJ, op->p.kappa, op->p.h);
     else e = energy_half_x3d(reps[o], g.s1, g.s2, op->p.betaJ, op->p.kappa, op->p.h);
@@ -373,6 +382,32 @@ tm_op *make_op(const tm_params &p)
     op->diag = DiagSpec{DIAG_ZBT, L, 0, 0, 0, 0, 0, 0};
     apply_consts(op.get(), consts_zbt<double>(L, bJ, k, p.kappa2, h));  // TMat/tmat.cpp:480-518
   };
+  // Reduced operators over a full-space gate circuit: T_red = P^T T P with
+  // P(s, o) = [s in o] / sqrt(|o|), the algebra of XDMMatVecProd2D::perform_op
+  // (TMat/tmatreduce.cpp:1054-1228) and ZBMMatVecProd2D::perform_op_internal
+  // (TMat/tmatreduce.cpp:1457-1746): the scatter of Tvec*x over every image
+  // of an orbit, the transfer over the expanded layer, and the gather
+  // y *= dup*Tvec at the representative.  Both are periodic only
+  // (TMat/tmatreduce.cpp:1047-1049, 1446-1448).
+  auto make_xdm = [&](int L, int ext) {
+    TM_CHECK(L >= 4 && L <= 30, TM_EINVAL, "XDM width must be in [4, 30]");
+    make_xdt(L, 0);  // the full-space DNNI circuit and its constants
+    op->kind = TM_KIND_XDM;
+    op->fullkind = TM_KIND_XDT;
+    op->censym = (h == 0 && (ext & 2) == 0);  // TMat/tmatreduce.cpp:1028
+    GroupSpec g{2, L, 0, 0, 0, op->censym, 0, 0};
+    build_reduced(op.get(), g, 0.0, false, true);
+  };
+  auto make_zbm = [&](int L, int ext) {
+    TM_CHECK(L >= 4 && L <= 30 && (L % 2) == 0, TM_EINVAL,
+             "ZBM needs an even width in [4, 30] (translations by two sites)");
+    make_zbt(L);
+    op->kind = TM_KIND_ZBM;
+    op->fullkind = TM_KIND_ZBT;
+    op->censym = (h == 0 && (ext & 2) == 0);  // TMat/tmatreduce.cpp:1406
+    GroupSpec g{2, L, 0, 0, 0, op->censym, 0, 1};
+    build_reduced(op.get(), g, 0.0, false, true);
+  };
 
   if (p.model == 'A' && p.dim == 2) {
     const int L = p.size1;
@@ -419,8 +454,9 @@ tm_op *make_op(const tm_params &p)
     TM_CHECK(p.dim == 2, TM_EINVAL, "BNNNI is 2D only");
     switch (p.axialdir) {
       case 3: make_zbt(p.size1); break;
-      case 6: case 7:
-        throw Error(TM_EUNSUPPORTED, "BNNNI reduced paths (axialdir 6, 7) are not built");
+      case 7: make_zbm(p.size1, multieigs); break;
+      case 6:
+        throw Error(TM_EUNSUPPORTED, "BNNNI parallel-TM reduced path (axialdir 6) is not built");
       default:
         throw Error(TM_EINVAL, "Function yet implemented (unknown axialdir)");
     }
@@ -430,8 +466,7 @@ tm_op *make_op(const tm_params &p)
       case 1: case 2: make_xdt(p.size1, 0); break;
       case 17: make_xdt(p.size1, 5); break;
       case 18: make_xdt(p.size1, 9); break;
-      case 5: case 6:
-        throw Error(TM_EUNSUPPORTED, "DNNI reduced path (axialdir 5, 6) is not built");
+      case 5: case 6: make_xdm(p.size1, multieigs); break;
       default:
         throw Error(TM_EINVAL, "Function yet implemented (unknown axialdir)");
     }
@@ -471,6 +506,25 @@ void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io,
   }
 }
 
+ltm::SweepPlan tm_op::sweep_plan() const
+{
+  ltm::SweepPlan sp{};
+  sp.kind = fullkind ? fullkind : kind;
+  sp.L = L;
+  sp.bc = bc;
+  sp.diag = diag;
+  sp.tab64 = tab64.as<double>();
+  sp.tab32 = tab32.as<float>();
+  for (int i = 0; i < 2; ++i) {
+    sp.b1[i] = bzwj1[i];
+    sp.b2[i] = bzwj2[i];
+    sp.b3[i] = bzwj3[i];
+  }
+  sp.nf = nf;
+  sp.tune = !(p.flags & TM_FLAG_NO_AUTOTUNE);
+  return sp;
+}
+
 void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype io,
                            tm_dtype comp, int transpose, const double *scale_in, hipStream_t s)
 {
@@ -497,27 +551,16 @@ void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtyp
       }
       case TM_KIND_XDT:
       case TM_KIND_ZBT: {
-        SweepPlan sp{};
-        sp.kind = kind;
-        sp.L = L;
-        sp.bc = bc;
-        sp.diag = diag;
-        sp.tab64 = tab64.as<double>();
-        sp.tab32 = tab32.as<float>();
-        for (int i = 0; i < 2; ++i) {
-          sp.b1[i] = bzwj1[i];
-          sp.b2[i] = bzwj2[i];
-          sp.b3[i] = bzwj3[i];
-        }
-        sp.nf = nf;
-        sp.tune = !(p.flags & TM_FLAG_NO_AUTOTUNE);
+        const SweepPlan sp = sweep_plan();
         work.ensure(sweep_work_bytes(kind, L, comp));
         launch_sweep(sp, xv, yv, io, comp, kind == TM_KIND_ZBT ? transpose : 0, work.get(),
                      scale_in, s);
         break;
       }
       case TM_KIND_XAM2D:
-      case TM_KIND_XAM3D: {
+      case TM_KIND_XAM3D:
+      case TM_KIND_XDM:
+      case TM_KIND_ZBM: {
         ReducedPlan rp{};
         rp.M = ncol;
         rp.L = L;
@@ -530,6 +573,12 @@ void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtyp
         rp.ft = ft.get();
         rp.ka = bzwj1[0];
         rp.kb = bzwj1[1];
+        SweepPlan sp{};
+        if (kind == TM_KIND_XDM || kind == TM_KIND_ZBM) {
+          sp = sweep_plan();
+          rp.sweep = &sp;
+          rp.transpose = kind == TM_KIND_ZBM ? transpose : 0;
+        }
         if (!rp.ft) TM_CHECK(rp.map != nullptr, TM_EINVAL, "reduced operator has no state map");
         work.ensure(reduced_work_bytes(rp, comp));
         launch_reduced(rp, xv, yv, io, comp, work.get(), scale_in, s);

@@ -43,7 +43,7 @@ using dev::pop;
 // group actions
 // ---------------------------------------------------------------------------
 struct GroupDev {
-  int dim, L, s1, s2, bc, censym, rotate;
+  int dim, L, s1, s2, bc, censym, rotate, zgroup;
   uint64_t mask;
   uint64_t cm1, cmr1;  // 3D column masks for row rotation by 1
   uint64_t rowmask;    // low s2 bits
@@ -91,7 +91,18 @@ __device__ uint64_t canon(uint64_t s, const GroupDev &g, int *stab)
     const int nf = g.censym ? 2 : 1;
     for (int f = 0; f < nf; ++f) {
       const uint64_t s0 = f ? (~s & g.mask) : s;
-      if (g.bc == 0) {
+      if (g.zgroup) {
+        // translations by two sites t and their mirror images i -> -i,
+        // brev(rotr(t, 1)) (TMat/tmatreduce.cpp:272-297)
+        uint64_t t = s0;
+        for (int k = 0; k < g.L / 2; ++k) {
+          const uint64_t r = dev::brev(dev::rotr(t, g.L, 1), g.L);
+          m = t < m ? t : m;
+          m = r < m ? r : m;
+          st += (t == s) + (r == s);
+          t = dev::rotr(t, g.L, 2);
+        }
+      } else if (g.bc == 0) {
         uint64_t t = s0;
         for (int k = 0; k < g.L; ++k) {
           const uint64_t r = dev::brev(t, g.L);
@@ -273,6 +284,7 @@ GroupDev to_dev(const GroupSpec &gs)
   g.bc = gs.bc;
   g.censym = gs.censym;
   g.rotate = gs.rotate;
+  g.zgroup = gs.zgroup;
   g.mask = (1ull << gs.L) - 1;
   if (gs.dim == 3) {
     uint64_t c1 = 0, cr1 = 0;
@@ -284,6 +296,8 @@ GroupDev to_dev(const GroupSpec &gs)
     g.cmr1 = cr1;
     g.rowmask = (1ull << gs.s2) - 1;
     g.order = gs.s1 * gs.s2 * 4 * (gs.censym ? 2 : 1) * (gs.rotate ? 2 : 1);
+  } else if (gs.zgroup) {
+    g.order = gs.L * (gs.censym ? 2 : 1);
   } else {
     g.order = (gs.bc == 0 ? 2 * gs.L : 2) * (gs.censym ? 2 : 1);
   }
@@ -749,13 +763,19 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
     TM_LAUNCH_CHECK();
     return;
   }
-  // generic: expand, Kronecker passes, gather
+  // generic: expand, full-space operator, gather
   const uint64_t N = 1ull << p.L;
   C *u = rest;
   hipLaunchKernelGGL((k_expand<C>), dim3(grid_for(N, 256)), dim3(256), 0, s, p.map, xt, u, N);
   TM_LAUNCH_CHECK();
   const int smallL = sizeof(C) == 8 ? 13 : 14;
-  if (p.L <= smallL) {
+  if (p.sweep) {
+    // gate-circuit operator (XDT or ZBT) with its own diagonal and normfactor
+    C *v = u + N;
+    const tm_dtype ct = sizeof(C) == 8 ? TM_F64 : TM_F32;
+    launch_sweep(*p.sweep, u, v, ct, ct, p.transpose, v + N, nullptr, s);
+    u = v;
+  } else if (p.L <= smallL) {
     hipLaunchKernelGGL((k_kron_small<C>), dim3(1), dim3(1024), (size_t(1) << p.L) * sizeof(C),
                        s, u, p.L, kr, dev::kron_coef<C>(p.ka, 0, p.kb, 0, p.L).f);
     TM_LAUNCH_CHECK();
@@ -786,6 +806,8 @@ size_t reduced_work_bytes(const ReducedPlan &p, tm_dtype comp)
   size_t b = (size_t)((p.M + 63) / 64) * 64 * es;
   if (p.ft) {
     b += (size_t)p.ft->nNk * p.ft->ldA * es;
+  } else if (p.sweep) {
+    b += 2 * (size_t(1) << p.L) * es + sweep_work_bytes(p.sweep->kind, p.L, comp);
   } else {
     b += (size_t(1) << p.L) * es;
   }

@@ -126,8 +126,11 @@ void write_column(std::ostream &os, const std::vector<T> &v)
 // orbit representatives, TMat/tmatreduce.cpp:565-613; the 3D reduced
 // operator reports NaN for the node counts, TMat/tmat3d.cpp:165-177).
 // Non-symmetric operators weight by ql_s qr_s / sum(ql qr) and have no extra
-// observables (TMat/tmatphys.cpp:106-132, 98-104).
-enum class PhysKind { Sym2D, Sym3D, Gen };
+// observables (TMat/tmatphys.cpp:106-132, 98-104; the reduced ZBM likewise
+// over its representatives, TMat/tmatreduce.cpp:1748-1781).  The reduced DNNI
+// operator reports <|m_st|>/L, <|m|>/L, <m_st^2>/L^2, <m^2>/L^2 instead, m_st
+// the magnetisation with every odd site flipped (TMat/tmatreduce.cpp:1289-1317).
+enum class PhysKind { Sym2D, Sym3D, Gen, Dnni };
 
 int nnode(uint64_t s, int L, int bc)  // TMat/utility.cpp:33-47
 {
@@ -145,6 +148,8 @@ void write_phys(const std::string &fn, PhysKind pk, int L, int bc,
   std::vector<double> ext;
   if (pk == PhysKind::Sym2D) ext.assign(2, 0.0);
   if (pk == PhysKind::Sym3D) ext.assign(2, NAN);
+  if (pk == PhysKind::Dnni) ext.assign(4, 0.0);
+  const uint64_t mask = L >= 64 ? ~0ull : (1ull << L) - 1;
   const size_t n = qr.size();
   for (size_t i = 0; i < n; ++i) {
     const uint64_t s = states.empty() ? (uint64_t)i : states[i];
@@ -161,6 +166,19 @@ void write_phys(const std::string &fn, PhysKind pk, int L, int bc,
       ext[0] += nd * w;
       ext[1] += nd * nd * w;
     }
+    if (pk == PhysKind::Dnni) {
+      const double ms = 2.0 * __builtin_popcountll((s ^ 0xaaaaaaaaaaaaaaaaull) & mask) - L;
+      ext[0] += std::fabs(ms) * w;
+      ext[1] += std::fabs(m) * w;
+      ext[2] += ms * ms * w;
+      ext[3] += m * m * w;
+    }
+  }
+  if (pk == PhysKind::Dnni) {
+    ext[0] /= L;
+    ext[1] /= L;
+    ext[2] /= (double)L * L;
+    ext[3] /= (double)L * L;
   }
   const double div = pk == PhysKind::Gen ? L * dnorm : (double)L;
   std::ofstream f(fn);
@@ -295,12 +313,14 @@ int main(int argc, char **argv)
       // X_phys.dat
       const tm_kind kind = tm_op_kind(op.handle());
       std::vector<uint64_t> states;
-      if (kind == TM_KIND_XAM2D || kind == TM_KIND_XAM3D) {
+      if (kind == TM_KIND_XAM2D || kind == TM_KIND_XAM3D || kind == TM_KIND_XDM ||
+          kind == TM_KIND_ZBM) {
         std::vector<uint16_t> dups(n);
         states.resize(n);
         latttmat::check(tm_op_orbits(op.handle(), states.data(), dups.data()));
       }
-      const PhysKind pk = !sym ? PhysKind::Gen
+      const PhysKind pk = !sym                    ? PhysKind::Gen
+                          : kind == TM_KIND_XDM ? PhysKind::Dnni
                           : (kind == TM_KIND_XAM3D || kind == TM_KIND_XAT3D) ? PhysKind::Sym3D
                                                                              : PhysKind::Sym2D;
       write_phys(in.outprefix + "_phys.dat", pk, op.getLayerSize(), phys_bc(in), states, ql, qr);

@@ -15,7 +15,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-XAT, XDT, ZBT, XAM2D, XAM3D, XAT3D = 1, 2, 3, 4, 5, 6
+XAT, XDT, ZBT, XAM2D, XAM3D, XAT3D, XDM, ZBM = 1, 2, 3, 4, 5, 6, 7, 8
 
 _lib = None
 
@@ -69,6 +69,8 @@ def lib():
         L.or_op_dups.argtypes = [C.c_void_p]
         L.or_orbits_x2d.restype = i64
         L.or_orbits_x2d.argtypes = [i32, i32, i32, C.c_void_p, C.c_void_p]
+        L.or_orbits_z2d.restype = i64
+        L.or_orbits_z2d.argtypes = [i32, i32, C.c_void_p, C.c_void_p]
         L.or_orbits_x3d.restype = i64
         L.or_orbits_x3d.argtypes = [i32, i32, i32, i32, C.c_void_p, C.c_void_p]
         L.or_tvec_xat.restype = None
@@ -141,6 +143,14 @@ def orbits_x2d(L, bc=0, censym=0):
     reps = np.empty(m, np.uint64)
     dups = np.empty(m, np.uint16)
     lib().or_orbits_x2d(L, bc, censym, reps.ctypes.data, dups.ctypes.data)
+    return reps, dups
+
+
+def orbits_z2d(L, censym=0):
+    m = lib().or_orbits_z2d(L, censym, None, None)
+    reps = np.empty(m, np.uint64)
+    dups = np.empty(m, np.uint16)
+    lib().or_orbits_z2d(L, censym, reps.ctypes.data, dups.ctypes.data)
     return reps, dups
 
 

@@ -329,6 +329,42 @@ int64_t or_orbits_x3d(int s1, int s2, int censym, int rotate, uint64_t *reps, ui
   return m;
 }
 
+/* TMat/tmatreduce.cpp:242-334 (genZ2DDuplist), periodic boundary: orbits of
+ * the translations by two sites and of the mirror image
+ * bitreversal(rotr(s, 1)) (i -> -i), times the spin flip if censym. */
+int64_t or_orbits_z2d(int L, int censym, uint64_t *reps, uint16_t *dups)
+{
+  uint64_t nstate = 1ULL << L, mask = nstate - 1;
+  bitset buf;
+  if (!bs_init(&buf, L)) return -1;
+  int64_t m = 0;
+  for (uint64_t now = 0; now < nstate; ++now) {
+    if (bs_get(&buf, now)) continue;
+    push_rep(reps, dups, m, now);
+    uint16_t d = 0;
+    uint64_t nx;
+    for (nx = now; !bs_get(&buf, nx); nx = (nx >> 2) | ((nx & 3) << (L - 2))) {
+      ++d; bs_set(&buf, nx);
+      if (censym) {
+        uint64_t fm = ~nx & mask;
+        if (!bs_get(&buf, fm)) { ++d; bs_set(&buf, fm); }
+      }
+    }
+    for (nx = or_bitreversal((now >> 1) | ((now & 1) << (L - 1)), L); !bs_get(&buf, nx);
+         nx = (nx >> 2) | ((nx & 3) << (L - 2))) {
+      ++d; bs_set(&buf, nx);
+      if (censym) {
+        uint64_t fm = ~nx & mask;
+        if (!bs_get(&buf, fm)) { ++d; bs_set(&buf, fm); }
+      }
+    }
+    if (dups) dups[m] = d;
+    ++m;
+  }
+  free(buf.w);
+  return m;
+}
+
 /* ------------------------------------------------------------------------ */
 /* operator construction                                                     */
 /* ------------------------------------------------------------------------ */
@@ -450,6 +486,79 @@ or_op *or_op_create(int kind, int size1, int size2, int bc,
         }
         double e = J1 * (2 * n1 - L) + J2 * (2 * n2 - L) + hJ * (2 * n0 - L) + Jn;
         op->tvec[rp] = exp(e) / sqrt((double)op->dups[rp]);
+      }
+      break;
+    }
+    case OR_XDM: {
+      /* TMat/tmatreduce.cpp:1024-1053 and genTvec :1367-1398 */
+      int L = size1;
+      if (L < 4 || L > 30) goto fail;
+      int extend = bc;
+      op->bc = 0;  /* periodic only (tmatreduce.cpp:1047-1049) */
+      op->L = L; op->nstate = 1LL << L;
+      op->nleadbit = L < 10 ? 1 : 2 + (int)log2((double)L);
+      op->censym = (h == 0 && (extend & 2) == 0);
+      double nfe = (kappa < 0.5) ? L * 2.0 * (kappa - 1.0) : -L * 2.0 * kappa;
+      op->nf = exp(betaJ * nfe);
+      int64_t m = or_orbits_x2d(L, 0, op->censym, NULL, NULL);
+      if (m <= 0) goto fail;
+      op->ncol = m;
+      op->reps = (uint64_t *)malloc(sizeof(uint64_t) * m);
+      op->dups = (uint16_t *)malloc(sizeof(uint16_t) * m);
+      op->tvec = (double *)malloc(sizeof(double) * m);
+      if (!op->reps || !op->dups || !op->tvec) goto fail;
+      or_orbits_x2d(L, 0, op->censym, op->reps, op->dups);
+      double hb = 0.5 * betaJ;
+      double J1 = -hb, hJ = -hb * h, Jn = hb * nfe;
+#pragma omp parallel for schedule(static)
+      for (int64_t rp = 0; rp < m; ++rp) {
+        uint64_t st = op->reps[rp];
+        int n1 = POP(st ^ or_offset_state(st, L, 1)), n0 = POP(st);
+        double e = J1 * (2 * n1 - L) + hJ * (2 * n0 - L) + Jn;
+        op->tvec[rp] = exp(e) / sqrt((double)op->dups[rp]);
+      }
+      break;
+    }
+    case OR_ZBM: {
+      /* TMat/tmatreduce.cpp:1403-1454 and genTvec :1813-1851 */
+      int L = size1;
+      if (L < 4 || L > 30 || (L & 1)) goto fail;
+      int extend = bc;
+      op->bc = 0;
+      op->L = L; op->nstate = 1LL << L;
+      op->nleadbit = L < 10 ? 2 : 2 + ((int)log2((double)L) & ~1);
+      op->censym = (h == 0 && (extend & 2) == 0);
+      op->bzwj2b[0] = exp(-2 * betaJ * kappa);
+      op->bzwj2b[3] = 1.0 / op->bzwj2b[0];
+      op->bzwj2b[1] = op->bzwj2b[2] = 1.0;
+      for (int rp = 0; rp < 16; ++rp) {
+        uint64_t ra = (uint64_t)rp >> 2, rb = (uint64_t)rp & 3;
+        op->bzwj2c[rp] = exp(betaJ * (kappa * (2 * POP(ra) - 2) + kappa2 * (2 * POP(rb) - 2)));
+      }
+      double nfe;
+      if (kappa + kappa2 < 0.5) nfe = L * 2.0 * (kappa + kappa2 - 1.0);
+      else if (kappa < 0.5 && kappa2 - kappa < 0.5) nfe = -L;
+      else if (kappa2 < 2 * kappa) nfe = -L * 2.0 * kappa;
+      else nfe = L * 2.0 * (kappa - kappa2);
+      op->nf = exp(betaJ * nfe);
+      int64_t m = or_orbits_z2d(L, op->censym, NULL, NULL);
+      if (m <= 0) goto fail;
+      op->ncol = m;
+      op->reps = (uint64_t *)malloc(sizeof(uint64_t) * m);
+      op->dups = (uint16_t *)malloc(sizeof(uint16_t) * m);
+      op->tvec = (double *)malloc(sizeof(double) * m);
+      if (!op->reps || !op->dups || !op->tvec) goto fail;
+      or_orbits_z2d(L, op->censym, op->reps, op->dups);
+      double hb = 0.5 * betaJ;
+      double J1 = -hb, J3 = hb * kappa2, hJ = -hb * h, Jn = hb * nfe;
+#pragma omp parallel for schedule(static)
+      for (int64_t rp = 0; rp < m; ++rp) {
+        uint64_t st = op->reps[rp];
+        int n1 = POP(st ^ or_offset_state(st, L, 1));
+        int n2 = POP(st ^ or_offset_state(st, L, 2));
+        int n0 = POP(st);
+        double e = bzexpn(L, n1, J1) + bzexpn(L, n2, J3) + bzexpn(L, n0, hJ);
+        op->tvec[rp] = exp(e + Jn) / sqrt((double)op->dups[rp]);
       }
       break;
     }
@@ -839,6 +948,251 @@ static void apply_xam3d(const or_op *op, const double *x, double *y)
   free(yt);
 }
 
+/* TMat/tmatreduce.cpp:1230-1259 (XDMMatVecProd2D::permute_mult_TL): the
+ * sub-lead bits one at a time, the auxiliary bit (index | 2^nsub) holding the
+ * previous old-layer spin; outbit is the new-layer spin past the last one. */
+static void xdm_permute_mult_tl(const or_op *op, double *yt, int outbit)
+{
+  uint64_t pn = (uint64_t)op->nstate >> op->nleadbit;
+  int nloop = op->L - op->nleadbit;
+  const double *b1 = op->bzwj1, *b2 = op->bzwj2;
+  for (int it = 0; it < nloop; ++it) {
+    uint64_t ms = 1ULL << it, bshift = ~ms & ((uint64_t)op->nstate - 1);
+#pragma omp parallel for schedule(static)
+    for (int64_t irp = 0; irp < (int64_t)pn; ++irp) {
+      uint64_t rp = (uint64_t)irp;
+      if (!(rp & ms)) continue;
+      int nb = outbit;
+      if (it < nloop - 1) nb = (int)((rp >> (it + 1)) & 1);
+      uint64_t i00 = rp & bshift, i01 = rp, i10 = (rp & bshift) | pn, i11 = rp | pn;
+      double d00 = yt[i00], d01 = yt[i01], d10 = yt[i10], d11 = yt[i11];
+      yt[i00] = b1[0] * b2[nb] * (b2[0] * d00 + b2[1] * d10);
+      yt[i01] = b1[1] * b2[nb ^ 1] * (b2[1] * d00 + b2[0] * d10);
+      yt[i10] = b1[1] * b2[nb] * (b2[0] * d01 + b2[1] * d11);
+      yt[i11] = b1[0] * b2[nb ^ 1] * (b2[1] * d01 + b2[0] * d11);
+    }
+  }
+}
+
+/* TMat/tmatreduce.cpp:1054-1228 (XDMMatVecProd2D::perform_op), periodic */
+static void apply_xdm(const or_op *op, const double *x, double *y)
+{
+  int L = op->L, nlb = op->nleadbit;
+  uint64_t N = (uint64_t)op->nstate;
+  uint64_t nloop = 1ULL << nlb, vecsize = N >> nlb;
+  uint64_t mask = N - 1, maskb = vecsize - 1, maskc = nloop - 1, maskd = maskc >> 1;
+  uint64_t maskbinv = ~maskb & (N - 1);
+  int nsub = L - nlb;
+  uint64_t nsubstate = 1ULL << nsub;
+  int64_t m = op->ncol;
+  double *xt = (double *)malloc(sizeof(double) * vecsize);
+  double *yt = (double *)malloc(sizeof(double) * 2 * vecsize);
+  const uint64_t *idx = op->reps;
+  const double *tv = op->tvec, *b1 = op->bzwj1, *b2 = op->bzwj2;
+  for (int64_t rp = 0; rp < m; ++rp) y[rp] = 0.0;
+  int cen = op->censym;
+  if (cen) nloop >>= 1;
+  for (uint64_t rdup = 0; rdup < nloop; ++rdup) {
+    uint64_t rh = rdup << nsub, rr = or_bitreversal(rdup, nlb);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t rp = 0; rp < m; ++rp) {
+      double d = tv[rp] * x[rp];
+      uint64_t now = idx[rp], nx = now;
+      do {
+        if (cen) {
+          if ((nx & maskbinv) == rh) xt[nx & maskb] = d;
+          else if ((~nx & maskbinv) == rh) xt[~nx & maskb] = d;
+          if ((nx & maskc) == rr) xt[or_bitreversal(nx, L) & maskb] = d;
+          else if ((~nx & maskc) == rr) xt[or_bitreversal(~nx, L) & maskb] = d;
+        } else {
+          if ((nx & maskbinv) == rh) xt[nx & maskb] = d;
+          if ((nx & maskc) == rr) xt[or_bitreversal(nx, L) & maskb] = d;
+        }
+        nx = (nx >> 1) | ((nx & 1) << (L - 1));
+      } while (nx != now);
+    }
+    /* fbit: highest bit of the new layer */
+    for (int fbit = 0; fbit < 2; ++fbit) {
+      /* the old layer's highest bit (the auxiliary bit); always 0 when folded */
+      uint64_t gbit = cen ? 0 : (rdup >> (nlb - 1)) << nsub;
+#pragma omp parallel for schedule(static)
+      for (int64_t irp = 0; irp < (int64_t)vecsize; ++irp) {
+        uint64_t rp = (uint64_t)irp;
+        yt[rp | gbit] = xt[rp] * b2[(rp & 1) ^ (uint64_t)fbit];
+        yt[rp | (gbit ^ nsubstate)] = 0.0;
+      }
+      xdm_permute_mult_tl(op, yt, (int)(rdup & 1));
+#pragma omp parallel for schedule(static)
+      for (int64_t rp = 0; rp < m; ++rp) {
+        uint64_t now = idx[rp];
+        if (cen) {
+          if (fbit) now = ~now & mask;
+        } else if ((now >> (L - 1)) ^ (uint64_t)fbit) {
+          continue;
+        }
+        uint64_t rdupb = now >> nsub;
+        double d = or_powdi(b1, or_spinbzexp(rdup, rdupb, nlb));
+        d *= or_powdi(b2, (2 * POP(((rdup >> 1) ^ rdupb) & maskd) - (nlb - 1)) +
+                              (2 * POP((rdup ^ (rdupb >> 1)) & maskd) - (nlb - 1)));
+        y[rp] += d * (yt[now & maskb] * b2[rdupb & 1] + yt[(now & maskb) | vecsize] * b2[(rdupb & 1) ^ 1]);
+      }
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t rp = 0; rp < m; ++rp) y[rp] *= op->dups[rp] * tv[rp];
+  free(xt);
+  free(yt);
+}
+
+/* TMat/tmatreduce.cpp:1615-1652 (ZBMMatVecProd2D::permute_mult_T1) and, with
+ * last >= 0, :1654-1691 (permute_mult_T1_last): one two-site step over the
+ * sub-lead bits, two auxiliary bits per entry.  dir 1 = transpose. */
+static void zbm_step(const or_op *op, const double *xin, double *yout, int last, int dir)
+{
+  int nsub = op->L - op->nleadbit;
+  uint64_t vecsize = (uint64_t)op->nstate >> op->nleadbit;
+  uint64_t nstateb = vecsize << 2, maskb = vecsize - 4;
+  const double *b1 = op->bzwj1, *b2b = op->bzwj2b, *b2c = op->bzwj2c;
+#pragma omp parallel for schedule(static)
+  for (int64_t irpb = 0; irpb < (int64_t)nstateb; ++irpb) {
+    uint64_t rpb = (uint64_t)irpb;
+    uint64_t auxp = rpb & 3, rp = rpb >> 2;
+    uint64_t rpp = (or_offset_state(rp, nsub, 2) << 2) | auxp;
+    uint64_t rq = ((rp & maskb) | auxp) << 2;
+    unsigned sbit = (unsigned)(rp & 3);
+    unsigned hi = last >= 0 ? ((unsigned)last << 2) : (unsigned)(rp & 0xc);
+    unsigned j2 = (hi | (unsigned)auxp) ^ (sbit << 2 | sbit);
+    double v, d;
+    if (!dir) {
+      unsigned srbit = (unsigned)(rp & 1), plbit = (unsigned)(auxp >> 1);
+      v = b1[srbit ^ plbit] * b2c[j2];
+      d = xin[rq] * b1[srbit] * b2b[sbit];
+      d += xin[rq + 1] * b1[srbit] * b2b[sbit ^ 1];
+      d += xin[rq + 2] * b1[srbit ^ 1] * b2b[sbit ^ 2];
+      d += xin[rq + 3] * b1[srbit ^ 1] * b2b[sbit ^ 3];
+    } else {
+      unsigned slbit = (unsigned)((rp >> 1) & 1), prbit = (unsigned)(auxp & 1);
+      unsigned frbit = last >= 0 ? ((unsigned)last & 1) : (unsigned)((rp >> 2) & 1);
+      v = b1[slbit ^ frbit] * b1[slbit ^ prbit] * b2c[j2];
+      d = xin[rq] * b2b[sbit];
+      d += xin[rq + 1] * b2b[sbit ^ 1];
+      d += xin[rq + 2] * b2b[sbit ^ 2];
+      d += xin[rq + 3] * b2b[sbit ^ 3];
+    }
+    yout[rpp] = v * d;
+  }
+}
+
+/* TMat/tmatreduce.cpp:1693-1746 (ZBMMatVecProd2D::leadingbz): the lead bits'
+ * interactions of state now with the transferred sub-lead part yt. */
+static double zbm_leadingbz(const or_op *op, const double *yt, uint64_t now, uint64_t lead,
+                            uint64_t lastbit, int dir)
+{
+  const uint64_t EVEN = 0x5555555555555555ULL;
+  int nlb = op->nleadbit, nsub = op->L - nlb;
+  uint64_t vecsize = (uint64_t)op->nstate >> nlb;
+  uint64_t maskb = vecsize - 1, maskc = (1ULL << nlb) - 1, maskd = maskc >> 2;
+  uint64_t nl = now >> nsub, nsh = (now & maskb) << 2;
+  const double *b1 = op->bzwj1, *b2 = op->bzwj2, *b2b = op->bzwj2b;
+  double df1 = or_powdi(op->bzwj3, or_spinbzexp(nl, lead, nlb));
+  df1 *= or_powdi(b2, or_spinbzexp(nl, (lead >> 2) | (lastbit << (nlb - 2)), nlb) +
+                          or_spinbzexp(nl >> 2, lead & maskd, nlb - 2));
+  uint64_t inow, inxt;
+  int iint;
+  double df2;
+  unsigned q = (unsigned)(nl & 3);
+  if (!dir) {
+    inow = nl & EVEN;
+    inow |= inow >> 1;
+    inxt = (lead >> 1) & EVEN;
+    inxt |= (inxt << 1) & (maskc >> 1);
+    iint = or_spinbzexp(inow, inxt, nlb - 1);
+    unsigned lb = (unsigned)(nl & 1);
+    df2 = b1[lb] * (b2b[q] * yt[nsh] + b2b[q ^ 1] * yt[nsh + 1]) +
+          b1[lb ^ 1] * (b2b[q ^ 2] * yt[nsh + 2] + b2b[q ^ 3] * yt[nsh + 3]);
+  } else {
+    inow = (nl >> 1) & EVEN;
+    inow |= inow << 1;
+    inxt = (lead | (lastbit << nlb)) & EVEN;
+    inxt = (inxt & maskc) | (inxt >> 1);
+    iint = or_spinbzexp(inow, inxt, nlb);
+    df2 = b2b[q] * yt[nsh] + b2b[q ^ 1] * yt[nsh + 1] + b2b[q ^ 2] * yt[nsh + 2] +
+          b2b[q ^ 3] * yt[nsh + 3];
+  }
+  df1 *= or_powdi(b1, iint);
+  return df1 * df2;
+}
+
+/* TMat/tmatreduce.cpp:1460-1612 (ZBMMatVecProd2D::perform_op_internal) */
+static void apply_zbm(const or_op *op, const double *x, double *y, int dir)
+{
+  int L = op->L, nlb = op->nleadbit;
+  uint64_t N = (uint64_t)op->nstate;
+  uint64_t nloop = 1ULL << nlb, vecsize = N >> nlb;
+  uint64_t mask = N - 1, maskb = vecsize - 1, maskc = nloop - 1;
+  uint64_t maskbinv = ~maskb & (N - 1);
+  int nsub = L - nlb;
+  int64_t m = op->ncol;
+  double *xt = (double *)malloc(sizeof(double) * vecsize);
+  double *p1 = (double *)malloc(sizeof(double) * 4 * vecsize);
+  double *p2 = (double *)malloc(sizeof(double) * 4 * vecsize);
+  const uint64_t *idx = op->reps;
+  const double *tv = op->tvec;
+  for (int64_t rp = 0; rp < m; ++rp) y[rp] = 0.0;
+  int cen = op->censym;
+  if (cen) nloop >>= 1;
+  for (uint64_t rdup = 0; rdup < nloop; ++rdup) {
+    uint64_t rh = rdup << nsub, rhinv = (~rdup & maskc) << nsub;
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t rp = 0; rp < m; ++rp) {
+      double d = tv[rp] * x[rp];
+      uint64_t now = idx[rp], nx = now;
+      do {
+        uint64_t ni = or_bitreversal((nx >> 1) | ((nx & 1) << (L - 1)), L);
+        if (cen) {
+          if ((nx & maskbinv) == rh) xt[nx & maskb] = d;
+          else if ((nx & maskbinv) == rhinv) xt[~nx & maskb] = d;
+          if ((ni & maskbinv) == rh) xt[ni & maskb] = d;
+          else if ((ni & maskbinv) == rhinv) xt[~ni & maskb] = d;
+        } else {
+          if ((nx & maskbinv) == rh) xt[nx & maskb] = d;
+          if ((ni & maskbinv) == rh) xt[ni & maskb] = d;
+        }
+        nx = (nx >> 2) | ((nx & 3) << (L - 2));
+      } while (nx != now);
+    }
+    uint64_t fbit = rdup >> (nlb - 2);
+    for (uint64_t lastbit = 0; lastbit < 4; ++lastbit) {
+#pragma omp parallel for schedule(static)
+      for (int64_t irp = 0; irp < (int64_t)vecsize; ++irp) {
+        uint64_t rp = (uint64_t)irp, rrp = (rp << 2) | fbit;
+        p1[rrp] = ((rp & 3) == lastbit) ? xt[rp] : 0.0;
+        p1[rrp ^ 1] = p1[rrp ^ 2] = p1[rrp ^ 3] = 0.0;
+      }
+      double *a = p1, *b = p2, *t;
+      for (int r = 0; r < nsub / 2 - 1; ++r) {
+        zbm_step(op, a, b, -1, dir);
+        t = a; a = b; b = t;
+      }
+      zbm_step(op, a, b, (int)(rdup & 3), dir);
+#pragma omp parallel for schedule(static)
+      for (int64_t rp = 0; rp < m; ++rp) {
+        uint64_t now = idx[rp];
+        double d = zbm_leadingbz(op, b, now, rdup, lastbit, dir);
+        if (cen) d += zbm_leadingbz(op, b, ~now & mask, rdup, lastbit, dir);
+        y[rp] += d;
+      }
+      /* the reference swaps its own buffer pointers; keep p1 the input side */
+      if (b == p1) { p1 = p2; p2 = b; }
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t rp = 0; rp < m; ++rp) y[rp] *= op->dups[rp] * tv[rp];
+  free(xt);
+  free(p1);
+  free(p2);
+}
+
 void or_op_apply(const or_op *op, const double *x, double *y, int transpose)
 {
   switch (op->kind) {
@@ -848,6 +1202,8 @@ void or_op_apply(const or_op *op, const double *x, double *y, int transpose)
     case OR_ZBT: apply_zbt(op, x, y, transpose ? 1 : 0); break;
     case OR_XAM2D: apply_xam2d(op, x, y); break;
     case OR_XAM3D: apply_xam3d(op, x, y); break;
+    case OR_XDM: apply_xdm(op, x, y); break;
+    case OR_ZBM: apply_zbm(op, x, y, transpose ? 1 : 0); break;
     default: break;
   }
 }

@@ -18,6 +18,9 @@
  *   orbit enumeration         brute-force closure, Burnside counts, min-rep invariant
  *   XAM 2D / 3D apply         projection of the dense full T onto orbit sums;
  *                             lambda0(reduced) == lambda0(full)
+ *   XDM / ZBM apply           projection of the dense full XDT / ZBT onto orbit
+ *                             sums (ZBM transposed too); lambda0(reduced) ==
+ *                             lambda0(full); group generators commute with T
  *   full 3D operator          dense T from the 3D Hamiltonian (2x3, 3x3 layers)
  */
 #ifndef TM_ORACLE_H
@@ -47,8 +50,11 @@ enum {
   OR_ZBT = 3,      /* TMat/tmat.cpp:480-662    BNNNI/3NN diagonal TM, full 2^L       */
   OR_XAM2D = 4,    /* TMat/tmatreduce.cpp:338-563  ANNNI reduced by D_L (x flip)     */
   OR_XAM3D = 5,    /* TMat/tmat3d.cpp:102-340  3D ANNNI reduced                       */
-  OR_XAT3D = 6     /* full 2^(s1*s2) 3D ANNNI operator (no reference counterpart);
+  OR_XAT3D = 6,    /* full 2^(s1*s2) 3D ANNNI operator (no reference counterpart);
                       Tv from TMat/tmat3d.cpp:179-201 energies, Th = (x)K            */
+  OR_XDM = 7,      /* TMat/tmatreduce.cpp:1024-1400 DNNI reduced by D_L (x flip)     */
+  OR_ZBM = 8       /* TMat/tmatreduce.cpp:1403-1868 BNNNI/3NN diagonal TM reduced by
+                      two-site translations and i -> -i (x flip)                     */
 };
 
 typedef struct or_op {
@@ -61,6 +67,7 @@ typedef struct or_op {
   double betaJ, kappa, kappa2, h;
   double nf;        /* normfactor                                    */
   double bzwj1[2], bzwj2[2], bzwj3[2];
+  double bzwj2b[4], bzwj2c[16];  /* ZBM step tables, TMat/tmatreduce.cpp:1416-1424 */
   int64_t nstate;   /* 2^L                                           */
   int64_t ncol;     /* rows()/cols()                                 */
   double *tvec;     /* length ncol                                   */
@@ -70,6 +77,7 @@ typedef struct or_op {
 
 /* Build an operator.  bc follows the reference: XAT 0,1,5,9,13,17; XDT 0,5,9;
  * XAM2D extend_cond (bit0 open boundary, 2 = no spin-flip folding);
+ * XDM, ZBM extend_cond (2 = no spin-flip folding; periodic only);
  * XAM3D extend_cond (bit0 ignored, as the reference warns).
  * Returns NULL on invalid input. */
 or_op *or_op_create(int kind, int size1, int size2, int bc,
@@ -88,6 +96,7 @@ const uint16_t *or_op_dups(const or_op *op);
  * count only, else must hold at least the returned count (call twice). */
 int64_t or_orbits_x2d(int L, int bc, int censym, uint64_t *reps, uint16_t *dups);
 int64_t or_orbits_x3d(int s1, int s2, int censym, int rotate, uint64_t *reps, uint16_t *dups);
+int64_t or_orbits_z2d(int L, int censym, uint64_t *reps, uint16_t *dups);  /* tmatreduce.cpp:242-334 */
 
 /* Diagonal Tv^(1/2) generators (genTvec) for the full operators, exactly as
  * the reference builds them (argument betaJ is the one passed to genTvec). */

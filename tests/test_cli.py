@@ -216,3 +216,67 @@ def test_cli_zbt_left_and_vecs(tmp_path, oracle):
     m = 2 * np.array([bin(int(v)).count("1") for v in s], float) - L
     ref = [float(np.sum(m ** k * wgt) / (L * wgt.sum())) for k in range(1, 5)]
     np.testing.assert_allclose(mom, ref, rtol=1e-8, atol=1e-10)
+
+
+@pytest.mark.gpu
+def test_cli_xdm_phys(tmp_path, oracle):
+    """Reduced DNNI (D, axialdir 5): X_phys.dat holds the four moments over
+    the orbit representatives and the four staggered / uniform magnetisation
+    observables of XDMMatVecProd2D::phys_extend (TMat/tmatreduce.cpp:1289-1317)."""
+    L = 12
+    p = write_input(str(tmp_path), model="D", axialdir=5, size=L, savevec="1 1", out="X",
+                    J=0.5, kappa="0.6 0")
+    r = run([p], str(tmp_path))
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    oop = oracle.OracleOp(oracle.XDM, L, 0, 0, 0.5, 0.6, 0.0, 0.0)
+    T = oop.dense()
+    w, V = np.linalg.eigh(0.5 * (T + T.T))
+    q = V[:, -1]
+    assert abs(float(np.loadtxt(tmp_path / "X_val.dat")) / (w[-1] / oop.normfactor) - 1) < 1e-12
+    reps = oop.reps()
+    pc = np.array([bin(int(s)).count("1") for s in reps], float)
+    m = 2 * pc - L
+    stag = 2 * np.array([bin(int(s) ^ 0xAAA).count("1") for s in reps], float) - L
+    q2 = q * q
+    mom = [float(np.sum(m ** k * q2)) / L for k in range(1, 5)]
+    ext = [float(np.sum(np.abs(stag) * q2)) / L, float(np.sum(np.abs(m) * q2)) / L,
+           float(np.sum(stag ** 2 * q2)) / L ** 2, float(np.sum(m ** 2 * q2)) / L ** 2]
+    g = read_phys(tmp_path / "X_phys.dat")
+    print(g, mom, ext, flush=True)
+    np.testing.assert_allclose(g[0], mom, rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(g[1], ext, rtol=1e-8, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_cli_zbm_left_and_phys(tmp_path, oracle):
+    """Reduced diagonal BNNNI (B, axialdir 7): complex X_val.dat, right and
+    left Perron vectors over the orbits, moments weighted by ql*qr (no second
+    line: ZBMMatVecProd2D::phys_extend reports nothing)."""
+    L = 10
+    p = write_input(str(tmp_path), model="B", axialdir=7, size=L, kappa="0.5 0.3",
+                    savevec="1 1", out="Z")
+    r = run([p], str(tmp_path))
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    D = oracle.OracleOp(oracle.ZBM, L, 0, 0, 0.5, 0.5, 0.3, 0.0)
+    T = D.dense()
+    w = np.linalg.eigvals(T)
+    lam = w[np.argmax(np.abs(w))].real
+    raw = open(tmp_path / "Z_val.dat").read()
+    re_, im_ = raw.strip("()").split(",")
+    # the solve stops at the reference's residual tolerance 1e-10
+    # (pfunc.hpp:52): for a non-symmetric matrix the eigenvalue error is first order
+    assert abs(float(re_) / (lam / D.normfactor) - 1) < 1e-10 and float(im_) == 0.0
+    vr = np.fromfile(tmp_path / "Z_vec.dat", dtype=np.float64)
+    vl = np.fromfile(tmp_path / "Z_vecl.dat", dtype=np.float64)
+    assert vr.size == vl.size == D.n
+    assert np.linalg.norm(T @ vr - lam * vr) / abs(lam) < 1e-8
+    assert np.linalg.norm(T.T @ vl - lam * vl) / abs(lam) < 1e-8
+    phys = open(tmp_path / "Z_phys.dat").read()
+    assert "\n" not in phys
+    mom = [float(x) for x in phys.split("\t")]
+    wgt = vl * vr
+    m = 2 * np.array([bin(int(v)).count("1") for v in D.reps()], float) - L
+    ref = [float(np.sum(m ** k * wgt) / (L * wgt.sum())) for k in range(1, 5)]
+    np.testing.assert_allclose(mom, ref, rtol=1e-8, atol=1e-10)

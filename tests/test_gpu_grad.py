@@ -22,7 +22,7 @@ def lt():
 
 def dense_lambda0(oracle, kind, s1, s2, bc, th):
     T = oracle.OracleOp(kind, s1, s2, bc, *th).dense()
-    if kind == oracle.ZBT:
+    if kind in (oracle.ZBT, oracle.ZBM):
         w = np.linalg.eigvals(T)
     else:
         w = np.linalg.eigvalsh(0.5 * (T + T.T))
@@ -49,6 +49,10 @@ CASES = [
     ("A", 2, 5, 12, "XAM2D", 12, 0, 0, (0.4, 0.45, 0.0, 0.0)),
     ("A", 2, 13, 12, "XAM2D", 12, 0, 1, (0.4, 0.3, 0.0, 0.15)),
     ("A", 3, 1, (3, 4), "XAM3D", 3, 4, 0, (0.25, 0.4, 0.0, 0.0)),
+    ("D", 2, 5, 10, "XDM", 10, 0, 0, (0.6, 0.7, 0.0, 0.1)),
+    ("D", 2, 6, 10, "XDM", 10, 0, 0, (0.5, 0.3, 0.0, 0.0)),
+    ("B", 2, 7, 8, "ZBM", 8, 0, 0, (0.5, 0.55, 0.3, 0.1)),
+    ("B", 2, 7, 10, "ZBM", 10, 0, 0, (0.4, 0.2, 0.6, 0.0)),
 ]
 
 

@@ -257,3 +257,133 @@ def test_reduced_lambda0_equals_full(oracle):
     wf = np.linalg.eigvalsh(full.dense())[-1]
     wr = max(abs(np.linalg.eigvals(red.dense())))
     assert abs(wr / wf - 1) < 1e-13
+
+
+def _closure(L, gens):
+    """Orbits of the group generated by gens on L-bit states: (reps ascending,
+    sizes, orbit index of every state), by breadth-first closure."""
+    N = 1 << L
+    orb = np.full(N, -1, dtype=np.int64)
+    reps, sizes = [], []
+    for s in range(N):
+        if orb[s] >= 0:
+            continue
+        o = len(reps)
+        orb[s] = o
+        todo, n = [s], 1
+        while todo:
+            t = todo.pop()
+            for g in gens:
+                u = g(t)
+                if orb[u] < 0:
+                    orb[u] = o
+                    n += 1
+                    todo.append(u)
+        reps.append(s)
+        sizes.append(n)
+    return np.array(reps, np.uint64), np.array(sizes, np.uint16), orb
+
+
+def _x_gens(L, censym):
+    mask = (1 << L) - 1
+    g = [lambda s: physics.rotr(s, L, 1), lambda s: int(format(s, f"0{L}b")[::-1], 2)]
+    return g + ([lambda s: ~s & mask] if censym else [])
+
+
+def _z_gens(L, censym):
+    # translations by two sites and the mirror i -> -i
+    mask = (1 << L) - 1
+    g = [lambda s: physics.rotr(s, L, 2),
+         lambda s: int(format(physics.rotr(s, L, 1), f"0{L}b")[::-1], 2)]
+    return g + ([lambda s: ~s & mask] if censym else [])
+
+
+def _projection(T, orb, sizes):
+    P = np.zeros((T.shape[0], len(sizes)))
+    P[np.arange(T.shape[0]), orb] = 1.0 / np.sqrt(sizes.astype(np.float64))[orb]
+    return P.T @ T @ P
+
+
+@pytest.mark.parametrize("L", [4, 6, 8, 10, 12])
+@pytest.mark.parametrize("censym", [0, 1])
+def test_orbits_z2d_closure(oracle, L, censym):
+    reps, dups = oracle.orbits_z2d(L, censym)
+    br, bd, _ = _closure(L, _z_gens(L, censym))
+    np.testing.assert_array_equal(reps, br)
+    np.testing.assert_array_equal(dups, bd)
+    assert int(dups.sum()) == 1 << L
+
+
+@pytest.mark.parametrize("L,bJ,k,h", [(8, 0.5, 0.3, 0.0), (9, 0.4, 0.7, 0.1), (10, 0.6, 0.4, 0.0)])
+def test_xdt_commutes_with_dihedral_group(L, bJ, k, h):
+    """The DNNI layer transfer matrix is invariant under the group XDM
+    reduces by (so P^T T P carries its symmetric-sector spectrum)."""
+    T = physics.xdt_dense(L, bJ, k, h)
+    for g in _x_gens(L, h == 0):
+        perm = np.array([g(s) for s in range(1 << L)])
+        assert np.max(np.abs(T[np.ix_(perm, perm)] - T)) <= 1e-14 * np.max(np.abs(T))
+
+
+@pytest.mark.parametrize("L,bJ,k,k2,h", [(8, 0.5, 0.5, 0.3, 0.0), (10, 0.3, 0.8, 0.1, 0.1)])
+def test_zbt_commutes_with_z_group(L, bJ, k, k2, h):
+    T = physics.zbt_dense(L, bJ, k, k2, h)
+    for g in _z_gens(L, h == 0):
+        perm = np.array([g(s) for s in range(1 << L)])
+        assert np.max(np.abs(T[np.ix_(perm, perm)] - T)) <= 1e-14 * np.max(np.abs(T))
+    # the plain mirror i -> L-1-i is not a symmetry of the diagonal TM
+    rev = np.array([int(format(s, f"0{L}b")[::-1], 2) for s in range(1 << L)])
+    assert np.max(np.abs(T[np.ix_(rev, rev)] - T)) > 1e-3 * np.max(np.abs(T))
+
+
+@pytest.mark.parametrize("L,ext,h", [(8, 0, 0.0), (8, 2, 0.0), (10, 0, 0.1), (10, 0, 0.0),
+                                     (9, 2, 0.0)])
+def test_xdm_projection(oracle, L, ext, h):
+    """The restated XDMMatVecProd2D::perform_op equals (d_o d_o')^-1/2 times
+    the orbit sums of the dense DNNI transfer matrix."""
+    bJ, k = 0.45, 0.35
+    red = oracle.OracleOp(oracle.XDM, L, 0, ext, bJ, k, 0.0, h)
+    censym = h == 0 and (ext & 2) == 0
+    cr, cd, orb = _closure(L, _x_gens(L, censym))
+    np.testing.assert_array_equal(red.reps(), cr)
+    np.testing.assert_array_equal(red.dups(), cd)
+    T = physics.xdt_dense(L, bJ, k, h) * red.normfactor
+    Tr = _projection(T, orb, cd)
+    R = red.dense()
+    assert np.max(np.abs(R - Tr)) / np.max(np.abs(Tr)) < 1e-13
+    assert np.max(np.abs(R - R.T)) / np.max(np.abs(R)) < 1e-13
+
+
+@pytest.mark.parametrize("L,ext,k,k2,h", [(8, 0, 0.5, 0.3, 0.0), (8, 2, 0.2, 0.1, 0.0),
+                                          (10, 0, 0.8, 0.4, 0.1), (10, 0, 0.3, 0.7, 0.0),
+                                          (10, 2, 0.6, 0.0, 0.0)])
+def test_zbm_projection(oracle, L, ext, k, k2, h):
+    """The restated ZBMMatVecProd2D::perform_op_internal (both directions)
+    equals the orbit-sum projection of the dense BNNNI/3NN diagonal TM."""
+    bJ = 0.4
+    red = oracle.OracleOp(oracle.ZBM, L, 0, ext, bJ, k, k2, h)
+    censym = h == 0 and (ext & 2) == 0
+    cr, cd, orb = _closure(L, _z_gens(L, censym))
+    np.testing.assert_array_equal(red.reps(), cr)
+    np.testing.assert_array_equal(red.dups(), cd)
+    T = physics.zbt_dense(L, bJ, k, k2, h) * red.normfactor
+    Tr = _projection(T, orb, cd)
+    R, RT = red.dense(), red.dense(transpose=True)
+    assert np.max(np.abs(R - Tr)) / np.max(np.abs(Tr)) < 1e-13
+    assert np.max(np.abs(RT - Tr.T)) / np.max(np.abs(Tr)) < 1e-13
+
+
+def test_xdm_zbm_lambda0_equal_full(oracle):
+    """The Perron vector lies in the symmetric sector: lambda0 of the reduced
+    operators equals that of the full ones (normfactors agree)."""
+    xdt = oracle.OracleOp(oracle.XDT, 10, 0, 0, 0.5, 0.6, 0.0, 0.0)
+    xdm = oracle.OracleOp(oracle.XDM, 10, 0, 0, 0.5, 0.6, 0.0, 0.0)
+    assert xdm.normfactor == xdt.normfactor
+    wf = np.linalg.eigvalsh(xdt.dense())[-1]
+    wr = np.linalg.eigvalsh(xdm.dense())[-1]
+    assert abs(wr / wf - 1) < 1e-13
+    zbt = oracle.OracleOp(oracle.ZBT, 10, 0, 0, 0.4, 0.6, 0.3, 0.0)
+    zbm = oracle.OracleOp(oracle.ZBM, 10, 0, 0, 0.4, 0.6, 0.3, 0.0)
+    assert abs(zbm.normfactor / zbt.normfactor - 1) < 1e-15
+    wf = max(abs(np.linalg.eigvals(zbt.dense())))
+    wr = max(abs(np.linalg.eigvals(zbm.dense())))
+    assert abs(wr / wf - 1) < 1e-13
