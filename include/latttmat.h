@@ -49,7 +49,9 @@ typedef enum {
   TM_KIND_XAM3D = 5,    /* TMat/tmat3d.cpp:102-340    3D ANNNI reduced          */
   TM_KIND_XAT3D = 6,    /* full 2^(s1*s2) 3D ANNNI (no reference counterpart)   */
   TM_KIND_XDM = 7,      /* TMat/tmatreduce.cpp:1024-1400 DNNI reduced           */
-  TM_KIND_ZBM = 8       /* TMat/tmatreduce.cpp:1403-1868 BNNNI/3NN diagonal reduced */
+  TM_KIND_ZBM = 8,      /* TMat/tmatreduce.cpp:1403-1868 BNNNI/3NN diagonal reduced */
+  TM_KIND_YAT = 9,      /* TMat/tmat.cpp:104-243      ANNNI axial TM, 4^L       */
+  TM_KIND_Y2AT = 10     /* TMat/tmat.cpp:246-310      ANNNI axial TM, layer pairs, 4^L */
 } tm_kind;
 
 /* Model parameters: the tmat input-file keys (TMat/read_input.cpp:29-60). */

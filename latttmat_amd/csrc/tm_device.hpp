@@ -128,17 +128,28 @@ __device__ __forceinline__ Idx3 diag_index_w(const DiagSpec &g, W s)
       r.i1 = popw(W((W(s << 1) | W(1)) ^ s));
       r.i2 = 0;
       break;
+    case DIAG_Y2TV: {
+      const W lf = s & mask, rt = s >> L;
+      r.i1 = popw(W(lf ^ rotrw<W>(lf, L, 1))) + popw(W(rt ^ rotrw<W>(rt, L, 1)));
+      r.i2 = popw(W(lf ^ rt));
+      break;
+    }
+    case DIAG_Y2D:
+      r.i1 = popw(W((s & mask) ^ (s >> L)));
+      r.i2 = 0;
+      break;
     default:
       r.i1 = r.i2 = 0;
   }
   return r;
 }
 
-// 32-bit words up to L = 30: the open-boundary branches shift left by up to
-// two bits without masking and count bits L and L+1 as well
+// 32-bit words up to 30 state bits: the open-boundary branches shift left by
+// up to two bits without masking and count bits L and L+1 as well
 __device__ __forceinline__ Idx3 diag_index(const DiagSpec &g, uint64_t s)
 {
-  return g.L <= 30 ? diag_index_w<uint32_t>(g, (uint32_t)s) : diag_index_w<uint64_t>(g, s);
+  const int bits = g.kind >= DIAG_Y2TV ? 2 * g.L : g.L;
+  return bits <= 30 ? diag_index_w<uint32_t>(g, (uint32_t)s) : diag_index_w<uint64_t>(g, s);
 }
 
 template <typename C>

@@ -160,7 +160,11 @@ enum DiagKind : int {
   DIAG_XDT_PBC = 40,  // DNNI periodic     (tmat.hpp:445-458)
   DIAG_XDT_C5 = 45,   //                   (tmat.hpp:475-487)
   DIAG_XDT_C9 = 49,   //                   (tmat.hpp:489-501)
-  DIAG_ZBT = 60       // BNNNI/3NN         (tmat.hpp:537-556)
+  DIAG_ZBT = 60,      // BNNNI/3NN         (tmat.hpp:537-556)
+  // two-layer states (l | r << L, 2L bits; L is the layer size)
+  DIAG_Y2TV = 70,     // Y2AT pair energy  (tmat.hpp:396-416): i1 = NN bonds in
+                      // both layers, i2 = NN between them, i0 = down spins
+  DIAG_Y2D = 71       // NN between the layers of a pair only (i1)
 };
 
 struct DiagSpec {
@@ -182,6 +186,44 @@ extern int g_num_cu;  // compute units of the device (set at first operator crea
 // Kronecker operator y = Dout o (x)_{i<L} K o Din o x, K = [[ka, kb],[kb, ka]].
 // in/out are FP64 or FP32 (io), compute precision comp.  scale_in points to a
 // device double (input multiplied by it, output divided by it) or is null.
+// Kronecker butterflies on bits [b0, b0 + nbits) of an Ltot-bit index (the
+// other bits ride along): K = [[ka, kb], [kb, ka]] on every transformed bit,
+// optional diagonals on the first load (pro) and the last store (epi).
+// Tables are 192-entry device arrays of the compute type.  mid is a
+// compute-type buffer of 2^Ltot elements for the passes between the first
+// and the last (it may be out, or in when in is compute-type scratch).
+// scale (device double or null): input times *scale if sin, output divided
+// by it if sout.
+struct KronRange {
+  int Ltot, b0, nbits;
+  bool pro, epi;
+  DiagSpec dpro, depi;
+  const void *tpro, *tepi;
+  double ka, kb;
+};
+void launch_kron_range(const KronRange &kr, const void *in, void *out, void *mid,
+                       tm_dtype io_in, tm_dtype io_out, tm_dtype comp, const double *scale,
+                       bool sin, bool sout, bool tune, hipStream_t s);
+
+// ANNNI axial transfer matrices over two-layer states (tm_axial.hip).
+//   YAT:  y = nf D(out) Swap (x)_{high bits} K2 x      (TMat/tmat.cpp:207-242)
+//   YATT: y = nf (x)_{high bits} K2 Swap D(in) x       (TMat/tmat.cpp:126-183)
+//   Y2AT: y = Tv (x)_{low} K2 D1 (x)_{high} K2 Tv x    (TMat/tmat.cpp:265-302)
+// D(a, b) = Tv(layer) K(a, b) from the 192-entry table tab (i1 = NN bonds of
+// the layer, i2 = down spins of the layer, i0 = popcount(a ^ b)); Y2AT uses
+// tab (DIAG_Y2TV, sqrt(nf) folded) and tabb (DIAG_Y2D).
+struct AxialPlan {
+  int kind;  // TM_KIND_YAT or TM_KIND_Y2AT
+  int L;     // layer size; states are 2L bits
+  double ka, kb;  // K2 = [[ka, kb], [kb, ka]]
+  const double *tab64, *tabb64;
+  const float *tab32, *tabb32;
+  bool tune = false;
+};
+void launch_axial(const AxialPlan &plan, const void *in, void *out, tm_dtype io, tm_dtype comp,
+                  int transpose, void *work, const double *scale, hipStream_t s);
+size_t axial_work_bytes(int L, tm_dtype comp);
+
 struct KronPlan {
   int L;
   bool prologue, epilogue;  // apply D on the first / last pass
@@ -407,6 +449,7 @@ struct tm_op {
   // diagonal tables (full-space operators)
   ltm::DiagSpec diag{};
   ltm::DevBuf tab64, tab32;
+  ltm::DevBuf tabb64, tabb32;  // second table (Y2AT: the NN weight between layers)
   // reduced operators
   ltm::DevBuf reps, dups, map, tvec, dt64, tvec32, dt32;
   std::unique_ptr<ltm::FusedTables> ft;

@@ -166,13 +166,15 @@ uint64_t colmask(int s1, int s2, int c)
   return m;
 }
 
-void upload_tables(tm_op *op, const std::vector<double> &t)
+void upload_tables(tm_op *op, const std::vector<double> &t, bool second = false)
 {
   std::vector<float> f(t.begin(), t.end());
-  op->tab64.alloc(192 * sizeof(double));
-  op->tab32.alloc(192 * sizeof(float));
-  copy_sync(op->tab64.get(), t.data(), 192 * sizeof(double), hipMemcpyHostToDevice, op->stream);
-  copy_sync(op->tab32.get(), f.data(), 192 * sizeof(float), hipMemcpyHostToDevice, op->stream);
+  DevBuf &d64 = second ? op->tabb64 : op->tab64;
+  DevBuf &d32 = second ? op->tabb32 : op->tab32;
+  d64.alloc(192 * sizeof(double));
+  d32.alloc(192 * sizeof(float));
+  copy_sync(d64.get(), t.data(), 192 * sizeof(double), hipMemcpyHostToDevice, op->stream);
+  copy_sync(d32.get(), f.data(), 192 * sizeof(float), hipMemcpyHostToDevice, op->stream);
 }
 
 // normfactor of the X-type operators; exponent (units of betaJ) in *e
@@ -409,6 +411,44 @@ tm_op *make_op(const tm_params &p)
     build_reduced(op.get(), g, 0.0, false, true);
   };
 
+  // Axial ANNNI transfer over two-layer states (4^L).  K2 = e^{-bJ k} on
+  // equal spins, e^{bJ k} on opposite ones (the NNN coupling along the axis).
+  auto make_axial = [&](int L, bool pairs) {
+    TM_CHECK(L >= 2 && L <= 15, TM_EINVAL, "axial (Y) width must be in [2, 15] (4^L states)");
+    op->kind = pairs ? TM_KIND_Y2AT : TM_KIND_YAT;
+    op->symmetric = false;
+    op->L = L;
+    op->nstate = op->ncol = int64_t(1) << (2 * L);
+    const double Ld = L;
+    if (!pairs) {
+      // TMat/tmat.cpp:187-205: nf; Tvec = genXATmat2D({bJ, 0, 0, h, L}) as
+      // Tv(layer) K(layer, previous) (tmat.hpp:361-366, tmat.cpp:668-684)
+      op->nf = k < 0.5 ? std::exp(Ld * bJ * (k - 2.0)) : std::exp(-Ld * bJ * (k + 1.0));
+      std::vector<double> t(192, 0.0);
+      for (int n = 0; n <= L; ++n) {
+        t[n] = std::exp(-bJ * (2.0 * n - Ld)) * op->nf;  // NN bonds in the layer
+        t[64 + n] = std::exp(-bJ * h * (2.0 * n - Ld));   // field
+        t[128 + n] = std::exp(-bJ * (2.0 * n - Ld));      // NN to the previous layer
+      }
+      upload_tables(op.get(), t);
+    } else {
+      // TMat/tmat.cpp:246-263, Tv^(1/2) of tmat.hpp:396-416 with sqrt(nf)
+      op->nf = k < 0.5 ? std::exp(2 * Ld * bJ * (k - 2.0)) : std::exp(-2 * Ld * bJ * (k + 1.0));
+      const double J1 = -0.5 * bJ, hJ = -0.5 * bJ * h, sq = std::sqrt(op->nf);
+      std::vector<double> t(192, 0.0), d(192, 1.0);
+      for (int n = 0; n <= 2 * L; ++n) {
+        t[n] = std::exp(J1 * (2.0 * n - 2 * Ld)) * sq;  // NN bonds in both layers
+        t[128 + n] = std::exp(hJ * (2.0 * n - 2 * Ld));  // field on both layers
+      }
+      for (int n = 0; n <= L; ++n) {
+        t[64 + n] = std::exp(J1 * (2.0 * n - Ld));  // NN between the two layers
+        d[n] = std::exp(bJ * (Ld - 2.0 * n));       // old low layer to new high layer
+      }
+      upload_tables(op.get(), t);
+      upload_tables(op.get(), d, true);
+    }
+  };
+
   if (p.model == 'A' && p.dim == 2) {
     const int L = p.size1;
     TM_CHECK(L >= 4, TM_EINVAL, "Invalid size (<4)");  // TMat/read_input.cpp:86-90
@@ -421,8 +461,10 @@ tm_op *make_op(const tm_params &p)
       case 20: make_xat(L, 17); break;
       case 5: make_xam(L, multieigs); break;
       case 13: make_xam(L, multieigs | 1); break;
-      case 0: case 2: case 6:
-        throw Error(TM_EUNSUPPORTED, "ANNNI parallel-TM paths (axialdir 0, 2, 6) are not built");
+      case 0: make_axial(L, true); break;   // TMat/pfunc.cpp:30-31
+      case 2: make_axial(L, false); break;  // TMat/pfunc.cpp:35-36
+      case 6:
+        throw Error(TM_EUNSUPPORTED, "reduced ANNNI axial path (axialdir 6, YAM) is not built");
       default:
         throw Error(TM_EINVAL, "Function yet implemented (unknown axialdir)");
     }
@@ -555,6 +597,22 @@ void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtyp
         work.ensure(sweep_work_bytes(kind, L, comp));
         launch_sweep(sp, xv, yv, io, comp, kind == TM_KIND_ZBT ? transpose : 0, work.get(),
                      scale_in, s);
+        break;
+      }
+      case TM_KIND_YAT:
+      case TM_KIND_Y2AT: {
+        AxialPlan ap{};
+        ap.kind = kind;
+        ap.L = L;
+        ap.ka = std::exp(-p.betaJ * p.kappa);
+        ap.kb = std::exp(p.betaJ * p.kappa);
+        ap.tab64 = tab64.as<double>();
+        ap.tab32 = tab32.as<float>();
+        ap.tabb64 = tabb64.as<double>();
+        ap.tabb32 = tabb32.as<float>();
+        ap.tune = !(p.flags & TM_FLAG_NO_AUTOTUNE);
+        work.ensure(axial_work_bytes(L, comp));
+        launch_axial(ap, xv, yv, io, comp, transpose, work.get(), scale_in, s);
         break;
       }
       case TM_KIND_XAM2D:

@@ -130,7 +130,10 @@ void write_column(std::ostream &os, const std::vector<T> &v)
 // over its representatives, TMat/tmatreduce.cpp:1748-1781).  The reduced DNNI
 // operator reports <|m_st|>/L, <|m|>/L, <m_st^2>/L^2, <m^2>/L^2 instead, m_st
 // the magnetisation with every odd site flipped (TMat/tmatreduce.cpp:1289-1317).
-enum class PhysKind { Sym2D, Sym3D, Gen, Dnni };
+// The axial two-layer operators use the generic moments over all 4^L states
+// with the low layer's magnetisation (TMat/tmatphys.cpp:71-98); Y2AT writes
+// zeros (its moments are a stub, TMat/tmat.cpp:304-309).
+enum class PhysKind { Sym2D, Sym3D, Gen, Dnni, GenLayers, Zero };
 
 int nnode(uint64_t s, int L, int bc)  // TMat/utility.cpp:33-47
 {
@@ -153,8 +156,10 @@ void write_phys(const std::string &fn, PhysKind pk, int L, int bc,
   const size_t n = qr.size();
   for (size_t i = 0; i < n; ++i) {
     const uint64_t s = states.empty() ? (uint64_t)i : states[i];
-    const double m = 2.0 * __builtin_popcountll(s) - L;
-    const double w = pk == PhysKind::Gen ? ql[i] * qr[i] : ql[i] * ql[i];
+    const uint64_t sl = pk == PhysKind::GenLayers ? (s & mask) : s;
+    const double m = 2.0 * __builtin_popcountll(sl) - L;
+    const bool gen = pk == PhysKind::Gen || pk == PhysKind::GenLayers || pk == PhysKind::Zero;
+    const double w = gen ? ql[i] * qr[i] : ql[i] * ql[i];
     dnorm += w;
     double mm = m;
     for (int k = 0; k < 4; ++k) {
@@ -180,7 +185,9 @@ void write_phys(const std::string &fn, PhysKind pk, int L, int bc,
     ext[2] /= (double)L * L;
     ext[3] /= (double)L * L;
   }
-  const double div = pk == PhysKind::Gen ? L * dnorm : (double)L;
+  const double div =
+      (pk == PhysKind::Gen || pk == PhysKind::GenLayers) ? L * dnorm : (double)L;
+  if (pk == PhysKind::Zero) mom[0] = mom[1] = mom[2] = mom[3] = 0.0;
   std::ofstream f(fn);
   f << std::setprecision(10);
   for (int k = 0; k < 4; ++k) f << (k ? "\t" : "") << mom[k] / div;
@@ -319,7 +326,9 @@ int main(int argc, char **argv)
         states.resize(n);
         latttmat::check(tm_op_orbits(op.handle(), states.data(), dups.data()));
       }
-      const PhysKind pk = !sym                    ? PhysKind::Gen
+      const PhysKind pk = kind == TM_KIND_YAT    ? PhysKind::GenLayers
+                          : kind == TM_KIND_Y2AT ? PhysKind::Zero
+                          : !sym                 ? PhysKind::Gen
                           : kind == TM_KIND_XDM ? PhysKind::Dnni
                           : (kind == TM_KIND_XAM3D || kind == TM_KIND_XAT3D) ? PhysKind::Sym3D
                                                                              : PhysKind::Sym2D;

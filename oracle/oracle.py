@@ -15,7 +15,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-XAT, XDT, ZBT, XAM2D, XAM3D, XAT3D, XDM, ZBM = 1, 2, 3, 4, 5, 6, 7, 8
+XAT, XDT, ZBT, XAM2D, XAM3D, XAT3D, XDM, ZBM, YAT, Y2AT = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
 
 _lib = None
 

@@ -562,6 +562,51 @@ or_op *or_op_create(int kind, int size1, int size2, int bc,
       }
       break;
     }
+    case OR_YAT: {
+      /* TMat/tmat.cpp:187-205; Tvec = genXATmat2D({betaJ, 0, 0, h, L})
+       * (tmat.hpp:361-366, tmat.cpp:668-684) flattened column-major:
+       * Tvec[(j << L) | i] = V(i) exp(-betaJ (2 popcount(i ^ j) - L)),
+       * V = XATMatVecProd2D::genTvec with the full betaJ and kappa 0. */
+      int L = size1;
+      if (L < 2 || L > 13) goto fail;
+      op->L = L; op->nstate = op->ncol = 1LL << (2 * L);
+      op->nf = (kappa < 0.5) ? exp(L * betaJ * (kappa - 2.0)) : exp(-L * betaJ * (kappa + 1.0));
+      uint64_t n1 = 1ULL << L;
+      double *V = (double *)malloc(sizeof(double) * n1);
+      op->tvec = (double *)malloc(sizeof(double) * op->ncol);
+      if (!V || !op->tvec) { free(V); goto fail; }
+      or_tvec_xat(L, 0, betaJ, 0.0, h, V);
+      double J1 = -betaJ;
+#pragma omp parallel for schedule(static)
+      for (int64_t j = 0; j < (int64_t)n1; ++j)
+        for (uint64_t i = 0; i < n1; ++i)
+          op->tvec[((uint64_t)j << L) | i] = V[i] * exp(bzexpn(L, POP(i ^ (uint64_t)j), J1));
+      free(V);
+      break;
+    }
+    case OR_Y2AT: {
+      /* TMat/tmat.cpp:246-263 and genTvec tmat.hpp:396-416 (built with
+       * betaJ/2: Tvec is Tv^(1/2)) */
+      int L = size1;
+      if (L < 2 || L > 13) goto fail;
+      op->L = L; op->nstate = op->ncol = 1LL << (2 * L);
+      op->bzwj2[0] = exp(betaJ * (-2.0 * kappa));
+      op->bzwj2[1] = 1.0 / op->bzwj2[0];
+      op->nf = (kappa < 0.5) ? exp(2 * L * betaJ * (kappa - 2.0)) : exp(-2 * L * betaJ * (kappa + 1.0));
+      op->tvec = (double *)malloc(sizeof(double) * op->ncol);
+      if (!op->tvec) goto fail;
+      uint64_t mask = (1ULL << L) - 1;
+      double J1 = -0.5 * betaJ, hJ = -0.5 * betaJ * h;
+#pragma omp parallel for schedule(static)
+      for (int64_t is = 0; is < op->ncol; ++is) {
+        uint64_t st = (uint64_t)is, lf = st & mask, rt = st >> L;
+        int n1 = POP(lf ^ or_offset_state(lf, L, 1)), n2 = POP(rt ^ or_offset_state(rt, L, 1));
+        int n0 = POP(lf ^ rt);
+        int fa = 2 * (n1 + n2 + n0) - 3 * L, fb = 2 * (POP(st) - L);
+        op->tvec[st] = exp(J1 * fa + hJ * fb);
+      }
+      break;
+    }
     case OR_XAM3D: {
       /* TMat/tmat3d.cpp:102-134 and genTvec :179-201 */
       int size = size1 * size2;
@@ -1193,6 +1238,80 @@ static void apply_zbm(const or_op *op, const double *x, double *y, int dir)
   free(p2);
 }
 
+/* TMat/tmat.cpp:207-242 (YATMatVecProd2D::permute_mult_T1, perform_op) and,
+ * for the transpose, :126-183 (YATMatVecProd2D_T).  A state is two layers,
+ * index (s1 << L) | s2.  The reference's final layer swap reads the last
+ * step's buffer, which is y itself after an odd number of steps (odd L): the
+ * swap then overwrites entries it still has to read.  Here the steps always
+ * end in a scratch buffer, so the swap reads intact data for every L. */
+static void apply_yat(const or_op *op, const double *x, double *y, int dir)
+{
+  int L = op->L;
+  uint64_t N = (uint64_t)op->nstate, lm = (1ULL << L) - 1, hb = 1ULL << L;
+  const double *b2 = op->bzwj2, *tv = op->tvec;
+  double *t1 = (double *)malloc(sizeof(double) * N);
+  double *t2 = (double *)malloc(sizeof(double) * N);
+  double *p1 = t1, *p2 = t2, *sw;
+#pragma omp parallel for schedule(static)
+  for (int64_t rp = 0; rp < (int64_t)N; ++rp) p1[rp] = dir ? tv[rp] * x[rp] : x[rp];
+  for (int r = 0; r < L; ++r) {
+#pragma omp parallel for schedule(static)
+    for (int64_t irp = 0; irp < (int64_t)N; ++irp) {
+      uint64_t rp = (uint64_t)irp;
+      if (!dir) {
+        /* Kronecker step on the high layer, then rotate it right by one */
+        uint64_t s1 = rp >> L, s2 = rp & lm;
+        uint64_t rq = (or_offset_state(s1, L, 1) << L) | s2;
+        p2[rq] = b2[0] * p1[rp] + b2[1] * p1[rp ^ hb];
+      } else {
+        /* transposed step on the low layer (rotate left, flip the last bit) */
+        uint64_t rq = rp, s1 = rq & lm, s2 = rq >> L;
+        uint64_t src = (s2 << L) | or_lffset_state(s1, L, 1);
+        uint64_t srcf = (s2 << L) | or_lffset_state(s1 ^ 1ULL, L, 1);
+        p2[rq] = b2[0] * p1[src] + b2[1] * p1[srcf];
+      }
+    }
+    sw = p1; p1 = p2; p2 = sw;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t irp = 0; irp < (int64_t)N; ++irp) {
+    uint64_t rp = (uint64_t)irp, s1 = rp >> L, s2 = rp & lm, rq = (s2 << L) | s1;
+    if (!dir) y[rp] = op->nf * tv[rp] * p1[rq];
+    else y[rq] = op->nf * p1[rp];
+  }
+  free(t1);
+  free(t2);
+}
+
+/* TMat/tmat.cpp:265-302 (Y2ATMatVecProd2D::permute_mult_T1, perform_op) */
+static void apply_y2at(const or_op *op, const double *x, double *y)
+{
+  int L = op->L;
+  uint64_t N = (uint64_t)op->nstate;
+  uint64_t highbit = 1ULL << L, bothbit = highbit | 1, mask = highbit - 1;
+  const double *b1 = op->bzwj1, *b2 = op->bzwj2, *tv = op->tvec;
+  double *t1 = (double *)malloc(sizeof(double) * N);
+  double *t2 = (double *)malloc(sizeof(double) * N);
+  double *p1 = t1, *p2 = t2, *sw;
+#pragma omp parallel for schedule(static)
+  for (int64_t rp = 0; rp < (int64_t)N; ++rp) p1[rp] = tv[rp] * x[rp];
+  for (int r = 0; r < L; ++r) {
+#pragma omp parallel for schedule(static)
+    for (int64_t irp = 0; irp < (int64_t)N; ++irp) {
+      uint64_t rp = (uint64_t)irp, lf = rp & mask, rt = rp >> L;
+      uint64_t rq = (or_offset_state(rt, L, 1) << L) | or_offset_state(lf, L, 1);
+      unsigned jb = (unsigned)((lf ^ rt) & 1);
+      p2[rq] = b1[jb] * (b2[0] * p1[rp] + p1[rp ^ highbit]) +
+               b1[jb ^ 1] * (b2[1] * p1[rp ^ bothbit] + p1[rp ^ 1]);
+    }
+    sw = p1; p1 = p2; p2 = sw;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t rp = 0; rp < (int64_t)N; ++rp) y[rp] = op->nf * tv[rp] * p1[rp];
+  free(t1);
+  free(t2);
+}
+
 void or_op_apply(const or_op *op, const double *x, double *y, int transpose)
 {
   switch (op->kind) {
@@ -1204,6 +1323,8 @@ void or_op_apply(const or_op *op, const double *x, double *y, int transpose)
     case OR_XAM3D: apply_xam3d(op, x, y); break;
     case OR_XDM: apply_xdm(op, x, y); break;
     case OR_ZBM: apply_zbm(op, x, y, transpose ? 1 : 0); break;
+    case OR_YAT: apply_yat(op, x, y, transpose ? 1 : 0); break;
+    case OR_Y2AT: apply_y2at(op, x, y); break;  /* tmat.hpp:382-384: T^T := T */
     default: break;
   }
 }

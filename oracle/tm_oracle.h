@@ -18,6 +18,8 @@
  *   orbit enumeration         brute-force closure, Burnside counts, min-rep invariant
  *   XAM 2D / 3D apply         projection of the dense full T onto orbit sums;
  *                             lambda0(reduced) == lambda0(full)
+ *   YAT / Y2AT apply          dense T of the two-layer transfer built from the
+ *                             axial ANNNI Hamiltonian (tests/physics.py)
  *   XDM / ZBM apply           projection of the dense full XDT / ZBT onto orbit
  *                             sums (ZBM transposed too); lambda0(reduced) ==
  *                             lambda0(full); group generators commute with T
@@ -53,8 +55,12 @@ enum {
   OR_XAT3D = 6,    /* full 2^(s1*s2) 3D ANNNI operator (no reference counterpart);
                       Tv from TMat/tmat3d.cpp:179-201 energies, Th = (x)K            */
   OR_XDM = 7,      /* TMat/tmatreduce.cpp:1024-1400 DNNI reduced by D_L (x flip)     */
-  OR_ZBM = 8       /* TMat/tmatreduce.cpp:1403-1868 BNNNI/3NN diagonal TM reduced by
+  OR_ZBM = 8,      /* TMat/tmatreduce.cpp:1403-1868 BNNNI/3NN diagonal TM reduced by
                       two-site translations and i -> -i (x flip)                     */
+  OR_YAT = 9,      /* TMat/tmat.cpp:104-243   ANNNI axial TM, two layers, 4^L; the
+                      transpose is YATMatVecProd2D_T (tmat.cpp:104-184)             */
+  OR_Y2AT = 10     /* TMat/tmat.cpp:246-310   ANNNI, layer pairs, 4^L; its transposed
+                      instance is the operator itself (tmat.hpp:382-384)            */
 };
 
 typedef struct or_op {

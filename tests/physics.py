@@ -62,6 +62,47 @@ def xdt_dense(L, bJ, k, h):
     return T
 
 
+def yat_dense(L, bJ, k, h):
+    """ANNNI transfer along the axial (NNN) direction over two-layer states
+    (previous layer p, current layer c), index (p << L) | c; one step adds
+    layer n coupled to c (NN) and p (NNN, -kappa), with its own in-layer NN
+    bonds and field (without the normfactor)."""
+    n1 = 1 << L
+    lay = np.array([bJ * pairsum(s ^ rotr(s, L, 1), L) + h * bJ * pairsum(s, L)
+                    for s in range(n1)])
+    cr = np.array([[cross_nn(a, b, L) for b in range(n1)] for a in range(n1)], float)
+    T = np.zeros((n1 * n1, n1 * n1))
+    for p in range(n1):          # new state (c_old, n): previous = old current
+        for n in range(n1):
+            for q in range(n1):  # old state (q, p)
+                T[(p << L) | n, (q << L) | p] = np.exp(lay[n] + bJ * cr[n, p] - k * bJ * cr[n, q])
+    return T
+
+
+def y2at_dense(L, bJ, k, h):
+    """ANNNI transfer by two axial layers over layer pairs, index
+    (r << L) | l with r the older and l the newer layer of the pair: NN
+    between the new pair's older layer and the old pair's newer one, NNN
+    (-kappa) between equal positions of the two pairs; each pair's internal
+    energy split half to each side (without the normfactor)."""
+    n1 = 1 << L
+    N = n1 * n1
+    e = np.empty(N)
+    for s in range(N):
+        lf, rt = s & (n1 - 1), s >> L
+        e[s] = (bJ * (pairsum(lf ^ rotr(lf, L, 1), L) + pairsum(rt ^ rotr(rt, L, 1), L)
+                      + pairsum(lf ^ rt, L)) + h * bJ * pairsum(s, 2 * L))
+    cr = np.array([[cross_nn(a, b, L) for b in range(n1)] for a in range(n1)], float)
+    T = np.empty((N, N))
+    for new in range(N):
+        nl, nr = new & (n1 - 1), new >> L
+        for old in range(N):
+            ol, orr = old & (n1 - 1), old >> L
+            T[new, old] = np.exp(0.5 * e[new] + 0.5 * e[old] + bJ * cr[nr, ol]
+                                 - k * bJ * (cr[nl, ol] + cr[nr, orr]))
+    return T
+
+
 def zbt_dense(L, bJ, k, k2, h):
     """BNNNI + 3NN on the diagonal TM (row index = layer a, column = b)."""
     N = 1 << L

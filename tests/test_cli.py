@@ -280,3 +280,48 @@ def test_cli_zbm_left_and_phys(tmp_path, oracle):
     m = 2 * np.array([bin(int(v)).count("1") for v in D.reps()], float) - L
     ref = [float(np.sum(m ** k * wgt) / (L * wgt.sum())) for k in range(1, 5)]
     np.testing.assert_allclose(mom, ref, rtol=1e-8, atol=1e-10)
+
+
+@pytest.mark.gpu
+def test_cli_axial_outputs(tmp_path, oracle):
+    """Axial ANNNI over two-layer states: YAT (axialdir 2) writes its left
+    vector from the transpose class and the generic moments over 4^L states
+    with the low layer's magnetisation; Y2AT (axialdir 0) takes its own
+    operator as the transpose (TMat/tmat.hpp:382-384) and writes zero
+    moments (TMat/tmat.cpp:304-309)."""
+    L = 5
+    d1, d2 = tmp_path / "yat", tmp_path / "y2at"
+    d1.mkdir()
+    d2.mkdir()
+    p1 = write_input(str(d1), axialdir=2, size=L, savevec="1 1", out="Y", J=0.45, kappa="0.3 0")
+    p2 = write_input(str(d2), axialdir=0, size=L, savevec="1 1", out="W", J=0.45, kappa="0.3 0")
+    r1 = run([p1], str(d1))
+    r2 = run([p2], str(d2))
+    print(r1.stdout, r1.stderr, r2.stdout, r2.stderr, flush=True)
+    assert r1.returncode == 0 and r2.returncode == 0
+    D = oracle.OracleOp(oracle.YAT, L, 0, 0, 0.45, 0.3, 0.0, 0.0)
+    T = D.dense()
+    w = np.linalg.eigvals(T)
+    lam = w[np.argmax(np.abs(w))].real
+    re_, im_ = open(d1 / "Y_val.dat").read().strip("()").split(",")
+    assert abs(float(re_) / (lam / D.normfactor) - 1) < 1e-10 and float(im_) == 0.0
+    vr = np.fromfile(d1 / "Y_vec.dat", dtype=np.float64)
+    vl = np.fromfile(d1 / "Y_vecl.dat", dtype=np.float64)
+    assert vr.size == vl.size == 1 << (2 * L)
+    assert np.linalg.norm(T @ vr - lam * vr) / abs(lam) < 1e-8
+    assert np.linalg.norm(T.T @ vl - lam * vl) / abs(lam) < 1e-8
+    mom = [float(x) for x in open(d1 / "Y_phys.dat").read().split("\t")]
+    wgt = vl * vr
+    s = np.arange(1 << (2 * L))
+    m = 2 * np.array([bin(int(v) & ((1 << L) - 1)).count("1") for v in s], float) - L
+    ref = [float(np.sum(m ** k * wgt) / (L * wgt.sum())) for k in range(1, 5)]
+    np.testing.assert_allclose(mom, ref, rtol=1e-8, atol=1e-10)
+    # Y2AT
+    D2 = oracle.OracleOp(oracle.Y2AT, L, 0, 0, 0.45, 0.3, 0.0, 0.0)
+    w2 = np.linalg.eigvals(D2.dense())
+    lam2 = w2[np.argmax(np.abs(w2))].real
+    re_, im_ = open(d2 / "W_val.dat").read().strip("()").split(",")
+    assert abs(float(re_) / (lam2 / D2.normfactor) - 1) < 1e-10
+    chk = [ln for ln in r2.stdout.splitlines() if ln.startswith("Eigensolver check: ")]
+    assert len(chk) == 1 and abs(float(chk[0].split()[-1]) - 1) < 1e-9
+    assert open(d2 / "W_phys.dat").read() == "0\t0\t0\t0"

@@ -387,3 +387,25 @@ def test_xdm_zbm_lambda0_equal_full(oracle):
     wf = max(abs(np.linalg.eigvals(zbt.dense())))
     wr = max(abs(np.linalg.eigvals(zbm.dense())))
     assert abs(wr / wf - 1) < 1e-13
+
+
+@pytest.mark.parametrize("L,bJ,k,h", [(3, 0.5, 0.3, 0.0), (4, 0.4, 0.6, 0.1), (5, 0.3, 0.2, 0.0)])
+def test_yat_y2at_dense_hamiltonian(oracle, L, bJ, k, h):
+    """The restated YAT (and its transpose class) and Y2AT against the
+    two-layer transfer matrices built from the axial ANNNI Hamiltonian; odd L
+    included (the restatement keeps the final layer swap out of place)."""
+    yat = oracle.OracleOp(oracle.YAT, L, 0, 0, bJ, k, 0.0, h)
+    D = physics.yat_dense(L, bJ, k, h) * yat.normfactor
+    T = yat.dense()
+    assert np.max(np.abs(T - D)) / np.max(np.abs(D)) < 1e-14
+    assert np.max(np.abs(yat.dense(transpose=True) - T.T)) / np.max(np.abs(D)) < 1e-14
+    y2 = oracle.OracleOp(oracle.Y2AT, L, 0, 0, bJ, k, 0.0, h)
+    D2 = physics.y2at_dense(L, bJ, k, h) * y2.normfactor
+    T2 = y2.dense()
+    assert np.max(np.abs(T2 - D2)) / np.max(np.abs(D2)) < 1e-14
+    # the reference's transposed Y2AT instance is the operator itself
+    assert np.array_equal(y2.dense(transpose=True), T2)
+    # two axial layers per step: lambda0(Y2AT) / nf = (lambda0(YAT) / nf)^2
+    l1 = max(abs(np.linalg.eigvals(T))) / yat.normfactor
+    l2 = max(abs(np.linalg.eigvals(T2))) / y2.normfactor
+    assert abs(l1 * l1 / l2 - 1) < 1e-13
