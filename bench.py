@@ -45,9 +45,11 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="host threads of the scan batch")
     ap.add_argument("--profile-tx", action="store_true",
                     help="only a calibration copy and a few T.x applies (for counter runs)")
-    ap.add_argument("--workload", default="reduced", choices=("reduced", "scan", "3d", "grad", "small"),
+    ap.add_argument("--workload", default="reduced",
+                    choices=("reduced", "scan", "3d", "grad", "small", "ops"),
                     help="reduced: the headline line; scan: 96-instance BNNNI batch; "
-                         "3d: 5x5 3D layer, reduced vs full space")
+                         "3d: 5x5 3D layer, reduced vs full space; ops: T.x of every "
+                         "operator kind against the HBM roofline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of this "
@@ -66,6 +68,8 @@ def main():
         return bench_grad(lt, args)
     if args.workload == "small":
         return bench_small(lt, args)
+    if args.workload == "ops":
+        return bench_ops(lt, args)
 
     L, bJ, kappa, neigs = args.L, 0.4, 0.5, 4
     t0 = time.perf_counter()
@@ -378,6 +382,56 @@ def bench_small(lt, args):
         D = O.OracleOp(O.XAT, 12, 0, 0, 0.5, 0.6, 0.0, 0.0).dense()
         w = np.linalg.eigvalsh(0.5 * (D + D.T))
         out["rel_err_vs_dense_lapack"] = abs(r.evals[0] / w[-1] - 1)
+    print(json.dumps(out), flush=True)
+
+
+# (label, TransferMatrix args, kwargs): one operator of every kind at a size
+# where its vectors are resident in HBM well beyond the caches
+# (label, expanded states, TransferMatrix args, kwargs): one operator of
+# every kind at a size whose vectors are far larger than the caches
+OPS = [
+    ("XAT L=26", 1 << 26, ("A", 2, 1, 26), dict(J=0.4, kappa=0.5)),
+    ("XDT L=24", 1 << 24, ("D", 2, 1, 24), dict(J=0.6, kappa=0.7, h=0.1)),
+    ("ZBT L=24", 1 << 24, ("B", 2, 3, 24), dict(J=0.4, kappa=0.6, kappa2=0.3)),
+    ("XAM L=28 (fused)", 1 << 28, ("A", 2, 5, 28), dict(J=0.4, kappa=0.5, neigs=4)),
+    ("XAM3D 5x5", 1 << 25, ("A", 3, 1, (5, 5)), dict(J=0.25, kappa=0.4)),
+    ("XAT3D 5x5", 1 << 25, ("A", 3, 1, (5, 5)), dict(J=0.25, kappa=0.4, flags=1)),
+    ("XDM L=26", 1 << 26, ("D", 2, 5, 26), dict(J=0.6, kappa=0.7)),
+    ("ZBM L=26", 1 << 26, ("B", 2, 7, 26), dict(J=0.4, kappa=0.6, kappa2=0.3)),
+    ("YAT L=13 (4^13)", 1 << 26, ("A", 2, 2, 13), dict(J=0.4, kappa=0.5)),
+    ("Y2AT L=13 (4^13)", 1 << 26, ("A", 2, 0, 13), dict(J=0.4, kappa=0.5)),
+]
+
+
+def bench_ops(lt, args):
+    """T.x of every operator kind: mean device time over HIP events on the
+    operator's stream (vectors resident), and the algorithmic HBM rate of
+    one vector in and one out (reduced operators: the survey's figure
+    2^L (s_in + s_out) + M (s_in + s_out + 9), the expanded space charged once)."""
+    rows = []
+    for label, nstate, a, kw in OPS:
+        op = lt.TransferMatrix(*a, **kw)
+        n = op.cols()
+        reduced = n != nstate
+        r = {"op": label, "kind": op.kind(), "cols": n}
+        for name, io, comp in (("fp64", lt.TM_F64, lt.TM_F64), ("fp32", lt.TM_F64, lt.TM_F32),
+                               ("fp32_io32", lt.TM_F32, lt.TM_F32)):
+            ms = op.time_apply(io, comp, warmup=3, reps=10)
+            s = 8 if io == lt.TM_F64 else 4
+            alg = (nstate * 2 * s + n * (2 * s + 9)) if reduced else n * 2 * s
+            r[name + "_ms"] = round(ms, 4)
+            r[name + "_alg_GBs"] = round(alg / (ms * 1e-3) / 1e9, 1)
+        if not op.is_symmetric() and op.kind() not in ("Y2AT",):
+            ms = op.time_apply(lt.TM_F64, lt.TM_F64, transpose=True, warmup=3, reps=10)
+            r["fp64_transpose_ms"] = round(ms, 4)
+        log(f"[ops] {json.dumps(r)}")
+        rows.append(r)
+        op.close()
+    out = {"metric": "T.x time and algorithmic HBM GB/s per operator kind",
+           "value": len(rows), "unit": "operators", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f64 / f32", "data": "synthetic (model parameters; uniform vectors)",
+           "config": {"workload": "one T.x of each operator kind, vectors resident in HBM"},
+           "hbm_peak_GBs": 8000.0, "ops": rows}
     print(json.dumps(out), flush=True)
 
 
