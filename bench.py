@@ -400,6 +400,8 @@ OPS = [
     ("ZBM L=26", 1 << 26, ("B", 2, 7, 26), dict(J=0.4, kappa=0.6, kappa2=0.3)),
     ("YAT L=13 (4^13)", 1 << 26, ("A", 2, 2, 13), dict(J=0.4, kappa=0.5)),
     ("Y2AT L=13 (4^13)", 1 << 26, ("A", 2, 0, 13), dict(J=0.4, kappa=0.5)),
+    ("YAM L=13", 1 << 26, ("A", 2, 6, 13), dict(J=0.4, kappa=0.5)),
+    ("YBM L=13", 1 << 26, ("B", 2, 6, 13), dict(J=0.4, kappa=0.5, kappa2=0.3)),
 ]
 
 

@@ -51,7 +51,9 @@ typedef enum {
   TM_KIND_XDM = 7,      /* TMat/tmatreduce.cpp:1024-1400 DNNI reduced           */
   TM_KIND_ZBM = 8,      /* TMat/tmatreduce.cpp:1403-1868 BNNNI/3NN diagonal reduced */
   TM_KIND_YAT = 9,      /* TMat/tmat.cpp:104-243      ANNNI axial TM, 4^L       */
-  TM_KIND_Y2AT = 10     /* TMat/tmat.cpp:246-310      ANNNI axial TM, layer pairs, 4^L */
+  TM_KIND_Y2AT = 10,    /* TMat/tmat.cpp:246-310      ANNNI axial TM, layer pairs, 4^L */
+  TM_KIND_YAM = 11,     /* TMat/tmatreduce.cpp:661-1015 ANNNI axial TM reduced  */
+  TM_KIND_YBM = 12      /* TMat/tmatreduce.cpp:1017-1021 BNNNI axial TM reduced */
 } tm_kind;
 
 /* Model parameters: the tmat input-file keys (TMat/read_input.cpp:29-60). */

@@ -17,7 +17,7 @@ TM_FLAG_FULLSPACE = 1
 TM_FLAG_GENERIC_REDUCED = 2
 TM_FLAG_NO_AUTOTUNE = 4
 KIND_NAMES = {1: "XAT", 2: "XDT", 3: "ZBT", 4: "XAM2D", 5: "XAM3D", 6: "XAT3D", 7: "XDM",
-              8: "ZBM", 9: "YAT", 10: "Y2AT"}
+              8: "ZBM", 9: "YAT", 10: "Y2AT", 11: "YAM", 12: "YBM"}
 
 # every function the header declares (checked by the CPU tests)
 EXPORTS = [

@@ -264,8 +264,9 @@ tm_status tm_grad_hf(const tm_op *cop, const double *l, const double *r, double 
                      double dlnnf[4])
 {
   if (!cop || !l || !r || !dlam) return fail(TM_EINVAL, "null argument");
-  if (cop->kind == TM_KIND_YAT || cop->kind == TM_KIND_Y2AT)
-    return fail(TM_EUNSUPPORTED, "no dual-number kernels for the axial (4^L) operators");
+  if (cop->kind == TM_KIND_YAT || cop->kind == TM_KIND_Y2AT || cop->kind == TM_KIND_YAM ||
+      cop->kind == TM_KIND_YBM)
+    return fail(TM_EUNSUPPORTED, "no dual-number kernels for the axial (two-layer) operators");
   tm_op *op = const_cast<tm_op *>(cop);
   return guarded([&] {
     const size_t bytes = sizeof(double) * op->ncol;

@@ -27,16 +27,17 @@ constexpr int SW_T = 64;         // tile edge
 constexpr int SW_THREADS = 256;
 
 // out[(a << L) | b] = f(a, b) in[(b << L) | a] over one 64 x 64 tile (or the
-// whole 2^L x 2^L block when L < 6), with f(a, b) = tab[NN(v)] tab[64 +
-// pop(v)] tab[128 + pop(a ^ b)], v = b (tv_hi = 0) or a (tv_hi = 1).
+// whole 2^L x 2^L block when L < 6), with f = w_layers(b, a) (wsel 0),
+// w_layers(a, b) (wsel 1) or 1 (wsel 2).
 template <typename C, typename TI, typename TO>
 __global__ __launch_bounds__(SW_THREADS) void k_layer_swap(
-    const TI *__restrict__ in, TO *__restrict__ out, int L, const C *__restrict__ tab, int tv_hi,
-    const double *__restrict__ scale, int sflags)
+    const TI *__restrict__ in, TO *__restrict__ out, int L, const C *__restrict__ tab, int wsel,
+    int ybm, const double *__restrict__ scale, int sflags)
 {
   __shared__ C tile[SW_T][SW_T + 1];
-  __shared__ C stab[192];
-  for (int i = threadIdx.x; i < 192; i += blockDim.x) stab[i] = tab[i];
+  __shared__ C stab[320];
+  const int ntab = wsel == 2 ? 0 : (ybm ? 320 : 192);
+  for (int i = threadIdx.x; i < ntab; i += blockDim.x) stab[i] = tab[i];
   const int n1 = 1 << L;
   const int ts = n1 < SW_T ? n1 : SW_T;
   const int per = n1 / ts;  // tiles per layer index
@@ -53,39 +54,38 @@ __global__ __launch_bounds__(SW_THREADS) void k_layer_swap(
     tile[rb][ca] = (C)in[((uint64_t)(b0 + rb) << L) | (a0 + ca)];
   }
   __syncthreads();
-  const uint32_t m = (uint32_t)n1 - 1;
   // store rows a of the output, contiguous in b
   for (int i = threadIdx.x; i < ts * ts; i += blockDim.x) {
     const int ra = i / ts, cb = i % ts;
     const uint32_t a = a0 + ra, b = b0 + cb;
-    const uint32_t v = tv_hi ? a : b;
-    const uint32_t rv = ((v >> 1) | ((v & 1u) << (L - 1))) & m;
-    const C f = stab[__popc(v ^ rv)] * stab[64 + __popc(v)] * stab[128 + __popc(a ^ b)];
-    out[((uint64_t)a << L) | b] = (TO)(tile[cb][ra] * f * sc);
+    C f = sc;
+    if (wsel == 0) f *= dev::w_layers<C>(b, a, L, stab, ybm);
+    else if (wsel == 1) f *= dev::w_layers<C>(a, b, L, stab, ybm);
+    out[((uint64_t)a << L) | b] = (TO)(tile[cb][ra] * f);
   }
 }
 
 template <typename C, typename TI, typename TO>
-void layer_swap(const void *in, void *out, int L, const C *tab, int tv_hi, const double *scale,
-                int sflags, hipStream_t s)
+void layer_swap(const void *in, void *out, int L, const C *tab, int wsel, int ybm,
+                const double *scale, int sflags, hipStream_t s)
 {
   const int n1 = 1 << L;
   const int ts = n1 < SW_T ? n1 : SW_T;
   const uint64_t nblk = (uint64_t)(n1 / ts) * (uint64_t)(n1 / ts);
   hipLaunchKernelGGL((k_layer_swap<C, TI, TO>), dim3((unsigned)nblk), dim3(SW_THREADS), 0, s,
-                     static_cast<const TI *>(in), static_cast<TO *>(out), L, tab, tv_hi, scale,
-                     sflags);
+                     static_cast<const TI *>(in), static_cast<TO *>(out), L, tab, wsel, ybm,
+                     scale, sflags);
   TM_LAUNCH_CHECK();
 }
 
 template <typename C>
-void swap_io(const void *in, void *out, tm_dtype ti, tm_dtype to, int L, const C *tab, int tv_hi,
-             const double *scale, int sflags, hipStream_t s)
+void swap_io(const void *in, void *out, tm_dtype ti, tm_dtype to, int L, const C *tab, int wsel,
+             int ybm, const double *scale, int sflags, hipStream_t s)
 {
-  if (ti == TM_F64 && to == TM_F64) layer_swap<C, double, double>(in, out, L, tab, tv_hi, scale, sflags, s);
-  else if (ti == TM_F64) layer_swap<C, double, float>(in, out, L, tab, tv_hi, scale, sflags, s);
-  else if (to == TM_F64) layer_swap<C, float, double>(in, out, L, tab, tv_hi, scale, sflags, s);
-  else layer_swap<C, float, float>(in, out, L, tab, tv_hi, scale, sflags, s);
+  if (ti == TM_F64 && to == TM_F64) layer_swap<C, double, double>(in, out, L, tab, wsel, ybm, scale, sflags, s);
+  else if (ti == TM_F64) layer_swap<C, double, float>(in, out, L, tab, wsel, ybm, scale, sflags, s);
+  else if (to == TM_F64) layer_swap<C, float, double>(in, out, L, tab, wsel, ybm, scale, sflags, s);
+  else layer_swap<C, float, float>(in, out, L, tab, wsel, ybm, scale, sflags, s);
 }
 
 template <typename C>
@@ -101,10 +101,10 @@ void axial_typed(const AxialPlan &p, const void *in, void *out, tm_dtype io, int
     if (!transpose) {
       // z = (x)_{high} K2 x into the workspace, then the swap with Tv(low)
       launch_kron_range(hi, in, work, work, io, ct, ct, scale, true, false, p.tune, s);
-      swap_io<C>(work, out, ct, io, L, tab, 0, scale, 2, s);
+      swap_io<C>(work, out, ct, io, L, tab, 0, 0, scale, 2, s);
     } else {
       // the swap with Tv(high) into the workspace, then (x)_{high} K2
-      swap_io<C>(in, work, io, ct, L, tab, 1, scale, 1, s);
+      swap_io<C>(in, work, io, ct, L, tab, 1, 0, scale, 1, s);
       const bool out_c = io == ct;
       launch_kron_range(hi, work, out, out_c ? out : work, ct, io, ct, scale, false, true,
                         p.tune, s);
@@ -126,6 +126,18 @@ void axial_typed(const AxialPlan &p, const void *in, void *out, tm_dtype io, int
 }
 
 }  // namespace
+
+void launch_layer_swap(const void *in, void *out, tm_dtype ti, tm_dtype to, tm_dtype comp, int L,
+                       const void *tab, int wsel, int ybm, const double *scale, int sflags,
+                       hipStream_t s)
+{
+  if (comp == TM_F64)
+    swap_io<double>(in, out, ti, to, L, static_cast<const double *>(tab), wsel, ybm, scale,
+                    sflags, s);
+  else
+    swap_io<float>(in, out, ti, to, L, static_cast<const float *>(tab), wsel, ybm, scale,
+                   sflags, s);
+}
 
 size_t axial_work_bytes(int L, tm_dtype comp)
 {

@@ -159,6 +159,26 @@ __device__ __forceinline__ C diag_value(const DiagSpec &g, const C *tab, uint64_
   return tab[i.i1] * tab[64 + i.i2] * tab[128 + i.i0];
 }
 
+// Half the two-layer Boltzmann weight W(c, p) of the symmetrised axial
+// transfer (the YAM/YBM Tvec without 1/sqrt(dup), TMat/tmatreduce.cpp:717-780;
+// also the YAT diagonal Tv(c) K(c, p)), c the current layer, p the previous
+// one, from a popcount table: [0,64) NN bonds in c, [64,128) down spins of c,
+// [128,192) bonds between c and p; with ybm also [192,256) NNN bonds in c and
+// [256,320) the two diagonal bond sets between c and p.
+template <typename C>
+__device__ __forceinline__ C w_layers(uint32_t c, uint32_t p, int L, const C *tab, int ybm)
+{
+  const uint32_t m = (L >= 32) ? ~0u : ((1u << L) - 1);
+  const uint32_t r1 = ((c >> 1) | (c << (L - 1))) & m;
+  C w = tab[__popc(c ^ r1)] * tab[64 + __popc(c)] * tab[128 + __popc(c ^ p)];
+  if (ybm) {
+    const uint32_t r2 = ((c >> 2) | (c << (L - 2))) & m;
+    const uint32_t pr = ((p >> 1) | (p << (L - 1))) & m, pl = ((p << 1) | (p >> (L - 1))) & m;
+    w *= tab[192 + __popc(c ^ r2)] * tab[256 + __popc(c ^ pr) + __popc(c ^ pl)];
+  }
+  return w;
+}
+
 // LDS swizzle for 8-byte elements: XOR the low 5 index bits with the next 5.
 // A thread that owns 16 consecutive elements (or 16 elements strided by 16)
 // then touches distinct banks across its wavefront.  Involution, identity

@@ -223,6 +223,12 @@ struct AxialPlan {
 void launch_axial(const AxialPlan &plan, const void *in, void *out, tm_dtype io, tm_dtype comp,
                   int transpose, void *work, const double *scale, hipStream_t s);
 size_t axial_work_bytes(int L, tm_dtype comp);
+// out[(a << L) | b] = f in[(b << L) | a] with f = W(b, a) (wsel 0), W(a, b)
+// (wsel 1) or 1 (wsel 2), W = dev::w_layers over tab (compute type; 192
+// entries, 320 with ybm); sflags 1: times *scale, 2: divided by it.
+void launch_layer_swap(const void *in, void *out, tm_dtype ti, tm_dtype to, tm_dtype comp, int L,
+                       const void *tab, int wsel, int ybm, const double *scale, int sflags,
+                       hipStream_t s);
 
 struct KronPlan {
   int L;
@@ -276,6 +282,9 @@ struct GroupSpec {
   int zgroup;       // 2D periodic: translations by two sites and the
                     // reflection i -> -i (genZ2DDuplist, TMat/tmatreduce.cpp:242-334)
                     // instead of all translations and i -> L-1-i
+  int ylayer;       // > 0: two-layer states of this layer size (L = 2 ylayer
+                    // bits): rotations and reflection of both layers
+                    // together (genY2DDuplist, TMat/tmatreduce.cpp:114-239)
 };
 // Enumerate orbits on the device.  Fills reps (ascending) and dups (orbit
 // sizes), returns M.  If map != nullptr it receives map[s] = orbit index of s.
@@ -333,6 +342,13 @@ struct ReducedPlan {
   // Kronecker product of K = [[ka, kb], [kb, ka]] (XAM)
   const SweepPlan *sweep;
   int transpose;
+  // > 0: symmetrised axial transfer over two-layer states of this layer
+  // size (YAM, YBM): K2 = [[ka, kb], [kb, ka]] on the previous layer, the
+  // layer swap, W from wtab (dev::w_layers)
+  int ylayer;
+  int ybm;
+  const double *wtab64;
+  const float *wtab32;
 };
 void launch_reduced(const ReducedPlan &plan, const void *in, void *out, tm_dtype io,
                     tm_dtype comp, void *work, const double *scale_in, hipStream_t s);
@@ -481,7 +497,8 @@ namespace ltm {
 // Operators over orbit representatives (the reduced matrices).
 inline bool is_reduced(tm_kind k)
 {
-  return k == TM_KIND_XAM2D || k == TM_KIND_XAM3D || k == TM_KIND_XDM || k == TM_KIND_ZBM;
+  return k == TM_KIND_XAM2D || k == TM_KIND_XAM3D || k == TM_KIND_XDM || k == TM_KIND_ZBM ||
+         k == TM_KIND_YAM || k == TM_KIND_YBM;
 }
 // Construct an operator from parameters (pfunc dispatch); throws Error.
 tm_op *make_op(const tm_params &p);

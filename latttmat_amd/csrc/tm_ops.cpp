@@ -171,10 +171,10 @@ void upload_tables(tm_op *op, const std::vector<double> &t, bool second = false)
   std::vector<float> f(t.begin(), t.end());
   DevBuf &d64 = second ? op->tabb64 : op->tab64;
   DevBuf &d32 = second ? op->tabb32 : op->tab32;
-  d64.alloc(192 * sizeof(double));
-  d32.alloc(192 * sizeof(float));
-  copy_sync(d64.get(), t.data(), 192 * sizeof(double), hipMemcpyHostToDevice, op->stream);
-  copy_sync(d32.get(), f.data(), 192 * sizeof(float), hipMemcpyHostToDevice, op->stream);
+  d64.alloc(t.size() * sizeof(double));
+  d32.alloc(t.size() * sizeof(float));
+  copy_sync(d64.get(), t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, op->stream);
+  copy_sync(d32.get(), f.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, op->stream);
 }
 
 // normfactor of the X-type operators; exponent (units of betaJ) in *e
@@ -449,6 +449,43 @@ tm_op *make_op(const tm_params &p)
     }
   };
 
+  // Reduced axial transfer over two-layer states (c current, p previous):
+  // T_red = P^T T P with T(n, c; c, p) = W(n, c) K2(n, p) W(c, p) and P over
+  // the orbits of rotating / reflecting (/ flipping) both layers together
+  // (YAMMatVecProd2D, TMat/tmatreduce.cpp:661-949; YBM = layertype 1,
+  // :1017-1021).  W carries half of the layer's bonds and field and of the
+  // bonds to the previous layer, sqrt(nf) folded in (genTvec :717-745,
+  // genTvec1 :748-780).
+  auto make_yam = [&](int L, int ext, bool ybm) {
+    TM_CHECK(L >= 4 && L <= 15, TM_EINVAL, "reduced axial (Y) width must be in [4, 15]");
+    op->kind = ybm ? TM_KIND_YBM : TM_KIND_YAM;
+    op->symmetric = false;
+    op->L = L;
+    op->nstate = int64_t(1) << (2 * L);
+    op->censym = (h == 0.0 && (ext & 2) == 0);  // TMat/tmatreduce.cpp:664
+    const double Ld = L, k2 = p.kappa2;
+    double nfe;
+    if (!ybm) nfe = k < 0.5 ? Ld * (k - 2.0) : -Ld * (k + 1.0);
+    else if (k + k2 < 0.5) nfe = Ld * 2.0 * (k + k2 - 1.0);
+    else if (k < 0.5 && k2 - k < 0.5) nfe = -Ld;
+    else if (k2 < 2 * k) nfe = -Ld * 2.0 * k;
+    else nfe = Ld * 2.0 * (k - k2);
+    op->nf = std::exp(bJ * nfe);
+    const double hb = 0.5 * bJ, sq = std::exp(hb * nfe);
+    std::vector<double> t(ybm ? 320 : 192, 0.0);
+    for (int n = 0; n <= L; ++n) {
+      t[n] = std::exp(hb * (Ld - 2.0 * n)) * sq;      // NN bonds in the layer
+      t[64 + n] = std::exp(hb * h * (Ld - 2.0 * n));   // field
+      t[128 + n] = std::exp(hb * (Ld - 2.0 * n));      // bonds to the previous layer
+      if (ybm) t[192 + n] = std::exp(-hb * k * (Ld - 2.0 * n));  // NNN in the layer
+    }
+    if (ybm)
+      for (int n = 0; n <= 2 * L; ++n) t[256 + n] = std::exp(-hb * k2 * (2 * Ld - 2.0 * n));
+    upload_tables(op.get(), t);
+    GroupSpec g{2, 2 * L, 0, 0, 0, op->censym, 0, 0, L};
+    build_reduced(op.get(), g, 0.0, false, true);
+  };
+
   if (p.model == 'A' && p.dim == 2) {
     const int L = p.size1;
     TM_CHECK(L >= 4, TM_EINVAL, "Invalid size (<4)");  // TMat/read_input.cpp:86-90
@@ -463,8 +500,7 @@ tm_op *make_op(const tm_params &p)
       case 13: make_xam(L, multieigs | 1); break;
       case 0: make_axial(L, true); break;   // TMat/pfunc.cpp:30-31
       case 2: make_axial(L, false); break;  // TMat/pfunc.cpp:35-36
-      case 6:
-        throw Error(TM_EUNSUPPORTED, "reduced ANNNI axial path (axialdir 6, YAM) is not built");
+      case 6: make_yam(L, multieigs, false); break;  // TMat/pfunc.cpp:41-42
       default:
         throw Error(TM_EINVAL, "Function yet implemented (unknown axialdir)");
     }
@@ -497,8 +533,7 @@ tm_op *make_op(const tm_params &p)
     switch (p.axialdir) {
       case 3: make_zbt(p.size1); break;
       case 7: make_zbm(p.size1, multieigs); break;
-      case 6:
-        throw Error(TM_EUNSUPPORTED, "BNNNI parallel-TM reduced path (axialdir 6) is not built");
+      case 6: make_yam(p.size1, multieigs, true); break;  // TMat/pfunc.cpp:81
       default:
         throw Error(TM_EINVAL, "Function yet implemented (unknown axialdir)");
     }
@@ -618,7 +653,9 @@ void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtyp
       case TM_KIND_XAM2D:
       case TM_KIND_XAM3D:
       case TM_KIND_XDM:
-      case TM_KIND_ZBM: {
+      case TM_KIND_ZBM:
+      case TM_KIND_YAM:
+      case TM_KIND_YBM: {
         ReducedPlan rp{};
         rp.M = ncol;
         rp.L = L;
@@ -636,6 +673,16 @@ void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtyp
           sp = sweep_plan();
           rp.sweep = &sp;
           rp.transpose = kind == TM_KIND_ZBM ? transpose : 0;
+        }
+        if (kind == TM_KIND_YAM || kind == TM_KIND_YBM) {
+          rp.L = 2 * L;
+          rp.ylayer = L;
+          rp.ybm = kind == TM_KIND_YBM;
+          rp.ka = bzwj2[0];  // K2 on the previous layer (permute_mult_TL)
+          rp.kb = bzwj2[1];
+          rp.wtab64 = tab64.as<double>();
+          rp.wtab32 = tab32.as<float>();
+          rp.transpose = transpose;
         }
         if (!rp.ft) TM_CHECK(rp.map != nullptr, TM_EINVAL, "reduced operator has no state map");
         work.ensure(reduced_work_bytes(rp, comp));

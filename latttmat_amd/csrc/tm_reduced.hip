@@ -43,7 +43,7 @@ using dev::pop;
 // group actions
 // ---------------------------------------------------------------------------
 struct GroupDev {
-  int dim, L, s1, s2, bc, censym, rotate, zgroup;
+  int dim, L, s1, s2, bc, censym, rotate, zgroup, ylayer;
   uint64_t mask;
   uint64_t cm1, cmr1;  // 3D column masks for row rotation by 1
   uint64_t rowmask;    // low s2 bits
@@ -91,7 +91,20 @@ __device__ uint64_t canon(uint64_t s, const GroupDev &g, int *stab)
     const int nf = g.censym ? 2 : 1;
     for (int f = 0; f < nf; ++f) {
       const uint64_t s0 = f ? (~s & g.mask) : s;
-      if (g.zgroup) {
+      if (g.ylayer) {
+        // both layers rotated together, and both reflected
+        // (TMat/tmatreduce.cpp:165-196)
+        const int n = g.ylayer;
+        const uint64_t lm = (1ull << n) - 1;
+        for (int r = 0; r < 2; ++r) {
+          uint64_t t = r ? ((dev::brev(s0 >> n, n) << n) | dev::brev(s0 & lm, n)) : s0;
+          for (int k = 0; k < n; ++k) {
+            m = t < m ? t : m;
+            st += (t == s);
+            t = (dev::rotr(t >> n, n, 1) << n) | dev::rotr(t & lm, n, 1);
+          }
+        }
+      } else if (g.zgroup) {
         // translations by two sites t and their mirror images i -> -i,
         // brev(rotr(t, 1)) (TMat/tmatreduce.cpp:272-297)
         uint64_t t = s0;
@@ -285,6 +298,7 @@ GroupDev to_dev(const GroupSpec &gs)
   g.censym = gs.censym;
   g.rotate = gs.rotate;
   g.zgroup = gs.zgroup;
+  g.ylayer = gs.ylayer;
   g.mask = (1ull << gs.L) - 1;
   if (gs.dim == 3) {
     uint64_t c1 = 0, cr1 = 0;
@@ -296,6 +310,8 @@ GroupDev to_dev(const GroupSpec &gs)
     g.cmr1 = cr1;
     g.rowmask = (1ull << gs.s2) - 1;
     g.order = gs.s1 * gs.s2 * 4 * (gs.censym ? 2 : 1) * (gs.rotate ? 2 : 1);
+  } else if (gs.ylayer) {
+    g.order = 2 * gs.ylayer * (gs.censym ? 2 : 1);
   } else if (gs.zgroup) {
     g.order = gs.L * (gs.censym ? 2 : 1);
   } else {
@@ -574,6 +590,41 @@ __global__ void k_expand(const uint32_t *__restrict__ map, const C *__restrict__
     u[s] = xt[map[s]];
 }
 
+// expansion times the two-layer weight of the state (c = high layer)
+template <typename C>
+__global__ void k_expand_w(const uint32_t *__restrict__ map, const C *__restrict__ xt,
+                           C *__restrict__ u, uint64_t nstate, int Ly, const C *__restrict__ wtab,
+                           int ybm)
+{
+  __shared__ C stab[320];
+  for (int i = threadIdx.x; i < (ybm ? 320 : 192); i += blockDim.x) stab[i] = wtab[i];
+  __syncthreads();
+  const uint32_t lm = (1u << Ly) - 1;
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < nstate;
+       s += (uint64_t)gridDim.x * blockDim.x)
+    u[s] = xt[map[s]] * dev::w_layers<C>((uint32_t)(s >> Ly), (uint32_t)s & lm, Ly, stab, ybm);
+}
+
+// gather at the representatives, times their two-layer weight
+template <typename C, typename TO>
+__global__ void k_gather_w(const C *__restrict__ u, const uint64_t *__restrict__ reps,
+                           const C *__restrict__ dt, TO *__restrict__ y, int64_t M,
+                           const double *__restrict__ scale, int Ly, const C *__restrict__ wtab,
+                           int ybm)
+{
+  __shared__ C stab[320];
+  for (int i = threadIdx.x; i < (ybm ? 320 : 192); i += blockDim.x) stab[i] = wtab[i];
+  __syncthreads();
+  const C inv = scale ? (C)(1.0 / *scale) : (C)1;
+  const uint32_t lm = (1u << Ly) - 1;
+  for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < M;
+       o += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = reps[o];
+    y[o] = (TO)(dt[o] * u[r] * inv *
+                dev::w_layers<C>((uint32_t)(r >> Ly), (uint32_t)r & lm, Ly, stab, ybm));
+  }
+}
+
 template <typename C, typename TO>
 __global__ void k_gather(const C *__restrict__ u, const uint64_t *__restrict__ reps,
                          const C *__restrict__ dt, TO *__restrict__ y, int64_t M,
@@ -766,6 +817,44 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
   // generic: expand, full-space operator, gather
   const uint64_t N = 1ull << p.L;
   C *u = rest;
+  if (p.ylayer) {
+    // symmetrised axial transfer y(n, c) = W(n, c) sum_p K2(n, p) W(c, p) x(c, p)
+    // (TMat/tmatreduce.cpp:832-873); its transpose (:957-1015) in reverse order
+    const int Ly = p.ylayer;
+    const C *wt = sizeof(C) == 8 ? reinterpret_cast<const C *>(p.wtab64)
+                                 : reinterpret_cast<const C *>(p.wtab32);
+    const tm_dtype ct = sizeof(C) == 8 ? TM_F64 : TM_F32;
+    hipLaunchKernelGGL((k_expand_w<C>), dim3(grid_for(N, 256)), dim3(256), 0, s, p.map, xt, u, N,
+                       Ly, wt, p.ybm);
+    TM_LAUNCH_CHECK();
+    C *v = u + N;
+    DiagSpec none{};
+    none.kind = -1;
+    const KronRange lo{2 * Ly, 0, Ly, false, false, none, none, nullptr, nullptr, p.ka, p.kb};
+    if (!p.transpose) {
+      launch_kron_range(lo, u, u, u, ct, ct, ct, nullptr, false, false, false, s);
+      launch_layer_swap(u, v, ct, ct, ct, Ly, wt, 1, p.ybm, nullptr, 0, s);
+    } else {
+      launch_layer_swap(u, v, ct, ct, ct, Ly, wt, 2, p.ybm, nullptr, 0, s);
+      launch_kron_range(lo, v, v, v, ct, ct, ct, nullptr, false, false, false, s);
+    }
+    // the forward operator applied W(n, c) in the swap; the transpose ends
+    // with W of the output state
+    if (!p.transpose && io == TM_F64)
+      hipLaunchKernelGGL((k_gather<C, double>), dim3(gM), dim3(256), 0, s, v, p.reps, dt,
+                         static_cast<double *>(out), M, scale);
+    else if (!p.transpose)
+      hipLaunchKernelGGL((k_gather<C, float>), dim3(gM), dim3(256), 0, s, v, p.reps, dt,
+                         static_cast<float *>(out), M, scale);
+    else if (io == TM_F64)
+      hipLaunchKernelGGL((k_gather_w<C, double>), dim3(gM), dim3(256), 0, s, v, p.reps, dt,
+                         static_cast<double *>(out), M, scale, Ly, wt, p.ybm);
+    else
+      hipLaunchKernelGGL((k_gather_w<C, float>), dim3(gM), dim3(256), 0, s, v, p.reps, dt,
+                         static_cast<float *>(out), M, scale, Ly, wt, p.ybm);
+    TM_LAUNCH_CHECK();
+    return;
+  }
   hipLaunchKernelGGL((k_expand<C>), dim3(grid_for(N, 256)), dim3(256), 0, s, p.map, xt, u, N);
   TM_LAUNCH_CHECK();
   const int smallL = sizeof(C) == 8 ? 13 : 14;
@@ -808,6 +897,8 @@ size_t reduced_work_bytes(const ReducedPlan &p, tm_dtype comp)
     b += (size_t)p.ft->nNk * p.ft->ldA * es;
   } else if (p.sweep) {
     b += 2 * (size_t(1) << p.L) * es + sweep_work_bytes(p.sweep->kind, p.L, comp);
+  } else if (p.ylayer) {
+    b += 2 * (size_t(1) << p.L) * es;
   } else {
     b += (size_t(1) << p.L) * es;
   }

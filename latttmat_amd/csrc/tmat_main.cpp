@@ -133,7 +133,9 @@ void write_column(std::ostream &os, const std::vector<T> &v)
 // The axial two-layer operators use the generic moments over all 4^L states
 // with the low layer's magnetisation (TMat/tmatphys.cpp:71-98); Y2AT writes
 // zeros (its moments are a stub, TMat/tmat.cpp:304-309).
-enum class PhysKind { Sym2D, Sym3D, Gen, Dnni, GenLayers, Zero };
+// The reduced axial operators weight by the representative's current (high)
+// layer (TMat/tmatreduce.cpp:882-909).
+enum class PhysKind { Sym2D, Sym3D, Gen, Dnni, GenLayers, GenHigh, Zero };
 
 int nnode(uint64_t s, int L, int bc)  // TMat/utility.cpp:33-47
 {
@@ -156,9 +158,10 @@ void write_phys(const std::string &fn, PhysKind pk, int L, int bc,
   const size_t n = qr.size();
   for (size_t i = 0; i < n; ++i) {
     const uint64_t s = states.empty() ? (uint64_t)i : states[i];
-    const uint64_t sl = pk == PhysKind::GenLayers ? (s & mask) : s;
+    const uint64_t sl = pk == PhysKind::GenLayers ? (s & mask) : pk == PhysKind::GenHigh ? (s >> L) : s;
     const double m = 2.0 * __builtin_popcountll(sl) - L;
-    const bool gen = pk == PhysKind::Gen || pk == PhysKind::GenLayers || pk == PhysKind::Zero;
+    const bool gen = pk == PhysKind::Gen || pk == PhysKind::GenLayers ||
+                     pk == PhysKind::GenHigh || pk == PhysKind::Zero;
     const double w = gen ? ql[i] * qr[i] : ql[i] * ql[i];
     dnorm += w;
     double mm = m;
@@ -185,8 +188,9 @@ void write_phys(const std::string &fn, PhysKind pk, int L, int bc,
     ext[2] /= (double)L * L;
     ext[3] /= (double)L * L;
   }
-  const double div =
-      (pk == PhysKind::Gen || pk == PhysKind::GenLayers) ? L * dnorm : (double)L;
+  const double div = (pk == PhysKind::Gen || pk == PhysKind::GenLayers || pk == PhysKind::GenHigh)
+                         ? L * dnorm
+                         : (double)L;
   if (pk == PhysKind::Zero) mom[0] = mom[1] = mom[2] = mom[3] = 0.0;
   std::ofstream f(fn);
   f << std::setprecision(10);
@@ -321,12 +325,13 @@ int main(int argc, char **argv)
       const tm_kind kind = tm_op_kind(op.handle());
       std::vector<uint64_t> states;
       if (kind == TM_KIND_XAM2D || kind == TM_KIND_XAM3D || kind == TM_KIND_XDM ||
-          kind == TM_KIND_ZBM) {
+          kind == TM_KIND_ZBM || kind == TM_KIND_YAM || kind == TM_KIND_YBM) {
         std::vector<uint16_t> dups(n);
         states.resize(n);
         latttmat::check(tm_op_orbits(op.handle(), states.data(), dups.data()));
       }
       const PhysKind pk = kind == TM_KIND_YAT    ? PhysKind::GenLayers
+                          : (kind == TM_KIND_YAM || kind == TM_KIND_YBM) ? PhysKind::GenHigh
                           : kind == TM_KIND_Y2AT ? PhysKind::Zero
                           : !sym                 ? PhysKind::Gen
                           : kind == TM_KIND_XDM ? PhysKind::Dnni

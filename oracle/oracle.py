@@ -15,7 +15,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-XAT, XDT, ZBT, XAM2D, XAM3D, XAT3D, XDM, ZBM, YAT, Y2AT = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
+XAT, XDT, ZBT, XAM2D, XAM3D, XAT3D, XDM, ZBM, YAT, Y2AT, YAM, YBM = range(1, 13)
 
 _lib = None
 
@@ -71,6 +71,8 @@ def lib():
         L.or_orbits_x2d.argtypes = [i32, i32, i32, C.c_void_p, C.c_void_p]
         L.or_orbits_z2d.restype = i64
         L.or_orbits_z2d.argtypes = [i32, i32, C.c_void_p, C.c_void_p]
+        L.or_orbits_y2d.restype = i64
+        L.or_orbits_y2d.argtypes = [i32, i32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_orbits_x3d.restype = i64
         L.or_orbits_x3d.argtypes = [i32, i32, i32, i32, C.c_void_p, C.c_void_p]
         L.or_tvec_xat.restype = None
@@ -152,6 +154,15 @@ def orbits_z2d(L, censym=0):
     dups = np.empty(m, np.uint16)
     lib().or_orbits_z2d(L, censym, reps.ctypes.data, dups.ctypes.data)
     return reps, dups
+
+
+def orbits_y2d(L, censym=0):
+    m = lib().or_orbits_y2d(L, censym, None, None, None)
+    reps = np.empty(m, np.uint64)
+    dups = np.empty(m, np.uint16)
+    swp = np.empty(m, np.int64)
+    lib().or_orbits_y2d(L, censym, reps.ctypes.data, dups.ctypes.data, swp.ctypes.data)
+    return reps, dups, swp
 
 
 def orbits_x3d(s1, s2, censym=1, rotate=0):

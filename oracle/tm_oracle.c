@@ -365,6 +365,133 @@ int64_t or_orbits_z2d(int L, int censym, uint64_t *reps, uint16_t *dups)
   return m;
 }
 
+/* TMat/utility.hpp:108-118: rotations of nowstate (right by one) until it
+ * equals target; nspin if never */
+static int findcyclic(uint64_t nowstate, uint64_t target, int nspin)
+{
+  uint64_t nx = nowstate;
+  int rp;
+  for (rp = 0; rp < nspin; ++rp) {
+    if (nx == target) break;
+    nx = (nx >> 1) | ((nx & 1) << (nspin - 1));
+  }
+  return rp;
+}
+
+typedef struct {
+  uint64_t *reps; uint16_t *dups; int64_t *swp; unsigned char (*lead)[4];
+  int64_t m, nlead, cap, capl;
+} ylists;
+
+static int ygrow(ylists *y)
+{
+  if (y->m < y->cap) return 1;
+  int64_t c = y->cap ? 2 * y->cap : 1024;
+  uint64_t *r = (uint64_t *)realloc(y->reps, sizeof(uint64_t) * c);
+  if (!r) return 0;
+  y->reps = r;
+  uint16_t *d = (uint16_t *)realloc(y->dups, sizeof(uint16_t) * c);
+  if (!d) return 0;
+  y->dups = d;
+  int64_t *w = (int64_t *)realloc(y->swp, sizeof(int64_t) * c);
+  if (!w) return 0;
+  y->swp = w;
+  y->cap = c;
+  return 1;
+}
+
+/* one rotation chain of genY2DDuplist: both layers rotated right together */
+static void ychain(uint64_t start, int L, bitset *eqb, uint16_t *d, uint64_t *smallest)
+{
+  uint64_t lm = (1ULL << L) - 1, lf = start >> L, rt = start & lm, nx;
+  for (nx = start; !bs_get(eqb, nx);) {
+    uint64_t tmp = (rt << L) | lf;
+    if (tmp < *smallest) *smallest = tmp;
+    ++*d;
+    bs_set(eqb, nx);
+    lf = (lf >> 1) | ((lf & 1) << (L - 1));
+    rt = (rt >> 1) | ((rt & 1) << (L - 1));
+    nx = (lf << L) | rt;
+  }
+}
+
+/* TMat/tmatreduce.cpp:114-239 (genY2DDuplist): orbits of two-layer states
+ * under rotations and the reflection of both layers together (and the flip
+ * of both); swp pairs an orbit with the orbit of its layer-swapped
+ * representative; lead holds the periods of each distinct high layer. */
+static int orbits_y2d(int L, int censym, ylists *y)
+{
+  uint64_t nstate = 1ULL << (2 * L), lm = (1ULL << L) - 1, mask = nstate - 1;
+  bitset eqb;
+  if (!bs_init(&eqb, 2 * L)) return 0;
+  memset(y, 0, sizeof *y);
+  for (uint64_t now = 0; now < nstate; ++now) {
+    if (bs_get(&eqb, now)) continue;
+    uint64_t lf = now >> L;
+    if (y->m == 0 || (y->reps[y->m - 1] >> L) != lf) {
+      if (y->nlead == y->capl) {
+        int64_t c = y->capl ? 2 * y->capl : 256;
+        unsigned char(*t)[4] = (unsigned char(*)[4])realloc(y->lead, 4 * (size_t)c);
+        if (!t) goto fail;
+        y->lead = t;
+        y->capl = c;
+      }
+      unsigned char *ld = y->lead[y->nlead++];
+      ld[0] = ld[1] = ld[2] = ld[3] = (unsigned char)L;
+      int count = findcyclic((lf >> 1) | ((lf & 1) << (L - 1)), lf, L);
+      if (count < L) ld[0] = (unsigned char)(++count);
+      ld[1] = (unsigned char)findcyclic(or_bitreversal(lf, L), lf, L);
+      if (censym && POP(lf) * 2 == L) {
+        ld[2] = (unsigned char)findcyclic(~lf & lm, lf, L);
+        ld[3] = (unsigned char)findcyclic(or_bitreversal(~lf & lm, L), lf, L);
+      }
+    }
+    if (!ygrow(y)) goto fail;
+    int64_t me = y->m++;
+    y->swp[me] = me;
+    y->reps[me] = now;
+    uint16_t d = 0;
+    uint64_t smallest = ((now & lm) << L) | lf;
+    ychain(now, L, &eqb, &d, &smallest);
+    ychain((or_bitreversal(now >> L, L) << L) | or_bitreversal(now & lm, L), L, &eqb, &d,
+           &smallest);
+    if (censym) {
+      uint64_t ni = ~now & mask;
+      ychain(ni, L, &eqb, &d, &smallest);
+      ychain((or_bitreversal(ni >> L, L) << L) | or_bitreversal(ni & lm, L), L, &eqb, &d,
+             &smallest);
+    }
+    y->dups[me] = d;
+    if (smallest < now) {
+      int64_t lo = 0, hi = me;  /* lower_bound over the ascending reps */
+      while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (y->reps[mid] < smallest) lo = mid + 1; else hi = mid;
+      }
+      if (lo >= me || y->reps[lo] != smallest) goto fail;  /* tmatreduce.cpp:232-236 */
+      y->swp[me] = lo;
+      y->swp[lo] = me;
+    }
+  }
+  free(eqb.w);
+  return 1;
+fail:
+  free(eqb.w);
+  return 0;
+}
+
+int64_t or_orbits_y2d(int L, int censym, uint64_t *reps, uint16_t *dups, int64_t *swp)
+{
+  ylists y;
+  if (L < 1 || L > 15 || !orbits_y2d(L, censym, &y)) return -1;
+  if (reps) memcpy(reps, y.reps, sizeof(uint64_t) * y.m);
+  if (dups) memcpy(dups, y.dups, sizeof(uint16_t) * y.m);
+  if (swp) memcpy(swp, y.swp, sizeof(int64_t) * y.m);
+  int64_t m = y.m;
+  free(y.reps); free(y.dups); free(y.swp); free(y.lead);
+  return m;
+}
+
 /* ------------------------------------------------------------------------ */
 /* operator construction                                                     */
 /* ------------------------------------------------------------------------ */
@@ -607,6 +734,54 @@ or_op *or_op_create(int kind, int size1, int size2, int bc,
       }
       break;
     }
+    case OR_YAM:
+    case OR_YBM: {
+      /* TMat/tmatreduce.cpp:661-714, genTvec :717-745 (layertype 0) and
+       * genTvec1 :748-780 (layertype 1 = YBM, tmatreduce.cpp:1017-1021) */
+      int L = size1;
+      if (L < 4 || L > 12) goto fail;
+      int extend = bc, lt1 = kind == OR_YBM;
+      op->L = L; op->nstate = 1LL << (2 * L);
+      op->censym = (h == 0.0 && (extend & 2) == 0);
+      double nfe;
+      if (!lt1) {
+        nfe = (kappa < 0.5) ? L * (kappa - 2.0) : -L * (kappa + 1.0);
+      } else if (kappa + kappa2 < 0.5) {
+        nfe = L * 2.0 * (kappa + kappa2 - 1.0);
+      } else if (kappa < 0.5 && kappa2 - kappa < 0.5) {
+        nfe = -L;
+      } else if (kappa2 < 2 * kappa) {
+        nfe = -L * 2.0 * kappa;
+      } else {
+        nfe = L * 2.0 * (kappa - kappa2);
+      }
+      op->nf = exp(betaJ * nfe);
+      ylists y;
+      if (!orbits_y2d(L, op->censym, &y)) goto fail;
+      op->reps = y.reps; op->dups = y.dups; op->swp = y.swp; op->lead = y.lead;
+      op->nlead = y.nlead;
+      op->ncol = y.m;
+      op->tvec = (double *)malloc(sizeof(double) * y.m);
+      if (!op->tvec) goto fail;
+      uint64_t lm = (1ULL << L) - 1;
+      double hb = 0.5 * betaJ, J1 = -hb, hJ = -hb * h, Jn = hb * nfe;
+      double k1J = hb * kappa, k2J = hb * kappa2;
+      for (int64_t rp = 0; rp < y.m; ++rp) {
+        uint64_t st = op->reps[rp], lf = st >> L, rt = st & lm;
+        int n1 = POP(lf ^ or_offset_state(lf, L, 1)), n2 = POP(lf ^ rt), n0 = POP(lf);
+        double e;
+        if (!lt1) {
+          e = J1 * (2 * (n1 + n2 - L)) + hJ * (2 * n0 - L) + Jn;
+        } else {
+          int b1 = POP(lf ^ or_offset_state(lf, L, 2));
+          int d1 = POP(lf ^ or_offset_state(rt, L, 1)), d2 = POP(lf ^ or_lffset_state(rt, L, 1));
+          e = J1 * (2 * (n1 + n2 - L)) + k1J * (2 * b1 - L) + k2J * (2 * (d1 + d2 - L)) +
+              hJ * (2 * n0 - L) + Jn;
+        }
+        op->tvec[rp] = exp(e) / sqrt((double)op->dups[rp]);
+      }
+      break;
+    }
     case OR_XAM3D: {
       /* TMat/tmat3d.cpp:102-134 and genTvec :179-201 */
       int size = size1 * size2;
@@ -668,6 +843,8 @@ void or_op_destroy(or_op *op)
   free(op->tvec);
   free(op->reps);
   free(op->dups);
+  free(op->swp);
+  free(op->lead);
   free(op);
 }
 
@@ -1312,6 +1489,66 @@ static void apply_y2at(const or_op *op, const double *x, double *y)
   free(t2);
 }
 
+/* TMat/tmatreduce.cpp:811-873 (YAMMatVecProd2D::permute_mult_TL, perform_op)
+ * and, dir = 1, :957-1015 (YAMMatVecProd2D_T::perform_op): per group of
+ * orbits sharing a high layer, scatter Tvec x over the low layer of every
+ * image with that high layer, transfer the low layer by (x)K2, and gather at
+ * the layer-swapped orbit (dir 0) or at the orbit itself from the swapped
+ * input (dir 1). */
+static void apply_yam(const or_op *op, const double *x, double *y, int dir)
+{
+  int L = op->L;
+  uint64_t ls = 1ULL << L, lm = ls - 1;
+  int64_t m = op->ncol, rp, rq, leadrp = 0;
+  const double *tv = op->tvec, *b2 = op->bzwj2;
+  double *yt = (double *)malloc(sizeof(double) * ls);
+  for (rp = 0; rp < m; rp = rq, ++leadrp) {
+    uint64_t lf = op->reps[rp] >> L;
+    const unsigned char *ld = op->lead[leadrp];
+    for (rq = rp; rq < m && (op->reps[rq] >> L) == lf; ++rq) {
+      uint64_t rt = op->reps[rq] & lm;
+      double d;
+      if (!dir) {
+        d = tv[rq] * x[rq];
+      } else {
+        int64_t sw = op->swp[rq];
+        d = tv[sw] * x[sw];
+      }
+      for (int ch = 0; ch < 4; ++ch) {
+        unsigned char sh = ld[ch];
+        uint64_t rn = rt;
+        if (ch && sh == (unsigned char)L) continue;
+        switch (ch) {
+          case 1: rn = or_bitreversal(rt, L); break;
+          case 2: rn = ~rt & lm; break;
+          case 3: rn = or_bitreversal(~rt & lm, L); break;
+          default: break;
+        }
+        if (ch) rn = or_offset_state(rn, L, sh);
+        for (int sa = 0; sa < L; sa += ld[0]) {
+          yt[rn] = d;
+          rn = or_offset_state(rn, L, ld[0]);
+        }
+      }
+    }
+    for (int it = 0; it < L; ++it) {
+      uint64_t ms = 1ULL << it;
+      for (uint64_t r = 0; r < ls; ++r) {
+        if (!(r & ms)) continue;
+        double t = yt[r];
+        yt[r] = b2[0] * t + b2[1] * yt[r ^ ms];
+        yt[r ^ ms] = b2[1] * t + b2[0] * yt[r ^ ms];
+      }
+    }
+    for (int64_t rr = rp; rr < rq; ++rr) {
+      uint64_t rt = op->reps[rr] & lm;
+      int64_t o = dir ? rr : op->swp[rr];
+      y[o] = tv[o] * op->dups[o] * yt[rt];
+    }
+  }
+  free(yt);
+}
+
 void or_op_apply(const or_op *op, const double *x, double *y, int transpose)
 {
   switch (op->kind) {
@@ -1325,6 +1562,8 @@ void or_op_apply(const or_op *op, const double *x, double *y, int transpose)
     case OR_ZBM: apply_zbm(op, x, y, transpose ? 1 : 0); break;
     case OR_YAT: apply_yat(op, x, y, transpose ? 1 : 0); break;
     case OR_Y2AT: apply_y2at(op, x, y); break;  /* tmat.hpp:382-384: T^T := T */
+    case OR_YAM:
+    case OR_YBM: apply_yam(op, x, y, transpose ? 1 : 0); break;
     default: break;
   }
 }

@@ -20,6 +20,9 @@
  *                             lambda0(reduced) == lambda0(full)
  *   YAT / Y2AT apply          dense T of the two-layer transfer built from the
  *                             axial ANNNI Hamiltonian (tests/physics.py)
+ *   YAM / YBM apply (and T)   projection of the dense symmetrised two-layer
+ *                             Hamiltonian matrix onto layer-wise orbit sums;
+ *                             lambda0(YAM) == lambda0(YAT)
  *   XDM / ZBM apply           projection of the dense full XDT / ZBT onto orbit
  *                             sums (ZBM transposed too); lambda0(reduced) ==
  *                             lambda0(full); group generators commute with T
@@ -59,8 +62,11 @@ enum {
                       two-site translations and i -> -i (x flip)                     */
   OR_YAT = 9,      /* TMat/tmat.cpp:104-243   ANNNI axial TM, two layers, 4^L; the
                       transpose is YATMatVecProd2D_T (tmat.cpp:104-184)             */
-  OR_Y2AT = 10     /* TMat/tmat.cpp:246-310   ANNNI, layer pairs, 4^L; its transposed
+  OR_Y2AT = 10,    /* TMat/tmat.cpp:246-310   ANNNI, layer pairs, 4^L; its transposed
                       instance is the operator itself (tmat.hpp:382-384)            */
+  OR_YAM = 11,     /* TMat/tmatreduce.cpp:661-949 ANNNI axial TM reduced by the
+                      layer-wise D_L (x flip); transpose YAMMatVecProd2D_T :951-1015 */
+  OR_YBM = 12      /* TMat/tmatreduce.cpp:1017-1021 BNNNI axial TM (YAM, layertype 1) */
 };
 
 typedef struct or_op {
@@ -79,6 +85,9 @@ typedef struct or_op {
   double *tvec;     /* length ncol                                   */
   uint64_t *reps;   /* reduced only: orbit representatives, ascending */
   uint16_t *dups;   /* reduced only: orbit sizes                      */
+  int64_t *swp;     /* YAM/YBM: orbit of the layer-swapped representative */
+  unsigned char (*lead)[4];  /* YAM/YBM: lead-layer periods (leadprdlist) */
+  int64_t nlead;
 } or_op;
 
 /* Build an operator.  bc follows the reference: XAT 0,1,5,9,13,17; XDT 0,5,9;
@@ -103,6 +112,9 @@ const uint16_t *or_op_dups(const or_op *op);
 int64_t or_orbits_x2d(int L, int bc, int censym, uint64_t *reps, uint16_t *dups);
 int64_t or_orbits_x3d(int s1, int s2, int censym, int rotate, uint64_t *reps, uint16_t *dups);
 int64_t or_orbits_z2d(int L, int censym, uint64_t *reps, uint16_t *dups);  /* tmatreduce.cpp:242-334 */
+/* tmatreduce.cpp:114-239: two-layer orbits; reps/dups/swp of length M
+ * (call with NULL arrays to count) */
+int64_t or_orbits_y2d(int L, int censym, uint64_t *reps, uint16_t *dups, int64_t *swp);
 
 /* Diagonal Tv^(1/2) generators (genTvec) for the full operators, exactly as
  * the reference builds them (argument betaJ is the one passed to genTvec). */

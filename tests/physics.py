@@ -79,6 +79,31 @@ def yat_dense(L, bJ, k, h):
     return T
 
 
+def yam_full_dense(L, bJ, k, h, k2=None):
+    """The symmetrised axial transfer over two-layer states (c, p) = (current,
+    previous), index (c << L) | p: T[(n, c), (c, p)] = W(n, c) K2(n, p) W(c, p)
+    with W(c, p) = exp(half the in-layer energy of c and of the c-p bonds),
+    K2 the axial NNN (-kappa).  k2 given: the BNNNI layer (NNN -kappa inside
+    the layer, diagonal -k2 between the layers).  Without the normfactor."""
+    n1 = 1 << L
+    W = np.empty((n1, n1))
+    for c in range(n1):
+        for p in range(n1):
+            e = (bJ * pairsum(c ^ rotr(c, L, 1), L) + bJ * cross_nn(c, p, L)
+                 + h * bJ * pairsum(c, L))
+            if k2 is not None:
+                e += (-k * bJ * pairsum(c ^ rotr(c, L, 2), L)
+                      - k2 * bJ * (cross_nn(c, rotr(p, L, 1), L) + cross_nn(c, rotl(p, L, 1), L)))
+            W[c, p] = np.exp(0.5 * e)
+    cr = np.array([[cross_nn(a, b, L) for b in range(n1)] for a in range(n1)], float)
+    T = np.zeros((n1 * n1, n1 * n1))
+    for n in range(n1):
+        for c in range(n1):
+            for p in range(n1):
+                T[(n << L) | c, (c << L) | p] = W[n, c] * np.exp(-k * bJ * cr[n, p]) * W[c, p]
+    return T
+
+
 def y2at_dense(L, bJ, k, h):
     """ANNNI transfer by two axial layers over layer pairs, index
     (r << L) | l with r the older and l the newer layer of the pair: NN

@@ -325,3 +325,32 @@ def test_cli_axial_outputs(tmp_path, oracle):
     chk = [ln for ln in r2.stdout.splitlines() if ln.startswith("Eigensolver check: ")]
     assert len(chk) == 1 and abs(float(chk[0].split()[-1]) - 1) < 1e-9
     assert open(d2 / "W_phys.dat").read() == "0\t0\t0\t0"
+
+
+@pytest.mark.gpu
+def test_cli_yam_phys(tmp_path, oracle):
+    """Reduced axial ANNNI (A, axialdir 6): moments over the representatives
+    with the current (high) layer's magnetisation, weighted by ql*qr
+    (TMat/tmatreduce.cpp:882-909)."""
+    L = 6
+    p = write_input(str(tmp_path), axialdir=6, size=L, savevec="1 1", out="M", J=0.45,
+                    kappa="0.3 0")
+    r = run([p], str(tmp_path))
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    D = oracle.OracleOp(oracle.YAM, L, 0, 0, 0.45, 0.3, 0.0, 0.0)
+    T = D.dense()
+    w = np.linalg.eigvals(T)
+    lam = w[np.argmax(np.abs(w))].real
+    re_, im_ = open(tmp_path / "M_val.dat").read().strip("()").split(",")
+    assert abs(float(re_) / (lam / D.normfactor) - 1) < 1e-10
+    vr = np.fromfile(tmp_path / "M_vec.dat", dtype=np.float64)
+    vl = np.fromfile(tmp_path / "M_vecl.dat", dtype=np.float64)
+    assert vr.size == vl.size == D.n
+    assert np.linalg.norm(T @ vr - lam * vr) / abs(lam) < 1e-8
+    assert np.linalg.norm(T.T @ vl - lam * vl) / abs(lam) < 1e-8
+    mom = [float(x) for x in open(tmp_path / "M_phys.dat").read().split("\t")]
+    wgt = vl * vr
+    m = 2 * np.array([bin(int(v) >> L).count("1") for v in D.reps()], float) - L
+    ref = [float(np.sum(m ** k * wgt) / (L * wgt.sum())) for k in range(1, 5)]
+    np.testing.assert_allclose(mom, ref, rtol=1e-8, atol=1e-10)

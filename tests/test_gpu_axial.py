@@ -107,3 +107,45 @@ def test_axial_grad_unsupported(lt):
     with pytest.raises(lt.TMError) as e:
         op.grad_hf(np.ones(op.cols()), np.ones(op.cols()))
     assert e.value.status == 5
+
+
+@pytest.mark.parametrize("kind,size,ext,k,k2,h", [("YAM", 4, 0, 0.35, 0.0, 0.0), ("YAM", 5, 2, 0.35, 0.0, 0.0),
+                                                  ("YAM", 6, 0, 0.6, 0.0, 0.1), ("YAM", 8, 0, 0.3, 0.0, 0.0),
+                                                  ("YAM", 9, 2, 0.3, 0.0, 0.0), ("YBM", 4, 0, 0.3, 0.2, 0.0),
+                                                  ("YBM", 7, 0, 0.6, 0.4, 0.1), ("YBM", 8, 2, 0.2, 0.6, 0.0),
+                                                  ("YAM", 10, 0, 0.45, 0.0, 0.0)])
+def test_yam_ybm_apply(lt, oracle, kind, size, ext, k, k2, h):
+    """Reduced axial operators: orbit tables bit-exact, T.x and T^T.x against
+    the restated YAMMatVecProd2D(_T) (layertype 1 for YBM)."""
+    model = "A" if kind == "YAM" else "B"
+    op = lt.TransferMatrix(model, 2, 6, size, J=0.45, kappa=k, kappa2=k2, h=h,
+                           neigs=4 if ext == 2 else 1)
+    oop = oracle.OracleOp(getattr(oracle, kind), size, 0, ext, 0.45, k, k2, h)
+    assert op.kind() == kind and not op.is_symmetric()
+    reps, dups = op.orbits()
+    np.testing.assert_array_equal(reps, oop.reps())
+    np.testing.assert_array_equal(dups, oop.dups())
+    assert abs(op.getnormfactor() / oop.normfactor - 1) < 1e-15
+    for tr in (False, True):
+        x = _vec(op.cols(), 7 + tr)
+        yr = oop.apply(x, tr)
+        e64 = _rel(op.apply(x, tr), yr)
+        e32 = _rel(op.apply_prec(x, lt.TM_F32, tr), yr)
+        print(f"  {kind} L={size} M={op.cols()} tr={tr} fp64 {e64:.2e} fp32 {e32:.2e}", flush=True)
+        assert e64 <= TOL64, e64
+        assert e32 <= TOL32, e32
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_yam_lambda0_equals_yat(lt, oracle, prec):
+    L = 10
+    yam = lt.TransferMatrix("A", 2, 6, L, J=0.35, kappa=0.4)
+    yat = lt.TransferMatrix("A", 2, 2, L, J=0.35, kappa=0.4)
+    r1 = lt.solve(yam, nev=1, tol=1e-12, want_left=True, want_vectors=True, precision=prec)
+    r2 = lt.solve(yat, nev=1, tol=1e-12, precision=prec)
+    print("YAM", yam.cols(), r1.evals, r1.eval_left, "YAT", r2.evals, flush=True)
+    assert abs(r1.evals[0].real / r2.evals[0].real - 1) < 1e-11
+    assert abs(r1.eval_left / r1.evals[0].real - 1) < 1e-11
+    v = r1.vecl
+    w = yam.apply(v, transpose=True)
+    assert np.linalg.norm(w - r1.eval_left * v) / (abs(r1.eval_left) * np.linalg.norm(v)) < 1e-9

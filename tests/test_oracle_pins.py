@@ -409,3 +409,52 @@ def test_yat_y2at_dense_hamiltonian(oracle, L, bJ, k, h):
     l1 = max(abs(np.linalg.eigvals(T))) / yat.normfactor
     l2 = max(abs(np.linalg.eigvals(T2))) / y2.normfactor
     assert abs(l1 * l1 / l2 - 1) < 1e-13
+
+
+def _closure_layers(L, censym):
+    """Orbits of two-layer states (index (c << L) | p) under rotation and
+    reflection of both layers together (and the flip of both)."""
+    lm = (1 << L) - 1
+
+    def rot(s):
+        return (physics.rotr(s >> L, L, 1) << L) | physics.rotr(s & lm, L, 1)
+
+    def ref(s):
+        return ((int(format(s >> L, f"0{L}b")[::-1], 2) << L)
+                | int(format(s & lm, f"0{L}b")[::-1], 2))
+    gens = [rot, ref] + ([lambda s: ~s & ((1 << (2 * L)) - 1)] if censym else [])
+    return _closure(2 * L, gens)
+
+
+@pytest.mark.parametrize("kind,L,k,k2,h,ext", [("YAM", 4, 0.3, 0.0, 0.0, 0), ("YAM", 4, 0.3, 0.0, 0.0, 2),
+                                               ("YAM", 5, 0.6, 0.0, 0.1, 0), ("YBM", 4, 0.3, 0.2, 0.0, 0),
+                                               ("YBM", 5, 0.6, 0.4, 0.0, 2)])
+def test_yam_ybm_projection(oracle, kind, L, k, k2, h, ext):
+    """The restated YAMMatVecProd2D::perform_op (and _T, and YBM's layer)
+    equals the orbit-sum projection of the symmetrised two-layer transfer
+    matrix built from the Hamiltonian; the orbit tables and the layer-swap
+    pairing match a brute-force closure."""
+    bJ = 0.45
+    red = oracle.OracleOp(getattr(oracle, kind), L, 0, ext, bJ, k, k2, h)
+    censym = h == 0 and (ext & 2) == 0
+    cr, cd, orb = _closure_layers(L, censym)
+    np.testing.assert_array_equal(red.reps(), cr)
+    np.testing.assert_array_equal(red.dups(), cd)
+    reps, dups, swp = oracle.orbits_y2d(L, int(censym))
+    lm = (1 << L) - 1
+    for o, r in enumerate(reps):
+        r = int(r)
+        assert orb[((r & lm) << L) | (r >> L)] == swp[o]
+    T = physics.yam_full_dense(L, bJ, k, h, k2 if kind == "YBM" else None) * red.normfactor
+    Tr = _projection(T, orb, cd)
+    assert np.max(np.abs(red.dense() - Tr)) / np.max(np.abs(Tr)) < 1e-13
+    assert np.max(np.abs(red.dense(transpose=True) - Tr.T)) / np.max(np.abs(Tr)) < 1e-13
+
+
+def test_yam_lambda0_equals_yat(oracle):
+    yam = oracle.OracleOp(oracle.YAM, 5, 0, 0, 0.45, 0.35, 0.0, 0.0)
+    yat = oracle.OracleOp(oracle.YAT, 5, 0, 0, 0.45, 0.35, 0.0, 0.0)
+    assert yam.normfactor == yat.normfactor
+    l1 = max(abs(np.linalg.eigvals(yam.dense())))
+    l2 = max(abs(np.linalg.eigvals(yat.dense())))
+    assert abs(l1 / l2 - 1) < 1e-13
