@@ -74,11 +74,10 @@ tm_status tm_op_apply(const tm_op *cop, const double *x, double *y, int transpos
     const size_t bytes = sizeof(double) * op->ncol;
     op->hx.ensure(bytes);
     op->hy.ensure(bytes);
-    TM_HIP(hipMemcpyAsync(op->hx.get(), x, bytes, hipMemcpyHostToDevice, op->stream));
+    ltm::copy_large(op->hx.get(), x, bytes, hipMemcpyHostToDevice, op->stream);
     op->apply_dev(op->hx.get(), op->hy.get(), op->ncol, 1, TM_F64, TM_F64, transpose, nullptr,
                   op->stream);
-    TM_HIP(hipMemcpyAsync(y, op->hy.get(), bytes, hipMemcpyDeviceToHost, op->stream));
-    TM_HIP(hipStreamSynchronize(op->stream));
+    ltm::copy_large(y, op->hy.get(), bytes, hipMemcpyDeviceToHost, op->stream);
     return TM_OK;
   });
 }
@@ -126,7 +125,7 @@ tm_status tm_op_apply_prec(const tm_op *cop, const double *x, double *y, int tra
     const size_t bytes = sizeof(double) * op->ncol;
     op->hx.ensure(bytes);
     op->hy.ensure(bytes);
-    TM_HIP(hipMemcpyAsync(op->hx.get(), x, bytes, hipMemcpyHostToDevice, op->stream));
+    ltm::copy_large(op->hx.get(), x, bytes, hipMemcpyHostToDevice, op->stream);
     ltm::DevBuf partial(sizeof(double) * ltm::kry::NBLK), scale(sizeof(double));
     const double *sc = nullptr;
     if (compute == TM_F32) {
@@ -136,8 +135,7 @@ tm_status tm_op_apply_prec(const tm_op *cop, const double *x, double *y, int tra
     }
     op->apply_dev(op->hx.get(), op->hy.get(), op->ncol, 1, TM_F64, compute, transpose, sc,
                   op->stream);
-    TM_HIP(hipMemcpyAsync(y, op->hy.get(), bytes, hipMemcpyDeviceToHost, op->stream));
-    TM_HIP(hipStreamSynchronize(op->stream));
+    ltm::copy_large(y, op->hy.get(), bytes, hipMemcpyDeviceToHost, op->stream);
     return TM_OK;
   });
 }
@@ -271,8 +269,8 @@ tm_status tm_grad_hf(const tm_op *cop, const double *l, const double *r, double 
   return guarded([&] {
     const size_t bytes = sizeof(double) * op->ncol;
     ltm::DevBuf dl(bytes), dr(bytes);
-    ltm::copy_sync(dl.get(), l, bytes, hipMemcpyHostToDevice, op->stream);
-    ltm::copy_sync(dr.get(), r, bytes, hipMemcpyHostToDevice, op->stream);
+    ltm::copy_large(dl.get(), l, bytes, hipMemcpyHostToDevice, op->stream);
+    ltm::copy_large(dr.get(), r, bytes, hipMemcpyHostToDevice, op->stream);
     double num[5];
     ltm::grad_numerators(op, dl.as<double>(), dr.as<double>(), num);
     // num[4] = <l, r>

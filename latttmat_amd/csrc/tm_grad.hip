@@ -109,7 +109,12 @@ void full_numerators(const tm_op *cop, tm_op *op, const double *l, const double 
     work.alloc(sweep_dual_work_bytes(op->kind, op->L));
   std::vector<double> hp(3 * DOT_BLOCKS);
   double lr = 0;
+  const bool has_k2 = cop->kind == TM_KIND_ZBT || cop->kind == TM_KIND_ZBM;
   for (int th = 0; th < 4; ++th) {
+    if (th == 2 && !has_k2) {
+      num[2] = 0.0;  // kappa2 enters only the diagonal-TM couplings
+      continue;
+    }
     const OpConsts<HDual> c = seeded(cop, th);
     Dual tab[192];
     for (int i = 0; i < 192; ++i) tab[i] = Dual(c.tab[i].v, c.tab[i].d);
@@ -155,7 +160,7 @@ void full_numerators(const tm_op *cop, tm_op *op, const double *l, const double 
     num[th] = d;
   }
   num[4] = lr;
-  op->matvecs += 4;
+  op->matvecs += has_k2 ? 4 : 3;
 }
 
 }  // namespace

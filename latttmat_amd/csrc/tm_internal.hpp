@@ -65,6 +65,9 @@ int autotune_tile(uint32_t kernel, int L, int es, const std::vector<int> &cands,
 // every blocking stream would break a graph capture running on another
 // thread.
 void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
+// The same for large host <-> device copies of pageable host memory: through
+// a pinned double buffer, DMA overlapping a multi-threaded host copy.
+void copy_large(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
 
 // Graph capture and device allocation exclude each other across threads:
 // a capture holds this lock exclusively (and marks its thread), allocations

@@ -3,7 +3,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <limits>
+#include <thread>
 #include <map>
 #include <tuple>
 #include <mutex>
@@ -93,6 +95,86 @@ void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hip
 {
   TM_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
   TM_HIP(hipStreamSynchronize(s));
+}
+
+namespace {
+// Double-buffered pinned staging for copy_large, one per host thread (the
+// batch driver copies from several threads at once).
+struct Stage {
+  PinnedBuf buf[2];
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  ~Stage()
+  {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+thread_local Stage t_stage;
+
+// host memcpy split over up to four threads (one host thread moves about
+// 10 GB/s, below what the DMA engines deliver into pinned memory)
+void host_copy(void *d, const void *src, size_t n)
+{
+  const int nt = n >= (size_t(4) << 20) ? 4 : 1;
+  const size_t part = ((n + nt - 1) / nt + 63) & ~size_t(63);
+  std::thread th[3];
+  for (int t = 1; t < nt; ++t) {
+    const size_t o = part * t;
+    if (o >= n) break;
+    th[t - 1] = std::thread([=] {
+      std::memcpy(static_cast<char *>(d) + o, static_cast<const char *>(src) + o,
+                  std::min(part, n - o));
+    });
+  }
+  std::memcpy(d, src, std::min(part, n));
+  for (auto &x : th)
+    if (x.joinable()) x.join();
+}
+}  // namespace
+
+void copy_large(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s)
+{
+  constexpr size_t CH = size_t(8) << 20;
+  if (bytes < (size_t(2) << 20) ||
+      (kind != hipMemcpyDeviceToHost && kind != hipMemcpyHostToDevice)) {
+    copy_sync(dst, src, bytes, kind, s);
+    return;
+  }
+  Stage &st = t_stage;
+  for (int b = 0; b < 2; ++b) {
+    st.buf[b].ensure(CH);
+    if (!st.ev[b]) TM_HIP(hipEventCreateWithFlags(&st.ev[b], hipEventDisableTiming));
+  }
+  const size_t nch = (bytes + CH - 1) / CH;
+  char *d = static_cast<char *>(dst);
+  const char *sp = static_cast<const char *>(src);
+  if (kind == hipMemcpyDeviceToHost) {
+    // DMA of chunk i into one buffer while the host copies chunk i-1 out of
+    // the other
+    for (size_t i = 0; i <= nch; ++i) {
+      if (i < nch) {
+        const size_t off = i * CH, n = std::min(CH, bytes - off);
+        TM_HIP(hipMemcpyAsync(st.buf[i & 1].as<char>(), sp + off, n, kind, s));
+        TM_HIP(hipEventRecord(st.ev[i & 1], s));
+      }
+      if (i > 0) {
+        const size_t j = i - 1, off = j * CH, n = std::min(CH, bytes - off);
+        TM_HIP(hipEventSynchronize(st.ev[j & 1]));
+        host_copy(d + off, st.buf[j & 1].as<char>(), n);
+      }
+    }
+  } else {
+    // the host fills one buffer while the DMA of the previous chunk runs; a
+    // buffer is refilled once the DMA that read it has finished
+    for (size_t i = 0; i < nch; ++i) {
+      const size_t off = i * CH, n = std::min(CH, bytes - off);
+      if (i >= 2) TM_HIP(hipEventSynchronize(st.ev[i & 1]));
+      host_copy(st.buf[i & 1].as<char>(), sp + off, n);
+      TM_HIP(hipMemcpyAsync(d + off, st.buf[i & 1].as<char>(), n, kind, s));
+      TM_HIP(hipEventRecord(st.ev[i & 1], s));
+    }
+    TM_HIP(hipStreamSynchronize(s));
+  }
 }
 
 void PinnedBuf::alloc(size_t bytes)

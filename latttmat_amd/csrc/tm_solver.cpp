@@ -883,12 +883,12 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
       for (int i = 0; i < nev; ++i) {
         double *dst = out->vecs + (size_t)i * n;
         if (op->perm.empty()) {
-          copy_sync(dst, xr + (size_t)i * ldv, sizeof(double) * n,
-                    hipMemcpyDeviceToHost, op->stream);
+          copy_large(dst, xr + (size_t)i * ldv, sizeof(double) * n, hipMemcpyDeviceToHost,
+                     op->stream);
         } else {
           // internal orbit order -> ascending representatives
-          copy_sync(tmp.data(), xr + (size_t)i * ldv, sizeof(double) * n,
-                    hipMemcpyDeviceToHost, op->stream);
+          copy_large(tmp.data(), xr + (size_t)i * ldv, sizeof(double) * n,
+                     hipMemcpyDeviceToHost, op->stream);
           for (int64_t t = 0; t < n; ++t) dst[op->perm[t]] = tmp[t];
         }
       }
@@ -896,8 +896,10 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
       // cols() x nev complex128 interleaved, column-major (pfunc.hpp:139)
       std::vector<double> re(n), im(n);
       for (int i = 0; i < nev; ++i) {
-        copy_sync(re.data(), xr + (size_t)i * ldv, sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
-        copy_sync(im.data(), xi + (size_t)i * ldv, sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
+        copy_large(re.data(), xr + (size_t)i * ldv, sizeof(double) * n, hipMemcpyDeviceToHost,
+                   op->stream);
+        copy_large(im.data(), xi + (size_t)i * ldv, sizeof(double) * n, hipMemcpyDeviceToHost,
+                   op->stream);
         for (int64_t t = 0; t < n; ++t) {
           out->vecs[2 * ((size_t)i * n + t)] = re[t];
           out->vecs[2 * ((size_t)i * n + t) + 1] = im[t];
@@ -922,7 +924,7 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
         DevBuf &XL = cache.dev[SLOT_XI];  // the right vectors are on the host by now
         XL.ensure(sizeof(double) * kl.ldv());
         kl.ritz_vectors(rl, 1, XL.as<double>(), nullptr);
-        copy_sync(out->vecl, XL.get(), sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
+        copy_large(out->vecl, XL.get(), sizeof(double) * n, hipMemcpyDeviceToHost, op->stream);
       }
     } else if (o.verbose) {
       std::fprintf(stderr, "Warning: left eigenvalue calculation failed.\n");
