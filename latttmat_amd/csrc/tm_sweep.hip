@@ -22,9 +22,12 @@ namespace {
 
 // Workgroup size: 512 threads, or 1024 (one 128 KB tile per CU then still
 // keeps 16 wavefronts resident) -- an autotuner choice for FP64/FP32; a tile
-// candidate encodes the tile exponent in bits 0-7 and the 1024-thread choice
-// in bit 8.
+// candidate encodes the tile exponent in bits 0-7, the 1024-thread choice in
+// bit 8 and "keep lastbit in the first pass's tile" in bit 9 (the split first
+// pass writes 16-byte pieces of each state's aux values, which some widths
+// and precisions pay for more than the pass it saves).
 constexpr int SWEEP_WIDE = 256;
+constexpr int SWEEP_NOSPLIT = 512;
 
 struct SweepArgs {
   int L;
@@ -38,6 +41,9 @@ struct SweepArgs {
   int nbits;        // state bits owned by the pass
   int logW;
   int first, last;
+  int lbsplit;      // the lastbit values in separate workgroups (a first pass
+                    // that is not the last: no gate of it reads lastbit)
+  int64_t nblk;     // lbsplit: workgroups (tiles x lastbit values) before padding
   DiagSpec dg;
   double b1[2], b2[2], b3[2];
   double outfac;    // XDT: normfactor; ZBT: 1 (folded into the diagonal)
@@ -165,10 +171,24 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
                                                                const double *__restrict__ scale)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tileLog = a.nlbl + a.nal + a.nbits + a.logW;
+  // lastbit bits held in the tile (none when split over workgroups)
+  const int nlbt = a.lbsplit ? 0 : a.nlbl;
+  const int tileLog = nlbt + a.nal + a.nbits + a.logW;
   const int T = 1 << tileLog;
   C *lds = reinterpret_cast<C *>(smem);
   C *stab = lds + T;
+  // split: workgroup -> (tile, lastbit) with the lastbit values of a tile on
+  // one XCD (their interleaved writes to the state-major aux array then meet
+  // in the same L2)
+  uint64_t tile = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  int wlb = 0;
+  if (a.lbsplit) {
+    if (tile >= (uint64_t)((a.nblk + 7) / 8 * 8)) return;  // grid padding
+    const int64_t q = dev::xcd_remap((int64_t)tile, a.nblk);
+    if (q < 0) return;
+    wlb = (int)(q & (a.nlb - 1));
+    tile = (uint64_t)q >> a.nlbl;
+  }
   if (a.first || a.last)
     for (int i = threadIdx.x; i < 192; i += blockDim.x) stab[i] = tab[i];
   Coef<C> k;
@@ -179,7 +199,6 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
   }
   const ZCoef<C> zk = make_zcoef<C, DIR>(k);
   const int L = a.L;
-  const uint64_t tile = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
   const uint64_t innerBlocks = (1ull << a.bitlo) >> a.logW;
   const uint64_t o = tile / innerBlocks, ib = tile % innerBlocks;
   const uint64_t base = (o << (a.bitlo + a.nbits)) | (ib << a.logW);
@@ -193,8 +212,10 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
   // that order; the LDS tile keeps (aux, lastbit) in its high bits and XORs
   // them into the low bits (swz) so that those walks stay conflict-free.
   const int lgz = a.nlbl + a.nal;
-  const int nz = 1 << lgz;
-  const int zx = (1 << (lgz < posAux ? lgz : posAux)) - 1;
+  const int lgt = nlbt + a.nal;  // aux (and lastbit) bits in the tile
+  const int nz = 1 << lgt;
+  const int zoff = wlb << a.nal;  // this workgroup's lastbit in the HBM z index
+  const int zx = (1 << (lgt < posAux ? lgt : posAux)) - 1;
   auto swz = [&](int e) { return e ^ ((e >> posAux) & zx); };
   C sin = 1, sout = 1;
   if (scale) {
@@ -215,13 +236,13 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
       v[uu] = 0;
       if (g >= T) continue;
       const int z = g & (nz - 1);
-      const int rest = g >> lgz;
+      const int rest = g >> lgt;
       const int w = rest & (W - 1);
       const int pl = rest >> a.logW;
       const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
       if (a.first) {
         const int aux = z & (a.na - 1);
-        const int lb = z >> a.nal;
+        const int lb = (z | zoff) >> a.nal;
         if (MODEL == 0) {
           if (a.bc == 0) {
             const int auxw = (int)((P >> (L - 1)) & 1);
@@ -243,7 +264,7 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
             v[uu] = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P);
         }
       } else {
-        v[uu] = zin[(P << lgz) | (uint64_t)z];
+        v[uu] = zin[(P << lgz) | (uint64_t)(z | zoff)];
       }
     }
 #pragma unroll
@@ -251,7 +272,7 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
       const int g = g0 + uu * blockDim.x;
       if (g >= T) continue;
       const int z = g & (nz - 1);
-      const int rest = g >> lgz;
+      const int rest = g >> lgt;
       const int w = rest & (W - 1);
       const int pl = rest >> a.logW;
       lds[swz(w | (pl << a.logW) | (z << posAux))] = v[uu];
@@ -268,7 +289,7 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
       for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
         // insert zeros at the two varying positions (ascending)
         const uint32_t b0 = ins0(ins0((uint32_t)q, jb), posAux);
-        const int lb = (int)(b0 >> posLb);
+        const int lb = a.lbsplit ? wlb : (int)(b0 >> posLb);
         const int w = b0 & (W - 1);
         const int pl = (b0 >> a.logW) & pmask;
         const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
@@ -294,7 +315,7 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
         b0 = ins0(b0, jb + 1);
         b0 = ins0(b0, posAux);
         b0 = ins0(b0, posAux + 1);
-        const int lb = (int)(b0 >> posLb);
+        const int lb = a.lbsplit ? wlb : (int)(b0 >> posLb);
         const int w = b0 & (W - 1);
         const int pl = (b0 >> a.logW) & pmask;
         const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
@@ -325,12 +346,12 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
   if (!a.last) {
     for (int g = threadIdx.x; g < T; g += blockDim.x) {
       const int z = g & (nz - 1);
-      const int rest = g >> lgz;
+      const int rest = g >> lgt;
       const int w = rest & (W - 1);
       const int pl = rest >> a.logW;
       const int e = w | (pl << a.logW) | (z << posAux);
       const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
-      zout[(P << lgz) | (uint64_t)z] = lds[swz(e)];
+      zout[(P << lgz) | (uint64_t)(z | zoff)] = lds[swz(e)];
     }
   } else {
     const int NP = 1 << (a.nbits + a.logW);
@@ -346,30 +367,43 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
   }
 }
 
-struct SPass { int t0, ng, bitlo, nbits, logW; };
+struct SPass { int t0, ng, bitlo, nbits, logW, split; };
 
 // Tiles of 128 KB (2^14 FP64, 2^15 FP32, 2^13 dual elements) on the gfx950
-// LDS, half that on a smaller one; passes after the first take W >= 64 bytes
-// of consecutive states (each state's 2^lgz aux values are contiguous too).
+// LDS, half that on a smaller one.  Passes after the first move runs of W
+// consecutive states times their contiguous 2^(lastbit + aux bits) values,
+// at least 128 bytes.  A first pass that is not the last leaves lastbit to
+// the grid (no gate before the last reads it), which frees its bits for one
+// more gate: ZBT at L = 22 then takes two passes instead of three.
 int sweep_log_es(int es) { return es == 4 ? 2 : (es == 8 ? 3 : 4); }
 // default tile exponent first, then the autotuner's other candidate
 std::vector<int> sweep_tile_candidates(int es)
 {
   const bool big = g_lds_per_block >= (size_t)(136 * 1024);
   const int d = (big ? 17 : 16) - sweep_log_es(es);
-  if (es <= 8) return {d, d - 1, d | SWEEP_WIDE};
-  return {d, d - 1};
+  if (es <= 8)
+    return {d, d - 1, d | SWEEP_WIDE, d | SWEEP_NOSPLIT, (d - 1) | SWEEP_NOSPLIT,
+            d | SWEEP_WIDE | SWEEP_NOSPLIT};
+  return {d, d - 1, d | SWEEP_NOSPLIT};
 }
 
-std::vector<SPass> plan_sweep(int model, int L, int bc, int es, int tl)
+std::vector<SPass> plan_sweep(int model, int L, int bc, int es, int tl, bool split_ok)
 {
   const int les = sweep_log_es(es);
-  const int wl = 6 - les;
   const int fixed = model == 0 ? (bc == 0 ? 2 : 1) : 4;  // log2(nlb * na)
+  const int fixa = model == 0 ? 1 : 2;                     // log2(na)
+  const int wl = 7 - les - fixed > 0 ? 7 - les - fixed : 0;
   const int bpg = model == 0 ? 1 : 2;                      // state bits per gate
   const int ngates = model == 0 ? L : L / 2;
   std::vector<SPass> ps;
   int t = 0;
+  if (split_ok && fixed > fixa && ngates * bpg + fixed > tl) {
+    // first pass with lastbit in the grid; at least one gate left over
+    int ng = (tl - fixa) / bpg;
+    if (ng > ngates - 1) ng = ngates - 1;
+    ps.push_back({0, ng, 0, ng * bpg, 0, 1});
+    t = ng;
+  }
   while (t < ngates) {
     const int bitlo = t * bpg;
     int budget = tl - fixed;
@@ -387,7 +421,7 @@ std::vector<SPass> plan_sweep(int model, int L, int bc, int es, int tl)
       int extra = tl - fixed - nbits - logW;
       while (extra > 0 && logW < bitlo) { ++logW; --extra; }
     }
-    ps.push_back({t, ng, bitlo, nbits, logW});
+    ps.push_back({t, ng, bitlo, nbits, logW, 0});
     t += ng;
   }
   return ps;
@@ -402,11 +436,17 @@ void run_sweep_pass(const SPass &p, SweepArgs a, const void *x, const C *zin, C 
   a.bitlo = p.bitlo;
   a.nbits = p.nbits;
   a.logW = p.logW;
-  const int tileLog = a.nlbl + a.nal + p.nbits + p.logW;
-  const uint64_t ntile = 1ull << (a.L - p.nbits - p.logW);
+  a.lbsplit = p.split;
+  const int tileLog = (p.split ? 0 : a.nlbl) + a.nal + p.nbits + p.logW;
+  uint64_t ntile = 1ull << (a.L - p.nbits - p.logW);
+  if (p.split) {
+    // tiles x lastbit values, padded to a multiple of 8 for the XCD remap
+    a.nblk = (int64_t)(ntile << a.nlbl);
+    ntile = (uint64_t)((a.nblk + 7) / 8 * 8);
+  }
   dim3 grid;
   if (ntile <= 65535) grid = dim3((unsigned)ntile);
-  else grid = dim3(65536, (unsigned)(ntile / 65536));
+  else grid = dim3(65536, (unsigned)((ntile + 65535) / 65536));
   const size_t shm = ((size_t(1) << tileLog) + 192) * sizeof(C);
   if constexpr (sizeof(C) <= 8) {
     if (wide) {
@@ -461,7 +501,8 @@ void sweep_run_tl(const SweepPlan &plan, const void *in, void *out, void *work,
 {
   SweepArgs a = make_args<MODEL, DIR>(plan);
   const bool wide = (cand & SWEEP_WIDE) != 0;
-  auto ps = plan_sweep(MODEL, plan.L, plan.bc, (int)sizeof(C), cand & 0xFF);
+  auto ps = plan_sweep(MODEL, plan.L, plan.bc, (int)sizeof(C), cand & 0xFF,
+                       (cand & SWEEP_NOSPLIT) == 0);
   C *z = static_cast<C *>(work);
   const int np = (int)ps.size();
   for (int i = 0; i < np; ++i) {

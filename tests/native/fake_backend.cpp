@@ -47,6 +47,10 @@ void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind, hipStrea
 {
   std::memmove(dst, src, bytes);
 }
+void copy_large(void *dst, const void *src, size_t bytes, hipMemcpyKind, hipStream_t)
+{
+  std::memmove(dst, src, bytes);
+}
 namespace kry {
 template <typename T>
 unsigned multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *, double *out,
