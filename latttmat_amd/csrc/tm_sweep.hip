@@ -283,9 +283,17 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
   // ---- gates ----
   for (int g = 0; g < a.ng; ++g) {
     const int t = a.t0 + g;
+    // swz is linear over XOR and an item's base b0 has zeros where the gate's
+    // offsets have their bits: each LDS address is swz(b0) ^ swz(offset),
+    // with the offsets' swizzles computed once per gate
     if (MODEL == 0) {
       const int jb = a.logW + (t - a.bitlo);  // tile position of state bit t
       const int nitems = T >> 2;
+      int off[2][2];
+#pragma unroll
+      for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) off[ax][u] = swz((ax << posAux) | (u << jb));
       for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
         // insert zeros at the two varying positions (ascending)
         const uint32_t b0 = ins0(ins0((uint32_t)q, jb), posAux);
@@ -296,20 +304,26 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
         int c;
         if (t + 1 < L) c = (int)((P >> (t + 1)) & 1);
         else c = (a.bc == 0) ? lb : 0;
+        const int sb = swz((int)b0);
         C v[2][2];
 #pragma unroll
         for (int ax = 0; ax < 2; ++ax)
 #pragma unroll
-          for (int u = 0; u < 2; ++u) v[ax][u] = lds[swz(b0 | (ax << posAux) | (u << jb))];
+          for (int u = 0; u < 2; ++u) v[ax][u] = lds[sb ^ off[ax][u]];
         xdt_gate(v, c, k);
 #pragma unroll
         for (int ax = 0; ax < 2; ++ax)
 #pragma unroll
-          for (int u = 0; u < 2; ++u) lds[swz(b0 | (ax << posAux) | (u << jb))] = v[ax][u];
+          for (int u = 0; u < 2; ++u) lds[sb ^ off[ax][u]] = v[ax][u];
       }
     } else {
       const int jb = a.logW + (2 * t - a.bitlo);  // tile position of bit 2t
       const int nitems = T >> 4;
+      int off[4][4];
+#pragma unroll
+      for (int ax = 0; ax < 4; ++ax)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) off[ax][u] = swz((ax << posAux) | (u << jb));
       for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
         uint32_t b0 = ins0((uint32_t)q, jb);
         b0 = ins0(b0, jb + 1);
@@ -327,16 +341,17 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
           fl = lb >> 1;
           fr = lb & 1;
         }
+        const int sb = swz((int)b0);
         C v[4][4];
 #pragma unroll
         for (int ax = 0; ax < 4; ++ax)
 #pragma unroll
-          for (int u = 0; u < 4; ++u) v[ax][u] = lds[swz(b0 | (ax << posAux) | (u << jb))];
+          for (int u = 0; u < 4; ++u) v[ax][u] = lds[sb ^ off[ax][u]];
         zbt_gate<C, DIR>(v, fl, fr, zk);
 #pragma unroll
         for (int ax = 0; ax < 4; ++ax)
 #pragma unroll
-          for (int u = 0; u < 4; ++u) lds[swz(b0 | (ax << posAux) | (u << jb))] = v[ax][u];
+          for (int u = 0; u < 4; ++u) lds[sb ^ off[ax][u]] = v[ax][u];
       }
     }
     __syncthreads();
