@@ -89,15 +89,33 @@ __device__ __forceinline__ void xdt_gate(C v[2][2], int c, const Coef<C> &k)
 // controls (fl, fr).  Output aux p = (pl,pr) takes the old bit pair, the
 // new bit pair is s = (sl, sr):
 //   out[p][s] = A[p][s] F_{fl,fr}[s] sum_aq Cm[s][aq] in[aq][p]
-// with the control-independent factors A, Cm tabulated once per thread
-// (ZCoef) and F built from two selects per item.
 //   T   (DIR 0): A = b1[sr^pl] b3[sr^pr] b3[sl^pl], Cm = b1[sr^al] b2[sr^ar] b2[sl^al],
 //                F = b2[sr^fr] b2[sl^fl]
 //   T^T (DIR 1): A = b1[sl^pr] b3[sr^pr] b3[sl^pl], Cm = b2[sr^ar] b2[sl^al],
 //                F = b1[sl^fr] b2[sr^fr] b2[sl^fl]
+// The sum over aq factors into two 2x2 stages in ratio form (b[x] = b[0]
+// rho^x): w[al][sr] = in[al,sr] + rho2 in[al,1-sr] over ar, then over al
+//   DIR 0: acc = w[0][sr] + kap[s] w[1][sr],  kap = rho1^(1-2sr) rho2^(1-2sl)
+//   DIR 1: acc = w[sl][sr] + rho2 w[1-sl][sr]
+// one FMA per value each, and out = (Q[p][s] F[s]) acc with the
+// control-independent Q (A and the pulled-out factors) tabulated once per
+// thread.
+template <typename C>
+__device__ __forceinline__ C cdiv(C a, C b)
+{
+  return a / b;
+}
+template <>
+__device__ __forceinline__ dev::Dual cdiv<dev::Dual>(dev::Dual a, dev::Dual b)
+{
+  const double q = a.v / b.v;
+  return dev::Dual(q, (a.d - q * b.d) / b.v);
+}
+
 template <typename C>
 struct ZCoef {
-  C A[4][4], Cm[4][4];
+  C Q[4][4], kap[4];
+  C r2;
   C b1[2], b2[2];
 };
 
@@ -105,21 +123,24 @@ template <typename C, int DIR>
 __device__ __forceinline__ ZCoef<C> make_zcoef(const Coef<C> &k)
 {
   ZCoef<C> z;
+  const C r1 = cdiv(k.b1[1], k.b1[0]), r2 = cdiv(k.b2[1], k.b2[0]);
+  const C ir1 = cdiv(C(1.0), r1), ir2 = cdiv(C(1.0), r2);
+  const C pre = k.b1[0] * k.b2[0] * k.b2[0];
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int sl = s >> 1, sr = s & 1;
+    z.kap[s] = (sr ? ir1 : r1) * (sl ? ir2 : r2);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int ql = q >> 1, qr = q & 1;  // q = p (for A) or aq (for Cm)
-      if (DIR == 0) {
-        z.A[q][s] = k.b1[sr ^ ql] * k.b3[sr ^ qr] * k.b3[sl ^ ql];
-        z.Cm[s][q] = k.b1[sr ^ ql] * k.b2[sr ^ qr] * k.b2[sl ^ ql];
-      } else {
-        z.A[q][s] = k.b1[sl ^ qr] * k.b3[sr ^ qr] * k.b3[sl ^ ql];
-        z.Cm[s][q] = k.b2[sr ^ qr] * k.b2[sl ^ ql];
-      }
+    for (int p = 0; p < 4; ++p) {
+      const int pl = p >> 1, pr = p & 1;
+      if (DIR == 0)
+        z.Q[p][s] = k.b1[sr ^ pl] * k.b3[sr ^ pr] * k.b3[sl ^ pl] * pre * (sr ? r1 : C(1.0)) *
+                    (sl ? r2 : C(1.0));
+      else
+        z.Q[p][s] = k.b1[sl ^ pr] * k.b3[sr ^ pr] * k.b3[sl ^ pl] * (k.b2[0] * k.b2[0]);
     }
   }
+  z.r2 = r2;
   for (int i = 0; i < 2; ++i) {
     z.b1[i] = k.b1[i];
     z.b2[i] = k.b2[i];
@@ -145,15 +166,22 @@ __device__ __forceinline__ void zbt_gate(C v[4][4], int fl, int fr, const ZCoef<
     F[2] *= g1;
     F[3] *= g1;
   }
+  C w[2][2][4];
+#pragma unroll
+  for (int al = 0; al < 2; ++al)
+#pragma unroll
+    for (int sr = 0; sr < 2; ++sr)
+#pragma unroll
+      for (int p = 0; p < 4; ++p) w[al][sr][p] = v[al * 2 + sr][p] + z.r2 * v[al * 2 + (sr ^ 1)][p];
   C o[4][4];
 #pragma unroll
-  for (int p = 0; p < 4; ++p) {
+  for (int s = 0; s < 4; ++s) {
+    const int sl = s >> 1, sr = s & 1;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      C acc = z.Cm[s][0] * v[0][p];
-#pragma unroll
-      for (int aq = 1; aq < 4; ++aq) acc += z.Cm[s][aq] * v[aq][p];
-      o[p][s] = (z.A[p][s] * F[s]) * acc;
+    for (int p = 0; p < 4; ++p) {
+      const C acc = DIR == 0 ? w[0][sr][p] + z.kap[s] * w[1][sr][p]
+                             : w[sl][sr][p] + z.r2 * w[sl ^ 1][sr][p];
+      o[p][s] = (z.Q[p][s] * F[s]) * acc;
     }
   }
 #pragma unroll
