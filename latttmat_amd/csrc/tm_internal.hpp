@@ -126,8 +126,8 @@ class DevBuf {
 };
 
 // Device and pinned buffers plus captured restart-cycle graphs that the
-// eigen driver keeps on an operator between solves (slot numbers are
-// assigned by the driver).  A graph is keyed by every pointer and size its
+// eigensolver keeps on an operator between solves (slot numbers are
+// assigned by the eigensolver).  A graph is keyed by every pointer and size its
 // launches embed, so a key match means identical kernel arguments.
 struct SolverCache {
   static constexpr int NDEV = 24, NPIN = 4;

@@ -50,7 +50,7 @@ struct Ritz {
   std::vector<double> res;        // Krylov residual estimates
 };
 
-// What the driver needs of a Krylov-Schur run, whatever the basis storage.
+// What solve_impl needs of a Krylov-Schur run, whatever the basis storage.
 class KrylovRun {
  public:
   virtual ~KrylovRun() = default;

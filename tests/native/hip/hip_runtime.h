@@ -18,7 +18,7 @@ inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
 inline hipError_t hipGetLastError() { return 0; }
 inline const char *hipGetErrorString(hipError_t) { return "fake"; }
 struct dim3 { unsigned x, y, z; dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {} };
-// graphs: capture is refused, so the driver stays eager
+// graphs: capture is refused, so the eigensolver stays eager
 typedef void *hipGraph_t;
 typedef void *hipGraphExec_t;
 enum { hipStreamCaptureModeThreadLocal = 1 };

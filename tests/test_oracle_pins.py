@@ -195,8 +195,8 @@ def test_orbits_3d_closure(oracle):
 
 
 def test_orbit_count_3d_5x5_order(oracle):
-    # group order 200 on a 5x5 layer (no rotation): the count the survey
-    # computed by Burnside, 170,440 -- checked only through the sum of sizes
+    # group order 200 on a 5x5 layer (no rotation): Burnside's count is
+    # 170,440 -- checked only through the sum of sizes
     # at a smaller layer here; the 5x5 count is asserted on the GPU.
     reps, dups = oracle.orbits_x3d(3, 4, 1, 0)
     assert int(dups.sum()) == 1 << 12
