@@ -8,7 +8,7 @@ print, TMat/tests.cpp:17-196), so these checks are what pins the oracle."""
 import numpy as np
 import pytest
 
-from tests import physics
+from tests import groups, physics
 
 
 def test_bit_primitives_exhaustive(oracle):
@@ -161,6 +161,30 @@ def test_orbit_counts_burnside(oracle, L):
     assert len(oracle.orbits_x2d(L, 0, 1)[0]) == BURNSIDE_FLIP[L] == _burnside(L, True)
 
 
+@pytest.mark.parametrize("L", range(4, 31))
+def test_x_closed_form_equals_cycle_index(L):
+    """The dihedral closed form above against the cycle-index count."""
+    gx = groups.gens_x(L)
+    assert _burnside(L, False) == groups.burnside_count(gx, False)
+    assert _burnside(L, True) == groups.burnside_count(gx, True)
+
+
+@pytest.mark.parametrize("censym", [0, 1])
+def test_orbit_tables_against_groups(oracle, censym):
+    """Orbit tables of every group (X, Z, two-layer Y, 3D with and without
+    the quarter turn) against the group itself: representatives are orbit
+    minima, sizes are image counts, counts are Burnside's."""
+    for L in range(4, 21):
+        groups.check_orbit_tables(*oracle.orbits_x2d(L, 0, censym), groups.gens_x(L), censym)
+        groups.check_orbit_tables(*oracle.orbits_z2d(L, censym), groups.gens_z(L), censym)
+    for L in range(3, 10):
+        reps, dups, _ = oracle.orbits_y2d(L, censym)
+        groups.check_orbit_tables(reps, dups, groups.gens_y(L), censym)
+    for s1, s2, rot in ((3, 3, 1), (3, 4, 0), (4, 4, 1), (4, 4, 0), (4, 5, 0)):
+        groups.check_orbit_tables(*oracle.orbits_x3d(s1, s2, censym, rot),
+                                  groups.gens_3d(s1, s2, rot), censym)
+
+
 def test_orbits_3d_closure(oracle):
     """3D orbits (translations x row/column mirrors x flip [x rotation]) by
     brute-force closure on 3x3 and 3x4 layers."""
@@ -195,9 +219,12 @@ def test_orbits_3d_closure(oracle):
 
 
 def test_orbit_count_3d_5x5_order(oracle):
-    # group order 200 on a 5x5 layer (no rotation): Burnside's count is
-    # 170,440 -- checked only through the sum of sizes
-    # at a smaller layer here; the 5x5 count is asserted on the GPU.
+    # group order 200 on a 5x5 layer without the quarter turn (400 with it):
+    # the literal counts the full-size GPU tests assert, from the cycle index
+    assert len(groups.closure(groups.gens_3d(5, 5, False))) * 2 == 200
+    assert groups.burnside_count(groups.gens_3d(5, 5, False), True) == 170440
+    assert groups.burnside_count(groups.gens_3d(5, 5, True), True) == 86056
+    assert groups.burnside_count(groups.gens_x(28), False) == 4806078
     reps, dups = oracle.orbits_x3d(3, 4, 1, 0)
     assert int(dups.sum()) == 1 << 12
 
