@@ -217,6 +217,26 @@ tm_status tm_solve_batch(tm_op *const *ops, int n, const tm_solve_opts *opts,
 tm_status tm_grad_hf(const tm_op *op, const double *l, const double *r,
                      double dlam[4], double dlnnf[4]);
 
+/* Observables of the leading eigenvector pair, reduced on the device: the
+ * figures the reference writes to X_phys.dat (TMat/pfunc.hpp:167-200).
+ *   out[0..3]  the magnetisation moments <m^k>/L, k = 1..4
+ *              (SymMatPhys::moments, TMat/tmatphys.cpp:13-37; GenMatPhys and
+ *              ZMatVecProd2D, :71-98, :106-132, normalised by sum ql qr;
+ *              the reduced operators over their representatives,
+ *              TMat/tmatreduce.cpp:565-590, 882-909, 1748-1775,
+ *              TMat/tmat3d.cpp:137-163; zeros for Y2AT, TMat/tmat.cpp:312-317);
+ *   out[4..]   the operator's extra observables (phys_extend): <node>,
+ *              <node^2> (TMat/tmatphys.cpp:39-57, TMat/tmatreduce.cpp:593-613;
+ *              NaN for the 3D operators, TMat/tmat3d.cpp:165-177), or
+ *              <|m_st|>/L, <|m|>/L, <m_st^2>/L^2, <m^2>/L^2 for the reduced
+ *              DNNI operator (TMat/tmatreduce.cpp:1289-1320), or none;
+ *   *nout      4 + the number of extra observables (out holds up to 8).
+ * qr and ql are host vectors of length cols() as returned by tm_solve; ql
+ * may be NULL (then ql = qr, as TMat/pfunc.hpp:170-172 does), and the
+ * symmetric operators weight by ql^2. */
+tm_status tm_op_moments(const tm_op *op, const double *ql, const double *qr, double out[8],
+                        int *nout);
+
 /* Thread-local message for the last non-OK status. */
 const char *tm_last_error(void);
 /* Library version string. */

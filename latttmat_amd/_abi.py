@@ -24,7 +24,7 @@ EXPORTS = [
     "tm_op_create", "tm_op_destroy", "tm_op_rows", "tm_op_cols", "tm_op_normfactor",
     "tm_op_is_symmetric", "tm_op_kind", "tm_op_suggested_ncv", "tm_op_layer_size",
     "tm_op_matvec_count", "tm_op_apply", "tm_op_apply_dev", "tm_op_apply_prec",
-    "tm_op_time_apply", "tm_op_orbits",
+    "tm_op_time_apply", "tm_op_orbits", "tm_op_moments",
     "tm_solve_opts_default", "tm_solve", "tm_solve_batch", "tm_grad_hf", "tm_last_error",
     "tm_version",
 ]
@@ -108,6 +108,8 @@ def lib():
         L.tm_op_time_apply.argtypes = [vp, i32, i32, i32, i32, i32, pd]
         L.tm_op_orbits.restype = i32
         L.tm_op_orbits.argtypes = [vp, vp, vp]
+        L.tm_op_moments.restype = i32
+        L.tm_op_moments.argtypes = [vp, pd, pd, pd, C.POINTER(i32)]
         L.tm_solve_opts_default.restype = None
         L.tm_solve_opts_default.argtypes = [C.POINTER(SolveOpts)]
         L.tm_solve.restype = i32
@@ -238,6 +240,18 @@ class TransferMatrix:
         dups = np.empty(n, np.uint16)
         _check(lib().tm_op_orbits(self._h, reps.ctypes.data, dups.ctypes.data))
         return reps, dups
+
+    def moments(self, qr, ql=None):
+        """X_phys.dat figures of an eigenvector pair, reduced on the GPU: the
+        four magnetisation moments, then the kind's extra observables."""
+        qr = np.ascontiguousarray(qr, dtype=np.float64)
+        if ql is not None:
+            ql = np.ascontiguousarray(ql, dtype=np.float64)
+        out = np.empty(8)
+        nout = C.c_int(0)
+        _check(lib().tm_op_moments(self._h, None if ql is None else _pd(ql), _pd(qr), _pd(out),
+                                   C.byref(nout)))
+        return out[:nout.value].copy()
 
     def grad_hf(self, l, r):
         l = np.ascontiguousarray(l, dtype=np.float64)

@@ -29,6 +29,7 @@ COMMON = [
 LIB_SOURCES = [
     "tm_api.cpp", "tm_ops.cpp", "tm_solver.cpp", "tm_dense.cpp", "tm_grad.hip",
     "tm_kron.hip", "tm_reduced.hip", "tm_sweep.hip", "tm_krylov.hip", "tm_axial.hip",
+    "tm_phys.hip",
 ]
 
 

@@ -258,6 +258,62 @@ tm_status tm_solve_batch(tm_op *const *ops, int n, const tm_solve_opts *opts, tm
   return first;
 }
 
+tm_status tm_op_moments(const tm_op *op, const double *ql, const double *qr, double out[8],
+                        int *nout)
+{
+  if (!op || !qr || !out || !nout) return fail(TM_EINVAL, "null argument");
+  using namespace ltm;
+  int pk, bc = 0;
+  switch (op->kind) {
+    case TM_KIND_XAT: pk = PHYS_SYM2D; bc = op->bc; break;      // TMat/tmat.cpp:39
+    case TM_KIND_XAM2D: pk = PHYS_SYM2D; bc = op->bc; break;    // ext & 1, TMat/tmatreduce.cpp:338
+    case TM_KIND_XDT: pk = PHYS_SYM2D; break;                   // periodic, TMat/tmat.cpp:322-323
+    case TM_KIND_XAT3D: case TM_KIND_XAM3D: pk = PHYS_SYM3D; break;
+    case TM_KIND_XDM: pk = PHYS_DNNI; break;
+    case TM_KIND_ZBT: case TM_KIND_ZBM: pk = PHYS_GEN; break;
+    case TM_KIND_YAT: pk = PHYS_GEN_LOW; break;
+    case TM_KIND_YAM: case TM_KIND_YBM: pk = PHYS_GEN_HIGH; break;
+    case TM_KIND_Y2AT: pk = PHYS_ZERO; break;
+    default: return fail(TM_EINVAL, "unknown operator kind");
+  }
+  return guarded([&] {
+    const int L = op->L;
+    for (int k = 0; k < 8; ++k) out[k] = 0.0;
+    if (pk == PHYS_ZERO) {
+      *nout = 4;
+      return TM_OK;
+    }
+    const bool gen = pk == PHYS_GEN || pk == PHYS_GEN_LOW || pk == PHYS_GEN_HIGH;
+    const size_t bytes = sizeof(double) * op->ncol;
+    DevBuf dl(bytes), dr(gen ? bytes : 0), work(phys_work_bytes());
+    copy_large(dl.get(), ql ? ql : qr, bytes, hipMemcpyHostToDevice, op->stream);
+    if (gen) copy_large(dr.get(), qr, bytes, hipMemcpyHostToDevice, op->stream);
+    double s[9];
+    phys_sums(pk, dl.as<double>(), gen ? dr.as<double>() : nullptr,
+              is_reduced(op->kind) ? op->reps.as<uint64_t>() : nullptr, op->ncol, L, bc,
+              work.get(), s, op->stream);
+    const double div = gen ? L * s[0] : (double)L;
+    for (int k = 0; k < 4; ++k) out[k] = s[1 + k] / div;
+    if (pk == PHYS_SYM2D) {
+      out[4] = s[5];
+      out[5] = s[6];
+      *nout = 6;
+    } else if (pk == PHYS_SYM3D) {
+      out[4] = out[5] = NAN;
+      *nout = 6;
+    } else if (pk == PHYS_DNNI) {
+      out[4] = s[5] / L;
+      out[5] = s[6] / L;
+      out[6] = s[7] / ((double)L * L);
+      out[7] = s[8] / ((double)L * L);
+      *nout = 8;
+    } else {
+      *nout = 4;
+    }
+    return TM_OK;
+  });
+}
+
 tm_status tm_grad_hf(const tm_op *cop, const double *l, const double *r, double dlam[4],
                      double dlnnf[4])
 {

@@ -510,4 +510,13 @@ tm_op *make_op(const tm_params &p);
 void grad_numerators(tm_op *op, const double *l_dev, const double *r_dev, double num[5]);
 // d ln(normfactor)/d theta for the 4 params.
 void dlnnf(const tm_op *op, double out[4]);
+// Observable kinds of an eigenvector pair (which weight, which bits carry
+// the magnetisation, which extra observables).
+enum PhysKind { PHYS_SYM2D, PHYS_SYM3D, PHYS_DNNI, PHYS_GEN, PHYS_GEN_LOW, PHYS_GEN_HIGH,
+                PHYS_ZERO };
+// sums[0] = sum w, sums[1..4] = sum w m^k, sums[5..8] = the kind's extra
+// sums, over n device entries of ql (and qr), states reps[i] or i.
+void phys_sums(int pk, const double *ql, const double *qr, const uint64_t *reps, int64_t n, int L,
+               int bc, void *work, double sums[9], hipStream_t s);
+size_t phys_work_bytes();
 }  // namespace ltm

@@ -119,101 +119,21 @@ void write_column(std::ostream &os, const std::vector<T> &v)
   }
 }
 
-// Phys output X_phys.dat: first line the TMAT_NMOMENTS=4 magnetisation
-// moments <m^k>/L (m = 2 popcount(s) - L), second line the extra
-// observables.  Symmetric operators weight state s by q_s^2 and add
-// <node>, <node^2> (TMat/tmatphys.cpp:13-57; reduced operators sum over
-// orbit representatives, TMat/tmatreduce.cpp:565-613; the 3D reduced
-// operator reports NaN for the node counts, TMat/tmat3d.cpp:165-177).
-// Non-symmetric operators weight by ql_s qr_s / sum(ql qr) and have no extra
-// observables (TMat/tmatphys.cpp:106-132, 98-104; the reduced ZBM likewise
-// over its representatives, TMat/tmatreduce.cpp:1748-1781).  The reduced DNNI
-// operator reports <|m_st|>/L, <|m|>/L, <m_st^2>/L^2, <m^2>/L^2 instead, m_st
-// the magnetisation with every odd site flipped (TMat/tmatreduce.cpp:1289-1317).
-// The axial two-layer operators use the generic moments over all 4^L states
-// with the low layer's magnetisation (TMat/tmatphys.cpp:71-98); Y2AT writes
-// zeros (its moments are a stub, TMat/tmat.cpp:304-309).
-// The reduced axial operators weight by the representative's current (high)
-// layer (TMat/tmatreduce.cpp:882-909).
-enum class PhysKind { Sym2D, Sym3D, Gen, Dnni, GenLayers, GenHigh, Zero };
-
-int nnode(uint64_t s, int L, int bc)  // TMat/utility.cpp:33-47
+// Phys output X_phys.dat (TMat/pfunc.hpp:167-200): first line the
+// TMAT_NMOMENTS=4 magnetisation moments, second line the operator's extra
+// observables, both as tm_op_moments reduces them on the GPU (which
+// observables each operator kind reports is documented there).
+void write_phys(const std::string &fn, const tm_op *op, const double *ql, const double *qr)
 {
-  uint64_t sh = s >> 1;
-  if (!bc) sh |= (s & 1) << (L - 1);
-  else sh |= (s & (uint64_t)(L - 1)) << 1;  // mask as written there
-  return __builtin_popcountll(s ^ sh);
-}
-
-void write_phys(const std::string &fn, PhysKind pk, int L, int bc,
-                const std::vector<uint64_t> &states, const std::vector<double> &ql,
-                const std::vector<double> &qr)
-{
-  double mom[4] = {0, 0, 0, 0}, dnorm = 0;
-  std::vector<double> ext;
-  if (pk == PhysKind::Sym2D) ext.assign(2, 0.0);
-  if (pk == PhysKind::Sym3D) ext.assign(2, NAN);
-  if (pk == PhysKind::Dnni) ext.assign(4, 0.0);
-  const uint64_t mask = L >= 64 ? ~0ull : (1ull << L) - 1;
-  const size_t n = qr.size();
-  for (size_t i = 0; i < n; ++i) {
-    const uint64_t s = states.empty() ? (uint64_t)i : states[i];
-    const uint64_t sl = pk == PhysKind::GenLayers ? (s & mask) : pk == PhysKind::GenHigh ? (s >> L) : s;
-    const double m = 2.0 * __builtin_popcountll(sl) - L;
-    const bool gen = pk == PhysKind::Gen || pk == PhysKind::GenLayers ||
-                     pk == PhysKind::GenHigh || pk == PhysKind::Zero;
-    const double w = gen ? ql[i] * qr[i] : ql[i] * ql[i];
-    dnorm += w;
-    double mm = m;
-    for (int k = 0; k < 4; ++k) {
-      mom[k] += mm * w;
-      mm *= m;
-    }
-    if (pk == PhysKind::Sym2D) {
-      const double nd = nnode(s, L, bc);
-      ext[0] += nd * w;
-      ext[1] += nd * nd * w;
-    }
-    if (pk == PhysKind::Dnni) {
-      const double ms = 2.0 * __builtin_popcountll((s ^ 0xaaaaaaaaaaaaaaaaull) & mask) - L;
-      ext[0] += std::fabs(ms) * w;
-      ext[1] += std::fabs(m) * w;
-      ext[2] += ms * ms * w;
-      ext[3] += m * m * w;
-    }
-  }
-  if (pk == PhysKind::Dnni) {
-    ext[0] /= L;
-    ext[1] /= L;
-    ext[2] /= (double)L * L;
-    ext[3] /= (double)L * L;
-  }
-  const double div = (pk == PhysKind::Gen || pk == PhysKind::GenLayers || pk == PhysKind::GenHigh)
-                         ? L * dnorm
-                         : (double)L;
-  if (pk == PhysKind::Zero) mom[0] = mom[1] = mom[2] = mom[3] = 0.0;
+  double v[8];
+  int nv = 0;
+  latttmat::check(tm_op_moments(op, ql, qr, v, &nv));
   std::ofstream f(fn);
   f << std::setprecision(10);
-  for (int k = 0; k < 4; ++k) f << (k ? "\t" : "") << mom[k] / div;
-  for (size_t k = 0; k < ext.size(); ++k) f << (k ? '\t' : '\n') << ext[k];
+  for (int k = 0; k < 4; ++k) f << (k ? "\t" : "") << v[k];
+  for (int k = 4; k < nv; ++k) f << (k > 4 ? '\t' : '\n') << v[k];
 }
 
-// Boundary flag the operator's physics uses for node counting: the XAT
-// boundary code (TMat/tmat.cpp:39), ext & 1 for the reduced operator
-// (TMat/tmatreduce.cpp:338); XDT keeps the periodic default
-// (TMat/tmat.cpp:322-323).
-int phys_bc(const ReadInput &in)
-{
-  if (in.model != 'A' || in.dim != 2) return 0;
-  switch (in.axialdir) {
-    case 9: case 13: return 1;
-    case 17: return 5;
-    case 18: return 9;
-    case 19: return 13;
-    case 20: return 17;
-    default: return 0;
-  }
-}
 
 }  // namespace
 
@@ -322,22 +242,7 @@ int main(int argc, char **argv)
         ql = qr;
       }
       // X_phys.dat
-      const tm_kind kind = tm_op_kind(op.handle());
-      std::vector<uint64_t> states;
-      if (kind == TM_KIND_XAM2D || kind == TM_KIND_XAM3D || kind == TM_KIND_XDM ||
-          kind == TM_KIND_ZBM || kind == TM_KIND_YAM || kind == TM_KIND_YBM) {
-        std::vector<uint16_t> dups(n);
-        states.resize(n);
-        latttmat::check(tm_op_orbits(op.handle(), states.data(), dups.data()));
-      }
-      const PhysKind pk = kind == TM_KIND_YAT    ? PhysKind::GenLayers
-                          : (kind == TM_KIND_YAM || kind == TM_KIND_YBM) ? PhysKind::GenHigh
-                          : kind == TM_KIND_Y2AT ? PhysKind::Zero
-                          : !sym                 ? PhysKind::Gen
-                          : kind == TM_KIND_XDM ? PhysKind::Dnni
-                          : (kind == TM_KIND_XAM3D || kind == TM_KIND_XAT3D) ? PhysKind::Sym3D
-                                                                             : PhysKind::Sym2D;
-      write_phys(in.outprefix + "_phys.dat", pk, op.getLayerSize(), phys_bc(in), states, ql, qr);
+      write_phys(in.outprefix + "_phys.dat", op.handle(), ql.data(), qr.data());
     }
     if (verbose)
       std::cerr << "[tmat] matvecs fp64 " << out.matvecs_f64 << " fp32 " << out.matvecs_f32

@@ -81,6 +81,8 @@ def lib():
         L.or_tvec_xdt.argtypes = [i32, i32, dbl, dbl, pdbl]
         L.or_tvec_zbt.restype = None
         L.or_tvec_zbt.argtypes = [i32, dbl, dbl, dbl, pdbl]
+        L.or_op_moments.restype = None
+        L.or_op_moments.argtypes = [C.c_void_p, C.c_void_p, pdbl, pdbl, C.POINTER(i32)]
         L.or_num_threads.restype = i32
         L.or_num_threads.argtypes = []
         _lib = L
@@ -130,6 +132,19 @@ class OracleOp:
 
     def tvec(self):
         return np.ctypeslib.as_array(lib().or_op_tvec(self._h), shape=(self.n,)).copy()
+
+    def moments(self, qr, ql=None):
+        """X_phys.dat figures: four moments, then the extra observables."""
+        qr = np.ascontiguousarray(qr, dtype=np.float64)
+        assert qr.shape == (self.n,)
+        if ql is not None:
+            ql = np.ascontiguousarray(ql, dtype=np.float64)
+            assert ql.shape == (self.n,)
+        out = np.empty(8)
+        nout = C.c_int(0)
+        lib().or_op_moments(self._h, None if ql is None else ql.ctypes.data, _p(qr), _p(out),
+                            C.byref(nout))
+        return out[:nout.value].copy()
 
     def reps(self):
         p = lib().or_op_reps(self._h)

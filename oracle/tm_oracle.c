@@ -848,6 +848,100 @@ void or_op_destroy(or_op *op)
   free(op);
 }
 
+/* TMat/utility.cpp:33-47 (open boundary: the mask nspin-1 as written there) */
+static int or_nnode(uint64_t s, int L, int bc)
+{
+  uint64_t sh = s >> 1;
+  if (!bc) sh |= (s & 1ull) << (L - 1);
+  else sh |= (s & (uint64_t)(L - 1)) << 1;
+  return POP(s ^ sh);
+}
+
+void or_op_moments(const or_op *op, const double *ql, const double *qr, double out[8], int *nout)
+{
+  const int L = op->L;
+  const int64_t n = op->ncol;
+  const uint64_t lmask = (L >= 64) ? ~0ull : (1ull << L) - 1;
+  int k;
+  if (!ql) ql = qr;
+  for (k = 0; k < 8; ++k) out[k] = 0.0;
+  *nout = 4;
+  switch (op->kind) {
+    case OR_Y2AT:
+      return;  /* tmat.cpp:312-317 */
+    case OR_XAT: case OR_XDT: case OR_XAM2D: case OR_XAT3D: case OR_XAM3D: case OR_XDM: {
+      /* SymMatPhys::moments (tmatphys.cpp:13-37) / XAMMatVecProd2D::moments
+       * (tmatreduce.cpp:565-590): weight |q|^2, divide by size */
+      const int red = op->kind == OR_XAM2D || op->kind == OR_XAM3D || op->kind == OR_XDM;
+      /* node boundary: XAT its own code (tmat.cpp:39), XAM ext & 1
+       * (tmatreduce.cpp:338), XDT the MatPhys default 0 (tmat.cpp:320-323) */
+      const int nbc = op->kind == OR_XAT ? op->bc : op->kind == OR_XAM2D ? (op->bc & 1) : 0;
+      double ext[4] = {0, 0, 0, 0};
+      int64_t i;
+      for (i = 0; i < n; ++i) {
+        const uint64_t s = red ? op->reps[i] : (uint64_t)i;
+        const double dpara = 2.0 * POP(s) - L;
+        const double w = ql[i] * ql[i];
+        double dm = dpara;
+        for (k = 0; k < 4; ++k) {
+          out[k] += dm * w;
+          dm *= dpara;
+        }
+        if (op->kind == OR_XDM) {  /* tmatreduce.cpp:1289-1320 */
+          const double ms = 2.0 * POP((s ^ 0xaaaaaaaaaaaaaaaaull) & lmask) - L;
+          ext[0] += fabs(ms) * w;
+          ext[1] += fabs(dpara) * w;
+          ext[2] += ms * ms * w;
+          ext[3] += dpara * dpara * w;
+        } else if (op->kind != OR_XAT3D && op->kind != OR_XAM3D) {  /* tmatphys.cpp:39-57 */
+          const double nd = or_nnode(s, L, nbc);
+          ext[0] += nd * w;
+          ext[1] += nd * nd * w;
+        }
+      }
+      for (k = 0; k < 4; ++k) out[k] /= L;
+      if (op->kind == OR_XDM) {
+        out[4] = ext[0] / L;
+        out[5] = ext[1] / L;
+        out[6] = ext[2] / ((double)L * L);
+        out[7] = ext[3] / ((double)L * L);
+        *nout = 8;
+      } else if (op->kind == OR_XAT3D || op->kind == OR_XAM3D) {
+        out[4] = out[5] = NAN;  /* tmat3d.cpp:165-177 */
+        *nout = 6;
+      } else {
+        out[4] = ext[0];
+        out[5] = ext[1];
+        *nout = 6;
+      }
+      return;
+    }
+    default: {
+      /* GenMatPhys (tmatphys.cpp:71-98: low layer of a 4^L state),
+       * ZMatVecProd2D (:106-132), ZBM (tmatreduce.cpp:1748-1775), YAM/YBM
+       * (tmatreduce.cpp:882-909: the representative's high layer):
+       * weight ql qr, divide by size * sum ql qr; no extra observables */
+      double dnorm = 0;
+      int64_t i;
+      for (i = 0; i < n; ++i) {
+        uint64_t s = op->reps ? op->reps[i] : (uint64_t)i;
+        if (op->kind == OR_YAT) s &= lmask;
+        if (op->kind == OR_YAM || op->kind == OR_YBM) s >>= L;
+        const double dpara = 2.0 * POP(s) - L;
+        const double w = ql[i] * qr[i];
+        double dm = dpara;
+        dnorm += w;
+        for (k = 0; k < 4; ++k) {
+          out[k] += dm * w;
+          dm *= dpara;
+        }
+      }
+      for (k = 0; k < 4; ++k) out[k] /= L * dnorm;
+      return;
+    }
+  }
+}
+
 int64_t or_op_ncol(const or_op *op) { return op->ncol; }
 double or_op_normfactor(const or_op *op) { return op->nf; }
 const double *or_op_tvec(const or_op *op) { return op->tvec; }

@@ -122,6 +122,19 @@ void or_tvec_xat(int L, int bc, double betaJ, double kappa, double h, double *V)
 void or_tvec_xdt(int L, int bc, double betaJ, double h, double *V);               /* tmat.hpp:437-504 */
 void or_tvec_zbt(int L, double betaJ, double kappa2, double h, double *V);        /* tmat.hpp:537-556 */
 
+/* Observables of an eigenvector pair (the X_phys.dat figures), sequential
+ * sums in the reference's order: out[0..3] the moments (moments()), out[4..]
+ * the extra observables (phys_extend()), *nout = 4 + their count.  ql may be
+ * NULL (ql = qr, TMat/pfunc.hpp:170-172).
+ *   XAT, XDT, XAM2D  TMat/tmatphys.cpp:13-57, TMat/tmatreduce.cpp:565-613
+ *   XAT3D, XAM3D     TMat/tmatphys.cpp:13-37, TMat/tmat3d.cpp:137-177
+ *   XDM              TMat/tmatreduce.cpp:565-590 (its moments), 1289-1320
+ *   ZBT, ZBM         TMat/tmatphys.cpp:106-132, TMat/tmatreduce.cpp:1748-1781
+ *   YAT              TMat/tmatphys.cpp:71-104
+ *   YAM, YBM         TMat/tmatreduce.cpp:882-909
+ *   Y2AT             TMat/tmat.cpp:312-317 (zeros) */
+void or_op_moments(const or_op *op, const double *ql, const double *qr, double out[8], int *nout);
+
 int or_num_threads(void);
 
 #ifdef __cplusplus

@@ -288,7 +288,7 @@ def test_cli_axial_outputs(tmp_path, oracle):
     vector from the transpose class and the generic moments over 4^L states
     with the low layer's magnetisation; Y2AT (axialdir 0) takes its own
     operator as the transpose (TMat/tmat.hpp:382-384) and writes zero
-    moments (TMat/tmat.cpp:304-309)."""
+    moments (TMat/tmat.cpp:312-317)."""
     L = 5
     d1, d2 = tmp_path / "yat", tmp_path / "y2at"
     d1.mkdir()
