@@ -24,7 +24,7 @@ EXPORTS = [
     "tm_op_create", "tm_op_destroy", "tm_op_rows", "tm_op_cols", "tm_op_normfactor",
     "tm_op_is_symmetric", "tm_op_kind", "tm_op_suggested_ncv", "tm_op_layer_size",
     "tm_op_matvec_count", "tm_op_apply", "tm_op_apply_dev", "tm_op_apply_prec",
-    "tm_op_time_apply", "tm_op_orbits", "tm_op_moments",
+    "tm_op_time_apply", "tm_op_orbits", "tm_op_moments", "tm_sample_configs",
     "tm_solve_opts_default", "tm_solve", "tm_solve_batch", "tm_grad_hf", "tm_last_error",
     "tm_version",
 ]
@@ -110,6 +110,8 @@ def lib():
         L.tm_op_orbits.argtypes = [vp, vp, vp]
         L.tm_op_moments.restype = i32
         L.tm_op_moments.argtypes = [vp, pd, pd, pd, C.POINTER(i32)]
+        L.tm_sample_configs.restype = i32
+        L.tm_sample_configs.argtypes = [vp, pd, pd, i32, i32, vp]
         L.tm_solve_opts_default.restype = None
         L.tm_solve_opts_default.argtypes = [C.POINTER(SolveOpts)]
         L.tm_solve.restype = i32
@@ -252,6 +254,23 @@ class TransferMatrix:
         _check(lib().tm_op_moments(self._h, None if ql is None else _pd(ql), _pd(qr), _pd(out),
                                    C.byref(nout)))
         return out[:nout.value].copy()
+
+    def sample_uniforms_needed(self, nchain, length):
+        """Uniform draws one tm_sample_configs call consumes."""
+        per = length if self.kind() == "XAT" else 2 * length - 1
+        return nchain * per
+
+    def sample_configs(self, vec, u, nchain, length):
+        """Configuration chains planted from the eigenvector vec with the
+        uniforms u (dumpconfig's draws, in order): (nchain, length) states."""
+        vec = np.ascontiguousarray(vec, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert vec.shape == (self.cols(),)
+        assert u.size >= self.sample_uniforms_needed(nchain, length)
+        out = np.empty((nchain, length), np.uint64)
+        _check(lib().tm_sample_configs(self._h, _pd(vec), _pd(u), nchain, length,
+                                       out.ctypes.data))
+        return out
 
     def grad_hf(self, l, r):
         l = np.ascontiguousarray(l, dtype=np.float64)

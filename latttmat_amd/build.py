@@ -1,5 +1,5 @@
 """Build the native library ``liblatttmat.so`` (HIP, gfx950) and the ``tmat``
-command-line program in-tree with hipcc.  Called by ``__graft_entry__.build``.
+and ``dumpconfig`` command-line programs in-tree with hipcc.  Called by ``__graft_entry__.build``.
 """
 from __future__ import annotations
 
@@ -14,6 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblatttmat.so")
 CLI = os.path.join(HERE, "tmat")
+DUMPCONFIG = os.path.join(HERE, "dumpconfig")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TM_OFFLOAD_ARCH", "gfx950")
@@ -29,7 +30,7 @@ COMMON = [
 LIB_SOURCES = [
     "tm_api.cpp", "tm_ops.cpp", "tm_solver.cpp", "tm_dense.cpp", "tm_grad.hip",
     "tm_kron.hip", "tm_reduced.hip", "tm_sweep.hip", "tm_krylov.hip", "tm_axial.hip",
-    "tm_phys.hip",
+    "tm_phys.hip", "tm_sample.hip",
 ]
 
 
@@ -53,15 +54,15 @@ def build(verbose: bool = True) -> None:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    cli_src = os.path.join(CSRC, "tmat_main.cpp")
-    if os.path.exists(cli_src):
-        tmpc = CLI + ".tmp"
+    for src, exe in (("tmat_main.cpp", CLI), ("dumpconfig_main.cpp", DUMPCONFIG)):
+        tmpc = exe + ".tmp"
         cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"),
-               cli_src, "-o", tmpc, "-L", HERE, "-llatttmat", "-Wl,-rpath,$ORIGIN"]
+               os.path.join(CSRC, src), "-o", tmpc, "-L", HERE, "-llatttmat",
+               "-Wl,-rpath,$ORIGIN"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
-            raise RuntimeError(f"tmat build failed\n{r.stdout}\n{r.stderr}")
-        os.replace(tmpc, CLI)
+            raise RuntimeError(f"{os.path.basename(exe)} build failed\n{r.stdout}\n{r.stderr}")
+        os.replace(tmpc, exe)
     if verbose:
         print(f"built {LIB}")
 

@@ -83,6 +83,8 @@ def lib():
         L.or_tvec_zbt.argtypes = [i32, dbl, dbl, dbl, pdbl]
         L.or_op_moments.restype = None
         L.or_op_moments.argtypes = [C.c_void_p, C.c_void_p, pdbl, pdbl, C.POINTER(i32)]
+        L.or_sample_configs.restype = i32
+        L.or_sample_configs.argtypes = [C.c_void_p, pdbl, pdbl, i32, i32, C.c_void_p]
         L.or_num_threads.restype = i32
         L.or_num_threads.argtypes = []
         _lib = L
@@ -132,6 +134,16 @@ class OracleOp:
 
     def tvec(self):
         return np.ctypeslib.as_array(lib().or_op_tvec(self._h), shape=(self.n,)).copy()
+
+    def sample_configs(self, vec, u, nchain, length):
+        """dumpconfig's chains with the given draws: (nchain, length) states."""
+        vec = np.ascontiguousarray(vec, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert vec.shape == (self.n,)
+        out = np.empty((nchain, length), np.uint64)
+        if lib().or_sample_configs(self._h, _p(vec), _p(u), nchain, length, out.ctypes.data):
+            raise ValueError("no sampler for this operator kind")
+        return out
 
     def moments(self, qr, ql=None):
         """X_phys.dat figures: four moments, then the extra observables."""

@@ -942,6 +942,136 @@ void or_op_moments(const or_op *op, const double *ql, const double *qr, double o
   }
 }
 
+/* std::upper_bound over an ascending cumulative array: first i with
+ * cum[i] > t; n if none (then the caller clamps). */
+static int64_t or_upper_bound(const double *cum, int64_t n, double t)
+{
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (cum[mid] > t) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+/* the last index with a positive increment of cum (0 if none) */
+static int64_t or_last_positive(const double *cum, int64_t n)
+{
+  int64_t i;
+  for (i = n - 1; i > 0; --i)
+    if (cum[i] > cum[i - 1]) return i;
+  return 0;
+}
+
+static int64_t or_draw(const double *cum, int64_t n, double u)
+{
+  const int64_t i = or_upper_bound(cum, n, u * cum[n - 1]);
+  return i < n ? i : or_last_positive(cum, n);
+}
+
+static int or_cmp_u64(const void *a, const void *b)
+{
+  const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+int or_sample_configs(const or_op *op, const double *vec, const double *u, int nchain, int length,
+                      uint64_t *states)
+{
+  const int L = op->L;
+  const int64_t n = op->ncol;
+  const uint64_t mask = (L >= 64) ? ~0ull : (1ull << L) - 1;
+  const double j0 = op->betaJ, j1 = -op->kappa * op->betaJ, j2 = op->h * op->betaJ;
+  const int red = op->kind != OR_XAT;
+  double *v, *cum, *e = NULL, *y = NULL;
+  int64_t i, ui = 0;
+  int c, j;
+  if (!(op->kind == OR_XAT || op->kind == OR_XAM2D || op->kind == OR_XDM)) return -1;
+  v = (double *)malloc(sizeof(double) * n);
+  cum = (double *)malloc(sizeof(double) * n);
+  for (i = 0; i < n; ++i) v[i] = fabs(vec[i]);
+  if (red) {
+    e = (double *)calloc((size_t)n, sizeof(double));
+    y = (double *)malloc(sizeof(double) * n);
+  }
+  for (c = 0; c < nchain; ++c) {
+    uint64_t *st = states + (int64_t)c * length;
+    double d = 0;
+    int64_t qa;
+    for (i = 0; i < n; ++i) {  /* SymDumpConfig::initq */
+      d += v[i] * v[i];
+      cum[i] = d;
+    }
+    qa = or_draw(cum, n, u[ui++]);
+    st[0] = red ? op->reps[qa] : (uint64_t)qa;
+    for (j = 1; j < length; ++j) {
+      const uint64_t ra = st[j - 1];
+      if (!red) {  /* dumpconfig.cpp:110-129 */
+        const double intr1 = j2 * (L - 2.0 * POP(ra))
+                             + j0 * (L - 2.0 * POP(ra ^ or_offset_state(ra, L, 1)))
+                             + j1 * (L - 2.0 * POP(ra ^ or_offset_state(ra, L, 2)));
+        d = 0;
+        for (i = 0; i < n; ++i) {
+          const double intr2 = j0 * (L - 2.0 * POP(ra ^ (uint64_t)i));
+          d += exp(intr1 + intr2) * v[i];
+          cum[i] = d;
+        }
+        st[j] = (uint64_t)or_draw(cum, n, u[ui++]);
+        continue;
+      }
+      /* dumpreduce.cpp:76-118: the column T e_qa, then an orbit member */
+      e[qa] = 1.0;
+      or_op_apply(op, e, y, 0);
+      e[qa] = 0.0;
+      d = 0;
+      for (i = 0; i < n; ++i) {
+        d += y[i] * v[i];
+        cum[i] = d;
+      }
+      qa = or_draw(cum, n, u[ui++]);
+      {
+        const uint64_t rep = op->reps[qa], rev = or_bitreversal(rep, L);
+        uint64_t mem[256];
+        double qc[256];
+        int m = 0, k, r;
+        if (op->bc & 1) {
+          mem[m++] = rep;
+          mem[m++] = rev;
+        } else {
+          for (r = 0; r < L; ++r) {
+            mem[m++] = or_offset_state(rep, L, r);
+            mem[m++] = or_offset_state(rev, L, r);
+          }
+        }
+        if (op->censym) {
+          const int m0 = m;
+          for (k = 0; k < m0; ++k) mem[m++] = ~mem[k] & mask;
+        }
+        qsort(mem, (size_t)m, sizeof(uint64_t), or_cmp_u64);
+        for (k = 1, r = 1; k < m; ++k)
+          if (mem[k] != mem[r - 1]) mem[r++] = mem[k];
+        m = r;
+        d = 0;
+        for (k = 0; k < m; ++k) {
+          double x = j0 * (L - 2.0 * POP(ra ^ mem[k]));
+          if (op->kind == OR_XDM)  /* dumpreduce.cpp:509-514 */
+            x += j1 * (L - 2.0 * POP(ra ^ or_offset_state(mem[k], L, 1)))
+                 + j1 * (L - 2.0 * POP(ra ^ or_lffset_state(mem[k], L, 1)));
+          d += exp(x);
+          qc[k] = d;
+        }
+        st[j] = mem[or_draw(qc, m, u[ui++])];
+      }
+    }
+  }
+  free(v);
+  free(cum);
+  free(e);
+  free(y);
+  return 0;
+}
+
 int64_t or_op_ncol(const or_op *op) { return op->ncol; }
 double or_op_normfactor(const or_op *op) { return op->nf; }
 const double *or_op_tvec(const or_op *op) { return op->tvec; }

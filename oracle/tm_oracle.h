@@ -135,6 +135,18 @@ void or_tvec_zbt(int L, double betaJ, double kappa2, double h, double *V);      
  *   Y2AT             TMat/tmat.cpp:312-317 (zeros) */
 void or_op_moments(const or_op *op, const double *ql, const double *qr, double out[8], int *nout);
 
+/* Configuration chains (DumpConfig): XAT2DDumpConfig::dump
+ * (DumpConfig/dumpconfig.cpp:88-136, start weights SymDumpConfig::initq :24-35),
+ * XAM2DDumpConfig::dump (DumpConfig/dumpreduce.cpp:47-122) and
+ * XDM2DDumpConfig::dump (:516-590), sequential, without the Q cache (which
+ * only saves recomputation).  vec is taken in absolute value; u holds the
+ * rand() draws in call order; states receives nchain x length states.  Orbit
+ * members are taken in ascending order (the reference iterates an
+ * unordered_set, TMat/tmatreduce.cpp:615-657).  Returns 0, or -1 for an
+ * operator kind without a sampler. */
+int or_sample_configs(const or_op *op, const double *vec, const double *u, int nchain, int length,
+                      uint64_t *states);
+
 int or_num_threads(void);
 
 #ifdef __cplusplus
