@@ -240,23 +240,32 @@ tm_status tm_op_moments(const tm_op *op, const double *ql, const double *qr, dou
 /* Plant equilibrium configurations from the leading eigenvector, as the
  * reference's dumpconfig program does (DumpConfig/main.cpp:30-144), as
  * Markov chains on the device:
- *   XAT (A, axialdir 1)      XAT2DDumpConfig::dump, DumpConfig/dumpconfig.cpp:88-136
- *   XAM2D (A, axialdir 5/13) XAM2DDumpConfig::dump, DumpConfig/dumpreduce.cpp:47-122
- *   XDM (D, axialdir 5/6)    XDM2DDumpConfig::dump, DumpConfig/dumpreduce.cpp:516-590
- * vec: the operator's eigenvector as X_vec.dat holds it (cols() entries;
- * absolute values are taken, DumpConfig/dumpconfig.hpp:71).  u: uniforms in
+ *   XAT (A, axialdir 1)       XAT2DDumpConfig::dump, DumpConfig/dumpconfig.cpp:88-136
+ *   YAT (A, axialdir 2)       YAT2DDumpConfig::dump, DumpConfig/dumpconfig.cpp:171-222
+ *   XAM2D (A, axialdir 5/13)  XAM2DDumpConfig::dump, DumpConfig/dumpreduce.cpp:47-122
+ *   YAM (A 6), YBM (B 6)      YAM2DDumpConfig::dump, YBM2DDumpConfig::dump,
+ *                             DumpConfig/dumpreduce.cpp:156-233, 279-356
+ *   ZBM (B, axialdir 7)       ZBM2DDumpConfig::dump, DumpConfig/dumpreduce.cpp:409-470
+ *   XDM (D, axialdir 5/6)     XDM2DDumpConfig::dump, DumpConfig/dumpreduce.cpp:516-590
+ * vec: the right eigenvector as X_vec.dat holds it (cols() entries); vecl:
+ * the left eigenvector (X_vecl.dat) for the non-symmetric kinds, NULL
+ * otherwise; absolute values are taken (DumpConfig/dumpconfig.hpp:71).  The
+ * start is drawn with weight vec^2 (symmetric) or vec vecl.  u: uniforms in
  * [0, 1) in the order MyUtility::rand() is called, chain after chain:
- * 1 + (length - 1) per chain for XAT, 1 + 2 (length - 1) for XAM2D / XDM.
- * states: nchain * length layer states, chain-major (the lines of
- * X_config_<chain>.dat, DumpConfig_Base::output, dumpconfig.hpp:32-40).
+ * length per chain for XAT and YAT, 2 length - 1 for the reduced kinds.
+ * states: nchain * length states, chain-major: layer states (XAT, XAM2D,
+ * XDM, ZBM) or two-layer states (YAT, YAM, YBM; index (high << L) | low).
+ * The reference's output lines are the states themselves, the low layers
+ * (YAM, YBM), or the first state's low layer followed by every state's high
+ * layer (YAT); DumpConfig_Base::output, dumpconfig.hpp:32-40.
  * Each draw picks the first index whose running weight exceeds u times the
  * total (std::upper_bound on the cumulative sums).  One difference: an
  * orbit's members are taken in ascending order, where the reference takes
  * the iteration order of an unordered_set (getEqvStates,
- * TMat/tmatreduce.cpp:615-657); the probabilities are the same.
- * Other operator kinds return TM_EUNSUPPORTED. */
-tm_status tm_sample_configs(const tm_op *op, const double *vec, const double *u, int nchain,
-                            int length, uint64_t *states);
+ * TMat/tmatreduce.cpp:615-657, 918-943, 1783-1810); the probabilities are
+ * the same.  Other operator kinds return TM_EUNSUPPORTED. */
+tm_status tm_sample_configs(const tm_op *op, const double *vec, const double *vecl,
+                            const double *u, int nchain, int length, uint64_t *states);
 
 /* Thread-local message for the last non-OK status. */
 const char *tm_last_error(void);

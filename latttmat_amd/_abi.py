@@ -111,7 +111,7 @@ def lib():
         L.tm_op_moments.restype = i32
         L.tm_op_moments.argtypes = [vp, pd, pd, pd, C.POINTER(i32)]
         L.tm_sample_configs.restype = i32
-        L.tm_sample_configs.argtypes = [vp, pd, pd, i32, i32, vp]
+        L.tm_sample_configs.argtypes = [vp, pd, pd, pd, i32, i32, vp]
         L.tm_solve_opts_default.restype = None
         L.tm_solve_opts_default.argtypes = [C.POINTER(SolveOpts)]
         L.tm_solve.restype = i32
@@ -257,19 +257,23 @@ class TransferMatrix:
 
     def sample_uniforms_needed(self, nchain, length):
         """Uniform draws one tm_sample_configs call consumes."""
-        per = length if self.kind() == "XAT" else 2 * length - 1
+        per = length if self.kind() in ("XAT", "YAT") else 2 * length - 1
         return nchain * per
 
-    def sample_configs(self, vec, u, nchain, length):
-        """Configuration chains planted from the eigenvector vec with the
-        uniforms u (dumpconfig's draws, in order): (nchain, length) states."""
+    def sample_configs(self, vec, u, nchain, length, vecl=None):
+        """Configuration chains planted from the eigenvector vec (and the left
+        eigenvector vecl for the non-symmetric kinds) with the uniforms u
+        (dumpconfig's draws, in order): (nchain, length) states."""
         vec = np.ascontiguousarray(vec, dtype=np.float64)
         u = np.ascontiguousarray(u, dtype=np.float64)
         assert vec.shape == (self.cols(),)
         assert u.size >= self.sample_uniforms_needed(nchain, length)
+        if vecl is not None:
+            vecl = np.ascontiguousarray(vecl, dtype=np.float64)
+            assert vecl.shape == (self.cols(),)
         out = np.empty((nchain, length), np.uint64)
-        _check(lib().tm_sample_configs(self._h, _pd(vec), _pd(u), nchain, length,
-                                       out.ctypes.data))
+        _check(lib().tm_sample_configs(self._h, _pd(vec), None if vecl is None else _pd(vecl),
+                                       _pd(u), nchain, length, out.ctypes.data))
         return out
 
     def grad_hf(self, l, r):

@@ -314,32 +314,44 @@ tm_status tm_op_moments(const tm_op *op, const double *ql, const double *qr, dou
   });
 }
 
-tm_status tm_sample_configs(const tm_op *cop, const double *vec, const double *u, int nchain,
-                            int length, uint64_t *states)
+tm_status tm_sample_configs(const tm_op *cop, const double *vec, const double *vecl,
+                            const double *u, int nchain, int length, uint64_t *states)
 {
   if (!cop || !vec || !u || !states) return fail(TM_EINVAL, "null argument");
   if (nchain < 1 || length < 1) return fail(TM_EINVAL, "nchain and length must be positive");
-  const bool xat = cop->kind == TM_KIND_XAT;
-  if (!(xat || cop->kind == TM_KIND_XAM2D || cop->kind == TM_KIND_XDM))
-    return fail(TM_EUNSUPPORTED, "configuration sampling is built for XAT, XAM2D and XDM");
-  if (xat && cop->bc != 0)
+  const tm_kind k = cop->kind;
+  const bool sym = k == TM_KIND_XAT || k == TM_KIND_XAM2D || k == TM_KIND_XDM;
+  const bool gen = k == TM_KIND_YAT || k == TM_KIND_ZBM || k == TM_KIND_YAM || k == TM_KIND_YBM;
+  if (!sym && !gen)
+    return fail(TM_EUNSUPPORTED,
+                "configuration sampling is built for the models dumpconfig serves "
+                "(XAT, YAT, XAM2D, XDM, ZBM, YAM, YBM)");
+  if (k == TM_KIND_XAT && cop->bc != 0)
     return fail(TM_EUNSUPPORTED, "the XAT sampler is periodic (dumpconfig serves axialdir 1)");
+  if (gen && !vecl) return fail(TM_EINVAL, "this operator's sampler needs the left eigenvector");
   tm_op *op = const_cast<tm_op *>(cop);
   return guarded([&] {
     using namespace ltm;
     const int64_t n = op->ncol;
-    const int64_t per = xat ? length : 2 * (int64_t)length - 1;
+    const bool onestep = k == TM_KIND_XAT || k == TM_KIND_YAT;
+    const int64_t per = onestep ? length : 2 * (int64_t)length - 1;
     const int64_t nu = per * nchain;
+    DevBuf dv(sizeof(double) * n), dl(gen ? sizeof(double) * n : 0), du(sizeof(double) * nu),
+        ds(sizeof(uint64_t) * nchain * (int64_t)length), derr(sizeof(int));
+    // absolute values, as DumpConfig<S>::readfile takes them (dumpconfig.hpp:71)
     std::vector<double> av(vec, vec + n);
     for (double &x : av) x = std::fabs(x);
-    DevBuf dv(sizeof(double) * n), du(sizeof(double) * nu),
-        ds(sizeof(uint64_t) * nchain * (int64_t)length), derr(sizeof(int));
     copy_large(dv.get(), av.data(), sizeof(double) * n, hipMemcpyHostToDevice, op->stream);
+    if (gen) {
+      av.assign(vecl, vecl + n);
+      for (double &x : av) x = std::fabs(x);
+      copy_large(dl.get(), av.data(), sizeof(double) * n, hipMemcpyHostToDevice, op->stream);
+    }
     copy_sync(du.get(), u, sizeof(double) * nu, hipMemcpyHostToDevice, op->stream);
     const int zero = 0;
     copy_sync(derr.get(), &zero, sizeof zero, hipMemcpyHostToDevice, op->stream);
-    sample_chains(op, dv.as<double>(), du.as<double>(), nchain, length, ds.as<uint64_t>(),
-                  derr.as<int>());
+    sample_chains(op, dv.as<double>(), gen ? dl.as<double>() : nullptr, du.as<double>(), nchain,
+                  length, ds.as<uint64_t>(), derr.as<int>());
     copy_sync(states, ds.get(), sizeof(uint64_t) * nchain * (int64_t)length,
               hipMemcpyDeviceToHost, op->stream);
     int err = 0;

@@ -519,10 +519,11 @@ enum PhysKind { PHYS_SYM2D, PHYS_SYM3D, PHYS_DNNI, PHYS_GEN, PHYS_GEN_LOW, PHYS_
 void phys_sums(int pk, const double *ql, const double *qr, const uint64_t *reps, int64_t n, int L,
                int bc, void *work, double sums[9], hipStream_t s);
 size_t phys_work_bytes();
-// Markov chains of configurations (XAT, XAM2D, XDM): vec = |eigenvector| and
-// u = the uniforms, both device; states (nchain * length, device) receives the
-// chains; *err (device) is set if an orbit's member count disagrees with its
-// size.
-void sample_chains(tm_op *op, const double *vec, const double *u, int nchain, int length,
-                   uint64_t *states, int *err);
+// Markov chains of configurations (XAT, YAT and the reduced XAM2D, XDM, ZBM,
+// YAM, YBM): vec = |right eigenvector|, vecl = |left eigenvector| (null for
+// the symmetric kinds; the start weight is then vec^2) and u = the uniforms,
+// all device; states (nchain * length, device) receives the chains; *err
+// (device) is set if an orbit's member count disagrees with its size.
+void sample_chains(tm_op *op, const double *vec, const double *vecl, const double *u, int nchain,
+                   int length, uint64_t *states, int *err);
 }  // namespace ltm

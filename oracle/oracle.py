@@ -84,7 +84,7 @@ def lib():
         L.or_op_moments.restype = None
         L.or_op_moments.argtypes = [C.c_void_p, C.c_void_p, pdbl, pdbl, C.POINTER(i32)]
         L.or_sample_configs.restype = i32
-        L.or_sample_configs.argtypes = [C.c_void_p, pdbl, pdbl, i32, i32, C.c_void_p]
+        L.or_sample_configs.argtypes = [C.c_void_p, pdbl, C.c_void_p, pdbl, i32, i32, C.c_void_p]
         L.or_num_threads.restype = i32
         L.or_num_threads.argtypes = []
         _lib = L
@@ -135,13 +135,17 @@ class OracleOp:
     def tvec(self):
         return np.ctypeslib.as_array(lib().or_op_tvec(self._h), shape=(self.n,)).copy()
 
-    def sample_configs(self, vec, u, nchain, length):
+    def sample_configs(self, vec, u, nchain, length, vecl=None):
         """dumpconfig's chains with the given draws: (nchain, length) states."""
         vec = np.ascontiguousarray(vec, dtype=np.float64)
         u = np.ascontiguousarray(u, dtype=np.float64)
         assert vec.shape == (self.n,)
+        if vecl is not None:
+            vecl = np.ascontiguousarray(vecl, dtype=np.float64)
+            assert vecl.shape == (self.n,)
         out = np.empty((nchain, length), np.uint64)
-        if lib().or_sample_configs(self._h, _p(vec), _p(u), nchain, length, out.ctypes.data):
+        if lib().or_sample_configs(self._h, _p(vec), None if vecl is None else vecl.ctypes.data,
+                                   _p(u), nchain, length, out.ctypes.data):
             raise ValueError("no sampler for this operator kind")
         return out
 

@@ -976,21 +976,113 @@ static int or_cmp_u64(const void *a, const void *b)
   return x < y ? -1 : x > y;
 }
 
-int or_sample_configs(const or_op *op, const double *vec, const double *u, int nchain, int length,
-                      uint64_t *states)
+/* exponent of an orbit member's weight (a the current state, s the member);
+ * *ok = 0 for a member the move cannot reach */
+static double or_member_cross(const or_op *op, uint64_t a, uint64_t s, int *ok)
+{
+  const int L = op->L;
+  const uint64_t lm = (1ull << L) - 1;
+  const double i0 = op->betaJ, i1 = -op->kappa * op->betaJ, i2 = -op->kappa2 * op->betaJ;
+  *ok = 1;
+  if (op->kind == OR_YAM || op->kind == OR_YBM) {  /* dumpreduce.cpp:213-221 */
+    if ((s & lm) != (a >> L)) {
+      *ok = 0;
+      return 0.0;
+    }
+    return i1 * (L - 2.0 * POP((a & lm) ^ (s >> L)));
+  }
+  if (op->kind == OR_ZBM) {  /* ZBM2DDumpConfig::crossintr, dumpreduce.cpp:389-407 */
+    double x = i2 * (L - 2.0 * POP(a ^ s));
+    uint64_t ra = a, rb;
+    int rp, nanti = 0;
+    x += i1 * (L - 2.0 * POP(a ^ or_offset_state(s, L, 2)));
+    x += i1 * (L - 2.0 * POP(a ^ or_lffset_state(s, L, 2)));
+    rb = s << 1 | s >> (L - 1);
+    for (rp = 0; rp < L / 2; ++rp) {
+      nanti += (int)((ra ^ rb) & 1);
+      rb >>= 2;
+      nanti += (int)((ra ^ rb) & 1);
+      ra >>= 2;
+    }
+    return x + i0 * (L - 2.0 * nanti);
+  }
+  {
+    double x = i0 * (L - 2.0 * POP(a ^ s));
+    if (op->kind == OR_XDM)  /* dumpreduce.cpp:509-514 */
+      x += i1 * (L - 2.0 * POP(a ^ or_offset_state(s, L, 1)))
+           + i1 * (L - 2.0 * POP(a ^ or_lffset_state(s, L, 1)));
+    return x;
+  }
+}
+
+/* the members of the orbit of rep, ascending; returns their count */
+static int or_orbit_members(const or_op *op, uint64_t rep, uint64_t *mem)
+{
+  const int L = op->L;
+  const uint64_t lm = (1ull << L) - 1;
+  int m = 0, k, r;
+  if (op->kind == OR_YAM || op->kind == OR_YBM) {
+    const uint64_t mir = or_bitreversal(rep >> L, L) << L | or_bitreversal(rep & lm, L);
+    for (r = 0; r < L; ++r) {
+      mem[m++] = or_offset_state(rep >> L, L, r) << L | or_offset_state(rep & lm, L, r);
+      mem[m++] = or_offset_state(mir >> L, L, r) << L | or_offset_state(mir & lm, L, r);
+    }
+    if (op->censym) {
+      const int m0 = m;
+      const uint64_t full = (2 * L >= 64) ? ~0ull : (1ull << (2 * L)) - 1;
+      for (k = 0; k < m0; ++k) mem[m++] = ~mem[k] & full;
+    }
+  } else {
+    uint64_t mir;
+    if (op->kind == OR_ZBM) {
+      mir = or_bitreversal(or_offset_state(rep, L, 1), L);
+      for (r = 0; r < L; r += 2) {
+        mem[m++] = or_offset_state(rep, L, r);
+        mem[m++] = or_offset_state(mir, L, r);
+      }
+    } else if (op->bc & 1) {
+      mem[m++] = rep;
+      mem[m++] = or_bitreversal(rep, L);
+    } else {
+      mir = or_bitreversal(rep, L);
+      for (r = 0; r < L; ++r) {
+        mem[m++] = or_offset_state(rep, L, r);
+        mem[m++] = or_offset_state(mir, L, r);
+      }
+    }
+    if (op->censym) {
+      const int m0 = m;
+      for (k = 0; k < m0; ++k) mem[m++] = ~mem[k] & lm;
+    }
+  }
+  qsort(mem, (size_t)m, sizeof(uint64_t), or_cmp_u64);
+  for (k = 1, r = 1; k < m; ++k)
+    if (mem[k] != mem[r - 1]) mem[r++] = mem[k];
+  return m ? r : 0;
+}
+
+int or_sample_configs(const or_op *op, const double *vec, const double *vecl, const double *u,
+                      int nchain, int length, uint64_t *states)
 {
   const int L = op->L;
   const int64_t n = op->ncol;
-  const uint64_t mask = (L >= 64) ? ~0ull : (1ull << L) - 1;
+  const uint64_t lm = (1ull << L) - 1;
   const double j0 = op->betaJ, j1 = -op->kappa * op->betaJ, j2 = op->h * op->betaJ;
-  const int red = op->kind != OR_XAT;
-  double *v, *cum, *e = NULL, *y = NULL;
+  const int red = op->kind != OR_XAT && op->kind != OR_YAT;
+  const int gen = op->kind == OR_YAT || op->kind == OR_ZBM || op->kind == OR_YAM ||
+                  op->kind == OR_YBM;
+  double *v, *vl = NULL, *cum, *e = NULL, *y = NULL;
   int64_t i, ui = 0;
   int c, j;
-  if (!(op->kind == OR_XAT || op->kind == OR_XAM2D || op->kind == OR_XDM)) return -1;
+  if (!(gen || op->kind == OR_XAT || op->kind == OR_XAM2D || op->kind == OR_XDM)) return -1;
+  if (gen && !vecl) return -1;
   v = (double *)malloc(sizeof(double) * n);
   cum = (double *)malloc(sizeof(double) * n);
   for (i = 0; i < n; ++i) v[i] = fabs(vec[i]);
+  if (gen) {
+    vl = (double *)malloc(sizeof(double) * n);
+    for (i = 0; i < n; ++i) vl[i] = fabs(vecl[i]);
+  }
   if (red) {
     e = (double *)calloc((size_t)n, sizeof(double));
     y = (double *)malloc(sizeof(double) * n);
@@ -999,15 +1091,15 @@ int or_sample_configs(const or_op *op, const double *vec, const double *u, int n
     uint64_t *st = states + (int64_t)c * length;
     double d = 0;
     int64_t qa;
-    for (i = 0; i < n; ++i) {  /* SymDumpConfig::initq */
-      d += v[i] * v[i];
+    for (i = 0; i < n; ++i) {  /* initq */
+      d += gen ? v[i] * vl[i] : v[i] * v[i];
       cum[i] = d;
     }
     qa = or_draw(cum, n, u[ui++]);
     st[0] = red ? op->reps[qa] : (uint64_t)qa;
     for (j = 1; j < length; ++j) {
       const uint64_t ra = st[j - 1];
-      if (!red) {  /* dumpconfig.cpp:110-129 */
+      if (op->kind == OR_XAT) {  /* dumpconfig.cpp:110-129 */
         const double intr1 = j2 * (L - 2.0 * POP(ra))
                              + j0 * (L - 2.0 * POP(ra ^ or_offset_state(ra, L, 1)))
                              + j1 * (L - 2.0 * POP(ra ^ or_offset_state(ra, L, 2)));
@@ -1020,7 +1112,22 @@ int or_sample_configs(const or_op *op, const double *vec, const double *u, int n
         st[j] = (uint64_t)or_draw(cum, n, u[ui++]);
         continue;
       }
-      /* dumpreduce.cpp:76-118: the column T e_qa, then an orbit member */
+      if (op->kind == OR_YAT) {  /* dumpconfig.cpp:189-218: (b, a) -> (c, b) */
+        const uint64_t a = ra & lm, b = ra >> L;
+        const int64_t m = (int64_t)1 << L;
+        const double intr1 = j2 * (L - 2.0 * POP(a))
+                             + j0 * (L - 2.0 * POP(a ^ or_offset_state(a, L, 1)))
+                             + j0 * (L - 2.0 * POP(a ^ b));
+        d = 0;
+        for (i = 0; i < m; ++i) {
+          const double intr2 = j1 * (L - 2.0 * POP(a ^ (uint64_t)i));
+          d += exp(intr1 + intr2) * v[(uint64_t)i << L | b];
+          cum[i] = d;
+        }
+        st[j] = (uint64_t)or_draw(cum, m, u[ui++]) << L | b;
+        continue;
+      }
+      /* the column T e_qa, then an orbit member */
       e[qa] = 1.0;
       or_op_apply(op, e, y, 0);
       e[qa] = 0.0;
@@ -1031,34 +1138,14 @@ int or_sample_configs(const or_op *op, const double *vec, const double *u, int n
       }
       qa = or_draw(cum, n, u[ui++]);
       {
-        const uint64_t rep = op->reps[qa], rev = or_bitreversal(rep, L);
-        uint64_t mem[256];
-        double qc[256];
-        int m = 0, k, r;
-        if (op->bc & 1) {
-          mem[m++] = rep;
-          mem[m++] = rev;
-        } else {
-          for (r = 0; r < L; ++r) {
-            mem[m++] = or_offset_state(rep, L, r);
-            mem[m++] = or_offset_state(rev, L, r);
-          }
-        }
-        if (op->censym) {
-          const int m0 = m;
-          for (k = 0; k < m0; ++k) mem[m++] = ~mem[k] & mask;
-        }
-        qsort(mem, (size_t)m, sizeof(uint64_t), or_cmp_u64);
-        for (k = 1, r = 1; k < m; ++k)
-          if (mem[k] != mem[r - 1]) mem[r++] = mem[k];
-        m = r;
+        uint64_t mem[512];
+        double qc[512];
+        const int m = or_orbit_members(op, op->reps[qa], mem);
+        int k, ok;
         d = 0;
         for (k = 0; k < m; ++k) {
-          double x = j0 * (L - 2.0 * POP(ra ^ mem[k]));
-          if (op->kind == OR_XDM)  /* dumpreduce.cpp:509-514 */
-            x += j1 * (L - 2.0 * POP(ra ^ or_offset_state(mem[k], L, 1)))
-                 + j1 * (L - 2.0 * POP(ra ^ or_lffset_state(mem[k], L, 1)));
-          d += exp(x);
+          const double x = or_member_cross(op, ra, mem[k], &ok);
+          if (ok) d += exp(x);
           qc[k] = d;
         }
         st[j] = mem[or_draw(qc, m, u[ui++])];
@@ -1066,6 +1153,7 @@ int or_sample_configs(const or_op *op, const double *vec, const double *u, int n
     }
   }
   free(v);
+  free(vl);
   free(cum);
   free(e);
   free(y);

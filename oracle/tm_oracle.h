@@ -135,17 +135,21 @@ void or_tvec_zbt(int L, double betaJ, double kappa2, double h, double *V);      
  *   Y2AT             TMat/tmat.cpp:312-317 (zeros) */
 void or_op_moments(const or_op *op, const double *ql, const double *qr, double out[8], int *nout);
 
-/* Configuration chains (DumpConfig): XAT2DDumpConfig::dump
- * (DumpConfig/dumpconfig.cpp:88-136, start weights SymDumpConfig::initq :24-35),
- * XAM2DDumpConfig::dump (DumpConfig/dumpreduce.cpp:47-122) and
- * XDM2DDumpConfig::dump (:516-590), sequential, without the Q cache (which
- * only saves recomputation).  vec is taken in absolute value; u holds the
- * rand() draws in call order; states receives nchain x length states.  Orbit
- * members are taken in ascending order (the reference iterates an
- * unordered_set, TMat/tmatreduce.cpp:615-657).  Returns 0, or -1 for an
- * operator kind without a sampler. */
-int or_sample_configs(const or_op *op, const double *vec, const double *u, int nchain, int length,
-                      uint64_t *states);
+/* Configuration chains (DumpConfig), sequential, without the Q cache (which
+ * only saves recomputation):
+ *   XAT   XAT2DDumpConfig::dump  DumpConfig/dumpconfig.cpp:88-136
+ *   YAT   YAT2DDumpConfig::dump  DumpConfig/dumpconfig.cpp:171-222
+ *   XAM2D, XDM, ZBM, YAM, YBM    DumpConfig/dumpreduce.cpp:47-122, 516-590,
+ *                                409-470, 156-233, 279-356
+ * Start weights vec^2 (SymDumpConfig::initq, dumpconfig.cpp:24-35) or
+ * vec vecl (GenDumpConfig::initq, :44-56; vecl non-NULL for YAT, ZBM, YAM,
+ * YBM).  vec and vecl are taken in absolute value; u holds the rand() draws
+ * in call order; states receives nchain x length states.  Orbit members are
+ * taken in ascending order (the reference iterates an unordered_set,
+ * TMat/tmatreduce.cpp:615-657, 918-943, 1783-1810).  Returns 0, or -1 for an
+ * operator kind without a sampler or a missing vecl. */
+int or_sample_configs(const or_op *op, const double *vec, const double *vecl, const double *u,
+                      int nchain, int length, uint64_t *states);
 
 int or_num_threads(void);
 

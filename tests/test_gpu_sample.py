@@ -1,8 +1,8 @@
 """GPU parity of tm_sample_configs (dumpconfig's configuration chains as
-device Markov chains) against the oracle's sequential restatement of
-XAT2DDumpConfig::dump, XAM2DDumpConfig::dump and XDM2DDumpConfig::dump
-(DumpConfig/dumpconfig.cpp:88-136, DumpConfig/dumpreduce.cpp:47-122,
-516-590): given the same uniforms, every state of every chain is the same
+device Markov chains) against the oracle's sequential restatement of the
+dump() of every DumpConfig model (XAT, YAT: DumpConfig/dumpconfig.cpp:88-222;
+XAM, YAM, YBM, ZBM, XDM: DumpConfig/dumpreduce.cpp:47-590): given the same
+uniforms, every state of every chain is the same
 (integer output, compared exactly; the weights are summed in a different
 order, so a draw could differ only when a uniform falls within rounding of a
 cumulative boundary).  Cases: eigenvectors from the device solver and random
@@ -39,6 +39,13 @@ CASES = [
     ("XAM-L12-unfolded", ("A", 2, 5, 12), {"neigs": 4}, "XAM2D", 2, "rand", 2, 15),
     ("XDM-L12-eig", ("D", 2, 5, 12), {}, "XDM", 0, "eig", 3, 20),
     ("XDM-L16-rand", ("D", 2, 5, 16), {"h": 0.05}, "XDM", 0, "rand", 2, 10),
+    ("YAT-L6-eig", ("A", 2, 2, 6), {}, "YAT", 0, "eig", 3, 20),
+    ("YAT-L9-rand", ("A", 2, 2, 9), {"h": 0.1}, "YAT", 0, "rand", 2, 12),
+    ("ZBM-L12-eig", ("B", 2, 7, 12), {"kappa2": 0.25}, "ZBM", 0, "eig", 3, 15),
+    ("ZBM-L16-rand", ("B", 2, 7, 16), {"kappa2": 0.25, "h": 0.1}, "ZBM", 0, "rand", 2, 10),
+    ("YAM-L6-eig", ("A", 2, 6, 6), {}, "YAM", 0, "eig", 3, 15),
+    ("YAM-L8-rand", ("A", 2, 6, 8), {}, "YAM", 0, "rand", 2, 10),
+    ("YBM-L7-rand", ("B", 2, 6, 7), {"kappa2": 0.25, "h": 0.1}, "YBM", 0, "rand", 2, 10),
 ]
 
 
@@ -47,22 +54,30 @@ CASES = [
 def test_chains_vs_oracle(lt, oracle, label, args, kw, okind, obc, src, nchain, length):
     kw = dict(kw)
     bJ, k, h = 0.45, 0.35, kw.pop("h", 0.0)
-    op = lt.TransferMatrix(*args, J=bJ, kappa=k, h=h, **kw)
+    k2 = kw.pop("kappa2", 0.0)
+    op = lt.TransferMatrix(*args, J=bJ, kappa=k, kappa2=k2, h=h, **kw)
     assert op.kind() == okind
-    oop = oracle.OracleOp(getattr(oracle, okind), args[3], 0, obc, bJ, k, 0.0, h)
+    oop = oracle.OracleOp(getattr(oracle, okind), args[3], 0, obc, bJ, k, k2, h)
     n = op.cols()
     assert n == oop.n
+    sym = op.is_symmetric()
+    vl = None
     if src == "eig":
-        r = lt.solve(op, nev=1, want_vectors=True)
+        r = lt.solve(op, nev=1, want_vectors=True, want_left=not sym)
         v = np.real(r.vecs[:, 0])
+        if not sym:
+            vl = np.asarray(r.vecl)
     else:
         v = _rvec(n, len(label))
+        if not sym:
+            vl = _rvec(n, 2 * len(label))
     u = np.random.default_rng(100 + len(label)).uniform(0, 1, op.sample_uniforms_needed(nchain, length))
-    g = op.sample_configs(v, u, nchain, length)
-    o = oop.sample_configs(v, u, nchain, length)
+    g = op.sample_configs(v, u, nchain, length, vl)
+    o = oop.sample_configs(v, u, nchain, length, vl)
     print(f"  {label}: n={n} first chain {g[0][:6]}", flush=True)
     np.testing.assert_array_equal(g, o)
-    assert np.all(g < (1 << args[3]))
+    nb = 2 * args[3] if okind in ("YAT", "YAM", "YBM") else args[3]
+    assert np.all(g < (1 << nb))
 
 
 @pytest.mark.parametrize("label,args,okind,length", [("XAT-L24", ("A", 2, 1, 24), "XAT", 6),
@@ -83,18 +98,25 @@ def test_chains_full_width(lt, oracle, label, args, okind, length):
 def test_sampler_rejects_unsupported(lt):
     op = lt.TransferMatrix("B", 2, 3, 10, J=0.4, kappa=0.3, kappa2=0.2)
     with pytest.raises(lt.TMError):
+        op.sample_configs(np.ones(op.cols()), np.full(100, 0.5), 1, 5, np.ones(op.cols()))
+    # the non-symmetric samplers need the left vector
+    op = lt.TransferMatrix("B", 2, 7, 10, J=0.4, kappa=0.3, kappa2=0.2)
+    with pytest.raises(lt.TMError):
         op.sample_configs(np.ones(op.cols()), np.full(100, 0.5), 1, 5)
 
 
 def _write_input(d, axialdir, size, out, model="A"):
     p = os.path.join(d, "input.dat")
+    kap = "0.35 0.25" if model == "B" else "0.35"
     with open(p, "w") as f:
         f.write(f"model = {model}\ndim = 2\naxialdir = {axialdir}\nneigs = 1\nsavevec = 1 1\n"
-                f"outprefix = {out}\nsize = {size}\nJ = 0.45\nkappa = 0.35\nh = 0\n")
+                f"outprefix = {out}\nsize = {size}\nJ = 0.45\nkappa = {kap}\nh = 0\n")
     return p
 
 
-@pytest.mark.parametrize("model,axialdir,size", [("A", 1, 10), ("A", 5, 14), ("D", 5, 12)])
+@pytest.mark.parametrize("model,axialdir,size", [("A", 1, 10), ("A", 5, 14), ("D", 5, 12),
+                                                 ("A", 2, 6), ("A", 6, 6), ("B", 6, 6),
+                                                 ("B", 7, 12)])
 def test_dumpconfig_program(lt, tmp_path, model, axialdir, size):
     """tmat writes X_vec.dat, dumpconfig plants N chains of the given length
     (DumpConfig/main.cpp:30-144): N files of `length` lines of `size` bits;
@@ -113,8 +135,9 @@ def test_dumpconfig_program(lt, tmp_path, model, axialdir, size):
         files = [open(tmp_path / f"P_config_{i}.dat").read() for i in range(3)]
         outs.append(files)
     assert outs[0] == outs[1]
+    nlines = 41 if (model, axialdir) == ("A", 2) else 40  # YAT adds the first low layer
     for txt in outs[0]:
         lines = txt.split("\n")
-        assert lines[-1] == "" and len(lines) == 41
+        assert lines[-1] == "" and len(lines) == nlines + 1
         for ln in lines[:-1]:
             assert len(ln) == size and set(ln) <= {"0", "1"}

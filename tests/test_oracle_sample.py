@@ -12,6 +12,11 @@ lattice couplings, not from the oracle's code:
   XAM / XDM move      p(o) ~ (T e_a)_o vec_o over orbits, then within orbit o
                       p(s) ~ exp(cross(a, s)), cross the in-plane coupling of
                       the two layers (DNNI: plus the diagonal -kappa bonds)
+  non-symmetric       start p(s) ~ |vec_s vecl_s|; YAT moves (b, a) -> (c, b)
+                      with p(c) ~ exp(-kappa betaJ sum s_i(a) s_i(c)) vec_(c,b);
+                      ZBM members ~ exp(cross) of the diagonal transfer matrix;
+                      YAM / YBM members restricted to low layer = current high
+                      layer, ~ exp(-kappa betaJ sum s_i(a_low) s_i(s_high))
 """
 import numpy as np
 import pytest
@@ -130,3 +135,92 @@ def test_planted_self_term_mistake_is_caught(oracle):
                   for c in range(op.n)])
     cnt = np.bincount(st[:, 1].astype(np.int64), minlength=op.n)
     assert np.max(np.abs(cnt - M * w / w.sum())) > 50
+
+
+def _cross_zbm(a, s, L, bJ, k, k2):
+    """ZBM2DDumpConfig::crossintr from the couplings: -kappa2 on the same
+    site, -kappa at distance two, betaJ from each even site of a to the odd
+    neighbours of s (DumpConfig/dumpreduce.cpp:389-407)."""
+    sa, sb = _spins(a, L), _spins(s, L)
+    x = -k2 * bJ * float(sa @ sb) - k * bJ * float(sa @ (np.roll(sb, -2) + np.roll(sb, 2)))
+    ev = np.arange(0, L, 2)
+    x += bJ * float(sa[ev] @ (np.roll(sb, 1)[ev] + np.roll(sb, -1)[ev]))
+    return x
+
+
+def test_yat_move_distribution(oracle):
+    """YAT: start ~ |vec vecl| over 4^L; the state (b, a) moves to (c, b) with
+    p(c) ~ exp(-kappa betaJ sum_i s_i(a) s_i(c)) |vec_(c,b)|."""
+    L, M, bJ, k = 4, 20000, 0.45, 0.35
+    op = oracle.OracleOp(oracle.YAT, L, 0, 0, bJ, k, 0.0, 0.1)
+    v, vl = _vec(op.n, 11), _vec(op.n, 13)
+    u = (np.arange(M) + 0.5) / M
+    st = op.sample_configs(v, u, M, 1, vl)[:, 0]
+    cnt = np.bincount(st.astype(np.int64), minlength=op.n)
+    p = np.abs(v * vl) / np.sum(np.abs(v * vl))
+    assert np.max(np.abs(cnt - M * p)) <= 1.0
+    uu = np.empty((M, 2))
+    uu[:, 0] = 0.43
+    uu[:, 1] = u
+    st = op.sample_configs(v, uu.ravel(), M, 2, vl)
+    s0 = int(st[0, 0])
+    lm = (1 << L) - 1
+    a, b = s0 & lm, s0 >> L
+    assert np.all(st[:, 1] & np.uint64(lm) == b)
+    w = np.array([np.exp(-k * bJ * float(_spins(a, L) @ _spins(c, L))) * abs(v[(c << L) | b])
+                  for c in range(1 << L)])
+    cnt = np.bincount((st[:, 1] >> np.uint64(L)).astype(np.int64), minlength=1 << L)
+    assert np.max(np.abs(cnt - M * w / w.sum())) <= 1.0
+
+
+@pytest.mark.parametrize("kind,L,ext,h", [("ZBM", 6, 0, 0.0), ("ZBM", 8, 2, 0.1), ("YAM", 4, 0, 0.0),
+                                          ("YBM", 4, 2, 0.0)])
+def test_nonsymmetric_reduced_move_distribution(oracle, kind, L, ext, h):
+    bJ, k, k2 = 0.45, 0.35, 0.25
+    op = oracle.OracleOp(getattr(oracle, kind), L, 0, ext, bJ, k, k2, h)
+    n = op.n
+    reps, dups = op.reps(), op.dups()
+    v, vl = _vec(n, 17), _vec(n, 19)
+    M1 = M2 = 150
+    u = np.empty((M1 * M2, 3))
+    u[:, 0] = 0.52
+    u[:, 1] = np.repeat((np.arange(M1) + 0.5) / M1, M2)
+    u[:, 2] = np.tile((np.arange(M2) + 0.5) / M2, M1)
+    st = op.sample_configs(v, u.ravel(), M1 * M2, 2, vl)
+    a = int(st[0, 0])
+    assert np.all(st[:, 0] == a)
+    qa = int(np.searchsorted(reps, a))
+    assert reps[qa] == a
+    e = np.zeros(n)
+    e[qa] = 1.0
+    po = op.apply(e) * np.abs(v)
+    po /= po.sum()
+    two = kind != "ZBM"
+    nb = 2 * L if two else L
+    G = groups.closure(groups.gens_y(L) if two else groups.gens_z(L))
+    flip = h == 0 and (ext & 2) == 0
+    full = (1 << nb) - 1
+    lm = (1 << L) - 1
+    want = {}
+    for o, r in enumerate(reps):
+        mem = set()
+        for g in G:
+            s = sum(((int(r) >> i) & 1) << g[i] for i in range(nb))
+            mem.add(s)
+            if flip:
+                mem.add(~s & full)
+        assert len(mem) == dups[o]
+        if two:
+            w = {s: np.exp(-k * bJ * float(_spins(a & lm, L) @ _spins(s >> L, L)))
+                 for s in mem if (s & lm) == (a >> L)}
+        else:
+            w = {s: np.exp(_cross_zbm(a, s, L, bJ, k, k2)) for s in mem}
+        tot = sum(w.values())
+        for s, ws in w.items():
+            want[s] = want.get(s, 0.0) + po[o] * ws / tot
+    cnt = np.bincount(st[:, 1].astype(np.int64), minlength=1 << nb)
+    exp_cnt = np.zeros(1 << nb)
+    for s, p in want.items():
+        exp_cnt[s] = M1 * M2 * p
+    assert np.max(np.abs(cnt - exp_cnt)) <= M1 + M2
+    assert np.sum(cnt[exp_cnt == 0]) == 0
