@@ -46,7 +46,7 @@ def main():
     ap.add_argument("--profile-tx", action="store_true",
                     help="only a calibration copy and a few T.x applies (for counter runs)")
     ap.add_argument("--workload", default="reduced",
-                    choices=("reduced", "scan", "3d", "grad", "small", "ops"),
+                    choices=("reduced", "scan", "3d", "grad", "small", "ops", "sample"),
                     help="reduced: the headline line; scan: 96-instance BNNNI batch; "
                          "3d: 5x5 3D layer, reduced vs full space; ops: T.x of every "
                          "operator kind against the HBM roofline")
@@ -70,6 +70,8 @@ def main():
         return bench_small(lt, args)
     if args.workload == "ops":
         return bench_ops(lt, args)
+    if args.workload == "sample":
+        return bench_sample(lt, args)
 
     L, bJ, kappa, neigs = args.L, 0.4, 0.5, 4
     t0 = time.perf_counter()
@@ -383,6 +385,61 @@ def bench_small(lt, args):
         w = np.linalg.eigvalsh(0.5 * (D + D.T))
         out["rel_err_vs_dense_lapack"] = abs(r.evals[0] / w[-1] - 1)
     print(json.dumps(out), flush=True)
+
+
+def bench_sample(lt, args):
+    """Configuration chains (dumpconfig) on the device: moves per second of
+    one chain for the full XAT L=24 (a draw over 2^24 states per move) and
+    the reduced XAM L=28 (one T.x of a unit vector plus a draw over 4.8M
+    orbits per move).  A move's time is the difference of a long and a
+    one-state chain divided by the moves, so the one-time upload of the
+    eigenvector is excluded; HBM bytes per move: XAT one pass over vec (8 B
+    per state), XAM vec and the column (16 B per orbit) plus the T.x.  CPU
+    baseline: the oracle's sequential move (16 host threads for its T.x)."""
+    import numpy as np
+    cases = [("XAT L=24", ("A", 2, 1, 24), 8), ("XAM L=28", ("A", 2, 5, 28), 16)]
+    for label, a, bpe in cases:
+        op = lt.TransferMatrix(*a, J=0.4, kappa=0.5)
+        n = op.cols()
+        v = np.random.default_rng(3).uniform(0.05, 1.0, n)
+        moves = 200 if label.startswith("XAT") else 60
+        u = np.random.default_rng(4).uniform(0, 1, op.sample_uniforms_needed(1, moves + 1))
+        op.sample_configs(v, u, 1, 3)  # warm-up (autotune of the T.x)
+        ts = time.perf_counter()
+        op.sample_configs(v, u, 1, 1)
+        t1 = time.perf_counter() - ts
+        ts = time.perf_counter()
+        st = op.sample_configs(v, u, 1, moves + 1)
+        tl = time.perf_counter() - ts
+        per = (tl - t1) / moves
+        out = {
+            "metric": f"{label} configuration-chain moves/s (dumpconfig)",
+            "value": round(1.0 / per, 2), "unit": "moves/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "f64", "data": "synthetic (random positive vector)",
+            "config": {"workload": f"{label}: one chain of {moves} moves", "cols": int(n)},
+            "ms_per_move": round(per * 1e3, 4),
+            "draw_hbm_GBs": round(bpe * n / per / 1e9, 1),
+            "first_states": [int(x) for x in st[0][:4]],
+        }
+        if not args.no_cpu_baseline:
+            from oracle import oracle as O
+            if not os.path.exists(O._LIB_PATH):
+                O.build()
+            kind = O.XAT if label.startswith("XAT") else O.XAM2D
+            oop = O.OracleOp(kind, a[3], 0, 0, 0.4, 0.5, 0.0, 0.0)
+            cm = 3 if label.startswith("XAT") else 2
+            ts = time.perf_counter()
+            oop.sample_configs(v, u, 1, 1)
+            c1 = time.perf_counter() - ts
+            ts = time.perf_counter()
+            so = oop.sample_configs(v, u, 1, cm + 1)
+            cpu = (time.perf_counter() - ts - c1) / cm
+            assert np.array_equal(so[0], st[0][:cm + 1])
+            out["cpu_baseline"] = {"value": round(1.0 / cpu, 3), "unit": "moves/s",
+                                   "cores": O.num_threads() if kind == O.XAM2D else 1,
+                                   "kind": "port",
+                                   "sample": f"{cm} oracle moves, identical states"}
+        print(json.dumps(out), flush=True)
 
 
 # (label, TransferMatrix args, kwargs): one operator of every kind at a size

@@ -18,9 +18,10 @@
 // running sum exceeds u times the total, u one of the caller's uniforms.
 //
 // A weighted draw is two kernels: every workgroup sums the weights of one
-// contiguous chunk (coalesced), then one workgroup scans the chunk totals,
-// re-sums the chosen chunk in 1024 contiguous segments, scans those and walks
-// the chosen segment.  The whole chain stays on the device (the current state
+// contiguous 2048-element chunk (coalesced), then one workgroup finds the
+// index in three levels -- groups of consecutive chunk totals, the chunks of
+// the chosen group, the element pairs of the chosen chunk -- each a 1024-wide
+// scan in LDS after a few independent loads per thread.  The whole chain stays on the device (the current state
 // is read by the next step's kernels), so a chain of any length is one queue
 // of launches with no host round trip.
 #include <algorithm>
@@ -31,8 +32,10 @@ namespace ltm {
 namespace {
 
 constexpr int SEL_THREADS = 256;
-constexpr int SEL_MAXBLK = 1024;
+constexpr int SEL_PER = 8;                          // elements per thread
+constexpr int SEL_CHUNK = SEL_THREADS * SEL_PER;     // elements per workgroup
 constexpr int FIND_THREADS = 1024;
+constexpr int64_t SEL_MAXN = int64_t(SEL_CHUNK) * FIND_THREADS * FIND_THREADS;  // 2^31
 
 enum SelMode { SEL_SQUARE = 0, SEL_XAT = 1, SEL_PRODUCT = 2, SEL_PAIR = 3, SEL_YAT = 4 };
 
@@ -42,7 +45,6 @@ struct SelArgs {
   const double *vecl;    // SEL_PAIR: |left eigenvector|
   const double *y;       // SEL_PRODUCT: T e_a
   int64_t n;
-  int64_t chunk;         // elements per workgroup (multiple of SEL_THREADS)
   int L;
   double j0, j1, j2;     // betaJ, -kappa betaJ, h betaJ (dumpconfig.cpp:61-63, 141-143)
   const uint64_t *cur;   // SEL_XAT, SEL_YAT: the current state
@@ -123,39 +125,70 @@ __global__ __launch_bounds__(SEL_THREADS) void k_sel_partials(SelArgs p, double 
     a = *p.cur;
     __syncthreads();
   }
-  const int64_t b0 = (int64_t)blockIdx.x * p.chunk;
-  const int64_t b1 = b0 + p.chunk < p.n ? b0 + p.chunk : p.n;
+  const int64_t b0 = (int64_t)blockIdx.x * SEL_CHUNK;
+  double w[SEL_PER];
+#pragma unroll
+  for (int k = 0; k < SEL_PER; ++k) {
+    const int64_t i = b0 + k * SEL_THREADS + threadIdx.x;
+    w[k] = i < p.n ? weight(p, i, e, a) : 0.0;
+  }
   double acc = 0;
-  for (int64_t i = b0 + threadIdx.x; i < b1; i += SEL_THREADS) acc += weight(p, i, e, a);
+#pragma unroll
+  for (int k = 0; k < SEL_PER; ++k) acc += w[k];
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0;
-    for (int w = 0; w < SEL_THREADS / 64; ++w) t += red[w];
+    for (int q = 0; q < SEL_THREADS / 64; ++q) t += red[q];
     partial[blockIdx.x] = t;
   }
 }
 
-// Inclusive scan of s[0..FIND_THREADS) in LDS (fixed order).
+// Inclusive scan of s[0..FIND_THREADS) in LDS (fixed order): each wave
+// scans its 64 values with shuffles, wave 0 scans the 16 wave totals, every
+// wave adds the total of the waves before it.
 __device__ void block_scan(double *s)
 {
-  for (int off = 1; off < FIND_THREADS; off <<= 1) {
-    const double v = threadIdx.x >= off ? s[threadIdx.x - off] : 0.0;
-    __syncthreads();
-    s[threadIdx.x] += v;
-    __syncthreads();
+  __shared__ double wsum[FIND_THREADS / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double v = s[threadIdx.x];
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
   }
+  if (lane == 63) wsum[w] = v;
+  __syncthreads();
+  if (w == 0) {
+    double x = lane < FIND_THREADS / 64 ? wsum[lane] : 0.0;
+#pragma unroll
+    for (int o = 1; o < FIND_THREADS / 64; o <<= 1) {
+      const double t = __shfl_up(x, o, 64);
+      if (lane >= o) x += t;
+    }
+    if (lane < FIND_THREADS / 64) wsum[lane] = x;
+  }
+  __syncthreads();
+  if (w > 0) v += wsum[w - 1];
+  s[threadIdx.x] = v;
+  __syncthreads();
 }
 
 // First position i in [0, m) with base + incl[i] > t; if none (rounding at
 // the very top), the last position with a positive increment (0 if none).
+// incl is a running sum of non-negative weights, so base + incl[i] never
+// decreases and the first position above t is the count of positions at or
+// below it (one LDS atomic per wave).
 __device__ int first_above(const double *incl, int m, double base, double t, int *slot)
 {
-  if (threadIdx.x == 0) *slot = m;
+  if (threadIdx.x == 0) *slot = 0;
   __syncthreads();
-  for (int i = threadIdx.x; i < m; i += blockDim.x)
-    if (base + incl[i] > t) atomicMin(slot, i);
+  int cnt = 0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) cnt += (base + incl[i] <= t) ? 1 : 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(slot, cnt);
   __syncthreads();
   int r = *slot;
   __syncthreads();
@@ -175,7 +208,7 @@ __device__ int first_above(const double *incl, int m, double base, double t, int
 // One workgroup: the index drawn with probability w_i / sum(w) for the
 // uniform u[ui], written to out[0].
 __global__ __launch_bounds__(FIND_THREADS) void k_sel_find(SelArgs p, const double *__restrict__ partial,
-                                                           int nblk, const double *__restrict__ u,
+                                                           int64_t nblk, const double *__restrict__ u,
                                                            int64_t ui, uint64_t *__restrict__ out)
 {
   __shared__ double s[FIND_THREADS];
@@ -186,43 +219,60 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sel_find(SelArgs p, const doub
     move_table(p, e);
     a = *p.cur;
   }
-  s[threadIdx.x] = threadIdx.x < nblk ? partial[threadIdx.x] : 0.0;
-  __syncthreads();
-  block_scan(s);
-  const double total = s[nblk - 1];
-  const double t = u[ui] * total;
-  const int b = first_above(s, nblk, 0.0, t, &slot);
-  const double base = b ? s[b - 1] : 0.0;
-  __syncthreads();
-  // the chosen chunk in FIND_THREADS contiguous segments
-  const int64_t c0 = (int64_t)b * p.chunk;
-  const int64_t c1 = c0 + p.chunk < p.n ? c0 + p.chunk : p.n;
-  const int64_t seg = (c1 - c0 + FIND_THREADS - 1) / FIND_THREADS;
-  const int64_t s0 = c0 + threadIdx.x * seg;
-  const int64_t s1 = s0 + seg < c1 ? s0 + seg : c1;
-  double acc = 0;
-  for (int64_t i = s0; i < s1; ++i) acc += weight(p, i, e, a);
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  block_scan(s);
-  const int g = first_above(s, FIND_THREADS, base, t, &slot);
-  if (threadIdx.x == 0) {
-    double run = base + (g ? s[g - 1] : 0.0);
-    const int64_t g0 = c0 + g * seg;
-    const int64_t g1 = g0 + seg < c1 ? g0 + seg : c1;
-    int64_t pick = -1, lastpos = g0 < c1 ? g0 : c1 - 1;
-    for (int64_t i = g0; i < g1; ++i) {
-      const double w = weight(p, i, e, a);
-      if (w > 0) lastpos = i;
-      run += w;
-      if (run > t) {
-        pick = i;
-        break;
+  const int j = threadIdx.x;
+  // level 1: groups of pp consecutive chunk totals
+  const int64_t pp = (nblk + FIND_THREADS - 1) / FIND_THREADS;
+  {
+    double acc = 0;
+    const int64_t g0 = (int64_t)j * pp;
+    for (int64_t q = 0; q < pp; q += 8) {
+      double v[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int64_t c = g0 + q + r;
+        v[r] = (q + r < pp && c < nblk) ? partial[c] : 0.0;
       }
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc += v[r];
     }
-    const uint64_t r = (uint64_t)(pick >= 0 ? pick : lastpos);
+    s[j] = acc;
+  }
+  __syncthreads();
+  block_scan(s);
+  const double total = s[FIND_THREADS - 1];
+  const double t = u[ui] * total;
+  const int g = first_above(s, FIND_THREADS, 0.0, t, &slot);
+  double base = g ? s[g - 1] : 0.0;
+  __syncthreads();
+  // level 2: the chunks of group g
+  {
+    const int64_t c = (int64_t)g * pp + j;
+    s[j] = (j < pp && c < nblk) ? partial[c] : 0.0;
+  }
+  __syncthreads();
+  block_scan(s);
+  const int64_t b = (int64_t)g * pp + first_above(s, FIND_THREADS, base, t, &slot);
+  {
+    const int ib = (int)(b - (int64_t)g * pp);
+    base += ib ? s[ib - 1] : 0.0;
+  }
+  __syncthreads();
+  // level 3: element pairs of chunk b
+  const int64_t i0 = b * SEL_CHUNK + 2 * j;
+  const double w0 = i0 < p.n ? weight(p, i0, e, a) : 0.0;
+  const double w1 = i0 + 1 < p.n ? weight(p, i0 + 1, e, a) : 0.0;
+  s[j] = w0 + w1;
+  __syncthreads();
+  block_scan(s);
+  const int k = first_above(s, FIND_THREADS, base, t, &slot);
+  if (j == k) {
+    const double before = base + (k ? s[k - 1] : 0.0);
+    int64_t r;
+    if (before + w0 > t && w0 > 0) r = i0;
+    else if (w1 > 0) r = i0 + 1;
+    else r = i0;
     // YAT: (b, a) -> (c, b)
-    out[0] = p.mode == SEL_YAT ? ((r << p.L) | (a >> p.L)) : r;
+    out[0] = p.mode == SEL_YAT ? (((uint64_t)r << p.L) | (a >> p.L)) : (uint64_t)r;
   }
 }
 
@@ -299,6 +349,8 @@ __global__ __launch_bounds__(64) void k_orbit_pick(const uint64_t *__restrict__ 
   __shared__ uint64_t mem[256];
   __shared__ double cum[256];
   __shared__ bool first[256];
+  __shared__ int nmem;
+  __shared__ double tgt;
   const int nbits = grp == GRP_Y ? 2 * L : L;
   const uint64_t lm = (1ull << L) - 1;
   const uint64_t full = nbits >= 64 ? ~0ull : (1ull << nbits) - 1;
@@ -336,40 +388,62 @@ __global__ __launch_bounds__(64) void k_orbit_pick(const uint64_t *__restrict__ 
   }
   __syncthreads();
   // distinct members in ascending order: rank among the first occurrences
+  // (loops without early exit, unrolled, so that the LDS reads overlap)
   for (int c = threadIdx.x; c < nc; c += 64) {
-    bool f = true;
-    for (int d = 0; d < c; ++d)
-      if (cand[d] == cand[c]) { f = false; break; }
-    first[c] = f;
+    const uint64_t v = cand[c];
+    bool dup = false;
+#pragma unroll 8
+    for (int d = 0; d < c; ++d) dup |= cand[d] == v;
+    first[c] = !dup;
   }
   __syncthreads();
   for (int c = threadIdx.x; c < nc; c += 64) {
     if (!first[c]) continue;
+    const uint64_t v = cand[c];
     int rank = 0;
-    for (int d = 0; d < nc; ++d) rank += (first[d] && cand[d] < cand[c]);
-    mem[rank] = cand[c];
+#pragma unroll 8
+    for (int d = 0; d < nc; ++d) rank += (first[d] && cand[d] < v);
+    mem[rank] = v;
   }
-  __syncthreads();
   if (threadIdx.x == 0) {
     int m = 0;
     for (int c = 0; c < nc; ++c) m += first[c];
+    nmem = m;
+  }
+  __syncthreads();
+  const int m = nmem;
+  // member weights on every lane, summed in ascending order by one lane
+  const uint64_t a = *prev;
+  for (int i = threadIdx.x; i < m; i += 64) {
+    bool ok;
+    const double x = member_cross(ck, a, mem[i], L, lm, c0, c1, c2, &ok);
+    cum[i] = ok ? exp(x) : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
     if (m != (int)dups[*qa]) *err = 1;
-    const uint64_t a = *prev;
     double tot = 0;
+#pragma unroll 8
     for (int i = 0; i < m; ++i) {
-      bool ok;
-      const double x = member_cross(ck, a, mem[i], L, lm, c0, c1, c2, &ok);
-      if (ok) tot += exp(x);
+      tot += cum[i];
       cum[i] = tot;
     }
-    const double t = u[ui] * tot;
-    int pick = -1, lastpos = 0;
-    for (int i = 0; i < m; ++i) {
-      if (cum[i] > (i ? cum[i - 1] : 0.0)) lastpos = i;
-      if (pick < 0 && cum[i] > t) pick = i;
-    }
-    out[0] = mem[pick >= 0 ? pick : lastpos];
+    tgt = u[ui] * tot;
   }
+  __syncthreads();
+  // first i with cum[i] > t (cum never decreases: the count of entries at or
+  // below t); else the last with a positive increment
+  int cnt = 0, last = 0;
+  for (int i = threadIdx.x; i < m; i += 64) {
+    cnt += cum[i] <= tgt ? 1 : 0;
+    if (cum[i] > (i ? cum[i - 1] : 0.0)) last = i;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    cnt += __shfl_down(cnt, o, 64);
+    last = max(last, __shfl_down(last, o, 64));
+  }
+  if (threadIdx.x == 0) out[0] = mem[cnt < m ? cnt : last];
 }
 
 __global__ void k_rep_of(const uint64_t *__restrict__ qa, const uint64_t *__restrict__ reps,
@@ -388,7 +462,9 @@ void sample_chains(tm_op *op, const double *vec, const double *vecl, const doubl
   const int64_t n = op->ncol;
   const bool red = is_reduced(kind);
   const bool yat = kind == TM_KIND_YAT;
-  DevBuf partial(sizeof(double) * SEL_MAXBLK), qa(sizeof(uint64_t)), e, y;
+  const int64_t mmax = std::max<int64_t>(n, kind == TM_KIND_YAT ? (int64_t(1) << op->L) : 0);
+  if (mmax > SEL_MAXN) throw Error(TM_EUNSUPPORTED, "sampling supports up to 2^31 states");
+  DevBuf partial(sizeof(double) * ((mmax + SEL_CHUNK - 1) / SEL_CHUNK)), qa(sizeof(uint64_t)), e, y;
   if (red) {
     e.alloc(sizeof(double) * n);
     y.alloc(sizeof(double) * n);
@@ -405,21 +481,16 @@ void sample_chains(tm_op *op, const double *vec, const double *vecl, const doubl
   // one weighted draw over [0, m)
   auto draw = [&](int mode, int64_t m, const uint64_t *cur, const double *yv, int64_t ui,
                   uint64_t *out) {
-    int64_t nblk = (m + SEL_THREADS * 4 - 1) / (SEL_THREADS * 4);
-    nblk = std::max<int64_t>(1, std::min<int64_t>(nblk, SEL_MAXBLK));
-    int64_t chunk = (m + nblk - 1) / nblk;
-    chunk = (chunk + SEL_THREADS - 1) / SEL_THREADS * SEL_THREADS;
-    nblk = (m + chunk - 1) / chunk;
+    const int64_t nblk = (m + SEL_CHUNK - 1) / SEL_CHUNK;
     a.mode = mode;
     a.n = m;
-    a.chunk = chunk;
     a.cur = cur;
     a.y = yv;
     hipLaunchKernelGGL(k_sel_partials, dim3((unsigned)nblk), dim3(SEL_THREADS), 0, s, a,
                        partial.as<double>());
     TM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sel_find, dim3(1), dim3(FIND_THREADS), 0, s, a, partial.as<double>(),
-                       (int)nblk, u, ui, out);
+                       nblk, u, ui, out);
     TM_LAUNCH_CHECK();
   };
   const int start = vecl ? SEL_PAIR : SEL_SQUARE;
