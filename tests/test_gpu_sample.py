@@ -141,3 +141,24 @@ def test_dumpconfig_program(lt, tmp_path, model, axialdir, size):
         assert lines[-1] == "" and len(lines) == nlines + 1
         for ln in lines[:-1]:
             assert len(ln) == size and set(ln) <= {"0", "1"}
+    # the same chains through the C ABI with the program's draws (std::mt19937
+    # seeded 7, restated in tests/std_random.py) and its output lines
+    from tests import std_random
+    kap2 = 0.25 if model == "B" else 0.0
+    op = lt.TransferMatrix(model, 2, axialdir, size, J=0.45, kappa=0.35, kappa2=kap2)
+    n = op.cols()
+    vec = np.fromfile(tmp_path / "P_vec.dat", dtype=np.float64)[:n]
+    vecl = None if op.is_symmetric() else np.fromfile(tmp_path / "P_vecl.dat", dtype=np.float64)[:n]
+    u = np.array(std_random.uniforms(7, op.sample_uniforms_needed(3, 40)))
+    st = op.sample_configs(vec, u, 3, 40, vecl)
+    lm = (1 << size) - 1
+    kind = op.kind()
+    for c in range(3):
+        if kind == "YAT":
+            want = [int(st[c, 0]) & lm] + [int(x) >> size for x in st[c]]
+        elif kind in ("YAM", "YBM"):
+            want = [int(x) & lm for x in st[c]]
+        else:
+            want = [int(x) for x in st[c]]
+        text = "".join("".join(str((s >> b) & 1) for b in range(size)) + "\n" for s in want)
+        assert outs[0][c] == text
