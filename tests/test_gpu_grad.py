@@ -92,7 +92,7 @@ def test_dnni_l22_free_energy_gradient(lt):
     f = free_energy(theta, spec)
     f.backward()
     g = theta.grad.numpy()
-    print("f", float(f), "grad", g, flush=True)
+    print("f", float(f.detach()), "grad", g, flush=True)
     for i in (0, 1, 3):
         tp = theta.detach().clone()
         tm = theta.detach().clone()
