@@ -18,9 +18,10 @@
 // running sum exceeds u times the total, u one of the caller's uniforms.
 //
 // A weighted draw is two kernels: every workgroup sums the weights of one
-// contiguous 2048-element chunk (coalesced), then one workgroup finds the
-// index in three levels -- groups of consecutive chunk totals, the chunks of
-// the chosen group, the element pairs of the chosen chunk -- each a 1024-wide
+// contiguous chunk of 2048 or 4096 elements (coalesced), then one workgroup
+// finds the index in three levels -- groups of consecutive chunk totals, the
+// chunks of the chosen group, 2 or 4 elements per thread of the chosen
+// chunk -- each a 1024-wide
 // scan in LDS after a few independent loads per thread.  The whole chain stays on the device (the current state
 // is read by the next step's kernels), so a chain of any length is one queue
 // of launches with no host round trip.
@@ -32,10 +33,11 @@ namespace ltm {
 namespace {
 
 constexpr int SEL_THREADS = 256;
-constexpr int SEL_PER = 8;                          // elements per thread
-constexpr int SEL_CHUNK = SEL_THREADS * SEL_PER;     // elements per workgroup
 constexpr int FIND_THREADS = 1024;
-constexpr int64_t SEL_MAXN = int64_t(SEL_CHUNK) * FIND_THREADS * FIND_THREADS;  // 2^31
+// elements per thread of the chunk sums: 16 for large draws (4096-element
+// chunks, fewer and longer streams), 8 below SEL_BIG (enough workgroups)
+constexpr int64_t SEL_BIG = int64_t(1) << 23;
+constexpr int64_t SEL_MAXN = int64_t(SEL_THREADS) * 16 * FIND_THREADS * FIND_THREADS;  // 2^32
 
 enum SelMode { SEL_SQUARE = 0, SEL_XAT = 1, SEL_PRODUCT = 2, SEL_PAIR = 3, SEL_YAT = 4 };
 
@@ -115,8 +117,10 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
+template <int PER>
 __global__ __launch_bounds__(SEL_THREADS) void k_sel_partials(SelArgs p, double *__restrict__ partial)
 {
+  constexpr int SEL_CHUNK = SEL_THREADS * PER;
   __shared__ double e[65];
   __shared__ double red[SEL_THREADS / 64];
   uint64_t a = 0;
@@ -126,15 +130,15 @@ __global__ __launch_bounds__(SEL_THREADS) void k_sel_partials(SelArgs p, double 
     __syncthreads();
   }
   const int64_t b0 = (int64_t)blockIdx.x * SEL_CHUNK;
-  double w[SEL_PER];
+  double w[PER];
 #pragma unroll
-  for (int k = 0; k < SEL_PER; ++k) {
+  for (int k = 0; k < PER; ++k) {
     const int64_t i = b0 + k * SEL_THREADS + threadIdx.x;
     w[k] = i < p.n ? weight(p, i, e, a) : 0.0;
   }
   double acc = 0;
 #pragma unroll
-  for (int k = 0; k < SEL_PER; ++k) acc += w[k];
+  for (int k = 0; k < PER; ++k) acc += w[k];
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
@@ -207,10 +211,13 @@ __device__ int first_above(const double *incl, int m, double base, double t, int
 
 // One workgroup: the index drawn with probability w_i / sum(w) for the
 // uniform u[ui], written to out[0].
+template <int PER>
 __global__ __launch_bounds__(FIND_THREADS) void k_sel_find(SelArgs p, const double *__restrict__ partial,
                                                            int64_t nblk, const double *__restrict__ u,
                                                            int64_t ui, uint64_t *__restrict__ out)
 {
+  constexpr int SEL_CHUNK = SEL_THREADS * PER;
+  constexpr int FIND_EPT = SEL_CHUNK / FIND_THREADS;  // elements per thread in the last level
   __shared__ double s[FIND_THREADS];
   __shared__ double e[65];
   __shared__ int slot;
@@ -257,22 +264,31 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sel_find(SelArgs p, const doub
     base += ib ? s[ib - 1] : 0.0;
   }
   __syncthreads();
-  // level 3: element pairs of chunk b
-  const int64_t i0 = b * SEL_CHUNK + 2 * j;
-  const double w0 = i0 < p.n ? weight(p, i0, e, a) : 0.0;
-  const double w1 = i0 + 1 < p.n ? weight(p, i0 + 1, e, a) : 0.0;
-  s[j] = w0 + w1;
+  // level 3: FIND_EPT consecutive elements per thread in chunk b
+  const int64_t i0 = b * SEL_CHUNK + (int64_t)FIND_EPT * j;
+  double w[FIND_EPT];
+  double acc = 0;
+#pragma unroll
+  for (int r = 0; r < FIND_EPT; ++r) {
+    w[r] = i0 + r < p.n ? weight(p, i0 + r, e, a) : 0.0;
+    acc += w[r];
+  }
+  s[j] = acc;
   __syncthreads();
   block_scan(s);
   const int k = first_above(s, FIND_THREADS, base, t, &slot);
   if (j == k) {
-    const double before = base + (k ? s[k - 1] : 0.0);
-    int64_t r;
-    if (before + w0 > t && w0 > 0) r = i0;
-    else if (w1 > 0) r = i0 + 1;
-    else r = i0;
+    double run = base + (k ? s[k - 1] : 0.0);
+    int64_t pick = -1, lastp = i0;
+#pragma unroll
+    for (int r = 0; r < FIND_EPT; ++r) {
+      if (w[r] > 0) lastp = i0 + r;
+      run += w[r];
+      if (pick < 0 && w[r] > 0 && run > t) pick = i0 + r;
+    }
+    const int64_t rr = pick >= 0 ? pick : lastp;
     // YAT: (b, a) -> (c, b)
-    out[0] = p.mode == SEL_YAT ? (((uint64_t)r << p.L) | (a >> p.L)) : (uint64_t)r;
+    out[0] = p.mode == SEL_YAT ? (((uint64_t)rr << p.L) | (a >> p.L)) : (uint64_t)rr;
   }
 }
 
@@ -463,8 +479,12 @@ void sample_chains(tm_op *op, const double *vec, const double *vecl, const doubl
   const bool red = is_reduced(kind);
   const bool yat = kind == TM_KIND_YAT;
   const int64_t mmax = std::max<int64_t>(n, kind == TM_KIND_YAT ? (int64_t(1) << op->L) : 0);
-  if (mmax > SEL_MAXN) throw Error(TM_EUNSUPPORTED, "sampling supports up to 2^31 states");
-  DevBuf partial(sizeof(double) * ((mmax + SEL_CHUNK - 1) / SEL_CHUNK)), qa(sizeof(uint64_t)), e, y;
+  if (mmax > SEL_MAXN) throw Error(TM_EUNSUPPORTED, "sampling supports up to 2^32 states");
+  // chunk totals of the largest draw (8 elements per thread below SEL_BIG)
+  const int64_t pmax = std::max<int64_t>((mmax + SEL_THREADS * 16 - 1) / (SEL_THREADS * 16),
+                                         (std::min<int64_t>(mmax, SEL_BIG) + SEL_THREADS * 8 - 1) /
+                                             (SEL_THREADS * 8));
+  DevBuf partial(sizeof(double) * pmax), qa(sizeof(uint64_t)), e, y;
   if (red) {
     e.alloc(sizeof(double) * n);
     y.alloc(sizeof(double) * n);
@@ -481,16 +501,26 @@ void sample_chains(tm_op *op, const double *vec, const double *vecl, const doubl
   // one weighted draw over [0, m)
   auto draw = [&](int mode, int64_t m, const uint64_t *cur, const double *yv, int64_t ui,
                   uint64_t *out) {
-    const int64_t nblk = (m + SEL_CHUNK - 1) / SEL_CHUNK;
+    const bool big = m >= SEL_BIG;
+    const int64_t chunk = SEL_THREADS * (big ? 16 : 8);
+    const int64_t nblk = (m + chunk - 1) / chunk;
     a.mode = mode;
     a.n = m;
     a.cur = cur;
     a.y = yv;
-    hipLaunchKernelGGL(k_sel_partials, dim3((unsigned)nblk), dim3(SEL_THREADS), 0, s, a,
-                       partial.as<double>());
-    TM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sel_find, dim3(1), dim3(FIND_THREADS), 0, s, a, partial.as<double>(),
-                       nblk, u, ui, out);
+    if (big) {
+      hipLaunchKernelGGL(k_sel_partials<16>, dim3((unsigned)nblk), dim3(SEL_THREADS), 0, s, a,
+                         partial.as<double>());
+      TM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_sel_find<16>, dim3(1), dim3(FIND_THREADS), 0, s, a,
+                         partial.as<double>(), nblk, u, ui, out);
+    } else {
+      hipLaunchKernelGGL(k_sel_partials<8>, dim3((unsigned)nblk), dim3(SEL_THREADS), 0, s, a,
+                         partial.as<double>());
+      TM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_sel_find<8>, dim3(1), dim3(FIND_THREADS), 0, s, a,
+                         partial.as<double>(), nblk, u, ui, out);
+    }
     TM_LAUNCH_CHECK();
   };
   const int start = vecl ? SEL_PAIR : SEL_SQUARE;
