@@ -444,6 +444,23 @@ def bench_sample(lt, args):
                                    "kind": "port",
                                    "sample": f"{cm} oracle moves, identical states"}
         print(json.dumps(out), flush=True)
+    # the X_phys.dat observables of the headline operator (one device reduction)
+    op = lt.TransferMatrix("A", 2, 5, 28, J=0.4, kappa=0.5, neigs=4)
+    n = op.cols()
+    q = np.random.default_rng(5).uniform(0.0, 1.0, n)
+    op.moments(q)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        f = op.moments(q)
+        ts.append(time.perf_counter() - t0)
+    print(json.dumps({
+        "metric": "XAM L=28 X_phys.dat observables (tm_op_moments), host vector in",
+        "value": round(min(ts) * 1e3, 3), "unit": "ms", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic (random vector)",
+        "config": {"workload": "reduced ANNNI L=28, neigs 4", "cols": int(n)},
+        "note": "includes the PCIe upload of the vector; the device reduction itself is in the "
+                "kernel trace (k_phys_sums)", "figures": [float(x) for x in f]}), flush=True)
 
 
 # (label, TransferMatrix args, kwargs): one operator of every kind at a size
