@@ -138,7 +138,8 @@ tm_status tm_op_orbits(const tm_op *op, uint64_t *reps, uint16_t *dups);
 
 /* Options of the eigen driver.  Defaults (tm_solve_opts_default): nev=1,
  * ncv=0 (=suggested_nev), tol=1e-10, maxit=1000, maxtrial=3 (pfunc.hpp:87),
- * precision FP64, seed 1, want_left=0, use_graph=1, basis_f32=1, refine_extra=1. */
+ * precision FP64, seed 1, want_left=0, use_graph=1, basis_f32=1, refine_extra=1,
+ * verify=1. */
 typedef struct {
   int          nev;
   int          ncv;
@@ -169,6 +170,11 @@ typedef struct {
                                a larger block widens the gap in the error
                                bound, a smaller one saves FP64 T.x; a block
                                whose bound is not met is iterated          */
+  int          verify;      /* 1 (default): before returning TM_OK, check the
+                               leading pair's explicit FP64 residual
+                               ||Tx - lambda x|| / |lambda| <= 1e-6 (one FP64
+                               T.x, or the refinement's explicit residual);
+                               a failed check returns TM_EHIP              */
 } tm_solve_opts;
 
 /* Results.  Eigenvalues are those of the operator as applied (the reference
@@ -190,6 +196,8 @@ typedef struct {
   int      trials;
   double   seconds;         /* wall time of the solve (setup excluded)        */
   int64_t  graph_cycles;    /* restart cycles replayed from a hipGraph        */
+  double   resid_check;     /* explicit FP64 relative residual of the leading
+                               pair (NaN if opts.verify is 0)                 */
 } tm_solve_out;
 
 void      tm_solve_opts_default(tm_solve_opts *o);

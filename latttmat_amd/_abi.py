@@ -50,7 +50,7 @@ class SolveOpts(C.Structure):
         ("maxtrial", C.c_int), ("precision", C.c_int), ("seed", C.c_uint64),
         ("want_left", C.c_int), ("want_vectors", C.c_int), ("mixed_tol", C.c_double),
         ("verbose", C.c_int), ("refine", C.c_int), ("use_graph", C.c_int),
-        ("basis_f32", C.c_int), ("refine_extra", C.c_int),
+        ("basis_f32", C.c_int), ("refine_extra", C.c_int), ("verify", C.c_int),
     ]
 
 
@@ -61,7 +61,7 @@ class SolveOut(C.Structure):
         ("vecs", C.POINTER(C.c_double)), ("vecl", C.POINTER(C.c_double)),
         ("eval_left", C.c_double), ("matvecs_f64", C.c_int64), ("matvecs_f32", C.c_int64),
         ("restarts", C.c_int64), ("trials", C.c_int), ("seconds", C.c_double),
-        ("graph_cycles", C.c_int64),
+        ("graph_cycles", C.c_int64), ("resid_check", C.c_double),
     ]
 
 
@@ -303,7 +303,7 @@ class SolveResult:
 
 
 def _make_opts(nev, ncv, tol, maxit, precision, seed, want_left, want_vectors, mixed_tol,
-               verbose, maxtrial, refine, use_graph, basis_f32=True, refine_extra=1):
+               verbose, maxtrial, refine, use_graph, basis_f32=True, refine_extra=1, verify=True):
     o = SolveOpts()
     lib().tm_solve_opts_default(C.byref(o))
     o.nev, o.ncv, o.tol, o.maxit, o.maxtrial = nev, ncv, tol, maxit, maxtrial
@@ -313,6 +313,7 @@ def _make_opts(nev, ncv, tol, maxit, precision, seed, want_left, want_vectors, m
     o.use_graph = int(use_graph)
     o.basis_f32 = int(basis_f32)
     o.refine_extra = int(refine_extra)
+    o.verify = int(verify)
     return o
 
 
@@ -346,15 +347,16 @@ class _OutBufs:
                 vecs = (v[..., 0] + 1j * v[..., 1]).T.copy()
         return SolveResult(st, out.nconv, evals, self.rs, vecs, self.vecl, out.eval_left,
                            out.matvecs_f64, out.matvecs_f32, out.restarts, out.trials,
-                           out.seconds, {"graph_cycles": int(out.graph_cycles)})
+                           out.seconds, {"graph_cycles": int(out.graph_cycles),
+                                         "resid_check": float(out.resid_check)})
 
 
 def solve(op: TransferMatrix, nev=1, ncv=0, tol=1e-10, maxit=1000, precision=TM_PREC_F64,
           seed=1, want_left=False, want_vectors=False, mixed_tol=2e-7, verbose=False,
           maxtrial=3, raise_on_noconv=True, refine=0, use_graph=True,
-          basis_f32=True, refine_extra=1) -> SolveResult:
+          basis_f32=True, refine_extra=1, verify=True) -> SolveResult:
     o = _make_opts(nev, ncv, tol, maxit, precision, seed, want_left, want_vectors, mixed_tol,
-                   verbose, maxtrial, refine, use_graph, basis_f32, refine_extra)
+                   verbose, maxtrial, refine, use_graph, basis_f32, refine_extra, verify)
     b = _OutBufs(op, nev, want_left, want_vectors)
     st = lib().tm_solve(op._h, C.byref(o), C.byref(b.out))
     if st not in (TM_OK, TM_ENOCONV) or (st == TM_ENOCONV and raise_on_noconv):
@@ -364,11 +366,11 @@ def solve(op: TransferMatrix, nev=1, ncv=0, tol=1e-10, maxit=1000, precision=TM_
 
 def solve_batch(ops, nev=1, ncv=0, tol=1e-10, maxit=1000, precision=TM_PREC_F64, seed=1,
                 want_left=False, want_vectors=False, mixed_tol=2e-7, maxtrial=3, refine=0,
-                use_graph=True, nworkers=4, raise_on_error=True, basis_f32=True):
+                use_graph=True, nworkers=4, raise_on_error=True, basis_f32=True, verify=True):
     """tm_solve_batch over a list of operators with one set of options."""
     n = len(ops)
     o = _make_opts(nev, ncv, tol, maxit, precision, seed, want_left, want_vectors, mixed_tol,
-                   False, maxtrial, refine, use_graph, basis_f32)
+                   False, maxtrial, refine, use_graph, basis_f32, 1, verify)
     opts = (SolveOpts * n)(*([o] * n))
     bufs = [_OutBufs(op, nev, want_left, want_vectors) for op in ops]
     outs = (SolveOut * n)(*[b.out for b in bufs])

@@ -4,6 +4,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -91,26 +92,40 @@ tm_status tm_op_apply_dev(const tm_op *cop, const void *x, void *y, int64_t ld, 
   tm_op *op = const_cast<tm_op *>(cop);
   return guarded([&] {
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (compute == TM_F32) {
-      // per-vector power-of-two scale so that max|x| is in [1/2, 1)
-      ltm::DevBuf partial(sizeof(double) * ltm::kry::NBLK);
-      ltm::DevBuf scale(sizeof(double) * nvec);
-      const size_t es = dtype == TM_F64 ? 8 : 4;
-      if (dtype == TM_F64) {
-        for (int v = 0; v < nvec; ++v) {
-          ltm::kry::pow2_scale(static_cast<const double *>(x) + (size_t)v * ld, op->ncol,
-                              partial.as<double>(), scale.as<double>() + v, s);
-          op->apply_dev(static_cast<const char *>(x) + (size_t)v * ld * es,
-                        static_cast<char *>(y) + (size_t)v * ld * es, ld, 1, dtype, compute,
-                        transpose, scale.as<double>() + v, s);
+    // The apply uses the operator's workspaces (op->work, px, py), which
+    // the operator's own stream also uses (tm_op_apply, tm_solve): order the
+    // caller's stream after the operator's pending work, and the operator's
+    // stream after this apply.
+    ltm::stream_wait(s, op->stream);
+    try {
+      if (compute == TM_F32) {
+        // per-vector power-of-two scale so that max|x| is in [1/2, 1)
+        ltm::DevBuf partial(sizeof(double) * ltm::kry::NBLK);
+        ltm::DevBuf scale(sizeof(double) * nvec);
+        const size_t es = dtype == TM_F64 ? 8 : 4;
+        if (dtype == TM_F64) {
+          for (int v = 0; v < nvec; ++v) {
+            ltm::kry::pow2_scale(static_cast<const double *>(x) + (size_t)v * ld, op->ncol,
+                                 partial.as<double>(), scale.as<double>() + v, s);
+            op->apply_dev(static_cast<const char *>(x) + (size_t)v * ld * es,
+                          static_cast<char *>(y) + (size_t)v * ld * es, ld, 1, dtype, compute,
+                          transpose, scale.as<double>() + v, s);
+          }
+        } else {
+          op->apply_dev(x, y, ld, nvec, dtype, compute, transpose, nullptr, s);
         }
+        TM_HIP(hipStreamSynchronize(s));  // scratch buffers are released on return
       } else {
         op->apply_dev(x, y, ld, nvec, dtype, compute, transpose, nullptr, s);
       }
-      TM_HIP(hipStreamSynchronize(s));  // scratch buffers are released on return
-    } else {
-      op->apply_dev(x, y, ld, nvec, dtype, compute, transpose, nullptr, s);
+    } catch (...) {
+      try {
+        ltm::stream_wait(op->stream, s);
+      } catch (...) {
+      }
+      throw;
     }
+    ltm::stream_wait(op->stream, s);
     return TM_OK;
   });
 }
@@ -203,6 +218,7 @@ void tm_solve_opts_default(tm_solve_opts *o)
   o->basis_f32 = 1;
   o->refine_extra = 1;
   o->refine = 0;
+  o->verify = 1;
 }
 
 tm_status tm_solve(tm_op *op, const tm_solve_opts *opts, tm_solve_out *out)

@@ -51,14 +51,25 @@ struct Error : std::runtime_error {
 #define TM_LAUNCH_CHECK() TM_HIP(hipGetLastError())
 
 // ---------------------------------------------------------------------------
-// Autotuner for tile sizes.  For a (kernel, L, element size) key, the first
-// call outside graph capture with tune == true runs every candidate tile
-// exponent through run(t) on stream s (one warm-up, then three timed with
+// Autotuner for tile sizes.  For a (family, kernel id, L, element size) key,
+// the first call outside graph capture with tune == true runs every
+// candidate through run(t) on stream s (one warm-up, then three timed with
 // HIP events), caches the fastest and returns it; later calls with tune ==
 // true return the cached choice.  With tune == false, or during a capture
-// before a choice exists, returns cands[0] (the LDS-size default).
-int autotune_tile(uint32_t kernel, int L, int es, const std::vector<int> &cands, bool tune,
-                  hipStream_t s, const std::function<void(int)> &run);
+// before a choice exists, returns cands[0] (the LDS-size default).  Each
+// kernel family has its own id space: a candidate value means something
+// different to each family's launcher.  run(t) must leave its input intact
+// (callers pass tune == false for in-place applies).
+enum TuneFamily : int { TUNE_KRON = 1, TUNE_KRON_DUAL = 2, TUNE_KRON_RANGE = 3, TUNE_SWEEP = 4 };
+int autotune_tile(TuneFamily fam, uint32_t id, int L, int es, const std::vector<int> &cands,
+                  bool tune, hipStream_t s, const std::function<void(int)> &run);
+// the cached choice for a key, or -1 (for tests)
+int autotune_cached(TuneFamily fam, uint32_t id, int L, int es);
+
+// Order all work enqueued on `waiter` from now on after the work enqueued
+// on `producer` so far (an event recorded on producer, waited on by
+// waiter).  No-op when the two are the same stream.
+void stream_wait(hipStream_t waiter, hipStream_t producer);
 
 // Stream-ordered copy that returns once the data has arrived.  The library
 // never uses the legacy null stream, whose implicit synchronisation with
@@ -78,6 +89,8 @@ struct CaptureGuard {
   CaptureGuard(const CaptureGuard &) = delete;
   CaptureGuard &operator=(const CaptureGuard &) = delete;
 };
+// true on a thread that holds a CaptureGuard
+bool capturing_here();
 // hipGraphExecDestroy under the shared side of the same lock
 void destroy_graph_exec(hipGraphExec_t g);
 
@@ -86,7 +99,11 @@ class PinnedBuf {
  public:
   PinnedBuf() = default;
   explicit PinnedBuf(size_t bytes) { alloc(bytes); }
-  ~PinnedBuf() { release(); }
+  // a buffer cannot be freed inside a graph capture: it is then leaked
+  ~PinnedBuf()
+  {
+    try { release(); } catch (...) {}
+  }
   PinnedBuf(const PinnedBuf &) = delete;
   PinnedBuf &operator=(const PinnedBuf &) = delete;
   void alloc(size_t bytes);
@@ -105,11 +122,15 @@ class DevBuf {
  public:
   DevBuf() = default;
   explicit DevBuf(size_t bytes) { alloc(bytes); }
-  ~DevBuf() { release(); }
+  // a buffer cannot be freed inside a graph capture: it is then leaked
+  ~DevBuf()
+  {
+    try { release(); } catch (...) {}
+  }
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
-  DevBuf &operator=(DevBuf &&o) noexcept {
+  DevBuf &operator=(DevBuf &&o) {
     if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
     return *this;
   }
@@ -130,7 +151,7 @@ class DevBuf {
 // assigned by the eigensolver).  A graph is keyed by every pointer and size its
 // launches embed, so a key match means identical kernel arguments.
 struct SolverCache {
-  static constexpr int NDEV = 24, NPIN = 4;
+  static constexpr int NDEV = 25, NPIN = 4;
   DevBuf dev[NDEV];
   PinnedBuf pinned[NPIN];
   struct Graph {

@@ -25,8 +25,11 @@ constexpr int KRON_WIDE = 256;
 enum { F_PRO = 1, F_EPI = 2, F_SIN = 4, F_SOUT = 8, F_TWO = 16 };
 
 template <typename C, typename TI, typename TO, int NT>
+// in and out may be the same array (passes after the first work in place):
+// each workgroup reads its whole tile before the barrier and writes the same
+// tile after it, so neither pointer is __restrict__.
 __global__ __launch_bounds__(NT) void k_kron_pass(
-    const TI *__restrict__ in, TO *__restrict__ out, int b0, int nb, int logW, int flags,
+    const TI *in, TO *out, int b0, int nb, int logW, int flags,
     DiagSpec dg, const C *__restrict__ tab, DiagSpec dge, const C *__restrict__ tabe, C r, C f,
     const double *__restrict__ scale)
 {
@@ -275,7 +278,8 @@ void launch_kron_dual(const KronPlan &plan, const double *in, void *out, const v
     }
   };
   const int es = (int)sizeof(Dual);
-  run(autotune_tile(1, L, es, kron_tile_candidates(es), plan.tune, s, run));
+  run(autotune_tile(TUNE_KRON_DUAL, 0, L, es, kron_tile_candidates(es), plan.tune && in != out, s,
+                  run));
 }
 
 void launch_kron(const KronPlan &plan, const void *in, void *out, tm_dtype io_in, tm_dtype io_out,
@@ -289,7 +293,8 @@ void launch_kron(const KronPlan &plan, const void *in, void *out, tm_dtype io_in
       kron_typed<float>(plan, in, out, io_in, io_out, work, scale, s, plan.tab32, cand);
   };
   // timing candidates rewrites out from in, so only out-of-place applies tune
-  run(autotune_tile(0, plan.L, es, kron_tile_candidates(es), plan.tune && in != out, s, run));
+  run(autotune_tile(TUNE_KRON, 0, plan.L, es, kron_tile_candidates(es), plan.tune && in != out, s,
+                  run));
 }
 
 void launch_kron_range(const KronRange &kr, const void *in, void *out, void *mid,
@@ -305,8 +310,9 @@ void launch_kron_range(const KronRange &kr, const void *in, void *out, void *mid
   };
   // a candidate run must leave its input intact to be repeated
   const bool safe = in != out && in != mid;
-  const uint32_t id = 8u + (uint32_t)kr.b0 * 64u + (uint32_t)kr.nbits;
-  run(autotune_tile(id, kr.Ltot, es, kron_tile_candidates(es), tune && safe, s, run));
+  const uint32_t id = (uint32_t)kr.b0 * 64u + (uint32_t)kr.nbits;
+  run(autotune_tile(TUNE_KRON_RANGE, id, kr.Ltot, es, kron_tile_candidates(es), tune && safe, s,
+                    run));
 }
 
 }  // namespace ltm

@@ -548,8 +548,10 @@ __global__ void k_normalize(T *__restrict__ w, int64_t n, const double *__restri
     w[i] = (T)((double)w[i] * inv);
 }
 
+// Vout may be Vin (the restart rotates the basis in place), so Vin is not
+// __restrict__: every read of row i happens before any write of row i.
 template <int M, typename Ti, typename To>
-__global__ __launch_bounds__(THREADS) void k_gemm_vy(const Ti *__restrict__ Vin, int64_t ldv,
+__global__ __launch_bounds__(THREADS) void k_gemm_vy(const Ti *Vin, int64_t ldv,
                                                      int m, const double *__restrict__ Y,
                                                      int ldy, int k, To *Vout, int64_t ldo,
                                                      int64_t n)

@@ -41,8 +41,19 @@ constexpr double EPS = 2.220446049250313e-16;
 // to the caller, have slots of their own.
 constexpr int SLOT_ROLE = 6;
 constexpr int SLOT_REFINE_B = 18, SLOT_REFINE_PARTIAL = 19, SLOT_REFINE_G = 20,
-              SLOT_REFINE_C = 21, SLOT_XR = 22, SLOT_XI = 23;
-static_assert(SLOT_XI < SolverCache::NDEV, "operator cache slots");
+              SLOT_REFINE_C = 21, SLOT_XR = 22, SLOT_XI = 23, SLOT_CHECK = 24;
+static_assert(SLOT_CHECK < SolverCache::NDEV, "operator cache slots");
+
+// A projected-matrix entry read back from the device.  A non-finite value
+// means the device results are invalid (an operator or Krylov kernel
+// produced garbage); the solve stops with an error instead of turning it
+// into eigenvalues.
+double finite_or_throw(double v)
+{
+  if (!std::isfinite(v))
+    throw Error(TM_EHIP, "non-finite value in the Krylov projection: device results are invalid");
+  return v;
+}
 
 struct Ritz {
   std::vector<double> wr, wi;     // eigenvalues (sorted by magnitude, descending)
@@ -180,7 +191,7 @@ class KrylovSchur final : public KrylovRun {
     reorths_ += (int64_t)fl[0];
     const int jb = fl[1] < 0 ? m_ : (int)fl[1];
     for (int j = k0; j < jb; ++j)
-      for (int i = 0; i <= j + 1; ++i) H(i, j) = hd[i + (size_t)j * ldh];
+      for (int i = 0; i <= j + 1; ++i) H(i, j) = finite_or_throw(hd[i + (size_t)j * ldh]);
     if (jb < m_) {
       matvecs_ -= m_ - jb;
       expand_host(jb);
@@ -292,8 +303,8 @@ class KrylovSchur final : public KrylovRun {
         ++reorths_;
       }
       double hn2 = 0;
-      for (int kk = 0; kk < nk; ++kk) hn2 += H(kk, j) * H(kk, j);
-      double beta = std::sqrt(std::max(beta2, 0.0));
+      for (int kk = 0; kk < nk; ++kk) hn2 += finite_or_throw(H(kk, j)) * H(kk, j);
+      double beta = std::sqrt(std::max(finite_or_throw(beta2), 0.0));
       if (beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) {
         // invariant subspace: continue with a fresh random direction
         // orthogonal to the basis; the coupling is zero.
@@ -606,7 +617,8 @@ struct RefineOut {
   std::vector<double> theta, res;
   double bound = 0;
   int64_t matvecs = 0;
-  bool failed = false;
+  bool failed = false;  // Y^T Y not positive definite
+  bool met = false;     // the last iteration met the residual or the error bound
 };
 
 // FP64 Rayleigh-Ritz refinement of the block Y = B[0..k) (leading dimension
@@ -621,7 +633,9 @@ struct RefineOut {
 // error bound r^2 / (gap |theta|) (gap to the Krylov Ritz values outside the
 // block) is below 1e-13; otherwise Y <- W Z diag(1/theta) (= A X scaled, one
 // subspace-iteration step) and repeat.  On return B[0..nev) hold X.
-// failed = true if Y^T Y is not positive definite (a rank-deficient block).
+// failed = true if Y^T Y is not positive definite (a rank-deficient block); met =
+// false if maxiter iterations end without meeting either test (the caller
+// then does not use the block).
 RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev, double tol,
                      int maxiter, int verbose, const std::vector<double> &outside)
 {
@@ -730,7 +744,8 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
       std::fprintf(stderr, "\n");
     }
     ro.bound = bound;
-    const bool done = worst <= tol || bound <= 1e-13 || it == maxiter - 1;
+    ro.met = worst <= tol || bound <= 1e-13;
+    const bool done = ro.met || it == maxiter - 1;
     if (done) {
       // X = Y Z[:, :nev] in place (every row is read before it is written)
       for (int c = 0; c < nev; ++c)
@@ -752,6 +767,56 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
 }
 
 int suggested_ncv(int neigs) { return neigs > 4 ? 2 * neigs + 1 : 2 * neigs + 3; }  // tmat.hpp:88
+
+// Explicit FP64 residual of the leading Ritz pair (lr + i li, xr + i xi),
+// vectors of leading dimension ld in the operator's internal order: one
+// FP64 T.x per real vector, then ||T x - lambda x|| / (|lambda| ||x||) with
+// every sum formed explicitly (no cancellation).  Independent of the Krylov
+// recurrence, so a corrupted basis or operator result cannot pass it.
+double leading_residual(tm_op *op, const double *xr, const double *xi, int64_t ld, int64_t n,
+                        double lr, double li)
+{
+  hipStream_t s = op->stream;
+  SolverCache &cache = op->solver_cache();
+  const bool cplx = li != 0 && xi != nullptr;
+  const int h = cplx ? 2 : 1, nv = 2 * h;
+  DevBuf &C = cache.dev[SLOT_CHECK], &partial = cache.dev[SLOT_REFINE_PARTIAL];
+  partial.ensure(sizeof(double) * kry::NBLK * 72);
+  C.ensure(sizeof(double) * ((size_t)ld * nv + 32));
+  double *U = C.as<double>();
+  double *cm = U + (size_t)ld * nv, *res = cm + 16;
+  kry::copy(xr, U, n, s);
+  if (cplx) kry::copy(xi, U + ld, n, s);
+  for (int c = 0; c < h; ++c)
+    op->apply_internal(U + (size_t)c * ld, U + (size_t)(h + c) * ld, ld, 1, TM_F64, TM_F64, 0,
+                       nullptr, s);
+  // columns: the residual(s) T x - lambda x, then the pieces of ||x||
+  std::vector<double> co((size_t)nv * nv, 0.0);
+  if (!cplx) {
+    co[0] = -lr;  // T x - lr x
+    co[1] = 1.0;
+    co[2] = 1.0;  // x
+  } else {
+    const double a = lr, b = li;
+    double *c0 = &co[0], *c1 = &co[4], *c2 = &co[8], *c3 = &co[12];
+    c0[0] = -a; c0[1] = b; c0[2] = 1.0;   // T xr - (a xr - b xi)
+    c1[0] = -b; c1[1] = -a; c1[3] = 1.0;  // T xi - (b xr + a xi)
+    c2[0] = 1.0;
+    c3[1] = 1.0;
+  }
+  copy_sync(cm, co.data(), sizeof(double) * co.size(), hipMemcpyHostToDevice, s);
+  kry::combine_norms(U, ld, nv, cm, nv, nv, n, partial.as<double>(), res, s);
+  double hr[4] = {0, 0, 0, 0};
+  copy_sync(hr, res, sizeof(double) * nv, hipMemcpyDeviceToHost, s);
+  const double rn = cplx ? hr[0] + hr[1] : hr[0], xn = cplx ? hr[2] + hr[3] : hr[1];
+  const double lam = std::hypot(lr, li);
+  return std::sqrt(std::max(rn, 0.0)) / (lam * std::sqrt(xn));
+}
+
+// Largest explicit relative residual of the leading pair accepted as a
+// converged result (the Krylov test asks for 1e-10; an invalid result is
+// O(1)).
+constexpr double VERIFY_MAX = 1e-6;
 
 }  // namespace
 
@@ -820,7 +885,7 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
     for (int d = k; d < ks->m(); ++d) outside.push_back(rz.wr[d]);
     RefineOut ro = refine_sym(op, B.as<double>(), ldv, n, k, nev, o.tol, 4, o.verbose, outside);
     mv64 += ro.matvecs;
-    if (!ro.failed) {
+    if (!ro.failed && ro.met) {
       for (int i = 0; i < nev; ++i) {
         evr[i] = ro.theta[i];
         evi[i] = 0;
@@ -831,7 +896,8 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
       xr = B.as<double>();
       refined = true;
     } else if (o.verbose) {
-      std::fprintf(stderr, "[tm_refine] rank-deficient block; FP64 Krylov-Schur instead\n");
+      std::fprintf(stderr, "[tm_refine] %s; FP64 Krylov-Schur instead\n",
+                   ro.failed ? "rank-deficient block" : "residual and error bound not met");
     }
   }
   if (!refined) {
@@ -869,6 +935,23 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
       evi[i] = rz.wi[i];
       const double mag = std::hypot(rz.wr[i], rz.wi[i]);
       res[i] = rz.res[i] / (mag > 0 ? mag : 1.0);
+    }
+  }
+  // explicit residual of the leading pair: the refinement's residuals are
+  // already explicit FP64 ones; otherwise one FP64 T.x of the Ritz vector
+  out->resid_check = std::nan("");
+  if (o.verify && nc >= 1) {
+    double rc = refined ? res[0]
+                        : leading_residual(op, xr, xi, ldv, n, evr[0], cplx ? evi[0] : 0.0);
+    if (!refined) ++mv64;
+    out->resid_check = rc;
+    if (!(rc <= VERIFY_MAX)) {
+      char msg[160];
+      std::snprintf(msg, sizeof msg,
+                    "explicit residual check failed: ||Tx - lambda x|| / |lambda| = %.3e for "
+                    "lambda = %.17g",
+                    rc, evr[0]);
+      throw Error(TM_EHIP, msg);
     }
   }
   out->nconv = std::min(nc, nev);

@@ -191,9 +191,12 @@ __device__ __forceinline__ void zbt_gate(C v[4][4], int fl, int fr, const ZCoef<
 }
 
 template <int MODEL, int DIR, typename C, typename TI, typename TO, int NT>
+// zin and zout are the same aux array (the passes work in place): each
+// workgroup reads its tile before the barrier and writes it back after it,
+// so they are not __restrict__.
 __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
-                                                               const C *__restrict__ zin,
-                                                               C *__restrict__ zout,
+                                                               const C *zin,
+                                                               C *zout,
                                                                TO *__restrict__ y, SweepArgs a,
                                                                const C *__restrict__ tab,
                                                                const double *__restrict__ scale)
@@ -564,8 +567,10 @@ void sweep_run(const SweepPlan &plan, const void *in, void *out, void *work, con
     sweep_run_tl<MODEL, DIR, C, TI, TO>(plan, in, out, work, scale, s, tab, tl);
   };
   const int es = (int)sizeof(C);
-  const uint32_t kid = 16 + 4 * MODEL + 2 * DIR + (plan.bc != 0 ? 1 : 0);
-  run(autotune_tile(kid, plan.L, es, sweep_tile_candidates(es), plan.tune, s, run));
+  const uint32_t kid = 4 * MODEL + 2 * DIR + (plan.bc != 0 ? 1 : 0);
+  // timing candidates rewrites out from in, so only out-of-place applies tune
+  run(autotune_tile(TUNE_SWEEP, kid, plan.L, es, sweep_tile_candidates(es),
+                    plan.tune && in != out, s, run));
 }
 
 template <int MODEL, int DIR, typename C>

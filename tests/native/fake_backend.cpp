@@ -1,5 +1,10 @@
-// CPU implementations of the Krylov vector kernels and a dense-matrix
-// operator, for unit tests of the eigen driver without a GPU.  Test
+// CPU implementations of the Krylov vector kernels, a dense-matrix operator
+// and stubs for the device paths the C ABI reaches, for unit tests of the
+// eigen driver and the C ABI without a GPU.  Every "kernel" is queued on its
+// stream through the deferred-execution runtime stand-in (fake_runtime.cpp)
+// and runs only when the host waits, in stream order, like a device launch.
+// The reductions keep the device data flow: a sweep writes its sums into the
+// partial buffer (one block), the reduce step reads them from there.  Test
 // infrastructure only.
 #include <cmath>
 #include <cstdint>
@@ -10,62 +15,19 @@
 #include "../../latttmat_amd/csrc/tm_internal.hpp"
 
 namespace ltm {
-size_t g_lds_per_block = 160 * 1024;
-int g_num_cu = 256;
-[[noreturn]] void throw_hip(hipError_t, const char *expr, const char *, int)
-{
-  throw Error(TM_EHIP, expr);
-}
-void PinnedBuf::alloc(size_t bytes)
-{
-  release();
-  p_ = bytes ? std::malloc(bytes) : nullptr;
-  n_ = p_ ? bytes : 0;
-}
-void PinnedBuf::release()
-{
-  std::free(p_);
-  p_ = nullptr;
-  n_ = 0;
-}
-void DevBuf::alloc(size_t bytes)
-{
-  release();
-  p_ = bytes ? std::calloc(bytes, 1) : nullptr;
-  n_ = bytes;
-}
-void DevBuf::release()
-{
-  std::free(p_);
-  p_ = nullptr;
-  n_ = 0;
-}
-CaptureGuard::CaptureGuard() {}
-CaptureGuard::~CaptureGuard() {}
-void destroy_graph_exec(hipGraphExec_t) {}
-void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind, hipStream_t)
-{
-  std::memmove(dst, src, bytes);
-}
-void copy_large(void *dst, const void *src, size_t bytes, hipMemcpyKind, hipStream_t)
-{
-  std::memmove(dst, src, bytes);
-}
 namespace kry {
+namespace {
 template <typename T>
-unsigned multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *, double *out,
-                  hipStream_t, bool)
+void dots_now(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *out)
 {
   for (int k = 0; k <= j; ++k) out[k] = 0;
   for (int64_t i = 0; i < n; ++i) {
     for (int k = 0; k < j; ++k) out[k] += (double)V[k * ldv + i] * (double)w[i];
     out[j] += (double)w[i] * (double)w[i];
   }
-  return 1;
 }
 template <typename T>
-unsigned update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h, double *,
-                     double *out, hipStream_t, bool)
+void subtract_now(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h)
 {
   std::vector<double> hh(h, h + j);
   for (int64_t i = 0; i < n; ++i) {
@@ -73,7 +35,29 @@ unsigned update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const doub
     for (int k = 0; k < j; ++k) wi -= hh[k] * (double)V[k * ldv + i];
     w[i] = (T)wi;
   }
-  return multidot(V, ldv, j, w, n, nullptr, out, nullptr, true);
+}
+}  // namespace
+
+template <typename T>
+unsigned multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double *partial,
+                  double *out, hipStream_t s, bool reduce)
+{
+  fake_enqueue(s, [=] {
+    dots_now(V, ldv, j, w, n, partial);
+    if (reduce) std::memcpy(out, partial, sizeof(double) * (j + 1));
+  });
+  return 1;
+}
+template <typename T>
+unsigned update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
+                     double *partial, double *out, hipStream_t s, bool reduce)
+{
+  fake_enqueue(s, [=] {
+    subtract_now(V, ldv, j, w, n, h);
+    dots_now(V, ldv, j, w, n, partial);
+    if (reduce) std::memcpy(out, partial, sizeof(double) * (j + 1));
+  });
+  return 1;
 }
 template <typename T>
 unsigned update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
@@ -82,139 +66,189 @@ unsigned update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, co
   return update_dots(V, ldv, j, w, n, h, p, out, st, r);
 }
 template <typename T>
-void update_norm(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h, double *, double *out, hipStream_t)
+void update_norm(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h,
+                 double *partial, double *out, hipStream_t s)
 {
-  std::vector<double> hh(h, h + j);
-  for (int64_t i = 0; i < n; ++i) {
-    double wi = w[i];
-    for (int k = 0; k < j; ++k) wi -= hh[k] * (double)V[k * ldv + i];
-    w[i] = (T)wi;
-  }
-  out[0] = 0;
-  for (int64_t i = 0; i < n; ++i) out[0] += (double)w[i] * (double)w[i];
+  fake_enqueue(s, [=] {
+    subtract_now(V, ldv, j, w, n, h);
+    double t = 0;
+    for (int64_t i = 0; i < n; ++i) t += (double)w[i] * (double)w[i];
+    partial[0] = t;
+    out[0] = partial[0];
+  });
 }
-// The sweeps above already leave the full sums in out (nothing to reduce):
-// these run only the step-closing arithmetic of the device kernels.
-void reduce_step_close(const double *, unsigned, int, double *out, int j, const double *hb1,
-                       double *Hd, int ldh, double *nrm, double *flags, hipStream_t)
+// The step-closing arithmetic of the device reductions (k_reduce_epi):
+// sums from the partial buffer into out, then the close.
+void reduce_step_close(const double *partial, unsigned, int cnt, double *out, int j,
+                       const double *hb1, double *Hd, int ldh, double *nrm, double *flags,
+                       hipStream_t s)
 {
-  const double *hb2 = out;
-  const int nk = j + 1;
-  const double w0 = hb1[nk], w1 = hb2[nk];
-  const bool re = !(w1 > 0.5 * w0);
-  double hn2 = 0, h22 = 0;
-  for (int k = 0; k < nk; ++k) {
-    const double hk = hb1[k] + (re ? hb2[k] : 0.0);
-    Hd[k + (int64_t)j * ldh] = hk;
-    hn2 += hk * hk;
-    h22 += hb2[k] * hb2[k];
-  }
-  const double beta2 = re ? w1 - h22 : w1;
-  const double beta = std::sqrt(std::fmax(beta2, 0.0));
-  Hd[nk + (int64_t)j * ldh] = beta;
-  nrm[0] = beta2;
-  if (re) flags[0] += 1.0;
-  const bool unsafe = re && !(h22 <= 0.5 * w1);
-  if ((unsafe || beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
+  fake_enqueue(s, [=] {
+    std::memcpy(out, partial, sizeof(double) * cnt);
+    const double *hb2 = out;
+    const int nk = j + 1;
+    const double w0 = hb1[nk], w1 = hb2[nk];
+    const bool re = !(w1 > 0.5 * w0);
+    double hn2 = 0, h22 = 0;
+    for (int k = 0; k < nk; ++k) {
+      const double hk = hb1[k] + (re ? hb2[k] : 0.0);
+      Hd[k + (int64_t)j * ldh] = hk;
+      hn2 += hk * hk;
+      h22 += hb2[k] * hb2[k];
+    }
+    const double beta2 = re ? w1 - h22 : w1;
+    const double beta = std::sqrt(std::fmax(beta2, 0.0));
+    Hd[nk + (int64_t)j * ldh] = beta;
+    nrm[0] = beta2;
+    if (re) flags[0] += 1.0;
+    const bool unsafe = re && !(h22 <= 0.5 * w1);
+    if ((unsafe || beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
+  });
 }
-void reduce_lanczos_coeffs(const double *, unsigned, int, double *out, int j, double *Hd, int ldh,
-                           double *hL, hipStream_t)
+void reduce_lanczos_coeffs(const double *partial, unsigned, int cnt, double *out, int j,
+                           double *Hd, int ldh, double *hL, hipStream_t s)
 {
-  for (int k = 0; k <= j; ++k) hL[k] = k == j ? out[0] : (k == j - 1 ? Hd[j + (int64_t)(j - 1) * ldh] : 0.0);
+  fake_enqueue(s, [=] {
+    std::memcpy(out, partial, sizeof(double) * cnt);
+    for (int k = 0; k <= j; ++k)
+      hL[k] = k == j ? out[0] : (k == j - 1 ? Hd[j + (int64_t)(j - 1) * ldh] : 0.0);
+  });
 }
-void reduce_step_close_lanczos(const double *, unsigned, int, double *out, int j, const double *hL,
-                               double *Hd, int ldh, double *nrm, double *flags, hipStream_t)
+void reduce_step_close_lanczos(const double *partial, unsigned, int cnt, double *out, int j,
+                               const double *hL, double *Hd, int ldh, double *nrm, double *flags,
+                               hipStream_t s)
 {
-  const double *hb2 = out;
-  const int nk = j + 1;
-  const double w1 = hb2[nk];
-  double hn2 = 0, h22 = 0;
-  for (int k = 0; k < nk; ++k) {
-    const double hk = hL[k] + hb2[k];
-    Hd[k + (int64_t)j * ldh] = hk;
-    hn2 += hk * hk;
-    h22 += hb2[k] * hb2[k];
-  }
-  const double beta2 = w1 - h22;
-  const double beta = std::sqrt(std::fmax(beta2, 0.0));
-  Hd[nk + (int64_t)j * ldh] = beta;
-  nrm[0] = beta2;
-  flags[0] += 1.0;
-  const bool unsafe = !(h22 <= 0.5 * w1);
-  if ((unsafe || beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
+  fake_enqueue(s, [=] {
+    std::memcpy(out, partial, sizeof(double) * cnt);
+    const double *hb2 = out;
+    const int nk = j + 1;
+    const double w1 = hb2[nk];
+    double hn2 = 0, h22 = 0;
+    for (int k = 0; k < nk; ++k) {
+      const double hk = hL[k] + hb2[k];
+      Hd[k + (int64_t)j * ldh] = hk;
+      hn2 += hk * hk;
+      h22 += hb2[k] * hb2[k];
+    }
+    const double beta2 = w1 - h22;
+    const double beta = std::sqrt(std::fmax(beta2, 0.0));
+    Hd[nk + (int64_t)j * ldh] = beta;
+    nrm[0] = beta2;
+    flags[0] += 1.0;
+    const bool unsafe = !(h22 <= 0.5 * w1);
+    if ((unsafe || beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
+  });
 }
 template <typename T>
 void update_normalize(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h2,
-                      const double *w0, const double *w1, const double *nrm2, int force, hipStream_t)
+                      const double *w0, const double *w1, const double *nrm2, int force,
+                      hipStream_t s)
 {
-  const bool re = force || !(w1[0] > 0.5 * w0[0]);
-  std::vector<double> hh(h2, h2 + j);
-  const double inv = 1.0 / std::sqrt(nrm2[0]);
-  for (int64_t i = 0; i < n; ++i) {
-    double wi = w[i];
-    if (re)
-      for (int k = 0; k < j; ++k) wi -= hh[k] * (double)V[k * ldv + i];
-    w[i] = (T)(wi * inv);
-  }
+  fake_enqueue(s, [=] {
+    const bool re = force || !(w1[0] > 0.5 * w0[0]);
+    std::vector<double> hh(h2, h2 + j);
+    const double inv = 1.0 / std::sqrt(nrm2[0]);
+    for (int64_t i = 0; i < n; ++i) {
+      double wi = w[i];
+      if (re)
+        for (int k = 0; k < j; ++k) wi -= hh[k] * (double)V[k * ldv + i];
+      w[i] = (T)(wi * inv);
+    }
+  });
 }
 template <typename T>
-void normalize(T *w, int64_t n, const double *nrm2, hipStream_t)
+void normalize(T *w, int64_t n, const double *nrm2, hipStream_t s)
 {
-  const double inv = 1.0 / std::sqrt(nrm2[0]);
-  for (int64_t i = 0; i < n; ++i) w[i] = (T)((double)w[i] * inv);
+  fake_enqueue(s, [=] {
+    const double inv = 1.0 / std::sqrt(nrm2[0]);
+    for (int64_t i = 0; i < n; ++i) w[i] = (T)((double)w[i] * inv);
+  });
 }
 template <typename Ti, typename To>
-void gemm_vy(const Ti *Vin, int64_t ldv, int m, const double *Y, int ldy, int k, To *Vout, int64_t ldo, int64_t n, hipStream_t)
+void gemm_vy(const Ti *Vin, int64_t ldv, int m, const double *Y, int ldy, int k, To *Vout,
+             int64_t ldo, int64_t n, hipStream_t s)
 {
-  std::vector<double> row(m);
-  for (int64_t i = 0; i < n; ++i) {
-    for (int r = 0; r < m; ++r) row[r] = Vin[r * ldv + i];
-    for (int c = 0; c < k; ++c) {
-      double s = 0;
-      for (int r = 0; r < m; ++r) s += row[r] * Y[r + (int64_t)c * ldy];
-      Vout[c * ldo + i] = (To)s;
+  TM_CHECK(m <= 64 && k <= m, TM_EINVAL, "restart block too large");
+  fake_enqueue(s, [=] {
+    std::vector<double> row(m);
+    for (int64_t i = 0; i < n; ++i) {
+      for (int r = 0; r < m; ++r) row[r] = Vin[r * ldv + i];
+      for (int c = 0; c < k; ++c) {
+        double t = 0;
+        for (int r = 0; r < m; ++r) t += row[r] * Y[r + (int64_t)c * ldy];
+        Vout[c * ldo + i] = (To)t;
+      }
     }
-  }
+  });
 }
 template <typename T>
-void random_uniform(T *v, int64_t n, uint64_t seed, hipStream_t)
+void random_uniform(T *v, int64_t n, uint64_t seed, hipStream_t s)
 {
-  uint64_t x = seed * 0x9e3779b97f4a7c15ull + 1;
-  for (int64_t i = 0; i < n; ++i) {
-    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
-    v[i] = (T)(((double)(x >> 11) + 0.5) / 9007199254740992.0);
-  }
+  fake_enqueue(s, [=] {
+    uint64_t x = seed * 0x9e3779b97f4a7c15ull + 1;
+    for (int64_t i = 0; i < n; ++i) {
+      x ^= x << 13;
+      x ^= x >> 7;
+      x ^= x << 17;
+      v[i] = (T)(((double)(x >> 11) + 0.5) / 9007199254740992.0);
+    }
+  });
 }
 template <typename Ta, typename Tb>
-void copy(const Ta *a, Tb *b, int64_t n, hipStream_t) { for (int64_t i = 0; i < n; ++i) b[i] = (Tb)a[i]; }
-void to_f32(const double *a, float *b, int64_t n, hipStream_t) { for (int64_t i = 0; i < n; ++i) b[i] = (float)a[i]; }
-void fill(double *a, double v, int64_t n, hipStream_t) { for (int64_t i = 0; i < n; ++i) a[i] = v; }
-void pow2_scale(const double *, int64_t, double *, double *scale, hipStream_t) { scale[0] = 1.0; }
-void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double *, double *G, hipStream_t)
+void copy(const Ta *a, Tb *b, int64_t n, hipStream_t s)
 {
-  for (int r = 0; r < k; ++r)
-    for (int c = 0; c < k; ++c) {
-      double s = 0;
-      for (int64_t i = 0; i < n; ++i) s += A[r * ld + i] * B[c * ld + i];
-      G[r * k + c] = s;
-    }
+  fake_enqueue(s, [=] {
+    for (int64_t i = 0; i < n; ++i) b[i] = (Tb)a[i];
+  });
 }
-void gram_sym(const double *U, int64_t ld, int nv, int64_t n, double *p, double *G, hipStream_t st)
+void to_f32(const double *a, float *b, int64_t n, hipStream_t s) { copy(a, b, n, s); }
+void fill(double *a, double v, int64_t n, hipStream_t s)
+{
+  fake_enqueue(s, [=] {
+    for (int64_t i = 0; i < n; ++i) a[i] = v;
+  });
+}
+void pow2_scale(const double *x, int64_t n, double *, double *scale, hipStream_t s)
+{
+  fake_enqueue(s, [=] {
+    double m = 0;
+    for (int64_t i = 0; i < n; ++i) m = std::fmax(m, std::fabs(x[i]));
+    int e = 0;
+    if (m > 0) std::frexp(m, &e);
+    scale[0] = m > 0 ? std::ldexp(1.0, -e) : 1.0;
+  });
+}
+void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double *,
+          double *G, hipStream_t s)
+{
+  fake_enqueue(s, [=] {
+    for (int r = 0; r < k; ++r)
+      for (int c = 0; c < k; ++c) {
+        double t = 0;
+        for (int64_t i = 0; i < n; ++i) t += A[r * ld + i] * B[c * ld + i];
+        G[r * k + c] = t;
+      }
+  });
+}
+void gram_sym(const double *U, int64_t ld, int nv, int64_t n, double *p, double *G,
+              hipStream_t st)
 {
   gram(U, U, ld, nv, n, p, G, st);
 }
-void combine_norms(const double *U, int64_t ld, int nu, const double *Cm, int ldc, int nc, int64_t n,
-                   double *, double *out, hipStream_t)
+void combine_norms(const double *U, int64_t ld, int nu, const double *Cm, int ldc, int nc,
+                   int64_t n, double *, double *out, hipStream_t s)
 {
-  for (int c = 0; c < nc; ++c) {
-    double s = 0;
-    for (int64_t i = 0; i < n; ++i) {
-      double x = 0;
-      for (int r = 0; r < nu; ++r) x += Cm[r + (int64_t)c * ldc] * U[r * ld + i];
-      s += x * x;
+  fake_enqueue(s, [=] {
+    for (int c = 0; c < nc; ++c) {
+      double t = 0;
+      for (int64_t i = 0; i < n; ++i) {
+        double x = 0;
+        for (int r = 0; r < nu; ++r) x += Cm[r + (int64_t)c * ldc] * U[r * ld + i];
+        t += x * x;
+      }
+      out[c] = t;
     }
-    out[c] = s;
-  }
+  });
 }
 #define FAKE_KRY_INST(T)                                                                        \
   template unsigned multidot<T>(const T *, int64_t, int, const T *, int64_t, double *,         \
@@ -244,33 +278,75 @@ template void copy<float, float>(const float *, float *, int64_t, hipStream_t);
 template void copy<double, float>(const double *, float *, int64_t, hipStream_t);
 template void copy<float, double>(const float *, double *, int64_t, hipStream_t);
 }  // namespace kry
+
+// The C ABI's device-only paths have no CPU model.
+tm_op *make_op(const tm_params &) { throw Error(TM_EUNSUPPORTED, "no operators in the CPU model"); }
+void grad_numerators(tm_op *, const double *, const double *, double *)
+{
+  throw Error(TM_EUNSUPPORTED, "no gradients in the CPU model");
+}
+void dlnnf(const tm_op *, double *) {}
+void phys_sums(int, const double *, const double *, const uint64_t *, int64_t, int, int, void *,
+               double *, hipStream_t)
+{
+  throw Error(TM_EUNSUPPORTED, "no observables in the CPU model");
+}
+size_t phys_work_bytes() { return 0; }
+void sample_chains(tm_op *, const double *, const double *, const double *, int, int, uint64_t *,
+                   int *)
+{
+  throw Error(TM_EUNSUPPORTED, "no sampler in the CPU model");
+}
 }  // namespace ltm
 
-// dense operator: op->tab64 holds the n x n column-major matrix
-tm_op::~tm_op() = default;
-void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp, int transpose,
-                      const double *sc, hipStream_t s)
+// Dense operator: op->tab64 holds the n x n column-major matrix.  Like the
+// device operators, an apply stages its input through the operator's
+// workspace op->work (so two applies of one operator on different streams
+// share it, as the device operators share theirs).
+// Fault injection for the CPU tests: from this many matvecs of an operator
+// on, its applies return a perturbed matrix's product (-1: never).
+extern "C" {
+int g_fake_perturb_after = -1;
+}
+
+tm_op::~tm_op()
+{
+  if (stream) (void)hipStreamDestroy(stream);
+}
+void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
+                      int transpose, const double *sc, hipStream_t s)
 {
   apply_internal(x, y, ld, nvec, io, comp, transpose, sc, s);
 }
-void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp, int transpose,
-                      const double *, hipStream_t)
+void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype io,
+                           tm_dtype comp, int transpose, const double *sc, hipStream_t s)
 {
   const double *A = tab64.as<double>();
   const int64_t n = ncol;
-  std::vector<double> xv(n);
+  work.ensure(sizeof(double) * n);
+  double *xv = work.as<double>();
   for (int v = 0; v < nvec; ++v) {
-    for (int64_t j = 0; j < n; ++j)
-      xv[j] = io == TM_F32 ? (double)static_cast<const float *>(x)[v * ld + j]
-                           : static_cast<const double *>(x)[v * ld + j];
-    for (int64_t i = 0; i < n; ++i) {
-      double s = 0;
-      for (int64_t j = 0; j < n; ++j) s += (transpose ? A[j + i * n] : A[i + j * n]) * xv[j];
-      // FP32 compute: round the result to single precision
-      if (comp == TM_F32) s = (double)(float)s;
-      if (io == TM_F32) static_cast<float *>(y)[v * ld + i] = (float)s;
-      else static_cast<double *>(y)[v * ld + i] = s;
-    }
+    const bool perturb = g_fake_perturb_after >= 0 && matvecs >= g_fake_perturb_after;
+    fake_enqueue(s, [=] {
+      // stage x (times the input scale) in the workspace
+      const double si = sc ? sc[0] : 1.0;
+      for (int64_t j = 0; j < n; ++j)
+        xv[j] = si * (io == TM_F32 ? (double)static_cast<const float *>(x)[v * ld + j]
+                                   : static_cast<const double *>(x)[v * ld + j]);
+    });
+    fake_enqueue(s, [=] {
+      const double so = sc ? 1.0 / sc[0] : 1.0;
+      for (int64_t i = 0; i < n; ++i) {
+        double t = 0;
+        for (int64_t j = 0; j < n; ++j) t += (transpose ? A[j + i * n] : A[i + j * n]) * xv[j];
+        if (perturb) t += 0.25 * xv[(i + 1) % n];
+        // FP32 compute: round the result to single precision
+        if (comp == TM_F32) t = (double)(float)t;
+        t *= so;
+        if (io == TM_F32) static_cast<float *>(y)[v * ld + i] = (float)t;
+        else static_cast<double *>(y)[v * ld + i] = t;
+      }
+    });
     ++matvecs;
   }
 }

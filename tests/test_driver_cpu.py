@@ -18,7 +18,7 @@ EXE = os.path.join(NATIVE, "driver_test")
 
 @pytest.fixture(scope="module")
 def driver():
-    subprocess.run(["make", "-s", "-C", NATIVE], check=True, capture_output=True)
+    subprocess.run(["make", "-s", "-j3", "-C", NATIVE], check=True, capture_output=True)
     return EXE
 
 
