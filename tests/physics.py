@@ -48,6 +48,20 @@ def xat_dense(L, bJ, k, h):
     return T
 
 
+def xat_dense_open(L, bJ, k, h):
+    """The open-chain ANNNI perpendicular TM (without the normfactor): L-1
+    nearest-neighbour and L-2 next-nearest-neighbour bonds in the layer."""
+    N = 1 << L
+    m1, m2 = (N - 1) >> 1, (N - 1) >> 2
+    e = np.array([bJ * pairsum((s & m1) ^ (s >> 1), L - 1)
+                  - k * bJ * pairsum((s & m2) ^ (s >> 2), L - 2)
+                  + h * bJ * pairsum(s, L) for s in range(N)])
+    pcs = np.array([pc(t) for t in range(N)])
+    idx = np.arange(N)
+    cross = bJ * (L - 2 * pcs[idx[:, None] ^ idx[None, :]])
+    return np.exp(0.5 * e[:, None] + 0.5 * e[None, :] + cross)
+
+
 def xdt_dense(L, bJ, k, h):
     """Symmetrised DNNI TM: NN in-layer + field on the layer, NN and the two
     diagonal NNN (-kappa) across layers."""

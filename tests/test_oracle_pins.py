@@ -54,14 +54,15 @@ def test_3d_ops(oracle):
     for s in rng.integers(0, 1 << 9, 100):
         s = int(s)
         g = [[(s >> (r * 3 + c)) & 1 for c in range(3)] for r in range(3)]
-        rot = [[g[2 - c][r] for c in range(3)] for r in range(3)]  # 90-degree rotation
+        # the quarter turn: site (r, c) moves to (c, size-1-r)
+        rot = [[g[2 - c][r] for c in range(3)] for r in range(3)]
         got = L.or_rotate3d(s, 3)
         gg = [[(got >> (r * 3 + c)) & 1 for c in range(3)] for r in range(3)]
-        # a rotation of order 4 whose square is the point reflection
+        assert gg == rot
+        # of order 4, its square the point reflection
         g2 = L.or_rotate3d(got, 3)
         assert g2 == int(format(s, "09b")[::-1], 2)
-        assert L.or_rotate3d(g2, 3) != s or s in (0, 511) or True
-        assert sum(map(sum, gg)) == sum(map(sum, g))
+        assert L.or_rotate3d(L.or_rotate3d(g2, 3), 3) == s
 
 
 @pytest.mark.parametrize("L,bJ,k,h", [(6, 0.5, 0.6, 0.0), (8, 0.3, 0.2, 0.1), (7, 0.8, 0.0, -0.2)])
@@ -96,6 +97,51 @@ def test_zbt_dense_hamiltonian_and_adjoint(oracle, L, bJ, k, k2, h):
     w = w[np.argsort(-np.abs(w))]
     assert abs(w[0].imag) < 1e-12 * abs(w[0]) and abs(w[0]) > abs(w[1])
     assert np.max(np.abs(T - T.T)) > 1e-3 * np.max(np.abs(T))
+
+
+@pytest.mark.parametrize("L,bJ,k,h", [(6, 0.5, 0.6, 0.0), (7, 0.3, 0.2, 0.1)])
+def test_xat_open_dense_hamiltonian(oracle, L, bJ, k, h):
+    # bc 1: the open chain's L-1 and L-2 bonds (TMat/tmat.hpp:233-247)
+    op = oracle.OracleOp(oracle.XAT, L, 0, 1, bJ, k, 0.0, h)
+    D = physics.xat_dense_open(L, bJ, k, h) * op.normfactor
+    T = op.dense()
+    assert np.max(np.abs(T - D)) / np.max(np.abs(D)) < 1e-14
+
+
+@pytest.mark.parametrize("s1,s2,k,h", [(3, 3, 0.4, 0.0), (3, 3, 0.0, 0.0), (3, 3, 0.4, 0.1),
+                                        (3, 4, 0.4, 0.0), (3, 4, 0.0, 0.05)])
+def test_xam3d_projection(oracle, s1, s2, k, h):
+    """3D reduced operator (TMat/tmat3d.cpp:102-201) against the projection
+    of the dense 3D Hamiltonian matrix onto orbit sums of its group: row and
+    column translations and mirrors, the flip when h = 0, and the quarter
+    turn when kappa = 0 on a square layer."""
+    bJ = 0.25
+    red = oracle.OracleOp(oracle.XAM3D, s1, s2, 0, bJ, k, 0.0, h)
+    L = s1 * s2
+    N = 1 << L
+    e = np.array([physics.layer3d_energy(t, s1, s2, bJ, k, h) for t in range(N)])
+    pcs = np.array([bin(t).count("1") for t in range(N)])
+    idx = np.arange(N)
+    cross = bJ * (L - 2 * pcs[idx[:, None] ^ idx[None, :]])
+    T = np.exp(0.5 * e[:, None] + 0.5 * e[None, :] + cross) * red.normfactor
+    censym = h == 0
+    rotate = k == 0 and s1 == s2
+    G = groups.closure(groups.gens_3d(s1, s2, rotate))
+    states = np.arange(N, dtype=np.uint64)
+    imgs = [groups._apply(states, g) for g in G]
+    if censym:
+        imgs += [~m & np.uint64(N - 1) for m in imgs]
+    orbit_min = np.min(np.stack(imgs), axis=0)
+    reps, dups = red.reps(), red.dups()
+    np.testing.assert_array_equal(reps, np.unique(orbit_min))
+    orb_of = np.searchsorted(reps, orbit_min)
+    np.testing.assert_array_equal(dups, np.bincount(orb_of))
+    P = np.zeros((N, len(reps)))
+    P[idx, orb_of] = 1.0
+    d = dups.astype(np.float64)
+    Tr = (P.T @ T @ P) / np.sqrt(np.outer(d, d))
+    Ro = red.dense()
+    assert np.max(np.abs(Ro - Tr)) / np.max(np.abs(Tr)) < 1e-13
 
 
 @pytest.mark.parametrize("s1,s2", [(2, 3), (3, 3), (2, 4)])
@@ -233,19 +279,24 @@ def test_orbit_count_3d_5x5_order(oracle):
 def test_xam_projection(oracle, L, ext):
     """T_red(o,o') = (d_o d_o')^-1/2 sum_{s in o, s' in o'} T(s,s') built from
     the dense Hamiltonian, against the oracle's reduced operator."""
-    h = 0.0 if ext != 1 else 0.0
+    h = 0.0
     bJ, k = 0.45, 0.55
     red = oracle.OracleOp(oracle.XAM2D, L, 0, ext, bJ, k, 0.0, h)
     bc = ext & 1
     if bc:
-        # open-boundary reduced energies use L for every bond count
-        # (TMat/tmatreduce.cpp:401); compare spectra with the full open TM
-        return
-    T = physics.xat_dense(L, bJ, k, h) * red.normfactor
+        # open boundary: the reduced operator's energies subtract L for every
+        # bond count (TMat/tmatreduce.cpp:388-399) where the physical chain
+        # has L-1 nearest- and L-2 next-nearest-neighbour bonds, so each half
+        # diagonal carries exp(bJ/2 - bJ kappa) more than the Hamiltonian's:
+        # the whole matrix is nf exp(bJ (1 - 2 kappa)) times the physical one
+        T = physics.xat_dense_open(L, bJ, k, h) * red.normfactor * np.exp(bJ * (1 - 2 * k))
+    else:
+        T = physics.xat_dense(L, bJ, k, h) * red.normfactor
     reps, dups = red.reps(), red.dups()
     censym = (h == 0 and (ext & 2) == 0)
-    cr, cd = physics.orbit_closure_2d(L, 0, censym)
+    cr, cd = physics.orbit_closure_2d(L, bc, censym)
     np.testing.assert_array_equal(reps, cr)
+    np.testing.assert_array_equal(dups, cd)
     N = 1 << L
     mask = N - 1
     orb_of = np.empty(N, int)
@@ -259,7 +310,8 @@ def test_xam_projection(oracle, L, ext):
             if t in orb:
                 continue
             orb.add(t)
-            fr += [rev(t), physics.rotr(t, L, 1)] + ([~t & mask] if censym else [])
+            fr += [rev(t)] + ([] if bc else [physics.rotr(t, L, 1)]) + \
+                ([~t & mask] if censym else [])
         for t in orb:
             orb_of[t] = o
     M = len(reps)
