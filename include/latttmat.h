@@ -119,6 +119,23 @@ tm_status tm_op_apply_dev(const tm_op *op, const void *x, void *y, int64_t ld,
                           int nvec, tm_dtype dtype, tm_dtype compute,
                           int transpose, void *stream);
 
+/* tm_op_apply_dev for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, kernels and results as
+ * tm_op_apply_dev (with the tile autotuner off: the default tiles); it calls
+ * no hipMalloc, hipFree or allocating library routine.  Every buffer beyond
+ * x and y (the FP32 input scaling's partial sums and scales, the reduced
+ * operators' permutation buffers, the operator workspace) comes from a
+ * file-scope device array of 32 MiB that each GPU holds until the process
+ * ends; an apply that needs more returns TM_EINVAL and names the size.  The
+ * array's device address is looked up on the first call on each device (with
+ * that device current).  Calls on one GPU share the array: calls that overlap
+ * on one GPU race (serialise them, e.g. on one stream).  The apply is not
+ * ordered against the operator's own stream (tm_op_apply, tm_solve). */
+tm_status tm_op_apply_dev_noalloc(const tm_op *op, const void *x, void *y, int64_t ld,
+                                  int nvec, tm_dtype dtype, tm_dtype compute,
+                                  int transpose, void *stream);
+
 /* Host-pointer apply with a chosen compute precision (TM_F32 = the
  * mixed-precision T.x with a power-of-two input scale).  No reference
  * counterpart; used by the parity tests. */

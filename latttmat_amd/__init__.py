@@ -13,6 +13,12 @@ solve mirrors ``pfunc_inner`` (TMat/pfunc.hpp:79-202).
 """
 from ._abi import (  # noqa: F401
     LIB_PATH,
+    TM_EHIP,
+    TM_EINVAL,
+    TM_ENOCONV,
+    TM_ENOMEM,
+    TM_EUNSUPPORTED,
+    TM_OK,
     TM_F32,
     TM_F64,
     TM_FLAG_FULLSPACE,
@@ -30,7 +36,7 @@ from ._abi import (  # noqa: F401
 )
 
 __all__ = [
-    "LIB_PATH", "TM_F32", "TM_F64", "TM_FLAG_FULLSPACE", "TM_FLAG_GENERIC_REDUCED", "TM_FLAG_NO_AUTOTUNE",
+    "LIB_PATH", "TM_OK", "TM_EINVAL", "TM_ENOMEM", "TM_EHIP", "TM_ENOCONV", "TM_EUNSUPPORTED", "TM_F32", "TM_F64", "TM_FLAG_FULLSPACE", "TM_FLAG_GENERIC_REDUCED", "TM_FLAG_NO_AUTOTUNE",
     "TM_PREC_F64", "TM_PREC_MIXED", "TMError", "TransferMatrix", "SolveResult",
     "exported_symbols", "lib", "solve", "solve_batch",
 ]

@@ -23,7 +23,8 @@ KIND_NAMES = {1: "XAT", 2: "XDT", 3: "ZBT", 4: "XAM2D", 5: "XAM3D", 6: "XAT3D", 
 EXPORTS = [
     "tm_op_create", "tm_op_destroy", "tm_op_rows", "tm_op_cols", "tm_op_normfactor",
     "tm_op_is_symmetric", "tm_op_kind", "tm_op_suggested_ncv", "tm_op_layer_size",
-    "tm_op_matvec_count", "tm_op_apply", "tm_op_apply_dev", "tm_op_apply_prec",
+    "tm_op_matvec_count", "tm_op_apply", "tm_op_apply_dev", "tm_op_apply_dev_noalloc",
+    "tm_op_apply_prec",
     "tm_op_time_apply", "tm_op_orbits", "tm_op_moments", "tm_sample_configs",
     "tm_solve_opts_default", "tm_solve", "tm_solve_batch", "tm_grad_hf", "tm_last_error",
     "tm_version",
@@ -102,6 +103,8 @@ def lib():
         L.tm_op_apply.argtypes = [vp, pd, pd, i32]
         L.tm_op_apply_dev.restype = i32
         L.tm_op_apply_dev.argtypes = [vp, vp, vp, i64, i32, i32, i32, i32, vp]
+        L.tm_op_apply_dev_noalloc.restype = i32
+        L.tm_op_apply_dev_noalloc.argtypes = [vp, vp, vp, i64, i32, i32, i32, i32, vp]
         L.tm_op_apply_prec.restype = i32
         L.tm_op_apply_prec.argtypes = [vp, pd, pd, i32, i32]
         L.tm_op_time_apply.restype = i32
@@ -235,6 +238,12 @@ class TransferMatrix:
         _check(lib().tm_op_apply_dev(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), ld, nvec,
                                      dtype, compute, 1 if transpose else 0,
                                      C.c_void_p(stream) if stream else None))
+
+    def apply_dev_noalloc(self, x_ptr, y_ptr, ld, nvec=1, dtype=TM_F64, compute=TM_F64,
+                          transpose=False, stream=None):
+        _check(lib().tm_op_apply_dev_noalloc(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), ld,
+                                             nvec, dtype, compute, 1 if transpose else 0,
+                                             C.c_void_p(stream) if stream else None))
 
     def orbits(self):
         n = self.cols()

@@ -30,7 +30,7 @@ COMMON = [
 LIB_SOURCES = [
     "tm_api.cpp", "tm_runtime.cpp", "tm_ops.cpp", "tm_solver.cpp", "tm_dense.cpp", "tm_grad.hip",
     "tm_kron.hip", "tm_reduced.hip", "tm_sweep.hip", "tm_krylov.hip", "tm_axial.hip",
-    "tm_phys.hip", "tm_sample.hip",
+    "tm_phys.hip", "tm_sample.hip", "tm_noalloc.hip",
 ]
 
 

@@ -130,6 +130,20 @@ tm_status tm_op_apply_dev(const tm_op *cop, const void *x, void *y, int64_t ld, 
   });
 }
 
+tm_status tm_op_apply_dev_noalloc(const tm_op *cop, const void *x, void *y, int64_t ld, int nvec,
+                                  tm_dtype dtype, tm_dtype compute, int transpose, void *stream)
+{
+  if (!cop || !x || !y || nvec < 1 || ld < cop->ncol) return fail(TM_EINVAL, "bad argument");
+  if ((dtype != TM_F64 && dtype != TM_F32) || (compute != TM_F64 && compute != TM_F32))
+    return fail(TM_EINVAL, "bad dtype");
+  tm_op *op = const_cast<tm_op *>(cop);
+  return guarded([&] {
+    ltm::apply_dev_noalloc(op, x, y, ld, nvec, dtype, compute, transpose,
+                           static_cast<hipStream_t>(stream));
+    return TM_OK;
+  });
+}
+
 tm_status tm_op_apply_prec(const tm_op *cop, const double *x, double *y, int transpose,
                            tm_dtype compute)
 {

@@ -469,6 +469,16 @@ void gen_eig(int n, const double *A, double *wr, double *wi, double *Zr, double 
 
 }  // namespace ltm
 
+namespace ltm {
+// Caller-provided device memory for an apply that must not allocate.
+struct Workspace {
+  void *work = nullptr;  // the operator workspace
+  size_t work_bytes = 0;
+  void *px = nullptr, *py = nullptr;  // permutation buffers (perm_bytes each)
+  size_t perm_bytes = 0;
+};
+}  // namespace ltm
+
 // ---------------------------------------------------------------------------
 // the operator object behind tm_op*
 // ---------------------------------------------------------------------------
@@ -509,12 +519,23 @@ struct tm_op {
   virtual ~tm_op();
   // gate-circuit plan of XDT/ZBT (or of the circuit a reduced XDM/ZBM wraps)
   ltm::SweepPlan sweep_plan() const;
-  // T.x on vectors in the internal order (the eigensolver's basis)
+  // Device workspace bytes of one apply with compute type comp (the
+  // operator's `work`), and of the two permutation buffers of an apply in
+  // the reference order (0 when the operator has no internal order).
+  size_t work_bytes(tm_dtype comp) const;
+  size_t perm_bytes(tm_dtype io) const;
+  // T.x on vectors in the internal order (the eigensolver's basis).  ws
+  // null: the operator's own workspace (grown on demand) and the tile
+  // autotuner; ws given: that memory (its size checked), no allocation, no
+  // tuning.
   void apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
-                      int transpose, const double *scale_in, hipStream_t s);
-  // T.x on vectors in the reference order (the API)
+                      int transpose, const double *scale_in, hipStream_t s,
+                      const ltm::Workspace *ws = nullptr);
+  // T.x on vectors in the reference order (the API); ws as above, its px /
+  // py used for the permutation
   void apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
-                 int transpose, const double *scale_in, hipStream_t s);
+                 int transpose, const double *scale_in, hipStream_t s,
+                 const ltm::Workspace *ws = nullptr);
 };
 
 namespace ltm {
@@ -524,6 +545,10 @@ inline bool is_reduced(tm_kind k)
   return k == TM_KIND_XAM2D || k == TM_KIND_XAM3D || k == TM_KIND_XDM || k == TM_KIND_ZBM ||
          k == TM_KIND_YAM || k == TM_KIND_YBM;
 }
+// tm_op_apply_dev on the no-allocation entry point's scratch array
+// (tm_noalloc.hip); throws Error.
+void apply_dev_noalloc(tm_op *op, const void *x, void *y, int64_t ld, int nvec, tm_dtype dtype,
+                       tm_dtype compute, int transpose, hipStream_t s);
 // Construct an operator from parameters (pfunc dispatch); throws Error.
 tm_op *make_op(const tm_params &p);
 // Hellmann-Feynman numerators: num[i] = <l, dT/dtheta_i r> for theta =

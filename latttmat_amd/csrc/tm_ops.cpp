@@ -426,25 +426,40 @@ tm_op::~tm_op()
 }
 
 void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
-                      int transpose, const double *scale_in, hipStream_t s)
+                      int transpose, const double *scale_in, hipStream_t s,
+                      const ltm::Workspace *ws)
 {
   using namespace ltm;
   if (perm.empty()) {
-    apply_internal(x, y, ld, nvec, io, comp, transpose, scale_in, s);
+    apply_internal(x, y, ld, nvec, io, comp, transpose, scale_in, s, ws);
     return;
   }
   const size_t es = io == TM_F64 ? 8 : 4;
-  px.ensure(es * ncol);
-  py.ensure(es * ncol);
+  void *bx, *by;
+  if (ws) {
+    TM_CHECK(ws->px && ws->py && ws->perm_bytes >= perm_bytes(io), TM_EINVAL,
+             "workspace too small for the permutation buffers");
+    bx = ws->px;
+    by = ws->py;
+  } else {
+    px.ensure(perm_bytes(io));
+    py.ensure(perm_bytes(io));
+    bx = px.get();
+    by = py.get();
+  }
   const uint32_t *pd = ft->olist.as<uint32_t>();
   for (int v = 0; v < nvec; ++v) {
     const void *xv = static_cast<const char *>(x) + (size_t)v * ld * es;
     void *yv = static_cast<char *>(y) + (size_t)v * ld * es;
-    permute_to_internal(xv, px.get(), pd, ncol, io, s);
-    apply_internal(px.get(), py.get(), ncol, 1, io, comp, transpose,
-                   scale_in ? scale_in + v : nullptr, s);
-    permute_to_external(py.get(), yv, pd, ncol, io, s);
+    permute_to_internal(xv, bx, pd, ncol, io, s);
+    apply_internal(bx, by, ncol, 1, io, comp, transpose, scale_in ? scale_in + v : nullptr, s, ws);
+    permute_to_external(by, yv, pd, ncol, io, s);
   }
+}
+
+size_t tm_op::perm_bytes(tm_dtype io) const
+{
+  return perm.empty() ? 0 : (io == TM_F64 ? 8 : 4) * (size_t)ncol;
 }
 
 ltm::SweepPlan tm_op::sweep_plan() const
@@ -466,11 +481,77 @@ ltm::SweepPlan tm_op::sweep_plan() const
   return sp;
 }
 
+namespace {
+// The reduced-operator plan of op (sp receives the circuit plan of XDM / ZBM).
+ltm::ReducedPlan reduced_plan(const tm_op &op, int transpose, ltm::SweepPlan &sp)
+{
+  using namespace ltm;
+  ReducedPlan rp{};
+  rp.M = op.ncol;
+  rp.L = op.L;
+  rp.tvec = op.tvec.as<double>();
+  rp.dt64 = op.dt64.as<double>();
+  rp.tvec32 = op.tvec32.as<float>();
+  rp.dt32 = op.dt32.as<float>();
+  rp.reps = op.reps.as<uint64_t>();
+  rp.map = op.map.as<uint32_t>();
+  rp.ft = op.ft.get();
+  rp.ka = op.bzwj1[0];
+  rp.kb = op.bzwj1[1];
+  if (op.kind == TM_KIND_XDM || op.kind == TM_KIND_ZBM) {
+    sp = op.sweep_plan();
+    rp.sweep = &sp;
+    rp.transpose = op.kind == TM_KIND_ZBM ? transpose : 0;
+  }
+  if (op.kind == TM_KIND_YAM || op.kind == TM_KIND_YBM) {
+    rp.L = 2 * op.L;
+    rp.ylayer = op.L;
+    rp.ybm = op.kind == TM_KIND_YBM;
+    rp.ka = op.bzwj2[0];  // K2 on the previous layer (permute_mult_TL)
+    rp.kb = op.bzwj2[1];
+    rp.wtab64 = op.tab64.as<double>();
+    rp.wtab32 = op.tab32.as<float>();
+    rp.transpose = transpose;
+  }
+  return rp;
+}
+}  // namespace
+
+size_t tm_op::work_bytes(tm_dtype comp) const
+{
+  using namespace ltm;
+  switch (kind) {
+    case TM_KIND_XAT:
+    case TM_KIND_XAT3D: return kron_work_bytes(L, comp);
+    case TM_KIND_XDT:
+    case TM_KIND_ZBT: return sweep_work_bytes(kind, L, comp);
+    case TM_KIND_YAT:
+    case TM_KIND_Y2AT: return axial_work_bytes(L, comp);
+    default: {
+      SweepPlan sp{};
+      return reduced_work_bytes(reduced_plan(*this, 0, sp), comp);
+    }
+  }
+}
+
 void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype io,
-                           tm_dtype comp, int transpose, const double *scale_in, hipStream_t s)
+                           tm_dtype comp, int transpose, const double *scale_in, hipStream_t s,
+                           const ltm::Workspace *ws)
 {
   using namespace ltm;
   const size_t es = io == TM_F64 ? 8 : 4;
+  const bool tune = !(p.flags & TM_FLAG_NO_AUTOTUNE) && !ws;
+  // the workspace: the caller's (checked) or the operator's own
+  const size_t need = work_bytes(comp);
+  void *wk;
+  if (ws) {
+    TM_CHECK(ws->work_bytes >= need && (need == 0 || ws->work), TM_EINVAL,
+             "workspace too small for this operator");
+    wk = ws->work;
+  } else {
+    work.ensure(need);
+    wk = work.get();
+  }
   for (int v = 0; v < nvec; ++v) {
     const void *xv = static_cast<const char *>(x) + (size_t)v * ld * es;
     void *yv = static_cast<char *>(y) + (size_t)v * ld * es;
@@ -485,17 +566,15 @@ void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtyp
         kp.tab32 = tab32.as<float>();
         kp.ka = bzwj1[0];
         kp.kb = bzwj1[1];
-        kp.tune = !(p.flags & TM_FLAG_NO_AUTOTUNE);
-        work.ensure(kron_work_bytes(L, comp));
-        launch_kron(kp, xv, yv, io, io, comp, work.get(), scale_in, s);
+        kp.tune = tune;
+        launch_kron(kp, xv, yv, io, io, comp, wk, scale_in, s);
         break;
       }
       case TM_KIND_XDT:
       case TM_KIND_ZBT: {
-        const SweepPlan sp = sweep_plan();
-        work.ensure(sweep_work_bytes(kind, L, comp));
-        launch_sweep(sp, xv, yv, io, comp, kind == TM_KIND_ZBT ? transpose : 0, work.get(),
-                     scale_in, s);
+        SweepPlan sp = sweep_plan();
+        sp.tune = tune;
+        launch_sweep(sp, xv, yv, io, comp, kind == TM_KIND_ZBT ? transpose : 0, wk, scale_in, s);
         break;
       }
       case TM_KIND_YAT:
@@ -509,9 +588,8 @@ void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtyp
         ap.tab32 = tab32.as<float>();
         ap.tabb64 = tabb64.as<double>();
         ap.tabb32 = tabb32.as<float>();
-        ap.tune = !(p.flags & TM_FLAG_NO_AUTOTUNE);
-        work.ensure(axial_work_bytes(L, comp));
-        launch_axial(ap, xv, yv, io, comp, transpose, work.get(), scale_in, s);
+        ap.tune = tune;
+        launch_axial(ap, xv, yv, io, comp, transpose, wk, scale_in, s);
         break;
       }
       case TM_KIND_XAM2D:
@@ -520,37 +598,11 @@ void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtyp
       case TM_KIND_ZBM:
       case TM_KIND_YAM:
       case TM_KIND_YBM: {
-        ReducedPlan rp{};
-        rp.M = ncol;
-        rp.L = L;
-        rp.tvec = tvec.as<double>();
-        rp.dt64 = dt64.as<double>();
-        rp.tvec32 = tvec32.as<float>();
-        rp.dt32 = dt32.as<float>();
-        rp.reps = reps.as<uint64_t>();
-        rp.map = map.as<uint32_t>();
-        rp.ft = ft.get();
-        rp.ka = bzwj1[0];
-        rp.kb = bzwj1[1];
         SweepPlan sp{};
-        if (kind == TM_KIND_XDM || kind == TM_KIND_ZBM) {
-          sp = sweep_plan();
-          rp.sweep = &sp;
-          rp.transpose = kind == TM_KIND_ZBM ? transpose : 0;
-        }
-        if (kind == TM_KIND_YAM || kind == TM_KIND_YBM) {
-          rp.L = 2 * L;
-          rp.ylayer = L;
-          rp.ybm = kind == TM_KIND_YBM;
-          rp.ka = bzwj2[0];  // K2 on the previous layer (permute_mult_TL)
-          rp.kb = bzwj2[1];
-          rp.wtab64 = tab64.as<double>();
-          rp.wtab32 = tab32.as<float>();
-          rp.transpose = transpose;
-        }
+        ReducedPlan rp = reduced_plan(*this, transpose, sp);
+        sp.tune = tune;
         if (!rp.ft) TM_CHECK(rp.map != nullptr, TM_EINVAL, "reduced operator has no state map");
-        work.ensure(reduced_work_bytes(rp, comp));
-        launch_reduced(rp, xv, yv, io, comp, work.get(), scale_in, s);
+        launch_reduced(rp, xv, yv, io, comp, wk, scale_in, s);
         break;
       }
       default:

@@ -292,6 +292,11 @@ void phys_sums(int, const double *, const double *, const uint64_t *, int64_t, i
   throw Error(TM_EUNSUPPORTED, "no observables in the CPU model");
 }
 size_t phys_work_bytes() { return 0; }
+void apply_dev_noalloc(tm_op *, const void *, void *, int64_t, int, tm_dtype, tm_dtype, int,
+                       hipStream_t)
+{
+  throw Error(TM_EUNSUPPORTED, "no scratch array in the CPU model");
+}
 void sample_chains(tm_op *, const double *, const double *, const double *, int, int, uint64_t *,
                    int *)
 {
@@ -314,17 +319,26 @@ tm_op::~tm_op()
   if (stream) (void)hipStreamDestroy(stream);
 }
 void tm_op::apply_dev(const void *x, void *y, int64_t ld, int nvec, tm_dtype io, tm_dtype comp,
-                      int transpose, const double *sc, hipStream_t s)
+                      int transpose, const double *sc, hipStream_t s, const ltm::Workspace *ws)
 {
-  apply_internal(x, y, ld, nvec, io, comp, transpose, sc, s);
+  apply_internal(x, y, ld, nvec, io, comp, transpose, sc, s, ws);
 }
+size_t tm_op::work_bytes(tm_dtype) const { return sizeof(double) * ncol; }
+size_t tm_op::perm_bytes(tm_dtype) const { return 0; }
 void tm_op::apply_internal(const void *x, void *y, int64_t ld, int nvec, tm_dtype io,
-                           tm_dtype comp, int transpose, const double *sc, hipStream_t s)
+                           tm_dtype comp, int transpose, const double *sc, hipStream_t s,
+                           const ltm::Workspace *ws)
 {
   const double *A = tab64.as<double>();
   const int64_t n = ncol;
-  work.ensure(sizeof(double) * n);
-  double *xv = work.as<double>();
+  double *xv;
+  if (ws) {
+    TM_CHECK(ws->work_bytes >= work_bytes(comp), TM_EINVAL, "workspace too small");
+    xv = static_cast<double *>(ws->work);
+  } else {
+    work.ensure(work_bytes(comp));
+    xv = work.as<double>();
+  }
   for (int v = 0; v < nvec; ++v) {
     const bool perturb = g_fake_perturb_after >= 0 && matvecs >= g_fake_perturb_after;
     fake_enqueue(s, [=] {
