@@ -40,6 +40,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--L", type=int, default=28)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure HBM bytes per T.x")
     ap.add_argument("--tx-reps", type=int, default=20)
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--workers", type=int, default=4, help="host threads of the scan batch")
@@ -84,12 +86,15 @@ def main():
         profile_tx(lt, op)
         return
 
-    # chosen method: FP32 phase with a 13-vector basis (measured best at this
-    # size) and an FP64 Rayleigh-Ritz block of the 4 wanted Ritz vectors
-    # (its error bound, 8e-14 for the 4th pair, meets the 1e-13 acceptance
-    # in one iteration; a block that does not is iterated); the plain FP64
-    # solve uses the reference's ncv = suggested_nev(4) = 11
-    NCV_CHOSEN = 13
+    # chosen method: FP32 phase with a 12-vector basis and an FP64
+    # Rayleigh-Ritz block of the 4 wanted Ritz vectors (measured at this size:
+    # ncv 11 -> 6.64 ms, 12 -> 6.11, 13 -> 6.99 (two refinement iterations),
+    # 14 -> 6.60, 15 -> 7.12 per solve; a block of 5 or 6 costs 0.2 / 1.2 ms
+    # more); a refinement block whose bound is not met is iterated, one that
+    # never meets it falls back to FP64 Krylov-Schur, so the choice affects
+    # time, not accuracy.  The plain FP64 solve uses the reference's
+    # ncv = suggested_nev(4) = 11
+    NCV_CHOSEN = 12
     REFINE_EXTRA = 0
 
     def solve(prec):
@@ -117,6 +122,13 @@ def main():
     res64 = [solve(lt.TM_PREC_F64) for _ in range(args.steps)]
     t_f64 = time.perf_counter() - ts
     log(f"[bench] fp64: {args.steps} solves in {t_f64:.3f}s")
+    # timed: plain FP64 with the chosen method's basis size (separates the
+    # gain of the basis size from that of the precision)
+    ts = time.perf_counter()
+    res64n = [lt.solve(op, nev=neigs, ncv=NCV_CHOSEN, precision=lt.TM_PREC_F64, seed=1)
+              for _ in range(args.steps)]
+    t_f64n = time.perf_counter() - ts
+    log(f"[bench] fp64 ncv {NCV_CHOSEN}: {args.steps} solves in {t_f64n:.3f}s")
 
     r, r64 = res[-1], res64[-1]
     lam = np.sort(np.abs(r.evals))[::-1]
@@ -127,17 +139,22 @@ def main():
     xi64 = 1.0 / math.log(lam64[0] / lam64[1])
     valid = bool(err_l0 <= 1e-12)
 
-    # T.x alone
-    ms32 = op.time_apply(lt.TM_F64, lt.TM_F32, warmup=3, reps=args.tx_reps)
+    # T.x alone: HIP events on the operator's stream, vectors resident; the
+    # chosen method's T.x reads and writes FP32 basis columns
+    ms32 = op.time_apply(lt.TM_F32, lt.TM_F32, warmup=3, reps=args.tx_reps)
+    ms32d = op.time_apply(lt.TM_F64, lt.TM_F32, warmup=3, reps=args.tx_reps)
     ms64 = op.time_apply(lt.TM_F64, lt.TM_F64, warmup=3, reps=args.tx_reps)
     N = 1 << L
-    # algorithmic bytes per T.x: one read + write of the expanded space and
-    # the reduced vectors and tables (8-byte rep + 1-byte size)
-    alg32 = N * (4 + 4) + M * (4 + 4 + 9)
-    alg64 = N * (8 + 8) + M * (8 + 8 + 9)
-    gbs32 = alg32 / (ms32 * 1e-3) / 1e9
-    gbs64 = alg64 / (ms64 * 1e-3) / 1e9
-    log(f"[bench] T.x fp32 {ms32:.3f} ms ({gbs32:.0f} GB/s alg), fp64 {ms64:.3f} ms ({gbs64:.0f} GB/s alg)")
+    alg32 = reduced_tx_bytes(M, L, 4, 4, 4)
+    alg32d = reduced_tx_bytes(M, L, 8, 8, 4)
+    alg64 = reduced_tx_bytes(M, L, 8, 8, 8)
+    gbs = {k: b["total"] / (ms * 1e-3) / 1e9 for k, b, ms in
+           (("fp32", alg32, ms32), ("fp32_f64vec", alg32d, ms32d), ("fp64", alg64, ms64))}
+    # the survey's figure: one pass over the expanded 2^L space, which the
+    # reference's algorithm streams (TMat/tmatreduce.cpp:419-470)
+    ref_alg32 = N * (4 + 4) + M * (4 + 4 + 9)
+    log(f"[bench] T.x fp32 {ms32:.4f} ms ({gbs['fp32']:.0f} GB/s of its {alg32['total']/1e6:.1f} MB), "
+        f"fp32 with fp64 vectors {ms32d:.4f} ms, fp64 {ms64:.4f} ms")
 
     # FP32 operator error level on a random unit vector
     xr = np.random.default_rng(3).uniform(0, 1, M)
@@ -147,22 +164,38 @@ def main():
     tx_err = float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64))
     log(f"[bench] FP32 T.x relative error {tx_err:.2e}")
 
-    # measured HBM bytes per FP32-compute T.x (FETCH_SIZE + WRITE_SIZE counter
-    # runs, scaled by a plain copy kernel's readings)
     traffic = None
-    tp = os.path.join(ROOT, "profiles", "xam_l28_traffic.json")
-    if os.path.exists(tp):
-        try:
-            traffic = json.load(open(tp)).get("fp32_bytes_per_tx")
-        except (OSError, ValueError):
-            traffic = None
-    traffic_gbs = traffic / (ms32 * 1e-3) / 1e9 if traffic else None
+    if not args.no_traffic:
+        traffic = measure_traffic(L, M)
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(L, bJ, kappa, r64.matvecs_f64)
 
     value = args.steps / t_mixed
+    roof = {
+        "bound": "hbm",
+        "kernel": "reduced T.x (k_xtilde + k_fused_pass1 + k_fused_pass2), FP32 compute on the "
+                  "FP32 basis columns the chosen method uses",
+        "achieved": round(gbs["fp32"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(gbs["fp32"] / HBM_PEAK_GBS, 4),
+        "traffic": traffic["fp32"]["bytes"] if traffic else None,
+        "alg_bytes": alg32["total"], "alg_bytes_by_kernel": alg32["kernels"],
+        "ms_per_launch": round(ms32, 5),
+        "fp32_f64_vectors": {"ms": round(ms32d, 5), "alg_bytes": alg32d["total"],
+                             "frac": round(gbs["fp32_f64vec"] / HBM_PEAK_GBS, 4)},
+        "fp64": {"ms": round(ms64, 5), "alg_bytes": alg64["total"],
+                 "frac": round(gbs["fp64"] / HBM_PEAK_GBS, 4)},
+        "vs_reference_algorithm": {
+            "alg_bytes": ref_alg32,
+            "note": "one pass over the 2^28 expanded space as the reference streams it; the "
+                    "two-pass necklace kernels never materialise that space",
+            "equivalent_GBs": round(ref_alg32 / (ms32 * 1e-3) / 1e9, 1)},
+    }
+    if traffic:
+        roof["traffic_frac_of_peak"] = round(traffic["fp32"]["bytes"] / (ms32 * 1e-3) / 1e9
+                                             / HBM_PEAK_GBS, 4)
+        roof["traffic_detail"] = traffic
     out = {
         "metric": "eigensolves/s at |dlambda0|/lambda0<=1e-12, chosen vs plain FP64; T.x achieved HBM GB/s",
         "value": round(value, 4),
@@ -174,38 +207,151 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 T.x + fp64 Krylov/refinement",
+        "dtype": "fp32 T.x on an fp32-stored phase-1 basis (fp64 accumulation) + fp64 refinement",
         "data": "synthetic (model parameters; seeded uniform(0,1) start vector)",
         "config": {
             "workload": "2D ANNNI perpendicular-TM, symmetry-reduced (axialdir=5), L=28, "
                         "betaJ=0.4, kappa1=0.5, h=0, neigs=4",
-            "states": N, "orbits": M, "ncv_chosen": NCV_CHOSEN, "refine_block": neigs + REFINE_EXTRA, "ncv_fp64": 11, "tol": 1e-10,
+            "states": N, "orbits": M, "ncv_chosen": NCV_CHOSEN,
+            "refine_block": neigs + REFINE_EXTRA, "ncv_fp64": 11, "tol": 1e-10,
         },
         "valid": valid,
         "plain_fp64": {
             "value": round(args.steps / t_f64, 4), "ms_per_step": round(1e3 * t_f64 / args.steps, 3),
-            "matvecs": int(r64.matvecs_f64),
+            "matvecs": int(r64.matvecs_f64), "ncv": 11,
+        },
+        "plain_fp64_ncv13": {
+            "value": round(args.steps / t_f64n, 4),
+            "ms_per_step": round(1e3 * t_f64n / args.steps, 3),
+            "matvecs": int(res64n[-1].matvecs_f64), "ncv": NCV_CHOSEN,
         },
         "speedup_vs_fp64": round((args.steps / t_mixed) / (args.steps / t_f64), 3),
+        "speedup_vs_fp64_same_basis": round((args.steps / t_mixed) / (args.steps / t_f64n), 3),
         "chosen_matvecs": {"fp32": int(r.matvecs_f32), "fp64": int(r.matvecs_f64)},
         "accuracy_vs_fp64": {"lambda0": err_l0, "lambda1": err_l1,
                              "xi": abs(xi - xi64) / xi64, "xi_value": xi},
-        "tx": {"fp32_ms": round(ms32, 4), "fp64_ms": round(ms64, 4), "fp32_rel_err": tx_err,
-               "fp32_alg_GBs": round(gbs32, 1), "fp64_alg_GBs": round(gbs64, 1)},
-        "roofline": {
-            "bound": "hbm", "kernel": "reduced T.x (xtilde + pass1 + pass2), FP32 compute",
-            "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": traffic,
-            # the algorithmic figure charges one pass over the 2^L expanded
-            # space, which the two-pass necklace kernels never stream; the
-            # measured bytes over the same time give the DRAM-side fraction
-            "traffic_GBs": round(traffic_gbs, 1) if traffic_gbs else None,
-            "traffic_frac": round(traffic_gbs / HBM_PEAK_GBS, 4) if traffic_gbs else None,
-        },
+        "resid_check": {"chosen": r.extra.get("resid_check"),
+                        "plain_fp64": r64.extra.get("resid_check")},
+        "tx": {"fp32_ms": round(ms32, 5), "fp32_f64vec_ms": round(ms32d, 5),
+               "fp64_ms": round(ms64, 5), "fp32_rel_err": tx_err},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(out), flush=True)
+
+
+def necklaces(n):
+    """Binary necklaces of length n (rotation classes), by Burnside's lemma."""
+    from math import gcd
+    tot = sum(2 ** gcd(k, n) for k in range(n))
+    return tot // n
+
+
+def bracelets(n):
+    """Binary bracelets of length n (classes under rotation and reversal)."""
+    m = (1 << n) - 1
+    seen, cnt = set(), 0
+    for w in range(1 << n):
+        if w in seen:
+            continue
+        cnt += 1
+        r = int(format(w, f"0{n}b")[::-1], 2)
+        for v in (w, r):
+            for k in range(n):
+                seen.add(((v << k) & m) | (v >> (n - k)) if k else v)
+    return cnt
+
+
+def reduced_tx_bytes(M, L, s_in, s_out, c):
+    """Algorithmic bytes of one fused reduced T.x (L = 2n even, periodic):
+    the bytes the three kernels must move at least, each table and vector
+    once.  s_in / s_out: bytes of an input / output vector element, c: of
+    the compute type (x~, the diagonal, the pass-1 intermediate A).  Pass 1
+    runs the nCol bracelet columns, pass 2 the nRow bracelet rows, over all
+    nNk necklace classes.
+      k_xtilde       x (M s_in), tvec (M c), x~ written (M c)
+      k_fused_pass1  cmap (nCol 2^n 4), x~ (every orbit once, M c),
+                     A written (nCol x nNk x 16 slots of c)
+      k_fused_pass2  A read (nRow x nNk x 16 c), Elist (M 2), dup*tvec (M c),
+                     y written (M s_out)
+    At n = 14: nNk = 1182 necklaces, nCol = nRow = 687 bracelets."""
+    n = L // 2
+    nNk = necklaces(n)
+    nb = bracelets(n)
+    k = {"k_xtilde": M * (s_in + c + c),
+         "k_fused_pass1": nb * (1 << n) * 4 + M * c + nb * nNk * 16 * c,
+         "k_fused_pass2": nb * nNk * 16 * c + M * (2 + c + s_out)}
+    return {"total": sum(k.values()), "kernels": k, "nNk": nNk, "bracelets": nb}
+
+
+def measure_traffic(L, M):
+    """HBM bytes per reduced T.x from two rocprofv3 counter passes (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch) over `bench.py --profile-tx`, each
+    scaled by what the same counter reads for the plain copy kernel
+    k_copy<double, double> of M doubles (M*8 bytes read, M*8 written) in the
+    same run.  Returns None if rocprofv3 is not available or a pass fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    rp = shutil.which("rocprofv3")
+    if not rp:
+        log("[bench] rocprofv3 not found: traffic not measured")
+        return None
+    base = os.path.join(ROOT, "gpurun_out") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) \
+        else tempfile.gettempdir()
+    per = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix=f"bench_{ctr.lower()}_", dir=base)
+        cmd = ["timeout", "-s", "KILL", "240", rp, "--pmc", ctr, "--output-format", "csv",
+               "-d", d, "-o", "tx", "--", sys.executable, os.path.abspath(__file__),
+               "--profile-tx", "--L", str(L)]
+        log(f"[bench] counter pass {ctr}")
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            log(f"[bench] counter pass {ctr} gave no results (rc {r.returncode}): "
+                f"{r.stderr[-400:]}")
+            return None
+        vals = {}
+        with open(files[0]) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != ctr:
+                    continue
+                vals.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+        per[ctr] = {k: sum(v) / len(v) for k, v in vals.items()}
+
+    def find(ctr, *parts):
+        for k, v in per[ctr].items():
+            if all(p in k for p in parts):
+                return v
+        return None
+    cal_f = find("FETCH_SIZE", "k_copy<double, double>")
+    cal_w = find("WRITE_SIZE", "k_copy<double, double>")
+    if not cal_f or not cal_w:
+        log("[bench] calibration copy kernel missing from the counter runs")
+        return None
+    ff = M * 8 / (cal_f * 1024.0)
+    fw = M * 8 / (cal_w * 1024.0)
+    out = {"calibration": {"kernel": "ltm::kry::k_copy<double, double>", "bytes_each_way": M * 8,
+                           "fetch_factor": round(ff, 4), "write_factor": round(fw, 4)}}
+    for name, sig in (("fp32", ("float, float",)), ("fp32_f64vec", ("float, double",)),
+                      ("fp64", ("double, double",))):
+        comp = "float" if name != "fp64" else "double"
+        ks = [("k_xtilde<",) + sig, (f"k_fused_pass1<{comp}>",), ("k_fused_pass2<",) + sig]
+        tot, det = 0.0, {}
+        for key in ks:
+            f_, w_ = find("FETCH_SIZE", *key), find("WRITE_SIZE", *key)
+            if f_ is None or w_ is None:
+                return None
+            b = f_ * 1024 * ff + w_ * 1024 * fw
+            det[key[0].rstrip("<")] = {"read": round(f_ * 1024 * ff), "write": round(w_ * 1024 * fw)}
+            tot += b
+        out[name] = {"bytes": round(tot), "kernels": det}
+    log(f"[bench] traffic per FP32 T.x {out['fp32']['bytes'] / 1e6:.1f} MB")
+    return out
 
 
 def scan_instances():
@@ -576,25 +722,40 @@ def bench_grad(lt, args):
 
 
 def profile_tx(lt, op):
-    """Counter-run workload: a plain device copy of known size (the
-    calibration for FETCH_SIZE / WRITE_SIZE), then 5 FP32 and 5 FP64 T.x."""
-    import torch
-    nbytes = 1 << 29  # 512 MiB read + 512 MiB written
-    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
+    """Counter-run workload: 5 T.x in each form the headline reports (FP32
+    compute on FP32 vectors, FP32 compute on FP64 vectors, FP64), each
+    preceded by the plain copy kernel k_copy of the operator's width that the
+    counters are calibrated with."""
+    op.time_apply(lt.TM_F32, lt.TM_F32, warmup=0, reps=5)
     op.time_apply(lt.TM_F64, lt.TM_F32, warmup=0, reps=5)
     op.time_apply(lt.TM_F64, lt.TM_F64, warmup=0, reps=5)
-    log(f"[bench] profile-tx done: copy {nbytes} B x3, T.x fp32 x5, fp64 x5")
+    log("[bench] profile-tx done: T.x fp32/fp32, fp32/fp64-vectors, fp64 x5 each")
+
+
+def cpu_model():
+    """CPU model name and core counts of this host (lscpu)."""
+    import subprocess
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for ln in txt.splitlines():
+            if ":" in ln:
+                k, v = ln.split(":", 1)
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"model": info.get("Model name"), "cpus": info.get("CPU(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"), "sockets": info.get("Socket(s)"),
+            "threads_per_core": info.get("Thread(s) per core")}
 
 
 def cpu_baseline(L, bJ, kappa, nmatvec):
-    """Time the CPU oracle (restatement of XAMMatVecProd2D::perform_op,
-    TMat/tmatreduce.cpp:406-529) on the host cores: orbit enumeration once,
-    then one T.x at the benchmark size; eigensolves/s is estimated as
-    1 / (t_Tx * number of T.x the plain FP64 eigensolve needed)."""
+    """The CPU oracle (restatement of XAMMatVecProd2D::perform_op,
+    TMat/tmatreduce.cpp:406-529) timed on the host cores: after one warm-up
+    T.x, the median of 5 T.x on all OpenMP threads and the median of 2 on one
+    thread (the reference's tmat_omp and tmat builds, tmat.make:39-58);
+    eigensolves/s = 1 / (median T.x time x the number of T.x the plain FP64
+    eigensolve takes).  The orbit set-up is excluded."""
     import numpy as np
 
     from oracle import oracle as O
@@ -604,17 +765,36 @@ def cpu_baseline(L, bJ, kappa, nmatvec):
     oop = O.OracleOp(O.XAM2D, L, 0, 2, bJ, kappa, 0.0, 0.0)
     t_setup = time.perf_counter() - t0
     x = np.random.default_rng(1).uniform(0, 1, oop.n)
-    t0 = time.perf_counter()
-    oop.apply(x)
-    t_tx = time.perf_counter() - t0
-    log(f"[bench] cpu oracle: setup {t_setup:.1f}s, one T.x {t_tx:.2f}s on {O.num_threads()} threads")
+    nthr = O.num_threads()
+
+    def timed(k):
+        ts = []
+        for _ in range(k):
+            t = time.perf_counter()
+            oop.apply(x)
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts)), ts
+    oop.apply(x)  # warm-up
+    t_all, ts_all = timed(5)
+    O.set_num_threads(1)
+    try:
+        t_one, ts_one = timed(2)
+    finally:
+        O.set_num_threads(nthr)
+    cpu = cpu_model()
+    log(f"[bench] cpu oracle: setup {t_setup:.1f}s, T.x median {t_all:.3f}s on {nthr} threads, "
+        f"{t_one:.2f}s on 1 thread ({cpu['model']})")
     return {
-        "value": round(1.0 / (t_tx * nmatvec), 6),
+        "value": round(1.0 / (t_all * nmatvec), 6),
         "unit": "eigensolves/s",
-        "cores": int(O.num_threads()),
+        "cores": int(nthr),
         "kind": "port",
-        "sample": f"one oracle T.x at L={L} ({t_tx:.2f}s) x {nmatvec} T.x per FP64 eigensolve; "
-                  f"orbit setup {t_setup:.1f}s excluded",
+        "sample": f"oracle T.x at L={L}: median of 5 after one warm-up ({t_all:.3f} s) x "
+                  f"{nmatvec} T.x per plain FP64 eigensolve; orbit set-up {t_setup:.1f} s excluded",
+        "single_thread": {"value": round(1.0 / (t_one * nmatvec), 6), "cores": 1,
+                          "tx_s": round(t_one, 3), "sample": "median of 2 T.x on one thread"},
+        "tx_s_all": [round(t, 4) for t in ts_all],
+        "host": cpu,
     }
 
 

@@ -320,22 +320,36 @@ void expand_orbits(const GroupSpec &g, const uint64_t *reps, const uint16_t *dup
 // Fused-path tables (2D periodic, L = 2n even).  A state s is split into
 // its even bits E and odd bits O (n bits each); a rotation of s by two sites
 // rotates E and O together, so the transform over O only has to be done for
-// columns E that are necklace representatives (minimal rotations).
+// columns E that are necklace representatives (minimal rotations).  The
+// site mirror i -> -i is in the group too: it maps E to rotl_1(rev E) and O
+// to rev O, so a column whose necklace is the mirror of another's is that
+// column with its O words reversed.  Pass 1 therefore stores one column per
+// bracelet (a necklace class and its mirror), and pass 2 runs only the rows
+// that are bracelet representatives (every orbit has evaluation points in
+// both a row and its mirror).  At n = 14: 1182 necklaces, 687 bracelets.
 struct FusedTables {
   int n;                 // L/2
   int nNk;               // number of n-bit necklaces
+  int nCol;              // stored columns (bracelet representatives)
+  int nRow;              // rows pass 2 runs (bracelet representatives)
   int64_t ldA;           // leading dimension of the pass-1 output (nNk*16, padded)
   DevBuf NR;             // uint16[nNk]   necklace representatives, ascending
   DevBuf period;         // uint8[nNk]    rotation period of each representative
-  DevBuf nkw;            // uint16[2^n]   necklace index of every n-bit word
-  DevBuf rotw;           // uint8[2^n]    word = rotl(NR[nkw], rotw)
   DevBuf slotO;          // uint16[nNk*16] LDS index sw(O) of (row necklace r, slot k),
-                         //   O = rotl_{(n-k) mod p_r} NR[r] for k < n; 0xFFFF unused
+                         //   O = rotr_k NR[r] (mod the period) for k < n; 0xFFFF unused
   DevBuf nkl;            // uint16[2^n]   LDS index sw(O) of the n-bit words O in
                          //               necklace-major order (class, rotation)
-  DevBuf cmap;           // uint32[nNk << n] orbit of the state (E = NR[c], O = nkl[j])
-  DevBuf ostart;         // int64[nNk+1]  internal-index range of the orbits whose
-                         //               evaluation point has O = NR[r]
+  DevBuf cmap;           // uint32[nCol << n] orbit of the state (E = NR[stored column],
+                         //               O = nkl[j])
+  DevBuf csrc;           // uint16[nNk]   stored column holding class c's data (its own,
+                         //               or its mirror's)
+  DevBuf crot;           // uint8[nNk]    255 for a stored class; for a mirrored one
+                         //               the m with NR[c] = rotl_m(rotl_1(rev NR[src]))
+  DevBuf rowS;           // uint16[nRow]  the row classes pass 2 runs
+  DevBuf rmir;           // uint16[nNk]   mirror class of each row
+  DevBuf rrot;           // uint8[nNk]    t with rev(NR[r]) = rotl_t NR[rmir[r]]
+  DevBuf ostart;         // int64[nRow+1] internal-index range of the orbits whose
+                         //               evaluation point has O = NR[rowS[q]]
   DevBuf olist;          // uint32[M]     internal index -> orbit (ascending-rep order)
   DevBuf Elist;          // uint16[M]     LDS index sw(E) of the E part of each orbit's
                          //               evaluation point

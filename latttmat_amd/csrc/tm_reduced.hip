@@ -474,38 +474,93 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   }
   ft.nNk = (int)NR.size();
   ft.ldA = (((int64_t)ft.nNk * 16 + 63) / 64) * 64;
+  const int nNk = ft.nNk;
+  // mirrors: the site mirror i -> -i maps the E word to rotl_1(rev E) and
+  // the O word to rev O; both land in the necklace class of the reversed word
+  auto rev = [&](uint32_t w) {
+    uint32_t r = 0;
+    for (int i = 0; i < n; ++i) r |= ((w >> i) & 1u) << (n - 1 - i);
+    return r;
+  };
+  std::vector<uint16_t> mcl(nNk);
+  for (int c = 0; c < nNk; ++c) mcl[c] = nk[rev(NR[c])];
+  // stored columns and rows: the bracelet representatives (c <= its mirror)
+  std::vector<uint16_t> colS, rowS;
+  std::vector<int32_t> sidx(nNk, -1);
+  for (int c = 0; c < nNk; ++c)
+    if (c <= mcl[c]) {
+      sidx[c] = (int32_t)colS.size();
+      colS.push_back((uint16_t)c);
+      rowS.push_back((uint16_t)c);
+    }
+  ft.nCol = (int)colS.size();
+  ft.nRow = (int)rowS.size();
+  std::vector<uint16_t> csrc(nNk), rmir(nNk);
+  std::vector<uint8_t> crot(nNk, 255), rrot(nNk, 0);
+  for (int c = 0; c < nNk; ++c) {
+    if (sidx[c] >= 0) {
+      csrc[c] = (uint16_t)sidx[c];
+    } else {
+      const int src = mcl[c];
+      csrc[c] = (uint16_t)sidx[src];
+      const uint32_t w = rotl(rev(NR[src]), 1);
+      int m = 0;
+      while (m < per[c] && rotl(w, m) != NR[c]) ++m;
+      TM_CHECK(m < per[c], TM_EINVAL, "mirror column has no rotation onto its representative");
+      crot[c] = (uint8_t)m;
+    }
+    rmir[c] = mcl[c];
+    const uint32_t rw = rev(NR[c]);
+    int t = 0;
+    while (t < per[c] && rotl(NR[mcl[c]], t) != rw) ++t;
+    TM_CHECK(t < per[c], TM_EINVAL, "mirror row has no rotation onto the reversed word");
+    rrot[c] = (uint8_t)t;
+  }
   // column order of cmap: O' words necklace-major (class, then rotation), so
   // that a wavefront's gathers fall into a few per-row clusters
   std::vector<uint16_t> order;
   order.reserve(nw);
-  for (int q = 0; q < ft.nNk; ++q)
+  for (int q = 0; q < nNk; ++q)
     for (int k = 0; k < per[q]; ++k) order.push_back((uint16_t)rotl(NR[q], k));
   // slot k of row r holds the O word that pass 2 needs for the rotation
-  // E = rotl_k(NR[c]) of any column: A(NR[c], rotr_k NR[r]) = A(NR[c],
+  // E = rotl_k(NR[c]) of a stored column: A(NR[c], rotr_k NR[r]) = A(NR[c],
   // rotl_{(n-k) mod p_r} NR[r]), k < n (k >= p_c is never read)
-  std::vector<uint16_t> slot((size_t)ft.nNk * 16, 0xFFFF);
-  for (int r = 0; r < ft.nNk; ++r)
+  std::vector<uint16_t> slot((size_t)nNk * 16, 0xFFFF);
+  for (int r = 0; r < nNk; ++r)
     for (int k = 0; k < n; ++k)
       slot[(size_t)r * 16 + k] = (uint16_t)rotl(NR[r], (n - k) % per[r]);
-  // evaluation point of every orbit: rotate the representative so that its
-  // odd-bit part becomes a necklace representative
+  // evaluation point of every orbit: a state of the orbit whose odd-bit part
+  // is the necklace representative of a stored row (the representative, or
+  // its mirror image if the representative's O class is not stored),
+  // rotated so that the O part is that representative
   std::vector<uint64_t> hreps(M);
   copy_sync(hreps.data(), reps.get(), M * 8, hipMemcpyDeviceToHost, s);
+  const int L = gs.L;
   auto compact = [&](uint64_t v) {  // bits 0,2,4,... -> 0,1,2,...
     uint32_t r = 0;
     for (int i = 0; i < n; ++i) r |= (uint32_t)((v >> (2 * i)) & 1) << i;
     return r;
   };
-  std::vector<int64_t> cnt(ft.nNk + 1, 0);
+  auto mirror = [&](uint64_t v) {  // site i -> -i mod L
+    uint64_t r = v & 1;
+    for (int i = 1; i < L; ++i) r |= ((v >> i) & 1ull) << (L - i);
+    return r;
+  };
+  std::vector<int32_t> rowq(nNk, -1);
+  for (int q = 0; q < ft.nRow; ++q) rowq[rowS[q]] = q;
+  std::vector<int64_t> cnt(ft.nRow + 1, 0);
   std::vector<uint16_t> pr(M), pe(M);
   for (int64_t o = 0; o < M; ++o) {
-    const uint32_t E = compact(hreps[o]), O = compact(hreps[o] >> 1);
+    uint64_t st = hreps[o];
+    if (rowq[nk[compact(st >> 1)]] < 0) st = mirror(st);
+    const uint32_t E = compact(st), O = compact(st >> 1);
     const int r = nk[O], k = rot[O];
-    pr[o] = (uint16_t)r;
+    TM_CHECK(rowq[r] >= 0, TM_EINVAL, "orbit has no evaluation point in a stored row");
+    pr[o] = (uint16_t)rowq[r];
     pe[o] = (uint16_t)rotl(E, n - k);  // rotate E right by k
-    ++cnt[r + 1];
+    ++cnt[rowq[r] + 1];
   }
-  for (int r = 0; r < ft.nNk; ++r) cnt[r + 1] += cnt[r];
+  for (int q = 0; q < ft.nRow; ++q) cnt[q + 1] += cnt[q];
   std::vector<uint32_t> olist(M);
   std::vector<uint16_t> elist(M);
   std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
@@ -513,8 +568,8 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   // Within a row, order the orbits by the necklace class (then rotation) of
   // their E part: pass 1 of column c gathers, from every row, only orbits
   // whose E part is a rotation of NR[c], which are then one contiguous run.
-  for (int r = 0; r < ft.nNk; ++r)
-    std::sort(olist.begin() + cnt[r], olist.begin() + cnt[r + 1], [&](uint32_t a, uint32_t b) {
+  for (int q = 0; q < ft.nRow; ++q)
+    std::sort(olist.begin() + cnt[q], olist.begin() + cnt[q + 1], [&](uint32_t a, uint32_t b) {
       const uint32_t ea = pe[a], eb = pe[b];
       if (nk[ea] != nk[eb]) return nk[ea] < nk[eb];
       if (rot[ea] != rot[eb]) return rot[ea] < rot[eb];
@@ -527,8 +582,11 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   };
   up(ft.NR, NR.data(), NR.size() * 2);
   up(ft.period, per.data(), per.size());
-  up(ft.nkw, nk.data(), nk.size() * 2);
-  up(ft.rotw, rot.data(), rot.size());
+  up(ft.csrc, csrc.data(), csrc.size() * 2);
+  up(ft.crot, crot.data(), crot.size());
+  up(ft.rowS, rowS.data(), rowS.size() * 2);
+  up(ft.rmir, rmir.data(), rmir.size() * 2);
+  up(ft.rrot, rrot.data(), rrot.size());
   // the pass kernels use these words only as LDS indices: store them swizzled
   for (auto &w : slot)
     if (w != 0xFFFF) w = (uint16_t)dev::sw(w);
@@ -545,15 +603,19 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   up(invd, inv.data(), inv.size() * 4);
   ft.perm = std::move(olist);
 
-  // column-major state->orbit map over the necklace columns
+  // column-major state->orbit map over the stored necklace columns
   const GroupDev g = to_dev(gs);
   OrbitScratch sc;
   const uint64_t M2 = orbit_bitmap(g, sc, s);
   TM_CHECK((int64_t)M2 == M, TM_EINVAL, "orbit count mismatch while building fused tables");
-  const uint64_t total = (uint64_t)ft.nNk << n;
+  std::vector<uint16_t> colw(ft.nCol);
+  for (int i = 0; i < ft.nCol; ++i) colw[i] = NR[colS[i]];
+  DevBuf colwd;
+  up(colwd, colw.data(), colw.size() * 2);
+  const uint64_t total = (uint64_t)ft.nCol << n;
   ft.cmap.alloc(total * 4);
   hipLaunchKernelGGL(k_build_cmap, dim3(grid_for(total, 256)), dim3(256), 0, s, g, n,
-                     ft.NR.as<uint16_t>(), (uint64_t)ft.nNk, ft.nkl.as<uint16_t>(),
+                     colwd.as<uint16_t>(), (uint64_t)ft.nCol, ft.nkl.as<uint16_t>(),
                      sc.bitmap.as<uint64_t>(), sc.wpre.as<uint64_t>(), ft.cmap.as<uint32_t>());
   TM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_remap_u32, dim3(grid_for(total, 256)), dim3(256), 0, s,
@@ -652,21 +714,22 @@ __global__ __launch_bounds__(1024) void k_kron_small(C *u, int L, C r, C f)
 
 constexpr int FUSED_THREADS = 1024;
 
-// Pass 1: one necklace column E = NR[c]; u(E, O') = xt[cmap[c][j]] with
-// O' over all words (LDS index nkl[j] = sw(O')); transform over O; store A(E, O) grouped as
-// [row necklace][slot].
+// Pass 1: one stored column (E = the necklace representative of a bracelet);
+// u(E, O') = xt[cmap[c][j]] with O' over all words (LDS index nkl[j] =
+// sw(O')); transform over O; store A(E, O) grouped as [row necklace][slot]
+// for every one of the nNk row necklaces.
 template <typename C>
 __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
     const uint32_t *__restrict__ cmap, const uint16_t *__restrict__ nkl, const C *__restrict__ xt,
-    C *__restrict__ Ap, const uint16_t *__restrict__ slotO, int n, int nNk, int64_t ldA, C kr,
-    C kf)
+    C *__restrict__ Ap, const uint16_t *__restrict__ slotO, int n, int nCol, int nNk,
+    int64_t ldA, C kr, C kf)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
   // within a row necklace the orbits are ordered by their E necklace, so
   // columns c, c+1, ... gather neighbouring runs of x~: keep consecutive
   // columns on one XCD so that they share its L2 lines
-  const int c = (int)dev::xcd_remap(blockIdx.x, nNk);
+  const int c = (int)dev::xcd_remap(blockIdx.x, nCol);
   if (c < 0) return;
   const int nh = 1 << n;
   const uint32_t *mrow = cmap + ((uint64_t)c << n);
@@ -711,47 +774,69 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
   }
 }
 
-// Pass 2: one row necklace O_r = NR[r]; rebuild A(E, O_r) for every E from
-// A(E, O_r) = A(NR[nk E], rotr_k O_r), E = rotl_k NR[nk E]; transform over E;
-// write y at the orbits whose evaluation point lies on this row.
+// Pass 2: one stored row necklace O_r = NR[r]; rebuild A(E, O_r) for every E
+// and transform over E; write y at the orbits whose evaluation point lies on
+// this row.  For E = rotl_k NR[c] of a stored column c,
+//   A(E, O_r) = A(NR[c], rotr_k O_r) = slot k of (c, r);
+// for a mirrored column c (NR[c] = rotl_m(rotl_1(rev NR[src]))) the mirror
+// and rotation invariance of A give
+//   A(rotr_{m+t+j} NR[c], O_r) = slot j of (src, mirror row of r),
+// with rev(NR[r]) = rotl_t NR[mirror row].
 template <typename C, typename TO>
 __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
     const C *__restrict__ Ap, const uint16_t *__restrict__ NR, const uint8_t *__restrict__ period,
-    int nNk, const int64_t *__restrict__ ostart, const uint16_t *__restrict__ Elist, const C *__restrict__ dt, TO *__restrict__ y, int n,
+    const uint16_t *__restrict__ csrc, const uint8_t *__restrict__ crot,
+    const uint16_t *__restrict__ rowS, const uint16_t *__restrict__ rmir,
+    const uint8_t *__restrict__ rrot, int nNk, int nRow, const int64_t *__restrict__ ostart,
+    const uint16_t *__restrict__ Elist, const C *__restrict__ dt, TO *__restrict__ y, int n,
     int64_t ldA, C kr, C kf, const double *__restrict__ scale)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
   // consecutive rows read neighbouring 16-slot groups of every column (the
   // same cache lines): keep them on one XCD
-  const int r = (int)dev::xcd_remap(blockIdx.x, nNk);
-  if (r < 0) return;
+  const int q = (int)dev::xcd_remap(blockIdx.x, nRow);
+  if (q < 0) return;
+  const int r = rowS[q], mr = rmir[r], tr = rrot[r];
   const uint32_t wmask = (1u << n) - 1;
   // one necklace column per thread: slot k of the 16-slot group holds the
   // value for the rotation E = rotl_k NR[c] (the slot table is ordered so)
-  const C *grp0 = Ap + (int64_t)r * 16;
   for (int c = threadIdx.x; c < nNk; c += blockDim.x) {
-    const C *g = grp0 + (int64_t)c * ldA;
+    const int m = crot[c];
+    const bool mir = m != 255;
+    const C *g = Ap + (int64_t)csrc[c] * ldA + (int64_t)(mir ? mr : r) * 16;
     C v[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = g[j];
     const uint32_t w = NR[c];
     const int pc = period[c];
-    uint32_t E = w;
+    if (!mir) {
+      uint32_t E = w;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (k >= pc) break;
-      lds[dev::sw(E)] = v[k];
-      E = ((E << 1) & wmask) | (E >> (n - 1));
+      for (int k = 0; k < 16; ++k) {
+        if (k >= pc) break;
+        lds[dev::sw(E)] = v[k];
+        E = ((E << 1) & wmask) | (E >> (n - 1));
+      }
+    } else {
+      // E_j = rotr_{m+t+j} NR[c]
+      const int d = (m + tr) % pc;
+      uint32_t E = d ? ((w >> d) | ((w << (n - d)) & wmask)) : w;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if (j >= pc) break;
+        lds[dev::sw(E)] = v[j];
+        E = (E >> 1) | ((E & 1u) << (n - 1));
+      }
     }
   }
   __syncthreads();
   dev::lds_kron(lds, n, kr);
   // 1 / scale of the input, and kf = a^n of this pass's butterflies
   const C inv = (scale ? (C)(1.0 / *scale) : (C)1) * kf;
-  // internal numbering: the orbits of this row are [ostart[r], ostart[r+1]);
+  // internal numbering: the orbits of this row are [ostart[q], ostart[q+1]);
   // UNR index and weight loads in flight per thread before the LDS reads
-  const int64_t i0 = ostart[r], i1 = ostart[r + 1];
+  const int64_t i0 = ostart[q], i1 = ostart[q + 1];
   constexpr int UNR = 4;
   for (int64_t b = i0 + threadIdx.x; b < i1; b += (int64_t)blockDim.x * UNR) {
     uint16_t E[UNR];
@@ -792,22 +877,26 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
     const int n = ft.n;
     C *Ap = rest;
     const size_t shm = (size_t(1) << n) * sizeof(C);
-    const unsigned g1 = 8u * (unsigned)((ft.nNk + 7) / 8);
+    const unsigned g1 = 8u * (unsigned)((ft.nCol + 7) / 8);
     hipLaunchKernelGGL((k_fused_pass1<C>), dim3(g1), dim3(FUSED_THREADS), shm, s,
                        ft.cmap.as<uint32_t>(), ft.nkl.as<uint16_t>(), xt, Ap,
-                       ft.slotO.as<uint16_t>(), n, ft.nNk, ft.ldA, kr,
+                       ft.slotO.as<uint16_t>(), n, ft.nCol, ft.nNk, ft.ldA, kr,
                        dev::kron_coef<C>(p.ka, 0, p.kb, 0, n).f);
     TM_LAUNCH_CHECK();
-    const unsigned g2 = 8u * (unsigned)((ft.nNk + 7) / 8);
+    const unsigned g2 = 8u * (unsigned)((ft.nRow + 7) / 8);
     if (io == TM_F64)
       hipLaunchKernelGGL((k_fused_pass2<C, double>), dim3(g2), dim3(FUSED_THREADS), shm, s,
-                         Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(), ft.nNk,
+                         Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(),
+                         ft.csrc.as<uint16_t>(), ft.crot.as<uint8_t>(), ft.rowS.as<uint16_t>(),
+                         ft.rmir.as<uint16_t>(), ft.rrot.as<uint8_t>(), ft.nNk, ft.nRow,
                          ft.ostart.as<int64_t>(), ft.Elist.as<uint16_t>(), dt,
                          static_cast<double *>(out), n, ft.ldA, kr,
                          dev::kron_coef<C>(p.ka, 0, p.kb, 0, n).f, scale);
     else
       hipLaunchKernelGGL((k_fused_pass2<C, float>), dim3(g2), dim3(FUSED_THREADS), shm, s,
-                         Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(), ft.nNk,
+                         Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(),
+                         ft.csrc.as<uint16_t>(), ft.crot.as<uint8_t>(), ft.rowS.as<uint16_t>(),
+                         ft.rmir.as<uint16_t>(), ft.rrot.as<uint8_t>(), ft.nNk, ft.nRow,
                          ft.ostart.as<int64_t>(), ft.Elist.as<uint16_t>(), dt,
                          static_cast<float *>(out), n, ft.ldA, kr,
                          dev::kron_coef<C>(p.ka, 0, p.kb, 0, n).f, scale);
@@ -894,7 +983,7 @@ size_t reduced_work_bytes(const ReducedPlan &p, tm_dtype comp)
   const size_t es = comp == TM_F64 ? 8 : 4;
   size_t b = (size_t)((p.M + 63) / 64) * 64 * es;
   if (p.ft) {
-    b += (size_t)p.ft->nNk * p.ft->ldA * es;
+    b += (size_t)p.ft->nCol * p.ft->ldA * es;
   } else if (p.sweep) {
     b += 2 * (size_t(1) << p.L) * es + sweep_work_bytes(p.sweep->kind, p.L, comp);
   } else if (p.ylayer) {

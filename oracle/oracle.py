@@ -87,6 +87,8 @@ def lib():
         L.or_sample_configs.argtypes = [C.c_void_p, pdbl, C.c_void_p, pdbl, i32, i32, C.c_void_p]
         L.or_num_threads.restype = i32
         L.or_num_threads.argtypes = []
+        L.or_set_num_threads.restype = None
+        L.or_set_num_threads.argtypes = [i32]
         _lib = L
     return _lib
 
@@ -212,3 +214,7 @@ def tvec_xat(L, bc, betaJ, kappa, h):
 
 def num_threads():
     return int(lib().or_num_threads())
+
+
+def set_num_threads(n):
+    lib().or_set_num_threads(int(n))

@@ -26,6 +26,15 @@ int or_num_threads(void)
 #endif
 }
 
+void or_set_num_threads(int n)
+{
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------------ */
 /* bit primitives                                                            */
 /* ------------------------------------------------------------------------ */

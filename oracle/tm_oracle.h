@@ -152,6 +152,8 @@ int or_sample_configs(const or_op *op, const double *vec, const double *vecl, co
                       int nchain, int length, uint64_t *states);
 
 int or_num_threads(void);
+/* threads of the following calls (OpenMP); the CPU baseline times 1 and all */
+void or_set_num_threads(int n);
 
 #ifdef __cplusplus
 }
