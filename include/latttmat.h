@@ -192,6 +192,13 @@ typedef struct {
                                ||Tx - lambda x|| / |lambda| <= 1e-6 (one FP64
                                T.x, or the refinement's explicit residual);
                                a failed check returns TM_EHIP              */
+  int          block;       /* symmetric operators: 0 or 1 (default) = one
+                               vector per Krylov step; b in [2, 8] = block
+                               Krylov-Schur with b vectors per step (needs
+                               ncv >= nev + b and ncv + 2b <= 64; the Gram
+                               of the block Gram-Schmidt on FP64 MFMA);
+                               operators with cols() < 8 (ncv + 2b) and
+                               nonsymmetric ones run the one-vector form */
 } tm_solve_opts;
 
 /* Results.  Eigenvalues are those of the operator as applied (the reference

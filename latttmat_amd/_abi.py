@@ -52,6 +52,7 @@ class SolveOpts(C.Structure):
         ("want_left", C.c_int), ("want_vectors", C.c_int), ("mixed_tol", C.c_double),
         ("verbose", C.c_int), ("refine", C.c_int), ("use_graph", C.c_int),
         ("basis_f32", C.c_int), ("refine_extra", C.c_int), ("verify", C.c_int),
+        ("block", C.c_int),
     ]
 
 
@@ -312,7 +313,8 @@ class SolveResult:
 
 
 def _make_opts(nev, ncv, tol, maxit, precision, seed, want_left, want_vectors, mixed_tol,
-               verbose, maxtrial, refine, use_graph, basis_f32=True, refine_extra=1, verify=True):
+               verbose, maxtrial, refine, use_graph, basis_f32=True, refine_extra=1, verify=True,
+               block=0):
     o = SolveOpts()
     lib().tm_solve_opts_default(C.byref(o))
     o.nev, o.ncv, o.tol, o.maxit, o.maxtrial = nev, ncv, tol, maxit, maxtrial
@@ -323,6 +325,7 @@ def _make_opts(nev, ncv, tol, maxit, precision, seed, want_left, want_vectors, m
     o.basis_f32 = int(basis_f32)
     o.refine_extra = int(refine_extra)
     o.verify = int(verify)
+    o.block = int(block)
     return o
 
 
@@ -363,9 +366,9 @@ class _OutBufs:
 def solve(op: TransferMatrix, nev=1, ncv=0, tol=1e-10, maxit=1000, precision=TM_PREC_F64,
           seed=1, want_left=False, want_vectors=False, mixed_tol=2e-7, verbose=False,
           maxtrial=3, raise_on_noconv=True, refine=0, use_graph=True,
-          basis_f32=True, refine_extra=1, verify=True) -> SolveResult:
+          basis_f32=True, refine_extra=1, verify=True, block=0) -> SolveResult:
     o = _make_opts(nev, ncv, tol, maxit, precision, seed, want_left, want_vectors, mixed_tol,
-                   verbose, maxtrial, refine, use_graph, basis_f32, refine_extra, verify)
+                   verbose, maxtrial, refine, use_graph, basis_f32, refine_extra, verify, block)
     b = _OutBufs(op, nev, want_left, want_vectors)
     st = lib().tm_solve(op._h, C.byref(o), C.byref(b.out))
     if st not in (TM_OK, TM_ENOCONV) or (st == TM_ENOCONV and raise_on_noconv):

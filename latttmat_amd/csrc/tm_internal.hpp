@@ -151,7 +151,7 @@ class DevBuf {
 // assigned by the eigensolver).  A graph is keyed by every pointer and size its
 // launches embed, so a key match means identical kernel arguments.
 struct SolverCache {
-  static constexpr int NDEV = 25, NPIN = 4;
+  static constexpr int NDEV = 29, NPIN = 4;
   DevBuf dev[NDEV];
   PinnedBuf pinned[NPIN];
   struct Graph {
@@ -470,6 +470,24 @@ void gram_sym(const double *U, int64_t ld, int nv, int64_t n, double *partial, d
 // the combinations (one sweep per 8 columns)
 void combine_norms(const double *U, int64_t ld, int nu, const double *Cm, int ldc, int nc,
                    int64_t n, double *partial, double *out, hipStream_t s);
+// ---- block Krylov (blocks of b <= 8 vectors) ----
+// G[r + c * nx] = <X_r, Y_c> for r < nx <= 64, c < b <= 16, in one sweep on
+// FP64 MFMA (X, Y columns of leading dimension ld); partial holds
+// gram_xy_partial_doubles() doubles.
+template <typename T>
+void gram_xy(const T *X, int64_t ld, int nx, const T *Y, int b, int64_t n, double *partial,
+             double *G, hipStream_t s);
+size_t gram_xy_partial_doubles();
+// W <- (W - X H) Rinv: H nx x b (leading dimension ldh, null: none), Rinv b x b
+// upper triangular (null: identity); X and W distinct columns of one basis
+template <typename T>
+void block_update(const T *X, int64_t ld, int nx, T *W, int b, const double *H, int ldh,
+                  const double *Rinv, int64_t n, hipStream_t s);
+// close a block step on the device (mode 0: projected-matrix column block
+// and R from the two projections; mode 1: CholQR's second pass); see
+// k_block_close
+void block_close(const double *G1, const double *G2, int nc, int b, double *Hd, int ldh, int col0,
+                 double *Rinv, double *flags, int step, int mode, hipStream_t s);
 }  // namespace kry
 
 // ---------------------------------------------------------------------------

@@ -1078,6 +1078,285 @@ void combine_norms(const double *U, int64_t ld, int nu, const double *Cm, int ld
   }
 }
 
+// ---- block Krylov kernels ----------------------------------------------
+namespace {
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int GRAM_THREADS = 256;
+constexpr int GRAM_S = 16;          // elements per lane group and sweep step
+constexpr int GRAM_MAXBLK = 512;    // partial-sum blocks of the MFMA Gram
+
+// Partial Gram G = X^T Y (X: nx <= 16 NG columns, Y: b <= 16 columns, both
+// of length n, leading dimension ld) on FP64 MFMA (v_mfma_f64_16x16x4f64).
+// In one MFMA the 64 lanes of a wave supply a 16 x 4 block of X^T (lane l:
+// column l mod 16 of the group, element slot l / 16) and the matching 4 x 16
+// block of Y (lane l: column l mod 16, element slot l / 16), so lanes
+// 16q..16q+15 feed the four k slots from their own contiguous run of GRAM_S
+// elements; the 16 x 16 accumulator of group g holds <X_{16g+i}, Y_j> in
+// lane (j + 16 (i mod 4)), register i / 4 (the accumulator layout of this
+// instruction on gfx950, measured with one-hot operands).  Each workgroup
+// adds its waves' accumulators in LDS and writes nx x 16 partial sums.
+template <int NG, typename T>
+__global__ __launch_bounds__(GRAM_THREADS) void k_gram_xy_mfma(const T *__restrict__ X, int64_t ld,
+                                                                int nx, const T *__restrict__ Y,
+                                                                int b, int64_t n,
+                                                                double *__restrict__ partial)
+{
+  __shared__ double red[GRAM_THREADS / 64][NG * 256];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  constexpr int NW = GRAM_THREADS / 64;
+  d4 acc[NG];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) acc[g] = d4{0.0, 0.0, 0.0, 0.0};
+  const int64_t chunk = 4 * GRAM_S;  // elements per wave and step
+  const T *yc = Y + (int64_t)col * ld;
+  const bool yon = col < b;
+  for (int64_t base = ((int64_t)blockIdx.x * NW + wid) * chunk; base < n;
+       base += (int64_t)gridDim.x * NW * chunk) {
+    const int64_t e0 = base + (int64_t)kq * GRAM_S;
+#pragma unroll 4
+    for (int s = 0; s < GRAM_S; ++s) {
+      const int64_t e = e0 + s;
+      const bool in = e < n;
+      const double yv = (yon && in) ? (double)yc[e] : 0.0;
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        const int r = g * 16 + col;
+        const double xv = (r < nx && in) ? (double)X[(int64_t)r * ld + e] : 0.0;
+        acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, yv, acc[g], 0, 0, 0);
+      }
+    }
+  }
+  // accumulator element (i, j) of group g: lane j + 16 (i mod 4), register
+  // i / 4, so register v of this lane holds row i = kq + 4 v, column col
+#pragma unroll
+  for (int g = 0; g < NG; ++g)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[wid][g * 256 + (kq + 4 * v) * 16 + col] = acc[g][v];
+  __syncthreads();
+  for (int t = threadIdx.x; t < nx * 16; t += blockDim.x) {
+    const int r = t >> 4, j = t & 15;
+    double sum = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) sum += red[w][(r >> 4) * 256 + (r & 15) * 16 + j];
+    partial[(int64_t)blockIdx.x * nx * 16 + t] = sum;
+  }
+}
+
+// G[r + c * nx] = sum over blocks of partial slot (r, c), r < nx, c < b
+// (one wavefront per output, fixed order)
+__global__ __launch_bounds__(64) void k_reduce_gram_xy(const double *__restrict__ partial,
+                                                       int nblk, int nx, int b,
+                                                       double *__restrict__ G)
+{
+  const int r = blockIdx.x / b, c = blockIdx.x % b;
+  const double s = lane_sum(partial + r * 16 + c, nblk, (int64_t)nx * 16, threadIdx.x);
+  if (threadIdx.x == 0) G[r + c * nx] = s;
+}
+
+// W <- (W - X H) Rinv over b <= 8 columns: H (nx x b, leading dimension
+// ldh) may be null (no subtraction), Rinv (b x b upper triangular,
+// column-major) may be null (no scaling).  X (nx <= NX columns) and W are
+// distinct columns of one basis.
+template <int NX, typename T>
+__global__ __launch_bounds__(THREADS) void k_block_update(const T *X, int64_t ld, int nx, T *W,
+                                                          int b, const double *__restrict__ H,
+                                                          int ldh,
+                                                          const double *__restrict__ Rinv,
+                                                          int64_t n)
+{
+  constexpr int BM = 8;
+  __shared__ double sH[NX * BM];
+  __shared__ double sR[BM * BM];
+  for (int t = threadIdx.x; t < NX * BM; t += blockDim.x) {
+    const int r = t % NX, c = t / NX;
+    sH[t] = (H && r < nx && c < b) ? H[r + (int64_t)c * ldh] : 0.0;
+  }
+  for (int t = threadIdx.x; t < BM * BM; t += blockDim.x) {
+    const int r = t % BM, c = t / BM;
+    sR[t] = Rinv ? ((r < b && c < b && r <= c) ? Rinv[r + c * b] : 0.0) : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)THREADS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * THREADS) {
+    double w[BM];
+#pragma unroll
+    for (int c = 0; c < BM; ++c) w[c] = c < b ? (double)W[(int64_t)c * ld + i] : 0.0;
+    if (H) {
+      for (int r = 0; r < nx; ++r) {
+        const double x = X[(int64_t)r * ld + i];
+#pragma unroll
+        for (int c = 0; c < BM; ++c) w[c] -= x * sH[r + c * NX];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < BM; ++c) {
+      if (c >= b) break;
+      double o = 0;
+#pragma unroll
+      for (int q = 0; q < BM; ++q)
+        if (q <= c) o += w[q] * sR[q + c * BM];
+      W[(int64_t)c * ld + i] = (T)o;
+    }
+  }
+}
+
+// Cholesky factor R (upper, b x b, column-major) of S = R^T R; false if a
+// pivot is not positive (relative to the largest diagonal entry)
+__device__ bool chol_upper(const double *S, int b, double *R)
+{
+  double dmax = 0;
+  for (int i = 0; i < b; ++i) dmax = fmax(dmax, S[i + i * b]);
+  for (int j = 0; j < b; ++j) {
+    for (int i = 0; i < b; ++i) R[i + j * b] = 0.0;
+  }
+  for (int j = 0; j < b; ++j) {
+    double d = S[j + j * b];
+    for (int k = 0; k < j; ++k) d -= R[k + j * b] * R[k + j * b];
+    if (!(d > 1e-24 * dmax) || !(dmax > 0)) return false;
+    d = sqrt(d);
+    R[j + j * b] = d;
+    for (int i = j + 1; i < b; ++i) {
+      double t = S[j + i * b];
+      for (int k = 0; k < j; ++k) t -= R[k + j * b] * R[k + i * b];
+      R[j + i * b] = t / d;
+    }
+  }
+  return true;
+}
+
+// inverse of an upper triangular R (b x b)
+__device__ void inv_upper(const double *R, int b, double *Ri)
+{
+  for (int j = 0; j < b; ++j) {
+    for (int i = 0; i < b; ++i) Ri[i + j * b] = 0.0;
+    Ri[j + j * b] = 1.0 / R[j + j * b];
+    for (int i = j - 1; i >= 0; --i) {
+      double t = 0;
+      for (int k = i + 1; k <= j; ++k) t += R[i + k * b] * Ri[k + j * b];
+      Ri[i + j * b] = -t / R[i + i * b];
+    }
+  }
+}
+
+// Close a block step (one thread).  mode 0, after the second projection:
+// G1, G2 = [V | W]^T W of the two passes (nx = nc + b rows, b columns);
+// column block col0 of the projected matrix gets h1 + h2 in rows [0, nc) and
+// R in rows [nc, nc + b), where R^T R = W2^T W2 = G2[W] - h2^T h2
+// (Pythagoras, V orthonormal), and Rinv = R^-1.  mode 1, CholQR's second
+// pass: G1 = W^T W (b x b), R2 = chol; the stored R block becomes R2 R and
+// Rinv = R2^-1.  A failed factorisation records the step in flags[1] (if
+// not yet set) and leaves Rinv = identity.
+__global__ void k_block_close(const double *__restrict__ G1, const double *__restrict__ G2, int nc,
+                              int b, double *__restrict__ Hd, int ldh, int col0,
+                              double *__restrict__ Rinv, double *__restrict__ flags, int step,
+                              int mode)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  constexpr int BM = 8;
+  double S[BM * BM], R[BM * BM], Ri[BM * BM];
+  const int nx = nc + b;
+  if (mode == 0) {
+    for (int c = 0; c < b; ++c)
+      for (int r = 0; r < nc; ++r)
+        Hd[r + (int64_t)(col0 + c) * ldh] = G1[r + c * nx] + G2[r + c * nx];
+    for (int c1 = 0; c1 < b; ++c1)
+      for (int c2 = 0; c2 < b; ++c2) {
+        double t = G2[nc + c1 + c2 * nx];
+        for (int r = 0; r < nc; ++r) t -= G2[r + c1 * nx] * G2[r + c2 * nx];
+        S[c1 + c2 * b] = t;
+      }
+  } else {
+    for (int t = 0; t < b * b; ++t) S[t] = G1[t];
+  }
+  // symmetrise
+  for (int c1 = 0; c1 < b; ++c1)
+    for (int c2 = c1 + 1; c2 < b; ++c2) {
+      const double a = 0.5 * (S[c1 + c2 * b] + S[c2 + c1 * b]);
+      S[c1 + c2 * b] = S[c2 + c1 * b] = a;
+    }
+  const bool ok = chol_upper(S, b, R);
+  if (!ok) {
+    // W is left as it is (Rinv = identity) with the stored coupling block
+    // such that W_before = W R_block (identity after a failed first
+    // factorisation); the host repairs the block from there
+    if (flags[1] < 0) flags[1] = step;
+    for (int t = 0; t < b * b; ++t) Rinv[t] = (t % (b + 1)) == 0 ? 1.0 : 0.0;
+    if (mode == 0)
+      for (int c = 0; c < b; ++c)
+        for (int r = 0; r < b; ++r) Hd[nc + r + (int64_t)(col0 + c) * ldh] = r == c ? 1.0 : 0.0;
+    return;
+  }
+  inv_upper(R, b, Ri);
+  for (int t = 0; t < b * b; ++t) Rinv[t] = Ri[t];
+  if (mode == 0) {
+    for (int c = 0; c < b; ++c)
+      for (int r = 0; r < b; ++r) Hd[nc + r + (int64_t)(col0 + c) * ldh] = R[r + c * b];
+  } else {
+    // the R block sits in rows [nc, nc + b) of column block col0 (nc is the
+    // row offset of W in that case)
+    double O[BM * BM];
+    for (int c = 0; c < b; ++c)
+      for (int r = 0; r < b; ++r) O[r + c * b] = Hd[nc + r + (int64_t)(col0 + c) * ldh];
+    for (int c = 0; c < b; ++c)
+      for (int r = 0; r < b; ++r) {
+        double t = 0;
+        for (int k = 0; k < b; ++k) t += R[r + k * b] * O[k + c * b];
+        Hd[nc + r + (int64_t)(col0 + c) * ldh] = t;
+      }
+  }
+}
+}  // namespace
+
+template <typename T>
+void gram_xy(const T *X, int64_t ld, int nx, const T *Y, int b, int64_t n, double *partial,
+             double *G, hipStream_t s)
+{
+  TM_CHECK(nx >= 1 && nx <= 64 && b >= 1 && b <= 16, TM_EINVAL, "block Gram too large");
+  int64_t nb = (n + GRAM_THREADS / 64 * 4 * GRAM_S - 1) / (GRAM_THREADS / 64 * 4 * GRAM_S);
+  if (nb > GRAM_MAXBLK) nb = GRAM_MAXBLK;
+  if (nb < 1) nb = 1;
+  const int ng = (nx + 15) / 16;
+  if (ng == 1)
+    hipLaunchKernelGGL((k_gram_xy_mfma<1, T>), dim3(nb), dim3(GRAM_THREADS), 0, s, X, ld, nx, Y, b, n, partial);
+  else if (ng == 2)
+    hipLaunchKernelGGL((k_gram_xy_mfma<2, T>), dim3(nb), dim3(GRAM_THREADS), 0, s, X, ld, nx, Y, b, n, partial);
+  else if (ng == 3)
+    hipLaunchKernelGGL((k_gram_xy_mfma<3, T>), dim3(nb), dim3(GRAM_THREADS), 0, s, X, ld, nx, Y, b, n, partial);
+  else
+    hipLaunchKernelGGL((k_gram_xy_mfma<4, T>), dim3(nb), dim3(GRAM_THREADS), 0, s, X, ld, nx, Y, b, n, partial);
+  TM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_gram_xy, dim3(nx * b), dim3(64), 0, s, partial, (int)nb, nx, b, G);
+  TM_LAUNCH_CHECK();
+}
+
+size_t gram_xy_partial_doubles() { return (size_t)GRAM_MAXBLK * 64 * 16; }
+
+template <typename T>
+void block_update(const T *X, int64_t ld, int nx, T *W, int b, const double *H, int ldh,
+                  const double *Rinv, int64_t n, hipStream_t s)
+{
+  TM_CHECK(nx >= 0 && nx <= 64 && b >= 1 && b <= 8, TM_EINVAL, "block update too large");
+  const unsigned nb = nblocks(n);
+  const double *h = nx > 0 ? H : nullptr;
+  if (nx <= 16)
+    hipLaunchKernelGGL((k_block_update<16, T>), dim3(nb), dim3(THREADS), 0, s, X, ld, nx, W, b, h, ldh, Rinv, n);
+  else if (nx <= 32)
+    hipLaunchKernelGGL((k_block_update<32, T>), dim3(nb), dim3(THREADS), 0, s, X, ld, nx, W, b, h, ldh, Rinv, n);
+  else
+    hipLaunchKernelGGL((k_block_update<64, T>), dim3(nb), dim3(THREADS), 0, s, X, ld, nx, W, b, h, ldh, Rinv, n);
+  TM_LAUNCH_CHECK();
+}
+
+void block_close(const double *G1, const double *G2, int nc, int b, double *Hd, int ldh, int col0,
+                 double *Rinv, double *flags, int step, int mode, hipStream_t s)
+{
+  TM_CHECK(b >= 1 && b <= 8, TM_EINVAL, "block size above 8");
+  hipLaunchKernelGGL(k_block_close, dim3(1), dim3(64), 0, s, G1, G2, nc, b, Hd, ldh, col0, Rinv,
+                     flags, step, mode);
+  TM_LAUNCH_CHECK();
+}
+
 #define TM_KRY_INST(T)                                                                          \
   template unsigned multidot<T>(const T *, int64_t, int, const T *, int64_t, double *,         \
                                 double *, hipStream_t, bool);                                  \
@@ -1106,6 +1385,14 @@ template void copy<double, double>(const double *, double *, int64_t, hipStream_
 template void copy<float, float>(const float *, float *, int64_t, hipStream_t);
 template void copy<double, float>(const double *, float *, int64_t, hipStream_t);
 template void copy<float, double>(const float *, double *, int64_t, hipStream_t);
+template void gram_xy<double>(const double *, int64_t, int, const double *, int, int64_t, double *,
+                              double *, hipStream_t);
+template void gram_xy<float>(const float *, int64_t, int, const float *, int, int64_t, double *,
+                             double *, hipStream_t);
+template void block_update<double>(const double *, int64_t, int, double *, int, const double *,
+                                   int, const double *, int64_t, hipStream_t);
+template void block_update<float>(const float *, int64_t, int, float *, int, const double *, int,
+                                  const double *, int64_t, hipStream_t);
 
 }  // namespace kry
 }  // namespace ltm

@@ -42,7 +42,9 @@ constexpr double EPS = 2.220446049250313e-16;
 constexpr int SLOT_ROLE = 6;
 constexpr int SLOT_REFINE_B = 18, SLOT_REFINE_PARTIAL = 19, SLOT_REFINE_G = 20,
               SLOT_REFINE_C = 21, SLOT_XR = 22, SLOT_XI = 23, SLOT_CHECK = 24;
-static_assert(SLOT_CHECK < SolverCache::NDEV, "operator cache slots");
+// block Krylov-Schur (block runs use role 0's basis slot)
+constexpr int SLOT_BK_PART = 25, SLOT_BK_G = 26, SLOT_BK_HD = 27, SLOT_BK_Y = 28;
+static_assert(SLOT_BK_Y < SolverCache::NDEV, "operator cache slots");
 
 // A projected-matrix entry read back from the device.  A non-finite value
 // means the device results are invalid (an operator or Krylov kernel
@@ -594,6 +596,448 @@ class KrylovSchur final : public KrylovRun {
   std::vector<double> H_;
 };
 
+// Block Krylov-Schur for a symmetric operator: b vectors per step.  The
+// basis V (columns of leading dimension ldv; element type T as above) grows
+// by blocks: the active block V_A = V[nc-b, nc) is applied, W = T V_A lands
+// in V[nc, nc+b), and W is orthogonalised against V[0, nc) by block
+// classical Gram-Schmidt twice -- the Gram [V | W]^T W of each pass in one
+// sweep on FP64 MFMA (kry::gram_xy), the update W -= V h in one sweep -- then
+// orthonormalised by CholQR twice, with the small factorisations on the
+// device (kry::block_close), so that a cycle needs no host synchronisation
+// until its end.  The projected matrix H = V^T T V is block tridiagonal plus
+// the arrow of the kept Ritz vectors; after a cycle of projected size m the
+// decomposition is T V_m = V_m H_m + V_res R E^T, and the thick restart keeps
+// k Ritz vectors plus the residual block V_res.  Each pass reads the basis
+// once for b vectors, where the single-vector run reads it twice per vector.
+template <typename T>
+class BlockKrylovSchur final : public KrylovRun {
+ public:
+  BlockKrylovSchur(tm_op *op, int nev, int ncv, int b, double tol, int maxit, tm_dtype comp,
+                   uint64_t seed, int verbose)
+      : op_(op), nev_(nev), ncv_(ncv), b_(b), tol_(tol), maxit_(maxit), comp_(comp), seed_(seed),
+        verbose_(verbose), cache_(op->solver_cache()), V_(cache_.dev[0]),
+        part_(cache_.dev[SLOT_BK_PART]), G_(cache_.dev[SLOT_BK_G]), Hd_(cache_.dev[SLOT_BK_HD]),
+        Yd_(cache_.dev[SLOT_BK_Y]), stage_(cache_.pinned[0])
+  {
+    TM_CHECK(b >= 2 && b <= 8, TM_EINVAL, "block size must be in [2, 8]");
+    TM_CHECK(ncv >= nev + b, TM_EINVAL, "block Krylov needs ncv >= nev + block size");
+    n_ = op->ncol;
+    ldv_ = ((n_ + 63) / 64) * 64;
+    mmax_ = ncv + b;          // largest projected size
+    ncap_ = mmax_ + b;        // basis columns
+    TM_CHECK(ncap_ <= 64, TM_EINVAL, "block Krylov basis above 64 vectors");
+    ldh_ = ncap_;
+    V_.ensure(sizeof(T) * ldv_ * ncap_);
+    part_.ensure(sizeof(double) * kry::gram_xy_partial_doubles());
+    // G1, G2, G3 (64 x 16 each), Rinv (8 x 8)
+    G_.ensure(sizeof(double) * (3 * 64 * 16 + 64));
+    Hd_.ensure(sizeof(double) * ((size_t)ldh_ * mmax_ + 8));
+    Yd_.ensure(sizeof(double) * 64 * 64);
+    stage_.ensure(sizeof(double) * 72 * 72);
+    H_.assign((size_t)ldh_ * mmax_, 0.0);
+  }
+  void set_graph(bool) override {}
+  void set_stall_stop(bool v) override { stall_stop_ = v; }
+  int64_t graph_cycles() const override { return 0; }
+  int64_t ldv() const override { return ldv_; }
+  int64_t matvecs() const override { return matvecs_; }
+  int64_t restarts() const override { return restarts_; }
+  bool stalled() const override { return stalled_; }
+  int m() const override { return m_; }
+
+  T *V(int k) const { return V_.as<T>() + (size_t)k * ldv_; }
+  double &H(int i, int j) { return H_[i + (size_t)j * ldh_]; }
+  double *G1() const { return G_.as<double>(); }
+  double *G2() const { return G_.as<double>() + 64 * 16; }
+  double *G3() const { return G_.as<double>() + 2 * 64 * 16; }
+  double *Rinv() const { return G_.as<double>() + 3 * 64 * 16; }
+
+  // CholQR of the b columns at V(c0) on the host (two passes): used for the
+  // start block and for replacement blocks; false if the block is rank
+  // deficient
+  bool cholqr_host(int c0)
+  {
+    hipStream_t s = op_->stream;
+    const int b = b_;
+    for (int pass = 0; pass < 2; ++pass) {
+      kry::gram_xy(V(c0), ldv_, b, V(c0), b, n_, part_.as<double>(), G3(), s);
+      std::vector<double> G(b * b), R(b * b, 0.0), Ri(b * b, 0.0);
+      copy_sync(G.data(), G3(), sizeof(double) * b * b, hipMemcpyDeviceToHost, s);
+      double dmax = 0;
+      for (int j = 0; j < b; ++j) dmax = std::max(dmax, G[j + j * b]);
+      for (int j = 0; j < b; ++j) {
+        double d = G[j + j * b];
+        for (int k = 0; k < j; ++k) d -= R[k + j * b] * R[k + j * b];
+        if (!(d > 1e-24 * dmax) || !(dmax > 0)) {
+          if (verbose_) {
+            std::fprintf(stderr, "[tm_solve] block CholQR pass %d pivot %d = %.6g; Gram:", pass, j, d);
+            for (int t = 0; t < b * b; ++t) std::fprintf(stderr, " %.6g", G[t]);
+            std::fprintf(stderr, "\n");
+          }
+          return false;
+        }
+        d = std::sqrt(d);
+        R[j + j * b] = d;
+        for (int i = j + 1; i < b; ++i) {
+          double t = G[j + i * b];
+          for (int k = 0; k < j; ++k) t -= R[k + j * b] * R[k + i * b];
+          R[j + i * b] = t / d;
+        }
+      }
+      for (int j = 0; j < b; ++j) {
+        Ri[j + j * b] = 1.0 / R[j + j * b];
+        for (int i = j - 1; i >= 0; --i) {
+          double t = 0;
+          for (int k = i + 1; k <= j; ++k) t += R[i + k * b] * Ri[k + j * b];
+          Ri[i + j * b] = -t / R[i + i * b];
+        }
+      }
+      copy_sync(Rinv(), Ri.data(), sizeof(double) * b * b, hipMemcpyHostToDevice, s);
+      kry::block_update<T>(nullptr, ldv_, 0, V(c0), b, nullptr, 0, Rinv(), n_, s);
+    }
+    return true;
+  }
+
+  // random block at V(c0) orthogonal to V[0, c0) and orthonormal
+  void random_block(int c0, uint64_t salt)
+  {
+    hipStream_t s = op_->stream;
+    for (int c = 0; c < b_; ++c)
+      kry::random_uniform(V(c0 + c), n_, seed_ + 7919 * (uint64_t)(c + 1) + salt, s);
+    for (int pass = 0; pass < 2 && c0 > 0; ++pass) {
+      kry::gram_xy(V(0), ldv_, c0, V(c0), b_, n_, part_.as<double>(), G1(), s);
+      kry::block_update<T>(V(0), ldv_, c0, V(c0), b_, G1(), c0, nullptr, n_, s);
+    }
+    TM_CHECK(cholqr_host(c0), TM_ENOCONV, "block Krylov: could not build an orthonormal block");
+  }
+
+  void start(const double *v0)
+  {
+    random_block(0, 0);
+    if (v0) {
+      kry::copy(v0, V(0), n_, op_->stream);
+      TM_CHECK(cholqr_host(0), TM_ENOCONV, "block Krylov: start block is rank deficient");
+    }
+    nc_ = b_;
+    std::fill(H_.begin(), H_.end(), 0.0);
+  }
+
+  // the kernels of one block step at the current nc_ (no host synchronisation)
+  void issue_step(int step)
+  {
+    hipStream_t s = op_->stream;
+    const int b = b_, nc = nc_, col0 = nc - b, nx = nc + b;
+    double *Hd = Hd_.as<double>(), *flags = Hd + (size_t)ldh_ * mmax_;
+    for (int c = 0; c < b; ++c)
+      op_->apply_internal(V(col0 + c), V(nc + c), ldv_, 1, io_, comp_, 0, nullptr, s);
+    matvecs_ += b;
+    kry::gram_xy(V(0), ldv_, nx, V(nc), b, n_, part_.as<double>(), G1(), s);
+    kry::block_update<T>(V(0), ldv_, nc, V(nc), b, G1(), nx, nullptr, n_, s);
+    kry::gram_xy(V(0), ldv_, nx, V(nc), b, n_, part_.as<double>(), G2(), s);
+    kry::block_close(G1(), G2(), nc, b, Hd, ldh_, col0, Rinv(), flags, step, 0, s);
+    kry::block_update<T>(V(0), ldv_, nc, V(nc), b, G2(), nx, Rinv(), n_, s);
+    kry::gram_xy(V(nc), ldv_, b, V(nc), b, n_, part_.as<double>(), G3(), s);
+    kry::block_close(G3(), nullptr, nc, b, Hd, ldh_, col0, Rinv(), flags, step, 1, s);
+    kry::block_update<T>(nullptr, ldv_, 0, V(nc), b, nullptr, 0, Rinv(), n_, s);
+  }
+
+  // grow the projected size to at least ncv
+  void expand()
+  {
+    hipStream_t s = op_->stream;
+    const int b = b_;
+    double *Hd = Hd_.as<double>(), *flags = Hd + (size_t)ldh_ * mmax_;
+    const int nc0 = nc_;
+    kry::fill(flags, 0.0, 1, s);
+    kry::fill(flags + 1, -1.0, 1, s);
+    int steps = 0;
+    std::vector<int> ncs;
+    while (nc_ - b < ncv_) {
+      ncs.push_back(nc_);
+      issue_step(steps++);
+      nc_ += b;
+    }
+    // read the new column blocks and the flags
+    const int c_lo = nc0 - b, c_hi = nc_ - b;  // column range [c_lo, c_hi) of H
+    double *hd = stage_.as<double>();
+    TM_HIP(hipMemcpyAsync(hd, Hd + (size_t)c_lo * ldh_,
+                          sizeof(double) * (size_t)ldh_ * (c_hi - c_lo), hipMemcpyDeviceToHost,
+                          s));
+    double fl[2];
+    TM_HIP(hipMemcpyAsync(fl, flags, sizeof fl, hipMemcpyDeviceToHost, s));
+    TM_HIP(hipStreamSynchronize(s));
+    const int bad = fl[1] < 0 ? steps : (int)fl[1];
+    for (int q = 0; q < std::min(bad + 1, steps); ++q) {
+      const int nc = ncs[q], col0 = nc - b;
+      for (int c = 0; c < b; ++c)
+        for (int i = 0; i < nc + b; ++i)
+          H(i, col0 + c) = finite_or_throw(hd[i + (size_t)(col0 + c - c_lo) * ldh_]);
+    }
+    if (bad < steps) {
+      // (numerical) rank deficiency at step `bad`: repair its block on the
+      // host and continue from the next step
+      if (verbose_)
+        std::fprintf(stderr, "[tm_solve] block step %d rank deficient; repaired\n", bad);
+      nc_ = ncs[bad];
+      repair_block(nc_ - b);
+      nc_ += b;
+      matvecs_ -= (int64_t)b * (steps - bad - 1);
+      expand();
+    }
+  }
+
+  // W = V[nc, nc+b) of the block step with active block col0 = nc - b, and
+  // the stored coupling C (rows [nc, nc+b) of column block col0) with
+  // T V_A = V H_col + W C.  Rank-revealing re-orthonormalisation: with
+  // W^T W = Q diag(l) Q^T the directions l_i > 1e-16 (|H_col|^2 + max l)
+  // are kept (W Q_i / sqrt(l_i), coupling sqrt(l_i) Q_i^T), the others are
+  // replaced by random directions with zero coupling; the block is then
+  // projected once more against V and orthonormalised (CholQR twice), the
+  // projection folded into H_col and the factors into the coupling.
+  void repair_block(int col0)
+  {
+    hipStream_t s = op_->stream;
+    const int b = b_, nc = col0 + b;
+    kry::gram_xy(V(nc), ldv_, b, V(nc), b, n_, part_.as<double>(), G3(), s);
+    std::vector<double> WW(b * b);
+    copy_sync(WW.data(), G3(), sizeof(double) * b * b, hipMemcpyDeviceToHost, s);
+    std::vector<double> Hc((size_t)(nc + b) * b);
+    for (int c = 0; c < b; ++c)
+      for (int i = 0; i < nc + b; ++i) Hc[i + (size_t)c * (nc + b)] = H(i, col0 + c);
+    std::vector<double> lam(b), Q(b * b);
+    for (int c1 = 0; c1 < b; ++c1)
+      for (int c2 = 0; c2 < b; ++c2)
+        WW[c1 + c2 * b] = 0.5 * (WW[c1 + c2 * b] + WW[c2 + c1 * b]);
+    sym_eig(b, WW.data(), lam.data(), Q.data());
+    double hn = 0, lmax = 0;
+    for (int c = 0; c < b; ++c)
+      for (int i = 0; i < nc; ++i) hn += Hc[i + (size_t)c * (nc + b)] * Hc[i + (size_t)c * (nc + b)];
+    for (double l : lam) lmax = std::max(lmax, l);
+    std::vector<int> keep;
+    for (int i = 0; i < b; ++i)
+      if (lam[i] > 1e-16 * (hn + lmax)) keep.push_back(i);
+    const int r = (int)keep.size();
+    // Cm: W <- W Cm over the kept directions; Rp: their coupling (r x b)
+    std::vector<double> Cm((size_t)b * b, 0.0), Rp((size_t)b * b, 0.0);
+    for (int q = 0; q < r; ++q) {
+      const int i = keep[q];
+      const double sl = std::sqrt(lam[i]);
+      for (int t = 0; t < b; ++t) {
+        Cm[t + (size_t)q * b] = Q[t + (size_t)i * b] / sl;
+        Rp[q + (size_t)t * b] = sl * Q[t + (size_t)i * b];
+      }
+    }
+    if (r > 0) {
+      copy_sync(Yd_.get(), Cm.data(), sizeof(double) * b * b, hipMemcpyHostToDevice, s);
+      kry::gemm_vy(V(nc), ldv_, b, Yd_.as<double>(), b, r, V(nc), ldv_, n_, s);
+    }
+    for (int c = r; c < b; ++c)
+      kry::random_uniform(V(nc + c), n_, seed_ + 104729 * (uint64_t)(c + 1 + 17 * restarts_), s);
+    // project the whole block against V (twice) and orthonormalise it
+    std::vector<double> h3((size_t)nc * b, 0.0);
+    for (int pass = 0; pass < 2; ++pass) {
+      kry::gram_xy(V(0), ldv_, nc, V(nc), b, n_, part_.as<double>(), G1(), s);
+      std::vector<double> g((size_t)nc * b);
+      copy_sync(g.data(), G1(), sizeof(double) * nc * b, hipMemcpyDeviceToHost, s);
+      for (size_t t = 0; t < g.size(); ++t) h3[t] += g[t];
+      kry::block_update<T>(V(0), ldv_, nc, V(nc), b, G1(), nc, nullptr, n_, s);
+    }
+    // W_kept = W_new R3 + V h3 with R3 from CholQR (two passes, host)
+    std::vector<double> R3((size_t)b * b, 0.0);
+    for (int i = 0; i < b; ++i) R3[i + i * b] = 1.0;
+    for (int pass = 0; pass < 2; ++pass) {
+      kry::gram_xy(V(nc), ldv_, b, V(nc), b, n_, part_.as<double>(), G3(), s);
+      std::vector<double> G(b * b), R(b * b, 0.0), Ri(b * b, 0.0);
+      copy_sync(G.data(), G3(), sizeof(double) * b * b, hipMemcpyDeviceToHost, s);
+      for (int j = 0; j < b; ++j) {
+        double d = G[j + j * b];
+        for (int k = 0; k < j; ++k) d -= R[k + j * b] * R[k + j * b];
+        TM_CHECK(d > 0, TM_ENOCONV, "block Krylov: repaired block is rank deficient");
+        d = std::sqrt(d);
+        R[j + j * b] = d;
+        for (int i = j + 1; i < b; ++i) {
+          double t = G[j + i * b];
+          for (int k = 0; k < j; ++k) t -= R[k + j * b] * R[k + i * b];
+          R[j + i * b] = t / d;
+        }
+      }
+      for (int j = 0; j < b; ++j) {
+        Ri[j + j * b] = 1.0 / R[j + j * b];
+        for (int i = j - 1; i >= 0; --i) {
+          double t = 0;
+          for (int k = i + 1; k <= j; ++k) t += R[i + k * b] * Ri[k + j * b];
+          Ri[i + j * b] = -t / R[i + i * b];
+        }
+      }
+      copy_sync(Rinv(), Ri.data(), sizeof(double) * b * b, hipMemcpyHostToDevice, s);
+      kry::block_update<T>(nullptr, ldv_, 0, V(nc), b, nullptr, 0, Rinv(), n_, s);
+      std::vector<double> t3((size_t)b * b, 0.0);
+      for (int i = 0; i < b; ++i)
+        for (int j = 0; j < b; ++j)
+          for (int k = 0; k < b; ++k) t3[i + j * b] += R[i + k * b] * R3[k + j * b];
+      R3 = t3;
+    }
+    // T V_A = V Hc + W_old C_old, W_old = W_kept Rp (kept directions only),
+    // W_kept = W_new R3 + V h3 (h3 over the kept and the random columns;
+    // the random ones carry no coupling):
+    //   H_col += h3 Rp C_old, coupling = R3 Rp C_old
+    std::vector<double> Cold((size_t)b * b);
+    for (int c = 0; c < b; ++c)
+      for (int i = 0; i < b; ++i) Cold[i + (size_t)c * b] = H(nc + i, col0 + c);
+    std::vector<double> RC((size_t)b * b, 0.0);
+    for (int i = 0; i < b; ++i)
+      for (int c = 0; c < b; ++c)
+        for (int k = 0; k < b; ++k) RC[i + (size_t)c * b] += Rp[i + (size_t)k * b] * Cold[k + (size_t)c * b];
+    for (int c = 0; c < b; ++c) {
+      for (int i = 0; i < nc; ++i) {
+        double t = 0;
+        for (int k = 0; k < b; ++k) t += h3[i + (size_t)k * nc] * RC[k + (size_t)c * b];
+        H(i, col0 + c) += t;
+      }
+      for (int i = 0; i < b; ++i) {
+        double t = 0;
+        for (int k = 0; k < b; ++k) t += R3[i + (size_t)k * b] * RC[k + (size_t)c * b];
+        H(nc + i, col0 + c) = t;
+      }
+    }
+  }
+
+  Ritz ritz()
+  {
+    const int m = nc_ - b_;
+    m_ = m;
+    std::vector<double> A((size_t)m * m);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) A[i + (size_t)j * m] = 0.5 * (H(i, j) + H(j, i));
+    std::vector<double> w(m), Z((size_t)m * m);
+    sym_eig(m, A.data(), w.data(), Z.data());
+    std::vector<int> idx(m);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int c) { return std::fabs(w[a]) > std::fabs(w[c]); });
+    Ritz r;
+    r.wr.resize(m);
+    r.wi.assign(m, 0.0);
+    r.Zr.resize((size_t)m * m);
+    r.Zi.assign((size_t)m * m, 0.0);
+    r.res.resize(m);
+    for (int c = 0; c < m; ++c) {
+      r.wr[c] = w[idx[c]];
+      for (int i = 0; i < m; ++i) r.Zr[i + (size_t)c * m] = Z[i + (size_t)idx[c] * m];
+      // residual ||R z_last||, R = H[m .. m+b, m-b .. m)
+      double rr = 0;
+      for (int i = 0; i < b_; ++i) {
+        double t = 0;
+        for (int q = 0; q < b_; ++q) t += H(m + i, m - b_ + q) * r.Zr[(m - b_ + q) + (size_t)c * m];
+        rr += t * t;
+      }
+      r.res[c] = std::sqrt(rr);
+    }
+    return r;
+  }
+
+  int nconv(const Ritz &r, double tol) const
+  {
+    int nc = 0;
+    const double e23 = std::pow(EPS, 2.0 / 3.0);
+    for (int i = 0; i < nev_; ++i)
+      if (r.res[i] <= tol * std::max(e23, std::fabs(r.wr[i]))) ++nc;
+    return nc;
+  }
+
+  void restart(const Ritz &r, int k)
+  {
+    const int m = m_, b = b_;
+    hipStream_t s = op_->stream;
+    double *hy = stage_.as<double>();
+    for (int c = 0; c < k; ++c)
+      for (int i = 0; i < m; ++i) hy[i + (size_t)c * m] = r.Zr[i + (size_t)c * m];
+    TM_HIP(hipMemcpyAsync(Yd_.get(), hy, sizeof(double) * m * k, hipMemcpyHostToDevice, s));
+    kry::gemm_vy(V(0), ldv_, m, Yd_.as<double>(), m, k, V(0), ldv_, n_, s);
+    for (int c = 0; c < b; ++c) kry::copy(V(m + c), V(k + c), n_, s);
+    std::vector<double> Rm((size_t)b * b);
+    for (int i = 0; i < b; ++i)
+      for (int q = 0; q < b; ++q) Rm[i + (size_t)q * b] = H(m + i, m - b + q);
+    std::fill(H_.begin(), H_.end(), 0.0);
+    for (int c = 0; c < k; ++c) {
+      H(c, c) = r.wr[c];
+      for (int i = 0; i < b; ++i) {
+        double t = 0;
+        for (int q = 0; q < b; ++q) t += Rm[i + (size_t)q * b] * r.Zr[(m - b + q) + (size_t)c * m];
+        H(k + i, c) = t;
+      }
+    }
+    nc_ = k + b;
+    ++restarts_;
+    // the staging buffer is next written by a stream-ordered read-back
+  }
+
+  int run(Ritz &out, const double *v0) override
+  {
+    start(v0);
+    int nc = 0;
+    double best = 1e300;
+    for (int it = 0; it < maxit_; ++it) {
+      expand();
+      out = ritz();
+      nc = nconv(out, tol_);
+      double worst = 0;
+      for (int i = 0; i < nev_; ++i)
+        worst = std::max(worst, out.res[i] / std::max(std::fabs(out.wr[i]), 1e-300));
+      if (verbose_)
+        std::fprintf(stderr,
+                     "[tm_solve] block cycle %d matvecs %lld m %d lambda0 %.15g worst %.3e "
+                     "nconv %d\n",
+                     it, (long long)matvecs_, m_, out.wr[0], worst, nc);
+      if (nc >= nev_) break;
+      if (stall_stop_ && worst < 1e-4 && worst > 0.7 * best) {
+        stalled_ = true;
+        break;
+      }
+      best = std::min(best, worst);
+      const int m = m_;
+      int k = nev_ + std::min(nc, (m - nev_) / 2);
+      k = std::max(k, (nev_ + m) / 2);
+      k = std::min(k, m - b_);
+      k = std::max(k, nev_);
+      restart(out, k);
+    }
+    return nc;
+  }
+
+  void ritz_vectors(const Ritz &r, int cnt, double *Xr, double *Xi) override
+  {
+    const int m = m_;
+    hipStream_t s = op_->stream;
+    std::vector<double> Z((size_t)m * cnt);
+    for (int c = 0; c < cnt; ++c)
+      for (int t = 0; t < m; ++t) Z[t + (size_t)c * m] = r.Zr[t + (size_t)c * m];
+    TM_HIP(hipMemcpyAsync(Yd_.get(), Z.data(), sizeof(double) * m * cnt, hipMemcpyHostToDevice, s));
+    kry::gemm_vy(V(0), ldv_, m, Yd_.as<double>(), m, cnt, Xr, ldv_, n_, s);
+    TM_HIP(hipStreamSynchronize(s));
+    if (Xi) kry::fill(Xi, 0.0, ldv_ * cnt, s);
+  }
+
+ private:
+  tm_op *op_;
+  int nev_, ncv_, b_;
+  double tol_;
+  int maxit_;
+  tm_dtype comp_;
+  const tm_dtype io_ = sizeof(T) == 4 ? TM_F32 : TM_F64;
+  uint64_t seed_;
+  int verbose_;
+  SolverCache &cache_;
+  DevBuf &V_, &part_, &G_, &Hd_, &Yd_;
+  PinnedBuf &stage_;
+  int64_t n_ = 0, ldv_ = 0;
+  int mmax_ = 0, ncap_ = 0, ldh_ = 0;
+  int nc_ = 0, m_ = 0;
+  int64_t matvecs_ = 0, restarts_ = 0;
+  bool stall_stop_ = false, stalled_ = false;
+  std::vector<double> H_;
+};
+
 // Cholesky of a small SPD matrix (k x k, column-major), lower factor in place.
 bool cholesky(int k, std::vector<double> &G)
 {
@@ -842,7 +1286,16 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
   // pfunc.hpp:87-109: up to maxtrial attempts, ncv += 2 on failure
   for (;;) {
     const double tol1 = mixed ? std::max(o.mixed_tol, o.tol) : o.tol;
-    if (mixed && o.basis_f32)
+    // the block form for symmetric operators whose dimension is well above
+    // the basis (smaller ones run the single-vector form)
+    const bool blk = o.block > 1 && op->symmetric && n >= 8 * (int64_t)(ncv + 2 * o.block);
+    if (blk && mixed && o.basis_f32)
+      ks.reset(new BlockKrylovSchur<float>(op, nev, ncv, o.block, tol1, o.maxit, TM_F32, o.seed,
+                                           o.verbose));
+    else if (blk)
+      ks.reset(new BlockKrylovSchur<double>(op, nev, ncv, o.block, tol1, o.maxit,
+                                            mixed ? TM_F32 : TM_F64, o.seed, o.verbose));
+    else if (mixed && o.basis_f32)
       ks.reset(new KrylovSchur<float>(op, nev, ncv, tol1, o.maxit, TM_F32, 0, o.seed, o.verbose,
                                       0));
     else

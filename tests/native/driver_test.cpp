@@ -343,6 +343,41 @@ void sc_refine_unmet()
   tm_op_destroy(op);
 }
 
+// Block Krylov-Schur (b vectors per step), FP64 and mixed precision, with
+// repeated solves, against the known spectrum; and a start block that spans
+// an invariant subspace early (breakdown: a random replacement block).
+void sc_block()
+{
+  const int n = 240;
+  std::vector<double> w;
+  auto A = random_sym(n, 31, &w, 0.9);
+  tm_op *op = dense_op(A, n, true);
+  for (int b : {2, 4}) {
+    tm_solve_opts o = opts(4, 12);
+    o.block = b;
+    Solve a = run(op, o);
+    expect_evals(a, w, 4, 1e-10, ("block FP64 b=" + std::to_string(b)).c_str());
+    expect(a.out.resid_check <= 1e-6, "block explicit residual");
+    tm_solve_opts om = opts(4, 12, TM_PREC_MIXED);
+    om.block = b;
+    Solve c = run(op, om);
+    expect_evals(c, w, 4, 1e-10, ("block mixed b=" + std::to_string(b)).c_str());
+  }
+  tm_op_destroy(op);
+  // a matrix of rank 6: the block Krylov space is invariant after two steps
+  std::vector<double> d(n, 0.0);
+  for (int i = 0; i < 6; ++i) d[i] = 10.0 - i;
+  std::vector<double> w2;
+  auto B = random_sym(n, 37, &w2, 0.0, &d);
+  tm_op *op2 = dense_op(B, n, true);
+  tm_solve_opts o = opts(2, 10);
+  o.block = 4;
+  o.verbose = std::getenv("TM_SCENARIO_VERBOSE") ? 1 : 0;
+  Solve r = run(op2, o);
+  expect_evals(r, w2, 2, 1e-10, "block with breakdown");
+  tm_op_destroy(op2);
+}
+
 // Concurrent solves of several operators from host threads (the batch
 // driver), each against its dense spectrum.
 void sc_batch()
@@ -384,7 +419,7 @@ const Scenario SCENARIOS[] = {
     {"graph_reuse", sc_graph_reuse},     {"nonsym_left", sc_nonsym_left},
     {"apply_dev_streams", sc_apply_dev_streams}, {"alloc_in_capture", sc_alloc_in_capture},
     {"tune_families", sc_tune_families}, {"verify_rejects", sc_verify_rejects},
-    {"refine_unmet", sc_refine_unmet}, {"batch", sc_batch},
+    {"refine_unmet", sc_refine_unmet}, {"block", sc_block}, {"batch", sc_batch},
 };
 
 int scenarios(const char *only)

@@ -250,6 +250,122 @@ void combine_norms(const double *U, int64_t ld, int nu, const double *Cm, int ld
     }
   });
 }
+template <typename T>
+void gram_xy(const T *X, int64_t ld, int nx, const T *Y, int b, int64_t n, double *partial,
+             double *G, hipStream_t s)
+{
+  TM_CHECK(nx >= 1 && nx <= 64 && b >= 1 && b <= 16, TM_EINVAL, "block Gram too large");
+  fake_enqueue(s, [=] {
+    for (int c = 0; c < b; ++c)
+      for (int r = 0; r < nx; ++r) {
+        double t = 0;
+        for (int64_t i = 0; i < n; ++i) t += (double)X[r * ld + i] * (double)Y[c * ld + i];
+        partial[r + c * nx] = t;
+      }
+  });
+  fake_enqueue(s, [=] {
+    for (int t = 0; t < nx * b; ++t) G[t] = partial[t];
+  });
+}
+size_t gram_xy_partial_doubles() { return 64 * 16; }
+template <typename T>
+void block_update(const T *X, int64_t ld, int nx, T *W, int b, const double *H, int ldh,
+                  const double *Rinv, int64_t n, hipStream_t s)
+{
+  fake_enqueue(s, [=] {
+    std::vector<double> w(b), o(b);
+    for (int64_t i = 0; i < n; ++i) {
+      for (int c = 0; c < b; ++c) w[c] = W[c * ld + i];
+      if (H && nx > 0)
+        for (int r = 0; r < nx; ++r)
+          for (int c = 0; c < b; ++c) w[c] -= (double)X[r * ld + i] * H[r + (int64_t)c * ldh];
+      for (int c = 0; c < b; ++c) {
+        double t = 0;
+        for (int q = 0; q <= c; ++q) t += w[q] * (Rinv ? Rinv[q + c * b] : (q == c ? 1.0 : 0.0));
+        o[c] = t;
+      }
+      for (int c = 0; c < b; ++c) W[c * ld + i] = (T)o[c];
+    }
+  });
+}
+void block_close(const double *G1, const double *G2, int nc, int b, double *Hd, int ldh, int col0,
+                 double *Rinv, double *flags, int step, int mode, hipStream_t s)
+{
+  fake_enqueue(s, [=] {
+    const int nx = nc + b;
+    std::vector<double> S(b * b), R(b * b, 0.0), Ri(b * b, 0.0);
+    if (mode == 0) {
+      for (int c = 0; c < b; ++c)
+        for (int r = 0; r < nc; ++r)
+          Hd[r + (int64_t)(col0 + c) * ldh] = G1[r + c * nx] + G2[r + c * nx];
+      for (int c1 = 0; c1 < b; ++c1)
+        for (int c2 = 0; c2 < b; ++c2) {
+          double t = G2[nc + c1 + c2 * nx];
+          for (int r = 0; r < nc; ++r) t -= G2[r + c1 * nx] * G2[r + c2 * nx];
+          S[c1 + c2 * b] = t;
+        }
+    } else {
+      for (int t = 0; t < b * b; ++t) S[t] = G1[t];
+    }
+    double dmax = 0;
+    for (int i = 0; i < b; ++i) dmax = std::fmax(dmax, S[i + i * b]);
+    bool ok = dmax > 0;
+    for (int j = 0; ok && j < b; ++j) {
+      double d = S[j + j * b];
+      for (int k = 0; k < j; ++k) d -= R[k + j * b] * R[k + j * b];
+      if (!(d > 1e-24 * dmax)) {
+        ok = false;
+        break;
+      }
+      d = std::sqrt(d);
+      R[j + j * b] = d;
+      for (int i = j + 1; i < b; ++i) {
+        double t = 0.5 * (S[j + i * b] + S[i + j * b]);
+        for (int k = 0; k < j; ++k) t -= R[k + j * b] * R[k + i * b];
+        R[j + i * b] = t / d;
+      }
+    }
+    if (!ok) {
+      if (flags[1] < 0) flags[1] = step;
+      for (int t = 0; t < b * b; ++t) Rinv[t] = (t % (b + 1)) == 0 ? 1.0 : 0.0;
+      if (mode == 0)
+        for (int c = 0; c < b; ++c)
+          for (int r = 0; r < b; ++r) Hd[nc + r + (int64_t)(col0 + c) * ldh] = r == c ? 1.0 : 0.0;
+      return;
+    }
+    for (int j = 0; j < b; ++j) {
+      Ri[j + j * b] = 1.0 / R[j + j * b];
+      for (int i = j - 1; i >= 0; --i) {
+        double t = 0;
+        for (int k = i + 1; k <= j; ++k) t += R[i + k * b] * Ri[k + j * b];
+        Ri[i + j * b] = -t / R[i + i * b];
+      }
+    }
+    for (int t = 0; t < b * b; ++t) Rinv[t] = Ri[t];
+    if (mode == 0) {
+      for (int c = 0; c < b; ++c)
+        for (int r = 0; r < b; ++r) Hd[nc + r + (int64_t)(col0 + c) * ldh] = R[r + c * b];
+    } else {
+      std::vector<double> O(b * b);
+      for (int c = 0; c < b; ++c)
+        for (int r = 0; r < b; ++r) O[r + c * b] = Hd[nc + r + (int64_t)(col0 + c) * ldh];
+      for (int c = 0; c < b; ++c)
+        for (int r = 0; r < b; ++r) {
+          double t = 0;
+          for (int k = 0; k < b; ++k) t += R[r + k * b] * O[k + c * b];
+          Hd[nc + r + (int64_t)(col0 + c) * ldh] = t;
+        }
+    }
+  });
+}
+template void gram_xy<double>(const double *, int64_t, int, const double *, int, int64_t, double *,
+                              double *, hipStream_t);
+template void gram_xy<float>(const float *, int64_t, int, const float *, int, int64_t, double *,
+                             double *, hipStream_t);
+template void block_update<double>(const double *, int64_t, int, double *, int, const double *,
+                                   int, const double *, int64_t, hipStream_t);
+template void block_update<float>(const float *, int64_t, int, float *, int, const double *, int,
+                                  const double *, int64_t, hipStream_t);
 #define FAKE_KRY_INST(T)                                                                        \
   template unsigned multidot<T>(const T *, int64_t, int, const T *, int64_t, double *,         \
                                 double *, hipStream_t, bool);                                  \

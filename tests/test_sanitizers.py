@@ -35,7 +35,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 NATIVE = os.path.join(HERE, "native")
 ORACLE = os.path.join(os.path.dirname(HERE), "oracle")
 SCENARIOS = ["graph_reuse", "nonsym_left", "apply_dev_streams", "alloc_in_capture",
-             "tune_families", "verify_rejects", "refine_unmet", "batch"]
+             "tune_families", "verify_rejects", "refine_unmet", "batch", "block"]
 BUILDS = ["driver_test", "driver_test_asan", "driver_test_tsan"]
 
 
