@@ -47,6 +47,10 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="host threads of the scan batch")
     ap.add_argument("--profile-tx", action="store_true",
                     help="only a calibration copy and a few T.x applies (for counter runs)")
+    ap.add_argument("--profile-op", default=None,
+                    choices=("ZBT", "ZBT_T", "XDT", "XAT", "XAT3D"),
+                    help="only 5 FP64 and 5 FP32 T.x of one full-space operator of width "
+                         "--L (XAT3D: the 5x5 layer), for kernel traces and counter runs")
     ap.add_argument("--workload", default="reduced",
                     choices=("reduced", "scan", "3d", "grad", "small", "ops", "sample"),
                     help="reduced: the headline line; scan: 96-instance BNNNI batch; "
@@ -62,6 +66,8 @@ def main():
 
     import latttmat_amd as lt
 
+    if args.profile_op:
+        return profile_op(lt, args)
     if args.workload == "scan":
         return bench_scan(lt, args)
     if args.workload == "3d":
@@ -738,6 +744,26 @@ def profile_tx(lt, op):
     op.time_apply(lt.TM_F64, lt.TM_F32, warmup=0, reps=5)
     op.time_apply(lt.TM_F64, lt.TM_F64, warmup=0, reps=5)
     log("[bench] profile-tx done: T.x fp32/fp32, fp32/fp64-vectors, fp64 x5 each")
+
+
+def profile_op(lt, args):
+    """Counter-run workload for one full-space operator: 5 T.x in FP64 and 5
+    with FP32 compute (FP32 vectors), after one untimed apply of each (the
+    autotuner's choice is made there)."""
+    kind, L = args.profile_op, args.L
+    if kind in ("ZBT", "ZBT_T"):
+        op = lt.TransferMatrix("B", 2, 3, L, J=0.6, kappa=0.8, kappa2=0.5)
+    elif kind == "XDT":
+        op = lt.TransferMatrix("D", 2, 1, L, J=0.6, kappa=0.7, h=0.1)
+    elif kind == "XAT":
+        op = lt.TransferMatrix("A", 2, 1, L, J=0.4, kappa=0.5)
+    else:
+        op = lt.TransferMatrix("A", 3, 1, (5, 5), J=0.25, kappa=0.4, flags=lt.TM_FLAG_FULLSPACE)
+    tr = kind == "ZBT_T"
+    for io in (lt.TM_F64, lt.TM_F32):
+        ms = op.time_apply(io, io, transpose=tr, warmup=1, reps=5)
+        log(f"[bench] profile-op {kind} L={L} {'fp64' if io == lt.TM_F64 else 'fp32'}: "
+            f"{ms:.4f} ms per T.x")
 
 
 def cpu_model():

@@ -15,6 +15,8 @@
 // in LDS, applies its gates in order, and writes the tile back.  The first
 // pass builds Z from x on the fly (Tv^1/2 x placed at the right aux/lastbit)
 // and the last pass contracts over (lastbit, aux) and applies Tv^1/2.
+#include <type_traits>
+
 #include "tm_device.hpp"
 
 namespace ltm {
@@ -48,6 +50,11 @@ struct SweepArgs {
   double b1[2], b2[2], b3[2];
   double outfac;    // XDT: normfactor; ZBT: 1 (folded into the diagonal)
   double db1[2], db2[2], db3[2], doutfac;  // derivatives (dual-number kernels only)
+  // ZBT gate coefficients of the FP64 and FP32 kernels, computed on the
+  // host (wave-uniform kernel arguments): Q[p][s], kap[s] and rho2 of the
+  // ratio form (zbt_gate); the dual-number kernels build theirs on the device
+  double zQ[16], zkap[4], zr2;
+  float fQ[16], fkap[4], fr2, fb1[2], fb2[2];
 };
 
 __device__ __forceinline__ uint32_t ins0(uint32_t q, int pos)
@@ -190,6 +197,112 @@ __device__ __forceinline__ void zbt_gate(C v[4][4], int fl, int fr, const ZCoef<
     for (int s = 0; s < 4; ++s) v[p][s] = o[p][s];
 }
 
+// One column of a ZBT gate, in place and transposed: the four inputs
+// in[aq][p] (aq = (al, ar)) of a fixed old bit pair p become the outputs
+// out[p][s] (s = (sl, sr)), written back to the same four slots in s order
+// (slot aq holds s = aq afterwards).  qf[s] = Q[p][s] F[s].
+template <typename C, int DIR>
+__device__ __forceinline__ void zbt_col(C &v0, C &v1, C &v2, C &v3, C q0, C q1, C q2, C q3,
+                                        C r2, C k0, C k1, C k2, C k3)
+{
+  const C w00 = v0 + r2 * v1, w01 = v1 + r2 * v0;  // w[al = 0][sr]
+  const C w10 = v2 + r2 * v3, w11 = v3 + r2 * v2;  // w[al = 1][sr]
+  if (DIR == 0) {
+    v0 = q0 * (w00 + k0 * w10);
+    v1 = q1 * (w01 + k1 * w11);
+    v2 = q2 * (w00 + k2 * w10);
+    v3 = q3 * (w01 + k3 * w11);
+  } else {
+    v0 = q0 * (w00 + r2 * w10);
+    v1 = q1 * (w01 + r2 * w11);
+    v2 = q2 * (w10 + r2 * w00);
+    v3 = q3 * (w11 + r2 * w01);
+  }
+}
+
+template <typename C>
+struct ZArg;
+template <>
+struct ZArg<double> {
+  static __device__ __forceinline__ double Q(const SweepArgs &a, int i) { return a.zQ[i]; }
+  static __device__ __forceinline__ double kap(const SweepArgs &a, int i) { return a.zkap[i]; }
+  static __device__ __forceinline__ double r2(const SweepArgs &a) { return a.zr2; }
+  static __device__ __forceinline__ double b1(const SweepArgs &a, int i)
+  {
+    return i ? a.b1[1] : a.b1[0];
+  }
+  static __device__ __forceinline__ double b2(const SweepArgs &a, int i)
+  {
+    return i ? a.b2[1] : a.b2[0];
+  }
+};
+template <>
+struct ZArg<float> {
+  static __device__ __forceinline__ float Q(const SweepArgs &a, int i) { return a.fQ[i]; }
+  static __device__ __forceinline__ float kap(const SweepArgs &a, int i) { return a.fkap[i]; }
+  static __device__ __forceinline__ float r2(const SweepArgs &a) { return a.fr2; }
+  static __device__ __forceinline__ float b1(const SweepArgs &a, int i)
+  {
+    return i ? a.fb1[1] : a.fb1[0];
+  }
+  static __device__ __forceinline__ float b2(const SweepArgs &a, int i)
+  {
+    return i ? a.fb2[1] : a.fb2[0];
+  }
+};
+
+// F[s] = b2[sr ^ fr] b2[sl ^ fl] (times b1[sl ^ fr] for T^T) for run-time
+// controls (fl, fr)
+template <typename C, int DIR>
+__device__ __forceinline__ void zbt_F(C (&F)[4], int fl, int fr, const SweepArgs &a)
+{
+  using Z = ZArg<C>;
+  const C ur0 = Z::b2(a, fr), ur1 = Z::b2(a, fr ^ 1);
+  const C ul0 = Z::b2(a, fl), ul1 = Z::b2(a, fl ^ 1);
+  F[0] = ul0 * ur0;
+  F[1] = ul0 * ur1;
+  F[2] = ul1 * ur0;
+  F[3] = ul1 * ur1;
+  if (DIR == 1) {
+    const C g0 = Z::b1(a, fr), g1 = Z::b1(a, fr ^ 1);
+    F[0] *= g0;
+    F[1] *= g0;
+    F[2] *= g1;
+    F[3] *= g1;
+  }
+}
+
+// One ZBT gate on a 16-value item v[aux][bits] with run-time controls; on
+// return v[i][j] holds bits = i, aux = j.
+template <typename C, int DIR>
+__device__ __forceinline__ void zbt_gate_single(C (&v)[4][4], int fl, int fr, const SweepArgs &a)
+{
+  using Z = ZArg<C>;
+  C F[4];
+  zbt_F<C, DIR>(F, fl, fr, a);
+  const C r2 = Z::r2(a);
+  const C k0 = Z::kap(a, 0), k1 = Z::kap(a, 1), k2 = Z::kap(a, 2), k3 = Z::kap(a, 3);
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+    zbt_col<C, DIR>(v[0][p], v[1][p], v[2][p], v[3][p], Z::Q(a, p * 4 + 0) * F[0],
+                    Z::Q(a, p * 4 + 1) * F[1], Z::Q(a, p * 4 + 2) * F[2],
+                    Z::Q(a, p * 4 + 3) * F[3], r2, k0, k1, k2, k3);
+}
+
+// LDS bank swizzle of the tile: position e is stored at e ^ f(e), with f
+// GF(2)-linear from the tile bits [lo, lo + 12) into bits [0, 5): column j
+// is the image of bit lo + j (lo = 4 for 8- and 16-byte elements, 5 for
+// 4-byte ones; the map is triangular, so e -> e ^ f(e) is a bijection that
+// keeps every bit from 5 up).  The columns were chosen so that the access
+// patterns of the passes -- lanes on consecutive states (load, store,
+// contraction) and lanes on consecutive gate items of a gate at any even
+// tile position -- reach 16 (8-byte) / 32 (4-byte) distinct banks per lane
+// group on tiles of 2^11 and more elements.  f is applied through two
+// 64-entry tables over its 6-bit input chunks, built in LDS per workgroup.
+constexpr int SWZ_TAB = 128;
+constexpr uint32_t SWZ_COLS_8[12] = {0x6, 0xb, 0x13, 0x2, 0x12, 0x18, 0x12, 0x8, 0x4, 0x1a, 0xa, 0x5};
+constexpr uint32_t SWZ_COLS_4[12] = {0x6, 0x1b, 0x4, 0x15, 0x3, 0x15, 0x16, 0x1f, 0x15, 0x7, 0x0, 0x0};
+
 template <int MODEL, int DIR, typename C, typename TI, typename TO, int NT>
 // zin and zout are the same aux array (the passes work in place): each
 // workgroup reads its tile before the barrier and writes it back after it,
@@ -207,6 +320,7 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
   const int tileLog = nlbt + a.nal + a.nbits + a.logW;
   const int T = 1 << tileLog;
   C *lds = reinterpret_cast<C *>(smem);
+  char *ldsb = reinterpret_cast<char *>(smem);
   C *stab = lds + T;
   // split: workgroup -> (tile, lastbit) with the lastbit values of a tile on
   // one XCD (their interleaved writes to the state-major aux array then meet
@@ -222,32 +336,44 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
   }
   if (a.first || a.last)
     for (int i = threadIdx.x; i < 192; i += blockDim.x) stab[i] = tab[i];
+  int *swzt = reinterpret_cast<int *>(stab + 192);
+  constexpr int slo = sizeof(C) == 4 ? 5 : 4;
+  for (int i = threadIdx.x; i < SWZ_TAB; i += blockDim.x) {
+    const int chunk = i >> 6, idx = i & 63;
+    int v = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b)
+      if ((idx >> b) & 1) v ^= (int)(sizeof(C) == 4 ? SWZ_COLS_4 : SWZ_COLS_8)[chunk * 6 + b];
+    swzt[i] = v;
+  }
   Coef<C> k;
   for (int i = 0; i < 2; ++i) {
     k.b1[i] = dev::mk<C>(a.b1[i], a.db1[i]);
     k.b2[i] = dev::mk<C>(a.b2[i], a.db2[i]);
     k.b3[i] = dev::mk<C>(a.b3[i], a.db3[i]);
   }
-  const ZCoef<C> zk = make_zcoef<C, DIR>(k);
+  // dual-number kernels: gate coefficients on the device (FP64 and FP32
+  // kernels take theirs from the kernel arguments)
+  ZCoef<C> zk{};
+  if constexpr (sizeof(C) > 8) zk = make_zcoef<C, DIR>(k);
   const int L = a.L;
   const uint64_t innerBlocks = (1ull << a.bitlo) >> a.logW;
   const uint64_t o = tile / innerBlocks, ib = tile % innerBlocks;
   const uint64_t base = (o << (a.bitlo + a.nbits)) | (ib << a.logW);
   const int W = 1 << a.logW;
-  const int pmask = (1 << a.nbits) - 1;
   const int posAux = a.logW + a.nbits;
   const int posLb = posAux + a.nal;
   // The aux array is state-major in HBM: Z[(P << lgz) | z], z = (lastbit,
   // aux), so the 2^lgz values of a state are contiguous and a tile's runs
-  // are W * 2^lgz elements long.  The load and store loops walk the tile in
-  // that order; the LDS tile keeps (aux, lastbit) in its high bits and XORs
-  // them into the low bits (swz) so that those walks stay conflict-free.
+  // are W * 2^lgz elements long.  In the tile, (aux, lastbit) are the high
+  // bits (from posAux) of the position, stored swizzled (swz).
   const int lgz = a.nlbl + a.nal;
   const int lgt = nlbt + a.nal;  // aux (and lastbit) bits in the tile
   const int nz = 1 << lgt;
   const int zoff = wlb << a.nal;  // this workgroup's lastbit in the HBM z index
-  const int zx = (1 << (lgt < posAux ? lgt : posAux)) - 1;
-  auto swz = [&](int e) { return e ^ ((e >> posAux) & zx); };
+  auto swz = [&](int e) __attribute__((always_inline)) {
+    return e ^ swzt[(e >> slo) & 63] ^ swzt[64 + ((e >> (slo + 6)) & 63)];
+  };
   C sin = 1, sout = 1;
   if (scale) {
     if (a.first) sin = (C)(*scale);
@@ -256,59 +382,71 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
   __syncthreads();
 
   // ---- load / build the tile ----
-  // UNR independent loads per thread are issued before the LDS stores, so
-  // a wave keeps several HBM requests in flight (one workgroup per CU).
-  constexpr int UNR = 8;
-  for (int g0 = threadIdx.x; g0 < T; g0 += blockDim.x * UNR) {
-    C v[UNR];
+  // One state per thread and step: its nz values are contiguous in HBM
+  // (Z[(P << lgz) | z]) and sit at swz(es) ^ szt[z] in LDS (es = the state's
+  // tile position; swz is linear, szt[z] = swz(z << posAux)).  A lane group
+  // walks consecutive states.  The first pass writes the single nonzero
+  // value of a state (at z = (lastbit, aux) of its own bits) and zeros.
+  const int NS = T >> lgt;
+  int szt[16];
 #pragma unroll
-    for (int uu = 0; uu < UNR; ++uu) {
-      const int g = g0 + uu * blockDim.x;
-      v[uu] = 0;
-      if (g >= T) continue;
-      const int z = g & (nz - 1);
-      const int rest = g >> lgt;
-      const int w = rest & (W - 1);
-      const int pl = rest >> a.logW;
-      const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
-      if (a.first) {
-        const int aux = z & (a.na - 1);
-        const int lb = (z | zoff) >> a.nal;
-        if (MODEL == 0) {
-          if (a.bc == 0) {
-            const int auxw = (int)((P >> (L - 1)) & 1);
-            if ((int)(P & 1) == lb && aux == auxw)
-              v[uu] = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P);
-          } else {
-            // confined ends, TMat/tmat.cpp:440-458
-            const int top = (int)(P >> (L - 1));
-            const int want = a.bc == 5 ? 0 : 1;
-            if (aux == want) {
-              const int low = (int)(P & 1) ^ (a.bc == 5 ? 0 : 1);
-              v[uu] = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P) * pick(k.b2, low) *
-                      pick(k.b2, top);
-            }
-          }
+  for (int z = 0; z < 16; ++z) szt[z] = z < nz ? swz(z << posAux) : 0;
+  auto state_P = [&](int es) __attribute__((always_inline)) {
+    return base | ((uint64_t)(es >> a.logW) << a.bitlo) | (uint64_t)(es & (W - 1));
+  };
+  // Passes after the first move each state's nz values as CB-byte chunks on
+  // consecutive lanes (LAM lanes per state), so one load or store
+  // instruction of a wave covers whole contiguous 128-byte lines instead of
+  // one 16-byte piece of 64 different lines.
+  auto fill = [&](auto nzc) __attribute__((always_inline)) {
+    constexpr int NZ = decltype(nzc)::value;
+    if (a.first) {
+#pragma unroll 2
+      for (int es = threadIdx.x; es < NS; es += blockDim.x) {
+        const uint64_t P = state_P(es);
+        const int ses = swz(es);
+        int zi = -1;
+        C val = 0;
+        if (MODEL == 0 && a.bc != 0) {
+          // confined ends, TMat/tmat.cpp:440-458
+          const int top = (int)(P >> (L - 1));
+          const int low = (int)(P & 1) ^ (a.bc == 5 ? 0 : 1);
+          zi = a.bc == 5 ? 0 : 1;
+          val = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P) * pick(k.b2, low) *
+                pick(k.b2, top);
         } else {
-          const int auxw = (int)((P >> (L - 2)) & 3);
-          if ((int)(P & 3) == lb && aux == auxw)
-            v[uu] = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P);
+          // periodic: lastbit = the low bit(s), aux = the top bit(s)
+          const int lb = MODEL == 0 ? (int)(P & 1) : (int)(P & 3);
+          const int aux = MODEL == 0 ? (int)((P >> (L - 1)) & 1) : (int)((P >> (L - 2)) & 3);
+          if (nlbt == 0) zi = lb == wlb ? aux : -1;
+          else zi = (lb << a.nal) | aux;
+          if (zi >= 0) val = (C)x[P] * sin * dev::diag_value<C>(a.dg, stab, P);
         }
-      } else {
-        v[uu] = zin[(P << lgz) | (uint64_t)(z | zoff)];
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) lds[ses ^ szt[z]] = z == zi ? val : C(0);
+      }
+    } else {
+      constexpr int BS = NZ * (int)sizeof(C);
+      constexpr int CB = BS < 16 ? BS : 16;
+      constexpr int LAM = BS / CB, VPC = CB / (int)sizeof(C);
+#pragma unroll 4
+      for (int i = threadIdx.x; i < NS * LAM; i += blockDim.x) {
+        const int es = i / LAM, j = i % LAM;
+        const uint64_t P = state_P(es);
+        const C *src = static_cast<const C *>(__builtin_assume_aligned(
+            zin + ((P << lgz) | (uint64_t)zoff) + j * VPC, CB));
+        C v[VPC];
+#pragma unroll
+        for (int u = 0; u < VPC; ++u) v[u] = src[u];
+        const int sa = swz(es) ^ swz((j * VPC) << posAux);
+#pragma unroll
+        for (int u = 0; u < VPC; ++u) lds[sa ^ szt[u]] = v[u];
       }
     }
-#pragma unroll
-    for (int uu = 0; uu < UNR; ++uu) {
-      const int g = g0 + uu * blockDim.x;
-      if (g >= T) continue;
-      const int z = g & (nz - 1);
-      const int rest = g >> lgt;
-      const int w = rest & (W - 1);
-      const int pl = rest >> a.logW;
-      lds[swz(w | (pl << a.logW) | (z << posAux))] = v[uu];
-    }
-  }
+  };
+  if (MODEL == 1 && nz == 16) fill(std::integral_constant<int, 16>{});
+  else if (nz == 4) fill(std::integral_constant<int, 4>{});
+  else fill(std::integral_constant<int, 2>{});
   __syncthreads();
 
   // ---- gates ----
@@ -320,69 +458,93 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
     if (MODEL == 0) {
       const int jb = a.logW + (t - a.bitlo);  // tile position of state bit t
       const int nitems = T >> 2;
-      int off[2][2];
+      int offB[2][2];  // byte offsets of the item's values
 #pragma unroll
       for (int ax = 0; ax < 2; ++ax)
 #pragma unroll
-        for (int u = 0; u < 2; ++u) off[ax][u] = swz((ax << posAux) | (u << jb));
+        for (int u = 0; u < 2; ++u) offB[ax][u] = swz((ax << posAux) | (u << jb)) * (int)sizeof(C);
+      // the control (state bit t + 1): a tile bit of the item while the
+      // next gate is in this pass, else a bit of the workgroup's base (or
+      // lastbit / the confined end for the last gate)
+      const bool cin = g + 1 < a.ng;
+      const int cbase = t + 1 < L ? (int)((base >> (t + 1)) & 1) : 0;
       for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
         // insert zeros at the two varying positions (ascending)
         const uint32_t b0 = ins0(ins0((uint32_t)q, jb), posAux);
-        const int lb = a.lbsplit ? wlb : (int)(b0 >> posLb);
-        const int w = b0 & (W - 1);
-        const int pl = (b0 >> a.logW) & pmask;
-        const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
         int c;
-        if (t + 1 < L) c = (int)((P >> (t + 1)) & 1);
-        else c = (a.bc == 0) ? lb : 0;
-        const int sb = swz((int)b0);
+        if (cin) c = (int)((b0 >> (jb + 1)) & 1);
+        else if (t + 1 < L) c = cbase;
+        else c = (a.bc == 0) ? (a.lbsplit ? wlb : (int)(b0 >> posLb)) : 0;
+        int sbB = swz((int)b0) * (int)sizeof(C);
+        // keep the byte scaling on the item base (one XOR per access; the
+        // compiler would otherwise scale each access after the XOR)
+        asm volatile("" : "+v"(sbB));
         C v[2][2];
 #pragma unroll
         for (int ax = 0; ax < 2; ++ax)
 #pragma unroll
-          for (int u = 0; u < 2; ++u) v[ax][u] = lds[sb ^ off[ax][u]];
+          for (int u = 0; u < 2; ++u) v[ax][u] = *reinterpret_cast<C *>(ldsb + (sbB ^ offB[ax][u]));
         xdt_gate(v, c, k);
 #pragma unroll
         for (int ax = 0; ax < 2; ++ax)
 #pragma unroll
-          for (int u = 0; u < 2; ++u) lds[sb ^ off[ax][u]] = v[ax][u];
+          for (int u = 0; u < 2; ++u) *reinterpret_cast<C *>(ldsb + (sbB ^ offB[ax][u])) = v[ax][u];
       }
     } else {
       const int jb = a.logW + (2 * t - a.bitlo);  // tile position of bit 2t
       const int nitems = T >> 4;
-      int off[4][4];
+      int offB[4][4];  // byte offsets of the item's values
 #pragma unroll
       for (int ax = 0; ax < 4; ++ax)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) off[ax][u] = swz((ax << posAux) | (u << jb));
+        for (int u = 0; u < 4; ++u) offB[ax][u] = swz((ax << posAux) | (u << jb)) * (int)sizeof(C);
+      // the controls (state bits 2t+2, 2t+3): tile bits of the item while
+      // the next gate is in this pass, else bits of the workgroup's base (or
+      // lastbit for the last gate)
+      const bool cin = g + 1 < a.ng;
+      const int frb = 2 * t + 2 < L ? (int)((base >> (2 * t + 2)) & 1) : 0;
+      const int flb = 2 * t + 2 < L ? (int)((base >> (2 * t + 3)) & 1) : 0;
       for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
         uint32_t b0 = ins0((uint32_t)q, jb);
         b0 = ins0(b0, jb + 1);
         b0 = ins0(b0, posAux);
         b0 = ins0(b0, posAux + 1);
-        const int lb = a.lbsplit ? wlb : (int)(b0 >> posLb);
-        const int w = b0 & (W - 1);
-        const int pl = (b0 >> a.logW) & pmask;
-        const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
         int fl, fr;
-        if (2 * t + 2 < L) {
-          fr = (int)((P >> (2 * t + 2)) & 1);
-          fl = (int)((P >> (2 * t + 3)) & 1);
+        if (cin) {
+          fr = (int)((b0 >> (jb + 2)) & 1);
+          fl = (int)((b0 >> (jb + 3)) & 1);
+        } else if (2 * t + 2 < L) {
+          fr = frb;
+          fl = flb;
         } else {
+          const int lb = a.lbsplit ? wlb : (int)(b0 >> posLb);
           fl = lb >> 1;
           fr = lb & 1;
         }
-        const int sb = swz((int)b0);
+        int sbB = swz((int)b0) * (int)sizeof(C);
+        asm volatile("" : "+v"(sbB));  // one XOR per access (see the XDT loop)
+        auto at = [&](int ax, int u) -> C & {
+          return *reinterpret_cast<C *>(ldsb + (sbB ^ offB[ax][u]));
+        };
         C v[4][4];
 #pragma unroll
         for (int ax = 0; ax < 4; ++ax)
 #pragma unroll
-          for (int u = 0; u < 4; ++u) v[ax][u] = lds[sb ^ off[ax][u]];
-        zbt_gate<C, DIR>(v, fl, fr, zk);
+          for (int u = 0; u < 4; ++u) v[ax][u] = at(ax, u);
+        if constexpr (sizeof(C) > 8) {
+          zbt_gate<C, DIR>(v, fl, fr, zk);
 #pragma unroll
-        for (int ax = 0; ax < 4; ++ax)
+          for (int ax = 0; ax < 4; ++ax)
 #pragma unroll
-          for (int u = 0; u < 4; ++u) lds[sb ^ off[ax][u]] = v[ax][u];
+            for (int u = 0; u < 4; ++u) at(ax, u) = v[ax][u];
+        } else {
+          zbt_gate_single<C, DIR>(v, fl, fr, a);
+          // v[i][j]: bits = i, aux = j
+#pragma unroll
+          for (int ax = 0; ax < 4; ++ax)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) at(ax, u) = v[u][ax];
+        }
       }
     }
     __syncthreads();
@@ -390,24 +552,34 @@ __global__ __launch_bounds__(NT) void k_sweep_pass(const TI *__restrict__ x,
 
   // ---- store ----
   if (!a.last) {
-    for (int g = threadIdx.x; g < T; g += blockDim.x) {
-      const int z = g & (nz - 1);
-      const int rest = g >> lgt;
-      const int w = rest & (W - 1);
-      const int pl = rest >> a.logW;
-      const int e = w | (pl << a.logW) | (z << posAux);
-      const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
-      zout[(P << lgz) | (uint64_t)(z | zoff)] = lds[swz(e)];
-    }
+    auto drain = [&](auto nzc) __attribute__((always_inline)) {
+      constexpr int NZ = decltype(nzc)::value;
+      constexpr int BS = NZ * (int)sizeof(C);
+      constexpr int CB = BS < 16 ? BS : 16;
+      constexpr int LAM = BS / CB, VPC = CB / (int)sizeof(C);
+#pragma unroll 4
+      for (int i = threadIdx.x; i < NS * LAM; i += blockDim.x) {
+        const int es = i / LAM, j = i % LAM;
+        const uint64_t P = state_P(es);
+        const int sa = swz(es) ^ swz((j * VPC) << posAux);
+        C *dst = static_cast<C *>(
+            __builtin_assume_aligned(zout + ((P << lgz) | (uint64_t)zoff) + j * VPC, CB));
+#pragma unroll
+        for (int u = 0; u < VPC; ++u) dst[u] = lds[sa ^ szt[u]];
+      }
+    };
+    if (MODEL == 1 && nz == 16) drain(std::integral_constant<int, 16>{});
+    else if (nz == 4) drain(std::integral_constant<int, 4>{});
+    else drain(std::integral_constant<int, 2>{});
   } else {
-    const int NP = 1 << (a.nbits + a.logW);
     const C fac = dev::mk<C>(a.outfac, a.doutfac) * sout;
-    for (int e = threadIdx.x; e < NP; e += blockDim.x) {
-      const int w = e & (W - 1);
-      const int pl = e >> a.logW;
-      const uint64_t P = base | ((uint64_t)pl << a.bitlo) | (uint64_t)w;
+    for (int es = threadIdx.x; es < NS; es += blockDim.x) {
+      const uint64_t P = state_P(es);
+      const int ses = swz(es);
       C acc = 0;
-      for (int z = 0; z < nz; ++z) acc += lds[swz((z << posAux) | e)];
+#pragma unroll
+      for (int z = 0; z < 16; ++z)
+        if (z < nz) acc += lds[ses ^ szt[z]];
       y[P] = (TO)(fac * dev::diag_value<C>(a.dg, stab, P) * acc);
     }
   }
@@ -422,15 +594,22 @@ struct SPass { int t0, ng, bitlo, nbits, logW, split; };
 // the grid (no gate before the last reads it), which frees its bits for one
 // more gate: ZBT at L = 22 then takes two passes instead of three.
 int sweep_log_es(int es) { return es == 4 ? 2 : (es == 8 ? 3 : 4); }
-// default tile exponent first, then the autotuner's other candidate
-std::vector<int> sweep_tile_candidates(int es)
+// default tile exponent first, then the autotuner's other candidates.  At
+// small widths the default tile leaves too few workgroups to fill the chip
+// (2^(L + lgz) values in tiles of 2^d), so smaller tiles are offered down
+// to about 2048 workgroups.
+std::vector<int> sweep_tile_candidates(int es, int L, int lgz)
 {
   const bool big = g_lds_per_block >= (size_t)(136 * 1024);
   const int d = (big ? 17 : 16) - sweep_log_es(es);
+  std::vector<int> c;
   if (es <= 8)
-    return {d, d - 1, d | SWEEP_WIDE, d | SWEEP_NOSPLIT, (d - 1) | SWEEP_NOSPLIT,
-            d | SWEEP_WIDE | SWEEP_NOSPLIT};
-  return {d, d - 1, d | SWEEP_NOSPLIT};
+    c = {d, d - 1, d | SWEEP_WIDE, d | SWEEP_NOSPLIT, (d - 1) | SWEEP_NOSPLIT,
+         d | SWEEP_WIDE | SWEEP_NOSPLIT};
+  else
+    c = {d, d - 1, d | SWEEP_NOSPLIT};
+  for (int t = d - 2; t >= 8 && L + lgz - t <= 11; --t) c.push_back(t);
+  return c;
 }
 
 std::vector<SPass> plan_sweep(int model, int L, int bc, int es, int tl, bool split_ok)
@@ -493,7 +672,7 @@ void run_sweep_pass(const SPass &p, SweepArgs a, const void *x, const C *zin, C 
   dim3 grid;
   if (ntile <= 65535) grid = dim3((unsigned)ntile);
   else grid = dim3(65536, (unsigned)((ntile + 65535) / 65536));
-  const size_t shm = ((size_t(1) << tileLog) + 192) * sizeof(C);
+  const size_t shm = ((size_t(1) << tileLog) + 192) * sizeof(C) + SWZ_TAB * sizeof(int);
   if constexpr (sizeof(C) <= 8) {
     if (wide) {
       hipLaunchKernelGGL((k_sweep_pass<MODEL, DIR, C, TI, TO, 1024>), grid, dim3(1024), shm, s,
@@ -536,6 +715,31 @@ SweepArgs make_args(const SweepPlan &plan)
   }
   a.outfac = MODEL == 0 ? plan.nf : 1.0;
   a.doutfac = MODEL == 0 ? plan.dnf : 0.0;
+  if (MODEL == 1) {
+    // the ratio form of the ZBT gate (see zbt_gate)
+    const double *b1 = plan.b1, *b2 = plan.b2, *b3 = plan.b3;
+    const double r1 = b1[1] / b1[0], r2 = b2[1] / b2[0];
+    const double pre = b1[0] * b2[0] * b2[0];
+    for (int sidx = 0; sidx < 4; ++sidx) {
+      const int sl = sidx >> 1, sr = sidx & 1;
+      a.zkap[sidx] = (sr ? 1.0 / r1 : r1) * (sl ? 1.0 / r2 : r2);
+      for (int pidx = 0; pidx < 4; ++pidx) {
+        const int pl = pidx >> 1, pr = pidx & 1;
+        const double q = DIR == 0 ? b1[sr ^ pl] * b3[sr ^ pr] * b3[sl ^ pl] * pre *
+                                        (sr ? r1 : 1.0) * (sl ? r2 : 1.0)
+                                  : b1[sl ^ pr] * b3[sr ^ pr] * b3[sl ^ pl] * (b2[0] * b2[0]);
+        a.zQ[pidx * 4 + sidx] = q;
+      }
+    }
+    a.zr2 = r2;
+    for (int i = 0; i < 16; ++i) a.fQ[i] = (float)a.zQ[i];
+    for (int i = 0; i < 4; ++i) a.fkap[i] = (float)a.zkap[i];
+    a.fr2 = (float)a.zr2;
+    for (int i = 0; i < 2; ++i) {
+      a.fb1[i] = (float)b1[i];
+      a.fb2[i] = (float)b2[i];
+    }
+  }
   return a;
 }
 
@@ -569,7 +773,8 @@ void sweep_run(const SweepPlan &plan, const void *in, void *out, void *work, con
   const int es = (int)sizeof(C);
   const uint32_t kid = 4 * MODEL + 2 * DIR + (plan.bc != 0 ? 1 : 0);
   // timing candidates rewrites out from in, so only out-of-place applies tune
-  run(autotune_tile(TUNE_SWEEP, kid, plan.L, es, sweep_tile_candidates(es),
+  const int lgz = MODEL == 0 ? (plan.bc == 0 ? 2 : 1) : 4;
+  run(autotune_tile(TUNE_SWEEP, kid, plan.L, es, sweep_tile_candidates(es, plan.L, lgz),
                     plan.tune && in != out, s, run));
 }
 
