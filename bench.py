@@ -277,8 +277,10 @@ def reduced_tx_bytes(M, L, s_in, s_out, c):
     runs the nCol bracelet columns, pass 2 the nRow bracelet rows, over all
     nNk necklace classes.
       k_xtilde       x (M s_in), tvec (M c), x~ written (M c)
-      k_fused_pass1  cmap (nCol 2^n 4), x~ (every orbit once, M c),
-                     A written (nCol x nNk x 16 slots of c)
+      k_fused_pass1  column map: a 4-byte base per (column, O class) and a
+                     1-byte offset per (column, O word) (nCol nNk 4 +
+                     nCol 2^n), the entry table (2^n 4), x~ (every orbit
+                     once, M c), A written (nCol x nNk x 16 slots of c)
       k_fused_pass2  A read (nRow x nNk x 16 c), Elist (M 2), dup*tvec (M c),
                      y written (M s_out)
     At n = 14: nNk = 1182 necklaces, nCol = nRow = 687 bracelets."""
@@ -286,7 +288,8 @@ def reduced_tx_bytes(M, L, s_in, s_out, c):
     nNk = necklaces(n)
     nb = bracelets(n)
     k = {"k_xtilde": M * (s_in + c + c),
-         "k_fused_pass1": nb * (1 << n) * 4 + M * c + nb * nNk * 16 * c,
+         "k_fused_pass1": nb * nNk * 4 + nb * (1 << n) + (1 << n) * 4 + M * c
+         + nb * nNk * 16 * c,
          "k_fused_pass2": nb * nNk * 16 * c + M * (2 + c + s_out)}
     return {"total": sum(k.values()), "kernels": k, "nNk": nNk, "bracelets": nb}
 

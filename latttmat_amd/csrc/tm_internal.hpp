@@ -325,29 +325,41 @@ void expand_orbits(const GroupSpec &g, const uint64_t *reps, const uint16_t *dup
 // to rev O, so a column whose necklace is the mirror of another's is that
 // column with its O words reversed.  Pass 1 therefore stores one column per
 // bracelet (a necklace class and its mirror), and pass 2 runs only the rows
-// that are bracelet representatives (every orbit has evaluation points in
-// both a row and its mirror).  At n = 14: 1182 necklaces, 687 bracelets.
+// that are bracelet representatives.  At n = 14: 1182 necklaces, 687
+// bracelets.
+//
+// Evaluation rows.  The (E class, O class) pairs of an orbit's states form a
+// set closed under the one-site shift (a, b) -> (b, a), the mirror
+// (a, b) -> (mirror a, mirror b) and, when folded, the spin flip.  The row R
+// of an orbit is a function of that set (one of the stored classes among its
+// O classes, picked by a balancing rule), and inside a row the orbits are
+// ordered by a key that is also a function of the set (the least E class of
+// its states with O class R).  The states (E = NR[c], O = rotl_k NR[q]) for
+// all k share the pair set, so the orbits pass 1 of column c gathers for the
+// O class q form one run of at most 8n orbits: the column map is stored as a
+// 32-bit base per (column, O class) and an 8-bit offset per entry.
 struct FusedTables {
   int n;                 // L/2
   int nNk;               // number of n-bit necklaces
   int nCol;              // stored columns (bracelet representatives)
   int nRow;              // rows pass 2 runs (bracelet representatives)
-  int64_t ldA;           // leading dimension of the pass-1 output (nNk*16, padded)
-  DevBuf NR;             // uint16[nNk]   necklace representatives, ascending
-  DevBuf period;         // uint8[nNk]    rotation period of each representative
+  // pass-1 output A, row-major: A[r][s][k] (row necklace r < nNk, stored
+  // column s < nCol, slot k < 16) is A(E = NR[colS[s]], O = rotr_k NR[r])
   DevBuf slotO;          // uint16[nNk*16] LDS index sw(O) of (row necklace r, slot k),
                          //   O = rotr_k NR[r] (mod the period) for k < n; 0xFFFF unused
-  DevBuf nkl;            // uint16[2^n]   LDS index sw(O) of the n-bit words O in
-                         //               necklace-major order (class, rotation)
-  DevBuf cmap;           // uint32[nCol << n] orbit of the state (E = NR[stored column],
-                         //               O = nkl[j])
-  DevBuf csrc;           // uint16[nNk]   stored column holding class c's data (its own,
-                         //               or its mirror's)
-  DevBuf crot;           // uint8[nNk]    255 for a stored class; for a mirrored one
-                         //               the m with NR[c] = rotl_m(rotl_1(rev NR[src]))
-  DevBuf rowS;           // uint16[nRow]  the row classes pass 2 runs
-  DevBuf rmir;           // uint16[nNk]   mirror class of each row
-  DevBuf rrot;           // uint8[nNk]    t with rev(NR[r]) = rotl_t NR[rmir[r]]
+  DevBuf jtab;           // uint32[2^n]   entry j of a column (O words necklace-major:
+                         //               class q, then rotation): sw(O) | q << 16
+  DevBuf cbase;          // uint32[nCol*nNk] internal index of the first orbit of the run
+                         //               that column s gathers for O class q
+  DevBuf coff;           // uint8[nCol << n] offset of entry j in that run
+  DevBuf itab;           // uint64[nNk]   pass-2 items: the nCol stored columns, then the
+                         //   mirrored classes by source column; bits 0-15 source column,
+                         //   16-31 NR of the item's class, 32-39 its period, 40-47 the
+                         //   rotation m (255 for a stored class)
+  DevBuf rinfo;          // uint64[nRow]  pass-2 rows in launch order (most orbits
+                         //   first): bits 0-15 row necklace r, 16-31 its mirror class,
+                         //   32-39 t with rev(NR[r]) = rotl_t NR[mirror], 40-55 the
+                         //   row's index q into ostart
   DevBuf ostart;         // int64[nRow+1] internal-index range of the orbits whose
                          //               evaluation point has O = NR[rowS[q]]
   DevBuf olist;          // uint32[M]     internal index -> orbit (ascending-rep order)

@@ -143,7 +143,9 @@ void build_reduced(tm_op *op, const GroupSpec &g, double nfexp, bool want_fused,
                    bool diag_in_full = false)
 {
   const bool fused = want_fused && !g.zgroup && g.dim == 2 && g.bc == 0 && (g.L % 2) == 0 &&
-                     g.L >= 8 && ((size_t(1) << (g.L / 2)) * sizeof(double) <= g_lds_per_block);
+                     g.L >= 8 &&
+                     // FP64 column + the pass-1 bases (at most 1182 words at L = 28)
+                     ((size_t(1) << (g.L / 2)) * sizeof(double) + 8192 <= g_lds_per_block);
   const int64_t M = enumerate_orbits(g, op->reps, op->dups, fused ? nullptr : &op->map, op->stream);
   op->ncol = M;
   std::vector<uint64_t> reps(M);

@@ -473,17 +473,17 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
     rot[w] = (uint8_t)k;
   }
   ft.nNk = (int)NR.size();
-  ft.ldA = (((int64_t)ft.nNk * 16 + 63) / 64) * 64;
   const int nNk = ft.nNk;
-  // mirrors: the site mirror i -> -i maps the E word to rotl_1(rev E) and
-  // the O word to rev O; both land in the necklace class of the reversed word
-  auto rev = [&](uint32_t w) {
+  std::vector<uint16_t> revw(nw);
+  for (uint32_t w = 0; w < nw; ++w) {
     uint32_t r = 0;
     for (int i = 0; i < n; ++i) r |= ((w >> i) & 1u) << (n - 1 - i);
-    return r;
-  };
+    revw[w] = (uint16_t)r;
+  }
+  // mirrors: the site mirror i -> -i maps the E word to rotl_1(rev E) and
+  // the O word to rev O; both land in the necklace class of the reversed word
   std::vector<uint16_t> mcl(nNk);
-  for (int c = 0; c < nNk; ++c) mcl[c] = nk[rev(NR[c])];
+  for (int c = 0; c < nNk; ++c) mcl[c] = nk[revw[NR[c]]];
   // stored columns and rows: the bracelet representatives (c <= its mirror)
   std::vector<uint16_t> colS, rowS;
   std::vector<int32_t> sidx(nNk, -1);
@@ -495,7 +495,7 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
     }
   ft.nCol = (int)colS.size();
   ft.nRow = (int)rowS.size();
-  std::vector<uint16_t> csrc(nNk), rmir(nNk);
+  std::vector<uint16_t> csrc(nNk);
   std::vector<uint8_t> crot(nNk, 255), rrot(nNk, 0);
   for (int c = 0; c < nNk; ++c) {
     if (sidx[c] >= 0) {
@@ -503,128 +503,191 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
     } else {
       const int src = mcl[c];
       csrc[c] = (uint16_t)sidx[src];
-      const uint32_t w = rotl(rev(NR[src]), 1);
+      const uint32_t w = rotl(revw[NR[src]], 1);
       int m = 0;
       while (m < per[c] && rotl(w, m) != NR[c]) ++m;
       TM_CHECK(m < per[c], TM_EINVAL, "mirror column has no rotation onto its representative");
       crot[c] = (uint8_t)m;
     }
-    rmir[c] = mcl[c];
-    const uint32_t rw = rev(NR[c]);
+    const uint32_t rw = revw[NR[c]];
     int t = 0;
     while (t < per[c] && rotl(NR[mcl[c]], t) != rw) ++t;
     TM_CHECK(t < per[c], TM_EINVAL, "mirror row has no rotation onto the reversed word");
     rrot[c] = (uint8_t)t;
   }
-  // column order of cmap: O' words necklace-major (class, then rotation), so
-  // that a wavefront's gathers fall into a few per-row clusters
+  // column entries: O words necklace-major (class, then rotation), so that
+  // the entries of one O class are consecutive
   std::vector<uint16_t> order;
+  std::vector<uint16_t> qof;
   order.reserve(nw);
+  qof.reserve(nw);
   for (int q = 0; q < nNk; ++q)
-    for (int k = 0; k < per[q]; ++k) order.push_back((uint16_t)rotl(NR[q], k));
+    for (int k = 0; k < per[q]; ++k) {
+      order.push_back((uint16_t)rotl(NR[q], k));
+      qof.push_back((uint16_t)q);
+    }
   // slot k of row r holds the O word that pass 2 needs for the rotation
   // E = rotl_k(NR[c]) of a stored column: A(NR[c], rotr_k NR[r]) = A(NR[c],
   // rotl_{(n-k) mod p_r} NR[r]), k < n (k >= p_c is never read)
   std::vector<uint16_t> slot((size_t)nNk * 16, 0xFFFF);
   for (int r = 0; r < nNk; ++r)
     for (int k = 0; k < n; ++k)
-      slot[(size_t)r * 16 + k] = (uint16_t)rotl(NR[r], (n - k) % per[r]);
-  // evaluation point of every orbit: a state of the orbit whose odd-bit part
-  // is the necklace representative of a stored row (the representative, or
-  // its mirror image if the representative's O class is not stored),
-  // rotated so that the O part is that representative
+      slot[(size_t)r * 16 + k] = (uint16_t)dev::sw(rotl(NR[r], (n - k) % per[r]));
+  // evaluation point of every orbit (see FusedTables): the images of the
+  // representative under the one-site shift, the mirror and the flip carry
+  // every (E class, O class) pair of the orbit
   std::vector<uint64_t> hreps(M);
   copy_sync(hreps.data(), reps.get(), M * 8, hipMemcpyDeviceToHost, s);
-  const int L = gs.L;
+  std::vector<int32_t> rowq(nNk, -1);
+  for (int q = 0; q < ft.nRow; ++q) rowq[rowS[q]] = q;
   auto compact = [&](uint64_t v) {  // bits 0,2,4,... -> 0,1,2,...
     uint32_t r = 0;
     for (int i = 0; i < n; ++i) r |= (uint32_t)((v >> (2 * i)) & 1) << i;
     return r;
   };
-  auto mirror = [&](uint64_t v) {  // site i -> -i mod L
-    uint64_t r = v & 1;
-    for (int i = 1; i < L; ++i) r |= ((v >> i) & 1ull) << (L - i);
-    return r;
-  };
-  std::vector<int32_t> rowq(nNk, -1);
-  for (int q = 0; q < ft.nRow; ++q) rowq[rowS[q]] = q;
+  const bool flip = gs.censym != 0;
   std::vector<int64_t> cnt(ft.nRow + 1, 0);
-  std::vector<uint16_t> pr(M), pe(M);
+  std::vector<uint16_t> pr(M), pe(M), pkey(M);
   for (int64_t o = 0; o < M; ++o) {
-    uint64_t st = hreps[o];
-    if (rowq[nk[compact(st >> 1)]] < 0) st = mirror(st);
-    const uint32_t E = compact(st), O = compact(st >> 1);
-    const int r = nk[O], k = rot[O];
-    TM_CHECK(rowq[r] >= 0, TM_EINVAL, "orbit has no evaluation point in a stored row");
-    pr[o] = (uint16_t)rowq[r];
-    pe[o] = (uint16_t)rotl(E, n - k);  // rotate E right by k
-    ++cnt[rowq[r] + 1];
+    uint32_t iE[8], iO[8];
+    int ni = 0;
+    const uint32_t E0 = compact(hreps[o]), O0 = compact(hreps[o] >> 1);
+    for (int sh = 0; sh < 2; ++sh) {
+      // one-site shift: (E, O) -> (rotl_1 O, E)
+      const uint32_t E1 = sh ? rotl(O0, 1) : E0, O1 = sh ? E0 : O0;
+      for (int mi = 0; mi < 2; ++mi) {
+        const uint32_t E2 = mi ? rotl(revw[E1], 1) : E1, O2 = mi ? revw[O1] : O1;
+        for (int f = 0; f < (flip ? 2 : 1); ++f) {
+          iE[ni] = f ? (~E2 & wmask) : E2;
+          iO[ni] = f ? (~O2 & wmask) : O2;
+          ++ni;
+        }
+      }
+    }
+    // stored O classes of the orbit, ascending and distinct
+    int B[8], nb = 0;
+    for (int i = 0; i < ni; ++i) {
+      const int b = nk[iO[i]];
+      if (rowq[b] < 0) continue;
+      bool seen = false;
+      for (int j = 0; j < nb; ++j) seen |= B[j] == b;
+      if (!seen) B[nb++] = b;
+    }
+    TM_CHECK(nb > 0, TM_EINVAL, "orbit has no evaluation point in a stored row");
+    std::sort(B, B + nb);
+    int sum = 0;
+    for (int j = 0; j < nb; ++j) sum += B[j];
+    const int R = B[sum % nb];
+    int key = nNk, pick = -1;
+    for (int i = 0; i < ni; ++i)
+      if (nk[iO[i]] == R) {
+        key = std::min(key, (int)nk[iE[i]]);
+        if (pick < 0) pick = i;
+      }
+    const int k = rot[iO[pick]];
+    pr[o] = (uint16_t)rowq[R];
+    pe[o] = (uint16_t)rotl(iE[pick], n - k);  // rotate by k two-site steps back
+    pkey[o] = (uint16_t)key;
+    ++cnt[rowq[R] + 1];
   }
   for (int q = 0; q < ft.nRow; ++q) cnt[q + 1] += cnt[q];
   std::vector<uint32_t> olist(M);
   std::vector<uint16_t> elist(M);
   std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
   for (int64_t o = 0; o < M; ++o) olist[pos[pr[o]]++] = (uint32_t)o;
-  // Within a row, order the orbits by the necklace class (then rotation) of
-  // their E part: pass 1 of column c gathers, from every row, only orbits
-  // whose E part is a rotation of NR[c], which are then one contiguous run.
   for (int q = 0; q < ft.nRow; ++q)
     std::sort(olist.begin() + cnt[q], olist.begin() + cnt[q + 1], [&](uint32_t a, uint32_t b) {
+      if (pkey[a] != pkey[b]) return pkey[a] < pkey[b];
       const uint32_t ea = pe[a], eb = pe[b];
       if (nk[ea] != nk[eb]) return nk[ea] < nk[eb];
       if (rot[ea] != rot[eb]) return rot[ea] < rot[eb];
       return a < b;
     });
-  for (int64_t i = 0; i < M; ++i) elist[i] = pe[olist[i]];
+  for (int64_t i = 0; i < M; ++i) elist[i] = (uint16_t)dev::sw(pe[olist[i]]);
+  // pass-2 rows, the ones with most orbits launched first
+  std::vector<int> rorder(ft.nRow);
+  for (int q = 0; q < ft.nRow; ++q) rorder[q] = q;
+  std::stable_sort(rorder.begin(), rorder.end(), [&](int a, int b) {
+    return cnt[a + 1] - cnt[a] > cnt[b + 1] - cnt[b];
+  });
+  std::vector<uint64_t> rinfo(ft.nRow);
+  for (int i = 0; i < ft.nRow; ++i) {
+    const int q = rorder[i], r = rowS[q];
+    rinfo[i] = (uint64_t)r | ((uint64_t)mcl[r] << 16) | ((uint64_t)rrot[r] << 32) |
+               ((uint64_t)q << 40);
+  }
+  // pass-2 items: stored columns, then the mirrored classes by source column
+  std::vector<int> mirc;
+  for (int c = 0; c < nNk; ++c)
+    if (crot[c] != 255) mirc.push_back(c);
+  std::stable_sort(mirc.begin(), mirc.end(), [&](int a, int b) { return csrc[a] < csrc[b]; });
+  std::vector<uint64_t> itab;
+  itab.reserve(nNk);
+  for (int i = 0; i < ft.nCol; ++i) {
+    const int c = colS[i];
+    itab.push_back((uint64_t)i | ((uint64_t)NR[c] << 16) | ((uint64_t)per[c] << 32) |
+                   (255ull << 40));
+  }
+  for (int c : mirc)
+    itab.push_back((uint64_t)csrc[c] | ((uint64_t)NR[c] << 16) | ((uint64_t)per[c] << 32) |
+                   ((uint64_t)crot[c] << 40));
+  TM_CHECK((int)itab.size() == nNk, TM_EINVAL, "pass-2 item table does not cover every class");
   auto up = [&](DevBuf &d, const void *src, size_t bytes) {
     d.alloc(bytes);
     copy_sync(d.get(), src, bytes, hipMemcpyHostToDevice, s);
   };
-  up(ft.NR, NR.data(), NR.size() * 2);
-  up(ft.period, per.data(), per.size());
-  up(ft.csrc, csrc.data(), csrc.size() * 2);
-  up(ft.crot, crot.data(), crot.size());
-  up(ft.rowS, rowS.data(), rowS.size() * 2);
-  up(ft.rmir, rmir.data(), rmir.size() * 2);
-  up(ft.rrot, rrot.data(), rrot.size());
-  // the pass kernels use these words only as LDS indices: store them swizzled
-  for (auto &w : slot)
-    if (w != 0xFFFF) w = (uint16_t)dev::sw(w);
   up(ft.slotO, slot.data(), slot.size() * 2);
   up(ft.ostart, cnt.data(), cnt.size() * 8);
   up(ft.olist, olist.data(), olist.size() * 4);
-  for (auto &w : elist) w = (uint16_t)dev::sw(w);
   up(ft.Elist, elist.data(), elist.size() * 2);
-  up(ft.nkl, order.data(), order.size() * 2);
+  up(ft.itab, itab.data(), itab.size() * 8);
+  up(ft.rinfo, rinfo.data(), rinfo.size() * 8);
   // internal orbit numbering = position in olist (rows contiguous)
   std::vector<uint32_t> inv(M);
   for (int64_t i = 0; i < M; ++i) inv[olist[i]] = (uint32_t)i;
-  DevBuf invd;
+  DevBuf invd, orderd;
   up(invd, inv.data(), inv.size() * 4);
+  up(orderd, order.data(), order.size() * 2);
   ft.perm = std::move(olist);
 
-  // column-major state->orbit map over the stored necklace columns
+  // column map over the stored necklace columns, in internal numbering
   const GroupDev g = to_dev(gs);
   OrbitScratch sc;
   const uint64_t M2 = orbit_bitmap(g, sc, s);
   TM_CHECK((int64_t)M2 == M, TM_EINVAL, "orbit count mismatch while building fused tables");
   std::vector<uint16_t> colw(ft.nCol);
   for (int i = 0; i < ft.nCol; ++i) colw[i] = NR[colS[i]];
-  DevBuf colwd;
+  DevBuf colwd, cmapd;
   up(colwd, colw.data(), colw.size() * 2);
   const uint64_t total = (uint64_t)ft.nCol << n;
-  ft.cmap.alloc(total * 4);
+  cmapd.alloc(total * 4);
   hipLaunchKernelGGL(k_build_cmap, dim3(grid_for(total, 256)), dim3(256), 0, s, g, n,
-                     colwd.as<uint16_t>(), (uint64_t)ft.nCol, ft.nkl.as<uint16_t>(),
-                     sc.bitmap.as<uint64_t>(), sc.wpre.as<uint64_t>(), ft.cmap.as<uint32_t>());
+                     colwd.as<uint16_t>(), (uint64_t)ft.nCol, orderd.as<uint16_t>(),
+                     sc.bitmap.as<uint64_t>(), sc.wpre.as<uint64_t>(), cmapd.as<uint32_t>());
   TM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_remap_u32, dim3(grid_for(total, 256)), dim3(256), 0, s,
-                     ft.cmap.as<uint32_t>(), invd.as<uint32_t>(), total);
+                     cmapd.as<uint32_t>(), invd.as<uint32_t>(), total);
   TM_LAUNCH_CHECK();
-  TM_HIP(hipStreamSynchronize(s));
-  // the map is built from the plain words; pass 1 needs only their LDS index
-  for (auto &w : order) w = (uint16_t)dev::sw(w);
-  copy_sync(ft.nkl.get(), order.data(), order.size() * 2, hipMemcpyHostToDevice, s);
+  std::vector<uint32_t> cmap(total);
+  copy_sync(cmap.data(), cmapd.get(), total * 4, hipMemcpyDeviceToHost, s);
+  // compressed: base per (column, O class), 8-bit offset per entry
+  std::vector<uint32_t> cbase((size_t)ft.nCol * nNk, 0xFFFFFFFFu);
+  std::vector<uint8_t> coff(total);
+  for (int c = 0; c < ft.nCol; ++c) {
+    const uint32_t *mrow = cmap.data() + ((size_t)c << n);
+    uint32_t *b = cbase.data() + (size_t)c * nNk;
+    for (uint32_t j = 0; j < nw; ++j) b[qof[j]] = std::min(b[qof[j]], mrow[j]);
+    for (uint32_t j = 0; j < nw; ++j) {
+      const uint32_t d = mrow[j] - b[qof[j]];
+      TM_CHECK(d < 256, TM_EINVAL, "column map run longer than 256 orbits");
+      coff[((size_t)c << n) + j] = (uint8_t)d;
+    }
+  }
+  std::vector<uint32_t> jtab(nw);
+  for (uint32_t j = 0; j < nw; ++j) jtab[j] = (uint32_t)dev::sw(order[j]) | ((uint32_t)qof[j] << 16);
+  up(ft.cbase, cbase.data(), cbase.size() * 4);
+  up(ft.coff, coff.data(), coff.size());
+  up(ft.jtab, jtab.data(), jtab.size() * 4);
 }
 
 // ---------------------------------------------------------------------------
@@ -714,119 +777,189 @@ __global__ __launch_bounds__(1024) void k_kron_small(C *u, int L, C r, C f)
 
 constexpr int FUSED_THREADS = 1024;
 
+__device__ __forceinline__ void ld4(const float *p, float *v)
+{
+  const float4 a = *reinterpret_cast<const float4 *>(p);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
+}
+__device__ __forceinline__ void ld4(const double *p, double *v)
+{
+  const double2 a = reinterpret_cast<const double2 *>(p)[0];
+  const double2 b = reinterpret_cast<const double2 *>(p)[1];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+__device__ __forceinline__ void st4(float *p, const float *v)
+{
+  *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void st4(double *p, const double *v)
+{
+  reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
+}
+
 // Pass 1: one stored column (E = the necklace representative of a bracelet);
-// u(E, O') = xt[cmap[c][j]] with O' over all words (LDS index nkl[j] =
-// sw(O')); transform over O; store A(E, O) grouped as [row necklace][slot]
-// for every one of the nNk row necklaces.
+// u(E, O') = xt[cbase[c][q] + coff[c][j]] for every entry j (O' of class q,
+// LDS index in jtab); transform over O; store A(E, O) row-major, 16 slots
+// per (row necklace, column).  The column's bases are staged in LDS after
+// the 2^n values.
 template <typename C>
 __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
-    const uint32_t *__restrict__ cmap, const uint16_t *__restrict__ nkl, const C *__restrict__ xt,
-    C *__restrict__ Ap, const uint16_t *__restrict__ slotO, int n, int nCol, int nNk,
-    int64_t ldA, C kr, C kf)
+    const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ cbase,
+    const uint8_t *__restrict__ coff, const C *__restrict__ xt, C *__restrict__ Ap,
+    const uint16_t *__restrict__ slotO, int n, int nCol, int nNk, C kr, C kf)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
-  // within a row necklace the orbits are ordered by their E necklace, so
-  // columns c, c+1, ... gather neighbouring runs of x~: keep consecutive
-  // columns on one XCD so that they share its L2 lines
+  uint32_t *lbase = reinterpret_cast<uint32_t *>(smem + (sizeof(C) << n));
+  // neighbouring columns write neighbouring 16-slot groups of every row of
+  // A and gather neighbouring runs of x~: keep them on one XCD (one L2)
   const int c = (int)dev::xcd_remap(blockIdx.x, nCol);
   if (c < 0) return;
-  const int nh = 1 << n;
-  const uint32_t *mrow = cmap + ((uint64_t)c << n);
-  // cmap columns are in necklace-major O' order (nkl): consecutive lanes
-  // gather from a few contiguous runs of x~; UNR gathers in flight per thread
-  constexpr int UNG = 16;
-  for (int j0 = threadIdx.x; j0 < nh; j0 += blockDim.x * UNG) {
-    uint32_t idx[UNG];
-    uint16_t O[UNG];
+  const uint32_t *cb = cbase + (size_t)c * nNk;
+  for (int i = threadIdx.x; i < nNk; i += blockDim.x) lbase[i] = cb[i];
+  __syncthreads();
+  // groups of 4 consecutive entries: one 16-byte jtab load and one 4-byte
+  // offset load each; UNG groups (4*UNG gathers) in flight per thread
+  const int ng = 1 << (n - 2);
+  const uint4 *jt = reinterpret_cast<const uint4 *>(jtab);
+  const uint32_t *co = reinterpret_cast<const uint32_t *>(coff + ((size_t)c << n));
+  constexpr int UNG = 4;
+  for (int g0 = threadIdx.x; g0 < ng; g0 += blockDim.x * UNG) {
+    uint4 jj[UNG];
+    uint32_t oo[UNG];
 #pragma unroll
     for (int u = 0; u < UNG; ++u) {
-      const int j = j0 + u * blockDim.x;
-      idx[u] = j < nh ? mrow[j] : 0u;
-      O[u] = j < nh ? nkl[j] : (uint16_t)0;
+      const int g = g0 + u * (int)blockDim.x;
+      jj[u] = g < ng ? jt[g] : make_uint4(0, 0, 0, 0);
+      oo[u] = g < ng ? co[g] : 0u;
     }
-    C v[UNG];
+    uint32_t idx[4 * UNG];
 #pragma unroll
-    for (int u = 0; u < UNG; ++u) v[u] = xt[idx[u]];
+    for (int u = 0; u < UNG; ++u) {
+      idx[4 * u + 0] = lbase[jj[u].x >> 16] + (oo[u] & 255u);
+      idx[4 * u + 1] = lbase[jj[u].y >> 16] + ((oo[u] >> 8) & 255u);
+      idx[4 * u + 2] = lbase[jj[u].z >> 16] + ((oo[u] >> 16) & 255u);
+      idx[4 * u + 3] = lbase[jj[u].w >> 16] + (oo[u] >> 24);
+    }
+    C v[4 * UNG];
 #pragma unroll
-    for (int u = 0; u < UNG; ++u)
-      if (j0 + u * (int)blockDim.x < nh) lds[O[u]] = v[u];
+    for (int i = 0; i < 4 * UNG; ++i) v[i] = xt[idx[i]];
+#pragma unroll
+    for (int u = 0; u < UNG; ++u) {
+      if (g0 + u * (int)blockDim.x < ng) {
+        lds[jj[u].x & 0xFFFFu] = v[4 * u + 0];
+        lds[jj[u].y & 0xFFFFu] = v[4 * u + 1];
+        lds[jj[u].z & 0xFFFFu] = v[4 * u + 2];
+        lds[jj[u].w & 0xFFFFu] = v[4 * u + 3];
+      }
+    }
   }
   __syncthreads();
   dev::lds_kron(lds, n, kr);
-  // output (scaled by kf = a^n): UNR slot-table loads in flight per thread
-  // before the LDS reads
-  C *out = Ap + (int64_t)c * ldA;
-  const int ns = nNk * 16;
-  constexpr int UNR = 8;
-  for (int t0 = threadIdx.x; t0 < ns; t0 += blockDim.x * UNR) {
-    uint16_t O[UNR];
+  // output (scaled by kf = a^n): item t = (row necklace r, slot quarter),
+  // four slots from one 8-byte slot-table load, one vector store each
+  const int nt = nNk * 4;
+  const uint2 *sl = reinterpret_cast<const uint2 *>(slotO);
+  constexpr int UNR = 4;
+  for (int t0 = threadIdx.x; t0 < nt; t0 += blockDim.x * UNR) {
+    uint2 w[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-      const int t = t0 + u * blockDim.x;
-      O[u] = t < ns ? slotO[t] : (uint16_t)0xFFFF;
+      const int t = t0 + u * (int)blockDim.x;
+      w[u] = t < nt ? sl[t] : make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
     }
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-      const int t = t0 + u * blockDim.x;
-      if (t < ns) out[t] = O[u] != 0xFFFF ? lds[O[u]] * kf : (C)0;
+      const int t = t0 + u * (int)blockDim.x;
+      if (t >= nt) continue;
+      const uint32_t o[4] = {w[u].x & 0xFFFFu, w[u].x >> 16, w[u].y & 0xFFFFu, w[u].y >> 16};
+      C v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = o[i] != 0xFFFFu ? lds[o[i]] * kf : (C)0;
+      st4(Ap + ((size_t)(t >> 2) * nCol + c) * 16 + 4 * (t & 3), v);
     }
   }
 }
 
 // Pass 2: one stored row necklace O_r = NR[r]; rebuild A(E, O_r) for every E
 // and transform over E; write y at the orbits whose evaluation point lies on
-// this row.  For E = rotl_k NR[c] of a stored column c,
-//   A(E, O_r) = A(NR[c], rotr_k O_r) = slot k of (c, r);
-// for a mirrored column c (NR[c] = rotl_m(rotl_1(rev NR[src]))) the mirror
-// and rotation invariance of A give
-//   A(rotr_{m+t+j} NR[c], O_r) = slot j of (src, mirror row of r),
-// with rev(NR[r]) = rotl_t NR[mirror row].
+// this row.  Item i of itab is a necklace class c with its source column:
+// for a stored class, E = rotl_k NR[c] is slot k of A[r][c's column]; for a
+// mirrored class (NR[c] = rotl_m(rotl_1(rev NR[src]))) the mirror and
+// rotation invariance of A give
+//   A(rotr_{m+t+j} NR[c], O_r) = slot j of A[mirror row of r][src],
+// with rev(NR[r]) = rotl_t NR[mirror row].  Row r of A and the part of the
+// mirror row that the mirrored classes use are read as contiguous runs.
 template <typename C, typename TO>
 __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
-    const C *__restrict__ Ap, const uint16_t *__restrict__ NR, const uint8_t *__restrict__ period,
-    const uint16_t *__restrict__ csrc, const uint8_t *__restrict__ crot,
-    const uint16_t *__restrict__ rowS, const uint16_t *__restrict__ rmir,
-    const uint8_t *__restrict__ rrot, int nNk, int nRow, const int64_t *__restrict__ ostart,
-    const uint16_t *__restrict__ Elist, const C *__restrict__ dt, TO *__restrict__ y, int n,
-    int64_t ldA, C kr, C kf, const double *__restrict__ scale)
+    const C *__restrict__ Ap, const uint64_t *__restrict__ itab,
+    const uint64_t *__restrict__ rinfo, int nNk, int nCol, int nRow,
+    const int64_t *__restrict__ ostart, const uint16_t *__restrict__ Elist,
+    const C *__restrict__ dt, TO *__restrict__ y, int n, C kr, C kf,
+    const double *__restrict__ scale)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   C *lds = reinterpret_cast<C *>(smem);
-  // consecutive rows read neighbouring 16-slot groups of every column (the
-  // same cache lines): keep them on one XCD
-  const int q = (int)dev::xcd_remap(blockIdx.x, nRow);
-  if (q < 0) return;
-  const int r = rowS[q], mr = rmir[r], tr = rrot[r];
+  if ((int)blockIdx.x >= nRow) return;
+  const uint64_t ri = rinfo[blockIdx.x];
+  const int r = (int)(ri & 0xFFFF), mr = (int)((ri >> 16) & 0xFFFF);
+  const int tr = (int)((ri >> 32) & 0xFF), q = (int)((ri >> 40) & 0xFFFF);
   const uint32_t wmask = (1u << n) - 1;
-  // one necklace column per thread: slot k of the 16-slot group holds the
-  // value for the rotation E = rotl_k NR[c] (the slot table is ordered so)
-  for (int c = threadIdx.x; c < nNk; c += blockDim.x) {
-    const int m = crot[c];
-    const bool mir = m != 255;
-    const C *g = Ap + (int64_t)csrc[c] * ldA + (int64_t)(mir ? mr : r) * 16;
-    C v[16];
+  const C *Ar = Ap + (size_t)r * nCol * 16;
+  const C *Am = Ap + (size_t)mr * nCol * 16;
+  // item quarter t = (item, slots 4(t&3) .. 4(t&3)+3): one vector load each
+  const int nt = nNk * 4;
+  constexpr int UNR = 5;
+  for (int t0 = threadIdx.x; t0 < nt; t0 += blockDim.x * UNR) {
+    uint64_t it[UNR];
+    C v[UNR][4];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = g[j];
-    const uint32_t w = NR[c];
-    const int pc = period[c];
-    if (!mir) {
-      uint32_t E = w;
+    for (int u = 0; u < UNR; ++u) {
+      const int t = t0 + u * (int)blockDim.x;
+      it[u] = t < nt ? itab[t >> 2] : 0ull;
+    }
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if (k >= pc) break;
-        lds[dev::sw(E)] = v[k];
-        E = ((E << 1) & wmask) | (E >> (n - 1));
+    for (int u = 0; u < UNR; ++u) {
+      const int t = t0 + u * (int)blockDim.x;
+      const int pc = (int)((it[u] >> 32) & 0xFF), k0 = 4 * (t & 3);
+      if (t < nt && k0 < pc) {
+        const bool mir = ((it[u] >> 40) & 0xFF) != 255;
+        ld4((mir ? Am : Ar) + (size_t)(it[u] & 0xFFFF) * 16 + k0, v[u]);
       }
-    } else {
-      // E_j = rotr_{m+t+j} NR[c]
-      const int d = (m + tr) % pc;
-      uint32_t E = d ? ((w >> d) | ((w << (n - d)) & wmask)) : w;
+    }
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        if (j >= pc) break;
-        lds[dev::sw(E)] = v[j];
-        E = (E >> 1) | ((E & 1u) << (n - 1));
+    for (int u = 0; u < UNR; ++u) {
+      const int t = t0 + u * (int)blockDim.x;
+      const int pc = (int)((it[u] >> 32) & 0xFF), k0 = 4 * (t & 3);
+      if (t >= nt || k0 >= pc) continue;
+      const uint32_t w = (uint32_t)((it[u] >> 16) & 0xFFFF);
+      const int m = (int)((it[u] >> 40) & 0xFF);
+      if (m == 255) {
+        // E_k = rotl_k NR[c], k = k0 .. k0+3 (k0 < pc <= n)
+        uint32_t E = k0 ? (((w << k0) & wmask) | (w >> (n - k0))) : w;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (k0 + i >= pc) break;
+          lds[dev::sw(E)] = v[u][i];
+          E = ((E << 1) & wmask) | (E >> (n - 1));
+        }
+      } else {
+        // E_j = rotr_{m+t+j} NR[c], j = k0 .. k0+3
+        const int d = ((m + tr) % pc + k0) % n;
+        uint32_t E = d ? ((w >> d) | ((w << (n - d)) & wmask)) : w;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (k0 + i >= pc) break;
+          lds[dev::sw(E)] = v[u][i];
+          E = (E >> 1) | ((E & 1u) << (n - 1));
+        }
       }
     }
   }
@@ -837,18 +970,18 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
   // internal numbering: the orbits of this row are [ostart[q], ostart[q+1]);
   // UNR index and weight loads in flight per thread before the LDS reads
   const int64_t i0 = ostart[q], i1 = ostart[q + 1];
-  constexpr int UNR = 4;
-  for (int64_t b = i0 + threadIdx.x; b < i1; b += (int64_t)blockDim.x * UNR) {
-    uint16_t E[UNR];
-    C d[UNR];
+  constexpr int UNO = 4;
+  for (int64_t b = i0 + threadIdx.x; b < i1; b += (int64_t)blockDim.x * UNO) {
+    uint16_t E[UNO];
+    C d[UNO];
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) {
+    for (int u = 0; u < UNO; ++u) {
       const int64_t i = b + (int64_t)u * blockDim.x;
       E[u] = i < i1 ? Elist[i] : (uint16_t)0;
       d[u] = i < i1 ? dt[i] : (C)0;
     }
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) {
+    for (int u = 0; u < UNO; ++u) {
       const int64_t i = b + (int64_t)u * blockDim.x;
       if (i < i1) y[i] = (TO)(d[u] * lds[E[u]] * inv);
     }
@@ -876,30 +1009,24 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
     const FusedTables &ft = *p.ft;
     const int n = ft.n;
     C *Ap = rest;
-    const size_t shm = (size_t(1) << n) * sizeof(C);
+    const size_t shm1 = (size_t(1) << n) * sizeof(C) + (size_t)ft.nNk * 4;
+    const size_t shm2 = (size_t(1) << n) * sizeof(C);
+    const C kf = dev::kron_coef<C>(p.ka, 0, p.kb, 0, n).f;
     const unsigned g1 = 8u * (unsigned)((ft.nCol + 7) / 8);
-    hipLaunchKernelGGL((k_fused_pass1<C>), dim3(g1), dim3(FUSED_THREADS), shm, s,
-                       ft.cmap.as<uint32_t>(), ft.nkl.as<uint16_t>(), xt, Ap,
-                       ft.slotO.as<uint16_t>(), n, ft.nCol, ft.nNk, ft.ldA, kr,
-                       dev::kron_coef<C>(p.ka, 0, p.kb, 0, n).f);
+    hipLaunchKernelGGL((k_fused_pass1<C>), dim3(g1), dim3(FUSED_THREADS), shm1, s,
+                       ft.jtab.as<uint32_t>(), ft.cbase.as<uint32_t>(), ft.coff.as<uint8_t>(),
+                       xt, Ap, ft.slotO.as<uint16_t>(), n, ft.nCol, ft.nNk, kr, kf);
     TM_LAUNCH_CHECK();
-    const unsigned g2 = 8u * (unsigned)((ft.nRow + 7) / 8);
     if (io == TM_F64)
-      hipLaunchKernelGGL((k_fused_pass2<C, double>), dim3(g2), dim3(FUSED_THREADS), shm, s,
-                         Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(),
-                         ft.csrc.as<uint16_t>(), ft.crot.as<uint8_t>(), ft.rowS.as<uint16_t>(),
-                         ft.rmir.as<uint16_t>(), ft.rrot.as<uint8_t>(), ft.nNk, ft.nRow,
-                         ft.ostart.as<int64_t>(), ft.Elist.as<uint16_t>(), dt,
-                         static_cast<double *>(out), n, ft.ldA, kr,
-                         dev::kron_coef<C>(p.ka, 0, p.kb, 0, n).f, scale);
+      hipLaunchKernelGGL((k_fused_pass2<C, double>), dim3(ft.nRow), dim3(FUSED_THREADS), shm2, s,
+                         Ap, ft.itab.as<uint64_t>(), ft.rinfo.as<uint64_t>(), ft.nNk, ft.nCol,
+                         ft.nRow, ft.ostart.as<int64_t>(), ft.Elist.as<uint16_t>(), dt,
+                         static_cast<double *>(out), n, kr, kf, scale);
     else
-      hipLaunchKernelGGL((k_fused_pass2<C, float>), dim3(g2), dim3(FUSED_THREADS), shm, s,
-                         Ap, ft.NR.as<uint16_t>(), ft.period.as<uint8_t>(),
-                         ft.csrc.as<uint16_t>(), ft.crot.as<uint8_t>(), ft.rowS.as<uint16_t>(),
-                         ft.rmir.as<uint16_t>(), ft.rrot.as<uint8_t>(), ft.nNk, ft.nRow,
-                         ft.ostart.as<int64_t>(), ft.Elist.as<uint16_t>(), dt,
-                         static_cast<float *>(out), n, ft.ldA, kr,
-                         dev::kron_coef<C>(p.ka, 0, p.kb, 0, n).f, scale);
+      hipLaunchKernelGGL((k_fused_pass2<C, float>), dim3(ft.nRow), dim3(FUSED_THREADS), shm2, s,
+                         Ap, ft.itab.as<uint64_t>(), ft.rinfo.as<uint64_t>(), ft.nNk, ft.nCol,
+                         ft.nRow, ft.ostart.as<int64_t>(), ft.Elist.as<uint16_t>(), dt,
+                         static_cast<float *>(out), n, kr, kf, scale);
     TM_LAUNCH_CHECK();
     return;
   }
@@ -983,7 +1110,7 @@ size_t reduced_work_bytes(const ReducedPlan &p, tm_dtype comp)
   const size_t es = comp == TM_F64 ? 8 : 4;
   size_t b = (size_t)((p.M + 63) / 64) * 64 * es;
   if (p.ft) {
-    b += (size_t)p.ft->nCol * p.ft->ldA * es;
+    b += (size_t)p.ft->nCol * p.ft->nNk * 16 * es;
   } else if (p.sweep) {
     b += 2 * (size_t(1) << p.L) * es + sweep_work_bytes(p.sweep->kind, p.L, comp);
   } else if (p.ylayer) {
