@@ -333,6 +333,8 @@ def measure_traffic(L, M):
                 # named among the arguments cannot match a template search
                 name = row["Kernel_Name"].removeprefix("void ").split("(anonymous namespace)::")
                 name = "".join(name).split("(")[0]
+                # the packed (16-byte aligned) form of k_xtilde counts as k_xtilde
+                name = name.replace("k_xtilde4<", "k_xtilde<")
                 vals.setdefault(name, []).append(float(row["Counter_Value"]))
         per[ctr] = {k: sum(v) / len(v) for k, v in vals.items()}
 

@@ -347,11 +347,15 @@ struct FusedTables {
   // column s < nCol, slot k < 16) is A(E = NR[colS[s]], O = rotr_k NR[r])
   DevBuf slotO;          // uint16[nNk*16] LDS index sw(O) of (row necklace r, slot k),
                          //   O = rotr_k NR[r] (mod the period) for k < n; 0xFFFF unused
-  DevBuf jtab;           // uint32[2^n]   entry j of a column (O words necklace-major:
-                         //               class q, then rotation): sw(O) | q << 16
+  DevBuf jtab;           // uint32[2^n]   the entries of a column (O words necklace-major:
+                         //               class q, then rotation): sw(O) | q << 16, in
+                         //               an interleaved position order (each block of
+                         //               256 entries transposed as 64 x 4) so that a
+                         //               gather instruction covers 64 consecutive ones
   DevBuf cbase;          // uint32[nCol*nNk] internal index of the first orbit of the run
                          //               that column s gathers for O class q
-  DevBuf coff;           // uint8[nCol << n] offset of entry j in that run
+  DevBuf coff;           // uint8[nCol << n] offset of each entry in that run (same
+                         //               position order as jtab)
   DevBuf itab;           // uint64[nNk]   pass-2 items: the nCol stored columns, then the
                          //   mirrored classes by source column; bits 0-15 source column,
                          //   16-31 NR of the item's class, 32-39 its period, 40-47 the

@@ -474,6 +474,9 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   }
   ft.nNk = (int)NR.size();
   const int nNk = ft.nNk;
+  // the pass kernels cover a column (2^n entries, 4 per group) and a row
+  // (4 quarters per class) in one round of their thread's registers
+  TM_CHECK(n <= 14 && nNk * 4 <= 5 * 1024, TM_EINVAL, "fused reduced path needs L <= 28");
   std::vector<uint16_t> revw(nw);
   for (uint32_t w = 0; w < nw; ++w) {
     uint32_t r = 0;
@@ -670,6 +673,16 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   TM_LAUNCH_CHECK();
   std::vector<uint32_t> cmap(total);
   copy_sync(cmap.data(), cmapd.get(), total * 4, hipMemcpyDeviceToHost, s);
+  // Stored position of entry j: a wave's 64 threads load 4 consecutive
+  // positions each (one group), and gather instruction e of the group
+  // should cover 64 consecutive entries (a few runs of x~, not one entry in
+  // four of 256): in each block of B = min(256, 2^n) entries, entry
+  // e (B/4) + l sits at position 4 l + e.
+  const uint32_t Bq = std::min<uint32_t>(256, nw) / 4;
+  auto spos = [&](uint32_t j) {
+    const uint32_t blk = j / (4 * Bq), rem = j % (4 * Bq);
+    return blk * 4 * Bq + 4 * (rem % Bq) + rem / Bq;
+  };
   // compressed: base per (column, O class), 8-bit offset per entry
   std::vector<uint32_t> cbase((size_t)ft.nCol * nNk, 0xFFFFFFFFu);
   std::vector<uint8_t> coff(total);
@@ -680,11 +693,12 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
     for (uint32_t j = 0; j < nw; ++j) {
       const uint32_t d = mrow[j] - b[qof[j]];
       TM_CHECK(d < 256, TM_EINVAL, "column map run longer than 256 orbits");
-      coff[((size_t)c << n) + j] = (uint8_t)d;
+      coff[((size_t)c << n) + spos(j)] = (uint8_t)d;
     }
   }
   std::vector<uint32_t> jtab(nw);
-  for (uint32_t j = 0; j < nw; ++j) jtab[j] = (uint32_t)dev::sw(order[j]) | ((uint32_t)qof[j] << 16);
+  for (uint32_t j = 0; j < nw; ++j)
+    jtab[spos(j)] = (uint32_t)dev::sw(order[j]) | ((uint32_t)qof[j] << 16);
   up(ft.cbase, cbase.data(), cbase.size() * 4);
   up(ft.coff, coff.data(), coff.size());
   up(ft.jtab, jtab.data(), jtab.size() * 4);
@@ -804,6 +818,30 @@ __device__ __forceinline__ void st4(double *p, const double *v)
   reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
 }
 
+// k_xtilde over 4-element packs (x, tvec and xt 16-byte aligned); the M mod 4
+// tail by the first threads of block 0
+template <typename C, typename TI>
+__global__ void k_xtilde4(const TI *__restrict__ x, const C *__restrict__ tvec, C *__restrict__ xt,
+                          int64_t M, const double *__restrict__ scale)
+{
+  const C sc = scale ? (C)(*scale) : (C)1;
+  const int64_t M4 = M / 4;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < M4;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    TI a[4];
+    C t[4], o[4];
+    ld4(x + 4 * p, a);
+    ld4(tvec + 4 * p, t);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = (C)a[i] * sc * t[i];
+    st4(xt + 4 * p, o);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < M - 4 * M4) {
+    const int64_t i = 4 * M4 + threadIdx.x;
+    xt[i] = (C)x[i] * sc * tvec[i];
+  }
+}
+
 // Pass 1: one stored column (E = the necklace representative of a bracelet);
 // u(E, O') = xt[cbase[c][q] + coff[c][j]] for every entry j (O' of class q,
 // LDS index in jtab); transform over O; store A(E, O) row-major, 16 slots
@@ -822,69 +860,70 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
   // A and gather neighbouring runs of x~: keep them on one XCD (one L2)
   const int c = (int)dev::xcd_remap(blockIdx.x, nCol);
   if (c < 0) return;
-  const uint32_t *cb = cbase + (size_t)c * nNk;
-  for (int i = threadIdx.x; i < nNk; i += blockDim.x) lbase[i] = cb[i];
-  __syncthreads();
   // groups of 4 consecutive entries: one 16-byte jtab load and one 4-byte
-  // offset load each; UNG groups (4*UNG gathers) in flight per thread
+  // offset load each, issued before the bases are staged; one round of
+  // UNG groups per thread covers the column (2^n <= 4 UNG FUSED_THREADS,
+  // n <= 14 on the fused path)
   const int ng = 1 << (n - 2);
   const uint4 *jt = reinterpret_cast<const uint4 *>(jtab);
   const uint32_t *co = reinterpret_cast<const uint32_t *>(coff + ((size_t)c << n));
   constexpr int UNG = 4;
-  for (int g0 = threadIdx.x; g0 < ng; g0 += blockDim.x * UNG) {
-    uint4 jj[UNG];
-    uint32_t oo[UNG];
+  const int g0 = threadIdx.x;
+  uint4 jj[UNG];
+  uint32_t oo[UNG];
 #pragma unroll
-    for (int u = 0; u < UNG; ++u) {
-      const int g = g0 + u * (int)blockDim.x;
-      jj[u] = g < ng ? jt[g] : make_uint4(0, 0, 0, 0);
-      oo[u] = g < ng ? co[g] : 0u;
-    }
-    uint32_t idx[4 * UNG];
+  for (int u = 0; u < UNG; ++u) {
+    const int g = g0 + u * FUSED_THREADS;
+    jj[u] = g < ng ? jt[g] : make_uint4(0, 0, 0, 0);
+    oo[u] = g < ng ? co[g] : 0u;
+  }
+  const uint32_t *cb = cbase + (size_t)c * nNk;
+  for (int i = threadIdx.x; i < nNk; i += FUSED_THREADS) lbase[i] = cb[i];
+  __syncthreads();
+  uint32_t idx[4 * UNG];
 #pragma unroll
-    for (int u = 0; u < UNG; ++u) {
-      idx[4 * u + 0] = lbase[jj[u].x >> 16] + (oo[u] & 255u);
-      idx[4 * u + 1] = lbase[jj[u].y >> 16] + ((oo[u] >> 8) & 255u);
-      idx[4 * u + 2] = lbase[jj[u].z >> 16] + ((oo[u] >> 16) & 255u);
-      idx[4 * u + 3] = lbase[jj[u].w >> 16] + (oo[u] >> 24);
-    }
-    C v[4 * UNG];
+  for (int u = 0; u < UNG; ++u) {
+    idx[4 * u + 0] = lbase[jj[u].x >> 16] + (oo[u] & 255u);
+    idx[4 * u + 1] = lbase[jj[u].y >> 16] + ((oo[u] >> 8) & 255u);
+    idx[4 * u + 2] = lbase[jj[u].z >> 16] + ((oo[u] >> 16) & 255u);
+    idx[4 * u + 3] = lbase[jj[u].w >> 16] + (oo[u] >> 24);
+  }
+  C v[4 * UNG];
 #pragma unroll
-    for (int i = 0; i < 4 * UNG; ++i) v[i] = xt[idx[i]];
+  for (int i = 0; i < 4 * UNG; ++i) v[i] = xt[idx[i]];
+  // the output phase's slot-table words, loaded while the gathers are in
+  // flight (one round of UNR items per thread covers nNk*4 <= 5 FUSED_THREADS)
+  const int nt = nNk * 4;
+  const uint2 *sl = reinterpret_cast<const uint2 *>(slotO);
+  constexpr int UNR = 5;
+  uint2 w[UNR];
 #pragma unroll
-    for (int u = 0; u < UNG; ++u) {
-      if (g0 + u * (int)blockDim.x < ng) {
-        lds[jj[u].x & 0xFFFFu] = v[4 * u + 0];
-        lds[jj[u].y & 0xFFFFu] = v[4 * u + 1];
-        lds[jj[u].z & 0xFFFFu] = v[4 * u + 2];
-        lds[jj[u].w & 0xFFFFu] = v[4 * u + 3];
-      }
+  for (int u = 0; u < UNR; ++u) {
+    const int t = threadIdx.x + u * FUSED_THREADS;
+    w[u] = t < nt ? sl[t] : make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+  }
+#pragma unroll
+  for (int u = 0; u < UNG; ++u) {
+    if (g0 + u * FUSED_THREADS < ng) {
+      lds[jj[u].x & 0xFFFFu] = v[4 * u + 0];
+      lds[jj[u].y & 0xFFFFu] = v[4 * u + 1];
+      lds[jj[u].z & 0xFFFFu] = v[4 * u + 2];
+      lds[jj[u].w & 0xFFFFu] = v[4 * u + 3];
     }
   }
   __syncthreads();
   dev::lds_kron(lds, n, kr);
   // output (scaled by kf = a^n): item t = (row necklace r, slot quarter),
-  // four slots from one 8-byte slot-table load, one vector store each
-  const int nt = nNk * 4;
-  const uint2 *sl = reinterpret_cast<const uint2 *>(slotO);
-  constexpr int UNR = 4;
-  for (int t0 = threadIdx.x; t0 < nt; t0 += blockDim.x * UNR) {
-    uint2 w[UNR];
+  // four slots from one 8-byte slot-table word, one vector store each
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-      const int t = t0 + u * (int)blockDim.x;
-      w[u] = t < nt ? sl[t] : make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
-    }
+  for (int u = 0; u < UNR; ++u) {
+    const int t = threadIdx.x + u * FUSED_THREADS;
+    if (t >= nt) continue;
+    const uint32_t o[4] = {w[u].x & 0xFFFFu, w[u].x >> 16, w[u].y & 0xFFFFu, w[u].y >> 16};
+    C vv[4];
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-      const int t = t0 + u * (int)blockDim.x;
-      if (t >= nt) continue;
-      const uint32_t o[4] = {w[u].x & 0xFFFFu, w[u].x >> 16, w[u].y & 0xFFFFu, w[u].y >> 16};
-      C v[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = o[i] != 0xFFFFu ? lds[o[i]] * kf : (C)0;
-      st4(Ap + ((size_t)(t >> 2) * nCol + c) * 16 + 4 * (t & 3), v);
-    }
+    for (int i = 0; i < 4; ++i) vv[i] = o[i] != 0xFFFFu ? lds[o[i]] * kf : (C)0;
+    st4(Ap + ((size_t)(t >> 2) * nCol + c) * 16 + 4 * (t & 3), vv);
   }
 }
 
@@ -996,9 +1035,18 @@ void reduced_typed(const ReducedPlan &p, const void *in, void *out, tm_dtype io,
   C *xt = static_cast<C *>(work);
   C *rest = xt + ((M + 63) / 64) * 64;
   const unsigned gM = grid_for((uint64_t)M, 256);
-  if (io == TM_F64)
+  const bool a16 = ((uintptr_t)in % 16) == 0 && ((uintptr_t)tvec % 16) == 0 &&
+                   ((uintptr_t)xt % 16) == 0;
+  const unsigned g4 = grid_for((uint64_t)(M / 4 + 1), 256);
+  if (io == TM_F64 && a16)
+    hipLaunchKernelGGL((k_xtilde4<C, double>), dim3(g4), dim3(256), 0, s,
+                       static_cast<const double *>(in), tvec, xt, M, scale);
+  else if (io == TM_F64)
     hipLaunchKernelGGL((k_xtilde<C, double>), dim3(gM), dim3(256), 0, s,
                        static_cast<const double *>(in), tvec, xt, M, scale);
+  else if (a16)
+    hipLaunchKernelGGL((k_xtilde4<C, float>), dim3(g4), dim3(256), 0, s,
+                       static_cast<const float *>(in), tvec, xt, M, scale);
   else
     hipLaunchKernelGGL((k_xtilde<C, float>), dim3(gM), dim3(256), 0, s,
                        static_cast<const float *>(in), tvec, xt, M, scale);
