@@ -76,72 +76,72 @@ __device__ __forceinline__ W rotrw(W s, int n, int d)
   return (s >> d) | ((s & ((W(1) << d) - 1)) << (n - d));
 }
 
-template <typename W>
-__device__ __forceinline__ Idx3 diag_index_w(const DiagSpec &g, W s)
+// One diagonal kind K known at compile time (K = -1: no diagonal term
+// beyond the field popcount i0).
+template <int K, typename W>
+__device__ __forceinline__ Idx3 diag_index_kw(const DiagSpec &g, W s)
 {
   const int L = g.L;
   const W mask = (L >= (int)(8 * sizeof(W))) ? ~W(0) : ((W(1) << L) - 1);
   Idx3 r;
   r.i0 = popw(s);
-  switch (g.kind) {
-    case DIAG_X2D_PBC:
-    case DIAG_ZBT:
-      r.i1 = popw(W(s ^ rotrw<W>(s, L, 1)));
-      r.i2 = popw(W(s ^ rotrw<W>(s, L, 2)));
-      break;
-    case DIAG_X2D_OBC:
-      r.i1 = popw(W((s & (mask >> 1)) ^ (s >> 1)));
-      r.i2 = popw(W((s & (mask >> 2)) ^ (s >> 2)));
-      break;
-    case DIAG_X2D_C5:
-    case DIAG_X2D_C9: {
-      const W a = W(s << 1) | (g.kind == DIAG_X2D_C9 ? W(1) : W(0));
-      r.i1 = popw(W(s ^ a));
-      r.i2 = popw(W((a & mask) ^ (s >> 1)));
-      break;
-    }
-    case DIAG_X2D_C13:
-    case DIAG_X2D_C17: {
-      const bool one = g.kind == DIAG_X2D_C17;
-      const W a = W(s << 1) | (one ? W(1) : W(0));
-      const W b = W(s << 2) | (one ? W(3) : W(0));
-      r.i1 = popw(W(s ^ a));
-      r.i2 = popw(W(s ^ b));
-      break;
-    }
-    case DIAG_X3D: {
-      const W f1 = ((s >> 1) & W(g.cmr1)) | ((s & W(g.cm1)) << (g.s2 - 1));
-      const W f2 = ((s >> 2) & W(g.cmr2)) | ((s & W(g.cm2)) << (g.s2 - 2));
-      r.i1 = popw(W(s ^ f1)) + popw(W(s ^ rotrw<W>(s, L, g.s2)));
-      r.i2 = popw(W(s ^ f2));
-      break;
-    }
-    case DIAG_XDT_PBC:
-      r.i1 = popw(W(s ^ rotrw<W>(s, L, 1)));
-      r.i2 = 0;
-      break;
-    case DIAG_XDT_C5:
-      r.i1 = popw(W(W(s << 1) ^ s));
-      r.i2 = 0;
-      break;
-    case DIAG_XDT_C9:
-      r.i1 = popw(W((W(s << 1) | W(1)) ^ s));
-      r.i2 = 0;
-      break;
-    case DIAG_Y2TV: {
-      const W lf = s & mask, rt = s >> L;
-      r.i1 = popw(W(lf ^ rotrw<W>(lf, L, 1))) + popw(W(rt ^ rotrw<W>(rt, L, 1)));
-      r.i2 = popw(W(lf ^ rt));
-      break;
-    }
-    case DIAG_Y2D:
-      r.i1 = popw(W((s & mask) ^ (s >> L)));
-      r.i2 = 0;
-      break;
-    default:
-      r.i1 = r.i2 = 0;
+  r.i1 = r.i2 = 0;
+  if constexpr (K == DIAG_X2D_PBC || K == DIAG_ZBT) {
+    r.i1 = popw(W(s ^ rotrw<W>(s, L, 1)));
+    r.i2 = popw(W(s ^ rotrw<W>(s, L, 2)));
+  } else if constexpr (K == DIAG_X2D_OBC) {
+    r.i1 = popw(W((s & (mask >> 1)) ^ (s >> 1)));
+    r.i2 = popw(W((s & (mask >> 2)) ^ (s >> 2)));
+  } else if constexpr (K == DIAG_X2D_C5 || K == DIAG_X2D_C9) {
+    const W a = W(s << 1) | (K == DIAG_X2D_C9 ? W(1) : W(0));
+    r.i1 = popw(W(s ^ a));
+    r.i2 = popw(W((a & mask) ^ (s >> 1)));
+  } else if constexpr (K == DIAG_X2D_C13 || K == DIAG_X2D_C17) {
+    constexpr bool one = K == DIAG_X2D_C17;
+    const W a = W(s << 1) | (one ? W(1) : W(0));
+    const W b = W(s << 2) | (one ? W(3) : W(0));
+    r.i1 = popw(W(s ^ a));
+    r.i2 = popw(W(s ^ b));
+  } else if constexpr (K == DIAG_X3D) {
+    const W f1 = ((s >> 1) & W(g.cmr1)) | ((s & W(g.cm1)) << (g.s2 - 1));
+    const W f2 = ((s >> 2) & W(g.cmr2)) | ((s & W(g.cm2)) << (g.s2 - 2));
+    r.i1 = popw(W(s ^ f1)) + popw(W(s ^ rotrw<W>(s, L, g.s2)));
+    r.i2 = popw(W(s ^ f2));
+  } else if constexpr (K == DIAG_XDT_PBC) {
+    r.i1 = popw(W(s ^ rotrw<W>(s, L, 1)));
+  } else if constexpr (K == DIAG_XDT_C5) {
+    r.i1 = popw(W(W(s << 1) ^ s));
+  } else if constexpr (K == DIAG_XDT_C9) {
+    r.i1 = popw(W((W(s << 1) | W(1)) ^ s));
+  } else if constexpr (K == DIAG_Y2TV) {
+    const W lf = s & mask, rt = s >> L;
+    r.i1 = popw(W(lf ^ rotrw<W>(lf, L, 1))) + popw(W(rt ^ rotrw<W>(rt, L, 1)));
+    r.i2 = popw(W(lf ^ rt));
+  } else if constexpr (K == DIAG_Y2D) {
+    r.i1 = popw(W((s & mask) ^ (s >> L)));
   }
   return r;
+}
+
+template <typename W>
+__device__ __forceinline__ Idx3 diag_index_w(const DiagSpec &g, W s)
+{
+  switch (g.kind) {
+    case DIAG_X2D_PBC: return diag_index_kw<DIAG_X2D_PBC, W>(g, s);
+    case DIAG_ZBT: return diag_index_kw<DIAG_ZBT, W>(g, s);
+    case DIAG_X2D_OBC: return diag_index_kw<DIAG_X2D_OBC, W>(g, s);
+    case DIAG_X2D_C5: return diag_index_kw<DIAG_X2D_C5, W>(g, s);
+    case DIAG_X2D_C9: return diag_index_kw<DIAG_X2D_C9, W>(g, s);
+    case DIAG_X2D_C13: return diag_index_kw<DIAG_X2D_C13, W>(g, s);
+    case DIAG_X2D_C17: return diag_index_kw<DIAG_X2D_C17, W>(g, s);
+    case DIAG_X3D: return diag_index_kw<DIAG_X3D, W>(g, s);
+    case DIAG_XDT_PBC: return diag_index_kw<DIAG_XDT_PBC, W>(g, s);
+    case DIAG_XDT_C5: return diag_index_kw<DIAG_XDT_C5, W>(g, s);
+    case DIAG_XDT_C9: return diag_index_kw<DIAG_XDT_C9, W>(g, s);
+    case DIAG_Y2TV: return diag_index_kw<DIAG_Y2TV, W>(g, s);
+    case DIAG_Y2D: return diag_index_kw<DIAG_Y2D, W>(g, s);
+    default: return diag_index_kw<-1, W>(g, s);
+  }
 }
 
 // 32-bit words up to 30 state bits: the open-boundary branches shift left by
@@ -156,6 +156,13 @@ template <typename C>
 __device__ __forceinline__ C diag_value(const DiagSpec &g, const C *tab, uint64_t s)
 {
   Idx3 i = diag_index(g, s);
+  return tab[i.i1] * tab[64 + i.i2] * tab[128 + i.i0];
+}
+// the same for a compile-time kind and a state of at most 30 bits
+template <int K, typename C>
+__device__ __forceinline__ C diag_value_k(const DiagSpec &g, const C *tab, uint32_t s)
+{
+  Idx3 i = diag_index_kw<K, uint32_t>(g, s);
   return tab[i.i1] * tab[64 + i.i2] * tab[128 + i.i0];
 }
 
