@@ -10,6 +10,8 @@
 // update kernel subtracts V h and, in the same sweep over the data, already
 // accumulates the dot products needed by the next correction.
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 #include <type_traits>
 
 #include "tm_device.hpp"
@@ -795,19 +797,57 @@ unsigned nblocks(int64_t n)
 // Basis-size classes of the fused kernels: the register arrays are sized by
 // the template argument, so keep it close to the actual count.  T is the
 // vector element type.
-#define TM_JDISPATCH(KERNEL, T, jj, grid, ...)                                                 \
+// Launch KERNEL<J, T> for the smallest compiled J >= jj on nb workgroups,
+// nb first capped at what that kernel keeps resident on the device at once
+// (occupancy_cap): a grid-stride sweep with more workgroups than fit runs a
+// second, partial round with most CUs idle.  nb (an unsigned lvalue) holds
+// the grid launched, which the partial-sum reductions read.
+#define TM_JDISPATCH_ONE(KERNEL, JJ, T, nb, ...)                                                \
+  do {                                                                                          \
+    nb = occupancy_cap(reinterpret_cast<const void *>(&KERNEL<JJ, T>), THREADS, nb);           \
+    hipLaunchKernelGGL((KERNEL<JJ, T>), dim3(nb), dim3(THREADS), 0, s, __VA_ARGS__);           \
+  } while (0)
+#define TM_JDISPATCH(KERNEL, T, jj, nb, ...)                                                   \
   do {                                                                                         \
-    if ((jj) <= 4) hipLaunchKernelGGL((KERNEL<4, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__);  \
-    else if ((jj) <= 8) hipLaunchKernelGGL((KERNEL<8, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__); \
-    else if ((jj) <= 12) hipLaunchKernelGGL((KERNEL<12, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__); \
-    else if ((jj) <= 16) hipLaunchKernelGGL((KERNEL<16, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__); \
-    else if ((jj) <= 24) hipLaunchKernelGGL((KERNEL<24, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__); \
-    else if ((jj) <= 32) hipLaunchKernelGGL((KERNEL<32, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__); \
-    else hipLaunchKernelGGL((KERNEL<64, T>), grid, dim3(THREADS), 0, s, __VA_ARGS__);          \
+    if ((jj) <= 4) TM_JDISPATCH_ONE(KERNEL, 4, T, nb, __VA_ARGS__);                            \
+    else if ((jj) <= 8) TM_JDISPATCH_ONE(KERNEL, 8, T, nb, __VA_ARGS__);                       \
+    else if ((jj) <= 12) TM_JDISPATCH_ONE(KERNEL, 12, T, nb, __VA_ARGS__);                     \
+    else if ((jj) <= 16) TM_JDISPATCH_ONE(KERNEL, 16, T, nb, __VA_ARGS__);                     \
+    else if ((jj) <= 24) TM_JDISPATCH_ONE(KERNEL, 24, T, nb, __VA_ARGS__);                     \
+    else if ((jj) <= 32) TM_JDISPATCH_ONE(KERNEL, 32, T, nb, __VA_ARGS__);                     \
+    else TM_JDISPATCH_ONE(KERNEL, 64, T, nb, __VA_ARGS__);                                     \
     TM_LAUNCH_CHECK();                                                                         \
   } while (0)
 
 namespace {
+// Workgroups of `threads` threads that a kernel keeps resident on the whole
+// device at once (occupancy query, cached per kernel); nb capped at that.
+// 0 from the query (unknown) leaves nb as it is.
+unsigned occupancy_cap(const void *kernel, int threads, unsigned nb)
+{
+  static std::mutex mu;
+  static std::unordered_map<const void *, unsigned> cache;
+  unsigned cap;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(kernel);
+    if (it != cache.end()) {
+      cap = it->second;
+    } else {
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) !=
+              hipSuccess ||
+          per_cu <= 0) {
+        (void)hipGetLastError();
+        per_cu = 0;
+      }
+      cap = (unsigned)per_cu * (unsigned)(g_num_cu > 0 ? g_num_cu : 256);
+      cache.emplace(kernel, cap);
+    }
+  }
+  return cap ? std::min(nb, cap) : nb;
+}
+
 // the packed kernels apply: 16-byte aligned vectors, ld a multiple of the
 // pack width
 template <typename T>
@@ -830,9 +870,9 @@ unsigned multidot(const T *V, int64_t ldv, int j, const T *w, int64_t n, double 
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
   const bool pk = packs_ok<T>(V, w, ldv);
-  const unsigned nb = sweep_blocks<T>(pk, n);
-  if (pk) TM_JDISPATCH(k_multidot_pk, T, j, dim3(nb), V, ldv, j, w, n, partial);
-  else TM_JDISPATCH(k_multidot, T, j, dim3(nb), V, ldv, j, w, n, partial);
+  unsigned nb = sweep_blocks<T>(pk, n);
+  if (pk) TM_JDISPATCH(k_multidot_pk, T, j, nb, V, ldv, j, w, n, partial);
+  else TM_JDISPATCH(k_multidot, T, j, nb, V, ldv, j, w, n, partial);
   if (reduce) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1,
                        out);
@@ -847,12 +887,12 @@ unsigned update_dots(const T *V, int64_t ldv, int j, T *w, int64_t n, const doub
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
   const bool pk = packs_ok<T>(V, w, ldv);
-  const unsigned nb = sweep_blocks<T>(pk, n);
+  unsigned nb = sweep_blocks<T>(pk, n);
   if (pk)
-    TM_JDISPATCH(k_update_dots_pk, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 1, nullptr,
+    TM_JDISPATCH(k_update_dots_pk, T, j, nb, V, ldv, j, w, n, h, partial, 1, nullptr,
                  nullptr);
   else
-    TM_JDISPATCH(k_update_dots, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 1);
+    TM_JDISPATCH(k_update_dots, T, j, nb, V, ldv, j, w, n, h, partial, 1);
   if (reduce) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1,
                        out);
@@ -867,11 +907,11 @@ unsigned update_dots_lanczos(const T *V, int64_t ldv, int j, T *w, int64_t n, co
 {
   TM_CHECK(j >= 1 && j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
   const bool pk = packs_ok<T>(V, w, ldv);
-  const unsigned nb = sweep_blocks<T>(pk, n);
+  unsigned nb = sweep_blocks<T>(pk, n);
   if (pk)
-    TM_JDISPATCH(k_lanczos_dots_pk, T, j, dim3(nb), V, ldv, j, w, n, h, partial);
+    TM_JDISPATCH(k_lanczos_dots_pk, T, j, nb, V, ldv, j, w, n, h, partial);
   else
-    TM_JDISPATCH(k_update_dots, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 1);
+    TM_JDISPATCH(k_update_dots, T, j, nb, V, ldv, j, w, n, h, partial, 1);
   if (reduce) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(64), 0, s, partial, (int)nb, j + 1,
                        out);
@@ -886,12 +926,12 @@ void update_norm(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
   const bool pk = packs_ok<T>(V, w, ldv);
-  const unsigned nb = sweep_blocks<T>(pk, n);
+  unsigned nb = sweep_blocks<T>(pk, n);
   if (pk)
-    TM_JDISPATCH(k_update_dots_pk, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 0, nullptr,
+    TM_JDISPATCH(k_update_dots_pk, T, j, nb, V, ldv, j, w, n, h, partial, 0, nullptr,
                  nullptr);
   else
-    TM_JDISPATCH(k_update_dots, T, j, dim3(nb), V, ldv, j, w, n, h, partial, 0);
+    TM_JDISPATCH(k_update_dots, T, j, nb, V, ldv, j, w, n, h, partial, 0);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, (int)nb, 1, out);
   TM_LAUNCH_CHECK();
 }
@@ -912,12 +952,12 @@ void update_normalize(const T *V, int64_t ldv, int j, T *w, int64_t n, const dou
 {
   TM_CHECK(j <= MAXJ - 1, TM_EINVAL, "Krylov basis too large for the device kernels");
   const bool pk = packs_ok<T>(V, w, ldv);
-  const unsigned nb = sweep_blocks<T>(pk, n);
+  unsigned nb = sweep_blocks<T>(pk, n);
   if (pk)
-    TM_JDISPATCH(k_update_normalize_pk, T, j, dim3(nb), V, ldv, j, w, n, h2, w0, w1, nrm2,
+    TM_JDISPATCH(k_update_normalize_pk, T, j, nb, V, ldv, j, w, n, h2, w0, w1, nrm2,
                  force);
   else
-    TM_JDISPATCH(k_update_normalize, T, j, dim3(nb), V, ldv, j, w, n, h2, w0, w1, nrm2, force);
+    TM_JDISPATCH(k_update_normalize, T, j, nb, V, ldv, j, w, n, h2, w0, w1, nrm2, force);
 }
 
 void reduce_lanczos_coeffs(const double *partial, unsigned nb, int cnt, double *out, int j,
@@ -955,19 +995,18 @@ void gemm_vy(const Ti *Vin, int64_t ldv, int m, const double *Y, int ldy, int k,
   TM_CHECK(m <= MAXJ && k <= m, TM_EINVAL, "restart block too large");
   TM_CHECK(sizeof(Ti) == sizeof(To) || (const void *)Vin != (const void *)Vout, TM_EINVAL,
            "in-place basis rotation needs one element type");
+  // (more workgroups than fit at once: measured faster for this kernel than
+  // a grid capped at the resident count)
   const unsigned nb = nblocks(n) * 4;
-  if (m <= 8)
-    hipLaunchKernelGGL((k_gemm_vy<8, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y, ldy,
-                       k, Vout, ldo, n);
-  else if (m <= 16)
-    hipLaunchKernelGGL((k_gemm_vy<16, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y,
+  auto go = [&](auto m_tag) {
+    constexpr int MM = decltype(m_tag)::value;
+    hipLaunchKernelGGL((k_gemm_vy<MM, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y,
                        ldy, k, Vout, ldo, n);
-  else if (m <= 32)
-    hipLaunchKernelGGL((k_gemm_vy<32, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y,
-                       ldy, k, Vout, ldo, n);
-  else
-    hipLaunchKernelGGL((k_gemm_vy<64, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y,
-                       ldy, k, Vout, ldo, n);
+  };
+  if (m <= 8) go(std::integral_constant<int, 8>());
+  else if (m <= 16) go(std::integral_constant<int, 16>());
+  else if (m <= 32) go(std::integral_constant<int, 32>());
+  else go(std::integral_constant<int, 64>());
   TM_LAUNCH_CHECK();
 }
 
@@ -1005,7 +1044,7 @@ void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double
           double *G, hipStream_t s)
 {
   TM_CHECK(k <= MAXJ, TM_EINVAL, "gram block too large");
-  const unsigned nb = nblocks(n);
+  unsigned nb = nblocks(n);
   // row blocks of 4 (k <= 8) or 2 (k <= 16): each sweep reads RB + k vectors
   auto blocked = [&](auto rb_tag, auto j_tag) {
     constexpr int RB = decltype(rb_tag)::value, J = decltype(j_tag)::value;
@@ -1026,7 +1065,7 @@ void gram(const double *A, const double *B, int64_t ld, int k, int64_t n, double
     blocked(std::integral_constant<int, 2>(), std::integral_constant<int, 16>());
   } else {
     for (int row = 0; row < k; ++row) {
-      TM_JDISPATCH(k_gram_row, double, k, dim3(nb), A, B, ld, k, n, row, partial);
+      TM_JDISPATCH(k_gram_row, double, k, nb, A, B, ld, k, n, row, partial);
       // G[row + c*k] = <A_row, B_c>
       hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(64), 0, s, partial, (int)nb, k,
                          G + row * k);
@@ -1043,8 +1082,10 @@ void gram_sym(const double *U, int64_t ld, int nv, int64_t n, double *partial, d
     gram(U, U, ld, nv, n, partial, G, s);
     return;
   }
-  const unsigned nb = nblocks(n);
   const int nt = nv <= 6 ? 21 : 55;
+  const void *kf = nv <= 6 ? reinterpret_cast<const void *>(&k_gram_tri<6>)
+                           : reinterpret_cast<const void *>(&k_gram_tri<10>);
+  const unsigned nb = occupancy_cap(kf, THREADS, nblocks(n));
   if (nv <= 6)
     hipLaunchKernelGGL(k_gram_tri<6>, dim3(nb), dim3(THREADS), 0, s, U, ld, nv, n, partial);
   else
@@ -1057,7 +1098,11 @@ void combine_norms(const double *U, int64_t ld, int nu, const double *Cm, int ld
                    int64_t n, double *partial, double *out, hipStream_t s)
 {
   TM_CHECK(nu >= 1 && nu <= MAXJ && nc >= 1, TM_EINVAL, "combination block too large");
-  const unsigned nb = nblocks(n);
+  const void *kf = nu <= 8    ? reinterpret_cast<const void *>(&k_combine_norms<8>)
+                   : nu <= 16 ? reinterpret_cast<const void *>(&k_combine_norms<16>)
+                   : nu <= 32 ? reinterpret_cast<const void *>(&k_combine_norms<32>)
+                              : reinterpret_cast<const void *>(&k_combine_norms<64>);
+  const unsigned nb = occupancy_cap(kf, THREADS, nblocks(n));
   for (int c0 = 0; c0 < nc; c0 += 8) {
     const int m = std::min(8, nc - c0);
     if (nu <= 8)
