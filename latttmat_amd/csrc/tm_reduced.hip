@@ -436,6 +436,15 @@ __global__ void k_remap_u32(uint32_t *__restrict__ a, const uint32_t *__restrict
 }
 }  // namespace
 
+// LDS position of element e of a fused-pass column of n bits: the common
+// swizzle, plus (at n = 14) bits 12-13 folded into bank bits 3-4, which
+// keeps both layouts of the two-round transform (fused_kron14) free of bank
+// conflicts.  XOR-linear and a bijection on [0, 2^n).
+__host__ __device__ __forceinline__ int fsw(int e, int n)
+{
+  return dev::sw(e) ^ (n == 14 ? ((e >> 12) & 3) << 3 : 0);
+}
+
 void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, FusedTables &ft,
                         hipStream_t s)
 {
@@ -535,7 +544,7 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   std::vector<uint16_t> slot((size_t)nNk * 16, 0xFFFF);
   for (int r = 0; r < nNk; ++r)
     for (int k = 0; k < n; ++k)
-      slot[(size_t)r * 16 + k] = (uint16_t)dev::sw(rotl(NR[r], (n - k) % per[r]));
+      slot[(size_t)r * 16 + k] = (uint16_t)fsw(rotl(NR[r], (n - k) % per[r]), n);
   // evaluation point of every orbit (see FusedTables): the images of the
   // representative under the one-site shift, the mirror and the flip carry
   // every (E class, O class) pair of the orbit
@@ -606,7 +615,7 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
       if (rot[ea] != rot[eb]) return rot[ea] < rot[eb];
       return a < b;
     });
-  for (int64_t i = 0; i < M; ++i) elist[i] = (uint16_t)dev::sw(pe[olist[i]]);
+  for (int64_t i = 0; i < M; ++i) elist[i] = (uint16_t)fsw(pe[olist[i]], n);
   // pass-2 rows, the ones with most orbits launched first
   std::vector<int> rorder(ft.nRow);
   for (int q = 0; q < ft.nRow; ++q) rorder[q] = q;
@@ -698,7 +707,7 @@ void build_fused_tables(const GroupSpec &gs, const DevBuf &reps, int64_t M, Fuse
   }
   std::vector<uint32_t> jtab(nw);
   for (uint32_t j = 0; j < nw; ++j)
-    jtab[spos(j)] = (uint32_t)dev::sw(order[j]) | ((uint32_t)qof[j] << 16);
+    jtab[spos(j)] = (uint32_t)fsw(order[j], n) | ((uint32_t)qof[j] << 16);
   up(ft.cbase, cbase.data(), cbase.size() * 4);
   up(ft.coff, coff.data(), coff.size());
   up(ft.jtab, jtab.data(), jtab.size() * 4);
@@ -790,6 +799,13 @@ __global__ __launch_bounds__(1024) void k_kron_small(C *u, int L, C r, C f)
 }
 
 constexpr int FUSED_THREADS = 1024;
+// waves per SIMD the pass kernels are compiled for: two 1024-thread blocks
+// per CU with FP32 columns (64 KB of LDS each), one with FP64 (128 KB)
+template <typename C>
+constexpr int fused_waves()
+{
+  return sizeof(C) == 4 ? 8 : 4;
+}
 
 __device__ __forceinline__ void ld4(const float *p, float *v)
 {
@@ -842,13 +858,106 @@ __global__ void k_xtilde4(const TI *__restrict__ x, const C *__restrict__ tvec, 
   }
 }
 
+// Lane-exchange of a value across lane bit B (0..3) inside a 16-lane DPP
+// row: quad permutations for bits 0 and 1; row shifts by 4 / 8 for bits 2
+// and 3, each written only to the lanes whose partner it reaches (bank
+// masks: lanes 0-3, 8-11 of a row take lane + 4, lanes 4-7, 12-15 take
+// lane - 4; lanes 0-7 take lane + 8, lanes 8-15 take lane - 8).
+template <int CTRL, int BM>
+__device__ __forceinline__ float dpp_upd(float old, float x)
+{
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(x), CTRL, 0xF, BM, false));
+}
+template <int CTRL, int BM>
+__device__ __forceinline__ double dpp_upd(double old, double x)
+{
+  const long long o = __double_as_longlong(old), v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, 0xF, BM, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, 0xF, BM, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int B, typename C>
+__device__ __forceinline__ C lane_partner(C x)
+{
+  if constexpr (B == 0) {
+    return dpp_upd<0xB1, 0xF>(x, x);  // quad_perm [1, 0, 3, 2]
+  } else if constexpr (B == 1) {
+    return dpp_upd<0x4E, 0xF>(x, x);  // quad_perm [2, 3, 0, 1]
+  } else if constexpr (B == 2) {
+    const C p = dpp_upd<0x104, 0x5>(x, x);  // row_shl:4 into banks 0, 2
+    return dpp_upd<0x114, 0xA>(p, x);       // row_shr:4 into banks 1, 3
+  } else {
+    const C p = dpp_upd<0x108, 0x3>(x, x);  // row_shl:8 into banks 0, 1
+    return dpp_upd<0x118, 0xC>(p, x);       // row_shr:8 into banks 2, 3
+  }
+}
+
+// K' = [[1, r], [r, 1]] on all 14 bits of an LDS column (element e at
+// lds[fsw(e, 14)]), 1024 threads x 16 values, in two rounds instead of four:
+//   A: registers = bits 0-3, lane bits 0-3 = bits 4-7 (DPP), the other lane
+//      and wave bits = bits 8-13;
+//   B: registers = bits 8-11, lane bits 0-1 = bits 12-13 (DPP), the other
+//      lane and wave bits = bits 0-7.
+// Each thread writes back the positions it read, so a round needs no
+// barrier between its read and its write.  Ends with a block barrier.
+template <typename C>
+__device__ __forceinline__ void fused_kron14(C *lds, C r)
+{
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  C v[16];
+  {
+    const int sa = fsw(((l & 15) << 4) | ((l >> 4) << 8) | (w << 10), 14);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = lds[sa ^ i];
+    dev::kron_regs<4>(v, r);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      C x = v[i];
+      x = x + r * lane_partner<0>(x);
+      x = x + r * lane_partner<1>(x);
+      x = x + r * lane_partner<2>(x);
+      x = x + r * lane_partner<3>(x);
+      v[i] = x;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) lds[sa ^ i] = v[i];
+  }
+  __syncthreads();
+  {
+    const int sb = fsw(((l & 3) << 12) | (l >> 2) | (w << 4), 14);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = lds[sb ^ fsw(i << 8, 14)];
+    dev::kron_regs<4>(v, r);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      C x = v[i];
+      x = x + r * lane_partner<0>(x);
+      x = x + r * lane_partner<1>(x);
+      v[i] = x;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) lds[sb ^ fsw(i << 8, 14)] = v[i];
+  }
+  __syncthreads();
+}
+
+// the column transform of the fused passes: two rounds at n = 14 (1024
+// threads), the four-bit LDS rounds of lds_kron otherwise
+template <typename C>
+__device__ __forceinline__ void fused_kron(C *lds, int n, C r)
+{
+  if (n == 14) fused_kron14(lds, r);
+  else dev::lds_kron(lds, n, r);
+}
+
 // Pass 1: one stored column (E = the necklace representative of a bracelet);
 // u(E, O') = xt[cbase[c][q] + coff[c][j]] for every entry j (O' of class q,
 // LDS index in jtab); transform over O; store A(E, O) row-major, 16 slots
 // per (row necklace, column).  The column's bases are staged in LDS after
 // the 2^n values.
 template <typename C>
-__global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
+__global__ __launch_bounds__(FUSED_THREADS, fused_waves<C>()) void k_fused_pass1(
     const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ cbase,
     const uint8_t *__restrict__ coff, const C *__restrict__ xt, C *__restrict__ Ap,
     const uint16_t *__restrict__ slotO, int n, int nCol, int nNk, C kr, C kf)
@@ -912,7 +1021,7 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
     }
   }
   __syncthreads();
-  dev::lds_kron(lds, n, kr);
+  fused_kron(lds, n, kr);
   // output (scaled by kf = a^n): item t = (row necklace r, slot quarter),
   // four slots from one 8-byte slot-table word, one vector store each
 #pragma unroll
@@ -937,7 +1046,7 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass1(
 // with rev(NR[r]) = rotl_t NR[mirror row].  Row r of A and the part of the
 // mirror row that the mirrored classes use are read as contiguous runs.
 template <typename C, typename TO>
-__global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
+__global__ __launch_bounds__(FUSED_THREADS, fused_waves<C>()) void k_fused_pass2(
     const C *__restrict__ Ap, const uint64_t *__restrict__ itab,
     const uint64_t *__restrict__ rinfo, int nNk, int nCol, int nRow,
     const int64_t *__restrict__ ostart, const uint16_t *__restrict__ Elist,
@@ -986,7 +1095,7 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           if (k0 + i >= pc) break;
-          lds[dev::sw(E)] = v[u][i];
+          lds[fsw(E, n)] = v[u][i];
           E = ((E << 1) & wmask) | (E >> (n - 1));
         }
       } else {
@@ -996,14 +1105,14 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_fused_pass2(
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           if (k0 + i >= pc) break;
-          lds[dev::sw(E)] = v[u][i];
+          lds[fsw(E, n)] = v[u][i];
           E = (E >> 1) | ((E & 1u) << (n - 1));
         }
       }
     }
   }
   __syncthreads();
-  dev::lds_kron(lds, n, kr);
+  fused_kron(lds, n, kr);
   // 1 / scale of the input, and kf = a^n of this pass's butterflies
   const C inv = (scale ? (C)(1.0 / *scale) : (C)1) * kf;
   // internal numbering: the orbits of this row are [ostart[q], ostart[q+1]);
