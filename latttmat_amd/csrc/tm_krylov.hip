@@ -458,69 +458,23 @@ __global__ __launch_bounds__(THREADS) void k_update_normalize_pk(const T *__rest
 // 1e-13 |h| or whose correction removed more than half of w1 again (a
 // numerically invariant subspace: the host redoes the cycle from j with
 // explicit norms).
-__device__ void close_body(int j, const double *__restrict__ hb1,
-                           const double *__restrict__ hb2, double *__restrict__ Hd, int ldh,
-                           double *__restrict__ nrm, double *__restrict__ flags)
-{
-  const int nk = j + 1;
-  const double w0 = hb1[nk], w1 = hb2[nk];
-  const bool re = !(w1 > 0.5 * w0);
-  double hn2 = 0, h22 = 0;
-  for (int k = 0; k < nk; ++k) {
-    const double hk = hb1[k] + (re ? hb2[k] : 0.0);
-    Hd[k + (int64_t)j * ldh] = hk;
-    hn2 += hk * hk;
-    h22 += hb2[k] * hb2[k];
-  }
-  const double beta2 = re ? w1 - h22 : w1;
-  const double beta = sqrt(fmax(beta2, 0.0));
-  Hd[nk + (int64_t)j * ldh] = beta;
-  nrm[0] = beta2;
-  if (re) flags[0] += 1.0;
-  const bool unsafe = re && !(h22 <= 0.5 * w1);
-  if ((unsafe || beta <= 1e-13 * sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
-}
-
-// Symmetric operator, step j > first step of the cycle: T v_j couples to
-// v_{j-1}, v_j, v_{j+1} only.  hL[0..j] = (0, .., 0, beta_{j-1}, alpha) with
-// beta_{j-1} = Hd[j, j-1] and alpha = <v_j, T v_j> (hA[0]).
-__device__ void coeffs_body(int j, const double *__restrict__ hA, const double *__restrict__ Hd,
-                            int ldh, double *__restrict__ hL)
-{
-  for (int k = 0; k <= j; ++k)
-    hL[k] = k == j ? hA[0] : (k == j - 1 ? Hd[j + (int64_t)(j - 1) * ldh] : 0.0);
-}
-
-// Close a Lanczos step: column j = hL + h2 (the full reorthogonalisation pass
-// always runs), beta^2 = |w1|^2 - |h2|^2 (Pythagoras).  flags as close_body;
-// unsafe when the reorthogonalisation removed more than half of |w1|^2.
-__device__ void close_lanczos_body(int j, const double *__restrict__ hL,
-                                   const double *__restrict__ hb2, double *__restrict__ Hd,
-                                   int ldh, double *__restrict__ nrm, double *__restrict__ flags)
-{
-  const int nk = j + 1;
-  const double w1 = hb2[nk];
-  double hn2 = 0, h22 = 0;
-  for (int k = 0; k < nk; ++k) {
-    const double hk = hL[k] + hb2[k];
-    Hd[k + (int64_t)j * ldh] = hk;
-    hn2 += hk * hk;
-    h22 += hb2[k] * hb2[k];
-  }
-  const double beta2 = w1 - h22;
-  const double beta = sqrt(fmax(beta2, 0.0));
-  Hd[nk + (int64_t)j * ldh] = beta;
-  nrm[0] = beta2;
-  flags[0] += 1.0;
-  const bool unsafe = !(h22 <= 0.5 * w1);
-  if ((unsafe || beta <= 1e-13 * sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
-}
-
 // Partial sums of slots 0..cnt-1 (one wavefront per slot, the same order as
-// k_reduce_partials) into out[], then thread 0 closes the step from them:
-// EPI 1 = Lanczos coefficients (hA = out), 2 = close_body (hb1 = a,
-// hb2 = out), 3 = the Lanczos close (hL = a, hb2 = out).  Saves the launch
-// of a one-thread kernel per reduction.
+// k_reduce_partials) into out[], then the first wavefront closes the step
+// from them, one lane per projected-matrix row (values through LDS, the
+// norms by a wavefront sum):
+//   EPI 1, Lanczos coefficients: a[0..j] = hL = (0, .., 0, beta_{j-1},
+//     alpha) with beta_{j-1} = Hd[j, j-1] and alpha = <v_j, T v_j> = out[0]
+//     (T v_j couples to v_{j-1}, v_j, v_{j+1} only);
+//   EPI 2, general step (a = hb1, the first projection, out = hb2, the
+//     second): the DGKS correction is due when |w1|^2 <= |w0|^2 / 2; column j
+//     = hb1 + (correction ? hb2 : 0), beta^2 = |w1|^2 - |hb2|^2 after a
+//     correction, |w1|^2 otherwise; unsafe when the correction removed more
+//     than half of |w1|^2 again;
+//   EPI 3, Lanczos step (a = hL, out = hb2; the full reorthogonalisation
+//     always runs): column j = hL + hb2, beta^2 = |w1|^2 - |hb2|^2, unsafe as
+//     for EPI 2.
+// flags[0] counts corrections; flags[1] takes the first step whose beta
+// is tiny (<= 1e-13 |column|), not positive, or unsafe.  nrm[0] = beta^2.
 template <int EPI>
 __global__ __launch_bounds__(1024) void k_reduce_epi(const double *__restrict__ partial, int nblk,
                                                      int cnt, double *__restrict__ out, int j,
@@ -529,16 +483,44 @@ __global__ __launch_bounds__(1024) void k_reduce_epi(const double *__restrict__ 
                                                      double *__restrict__ nrm,
                                                      double *__restrict__ flags)
 {
+  __shared__ double sv[MAXJ + 2], sa[MAXJ + 2];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int nk = j + 1;
+  if (EPI != 1)
+    for (int k = threadIdx.x; k < (EPI == 2 ? nk + 1 : nk); k += blockDim.x) sa[k] = a[k];
   for (int k = wid; k < cnt; k += nw) {
     const double s = lane_sum(partial + k, nblk, cnt, lane);
-    if (lane == 0) out[k] = s;
+    if (lane == 0) {
+      out[k] = s;
+      sv[k] = s;
+    }
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  if (EPI == 1) coeffs_body(j, out, Hd, ldh, a);
-  if (EPI == 2) close_body(j, a, out, Hd, ldh, nrm, flags);
-  if (EPI == 3) close_lanczos_body(j, a, out, Hd, ldh, nrm, flags);
+  if (wid != 0) return;
+  if (EPI == 1) {
+    const double bprev = j >= 1 ? Hd[j + (int64_t)(j - 1) * ldh] : 0.0;
+    for (int k = lane; k <= j; k += 64) a[k] = k == j ? sv[0] : (k == j - 1 ? bprev : 0.0);
+    return;
+  }
+  const double w1 = sv[nk];
+  const bool re = EPI == 3 || !(w1 > 0.5 * sa[nk]);
+  double hn2 = 0, h22 = 0;
+  for (int k = lane; k < nk; k += 64) {
+    const double hk = sa[k] + (re ? sv[k] : 0.0);
+    Hd[k + (int64_t)j * ldh] = hk;
+    hn2 += hk * hk;
+    h22 += sv[k] * sv[k];
+  }
+  hn2 = wave_sum(hn2);
+  h22 = wave_sum(h22);
+  if (lane != 0) return;
+  const double beta2 = re ? w1 - h22 : w1;
+  const double beta = sqrt(fmax(beta2, 0.0));
+  Hd[nk + (int64_t)j * ldh] = beta;
+  nrm[0] = beta2;
+  if (re) flags[0] += 1.0;
+  const bool unsafe = re && !(h22 <= 0.5 * w1);
+  if ((unsafe || beta <= 1e-13 * sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
 }
 
 template <typename T>
