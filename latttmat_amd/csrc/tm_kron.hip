@@ -127,6 +127,7 @@ __global__ __launch_bounds__(NT) void k_kron_pass(
 struct RRound { int p0, act; };
 struct RPlan {
   int T, logW, b0, nb, nr;
+  int pair;  // 64-byte runs: the two tiles of each 128-byte line on one XCD, back to back
   RRound r[3];
 };
 constexpr int KRON_RPASS = 512;  // autotuner candidate flag: register-resident plan
@@ -151,7 +152,14 @@ __global__ __launch_bounds__(1 << TB) void k_kron_rpass(
     for (int i = threadIdx.x; i < 192; i += blockDim.x) stab[i] = tab[i];
   }
   // 32-bit element indices (the launcher checks Ltot <= 31)
-  const uint32_t tile = blockIdx.x + blockIdx.y * gridDim.x;
+  uint32_t tile = blockIdx.x + blockIdx.y * gridDim.x;
+  if (pl.pair) {
+    // workgroup b runs on XCD b mod 8; with 64-byte runs tiles 2t and 2t+1
+    // read the two halves of the same lines, so give them workgroups b and
+    // b + 8 (one XCD, one L2, launched one after the other)
+    const uint32_t x = tile & 7, k = tile >> 3;
+    tile = ((((k >> 1) << 3) | x) << 1) | (k & 1);
+  }
   const uint32_t innerBlocks = (1u << b0) >> logW;
   const uint32_t o = tile / innerBlocks, ib = tile % innerBlocks;
   const uint32_t base = (o << (b0 + nb)) | (ib << logW);
@@ -321,6 +329,7 @@ std::vector<RPlan> plan_rpass(int L, int es)
   f.b0 = 0;
   f.nb = T;
   f.nr = 3;
+  f.pair = 0;
   f.r[0] = {5, 31};
   f.r[1] = {0, 31};
   // bits [10, T) in the last round: register bits from T - 5 up
@@ -354,6 +363,7 @@ std::vector<RPlan> plan_rpass(int L, int es)
       hi = lo;
     }
     q.nr = nr;
+    q.pair = ((es << logW) < 128 && L - T >= 4) ? 1 : 0;
     ps.push_back(q);
     b += nb;
   }
