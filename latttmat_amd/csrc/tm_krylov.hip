@@ -1110,18 +1110,55 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int GRAM_THREADS = 256;
 constexpr int GRAM_S = 16;          // elements per lane group and sweep step
-constexpr int GRAM_MAXBLK = 512;    // partial-sum blocks of the MFMA Gram
+constexpr int GRAM_MAXBLK = 2048;   // partial-sum blocks of the MFMA Gram
 
 // Partial Gram G = X^T Y (X: nx <= 16 NG columns, Y: b <= 16 columns, both
 // of length n, leading dimension ld) on FP64 MFMA (v_mfma_f64_16x16x4f64).
 // In one MFMA the 64 lanes of a wave supply a 16 x 4 block of X^T (lane l:
 // column l mod 16 of the group, element slot l / 16) and the matching 4 x 16
-// block of Y (lane l: column l mod 16, element slot l / 16), so lanes
-// 16q..16q+15 feed the four k slots from their own contiguous run of GRAM_S
-// elements; the 16 x 16 accumulator of group g holds <X_{16g+i}, Y_j> in
+// block of Y (lane l: column l mod 16, element slot l / 16).  The sum runs
+// over every element, so which element fills which slot is free: lane
+// (column c, slot q) loads the four consecutive elements 16 s + 4 q .. +3 of
+// its columns with one vector load (the four lanes of a column cover 64
+// contiguous bytes) and feeds them to four MFMAs, one element each.  The
+// 16 x 16 accumulator of group g holds <X_{16g+i}, Y_j> in
 // lane (j + 16 (i mod 4)), register i / 4 (the accumulator layout of this
 // instruction on gfx950, measured with one-hot operands).  Each workgroup
 // adds its waves' accumulators in LDS and writes nx x 16 partial sums.
+// v[0..3] = p[e .. e+3] as doubles (zero where on is false or beyond n):
+// one 16-byte (float) or two 16-byte (double) loads when all four are in
+// range (p + e is then 16 / 32 byte aligned: e is a multiple of 4 and the
+// columns start on 64-element boundaries)
+__device__ __forceinline__ void gram_load4(const float *p, int64_t e, int64_t n, bool on,
+                                           double (&v)[4])
+{
+  if (on && e + 3 < n) {
+    const float4 a = *reinterpret_cast<const float4 *>(p + e);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = a.z;
+    v[3] = a.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = (on && e + i < n) ? (double)p[e + i] : 0.0;
+  }
+}
+__device__ __forceinline__ void gram_load4(const double *p, int64_t e, int64_t n, bool on,
+                                           double (&v)[4])
+{
+  if (on && e + 3 < n) {
+    const double2 a = reinterpret_cast<const double2 *>(p + e)[0];
+    const double2 b = reinterpret_cast<const double2 *>(p + e)[1];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = (on && e + i < n) ? p[e + i] : 0.0;
+  }
+}
+
 template <int NG, typename T>
 __global__ __launch_bounds__(GRAM_THREADS) void k_gram_xy_mfma(const T *__restrict__ X, int64_t ld,
                                                                 int nx, const T *__restrict__ Y,
@@ -1140,18 +1177,21 @@ __global__ __launch_bounds__(GRAM_THREADS) void k_gram_xy_mfma(const T *__restri
   const bool yon = col < b;
   for (int64_t base = ((int64_t)blockIdx.x * NW + wid) * chunk; base < n;
        base += (int64_t)gridDim.x * NW * chunk) {
-    const int64_t e0 = base + (int64_t)kq * GRAM_S;
-#pragma unroll 4
-    for (int s = 0; s < GRAM_S; ++s) {
-      const int64_t e = e0 + s;
-      const bool in = e < n;
-      const double yv = (yon && in) ? (double)yc[e] : 0.0;
+#pragma unroll
+    for (int s = 0; s < GRAM_S / 4; ++s) {
+      const int64_t e = base + 16 * s + 4 * kq;
+      double yv[4], xv[NG][4];
+      gram_load4(yc, e, n, yon, yv);
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         const int r = g * 16 + col;
-        const double xv = (r < nx && in) ? (double)X[(int64_t)r * ld + e] : 0.0;
-        acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, yv, acc[g], 0, 0, 0);
+        gram_load4(X + (int64_t)r * ld, e, n, r < nx, xv[g]);
       }
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+          acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[g][v], yv[v], acc[g], 0, 0, 0);
     }
   }
   // accumulator element (i, j) of group g: lane j + 16 (i mod 4), register
@@ -1340,18 +1380,25 @@ void gram_xy(const T *X, int64_t ld, int nx, const T *Y, int b, int64_t n, doubl
              double *G, hipStream_t s)
 {
   TM_CHECK(nx >= 1 && nx <= 64 && b >= 1 && b <= 16, TM_EINVAL, "block Gram too large");
+  TM_CHECK(((reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y)) %
+            (4 * sizeof(T))) == 0 && ld % 4 == 0,
+           TM_EINVAL, "block Gram needs columns aligned to 4 elements");
   int64_t nb = (n + GRAM_THREADS / 64 * 4 * GRAM_S - 1) / (GRAM_THREADS / 64 * 4 * GRAM_S);
   if (nb > GRAM_MAXBLK) nb = GRAM_MAXBLK;
   if (nb < 1) nb = 1;
   const int ng = (nx + 15) / 16;
-  if (ng == 1)
-    hipLaunchKernelGGL((k_gram_xy_mfma<1, T>), dim3(nb), dim3(GRAM_THREADS), 0, s, X, ld, nx, Y, b, n, partial);
-  else if (ng == 2)
-    hipLaunchKernelGGL((k_gram_xy_mfma<2, T>), dim3(nb), dim3(GRAM_THREADS), 0, s, X, ld, nx, Y, b, n, partial);
-  else if (ng == 3)
-    hipLaunchKernelGGL((k_gram_xy_mfma<3, T>), dim3(nb), dim3(GRAM_THREADS), 0, s, X, ld, nx, Y, b, n, partial);
-  else
-    hipLaunchKernelGGL((k_gram_xy_mfma<4, T>), dim3(nb), dim3(GRAM_THREADS), 0, s, X, ld, nx, Y, b, n, partial);
+  // at most the workgroups that stay resident (one round)
+  auto go = [&](auto ng_tag) {
+    constexpr int NG = decltype(ng_tag)::value;
+    nb = occupancy_cap(reinterpret_cast<const void *>(&k_gram_xy_mfma<NG, T>), GRAM_THREADS,
+                       (unsigned)nb);
+    hipLaunchKernelGGL((k_gram_xy_mfma<NG, T>), dim3((unsigned)nb), dim3(GRAM_THREADS), 0, s, X,
+                       ld, nx, Y, b, n, partial);
+  };
+  if (ng == 1) go(std::integral_constant<int, 1>());
+  else if (ng == 2) go(std::integral_constant<int, 2>());
+  else if (ng == 3) go(std::integral_constant<int, 3>());
+  else go(std::integral_constant<int, 4>());
   TM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_gram_xy, dim3(nx * b), dim3(64), 0, s, partial, (int)nb, nx, b, G);
   TM_LAUNCH_CHECK();
