@@ -453,6 +453,21 @@ void reduce_lanczos_coeffs(const double *partial, unsigned nb, int cnt, double *
 void reduce_step_close_lanczos(const double *partial, unsigned nb, int cnt, double *out, int j,
                                const double *hL, double *Hd, int ldh, double *nrm, double *flags,
                                hipStream_t s);
+// Lagged Lanczos steps (symmetric T): a step leaves its three-term residual
+// r in slot j+1 uncorrected and unnormalised; the next step applies T to r
+// and needs one sweep for both the correction of r and its own residual.
+// lag_close: from the partial sums of <r, z> (multidot of slot j against
+// slot j+1 = T r, 2 slots), the previous correction g (j values), beta^2 =
+// nrm[0] and the projected matrix Hd, the coefficients hL[0..j] and cf
+// (2 * 72 + 2 doubles) of lanczos_lag.  lanczos_lag: v_j into slot j, the
+// next residual into slot j+1, partial sums of the j+2 slots <V_k, w> (k <=
+// j), <w, w> for reduce_step_close_lanczos.  j <= LAG_MAXJ.
+constexpr int LAG_MAXJ = 16;
+void lag_close(const double *partial, unsigned nb, int j, const double *g, const double *Hd,
+               int ldh, const double *nrm, double *hL, double *cf, hipStream_t s);
+template <typename T>
+unsigned lanczos_lag(T *V, int64_t ldv, int j, int64_t n, const double *cf, double *partial,
+                     hipStream_t s);
 // w <- (w - c V h2) / sqrt(nrm2[0]), c = 1 iff force or !(w1 > w0 / 2)
 template <typename T>
 void update_normalize(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h2,

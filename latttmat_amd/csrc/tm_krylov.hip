@@ -523,6 +523,145 @@ __global__ __launch_bounds__(1024) void k_reduce_epi(const double *__restrict__ 
   if ((unsafe || beta <= 1e-13 * sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
 }
 
+// Lagged Lanczos step j.  The previous step left its three-term residual r
+// in slot j unnormalised and uncorrected, with its second projection g
+// (g_k = <P_k, r>, P = V_0 .. V_{j-1}) and beta^2 = nrm[0] = |r - P g|^2;
+// T was applied to r itself (z = T r in slot j+1).  Then
+//   v_j = (r - P g) / beta,
+// and the Krylov relation T P = P H + beta v_j e_{j-1}^T (H = the projected
+// matrix so far, Hd) gives
+//   T v_j = (z - P c) / beta - g_{j-1} v_j,   c = H_{0..j-1, 0..j-1} g,
+//   alpha = <v_j, T v_j> = (<r, z> - <g, c>) / beta^2 - 2 g_{j-1}
+// (T symmetric: <P_k, z> = <T P_k, r>).  One wavefront sums <r, z> (slot 0
+// of the partial blocks, cnt slots each) and writes hL[0..j] = (0 .., beta,
+// alpha) for the step close and the sweep coefficients
+//   cf[k] = g_k / beta, cf[LAG_LD + k] = c_k / beta + beta [k = j-1] (k < j),
+//   cf[2 LAG_LD] = alpha + g_{j-1}, cf[2 LAG_LD + 1] = 1 / beta.
+// An error in alpha or c only moves weight into the step's second
+// projection, which the full reorthogonalisation measures and the close adds
+// to the projected-matrix column.
+constexpr int LAG_LD = 72;
+__global__ __launch_bounds__(64) void k_lag_close(const double *__restrict__ partial, int nblk,
+                                                  int cnt, int j, const double *__restrict__ g,
+                                                  const double *__restrict__ Hd, int ldh,
+                                                  const double *__restrict__ nrm,
+                                                  double *__restrict__ hL, double *__restrict__ cf)
+{
+  const int lane = threadIdx.x;
+  const double rz = __shfl(lane_sum(partial, nblk, cnt, lane), 0, 64);
+  const double b2 = nrm[0], b = sqrt(fmax(b2, 0.0)), invb = 1.0 / b;
+  double gc = 0;
+  for (int i = lane; i < j; i += 64) {
+    double c = 0;
+    for (int t = 0; t < j; ++t) c += Hd[i + (int64_t)t * ldh] * g[t];
+    gc += g[i] * c;
+    cf[i] = g[i] * invb;
+    cf[LAG_LD + i] = c * invb + (i == j - 1 ? b : 0.0);
+  }
+  gc = __shfl(wave_sum(gc), 0, 64);
+  const double gl = g[j - 1];
+  const double alpha = (rz - gc) / b2 - 2.0 * gl;
+  for (int i = lane; i <= j; i += 64) hL[i] = i == j ? alpha : (i == j - 1 ? b : 0.0);
+  if (lane == 0) {
+    cf[2 * LAG_LD] = alpha + gl;
+    cf[2 * LAG_LD + 1] = invb;
+  }
+}
+
+// The sweep of a lagged Lanczos step j (coefficients from k_lag_close): per
+// element, from slot j (r), slot j+1 (z) and P,
+//   v_j = r / beta - sum_k (g_k / beta) P_k                 -> slot j
+//   w   = z / beta - sum_k d_k P_k - (alpha + g_{j-1}) v_j  -> slot j+1
+// (w = T v_j - alpha v_j - beta v_{j-1}, the next three-term residual), then
+// the slots <P_k, w> (k < j), <v_j, w>, <w, w> of the full
+// reorthogonalisation, as k_lanczos_dots_pk writes them.  One read of P,
+// r and z replaces the correction sweep of step j-1 and the dot sweep of
+// step j; each P_k of an element group stays in registers between its two
+// uses.  The dots use v_j and w as stored.
+template <int J, typename T>
+__global__ __launch_bounds__(THREADS) void k_lanczos_lag_pk(T *__restrict__ V, int64_t ldv, int j,
+                                                            int64_t n,
+                                                            const double *__restrict__ cf,
+                                                            double *__restrict__ partial)
+{
+  using PV = typename Pk<T>::V;
+  constexpr int P = Pk<T>::P;
+  __shared__ double sg[J], sd[J];
+  for (int k = threadIdx.x; k < J; k += blockDim.x) {
+    sg[k] = k < j ? cf[k] : 0.0;
+    sd[k] = k < j ? cf[LAG_LD + k] : 0.0;
+  }
+  const double ca = cf[2 * LAG_LD], invb = cf[2 * LAG_LD + 1];
+  __syncthreads();
+  PV *__restrict__ Vp = reinterpret_cast<PV *>(V);
+  const int64_t ldp = ldv / P, np = n / P;
+  double acc[J + 2];
+#pragma unroll
+  for (int k = 0; k < J + 2; ++k) acc[k] = 0;
+  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < np;
+       p += (int64_t)gridDim.x * THREADS) {
+    PV vk[J];
+    const PV r = Vp[j * ldp + p], z = Vp[(j + 1) * ldp + p];
+    double vd[P], wd[P];
+#pragma unroll
+    for (int e = 0; e < P; ++e) {
+      vd[e] = (double)r[e] * invb;
+      wd[e] = (double)z[e] * invb;
+    }
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k < j) {
+        vk[k] = Vp[k * ldp + p];
+#pragma unroll
+        for (int e = 0; e < P; ++e) {
+          vd[e] -= sg[k] * (double)vk[k][e];
+          wd[e] -= sd[k] * (double)vk[k][e];
+        }
+      }
+    PV vo, wo;
+#pragma unroll
+    for (int e = 0; e < P; ++e) vo[e] = (T)vd[e];
+#pragma unroll
+    for (int e = 0; e < P; ++e) wo[e] = (T)(wd[e] - ca * (double)vo[e]);
+    Vp[j * ldp + p] = vo;
+    Vp[(j + 1) * ldp + p] = wo;
+#pragma unroll
+    for (int k = 0; k < J; ++k)
+      if (k < j) acc[k] += pdot<PV, P>(vk[k], wo);
+    acc[J] += pdot<PV, P>(vo, wo);
+    acc[J + 1] += pdot<PV, P>(wo, wo);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int64_t i = np * P; i < n; ++i) {
+      double vk[J];
+      double vd = (double)V[j * ldv + i] * invb, wd = (double)V[(j + 1) * ldv + i] * invb;
+#pragma unroll
+      for (int k = 0; k < J; ++k)
+        if (k < j) {
+          vk[k] = V[k * ldv + i];
+          vd -= sg[k] * vk[k];
+          wd -= sd[k] * vk[k];
+        }
+      const T vs = (T)vd;
+      const T ws = (T)(wd - ca * (double)vs);
+      V[j * ldv + i] = vs;
+      V[(j + 1) * ldv + i] = ws;
+#pragma unroll
+      for (int k = 0; k < J; ++k)
+        if (k < j) acc[k] += vk[k] * (double)ws;
+      acc[J] += (double)vs * (double)ws;
+      acc[J + 1] += (double)ws * (double)ws;
+    }
+  // slots 0 .. j-1 stay; <v_j, w> into slot j, <w, w> into slot j+1 (j <= J)
+  const double a1 = acc[J], a2 = acc[J + 1];
+#pragma unroll
+  for (int k = 0; k < J + 2; ++k) {
+    if (k == j) acc[k] = a1;
+    else if (k == j + 1) acc[k] = a2;
+  }
+  block_reduce_write<J + 2>(acc, j + 2, partial + (int64_t)blockIdx.x * (j + 2));
+}
+
 template <typename T>
 __global__ void k_normalize(T *__restrict__ w, int64_t n, const double *__restrict__ nrm2)
 {
@@ -957,6 +1096,31 @@ void reduce_step_close_lanczos(const double *partial, unsigned nb, int cnt, doub
   hipLaunchKernelGGL(k_reduce_epi<3>, dim3(1), dim3(1024), 0, s, partial, (int)nb, cnt, out, j,
                      const_cast<double *>(hL), Hd, ldh, nrm, flags);
   TM_LAUNCH_CHECK();
+}
+
+void lag_close(const double *partial, unsigned nb, int j, const double *g, const double *Hd,
+               int ldh, const double *nrm, double *hL, double *cf, hipStream_t s)
+{
+  TM_CHECK(j >= 1 && j <= LAG_MAXJ, TM_EINVAL, "lagged Lanczos step out of range");
+  hipLaunchKernelGGL(k_lag_close, dim3(1), dim3(64), 0, s, partial, (int)nb, 2, j, g, Hd, ldh,
+                     nrm, hL, cf);
+  TM_LAUNCH_CHECK();
+}
+
+template <typename T>
+unsigned lanczos_lag(T *V, int64_t ldv, int j, int64_t n, const double *cf, double *partial,
+                     hipStream_t s)
+{
+  TM_CHECK(j >= 1 && j <= LAG_MAXJ, TM_EINVAL, "lagged Lanczos step out of range");
+  TM_CHECK(packs_ok<T>(V, V, ldv), TM_EINVAL,
+           "lagged Lanczos step: basis columns must be 16-byte aligned");
+  unsigned nb = sweep_blocks<T>(true, n);
+  if (j <= 4) TM_JDISPATCH_ONE(k_lanczos_lag_pk, 4, T, nb, V, ldv, j, n, cf, partial);
+  else if (j <= 8) TM_JDISPATCH_ONE(k_lanczos_lag_pk, 8, T, nb, V, ldv, j, n, cf, partial);
+  else if (j <= 12) TM_JDISPATCH_ONE(k_lanczos_lag_pk, 12, T, nb, V, ldv, j, n, cf, partial);
+  else TM_JDISPATCH_ONE(k_lanczos_lag_pk, 16, T, nb, V, ldv, j, n, cf, partial);
+  TM_LAUNCH_CHECK();
+  return nb;
 }
 
 template <typename T>
@@ -1445,6 +1609,8 @@ void block_close(const double *G1, const double *G2, int nc, int b, double *Hd, 
                                     const double *, const double *, const double *, int,       \
                                     hipStream_t);                                              \
   template void normalize<T>(T *, int64_t, const double *, hipStream_t);                        \
+  template unsigned lanczos_lag<T>(T *, int64_t, int, int64_t, const double *, double *,         \
+                                   hipStream_t);                                                \
   template void random_uniform<T>(T *, int64_t, uint64_t, hipStream_t);
 TM_KRY_INST(double)
 TM_KRY_INST(float)

@@ -103,11 +103,13 @@ class KrylovSchur final : public KrylovRun {
     sym_ = op->symmetric;
     V_.ensure(sizeof(T) * ldv_ * (m_ + 1));
     partial_.ensure(sizeof(double) * kry::NBLK * 72);
-    hbuf_.ensure(sizeof(double) * 3 * 72);
+    hbuf_.ensure(sizeof(double) * 6 * 72);
     Ydev_.ensure(sizeof(double) * 72 * 72);
     H_.assign((size_t)(m_ + 1) * m_, 0.0);
     Hd_.ensure(sizeof(double) * ((size_t)(m_ + 1) * m_ + 8));
-    stage_.ensure(sizeof(double) * 72 * 72);
+    // read-back of the projected matrix, then the restart's uploads (Y and
+    // the projected matrix)
+    stage_.ensure(sizeof(double) * 2 * 72 * 72);
   }
 
   KrylovSchur(const KrylovSchur &) = delete;
@@ -127,6 +129,7 @@ class KrylovSchur final : public KrylovRun {
     kry::random_uniform(V(0), n_, seed_, s);
     kry::multidot(V_.as<T>(), ldv_, 0, V(0), n_, partial_.as<double>(), hbuf_.as<double>(), s);
     kry::normalize(V(0), n_, hbuf_.as<double>(), s);
+    clear_hd();
     k_ = 0;
   }
 
@@ -137,8 +140,14 @@ class KrylovSchur final : public KrylovRun {
     kry::copy(v0, V(0), n_, s);
     kry::multidot(V_.as<T>(), ldv_, 0, V(0), n_, partial_.as<double>(), hbuf_.as<double>(), s);
     kry::normalize(V(0), n_, hbuf_.as<double>(), s);
+    clear_hd();
     k_ = 0;
   }
+
+  // The lagged Lanczos steps read whole leading blocks of the device
+  // projected matrix (entries below the subdiagonal included), so a solve
+  // starts from zeros there and every restart uploads the restarted matrix.
+  void clear_hd() { kry::fill(Hd_.as<double>(), 0.0, (int64_t)(m_ + 1) * m_, op_->stream); }
 
   // Expand the Krylov-Schur decomposition from column k_ to m_.
   void expand()
@@ -213,26 +222,45 @@ class KrylovSchur final : public KrylovRun {
     double *nrm = flags + 2;
     kry::fill(flags, 0.0, 1, s);
     kry::fill(flags + 1, -1.0, 1, s);
+    // lag: slot j holds the previous step's raw three-term residual r (its
+    // second projection at hb + 72, beta^2 in nrm), not yet v_j
+    bool lag = false;
+    double *cf = hb + 216;
     for (int j = k0; j < m_; ++j) {
       T *w = V(j + 1);
-      op_->apply_internal(V(j), w, ldv_, 1, io_, comp_, tr_, nullptr, s);
       const int nk = j + 1;
+      double *part = partial_.as<double>();
+      if (lag && j > kry::LAG_MAXJ) {
+        // beyond the lagged kernel's register budget: finish v_j first
+        kry::update_normalize(V_.as<T>(), ldv_, j, V(j), n_, hb + 72, hb, hb, nrm, 1, s);
+        lag = false;
+      }
+      op_->apply_internal(V(j), w, ldv_, 1, io_, comp_, tr_, nullptr, s);
+      if (lag) {
+        // lagged Lanczos step: T r is in slot j+1; one sweep finishes v_j and
+        // forms the next residual with its reorthogonalisation dots
+        unsigned nb = kry::multidot(V(j), ldv_, 1, w, n_, part, hb, s, false);
+        kry::lag_close(part, nb, j, hb + 72, Hd, ldh, nrm, hb + 144, cf, s);
+        nb = kry::lanczos_lag(V_.as<T>(), ldv_, j, n_, cf, part, s);
+        kry::reduce_step_close_lanczos(part, nb, nk + 1, hb + 72, j, hb + 144, Hd, ldh, nrm, flags,
+                                       s);
+        continue;
+      }
       if (sym_ && j > k0) {
         // Lanczos step: subtract alpha v_j + beta_{j-1} v_{j-1} inside the
-        // reorthogonalisation sweep, then one full correction sweep that also
-        // normalises (two sweeps over V instead of three)
+        // reorthogonalisation sweep; the correction and normalisation of
+        // the residual are left to the next step's sweep (lagged) or, at
+        // the end of the cycle, to one more sweep
         // (each reduction also closes its part of the step in the same launch)
-        double *part = partial_.as<double>();
         unsigned nb = kry::multidot(V(j), ldv_, 1, w, n_, part, hb, s, false);
         kry::reduce_lanczos_coeffs(part, nb, 2, hb, j, Hd, ldh, hb + 144, s);
         nb = kry::update_dots_lanczos(V_.as<T>(), ldv_, nk, w, n_, hb + 144, part, hb + 72, s,
                                       false);
         kry::reduce_step_close_lanczos(part, nb, nk + 1, hb + 72, j, hb + 144, Hd, ldh, nrm, flags,
                                        s);
-        kry::update_normalize(V_.as<T>(), ldv_, nk, w, n_, hb + 72, hb, hb, nrm, 1, s);
+        lag = true;
         continue;
       }
-      double *part = partial_.as<double>();
       kry::multidot(V_.as<T>(), ldv_, nk, w, n_, part, hb, s);
       const unsigned nb = kry::update_dots(V_.as<T>(), ldv_, nk, w, n_, hb, part, hb + 72, s,
                                            false);
@@ -242,6 +270,8 @@ class KrylovSchur final : public KrylovRun {
       kry::update_normalize(V_.as<T>(), ldv_, nk, w, n_, hb + 72, hb + nk, hb + 72 + nk, nrm,
                             0, s);
     }
+    if (lag)
+      kry::update_normalize(V_.as<T>(), ldv_, m_, V(m_), n_, hb + 72, hb, hb, nrm, 1, s);
   }
 
   // Record issue_cycle(k0) into an executable graph; nullptr if the runtime
@@ -503,6 +533,10 @@ class KrylovSchur final : public KrylovRun {
       for (int a = 0; a < k; ++a) H(a, c) = S[a + (size_t)c * k];
       H(k, c) = beta * Y[(m - 1) + (size_t)c * m];
     }
+    // the restarted projected matrix (zeros beyond column k) to the device
+    double *hup = stage_.as<double>() + 72 * 72;
+    std::copy(H_.begin(), H_.end(), hup);
+    TM_HIP(hipMemcpyAsync(Hd_.get(), hup, sizeof(double) * H_.size(), hipMemcpyHostToDevice, s));
     k_ = k;
     ++restarts_;
     // no synchronisation: the next cycle is ordered after V Y on the stream,

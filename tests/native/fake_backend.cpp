@@ -139,6 +139,51 @@ void reduce_step_close_lanczos(const double *partial, unsigned, int cnt, double 
     if ((unsafe || beta <= 1e-13 * std::sqrt(hn2) || !(beta > 0)) && flags[1] < 0) flags[1] = j;
   });
 }
+// lagged Lanczos step (k_lag_close, k_lanczos_lag_pk)
+void lag_close(const double *partial, unsigned, int j, const double *g, const double *Hd,
+               int ldh, const double *nrm, double *hL, double *cf, hipStream_t s)
+{
+  fake_enqueue(s, [=] {
+    const double rz = partial[0];
+    const double b2 = nrm[0], b = std::sqrt(std::fmax(b2, 0.0)), invb = 1.0 / b;
+    double gc = 0;
+    for (int i = 0; i < j; ++i) {
+      double c = 0;
+      for (int t = 0; t < j; ++t) c += Hd[i + (int64_t)t * ldh] * g[t];
+      gc += g[i] * c;
+      cf[i] = g[i] * invb;
+      cf[72 + i] = c * invb + (i == j - 1 ? b : 0.0);
+    }
+    const double alpha = (rz - gc) / b2 - 2.0 * g[j - 1];
+    for (int i = 0; i <= j; ++i) hL[i] = i == j ? alpha : (i == j - 1 ? b : 0.0);
+    cf[144] = alpha + g[j - 1];
+    cf[145] = invb;
+  });
+}
+template <typename T>
+unsigned lanczos_lag(T *V, int64_t ldv, int j, int64_t n, const double *cf, double *partial,
+                     hipStream_t s)
+{
+  fake_enqueue(s, [=] {
+    std::vector<double> acc(j + 2, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+      double vd = (double)V[j * ldv + i] * cf[145], wd = (double)V[(j + 1) * ldv + i] * cf[145];
+      for (int k = 0; k < j; ++k) {
+        vd -= cf[k] * (double)V[k * ldv + i];
+        wd -= cf[72 + k] * (double)V[k * ldv + i];
+      }
+      const T vs = (T)vd;
+      const T ws = (T)(wd - cf[144] * (double)vs);
+      V[j * ldv + i] = vs;
+      V[(j + 1) * ldv + i] = ws;
+      for (int k = 0; k < j; ++k) acc[k] += (double)V[k * ldv + i] * (double)ws;
+      acc[j] += (double)vs * (double)ws;
+      acc[j + 1] += (double)ws * (double)ws;
+    }
+    for (int k = 0; k < j + 2; ++k) partial[k] = acc[k];
+  });
+  return 1;
+}
 template <typename T>
 void update_normalize(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h2,
                       const double *w0, const double *w1, const double *nrm2, int force,
@@ -373,6 +418,8 @@ template void block_update<float>(const float *, int64_t, int, float *, int, con
                                    double *, double *, hipStream_t, bool);                     \
   template void update_norm<T>(const T *, int64_t, int, T *, int64_t, const double *, double *, \
                                double *, hipStream_t);                                         \
+  template unsigned lanczos_lag<T>(T *, int64_t, int, int64_t, const double *, double *,        \
+                                   hipStream_t);                                                \
   template unsigned update_dots_lanczos<T>(const T *, int64_t, int, T *, int64_t,               \
                                            const double *, double *, double *, hipStream_t,    \
                                            bool);                                              \
