@@ -444,7 +444,8 @@ void reduce_step_close(const double *partial, unsigned nb, int cnt, double *out,
                        const double *hb1, double *Hd, int ldh, double *nrm, double *flags,
                        hipStream_t s);
 // Sum into out (out[0] = alpha = <v_j, T v_j>), then the Lanczos coefficients
-// of step j: hL[0..j] = (0.., beta_{j-1} = Hd[j, j-1], alpha)
+// of step j: hL[0..j] = (row j of Hd left of the diagonal, alpha); the row
+// holds beta_{j-1} inside a cycle and the restart's coupling at its first step
 void reduce_lanczos_coeffs(const double *partial, unsigned nb, int cnt, double *out, int j,
                            double *Hd, int ldh, double *hL, hipStream_t s);
 // Sum into out (the projections h2 of the three-term residual and its

@@ -462,9 +462,11 @@ __global__ __launch_bounds__(THREADS) void k_update_normalize_pk(const T *__rest
 // k_reduce_partials) into out[], then the first wavefront closes the step
 // from them, one lane per projected-matrix row (values through LDS, the
 // norms by a wavefront sum):
-//   EPI 1, Lanczos coefficients: a[0..j] = hL = (0, .., 0, beta_{j-1},
-//     alpha) with beta_{j-1} = Hd[j, j-1] and alpha = <v_j, T v_j> = out[0]
-//     (T v_j couples to v_{j-1}, v_j, v_{j+1} only);
+//   EPI 1, Lanczos coefficients: a[0..j] = hL = (row j of Hd left of the
+//     diagonal, alpha) with alpha = <v_j, T v_j> = out[0]: T v_j couples to
+//     v_{j+1}, v_j and the vectors whose columns reach row j -- v_{j-1}
+//     (beta_{j-1} = Hd[j, j-1]) inside a cycle, every kept Ritz vector (the
+//     restart's coupling row) at its first step; the rest of the row is 0;
 //   EPI 2, general step (a = hb1, the first projection, out = hb2, the
 //     second): the DGKS correction is due when |w1|^2 <= |w0|^2 / 2; column j
 //     = hb1 + (correction ? hb2 : 0), beta^2 = |w1|^2 - |hb2|^2 after a
@@ -498,8 +500,7 @@ __global__ __launch_bounds__(1024) void k_reduce_epi(const double *__restrict__ 
   __syncthreads();
   if (wid != 0) return;
   if (EPI == 1) {
-    const double bprev = j >= 1 ? Hd[j + (int64_t)(j - 1) * ldh] : 0.0;
-    for (int k = lane; k <= j; k += 64) a[k] = k == j ? sv[0] : (k == j - 1 ? bprev : 0.0);
+    for (int k = lane; k <= j; k += 64) a[k] = k == j ? sv[0] : Hd[j + (int64_t)k * ldh];
     return;
   }
   const double w1 = sv[nk];
@@ -586,13 +587,10 @@ __global__ __launch_bounds__(THREADS) void k_lanczos_lag_pk(T *__restrict__ V, i
 {
   using PV = typename Pk<T>::V;
   constexpr int P = Pk<T>::P;
-  __shared__ double sg[J], sd[J];
-  for (int k = threadIdx.x; k < J; k += blockDim.x) {
-    sg[k] = k < j ? cf[k] : 0.0;
-    sd[k] = k < j ? cf[LAG_LD + k] : 0.0;
-  }
+  // the coefficients are uniform: scalar loads into scalar registers
+  const double *__restrict__ sg = cf;
+  const double *__restrict__ sd = cf + LAG_LD;
   const double ca = cf[2 * LAG_LD], invb = cf[2 * LAG_LD + 1];
-  __syncthreads();
   PV *__restrict__ Vp = reinterpret_cast<PV *>(V);
   const int64_t ldp = ldv / P, np = n / P;
   double acc[J + 2];

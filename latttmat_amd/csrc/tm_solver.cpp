@@ -106,7 +106,8 @@ class KrylovSchur final : public KrylovRun {
     hbuf_.ensure(sizeof(double) * 6 * 72);
     Ydev_.ensure(sizeof(double) * 72 * 72);
     H_.assign((size_t)(m_ + 1) * m_, 0.0);
-    Hd_.ensure(sizeof(double) * ((size_t)(m_ + 1) * m_ + 8));
+    // projected matrix, flags and norm (8), the last step's correction (m)
+    Hd_.ensure(sizeof(double) * ((size_t)(m_ + 1) * m_ + 8 + 72));
     // read-back of the projected matrix, then the restart's uploads (Y and
     // the projected matrix)
     stage_.ensure(sizeof(double) * 2 * 72 * 72);
@@ -147,7 +148,11 @@ class KrylovSchur final : public KrylovRun {
   // The lagged Lanczos steps read whole leading blocks of the device
   // projected matrix (entries below the subdiagonal included), so a solve
   // starts from zeros there and every restart uploads the restarted matrix.
-  void clear_hd() { kry::fill(Hd_.as<double>(), 0.0, (int64_t)(m_ + 1) * m_, op_->stream); }
+  void clear_hd()
+  {
+    kry::fill(Hd_.as<double>(), 0.0, (int64_t)(m_ + 1) * m_, op_->stream);
+    raw_m_ = false;
+  }
 
   // Expand the Krylov-Schur decomposition from column k_ to m_.
   void expand()
@@ -194,15 +199,20 @@ class KrylovSchur final : public KrylovRun {
       issue_cycle(k0);
     }
     matvecs_ += m_ - k0;
-    double *hd = stage_.as<double>();  // (m+1) m + 2 <= 72 * 72 doubles
-    TM_HIP(hipMemcpyAsync(hd, Hd, sizeof(double) * ((size_t)ldh * m_ + 2), hipMemcpyDeviceToHost,
-                          s));
+    double *hd = stage_.as<double>();  // (m+1) m + 8 + m <= 72 * 72 doubles
+    TM_HIP(hipMemcpyAsync(hd, Hd, sizeof(double) * ((size_t)ldh * m_ + 8 + m_),
+                          hipMemcpyDeviceToHost, s));
     TM_HIP(hipStreamSynchronize(s));
     const double *fl = hd + (size_t)ldh * m_;
     reorths_ += (int64_t)fl[0];
     const int jb = fl[1] < 0 ? m_ : (int)fl[1];
     for (int j = k0; j < jb; ++j)
       for (int i = 0; i <= j + 1; ++i) H(i, j) = finite_or_throw(hd[i + (size_t)j * ldh]);
+    // a symmetric cycle ends with slot m holding the last residual before
+    // its correction and normalisation; the restart applies them
+    raw_m_ = sym_ && jb == m_;
+    if (raw_m_)
+      for (int i = 0; i < m_; ++i) gm_[i] = finite_or_throw(fl[8 + i]);
     if (jb < m_) {
       matvecs_ -= m_ - jb;
       expand_host(jb);
@@ -246,16 +256,21 @@ class KrylovSchur final : public KrylovRun {
                                        s);
         continue;
       }
-      if (sym_ && j > k0) {
-        // Lanczos step: subtract alpha v_j + beta_{j-1} v_{j-1} inside the
-        // reorthogonalisation sweep; the correction and normalisation of
-        // the residual are left to the next step's sweep (lagged) or, at
-        // the end of the cycle, to one more sweep
+      if (sym_) {
+        // Lanczos step: the first projection is known but for alpha -- T v_j
+        // couples to v_{j-1} inside a cycle and, at the first step of a
+        // cycle, to every kept Ritz vector (the restart's coupling row) --
+        // and is subtracted inside the reorthogonalisation sweep; the
+        // correction and normalisation of the residual are left to the next
+        // step's sweep (lagged) or, after the last step, to the restart
         // (each reduction also closes its part of the step in the same launch)
         unsigned nb = kry::multidot(V(j), ldv_, 1, w, n_, part, hb, s, false);
         kry::reduce_lanczos_coeffs(part, nb, 2, hb, j, Hd, ldh, hb + 144, s);
-        nb = kry::update_dots_lanczos(V_.as<T>(), ldv_, nk, w, n_, hb + 144, part, hb + 72, s,
-                                      false);
+        if (j > k0)
+          nb = kry::update_dots_lanczos(V_.as<T>(), ldv_, nk, w, n_, hb + 144, part, hb + 72, s,
+                                        false);
+        else
+          nb = kry::update_dots(V_.as<T>(), ldv_, nk, w, n_, hb + 144, part, hb + 72, s, false);
         kry::reduce_step_close_lanczos(part, nb, nk + 1, hb + 72, j, hb + 144, Hd, ldh, nrm, flags,
                                        s);
         lag = true;
@@ -270,8 +285,8 @@ class KrylovSchur final : public KrylovRun {
       kry::update_normalize(V_.as<T>(), ldv_, nk, w, n_, hb + 72, hb + nk, hb + 72 + nk, nrm,
                             0, s);
     }
-    if (lag)
-      kry::update_normalize(V_.as<T>(), ldv_, m_, V(m_), n_, hb + 72, hb, hb, nrm, 1, s);
+    // the last step's correction next to the flags, for the restart
+    if (lag) kry::copy(hb + 72, flags + 8, m_, s);
   }
 
   // Record issue_cycle(k0) into an executable graph; nullptr if the runtime
@@ -523,11 +538,30 @@ class KrylovSchur final : public KrylovRun {
       }
     }
     hipStream_t s = op_->stream;
-    std::copy(Y.begin(), Y.begin() + (size_t)m * k, stage_.as<double>());
-    TM_HIP(hipMemcpyAsync(Ydev_.get(), stage_.as<double>(), sizeof(double) * m * k,
-                          hipMemcpyHostToDevice, s));
-    kry::gemm_vy(V(0), ldv_, m, Ydev_.as<double>(), m, k, V(0), ldv_, n_, s);
-    kry::copy(V(m), V(k), n_, s);
+    if (raw_m_) {
+      // slot m holds r with v_m = (r - V g) / beta: one rotation of the m+1
+      // columns by [[Y, -g / beta], [0, 1 / beta]] gives the kept Ritz
+      // vectors and v_m in slot k
+      TM_CHECK(beta > 0, TM_EHIP, "restart of a symmetric cycle with a zero residual");
+      const int m1 = m + 1;
+      double *y1 = stage_.as<double>();
+      for (int c = 0; c < k; ++c) {
+        for (int t = 0; t < m; ++t) y1[t + (size_t)c * m1] = Y[t + (size_t)c * m];
+        y1[m + (size_t)c * m1] = 0.0;
+      }
+      for (int t = 0; t < m; ++t) y1[t + (size_t)k * m1] = -gm_[t] / beta;
+      y1[m + (size_t)k * m1] = 1.0 / beta;
+      TM_HIP(hipMemcpyAsync(Ydev_.get(), y1, sizeof(double) * m1 * (k + 1),
+                            hipMemcpyHostToDevice, s));
+      kry::gemm_vy(V(0), ldv_, m1, Ydev_.as<double>(), m1, k + 1, V(0), ldv_, n_, s);
+      raw_m_ = false;
+    } else {
+      std::copy(Y.begin(), Y.begin() + (size_t)m * k, stage_.as<double>());
+      TM_HIP(hipMemcpyAsync(Ydev_.get(), stage_.as<double>(), sizeof(double) * m * k,
+                            hipMemcpyHostToDevice, s));
+      kry::gemm_vy(V(0), ldv_, m, Ydev_.as<double>(), m, k, V(0), ldv_, n_, s);
+      kry::copy(V(m), V(k), n_, s);
+    }
     std::fill(H_.begin(), H_.end(), 0.0);
     for (int c = 0; c < k; ++c) {
       for (int a = 0; a < k; ++a) H(a, c) = S[a + (size_t)c * k];
@@ -622,6 +656,10 @@ class KrylovSchur final : public KrylovRun {
   SolverCache &cache_;
   DevBuf &V_, &partial_, &hbuf_, &Ydev_, &Hd_;
   PinnedBuf &stage_;
+  // slot m after a device cycle of a symmetric run: the last residual
+  // before its correction gm_ (m values) and normalisation by H(m, m-1)
+  bool raw_m_ = false;
+  double gm_[72] = {};
   bool sym_ = true;
   int k_ = 0;
   int64_t matvecs_ = 0, restarts_ = 0;

@@ -110,8 +110,7 @@ void reduce_lanczos_coeffs(const double *partial, unsigned, int cnt, double *out
 {
   fake_enqueue(s, [=] {
     std::memcpy(out, partial, sizeof(double) * cnt);
-    for (int k = 0; k <= j; ++k)
-      hL[k] = k == j ? out[0] : (k == j - 1 ? Hd[j + (int64_t)(j - 1) * ldh] : 0.0);
+    for (int k = 0; k <= j; ++k) hL[k] = k == j ? out[0] : Hd[j + (int64_t)k * ldh];
   });
 }
 void reduce_step_close_lanczos(const double *partial, unsigned, int cnt, double *out, int j,
