@@ -707,6 +707,67 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t z)
 }
 
 // uniform(0,1) from a counter-based hash of (seed, index)
+// k_gemm_vy over element groups: Pk<Ti>::P consecutive elements per thread
+// with one 16-byte load per input column (Vin, Vout 16-byte aligned, ldv and
+// ldo multiples of P); block 0 thread 0 takes the n % P tail.  Vout may
+// alias Vin as above.
+template <int M, typename Ti, typename To>
+__global__ __launch_bounds__(THREADS) void k_gemm_vy_pk(const Ti *Vin, int64_t ldv, int m,
+                                                        const double *__restrict__ Y, int ldy,
+                                                        int k, To *Vout, int64_t ldo, int64_t n)
+{
+  using PV = typename Pk<Ti>::V;
+  constexpr int P = Pk<Ti>::P;
+  __shared__ double sY[M * M];
+  for (int t = threadIdx.x; t < m * k; t += blockDim.x) {
+    const int r = t % m, c = t / m;
+    sY[r + c * M] = Y[r + (int64_t)c * ldy];
+  }
+  __syncthreads();
+  const PV *Vp = reinterpret_cast<const PV *>(Vin);
+  const int64_t ldp = ldv / P, np = n / P;
+  for (int64_t p = blockIdx.x * (int64_t)THREADS + threadIdx.x; p < np;
+       p += (int64_t)gridDim.x * THREADS) {
+    PV v[M];
+#pragma unroll
+    for (int r = 0; r < M; ++r) v[r] = r < m ? Vp[r * ldp + p] : PV{};
+    for (int c = 0; c < k; ++c) {
+      double acc[P];
+#pragma unroll
+      for (int e = 0; e < P; ++e) acc[e] = 0;
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        const double y = sY[r + c * M];
+#pragma unroll
+        for (int e = 0; e < P; ++e) acc[e] += (double)v[r][e] * y;
+      }
+      To *o = Vout + c * ldo + p * P;
+      if constexpr (sizeof(To) * P == 16) {
+        typename Pk<To>::V ov;
+#pragma unroll
+        for (int e = 0; e < P; ++e) ov[e] = (To)acc[e];
+        *reinterpret_cast<typename Pk<To>::V *>(o) = ov;
+      } else {
+        // float in, double out: two 16-byte stores
+        reinterpret_cast<double2 *>(o)[0] = make_double2(acc[0], acc[1]);
+        reinterpret_cast<double2 *>(o)[1] = make_double2(acc[2], acc[3]);
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int64_t i = np * P; i < n; ++i) {
+      double v[M];
+#pragma unroll
+      for (int r = 0; r < M; ++r) v[r] = r < m ? (double)Vin[r * ldv + i] : 0.0;
+      for (int c = 0; c < k; ++c) {
+        double s = 0;
+#pragma unroll
+        for (int r = 0; r < M; ++r) s += v[r] * sY[r + c * M];
+        Vout[c * ldo + i] = (To)s;
+      }
+    }
+}
+
 template <typename T>
 __global__ void k_random(T *v, int64_t n, uint64_t seed)
 {
@@ -1141,11 +1202,17 @@ void gemm_vy(const Ti *Vin, int64_t ldv, int m, const double *Y, int ldy, int k,
            "in-place basis rotation needs one element type");
   // (more workgroups than fit at once: measured faster for this kernel than
   // a grid capped at the resident count)
-  const unsigned nb = nblocks(n) * 4;
+  // (the packed form holds M x P converted elements: up to 16 columns)
+  const bool pk = m <= 16 && packs_ok<Ti>(Vin, Vout, ldv) && ldo % Pk<Ti>::P == 0;
+  const unsigned nb = (pk ? nblocks(n / Pk<Ti>::P) : nblocks(n)) * 4;
   auto go = [&](auto m_tag) {
     constexpr int MM = decltype(m_tag)::value;
-    hipLaunchKernelGGL((k_gemm_vy<MM, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y,
-                       ldy, k, Vout, ldo, n);
+    if (MM <= 16 && pk)
+      hipLaunchKernelGGL((k_gemm_vy_pk<(MM <= 16 ? MM : 16), Ti, To>), dim3(nb), dim3(THREADS), 0,
+                         s, Vin, ldv, m, Y, ldy, k, Vout, ldo, n);
+    else
+      hipLaunchKernelGGL((k_gemm_vy<MM, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y,
+                         ldy, k, Vout, ldo, n);
   };
   if (m <= 8) go(std::integral_constant<int, 8>());
   else if (m <= 16) go(std::integral_constant<int, 16>());

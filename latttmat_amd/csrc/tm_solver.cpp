@@ -40,6 +40,10 @@ constexpr double EPS = 2.220446049250313e-16;
 // vector); the refinement block and its scratch, and the Ritz vectors handed
 // to the caller, have slots of their own.
 constexpr int SLOT_ROLE = 6;
+// device projected-matrix buffer: (m+1) m entries, then flags[0..1] and the
+// step norm (8 slots), the last step's correction (72 slots), then the
+// restart rotation at offset (m+1) m + HD_Y
+constexpr int HD_Y = 8 + 72;
 constexpr int SLOT_REFINE_B = 18, SLOT_REFINE_PARTIAL = 19, SLOT_REFINE_G = 20,
               SLOT_REFINE_C = 21, SLOT_XR = 22, SLOT_XI = 23, SLOT_CHECK = 24;
 // block Krylov-Schur (block runs use role 0's basis slot)
@@ -106,11 +110,11 @@ class KrylovSchur final : public KrylovRun {
     hbuf_.ensure(sizeof(double) * 6 * 72);
     Ydev_.ensure(sizeof(double) * 72 * 72);
     H_.assign((size_t)(m_ + 1) * m_, 0.0);
-    // projected matrix, flags and norm (8), the last step's correction (m)
-    Hd_.ensure(sizeof(double) * ((size_t)(m_ + 1) * m_ + 8 + 72));
-    // read-back of the projected matrix, then the restart's uploads (Y and
-    // the projected matrix)
-    stage_.ensure(sizeof(double) * 2 * 72 * 72);
+    // projected matrix, flags and norm (8), the last step's correction (72),
+    // the restart rotation (up to 72 x 72)
+    Hd_.ensure(sizeof(double) * ((size_t)(m_ + 1) * m_ + HD_Y + 72 * 72));
+    // read-back of the projected matrix, then the restart's upload
+    stage_.ensure(sizeof(double) * (72 * 72 + (size_t)(m_ + 1) * m_ + HD_Y + 72 * 72));
   }
 
   KrylovSchur(const KrylovSchur &) = delete;
@@ -150,7 +154,9 @@ class KrylovSchur final : public KrylovRun {
   // starts from zeros there and every restart uploads the restarted matrix.
   void clear_hd()
   {
-    kry::fill(Hd_.as<double>(), 0.0, (int64_t)(m_ + 1) * m_, op_->stream);
+    double *Hd = Hd_.as<double>();
+    kry::fill(Hd, 0.0, (int64_t)(m_ + 1) * m_ + 2, op_->stream);
+    kry::fill(Hd + (size_t)(m_ + 1) * m_ + 1, -1.0, 1, op_->stream);
     raw_m_ = false;
   }
 
@@ -230,8 +236,8 @@ class KrylovSchur final : public KrylovRun {
     double *Hd = Hd_.as<double>();
     double *flags = Hd + (size_t)ldh * m_;
     double *nrm = flags + 2;
-    kry::fill(flags, 0.0, 1, s);
-    kry::fill(flags + 1, -1.0, 1, s);
+    // flags[0] = 0 and flags[1] = -1 on entry: set by the solve's start and
+    // by every restart's upload
     // lag: slot j holds the previous step's raw three-term residual r (its
     // second projection at hb + 72, beta^2 in nrm), not yet v_j
     bool lag = false;
@@ -538,39 +544,42 @@ class KrylovSchur final : public KrylovRun {
       }
     }
     hipStream_t s = op_->stream;
-    if (raw_m_) {
-      // slot m holds r with v_m = (r - V g) / beta: one rotation of the m+1
-      // columns by [[Y, -g / beta], [0, 1 / beta]] gives the kept Ritz
-      // vectors and v_m in slot k
-      TM_CHECK(beta > 0, TM_EHIP, "restart of a symmetric cycle with a zero residual");
-      const int m1 = m + 1;
-      double *y1 = stage_.as<double>();
-      for (int c = 0; c < k; ++c) {
-        for (int t = 0; t < m; ++t) y1[t + (size_t)c * m1] = Y[t + (size_t)c * m];
-        y1[m + (size_t)c * m1] = 0.0;
-      }
-      for (int t = 0; t < m; ++t) y1[t + (size_t)k * m1] = -gm_[t] / beta;
-      y1[m + (size_t)k * m1] = 1.0 / beta;
-      TM_HIP(hipMemcpyAsync(Ydev_.get(), y1, sizeof(double) * m1 * (k + 1),
-                            hipMemcpyHostToDevice, s));
-      kry::gemm_vy(V(0), ldv_, m1, Ydev_.as<double>(), m1, k + 1, V(0), ldv_, n_, s);
-      raw_m_ = false;
-    } else {
-      std::copy(Y.begin(), Y.begin() + (size_t)m * k, stage_.as<double>());
-      TM_HIP(hipMemcpyAsync(Ydev_.get(), stage_.as<double>(), sizeof(double) * m * k,
-                            hipMemcpyHostToDevice, s));
-      kry::gemm_vy(V(0), ldv_, m, Ydev_.as<double>(), m, k, V(0), ldv_, n_, s);
-      kry::copy(V(m), V(k), n_, s);
-    }
     std::fill(H_.begin(), H_.end(), 0.0);
     for (int c = 0; c < k; ++c) {
       for (int a = 0; a < k; ++a) H(a, c) = S[a + (size_t)c * k];
       H(k, c) = beta * Y[(m - 1) + (size_t)c * m];
     }
-    // the restarted projected matrix (zeros beyond column k) to the device
-    double *hup = stage_.as<double>() + 72 * 72;
-    std::copy(H_.begin(), H_.end(), hup);
-    TM_HIP(hipMemcpyAsync(Hd_.get(), hup, sizeof(double) * H_.size(), hipMemcpyHostToDevice, s));
+    // one upload: the restarted projected matrix (zeros beyond column k),
+    // the step flags reset (0, -1), and the rotation
+    double *up = stage_.as<double>() + 72 * 72;
+    const size_t nh = H_.size();
+    std::copy(H_.begin(), H_.end(), up);
+    std::fill(up + nh, up + nh + HD_Y, 0.0);
+    up[nh + 1] = -1.0;
+    double *yu = up + nh + HD_Y;
+    const double *yd = Hd_.as<double>() + nh + HD_Y;
+    int mr = m, kr = k;
+    if (raw_m_) {
+      // slot m holds r with v_m = (r - V g) / beta: one rotation of the m+1
+      // columns by [[Y, -g / beta], [0, 1 / beta]] gives the kept Ritz
+      // vectors and v_m in slot k
+      TM_CHECK(beta > 0, TM_EHIP, "restart of a symmetric cycle with a zero residual");
+      mr = m + 1;
+      kr = k + 1;
+      for (int c = 0; c < k; ++c) {
+        for (int t = 0; t < m; ++t) yu[t + (size_t)c * mr] = Y[t + (size_t)c * m];
+        yu[m + (size_t)c * mr] = 0.0;
+      }
+      for (int t = 0; t < m; ++t) yu[t + (size_t)k * mr] = -gm_[t] / beta;
+      yu[m + (size_t)k * mr] = 1.0 / beta;
+    } else {
+      std::copy(Y.begin(), Y.begin() + (size_t)m * k, yu);
+    }
+    TM_HIP(hipMemcpyAsync(Hd_.get(), up, sizeof(double) * (nh + HD_Y + (size_t)mr * kr),
+                          hipMemcpyHostToDevice, s));
+    kry::gemm_vy(V(0), ldv_, mr, yd, mr, kr, V(0), ldv_, n_, s);
+    if (!raw_m_) kry::copy(V(m), V(k), n_, s);
+    raw_m_ = false;
     k_ = k;
     ++restarts_;
     // no synchronisation: the next cycle is ordered after V Y on the stream,
