@@ -1,6 +1,7 @@
 // tm_dense.cpp -- small dense eigensolvers for the projected (Ritz) problem.
 // The matrices are at most a few dozen rows (the Krylov basis size), so
-// plain O(n^3) algorithms are used: cyclic Jacobi for the symmetric case,
+// plain O(n^3) algorithms are used: Householder tridiagonalisation + implicit
+// symmetric QR for the symmetric case,
 // Householder reduction + shifted complex QR (Schur form) + triangular
 // back-substitution for the general case.
 #include <algorithm>
@@ -14,50 +15,106 @@ namespace ltm {
 
 void sym_eig(int n, const double *A, double *w, double *Z)
 {
-  std::vector<double> a(A, A + (size_t)n * n);
+  // Householder reduction to tridiagonal form T = Q^T A Q, then implicit
+  // symmetric QR steps with Wilkinson shifts on T (Golub & Van Loan, Matrix
+  // Computations, algorithms 8.3.1 and 8.3.2), the rotations accumulated
+  // into Z = Q.  The symmetric part of A is used.
+  std::vector<double> a(A, A + (size_t)n * n), v(n), p(n);
   auto at = [&](int i, int j) -> double & { return a[i + (size_t)j * n]; };
   auto zt = [&](int i, int j) -> double & { return Z[i + (size_t)j * n]; };
   for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) zt(i, j) = (i == j) ? 1.0 : 0.0;
-  // symmetrise
-  for (int j = 0; j < n; ++j)
     for (int i = j + 1; i < n; ++i) {
-      const double v = 0.5 * (at(i, j) + at(j, i));
-      at(i, j) = at(j, i) = v;
+      const double x = 0.5 * (at(i, j) + at(j, i));
+      at(i, j) = at(j, i) = x;
     }
-  double fro = 0;
-  for (double v : a) fro += v * v;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0;
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < n; ++i)
-        if (i != j) off += at(i, j) * at(i, j);
-    if (off <= 1e-32 * fro || off == 0) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = at(p, q);
-        if (std::fabs(apq) < 1e-300) continue;
-        const double theta = (at(q, q) - at(p, p)) / (2 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
-        const double c = 1 / std::sqrt(t * t + 1), s = t * c;
-        for (int k = 0; k < n; ++k) {
-          const double akp = at(k, p), akq = at(k, q);
-          at(k, p) = c * akp - s * akq;
-          at(k, q) = s * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double apk = at(p, k), aqk = at(q, k);
-          at(p, k) = c * apk - s * aqk;
-          at(q, k) = s * apk + c * aqk;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double zkp = zt(k, p), zkq = zt(k, q);
-          zt(k, p) = c * zkp - s * zkq;
-          zt(k, q) = s * zkp + c * zkq;
-        }
-      }
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) zt(i, j) = (i == j) ? 1.0 : 0.0;
+  for (int k = 0; k + 2 < n; ++k) {
+    // reflector I - tau v v^T mapping A[k+1.., k] onto alpha e_1
+    const int r0 = k + 1, len = n - r0;
+    double xn = 0;
+    for (int i = 0; i < len; ++i) xn += at(r0 + i, k) * at(r0 + i, k);
+    xn = std::sqrt(xn);
+    if (xn == 0) continue;
+    const double alpha = at(r0, k) > 0 ? -xn : xn;
+    double vn = 0;
+    for (int i = 0; i < len; ++i) {
+      v[i] = at(r0 + i, k) - (i == 0 ? alpha : 0.0);
+      vn += v[i] * v[i];
+    }
+    if (vn == 0) continue;
+    const double tau = 2.0 / vn;
+    // trailing block: A22 <- A22 - v q^T - q v^T, p = tau A22 v,
+    // q = p - (tau / 2) (v^T p) v
+    double vp = 0;
+    for (int i = 0; i < len; ++i) {
+      double t = 0;
+      for (int j = 0; j < len; ++j) t += at(r0 + i, r0 + j) * v[j];
+      p[i] = tau * t;
+      vp += v[i] * p[i];
+    }
+    const double kk = 0.5 * tau * vp;
+    for (int i = 0; i < len; ++i) p[i] -= kk * v[i];
+    for (int j = 0; j < len; ++j)
+      for (int i = 0; i < len; ++i) at(r0 + i, r0 + j) -= v[i] * p[j] + p[i] * v[j];
+    at(r0, k) = at(k, r0) = alpha;
+    for (int i = 1; i < len; ++i) at(r0 + i, k) = at(k, r0 + i) = 0.0;
+    // Z[:, r0..] <- Z[:, r0..] (I - tau v v^T)
+    for (int i = 0; i < n; ++i) {
+      double t = 0;
+      for (int j = 0; j < len; ++j) t += zt(i, r0 + j) * v[j];
+      t *= tau;
+      for (int j = 0; j < len; ++j) zt(i, r0 + j) -= t * v[j];
+    }
   }
+  // T: diagonal d = w, off-diagonal e[i] = T(i, i+1)
+  std::vector<double> e(n > 1 ? n - 1 : 1, 0.0);
   for (int i = 0; i < n; ++i) w[i] = at(i, i);
+  for (int i = 0; i + 1 < n; ++i) e[i] = at(i + 1, i);
+  const double eps = 2.220446049250313e-16;
+  for (int iter = 0; iter < 60 * n; ++iter) {
+    // deflate: the bottom hi of the last unreduced block [lo, hi]
+    int hi = n - 1;
+    while (hi > 0 && std::fabs(e[hi - 1]) <= eps * (std::fabs(w[hi - 1]) + std::fabs(w[hi]))) {
+      e[hi - 1] = 0;
+      --hi;
+    }
+    if (hi == 0) break;
+    int lo = hi - 1;
+    while (lo > 0) {
+      if (std::fabs(e[lo - 1]) <= eps * (std::fabs(w[lo - 1]) + std::fabs(w[lo]))) {
+        e[lo - 1] = 0;
+        break;
+      }
+      --lo;
+    }
+    // Wilkinson shift: the eigenvalue of the trailing 2 x 2 closer to w[hi]
+    const double dl = 0.5 * (w[hi - 1] - w[hi]), bb = e[hi - 1];
+    const double mu = w[hi] - bb * bb / (dl + std::copysign(std::sqrt(dl * dl + bb * bb), dl));
+    // one implicit QR step on [lo, hi]: rotation k on (k, k+1) is chosen
+    // from the shifted first column (k = lo) or to remove the bulge at
+    // (k-1, k+1), and leaves a bulge at (k, k+2)
+    double x = w[lo] - mu, z = e[lo];
+    for (int k = lo; k < hi; ++k) {
+      const double r = std::sqrt(x * x + z * z);
+      const double c = r > 0 ? x / r : 1.0, sn = r > 0 ? z / r : 0.0;
+      if (k > lo) e[k - 1] = r;
+      const double dk = w[k], ek = e[k], dk1 = w[k + 1];
+      w[k] = c * c * dk + 2 * c * sn * ek + sn * sn * dk1;
+      w[k + 1] = sn * sn * dk - 2 * c * sn * ek + c * c * dk1;
+      e[k] = c * sn * (dk1 - dk) + (c * c - sn * sn) * ek;
+      if (k + 1 < hi) {
+        z = sn * e[k + 1];
+        e[k + 1] *= c;
+        x = e[k];
+      }
+      for (int i = 0; i < n; ++i) {
+        const double zk = zt(i, k), zk1 = zt(i, k + 1);
+        zt(i, k) = c * zk + sn * zk1;
+        zt(i, k + 1) = c * zk1 - sn * zk;
+      }
+    }
+  }
 }
 
 void gen_eig(int n, const double *A, double *wr, double *wi, double *Zr, double *Zi)
