@@ -70,6 +70,11 @@ int autotune_cached(TuneFamily fam, uint32_t id, int L, int es);
 // on `producer` so far (an event recorded on producer, waited on by
 // waiter).  No-op when the two are the same stream.
 void stream_wait(hipStream_t waiter, hipStream_t producer);
+// Wait for everything queued on s by polling the stream from this thread
+// (hipStreamQuery): a blocking wait returns only after the runtime's
+// completion interrupt, tens of microseconds later, which the Krylov loop
+// would pay once per restart cycle with the device idle.
+void stream_sync_poll(hipStream_t s);
 
 // Stream-ordered copy that returns once the data has arrived.  The library
 // never uses the legacy null stream, whose implicit synchronisation with

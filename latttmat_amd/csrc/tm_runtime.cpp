@@ -100,6 +100,16 @@ void destroy_graph_exec(hipGraphExec_t g)
   (void)hipGraphExecDestroy(g);
 }
 
+void stream_sync_poll(hipStream_t s)
+{
+  for (;;) {
+    const hipError_t e = hipStreamQuery(s);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) TM_HIP(e);
+    std::this_thread::yield();
+  }
+}
+
 void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s)
 {
   TM_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));

@@ -208,7 +208,7 @@ class KrylovSchur final : public KrylovRun {
     double *hd = stage_.as<double>();  // (m+1) m + 8 + m <= 72 * 72 doubles
     TM_HIP(hipMemcpyAsync(hd, Hd, sizeof(double) * ((size_t)ldh * m_ + 8 + m_),
                           hipMemcpyDeviceToHost, s));
-    TM_HIP(hipStreamSynchronize(s));
+    stream_sync_poll(s);
     const double *fl = hd + (size_t)ldh * m_;
     reorths_ += (int64_t)fl[0];
     const int jb = fl[1] < 0 ? m_ : (int)fl[1];
@@ -340,7 +340,7 @@ class KrylovSchur final : public KrylovRun {
       kry::multidot(V_.as<T>(), ldv_, nk, w, n_, partial_.as<double>(), hb, s);
       kry::update_dots(V_.as<T>(), ldv_, nk, w, n_, hb, partial_.as<double>(), hb + 72, s);
       TM_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double) * 2 * 72, hipMemcpyDeviceToHost, s));
-      TM_HIP(hipStreamSynchronize(s));
+      stream_sync_poll(s);
       const double w0 = hh[nk], w1 = hh[72 + nk];
       const double *nrm_dev = hb + 72 + nk;
       double beta2 = w1;
@@ -349,7 +349,7 @@ class KrylovSchur final : public KrylovRun {
       if (reorth) {
         kry::update_norm(V_.as<T>(), ldv_, nk, w, n_, hb + 72, partial_.as<double>(), hb + 144, s);
         TM_HIP(hipMemcpyAsync(hh.data() + 144, hb + 144, sizeof(double), hipMemcpyDeviceToHost, s));
-        TM_HIP(hipStreamSynchronize(s));
+        stream_sync_poll(s);
         for (int kk = 0; kk < nk; ++kk) H(kk, j) += hh[72 + kk];
         beta2 = hh[144];
         nrm_dev = hb + 144;
@@ -629,13 +629,13 @@ class KrylovSchur final : public KrylovRun {
       for (int t = 0; t < m; ++t) Z[t + (size_t)c * m] = r.Zr[t + (size_t)c * m];
     TM_HIP(hipMemcpyAsync(Ydev_.get(), Z.data(), sizeof(double) * m * cnt, hipMemcpyHostToDevice, s));
     kry::gemm_vy(V(0), ldv_, m, Ydev_.as<double>(), m, cnt, Xr, ldv_, n_, s);
-    TM_HIP(hipStreamSynchronize(s));
+    stream_sync_poll(s);
     if (Xi) {
       for (int c = 0; c < cnt; ++c)
         for (int t = 0; t < m; ++t) Z[t + (size_t)c * m] = r.Zi[t + (size_t)c * m];
       TM_HIP(hipMemcpyAsync(Ydev_.get(), Z.data(), sizeof(double) * m * cnt, hipMemcpyHostToDevice, s));
       kry::gemm_vy(V(0), ldv_, m, Ydev_.as<double>(), m, cnt, Xi, ldv_, n_, s);
-      TM_HIP(hipStreamSynchronize(s));
+      stream_sync_poll(s);
     }
   }
 
@@ -846,7 +846,7 @@ class BlockKrylovSchur final : public KrylovRun {
                           s));
     double fl[2];
     TM_HIP(hipMemcpyAsync(fl, flags, sizeof fl, hipMemcpyDeviceToHost, s));
-    TM_HIP(hipStreamSynchronize(s));
+    stream_sync_poll(s);
     const int bad = fl[1] < 0 ? steps : (int)fl[1];
     for (int q = 0; q < std::min(bad + 1, steps); ++q) {
       const int nc = ncs[q], col0 = nc - b;
@@ -1095,7 +1095,7 @@ class BlockKrylovSchur final : public KrylovRun {
       for (int t = 0; t < m; ++t) Z[t + (size_t)c * m] = r.Zr[t + (size_t)c * m];
     TM_HIP(hipMemcpyAsync(Yd_.get(), Z.data(), sizeof(double) * m * cnt, hipMemcpyHostToDevice, s));
     kry::gemm_vy(V(0), ldv_, m, Yd_.as<double>(), m, cnt, Xr, ldv_, n_, s);
-    TM_HIP(hipStreamSynchronize(s));
+    stream_sync_poll(s);
     if (Xi) kry::fill(Xi, 0.0, ldv_ * cnt, s);
   }
 
@@ -1189,7 +1189,7 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
     }
     kry::gram_sym(col(0), ld, k2, n, partial.as<double>(), G.as<double>(), s);
     TM_HIP(hipMemcpyAsync(hG, G.get(), sizeof(double) * k2 * k2, hipMemcpyDeviceToHost, s));
-    TM_HIP(hipStreamSynchronize(s));
+    stream_sync_poll(s);
     // L = chol(Y^T Y), Li = L^-1, M = Li sym(Y^T W) Li^T
     std::vector<double> Lf((size_t)k * k), Li((size_t)k * k, 0.0), M((size_t)k * k);
     for (int i = 0; i < k; ++i)
@@ -1245,7 +1245,7 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
     TM_HIP(hipMemcpyAsync(dCr, hCr, sizeof(double) * k2 * nev, hipMemcpyHostToDevice, s));
     kry::combine_norms(col(0), ld, k2, dCr, k2, nev, n, partial.as<double>(), dR, s);
     TM_HIP(hipMemcpyAsync(hR, dR, sizeof(double) * nev, hipMemcpyDeviceToHost, s));
-    TM_HIP(hipStreamSynchronize(s));
+    stream_sync_poll(s);
     ro.theta = ths;
     ro.res.assign(k, 0.0);
     double worst = 0, bound = 0;
@@ -1277,7 +1277,7 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
         for (int t = 0; t < k; ++t) hZ[t + (size_t)c * k] = Z[t + (size_t)c * k];
       TM_HIP(hipMemcpyAsync(dZ, hZ, sizeof(double) * k * nev, hipMemcpyHostToDevice, s));
       kry::gemm_vy(col(0), ld, k, dZ, k, nev, col(0), ld, n, s);
-      TM_HIP(hipStreamSynchronize(s));
+      stream_sync_poll(s);
       break;
     }
     // next block Y = W Z diag(1/theta) (close to orthonormal)
@@ -1286,7 +1286,7 @@ RefineOut refine_sym(tm_op *op, double *B, int64_t ld, int64_t n, int k, int nev
         hZ[t + (size_t)c * k] = Z[t + (size_t)c * k] / (ths[c] != 0 ? ths[c] : 1.0);
     TM_HIP(hipMemcpyAsync(dZ, hZ, sizeof(double) * k * k, hipMemcpyHostToDevice, s));
     kry::gemm_vy(col(k), ld, k, dZ, k, k, col(0), ld, n, s);
-    TM_HIP(hipStreamSynchronize(s));
+    stream_sync_poll(s);
   }
   return ro;
 }

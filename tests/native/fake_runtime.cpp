@@ -229,6 +229,20 @@ hipError_t hipStreamSynchronize(hipStream_t s)
   return hipSuccess;
 }
 
+// a query is a host wait in this model: the queued work runs, then the
+// stream reports complete
+hipError_t hipStreamQuery(hipStream_t s)
+{
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  FakeStream *fs = S(s);
+  if (fs->cap) {
+    fs->cap_broken = true;
+    return hipErrorStreamCaptureUnsupported;
+  }
+  flush(fs);
+  return hipSuccess;
+}
+
 hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned)
 {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
