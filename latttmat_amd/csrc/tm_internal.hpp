@@ -468,7 +468,7 @@ void reduce_step_close_lanczos(const double *partial, unsigned nb, int cnt, doub
 // (2 * 72 + 2 doubles) of lanczos_lag.  lanczos_lag: v_j into slot j, the
 // next residual into slot j+1, partial sums of the j+2 slots <V_k, w> (k <=
 // j), <w, w> for reduce_step_close_lanczos.  j <= LAG_MAXJ.
-constexpr int LAG_MAXJ = 16;
+constexpr int LAG_MAXJ = 24;
 void lag_close(const double *partial, unsigned nb, int j, const double *g, const double *Hd,
                int ldh, const double *nrm, double *hL, double *cf, hipStream_t s);
 template <typename T>

@@ -1177,7 +1177,8 @@ unsigned lanczos_lag(T *V, int64_t ldv, int j, int64_t n, const double *cf, doub
   if (j <= 4) TM_JDISPATCH_ONE(k_lanczos_lag_pk, 4, T, nb, V, ldv, j, n, cf, partial);
   else if (j <= 8) TM_JDISPATCH_ONE(k_lanczos_lag_pk, 8, T, nb, V, ldv, j, n, cf, partial);
   else if (j <= 12) TM_JDISPATCH_ONE(k_lanczos_lag_pk, 12, T, nb, V, ldv, j, n, cf, partial);
-  else TM_JDISPATCH_ONE(k_lanczos_lag_pk, 16, T, nb, V, ldv, j, n, cf, partial);
+  else if (j <= 16) TM_JDISPATCH_ONE(k_lanczos_lag_pk, 16, T, nb, V, ldv, j, n, cf, partial);
+  else TM_JDISPATCH_ONE(k_lanczos_lag_pk, 24, T, nb, V, ldv, j, n, cf, partial);
   TM_LAUNCH_CHECK();
   return nb;
 }

@@ -108,11 +108,11 @@ def test_invariant_subspace_breakdown(driver, tmp_path):
     assert np.all(np.abs(got / np.array([5.0, 3.0]) - 1) < 1e-12), got
 
 
-@pytest.mark.parametrize("ncv", [5, 12, 24])
+@pytest.mark.parametrize("ncv", [5, 12, 30])
 def test_symmetric_lagged_steps(driver, oracle, tmp_path, ncv):
     # Lanczos steps whose correction is finished in the next step's sweep:
     # short cycles (one lagged step), the usual basis, and a basis beyond the
-    # lagged kernel's 16 vectors (the steps past it finish v_j first)
+    # lagged kernel's 24 vectors (the steps past it finish v_j first)
     D = oracle.OracleOp(oracle.XAT, 10, 0, 0, 0.45, 0.55, 0.0, 0.0).dense()
     D = 0.5 * (D + D.T)
     w = np.sort(np.linalg.eigvalsh(D))[::-1]
