@@ -92,15 +92,16 @@ def main():
         profile_tx(lt, op)
         return
 
-    # chosen method: FP32 phase with a 12-vector basis and an FP64
-    # Rayleigh-Ritz block of the 4 wanted Ritz vectors (measured at this size:
-    # ncv 11 -> 6.64 ms, 12 -> 6.11, 13 -> 6.99 (two refinement iterations),
-    # 14 -> 6.60, 15 -> 7.12 per solve; a block of 5 or 6 costs 0.2 / 1.2 ms
-    # more); a refinement block whose bound is not met is iterated, one that
-    # never meets it falls back to FP64 Krylov-Schur, so the choice affects
-    # time, not accuracy.  The plain FP64 solve uses the reference's
+    # chosen method: FP32 phase with a 16-vector basis and an FP64
+    # Rayleigh-Ritz block of the 4 wanted Ritz vectors (measured at this size,
+    # ms per solve with FP32 + FP64 T.x: ncv 12 -> 4.39 (24 + 4), 16 -> 4.31
+    # (22 + 4, one restart), 19 -> 5.59 (27 + 4), 20 -> 5.95, 21 -> 6.43,
+    # 22 -> 4.58 (22 + 4, no restart), 24 -> 5.11; a block of 5 costs about
+    # 0.17 ms more); a refinement block whose bound is not met is iterated, one
+    # that never meets it falls back to FP64 Krylov-Schur, so the choice
+    # affects time, not accuracy.  The plain FP64 solve uses the reference's
     # ncv = suggested_nev(4) = 11
-    NCV_CHOSEN = 12
+    NCV_CHOSEN = 16
     REFINE_EXTRA = 0
 
     def solve(prec):

@@ -49,10 +49,14 @@ __device__ __forceinline__ double lane_sum(const double *__restrict__ p, int nbl
 }
 
 // Block-reduce the first `cnt` of the J values acc[] (compile-time indexed,
-// so the array stays in registers) and write them to out[0..cnt).
+// so the array stays in registers) and write slot k of this workgroup to
+// partial[k * NBLK + blockIdx.x]: each slot's partial sums are contiguous,
+// so the reduction kernels read them with coalesced loads (grids of at most
+// NBLK workgroups).
 template <int J>
-__device__ __forceinline__ void block_reduce_write(const double (&acc)[J], int cnt, double *out)
+__device__ __forceinline__ void block_reduce_write(const double (&acc)[J], int cnt, double *partial)
 {
+  double *out = partial + blockIdx.x;
   __shared__ double red[THREADS / 64][J];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -66,7 +70,7 @@ __device__ __forceinline__ void block_reduce_write(const double (&acc)[J], int c
   for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
     double s = 0;
     for (int w = 0; w < THREADS / 64; ++w) s += red[w][k];
-    out[k] = s;
+    out[(int64_t)k * NBLK] = s;
   }
 }
 
@@ -121,7 +125,7 @@ __global__ __launch_bounds__(THREADS) void k_multidot(const T *__restrict__ V, i
 #pragma unroll
   for (int k = 0; k < J; ++k)
     if (k == j) acc[k] = acc[J];
-  block_reduce_write<J + 1>(acc, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
+  block_reduce_write<J + 1>(acc, j + 1, partial);
 }
 
 // w -= sum_k h[k] V_k; then (dots=1) slots <V_k, w>, <w,w> or (dots=0) slot 0 = <w,w>
@@ -163,10 +167,10 @@ __global__ __launch_bounds__(THREADS) void k_update_dots(const T *__restrict__ V
 #pragma unroll
     for (int k = 0; k < J; ++k)
       if (k == j) acc[k] = acc[J];
-    block_reduce_write<J + 1>(acc, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
+    block_reduce_write<J + 1>(acc, j + 1, partial);
   } else {
     acc[0] = acc[J];
-    block_reduce_write<J + 1>(acc, 1, partial + (int64_t)blockIdx.x);
+    block_reduce_write<J + 1>(acc, 1, partial);
   }
 }
 
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(64) void k_reduce_partials(const double *__restrict
                                                         double *__restrict__ out)
 {
   const int k = blockIdx.x;
-  const double s = lane_sum(partial + k, nblk, cnt, threadIdx.x);
+  const double s = lane_sum(partial + (int64_t)k * NBLK, nblk, 1, threadIdx.x);
   if (threadIdx.x == 0) out[k] = s;
 }
 
@@ -214,7 +218,7 @@ __global__ __launch_bounds__(THREADS) void k_multidot_pk(const T *__restrict__ V
 #pragma unroll
   for (int k = 0; k < J; ++k)
     if (k == j) acc[k] = acc[J];
-  block_reduce_write<J + 1>(acc, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
+  block_reduce_write<J + 1>(acc, j + 1, partial);
 }
 
 template <int J, typename T>
@@ -288,10 +292,10 @@ __global__ __launch_bounds__(THREADS) void k_update_dots_pk(const T *__restrict_
 #pragma unroll
     for (int k = 0; k < J; ++k)
       if (k == j) acc[k] = acc[J];
-    block_reduce_write<J + 1>(acc, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
+    block_reduce_write<J + 1>(acc, j + 1, partial);
   } else {
     acc[0] = acc[J];
-    block_reduce_write<J + 1>(acc, 1, partial + (int64_t)blockIdx.x);
+    block_reduce_write<J + 1>(acc, 1, partial);
   }
 }
 
@@ -354,7 +358,7 @@ __global__ __launch_bounds__(THREADS, sweep_waves(J)) void k_lanczos_dots_pk(con
 #pragma unroll
   for (int k = 0; k < J; ++k)
     if (k == j) acc[k] = acc[J];
-  block_reduce_write<J + 1>(acc, j + 1, partial + (int64_t)blockIdx.x * (j + 1));
+  block_reduce_write<J + 1>(acc, j + 1, partial);
 }
 
 template <typename T>
@@ -491,7 +495,7 @@ __global__ __launch_bounds__(1024) void k_reduce_epi(const double *__restrict__ 
   if (EPI != 1)
     for (int k = threadIdx.x; k < (EPI == 2 ? nk + 1 : nk); k += blockDim.x) sa[k] = a[k];
   for (int k = wid; k < cnt; k += nw) {
-    const double s = lane_sum(partial + k, nblk, cnt, lane);
+    const double s = lane_sum(partial + (int64_t)k * NBLK, nblk, 1, lane);
     if (lane == 0) {
       out[k] = s;
       sv[k] = s;
@@ -549,12 +553,17 @@ __global__ __launch_bounds__(64) void k_lag_close(const double *__restrict__ par
                                                   double *__restrict__ hL, double *__restrict__ cf)
 {
   const int lane = threadIdx.x;
-  const double rz = __shfl(lane_sum(partial, nblk, cnt, lane), 0, 64);
+  // H_{0..j-1, 0..j-1} and g staged in LDS by parallel loads (j <= LAG_MAXJ)
+  __shared__ double sh[LAG_MAXJ * LAG_MAXJ], sg[LAG_MAXJ];
+  for (int e = lane; e < j * j; e += 64) sh[e] = Hd[(e % j) + (int64_t)(e / j) * ldh];
+  for (int e = lane; e < j; e += 64) sg[e] = g[e];
+  const double rz = __shfl(lane_sum(partial, nblk, 1, lane), 0, 64);
   const double b2 = nrm[0], b = sqrt(fmax(b2, 0.0)), invb = 1.0 / b;
+  __syncthreads();
   double gc = 0;
   for (int i = lane; i < j; i += 64) {
     double c = 0;
-    for (int t = 0; t < j; ++t) c += Hd[i + (int64_t)t * ldh] * g[t];
+    for (int t = 0; t < j; ++t) c += sh[i + t * j] * sg[t];
     gc += g[i] * c;
     cf[i] = g[i] * invb;
     cf[LAG_LD + i] = c * invb + (i == j - 1 ? b : 0.0);
@@ -657,7 +666,7 @@ __global__ __launch_bounds__(THREADS) void k_lanczos_lag_pk(T *__restrict__ V, i
     if (k == j) acc[k] = a1;
     else if (k == j + 1) acc[k] = a2;
   }
-  block_reduce_write<J + 2>(acc, j + 2, partial + (int64_t)blockIdx.x * (j + 2));
+  block_reduce_write<J + 2>(acc, j + 2, partial);
 }
 
 template <typename T>
@@ -845,7 +854,7 @@ __global__ __launch_bounds__(THREADS) void k_gram_row(const double *__restrict__
     for (int c = 0; c < J; ++c)
       if (c < k) acc[c] += a * B[c * ld + i];
   }
-  block_reduce_write<J>(acc, k, partial + (int64_t)blockIdx.x * k);
+  block_reduce_write<J>(acc, k, partial);
 }
 
 // Rows [r0, r0 + RB) of the Gram matrix in one sweep: each vector of A's
@@ -872,7 +881,7 @@ __global__ __launch_bounds__(THREADS) void k_gram_block(const double *__restrict
         for (int r = 0; r < RB; ++r) acc[r * J + c] += a[r] * b;
       }
   }
-  block_reduce_write<RB * J>(acc, RB * J, partial + (int64_t)blockIdx.x * (RB * J));
+  block_reduce_write<RB * J>(acc, RB * J, partial);
 }
 
 // out[r * k + c] = sum over blocks of slot r * J + c, for r < nr, c < k (one
@@ -882,7 +891,7 @@ __global__ __launch_bounds__(64) void k_reduce_gram(const double *__restrict__ p
                                                     double *__restrict__ out)
 {
   const int r = blockIdx.x / k, c = blockIdx.x % k;
-  const double s = lane_sum(partial + r * J + c, nblk, stride, threadIdx.x);
+  const double s = lane_sum(partial + (int64_t)(r * J + c) * NBLK, nblk, 1, threadIdx.x);
   if (threadIdx.x == 0) out[r * k + c] = s;
 }
 
@@ -907,7 +916,7 @@ __global__ __launch_bounds__(THREADS) void k_gram_tri(const double *__restrict__
 #pragma unroll
       for (int b = 0; b <= a; ++b) acc[a * (a + 1) / 2 + b] += u[a] * u[b];
   }
-  block_reduce_write<NT>(acc, NT, partial + (int64_t)blockIdx.x * NT);
+  block_reduce_write<NT>(acc, NT, partial);
 }
 
 // G[a * nv + b] = G[b * nv + a] = sum over blocks of triangle slot (a, b)
@@ -918,7 +927,7 @@ __global__ __launch_bounds__(64) void k_reduce_tri(const double *__restrict__ pa
   int a = 0;
   while ((a + 1) * (a + 2) / 2 <= t) ++a;
   const int b = t - a * (a + 1) / 2;
-  const double s = lane_sum(partial + t, nblk, nt, threadIdx.x);
+  const double s = lane_sum(partial + (int64_t)t * NBLK, nblk, 1, threadIdx.x);
   if (threadIdx.x == 0 && a < nv) {
     G[a * nv + b] = s;
     G[b * nv + a] = s;
@@ -961,7 +970,7 @@ __global__ __launch_bounds__(THREADS) void k_combine_norms(const double *__restr
       acc[c] += x * x;
     }
   }
-  block_reduce_write<NC>(acc, NC, partial + (int64_t)blockIdx.x * NC);
+  block_reduce_write<NC>(acc, NC, partial);
 }
 
 unsigned nblocks(int64_t n)
@@ -1203,13 +1212,13 @@ void gemm_vy(const Ti *Vin, int64_t ldv, int m, const double *Y, int ldy, int k,
            "in-place basis rotation needs one element type");
   // (more workgroups than fit at once: measured faster for this kernel than
   // a grid capped at the resident count)
-  // (the packed form holds M x P converted elements: up to 16 columns)
-  const bool pk = m <= 16 && packs_ok<Ti>(Vin, Vout, ldv) && ldo % Pk<Ti>::P == 0;
+  // (the packed form holds M x P converted elements: up to 24 columns)
+  const bool pk = m <= 24 && packs_ok<Ti>(Vin, Vout, ldv) && ldo % Pk<Ti>::P == 0;
   const unsigned nb = (pk ? nblocks(n / Pk<Ti>::P) : nblocks(n)) * 4;
   auto go = [&](auto m_tag) {
     constexpr int MM = decltype(m_tag)::value;
-    if (MM <= 16 && pk)
-      hipLaunchKernelGGL((k_gemm_vy_pk<(MM <= 16 ? MM : 16), Ti, To>), dim3(nb), dim3(THREADS), 0,
+    if (MM <= 24 && pk)
+      hipLaunchKernelGGL((k_gemm_vy_pk<(MM <= 24 ? MM : 24), Ti, To>), dim3(nb), dim3(THREADS), 0,
                          s, Vin, ldv, m, Y, ldy, k, Vout, ldo, n);
     else
       hipLaunchKernelGGL((k_gemm_vy<MM, Ti, To>), dim3(nb), dim3(THREADS), 0, s, Vin, ldv, m, Y,
@@ -1217,6 +1226,7 @@ void gemm_vy(const Ti *Vin, int64_t ldv, int m, const double *Y, int ldy, int k,
   };
   if (m <= 8) go(std::integral_constant<int, 8>());
   else if (m <= 16) go(std::integral_constant<int, 16>());
+  else if (m <= 24) go(std::integral_constant<int, 24>());
   else if (m <= 32) go(std::integral_constant<int, 32>());
   else go(std::integral_constant<int, 64>());
   TM_LAUNCH_CHECK();
