@@ -101,3 +101,28 @@ def test_xdt_dense(lt):
     T = physics.xdt_dense(10, 0.6, 0.7, 0.1) * op.getnormfactor()
     lam0 = np.linalg.eigvalsh(T)[-1]
     assert abs(r.evals[0] / lam0 - 1) < 1e-12
+
+
+@pytest.mark.parametrize("ncv", [6, 16, 30])
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_lagged_lanczos_basis_sizes(lt, ncv, prec):
+    # symmetric solves run lagged Lanczos steps (the residual's correction
+    # and normalisation inside the next step's sweep, the last one inside the
+    # restart); a 30-vector basis also runs the steps past the lagged sweep's
+    # 24 vectors, which finish v_j first
+    # (the symmetry-reduced operator: the full one's momentum pairs are
+    # degenerate, and a single start vector finds one copy of a degenerate
+    # pair or two depending on rounding)
+    op = lt.TransferMatrix("A", 2, 5, 14, J=0.45, kappa=0.55, neigs=4)
+    p = lt.TM_PREC_F64 if prec == "f64" else lt.TM_PREC_MIXED
+    r = lt.solve(op, nev=4, ncv=ncv, precision=p, want_vectors=True)
+    from oracle import oracle as O
+    D = O.OracleOp(O.XAM2D, 14, 0, 2, 0.45, 0.55, 0.0, 0.0).dense()
+    D = 0.5 * (D + D.T)
+    w = np.sort(np.linalg.eigvalsh(D))[::-1]
+    print(ncv, prec, "restarts", r.restarts, r.matvecs_f32, r.matvecs_f64, flush=True)
+    assert np.all(np.abs(np.asarray(r.evals) / w[:4] - 1) < 1e-12), (r.evals, w[:4])
+    for c in range(4):
+        v = r.vecs[:, c]
+        res = np.linalg.norm(D @ v - r.evals[c] * v) / abs(r.evals[c])
+        assert res < 1e-9, (c, res)
