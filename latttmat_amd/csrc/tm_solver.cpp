@@ -1419,8 +1419,12 @@ tm_status solve_impl(tm_op *op, const tm_solve_opts &o, tm_solve_out *out)
     ks->ritz_vectors(rz, k, B.as<double>(), nullptr);
     std::vector<double> outside;
     for (int d = k; d < ks->m(); ++d) outside.push_back(rz.wr[d]);
-    RefineOut ro = refine_sym(op, B.as<double>(), ldv, n, k, nev, o.tol, 4, o.verbose, outside,
-                              o.want_vectors != 0);
+    // with eigenvectors wanted, one iteration: subspace iteration brings the
+    // FP32 Ritz vectors' residuals (about 1e-7) to tol only slowly, at k FP64
+    // T.x per iteration, so a block that misses tol after one goes straight
+    // to the FP64 Krylov-Schur below
+    RefineOut ro = refine_sym(op, B.as<double>(), ldv, n, k, nev, o.tol, o.want_vectors ? 1 : 4,
+                              o.verbose, outside, o.want_vectors != 0);
     mv64 += ro.matvecs;
     if (!ro.failed && ro.met) {
       for (int i = 0; i < nev; ++i) {
