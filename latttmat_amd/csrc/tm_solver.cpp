@@ -9,9 +9,10 @@
 // Ritz vectors spans the same restarted Krylov space as an implicit restart
 // with the unwanted Ritz values as shifts.
 //
-// Device: the basis V (ncv+1 vectors), T.x, CGS2 orthogonalisation, the
-// restart V <- V Y.  Host: the (ncv x ncv) projected matrix, its
-// eigen-decomposition, convergence tests.  One small device->host copy per
+// Device: the basis V (ncv+1 vectors), T.x, CGS2 orthogonalisation (for
+// symmetric T: lagged Lanczos steps, one sweep over V per step), the restart
+// V <- V Y.  Host: the (ncv x ncv) projected matrix, its eigen-decomposition,
+// convergence tests.  One small device->host copy and one upload per
 // restart cycle.
 //
 // Mixed precision (TM_PREC_MIXED): phase 1 runs the same loop with the
