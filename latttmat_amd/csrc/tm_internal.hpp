@@ -474,6 +474,13 @@ void lag_close(const double *partial, unsigned nb, int j, const double *g, const
 template <typename T>
 unsigned lanczos_lag(T *V, int64_t ldv, int j, int64_t n, const double *cf, double *partial,
                      hipStream_t s);
+// General T: from the partial sums of the multidot of V_0..V_j (slot j the
+// raw residual r) against T r (j+2 slots), the previous correction g, beta^2
+// = nrm[0] and Hd, the first projection of T v_j into hL[0..j] and the
+// coefficients cf of lanczos_lag.  j <= LAG_MAXJ.
+void arnoldi_lag_close(const double *partial, unsigned nb, int j, const double *g,
+                       const double *Hd, int ldh, const double *nrm, double *hL, double *cf,
+                       hipStream_t s);
 // w <- (w - c V h2) / sqrt(nrm2[0]), c = 1 iff force or !(w1 > w0 / 2)
 template <typename T>
 void update_normalize(const T *V, int64_t ldv, int j, T *w, int64_t n, const double *h2,

@@ -578,6 +578,59 @@ __global__ __launch_bounds__(64) void k_lag_close(const double *__restrict__ par
   }
 }
 
+// Lagged Arnoldi step j (general T; the previous step left r, g and beta^2 as
+// for k_lag_close).  The first projection of T v_j is not known beforehand:
+// a multidot of [P | r] against z = T r gives the slots <P_k, z> (k < j),
+// <r, z> (slot j) and <z, z> (slot j+1), and with c = H_{0..j-1, 0..j-1} g
+// and V orthonormal
+//   <P_k, T v_j> = (<P_k, z> - c_k) / beta,
+//   <v_j, T v_j> = (<r, z> - <g, P^T z>) / beta^2 - g_{j-1}.
+// Writes that projection to hL[0..j] (the step close adds the second one)
+// and the sweep coefficients of k_lanczos_lag_pk: cf[k] = g_k / beta,
+// cf[LAG_LD + k] = c_k / beta + hL[k], cf[2 LAG_LD] = g_{j-1} + hL[j],
+// cf[2 LAG_LD + 1] = 1 / beta.  Its waves sum the j+2 slots in parallel;
+// the first wavefront closes.
+__global__ __launch_bounds__(1024) void k_arnoldi_lag_close(const double *__restrict__ partial,
+                                                            int nblk, int j,
+                                                            const double *__restrict__ g,
+                                                            const double *__restrict__ Hd,
+                                                            int ldh,
+                                                            const double *__restrict__ nrm,
+                                                            double *__restrict__ hL,
+                                                            double *__restrict__ cf)
+{
+  __shared__ double sv[LAG_MAXJ + 2], sh[LAG_MAXJ * LAG_MAXJ], sg[LAG_MAXJ];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int e = threadIdx.x; e < j * j; e += blockDim.x)
+    sh[e] = Hd[(e % j) + (int64_t)(e / j) * ldh];
+  for (int e = threadIdx.x; e < j; e += blockDim.x) sg[e] = g[e];
+  for (int k = wid; k < j + 2; k += nw) {
+    const double sum = lane_sum(partial + (int64_t)k * NBLK, nblk, 1, lane);
+    if (lane == 0) sv[k] = sum;
+  }
+  __syncthreads();
+  if (wid != 0) return;
+  const double b2 = nrm[0], b = sqrt(fmax(b2, 0.0)), invb = 1.0 / b;
+  double gpz = 0;
+  for (int i = lane; i < j; i += 64) {
+    double c = 0;
+    for (int t = 0; t < j; ++t) c += sh[i + t * j] * sg[t];
+    const double hp = (sv[i] - c) * invb;
+    gpz += sg[i] * sv[i];
+    hL[i] = hp;
+    cf[i] = sg[i] * invb;
+    cf[LAG_LD + i] = c * invb + hp;
+  }
+  gpz = __shfl(wave_sum(gpz), 0, 64);
+  const double gl = sg[j - 1];
+  const double hj = (sv[j] - gpz) / b2 - gl;
+  if (lane == 0) {
+    hL[j] = hj;
+    cf[2 * LAG_LD] = gl + hj;
+    cf[2 * LAG_LD + 1] = invb;
+  }
+}
+
 // The sweep of a lagged Lanczos step j (coefficients from k_lag_close): per
 // element, from slot j (r), slot j+1 (z) and P,
 //   v_j = r / beta - sum_k (g_k / beta) P_k                 -> slot j
@@ -1172,6 +1225,16 @@ void lag_close(const double *partial, unsigned nb, int j, const double *g, const
   TM_CHECK(j >= 1 && j <= LAG_MAXJ, TM_EINVAL, "lagged Lanczos step out of range");
   hipLaunchKernelGGL(k_lag_close, dim3(1), dim3(64), 0, s, partial, (int)nb, 2, j, g, Hd, ldh,
                      nrm, hL, cf);
+  TM_LAUNCH_CHECK();
+}
+
+void arnoldi_lag_close(const double *partial, unsigned nb, int j, const double *g,
+                       const double *Hd, int ldh, const double *nrm, double *hL, double *cf,
+                       hipStream_t s)
+{
+  TM_CHECK(j >= 1 && j <= LAG_MAXJ, TM_EINVAL, "lagged Arnoldi step out of range");
+  hipLaunchKernelGGL(k_arnoldi_lag_close, dim3(1), dim3(1024), 0, s, partial, (int)nb, j, g, Hd,
+                     ldh, nrm, hL, cf);
   TM_LAUNCH_CHECK();
 }
 

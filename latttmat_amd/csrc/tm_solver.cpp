@@ -215,9 +215,9 @@ class KrylovSchur final : public KrylovRun {
     const int jb = fl[1] < 0 ? m_ : (int)fl[1];
     for (int j = k0; j < jb; ++j)
       for (int i = 0; i <= j + 1; ++i) H(i, j) = finite_or_throw(hd[i + (size_t)j * ldh]);
-    // a symmetric cycle ends with slot m holding the last residual before
-    // its correction and normalisation; the restart applies them
-    raw_m_ = sym_ && jb == m_;
+    // a device cycle ends with slot m holding the last residual before its
+    // correction and normalisation; the restart applies them
+    raw_m_ = jb == m_;
     if (raw_m_)
       for (int i = 0; i < m_; ++i) gm_[i] = finite_or_throw(fl[8 + i]);
     if (jb < m_) {
@@ -254,10 +254,18 @@ class KrylovSchur final : public KrylovRun {
       }
       op_->apply_internal(V(j), w, ldv_, 1, io_, comp_, tr_, nullptr, s);
       if (lag) {
-        // lagged Lanczos step: T r is in slot j+1; one sweep finishes v_j and
-        // forms the next residual with its reorthogonalisation dots
-        unsigned nb = kry::multidot(V(j), ldv_, 1, w, n_, part, hb, s, false);
-        kry::lag_close(part, nb, j, hb + 72, Hd, ldh, nrm, hb + 144, cf, s);
+        // lagged step: T r is in slot j+1; one sweep finishes v_j and forms
+        // the next residual with its reorthogonalisation dots.  Its first
+        // projection: for symmetric T from <r, T r> alone, otherwise from a
+        // sweep of the basis against T r
+        unsigned nb;
+        if (sym_) {
+          nb = kry::multidot(V(j), ldv_, 1, w, n_, part, hb, s, false);
+          kry::lag_close(part, nb, j, hb + 72, Hd, ldh, nrm, hb + 144, cf, s);
+        } else {
+          nb = kry::multidot(V_.as<T>(), ldv_, nk, w, n_, part, hb, s, false);
+          kry::arnoldi_lag_close(part, nb, j, hb + 72, Hd, ldh, nrm, hb + 144, cf, s);
+        }
         nb = kry::lanczos_lag(V_.as<T>(), ldv_, j, n_, cf, part, s);
         kry::reduce_step_close_lanczos(part, nb, nk + 1, hb + 72, j, hb + 144, Hd, ldh, nrm, flags,
                                        s);
@@ -283,14 +291,15 @@ class KrylovSchur final : public KrylovRun {
         lag = true;
         continue;
       }
+      // general step: classical Gram-Schmidt, the second projection
+      // measured in the same sweep; the correction and normalisation are left
+      // to the next (lagged) step's sweep or the restart (an unsafe
+      // correction sends the step to the host loop)
       kry::multidot(V_.as<T>(), ldv_, nk, w, n_, part, hb, s);
       const unsigned nb = kry::update_dots(V_.as<T>(), ldv_, nk, w, n_, hb, part, hb + 72, s,
                                            false);
-      // DGKS correction and normalisation in one sweep, the norm from the
-      // projections (an unsafe correction sends the step to the host loop)
-      kry::reduce_step_close(part, nb, nk + 1, hb + 72, j, hb, Hd, ldh, nrm, flags, s);
-      kry::update_normalize(V_.as<T>(), ldv_, nk, w, n_, hb + 72, hb + nk, hb + 72 + nk, nrm,
-                            0, s);
+      kry::reduce_step_close_lanczos(part, nb, nk + 1, hb + 72, j, hb, Hd, ldh, nrm, flags, s);
+      lag = true;
     }
     // the last step's correction next to the flags, for the restart
     if (lag) kry::copy(hb + 72, flags + 8, m_, s);

@@ -159,6 +159,28 @@ void lag_close(const double *partial, unsigned, int j, const double *g, const do
     cf[145] = invb;
   });
 }
+void arnoldi_lag_close(const double *partial, unsigned, int j, const double *g,
+                       const double *Hd, int ldh, const double *nrm, double *hL, double *cf,
+                       hipStream_t s)
+{
+  fake_enqueue(s, [=] {
+    const double b2 = nrm[0], b = std::sqrt(std::fmax(b2, 0.0)), invb = 1.0 / b;
+    double gpz = 0;
+    for (int i = 0; i < j; ++i) {
+      double c = 0;
+      for (int t = 0; t < j; ++t) c += Hd[i + (int64_t)t * ldh] * g[t];
+      const double hp = (partial[i] - c) * invb;
+      gpz += g[i] * partial[i];
+      hL[i] = hp;
+      cf[i] = g[i] * invb;
+      cf[72 + i] = c * invb + hp;
+    }
+    const double hj = (partial[j] - gpz) / b2 - g[j - 1];
+    hL[j] = hj;
+    cf[144] = g[j - 1] + hj;
+    cf[145] = invb;
+  });
+}
 template <typename T>
 unsigned lanczos_lag(T *V, int64_t ldv, int j, int64_t n, const double *cf, double *partial,
                      hipStream_t s)
