@@ -699,9 +699,10 @@ def bench_grad(lt, args):
     op = make(th0)
     # FP64 throughout: the HF derivative is first order in the eigenvector
     # error, so the FP32 phase's 1e-8 vectors would show (measured: 1.7e-8
-    # relative); an 8-vector basis (measured fastest) instead of the
-    # reference's suggested 5
-    NCV = 8
+    # relative); a 16-vector basis instead of the reference's suggested 5
+    # (measured, ms per solve with the vector and T.x: ncv 5 -> 15.9 (32),
+    # 8 -> 14.8 (29), 12 -> 15.9 (31), 16 -> 14.0 (25), 20 -> 16.7 (31))
+    NCV = 16
     lt.solve(op, nev=1, ncv=NCV, tol=1e-12, want_vectors=True)  # warm-up
     reps = max(3, args.steps)
     ts = time.perf_counter()
