@@ -1367,12 +1367,16 @@ void gram_sym(const double *U, int64_t ld, int nv, int64_t n, double *partial, d
     gram(U, U, ld, nv, n, partial, G, s);
     return;
   }
-  const int nt = nv <= 6 ? 21 : 55;
-  const void *kf = nv <= 6 ? reinterpret_cast<const void *>(&k_gram_tri<6>)
-                           : reinterpret_cast<const void *>(&k_gram_tri<10>);
+  // (the triangle's accumulators stay in registers: 6, 8 and 10 vectors)
+  const int nt = nv <= 6 ? 21 : (nv <= 8 ? 36 : 55);
+  const void *kf = nv <= 6   ? reinterpret_cast<const void *>(&k_gram_tri<6>)
+                   : nv <= 8 ? reinterpret_cast<const void *>(&k_gram_tri<8>)
+                             : reinterpret_cast<const void *>(&k_gram_tri<10>);
   const unsigned nb = occupancy_cap(kf, THREADS, nblocks(n));
   if (nv <= 6)
     hipLaunchKernelGGL(k_gram_tri<6>, dim3(nb), dim3(THREADS), 0, s, U, ld, nv, n, partial);
+  else if (nv <= 8)
+    hipLaunchKernelGGL(k_gram_tri<8>, dim3(nb), dim3(THREADS), 0, s, U, ld, nv, n, partial);
   else
     hipLaunchKernelGGL(k_gram_tri<10>, dim3(nb), dim3(THREADS), 0, s, U, ld, nv, n, partial);
   hipLaunchKernelGGL(k_reduce_tri, dim3(nt), dim3(64), 0, s, partial, (int)nb, nt, nv, G);
