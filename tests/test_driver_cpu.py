@@ -121,3 +121,18 @@ def test_symmetric_lagged_steps(driver, oracle, tmp_path, ncv):
         assert o["status"] == 0 and o["nconv"] == 4
         got = np.array([e.real for e in o["evals"]])
         assert np.all(np.abs(got / w[:4] - 1) < 1e-11), (ncv, prec, got, w[:4])
+
+
+@pytest.mark.parametrize("ncv", [5, 12, 30])
+def test_nonsymmetric_lagged_steps(driver, oracle, tmp_path, ncv):
+    # lagged Arnoldi steps (first projection from a sweep of the basis
+    # against T r, correction and normalisation in the next step's sweep or
+    # the restart); 30 vectors also run the steps past the lagged sweep's 24
+    D = oracle.OracleOp(oracle.ZBT, 10, 0, 0, 0.5, 0.5, 0.3, 0.0).dense()
+    w = np.linalg.eigvals(D)
+    w = w[np.argsort(-np.abs(w))]
+    o = run(driver, D, False, 2, ncv=ncv, tmp=str(tmp_path))
+    assert o["status"] == 0 and o["nconv"] == 2
+    assert abs(o["evals"][0] / w[0] - 1) < 1e-12, (ncv, o["evals"], w[:3])
+    # lambda1 is one of a complex-conjugate pair
+    assert min(abs(o["evals"][1] - w[1]), abs(o["evals"][1] - w[2])) / abs(w[1]) < 1e-9
