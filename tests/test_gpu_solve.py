@@ -126,3 +126,18 @@ def test_lagged_lanczos_basis_sizes(lt, ncv, prec):
         v = r.vecs[:, c]
         res = np.linalg.norm(D @ v - r.evals[c] * v) / abs(r.evals[c])
         assert res < 1e-9, (c, res)
+
+
+@pytest.mark.parametrize("ncv", [6, 16, 30])
+def test_lagged_arnoldi_basis_sizes(lt, ncv):
+    # non-symmetric solves run lagged Arnoldi steps; a 30-vector basis also
+    # runs the steps past the lagged sweep's 24 vectors
+    op = lt.TransferMatrix("B", 2, 3, 10, J=0.5, kappa=0.5, kappa2=0.3)
+    r = lt.solve(op, nev=2, ncv=ncv)
+    from oracle import oracle as O
+    D = O.OracleOp(O.ZBT, 10, 0, 0, 0.5, 0.5, 0.3, 0.0).dense()
+    w = np.linalg.eigvals(D)
+    w = w[np.argsort(-np.abs(w))]
+    print(ncv, "restarts", r.restarts, r.matvecs_f64, r.evals, flush=True)
+    assert abs(r.evals[0] / w[0] - 1) < 1e-12
+    assert min(abs(r.evals[1] - w[1]), abs(r.evals[1] - w[2])) / abs(w[1]) < 1e-9
