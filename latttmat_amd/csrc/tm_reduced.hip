@@ -1060,6 +1060,11 @@ __global__ __launch_bounds__(FUSED_THREADS, fused_waves<C>()) void k_fused_pass2
   const int r = (int)(ri & 0xFFFF), mr = (int)((ri >> 16) & 0xFFFF);
   const int tr = (int)((ri >> 32) & 0xFF), q = (int)((ri >> 40) & 0xFFFF);
   const uint32_t wmask = (1u << n) - 1;
+  // tr mod p for every period p <= n (the mirrored items' start rotation
+  // needs (m + tr) mod p with m < p)
+  __shared__ int trm[32];
+  if ((int)threadIdx.x <= n) trm[threadIdx.x] = threadIdx.x ? tr % (int)threadIdx.x : 0;
+  __syncthreads();
   const C *Ar = Ap + (size_t)r * nCol * 16;
   const C *Am = Ap + (size_t)mr * nCol * 16;
   // item quarter t = (item, slots 4(t&3) .. 4(t&3)+3): one vector load each
@@ -1099,8 +1104,12 @@ __global__ __launch_bounds__(FUSED_THREADS, fused_waves<C>()) void k_fused_pass2
           E = ((E << 1) & wmask) | (E >> (n - 1));
         }
       } else {
-        // E_j = rotr_{m+t+j} NR[c], j = k0 .. k0+3
-        const int d = ((m + tr) % pc + k0) % n;
+        // E_j = rotr_{m+t+j} NR[c], j = k0 .. k0+3: d = ((m + t) mod pc + k0)
+        // mod n with m < pc and k0 < pc <= n
+        int d = m + trm[pc];
+        if (d >= pc) d -= pc;
+        d += k0;
+        if (d >= n) d -= n;
         uint32_t E = d ? ((w >> d) | ((w << (n - d)) & wmask)) : w;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
