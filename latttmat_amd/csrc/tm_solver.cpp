@@ -9,9 +9,10 @@
 // Ritz vectors spans the same restarted Krylov space as an implicit restart
 // with the unwanted Ritz values as shifts.
 //
-// Device: the basis V (ncv+1 vectors), T.x, CGS2 orthogonalisation (for
-// symmetric T: lagged Lanczos steps, one sweep over V per step), the restart
-// V <- V Y.  Host: the (ncv x ncv) projected matrix, its eigen-decomposition,
+// Device: the basis V (ncv+1 vectors), T.x, CGS2 orthogonalisation in lagged
+// steps (a step's residual is corrected and normalised inside the next
+// step's sweep: one sweep over V per step for symmetric T, two otherwise),
+// the restart V <- V Y.  Host: the (ncv x ncv) projected matrix, its eigen-decomposition,
 // convergence tests.  One small device->host copy and one upload per
 // restart cycle.
 //
